@@ -1,0 +1,171 @@
+/*
+ * pbfmm.h -- C ABI of the MI355X BEM-FMM library (libpbfmm.so).
+ *
+ * Drop-in boundary for pi-BEM's fast multipole path: the BEMFMA<3> method set
+ * (include/bem_fma.h:74-202) and the FMA branch of BEMProblem<3>
+ * (include/bem_problem.h:131-242), re-expressed with plain pointers and sizes
+ * so that a thin C++ adapter can forward to it from the
+ * existing Driver.  Every call returns 0 on success and a negative status on
+ * failure; pbfmm_last_error() then holds the message (the reference reports
+ * the same conditions through AssertThrow / ExcMessage exceptions).
+ *
+ * Vectors are host arrays of n_dofs doubles in the caller's DoF numbering
+ * unless a name ends in _dev (device pointers on the handle's device).  The
+ * library owns all device memory; calls are ordered on the handle's stream;
+ * a handle is used from one host thread at a time (the reference methods are
+ * not reentrant either: bem_fma.h:256-293 mutates mutable members).
+ */
+#ifndef PBFMM_H
+#define PBFMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBFMM_OK 0
+#define PBFMM_ERR_ARG -1        /* invalid argument (null pointer, size, range) */
+#define PBFMM_ERR_STATE -2      /* call out of order (e.g. products before octree) */
+#define PBFMM_ERR_HIP -3        /* HIP runtime error; the GPU path is mandatory */
+#define PBFMM_ERR_ALLOC -4      /* device or host allocation failed */
+#define PBFMM_ERR_UNSUPPORTED -5
+
+#define PBFMM_FP64 0  /* every phase in FP64 */
+#define PBFMM_MIXED 1 /* FP32 near field (regular), expansions, M2L, SpMV; FP64
+                         singular quadrature, alpha, Arnoldi, residual and
+                         iterative refinement */
+
+typedef struct pbfmm_handle pbfmm_handle;
+
+/* Parameters of BEMFMA<3>::declare_parameters (bem_fma.cc:64-85) and of the
+ * quadrature rules handed to init_fma (bem_fma.cc:41-62). */
+typedef struct pbfmm_params {
+  int32_t num_octree_levels;  /* "Number of Octree Levels", default 10 */
+  int32_t max_nodes_per_leaf; /* "Maximum Number of Collocation Points per
+                                 Childless Block", default 20 */
+  int32_t trunc_order;        /* "FMA Truncation Order", default 6, 1..10 */
+  int32_t quad_order;         /* QGauss order of the regular rule, default 4 */
+  int32_t sing_quad_order;    /* QTelles order of the singular rule, default 5 */
+  int32_t precision;          /* PBFMM_FP64 or PBFMM_MIXED (solve_system) */
+  int32_t device;             /* HIP device ordinal */
+  void *stream;               /* hipStream_t, NULL = library-owned stream */
+} pbfmm_params;
+
+/* Defaults of bem_fma.cc:64-85 and bem_problem.cc:268-269. */
+void pbfmm_default_params(pbfmm_params *p);
+
+/* BEMFMA<3>::BEMFMA (bem_fma.h:94) + declare/parse_parameters (bem_fma.h:112-115). */
+int pbfmm_create(const pbfmm_params *p, pbfmm_handle **h);
+/* BEMFMA<3>::~BEMFMA (bem_fma.cc:31-39). */
+int pbfmm_destroy(pbfmm_handle *h);
+const char *pbfmm_last_error(const pbfmm_handle *h);
+
+/* BEMFMA<3>::init_fma (bem_fma.h:103-108, bem_fma.cc:41-62) with the mesh
+ * handed over as arrays instead of DoFHandler / Mapping:
+ *   support_xyz   3*n_dofs   DoF support points (map_dofs_to_support_points)
+ *   cell_dofs     n_cells*dpc  DoFs per cell in deal.II FE_Q numbering, dpc 4 (Q1) or 9 (Q2)
+ *   cell_geom     n_cells*dpc*3 mapping nodes (MappingFEField), NULL = support points
+ *   dbl_ptr/idx   CSR of double_nodes_set (each set may omit the DoF itself)
+ *   dirichlet_mask n_dofs, 1 = Dirichlet DoF (BEMProblem::dirichlet_nodes) */
+int pbfmm_init_fma(pbfmm_handle *h, int64_t n_dofs, const double *support_xyz, int64_t n_cells,
+                   int32_t dpc, const int64_t *cell_dofs, const double *cell_geom,
+                   const int64_t *dbl_ptr, const int64_t *dbl_idx, const uint8_t *dirichlet_mask);
+
+/* BEMFMA<3>::generate_octree_blocking (bem_fma.h:180, bem_fma.cc:3614-4801). */
+int pbfmm_generate_octree_blocking(pbfmm_handle *h);
+
+/* BEMFMA<3>::compute_m2l_flags (bem_fma.h:173, body bem_fma.cc:1846-1956,
+ * commented out in the reference).  Single process: writes the number of M2L
+ * translations per level, levels+1 entries, into m2l_count_per_level. */
+int pbfmm_compute_m2l_flags(pbfmm_handle *h, int64_t *m2l_count_per_level);
+
+/* BEMFMA<3>::direct_integrals (bem_fma.h:127, bem_fma.cc:107-1600): the
+ * near-field N and D matrices on their shared pattern, assembled on the GPU. */
+int pbfmm_direct_integrals(pbfmm_handle *h);
+
+/* BEMFMA<3>::multipole_integrals (bem_fma.h:137, bem_fma.cc:1608-1840).  The
+ * per-cell unit multipoles are not stored: leaf quadrature data is staged for
+ * the on-the-fly P2M of generate_multipole_expansions. */
+int pbfmm_multipole_integrals(pbfmm_handle *h);
+
+/* BEMFMA<3>::generate_multipole_expansions (bem_fma.h:149-150,
+ * bem_fma.cc:1970-2380): leaf P2M of phi (dipole, Ker1) and dphi_dn
+ * (monopole, Ker2), then M2M up the tree.  FP64. */
+int pbfmm_generate_multipole_expansions(pbfmm_handle *h, const double *phi, const double *dphi_dn);
+
+/* BEMFMA<3>::multipole_matr_vect_products (bem_fma.h:167-170,
+ * bem_fma.cc:2390-3072): near SpMV, L2L, M2L, M2P, L2P.  Overwrites
+ * matrVectProdN and matrVectProdD.  Uses the expansions of the last
+ * generate_multipole_expansions call.  FP64. */
+int pbfmm_multipole_matr_vect_products(pbfmm_handle *h, const double *phi, const double *dphi_dn,
+                                       double *matrVectProdN, double *matrVectProdD);
+
+/* BEMProblem<3>::compute_alpha, FMA branch (bem_problem.cc:1138-1162,
+ * upstream calls at 1159-1160): alpha = N.(-1) via the FMM.  Stored in the
+ * handle and copied to alpha (may be NULL). */
+int pbfmm_compute_alpha(pbfmm_handle *h, double *alpha);
+
+/* AffineConstraints built by BEMProblem::compute_constraints
+ * (bem_problem.cc:1376-1653), as CSR rows over constrained DoFs:
+ * row i (is_constrained[i]=1) reads dst(i) = src(i) - sum_k c_val[k] src(c_idx[k])
+ * in vmult (constrained_matrix.h:91-135) and rhs(i) = c_inhom[i]
+ * (distribute_rhs, constrained_matrix.h:137-147).  NULL pointers clear them. */
+int pbfmm_set_constraints(pbfmm_handle *h, const int64_t *c_ptr, const int64_t *c_idx,
+                          const double *c_val, const double *c_inhom, const uint8_t *is_constrained);
+
+/* ConstrainedOperator::vmult over BEMProblem<3>::vmult, FMA branch
+ * (constrained_matrix.h:91-135, bem_problem.cc:1164-1217):
+ * dst = N(src.neu) - D(src.dir) + alpha.(src.neu), pure-Neumann l2 shifts. */
+int pbfmm_vmult(pbfmm_handle *h, const double *src, double *dst);
+
+/* BEMProblem<3>::compute_rhs, FMA branch (bem_problem.cc:1219-1260). */
+int pbfmm_compute_rhs(pbfmm_handle *h, const double *tmp_rhs, double *rhs);
+
+/* BEMProblem<3>::solve_system (bem_problem.cc:1266-1347) with the FMA
+ * branch wired: alpha, rhs, constraints, GMRES(restart) on the GPU, then
+ * phi(i) = sol(i) at Neumann DoFs and dphi_dn(i) = sol(i) at Dirichlet DoFs
+ * (other entries untouched).  Stops on the true FP64 relative residual
+ * ||b - A x|| / ||b|| <= rel_tol.  PBFMM_MIXED runs FP32-operator GMRES
+ * inside FP64 iterative refinement. */
+int pbfmm_solve_system(pbfmm_handle *h, const double *tmp_rhs, double *phi, double *dphi_dn,
+                       double rel_tol, int32_t restart, int32_t max_it, int32_t *iters,
+                       double *relres);
+
+/* Device-pointer matvec used by benchmarks: the FMM operator
+ * (generate + products) on resident vectors; precision PBFMM_FP64 or
+ * PBFMM_MIXED (FP32 operator, FP64 in/out). */
+int pbfmm_matvec_dev(pbfmm_handle *h, const double *d_phi, const double *d_dphi, double *d_outN,
+                     double *d_outD, int32_t precision);
+
+/* Device-pointer solve: rhs_dev from pbfmm_compute_rhs semantics already
+ * applied (constrained rows set), x_dev initial guess in, solution out. */
+int pbfmm_gmres_dev(pbfmm_handle *h, const double *d_rhs, double *d_x, double rel_tol,
+                    int32_t restart, int32_t max_it, int32_t precision, int32_t *iters,
+                    double *relres);
+
+/* ---- introspection (tests, profiling) ---------------------------------- */
+/* counts: [0] num_blocks [1] levels [2] n_childless [3] n_sub (sum of
+ * sublevels) [4] n_intlist [5] n_nonintlist [6] n_node_entries [7] nnz near */
+int pbfmm_tree_counts(pbfmm_handle *h, int64_t *counts);
+/* Same layout as the counts above; lists per (block, sublevel) ascending. */
+int pbfmm_tree_export(pbfmm_handle *h, int32_t *level, int64_t *parent, double *pmin,
+                      double *delta, int64_t *startL, int64_t *endL, int64_t *childless,
+                      int64_t *sub_ptr, int64_t *il_ptr, int64_t *il_idx, int64_t *nil_ptr,
+                      int64_t *nil_idx, int64_t *node_ptr, int64_t *node_idx);
+/* Near field as CSR in DoF numbering, columns ascending (precision selects
+ * the FP64 or the mixed-mode FP32 values, widened). */
+int pbfmm_nearfield_export(pbfmm_handle *h, int32_t precision, int64_t *rowptr, int64_t *col,
+                           double *valN, double *valD);
+/* Expansions of every block after the last FP64 matvec: which 0 = multipole,
+ * 1 = local; Kt=(p+1)(p+2)/2 complex coefficients per block and kernel,
+ * interleaved (re, im), index n(n+1)/2+m as in multipole_expansion.h:101-117. */
+int pbfmm_get_expansions(pbfmm_handle *h, int32_t which, double *ker1, double *ker2);
+/* Per-phase device times of the last matvec in ms:
+ * [0] spmv [1] p2m [2] m2m [3] l2l [4] m2l [5] m2p [6] l2p [7] total. */
+int pbfmm_last_timings(pbfmm_handle *h, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PBFMM_H */

@@ -1,0 +1,546 @@
+// C ABI of libpbfmm (include/pbfmm.h): argument checks, error reporting, host
+// copies, and the call order of BEMFMA / BEMProblem.
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "engine.h"
+
+using namespace pbfmm;
+
+struct pbfmm_handle {
+  pbfmm_params prm;
+  Engine eng;
+  std::string err;
+  int stage = 0;  // 1 init_fma, 2 octree, 3 direct, 4 multipole
+  bool alpha_ready = false;
+  bool device_ready = false;
+};
+
+namespace {
+int fail(pbfmm_handle *h, int code, const std::string &msg) {
+  if (h) h->err = msg;
+  return code;
+}
+// Host-only steps (init_fma, generate_octree_blocking, tree export) run
+// without touching the GPU; every other call selects the handle's device and
+// initialises it on first use.
+template <typename F>
+int guarded(pbfmm_handle *h, F &&f, bool device = true) {
+  if (!h) return PBFMM_ERR_ARG;
+  try {
+    if (device) {
+      hipError_t e = hipSetDevice(h->prm.device);
+      if (e != hipSuccess) return fail(h, PBFMM_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+      if (!h->device_ready) {
+        h->eng.init_device(h->prm.device, h->prm.stream);
+        h->device_ready = true;
+      }
+    }
+    int rc = f();
+    if (rc == PBFMM_OK) h->err.clear();
+    return rc;
+  } catch (const HipError &ex) {
+    return fail(h, PBFMM_ERR_HIP, ex.what());
+  } catch (const std::bad_alloc &) {
+    return fail(h, PBFMM_ERR_ALLOC, "host allocation failed");
+  } catch (const std::exception &ex) {
+    return fail(h, PBFMM_ERR_ARG, ex.what());
+  }
+}
+void h2d(Engine &e, double *dst, const double *src, int64_t n) {
+  if (hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, e.stream) != hipSuccess)
+    throw HipError("host to device copy failed");
+}
+void d2h(Engine &e, double *dst, const double *src, int64_t n) {
+  if (hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, e.stream) != hipSuccess)
+    throw HipError("device to host copy failed");
+  e.sync();
+}
+int need(pbfmm_handle *h, int stage, const char *what) {
+  if (h->stage < stage) return fail(h, PBFMM_ERR_STATE, std::string(what) + " called before its set-up steps");
+  return PBFMM_OK;
+}
+void ensure_alpha(pbfmm_handle *h) {
+  if (!h->alpha_ready) {
+    h->eng.compute_alpha();
+    h->alpha_ready = true;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+void pbfmm_default_params(pbfmm_params *p) {
+  if (!p) return;
+  p->num_octree_levels = 10;
+  p->max_nodes_per_leaf = 20;
+  p->trunc_order = 6;
+  p->quad_order = 4;
+  p->sing_quad_order = 5;
+  p->precision = PBFMM_FP64;
+  p->device = 0;
+  p->stream = nullptr;
+}
+
+int pbfmm_create(const pbfmm_params *p, pbfmm_handle **out) {
+  if (!p || !out) return PBFMM_ERR_ARG;
+  *out = nullptr;
+  if (p->trunc_order < 1 || p->trunc_order > 10 || p->num_octree_levels < 1 || p->num_octree_levels > 20 ||
+      p->max_nodes_per_leaf < 1 || p->quad_order < 1 || p->quad_order > 8 || p->sing_quad_order < 1 ||
+      p->sing_quad_order > 16 || (p->precision != PBFMM_FP64 && p->precision != PBFMM_MIXED))
+    return PBFMM_ERR_ARG;
+  pbfmm_handle *h = nullptr;
+  try {
+    h = new pbfmm_handle();
+    h->prm = *p;
+    h->eng.p = p->trunc_order;
+    h->eng.kt = kt_of(p->trunc_order);
+    h->eng.kp = kp_of(p->trunc_order);
+    h->eng.L = p->num_octree_levels;
+    h->eng.sing_order = p->sing_quad_order;
+  } catch (...) {
+    delete h;
+    return PBFMM_ERR_HIP;
+  }
+  *out = h;
+  return PBFMM_OK;
+}
+
+int pbfmm_destroy(pbfmm_handle *h) {
+  if (!h) return PBFMM_ERR_ARG;
+  if (h->device_ready) (void)hipSetDevice(h->prm.device);
+  delete h;
+  return PBFMM_OK;
+}
+
+const char *pbfmm_last_error(const pbfmm_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+int pbfmm_init_fma(pbfmm_handle *h, int64_t n_dofs, const double *xyz, int64_t n_cells, int32_t dpc,
+                   const int64_t *cell_dofs, const double *cell_geom, const int64_t *dbl_ptr,
+                   const int64_t *dbl_idx, const uint8_t *dirichlet_mask) {
+  return guarded(h, [&]() -> int {
+    if (n_dofs <= 0 || n_cells <= 0 || !xyz || !cell_dofs || !dirichlet_mask || (dpc != 4 && dpc != 9))
+      return fail(h, PBFMM_ERR_ARG, "init_fma: need n_dofs > 0, n_cells > 0, dpc 4 or 9 and non-null arrays");
+    if (n_dofs >= (int64_t(1) << 30) || n_cells * dpc >= (int64_t(1) << 31))
+      return fail(h, PBFMM_ERR_ARG, "init_fma: sizes exceed the 32-bit index range of this build");
+    if ((dbl_ptr == nullptr) != (dbl_idx == nullptr))
+      return fail(h, PBFMM_ERR_ARG, "init_fma: dbl_ptr and dbl_idx must both be given or both be null");
+    Engine &e = h->eng;
+    HostMesh &m = e.mesh;
+    m = HostMesh();
+    m.n = n_dofs;
+    m.ncells = n_cells;
+    m.dpc = dpc;
+    m.xyz.assign(xyz, xyz + 3 * n_dofs);
+    m.cdofs.resize(n_cells * dpc);
+    for (int64_t i = 0; i < n_cells * dpc; ++i) {
+      if (cell_dofs[i] < 0 || cell_dofs[i] >= n_dofs)
+        return fail(h, PBFMM_ERR_ARG, "init_fma: cell_dofs entry out of range");
+      m.cdofs[i] = (int32_t)cell_dofs[i];
+    }
+    m.geom.resize(n_cells * dpc * 3);
+    for (int64_t c = 0; c < n_cells; ++c)
+      for (int j = 0; j < dpc; ++j)
+        for (int k = 0; k < 3; ++k)
+          m.geom[(c * dpc + j) * 3 + k] =
+              cell_geom ? cell_geom[(c * dpc + j) * 3 + k] : xyz[3 * (int64_t)m.cdofs[c * dpc + j] + k];
+    // double_nodes_set: sorted, unique, holding the dof itself (bem_problem.cc:413-454)
+    m.dbl_ptr.assign(n_dofs + 1, 0);
+    std::vector<int32_t> tmp;
+    for (int64_t i = 0; i < n_dofs; ++i) {
+      tmp.clear();
+      tmp.push_back((int32_t)i);
+      if (dbl_ptr) {
+        if (dbl_ptr[i + 1] < dbl_ptr[i]) return fail(h, PBFMM_ERR_ARG, "init_fma: dbl_ptr not monotone");
+        for (int64_t k = dbl_ptr[i]; k < dbl_ptr[i + 1]; ++k) {
+          if (dbl_idx[k] < 0 || dbl_idx[k] >= n_dofs) return fail(h, PBFMM_ERR_ARG, "init_fma: dbl_idx out of range");
+          tmp.push_back((int32_t)dbl_idx[k]);
+        }
+      }
+      std::sort(tmp.begin(), tmp.end());
+      tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+      m.dbl_idx.insert(m.dbl_idx.end(), tmp.begin(), tmp.end());
+      m.dbl_ptr[i + 1] = (int32_t)m.dbl_idx.size();
+    }
+    m.dir.assign(dirichlet_mask, dirichlet_mask + n_dofs);
+    e.have_dirichlet = false;
+    for (auto v : m.dir)
+      if (v) e.have_dirichlet = true;
+    // dof -> cells
+    m.d2c_ptr.assign(n_dofs + 1, 0);
+    for (int64_t i = 0; i < n_cells * dpc; ++i) m.d2c_ptr[m.cdofs[i] + 1]++;
+    for (int64_t i = 0; i < n_dofs; ++i) m.d2c_ptr[i + 1] += m.d2c_ptr[i];
+    m.d2c_idx.resize(n_cells * dpc);
+    std::vector<int32_t> fill(m.d2c_ptr.begin(), m.d2c_ptr.end() - 1);
+    for (int64_t c = 0; c < n_cells; ++c)
+      for (int j = 0; j < dpc; ++j) m.d2c_idx[fill[m.cdofs[c * dpc + j]]++] = (int32_t)c;
+    e.n = n_dofs;
+    e.dpc = dpc;
+    h->stage = 1;
+    h->alpha_ready = false;
+    e.have_constraints = false;
+    return PBFMM_OK;
+  }, false);
+}
+
+int pbfmm_generate_octree_blocking(pbfmm_handle *h) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 1, "generate_octree_blocking")) return rc;
+    Engine &e = h->eng;
+    std::vector<double> gx, gw;
+    gauss_legendre01(h->prm.quad_order, gx, gw);
+    Quadrature2D rule = tensor_rule(gx, gw, gx, gw);
+    compute_quad_data(e.mesh, rule, e.qd);
+    e.nq = e.qd.nq;
+    build_tree(e.mesh, e.qd, h->prm.num_octree_levels, h->prm.max_nodes_per_leaf, e.tree);
+    h->stage = 2;
+    h->alpha_ready = false;
+    return PBFMM_OK;
+  }, false);
+}
+
+int pbfmm_compute_m2l_flags(pbfmm_handle *h, int64_t *counts) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "compute_m2l_flags")) return rc;
+    if (!counts) return fail(h, PBFMM_ERR_ARG, "compute_m2l_flags: null output");
+    for (int l = 0; l <= h->eng.L; ++l) counts[l] = h->eng.far.m2l_per_level[l];
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_direct_integrals(pbfmm_handle *h) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 2, "direct_integrals")) return rc;
+    Engine &e = h->eng;
+    build_near_plan(e.mesh, e.qd, e.tree, e.near);
+    if (e.near.nnz >= (int64_t(1) << 40)) return fail(h, PBFMM_ERR_ARG, "direct_integrals: near field too large");
+    e.upload_mesh_and_tree();
+    e.upload_quad(h->prm.precision == PBFMM_MIXED);
+    e.upload_near_plan();
+    e.assemble_near(h->prm.precision == PBFMM_MIXED);
+    h->stage = 3;
+    h->alpha_ready = false;
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_multipole_integrals(pbfmm_handle *h) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 3, "multipole_integrals")) return rc;
+    Engine &e = h->eng;
+    build_far_plan(e.tree, e.p, e.far);
+    e.upload_far(h->prm.precision == PBFMM_MIXED);
+    h->stage = 4;
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_generate_multipole_expansions(pbfmm_handle *h, const double *phi, const double *dphi) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "generate_multipole_expansions")) return rc;
+    if (!phi || !dphi) return fail(h, PBFMM_ERR_ARG, "generate_multipole_expansions: null vector");
+    Engine &e = h->eng;
+    h2d(e, e.d_in.as<double>(), phi, e.n);
+    h2d(e, e.d_out.as<double>(), dphi, e.n);
+    e.generate<double>(e.d_in.as<double>(), e.d_out.as<double>());
+    e.sync();
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_multipole_matr_vect_products(pbfmm_handle *h, const double *phi, const double *dphi, double *outN,
+                                       double *outD) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "multipole_matr_vect_products")) return rc;
+    if (!phi || !dphi || !outN || !outD) return fail(h, PBFMM_ERR_ARG, "multipole_matr_vect_products: null vector");
+    Engine &e = h->eng;
+    h2d(e, e.d_in.as<double>(), phi, e.n);
+    h2d(e, e.d_out.as<double>(), dphi, e.n);
+    e.products<double>(e.d_in.as<double>(), e.d_out.as<double>(), e.d_outN.as<double>(), e.d_outD.as<double>());
+    d2h(e, outN, e.d_outN.as<double>(), e.n);
+    d2h(e, outD, e.d_outD.as<double>(), e.n);
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_compute_alpha(pbfmm_handle *h, double *alpha) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "compute_alpha")) return rc;
+    h->eng.compute_alpha();
+    h->alpha_ready = true;
+    if (alpha) d2h(h->eng, alpha, h->eng.d_alpha.as<double>(), h->eng.n);
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_set_constraints(pbfmm_handle *h, const int64_t *c_ptr, const int64_t *c_idx, const double *c_val,
+                          const double *c_inhom, const uint8_t *is_c) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 1, "set_constraints")) return rc;
+    Engine &e = h->eng;
+    if (!c_ptr || !is_c) {
+      e.have_constraints = false;
+      return PBFMM_OK;
+    }
+    const int64_t n = e.n;
+    std::vector<int32_t> ptr(n + 1), idx;
+    std::vector<double> val, inh(n, 0.0);
+    std::vector<uint8_t> isc(is_c, is_c + n);
+    for (int64_t i = 0; i < n; ++i) {
+      if (c_ptr[i + 1] < c_ptr[i]) return fail(h, PBFMM_ERR_ARG, "set_constraints: c_ptr not monotone");
+      ptr[i + 1] = (int32_t)c_ptr[i + 1];
+      if (c_inhom) inh[i] = c_inhom[i];
+    }
+    for (int64_t k = 0; k < c_ptr[n]; ++k) {
+      if (c_idx[k] < 0 || c_idx[k] >= n) return fail(h, PBFMM_ERR_ARG, "set_constraints: c_idx out of range");
+      idx.push_back((int32_t)c_idx[k]);
+      val.push_back(c_val ? c_val[k] : 1.0);
+    }
+    auto up = [&](DevBuf &b, const void *src, size_t bytes) {
+      b.release();
+      if (hipMalloc(&b.p, std::max<size_t>(bytes, 16)) != hipSuccess) throw HipError("hipMalloc failed");
+      b.bytes = bytes;
+      if (bytes && hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) throw HipError("hipMemcpy failed");
+    };
+    up(e.d_c_ptr, ptr.data(), ptr.size() * 4);
+    up(e.d_c_idx, idx.data(), idx.size() * 4);
+    up(e.d_c_val, val.data(), val.size() * 8);
+    up(e.d_c_inh, inh.data(), inh.size() * 8);
+    up(e.d_c_is, isc.data(), isc.size());
+    e.have_constraints = true;
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_vmult(pbfmm_handle *h, const double *src, double *dst) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "vmult")) return rc;
+    if (!src || !dst) return fail(h, PBFMM_ERR_ARG, "vmult: null vector");
+    ensure_alpha(h);
+    Engine &e = h->eng;
+    h2d(e, e.d_in.as<double>(), src, e.n);
+    e.vmult(e.d_in.as<double>(), e.d_out.as<double>(), PBFMM_FP64);
+    d2h(e, dst, e.d_out.as<double>(), e.n);
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_compute_rhs(pbfmm_handle *h, const double *tmp_rhs, double *rhs) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "compute_rhs")) return rc;
+    if (!tmp_rhs || !rhs) return fail(h, PBFMM_ERR_ARG, "compute_rhs: null vector");
+    ensure_alpha(h);
+    Engine &e = h->eng;
+    h2d(e, e.d_in.as<double>(), tmp_rhs, e.n);
+    e.compute_rhs(e.d_in.as<double>(), e.d_out.as<double>());
+    d2h(e, rhs, e.d_out.as<double>(), e.n);
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_solve_system(pbfmm_handle *h, const double *tmp_rhs, double *phi, double *dphi, double rel_tol,
+                       int32_t restart, int32_t max_it, int32_t *iters, double *relres) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "solve_system")) return rc;
+    if (!tmp_rhs || !phi || !dphi) return fail(h, PBFMM_ERR_ARG, "solve_system: null vector");
+    if (restart < 1 || restart > 127 || max_it < 1 || !(rel_tol > 0))
+      return fail(h, PBFMM_ERR_ARG, "solve_system: need 1 <= restart <= 127, max_it >= 1, rel_tol > 0");
+    ensure_alpha(h);
+    Engine &e = h->eng;
+    const int64_t n = e.n;
+    // rhs (compute_rhs) and distribute_rhs of the constrained rows
+    h2d(e, e.d_in.as<double>(), tmp_rhs, n);
+    DevBuf rhs, sol;
+    if (hipMalloc(&rhs.p, n * 8) != hipSuccess || hipMalloc(&sol.p, n * 8) != hipSuccess)
+      throw HipError("hipMalloc failed");
+    e.compute_rhs(e.d_in.as<double>(), rhs.as<double>());
+    std::vector<double> hr(n);
+    d2h(e, hr.data(), rhs.as<double>(), n);
+    if (e.have_constraints) {
+      std::vector<uint8_t> isc(n);
+      std::vector<double> inh(n);
+      if (hipMemcpy(isc.data(), e.d_c_is.p, n, hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(inh.data(), e.d_c_inh.p, n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+        throw HipError("constraint copy failed");
+      for (int64_t i = 0; i < n; ++i)
+        if (isc[i]) hr[i] = inh[i];
+      h2d(e, rhs.as<double>(), hr.data(), n);
+    }
+    // initial guess sol = rhs (bem_problem.cc:1295)
+    h2d(e, sol.as<double>(), hr.data(), n);
+    int it = 0;
+    double rr = 0;
+    e.gmres(rhs.as<double>(), sol.as<double>(), rel_tol, restart, max_it, h->prm.precision, &it, &rr);
+    std::vector<double> hs(n);
+    d2h(e, hs.data(), sol.as<double>(), n);
+    for (int64_t i = 0; i < n; ++i) {
+      if (e.mesh.dir[i]) dphi[i] = hs[i];
+      else phi[i] = hs[i];
+    }
+    if (iters) *iters = it;
+    if (relres) *relres = rr;
+    return rr <= rel_tol ? PBFMM_OK : fail(h, PBFMM_ERR_STATE, "solve_system: GMRES did not reach rel_tol");
+  });
+}
+
+int pbfmm_matvec_dev(pbfmm_handle *h, const double *phi, const double *dphi, double *outN, double *outD,
+                     int32_t precision) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "matvec_dev")) return rc;
+    if (!phi || !dphi || !outN || !outD) return fail(h, PBFMM_ERR_ARG, "matvec_dev: null vector");
+    Engine &e = h->eng;
+    if (precision == PBFMM_MIXED) {
+      if (!e.have_f32) return fail(h, PBFMM_ERR_STATE, "matvec_dev: handle was not created with PBFMM_MIXED");
+      e.matvec<float>(phi, dphi, outN, outD);
+    } else {
+      e.matvec<double>(phi, dphi, outN, outD);
+    }
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_gmres_dev(pbfmm_handle *h, const double *rhs, double *x, double rel_tol, int32_t restart, int32_t max_it,
+                    int32_t precision, int32_t *iters, double *relres) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "gmres_dev")) return rc;
+    if (!rhs || !x) return fail(h, PBFMM_ERR_ARG, "gmres_dev: null vector");
+    if (restart < 1 || restart > 127 || max_it < 1 || !(rel_tol > 0))
+      return fail(h, PBFMM_ERR_ARG, "gmres_dev: need 1 <= restart <= 127, max_it >= 1, rel_tol > 0");
+    if (precision == PBFMM_MIXED && !h->eng.have_f32)
+      return fail(h, PBFMM_ERR_STATE, "gmres_dev: handle was not created with PBFMM_MIXED");
+    ensure_alpha(h);
+    int it = 0;
+    double rr = 0;
+    h->eng.gmres(rhs, x, rel_tol, restart, max_it, precision, &it, &rr);
+    if (iters) *iters = it;
+    if (relres) *relres = rr;
+    return rr <= rel_tol ? PBFMM_OK : fail(h, PBFMM_ERR_STATE, "gmres_dev: GMRES did not reach rel_tol");
+  });
+}
+
+int pbfmm_tree_counts(pbfmm_handle *h, int64_t *cnt) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 2, "tree_counts")) return rc;
+    const Tree &t = h->eng.tree;
+    cnt[0] = t.nb;
+    cnt[1] = t.L;
+    cnt[2] = (int64_t)t.leaves.size();
+    cnt[3] = t.sub_beg[t.nb];
+    int64_t a = 0, b = 0;
+    for (auto &v : t.il) a += (int64_t)v.size();
+    for (auto &v : t.nil) b += (int64_t)v.size();
+    cnt[4] = a;
+    cnt[5] = b;
+    int64_t nn = 0;
+    for (int i = 0; i < t.nb; ++i) nn += t.node_cnt[i];
+    cnt[6] = nn;
+    cnt[7] = h->stage >= 3 ? h->eng.near.nnz : 0;
+    return PBFMM_OK;
+  }, false);
+}
+
+int pbfmm_tree_export(pbfmm_handle *h, int32_t *level, int64_t *parent, double *pmin, double *delta, int64_t *startL,
+                      int64_t *endL, int64_t *childless, int64_t *sub_ptr, int64_t *il_ptr, int64_t *il_idx,
+                      int64_t *nil_ptr, int64_t *nil_idx, int64_t *node_ptr, int64_t *node_idx) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 2, "tree_export")) return rc;
+    const Tree &t = h->eng.tree;
+    for (int l = 0; l <= t.L; ++l) {
+      startL[l] = t.startL[l];
+      endL[l] = t.endL[l];
+    }
+    for (size_t k = 0; k < t.leaves.size(); ++k) childless[k] = t.leaves[k];
+    int64_t a = 0, b = 0, nd = 0;
+    il_ptr[0] = nil_ptr[0] = node_ptr[0] = sub_ptr[0] = 0;
+    std::vector<int32_t> nodes;
+    for (int i = 0; i < t.nb; ++i) {
+      level[i] = t.level[i];
+      parent[i] = t.parent[i];
+      for (int k = 0; k < 3; ++k) pmin[3 * i + k] = t.pmin[3 * i + k];
+      delta[i] = t.delta[i];
+      for (int s = 0; s < t.nsub[i]; ++s) {
+        const int q = t.sub_beg[i] + s;
+        for (int x : t.il[q]) il_idx[a++] = x;
+        for (int x : t.nil[q]) nil_idx[b++] = x;
+        il_ptr[q + 1] = a;
+        nil_ptr[q + 1] = b;
+      }
+      sub_ptr[i + 1] = t.sub_beg[i + 1];
+      nodes.assign(t.nperm.begin() + t.node_beg[i], t.nperm.begin() + t.node_beg[i] + t.node_cnt[i]);
+      std::sort(nodes.begin(), nodes.end());
+      for (int x : nodes) node_idx[nd++] = x;
+      node_ptr[i + 1] = nd;
+    }
+    return PBFMM_OK;
+  }, false);
+}
+
+int pbfmm_nearfield_export(pbfmm_handle *h, int32_t precision, int64_t *rowptr, int64_t *col, double *valN,
+                           double *valD) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 3, "nearfield_export")) return rc;
+    Engine &e = h->eng;
+    const NearPlan &np = e.near;
+    std::vector<double> v(2 * np.nnz);
+    if (precision == PBFMM_MIXED) {
+      if (!e.have_f32) return fail(h, PBFMM_ERR_STATE, "nearfield_export: no FP32 near field in this handle");
+      std::vector<float> f(2 * np.nnz);
+      if (hipMemcpy(f.data(), e.f32.vals.p, f.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        throw HipError("copy failed");
+      for (size_t i = 0; i < f.size(); ++i) v[i] = f[i];
+    } else if (hipMemcpy(v.data(), e.f64.vals.p, v.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+      throw HipError("copy failed");
+    }
+    // rows in DoF order
+    std::vector<int32_t> row_of(e.n, -1);
+    for (size_t r = 0; r < np.rows.size(); ++r) row_of[np.rows[r]] = (int32_t)r;
+    rowptr[0] = 0;
+    int64_t k = 0;
+    for (int64_t i = 0; i < e.n; ++i) {
+      const int r = row_of[i];
+      if (r >= 0) {
+        const int leaf = np.row_leaf[r], ri = r - np.row_beg[leaf];
+        const int cb = np.col_beg[leaf], C = np.col_beg[leaf + 1] - cb;
+        const int64_t base = np.val_beg[leaf] + (int64_t)ri * C;
+        for (int c = 0; c < C; ++c) {
+          col[k] = np.cols[cb + c];
+          valN[k] = v[2 * (base + c)];
+          valD[k] = v[2 * (base + c) + 1];
+          ++k;
+        }
+      }
+      rowptr[i + 1] = k;
+    }
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_get_expansions(pbfmm_handle *h, int32_t which, double *k1, double *k2) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "get_expansions")) return rc;
+    Engine &e = h->eng;
+    const int nb = e.tree.nb, kp = e.kp, kt = e.kt;
+    std::vector<double> v((size_t)nb * 2 * kp);
+    const DevBuf &src = which == 0 ? e.f64.me : e.f64.le;
+    if (hipMemcpy(v.data(), src.p, v.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) throw HipError("copy failed");
+    for (int b = 0; b < nb; ++b)
+      for (int i = 0; i < kt; ++i) {
+        k1[(size_t)b * 2 * kt + 2 * i] = v[(size_t)b * 2 * kp + i];
+        k1[(size_t)b * 2 * kt + 2 * i + 1] = v[(size_t)b * 2 * kp + kt + i];
+        k2[(size_t)b * 2 * kt + 2 * i] = v[(size_t)b * 2 * kp + kp + i];
+        k2[(size_t)b * 2 * kt + 2 * i + 1] = v[(size_t)b * 2 * kp + kp + kt + i];
+      }
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_last_timings(pbfmm_handle *h, double *ms) {
+  return guarded(h, [&]() -> int {
+    for (int i = 0; i < 8; ++i) ms[i] = h->eng.last_ms[i];
+    return PBFMM_OK;
+  }, false);
+}
+
+}  // extern "C"

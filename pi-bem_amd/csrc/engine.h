@@ -1,0 +1,103 @@
+// Device-resident state of one handle and the host orchestration of the
+// BEM-FMM kernels (engine.hip).
+#pragma once
+#include <stdexcept>
+
+#include "pbfmm_impl.h"
+
+namespace pbfmm {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// Per-precision copies of the far-field operators and expansion storage.
+template <typename T>
+struct FarDev {
+  DevBuf m2m, l2l, m2l;  // transposed operators
+  DevBuf me, le;         // [nb][2 kernels][kp]
+  DevBuf qx, qn, qw, qsh;  // regular quadrature data in T
+  DevBuf vals;             // near-field values (N, D) pairs, leaf-blocked
+};
+
+struct Engine {
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int device = 0;
+  int p = 6, kt = 0, kp = 0, L = 10;
+  int dpc = 4, nq = 16, sing_order = 5;
+  int64_t n = 0;
+  // host plans
+  HostMesh mesh;
+  QuadData qd;
+  Tree tree;
+  NearPlan near;
+  FarPlan far;
+  bool have_dirichlet = false;
+  // device: mesh
+  DevBuf d_xyz, d_geom, d_cdofs, d_dbl_ptr, d_dbl_idx, d_dir;
+  // device: tree
+  DevBuf d_pmin, d_delta, d_parent, d_nchild, d_child, d_octant, d_node_beg, d_node_cnt, d_nperm,
+      d_quad_beg, d_quad_cnt, d_qperm;
+  // device: plans
+  DevBuf d_p2m_leaf, d_m2m_par, d_l2l_tgt, d_m2l_tgt, d_m2l_src, d_chunk_op, d_chunk_beg,
+      d_chunk_cnt, d_m2p_tgt, d_m2p_src, d_m2p_work;
+  std::vector<int> m2m_off, l2l_off;  // per level offsets into d_m2m_par / d_l2l_tgt
+  DevBuf d_row_leaf, d_row_beg, d_rows, d_col_beg, d_cols, d_val_beg, d_ent_beg, d_ent_cell,
+      d_ent_mask, d_ent_near, d_ent_colpos, d_work_beg, d_leaf_block;
+  DevBuf d_sing;  // singular rules: per js [w, N(dpc), dNx(dpc), dNy(dpc)] per point
+  int sing_nq = 0;
+  FarDev<double> f64;
+  FarDev<float> f32;
+  bool have_f32 = false;
+  // operator vectors
+  DevBuf d_alpha, d_sp, d_sd, d_outN, d_outD, d_in, d_out, d_red;
+  DevBuf d_c_ptr, d_c_idx, d_c_val, d_c_inh, d_c_is;
+  bool have_constraints = false;
+  // GMRES workspace
+  DevBuf d_V, d_w, d_r, d_x, d_h;
+  int gm_restart = 0;
+  // timings of the last matvec (ms)
+  double last_ms[8] = {0};
+  bool time_phases = false;
+
+  void init_device(int dev, void *stream);
+  ~Engine();
+  void upload_mesh_and_tree();
+  void upload_near_plan();
+  void upload_quad(bool want_f32);
+  void upload_far(bool want_f32);
+  void assemble_near(bool want_f32);
+  template <typename T>
+  void matvec(const double *phi, const double *dphi, double *outN, double *outD);
+  template <typename T>
+  void generate(const double *phi, const double *dphi);
+  template <typename T>
+  void products(const double *phi, const double *dphi, double *outN, double *outD);
+  // BEMProblem operator (device vectors of n doubles)
+  void vmult(const double *src, double *dst, int precision);
+  void compute_alpha();
+  void compute_rhs(const double *g, double *rhs);
+  int gmres(const double *rhs, double *x, double tol, int restart, int maxit, int precision,
+            int *iters, double *relres);
+  double norm2(const double *v);
+  void sync();
+};
+
+}  // namespace pbfmm

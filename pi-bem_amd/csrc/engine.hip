@@ -1,0 +1,1146 @@
+// HIP kernels of the BEM-FMM path for gfx950 (CDNA4) and their host-side
+// orchestration.  One wavefront is 64 lanes; every launch below sizes its grid
+// from the plan arrays built in setup.cpp and checks its indices against them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "engine.h"
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess)                                                                \
+      throw pbfmm::HipError(std::string(#x) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+namespace pbfmm {
+
+// ---------------------------------------------------------------------------
+// Legendre recurrence tables (degree <= 20):
+//   Pt(m,m) = pmm(m) (-s)^m,  pmm(m) = prod_k sqrt((2k-1)/(2k))
+//   Pt(n,m) = ra(n,m) x Pt(n-1,m) - rb(n,m) Pt(n-2,m)
+//   ra = (2n-1)/sqrt(n^2-m^2), rb = sqrt((n-1)^2-m^2)/sqrt(n^2-m^2), rs = sqrt(n^2-m^2)
+// ---------------------------------------------------------------------------
+constexpr int kLd = 21;
+__constant__ double c_ra_d[kLd * kLd], c_rb_d[kLd * kLd], c_rs_d[kLd * kLd], c_pmm_d[kLd], c_pmf_d[kLd];
+__constant__ float c_ra_f[kLd * kLd], c_rb_f[kLd * kLd], c_rs_f[kLd * kLd], c_pmm_f[kLd], c_pmf_f[kLd];
+
+template <typename T> __device__ __forceinline__ T RA(int n, int m);
+template <> __device__ __forceinline__ double RA<double>(int n, int m) { return c_ra_d[n * kLd + m]; }
+template <> __device__ __forceinline__ float RA<float>(int n, int m) { return c_ra_f[n * kLd + m]; }
+template <typename T> __device__ __forceinline__ T RB(int n, int m);
+template <> __device__ __forceinline__ double RB<double>(int n, int m) { return c_rb_d[n * kLd + m]; }
+template <> __device__ __forceinline__ float RB<float>(int n, int m) { return c_rb_f[n * kLd + m]; }
+template <typename T> __device__ __forceinline__ T RS(int n, int m);
+template <> __device__ __forceinline__ double RS<double>(int n, int m) { return c_rs_d[n * kLd + m]; }
+template <> __device__ __forceinline__ float RS<float>(int n, int m) { return c_rs_f[n * kLd + m]; }
+template <typename T> __device__ __forceinline__ T PMM(int m);
+template <> __device__ __forceinline__ double PMM<double>(int m) { return c_pmm_d[m]; }
+template <> __device__ __forceinline__ float PMM<float>(int m) { return c_pmm_f[m]; }
+// single-step factor sqrt((2m-1)/(2m))
+template <typename T> __device__ __forceinline__ T PMF(int m);
+template <> __device__ __forceinline__ double PMF<double>(int m) { return c_pmf_d[m]; }
+template <> __device__ __forceinline__ float PMF<float>(int m) { return c_pmf_f[m]; }
+
+__device__ __forceinline__ void dsincos(double a, double *s, double *c) { sincos(a, s, c); }
+__device__ __forceinline__ void dsincos(float a, float *s, float *c) { sincosf(a, s, c); }
+__device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float dsqrt(float a) { return sqrtf(a); }
+__device__ __forceinline__ double datan2(double y, double x) { return atan2(y, x); }
+__device__ __forceinline__ float datan2(float y, float x) { return atan2f(y, x); }
+
+static void upload_legendre_tables() {
+  double ra[kLd * kLd] = {0}, rb[kLd * kLd] = {0}, rs[kLd * kLd] = {0}, pmm[kLd], pmf[kLd];
+  float raf[kLd * kLd], rbf[kLd * kLd], rsf[kLd * kLd], pmmf[kLd], pmff[kLd];
+  for (int n = 0; n < kLd; ++n)
+    for (int m = 0; m <= n; ++m) {
+      const double q = std::sqrt(double(n * n - m * m));
+      rs[n * kLd + m] = q;
+      if (n > m) {
+        ra[n * kLd + m] = (2.0 * n - 1.0) / q;
+        rb[n * kLd + m] = std::sqrt(double((n - 1) * (n - 1) - m * m)) / q;
+      }
+    }
+  pmm[0] = 1.0;
+  pmf[0] = 1.0;
+  for (int m = 1; m < kLd; ++m) {
+    pmf[m] = std::sqrt((2.0 * m - 1.0) / (2.0 * m));
+    pmm[m] = pmm[m - 1] * pmf[m];
+  }
+  for (int i = 0; i < kLd * kLd; ++i) {
+    raf[i] = (float)ra[i];
+    rbf[i] = (float)rb[i];
+    rsf[i] = (float)rs[i];
+  }
+  for (int i = 0; i < kLd; ++i) {
+    pmmf[i] = (float)pmm[i];
+    pmff[i] = (float)pmf[i];
+  }
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_ra_d), ra, sizeof(ra)));
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_rb_d), rb, sizeof(rb)));
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_rs_d), rs, sizeof(rs)));
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_pmm_d), pmm, sizeof(pmm)));
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_pmf_d), pmf, sizeof(pmf)));
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_ra_f), raf, sizeof(raf)));
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_rb_f), rbf, sizeof(rbf)));
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_rs_f), rsf, sizeof(rsf)));
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_pmm_f), pmmf, sizeof(pmmf)));
+  HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_pmf_f), pmff, sizeof(pmff)));
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// Near-field assembly (BEMFMA::direct_integrals, bem_fma.cc:670-913 regular
+// and singular, 997-1126 bigger blocks).  One thread per (row, entry): the
+// regular rule over the entry's quad points in T, or the Telles rule over the
+// whole cell in FP64 when the cell holds a double of the row's node.  Results
+// are added into the leaf's rows x cols block of (N, D) pairs.
+// ---------------------------------------------------------------------------
+// Contraction is off in this kernel: near a singular vertex R = X - x_i is
+// ~1e-5 of |X| and R.n cancels to O(r^2), so the singular double-layer
+// entries follow the last bit of X; evaluating X and R exactly as the
+// reference's FEValues path does (separately rounded products and sums) keeps
+// FP64 entries reproducible to rounding.
+template <typename T>
+__global__ void __launch_bounds__(256) k_assemble_near(
+    int nleaf, const int64_t *__restrict__ work_beg, const int *__restrict__ row_beg,
+    const int *__restrict__ rows, const int *__restrict__ col_beg, const int *__restrict__ ent_beg,
+    const int *__restrict__ ent_cell, const uint64_t *__restrict__ ent_mask,
+    const uint8_t *__restrict__ ent_near, const int *__restrict__ ent_colpos,
+    const int64_t *__restrict__ val_beg, const int *__restrict__ cdofs, int dpc, int nq,
+    const int *__restrict__ dbl_ptr, const int *__restrict__ dbl_idx, const double *__restrict__ xyz,
+    const T *__restrict__ qx, const T *__restrict__ qn, const T *__restrict__ qw,
+    const T *__restrict__ qsh, const double *__restrict__ geom, const double *__restrict__ sing,
+    const int *__restrict__ sing_cnt, int sing_stride, T *__restrict__ vals) {
+#pragma clang fp contract(off)
+  const int64_t total = work_beg[nleaf];
+  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < total;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = nleaf - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (work_beg[mid] <= w) lo = mid;
+      else hi = mid - 1;
+    }
+    const int leaf = lo;
+    const int e0 = ent_beg[leaf], ne = ent_beg[leaf + 1] - e0;
+    const int64_t local = w - work_beg[leaf];
+    const int r = (int)(local / ne);
+    const int e = e0 + (int)(local % ne);
+    const int node = rows[row_beg[leaf] + r];
+    const int cell = ent_cell[e];
+    const int *cd = cdofs + (int64_t)cell * dpc;
+    int js = -1;
+    if (ent_near[e]) {
+      for (int j = 0; j < dpc && js < 0; ++j)
+        for (int k = dbl_ptr[node]; k < dbl_ptr[node + 1]; ++k)
+          if (dbl_idx[k] == cd[j]) {
+            js = j;
+            break;
+          }
+    }
+    const int C = col_beg[leaf + 1] - col_beg[leaf];
+    T *out = vals + 2 * (val_beg[leaf] + (int64_t)r * C);
+    const int *cp = ent_colpos + (int64_t)e * dpc;
+    if (js < 0) {
+      T Nj[9], Dj[9];
+      for (int j = 0; j < 9; ++j) Nj[j] = Dj[j] = T(0);
+      const T x0 = (T)xyz[3 * node], x1 = (T)xyz[3 * node + 1], x2 = (T)xyz[3 * node + 2];
+      uint64_t mask = ent_mask[e];
+      while (mask) {
+        const int q = __ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1;
+        const int64_t g = (int64_t)cell * nq + q;
+        const T R0 = qx[3 * g] - x0, R1 = qx[3 * g + 1] - x1, R2 = qx[3 * g + 2] - x2;
+        const T r = dsqrt(R0 * R0 + R1 * R1 + R2 * R2);
+        const T r2 = r * r;
+        const T s = T(1) / (r * T(4 * kPi));
+        const T den = T(-4 * kPi) * r2 * r;
+        const T dn = (R0 / den) * qn[3 * g] + (R1 / den) * qn[3 * g + 1] + (R2 / den) * qn[3 * g + 2];
+        const T wq = qw[g];
+        for (int j = 0; j < dpc; ++j) {
+          const T sh = qsh[g * dpc + j];
+          Nj[j] += dn * sh * wq;
+          Dj[j] += s * sh * wq;
+        }
+      }
+      for (int j = 0; j < dpc; ++j) {
+        atomicAdd(out + 2 * cp[j], Nj[j]);
+        atomicAdd(out + 2 * cp[j] + 1, Dj[j]);
+      }
+    } else {
+      double Nj[9], Dj[9];
+      for (int j = 0; j < 9; ++j) Nj[j] = Dj[j] = 0.0;
+      const double x0 = xyz[3 * node], x1 = xyz[3 * node + 1], x2 = xyz[3 * node + 2];
+      const double *G = geom + (int64_t)cell * dpc * 3;
+      const double *rule = sing + (int64_t)js * sing_stride;
+      const int nsq = sing_cnt[js];
+      const int ps = 1 + 3 * dpc;
+      for (int q = 0; q < nsq; ++q) {
+        const double *pt = rule + q * ps;
+        double X[3] = {0, 0, 0}, a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+        for (int j = 0; j < dpc; ++j)
+          for (int k = 0; k < 3; ++k) {
+            X[k] += pt[1 + j] * G[3 * j + k];
+            a[k] += pt[1 + dpc + j] * G[3 * j + k];
+            b[k] += pt[1 + 2 * dpc + j] * G[3 * j + k];
+          }
+        const double cr0 = a[1] * b[2] - a[2] * b[1], cr1 = a[2] * b[0] - a[0] * b[2],
+                     cr2 = a[0] * b[1] - a[1] * b[0];
+        const double J = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+        const double R0 = X[0] - x0, R1 = X[1] - x1, R2 = X[2] - x2;
+        const double r = sqrt(R0 * R0 + R1 * R1 + R2 * R2);
+        const double r2 = r * r;
+        const double s = 1. / (r * 4 * kPi);
+        const double den = -4 * kPi * r2 * r;
+        const double dn = (R0 / den) * (cr0 / J) + (R1 / den) * (cr1 / J) + (R2 / den) * (cr2 / J);
+        const double wq = J * pt[0];
+        for (int j = 0; j < dpc; ++j) {
+          Nj[j] += dn * pt[1 + j] * wq;
+          Dj[j] += s * pt[1 + j] * wq;
+        }
+      }
+      for (int j = 0; j < dpc; ++j) {
+        atomicAdd(out + 2 * cp[j], (T)Nj[j]);
+        atomicAdd(out + 2 * cp[j] + 1, (T)Dj[j]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Near-field SpMV, fused N.phi and D.dphi on the shared pattern
+// (prec_neumann_matrix.vmult / prec_dirichlet_matrix.vmult, bem_fma.cc:2408-2409).
+// One wavefront per row; the (N, D) pairs of a row are contiguous.
+// ---------------------------------------------------------------------------
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_spmv(int nrows, const int *__restrict__ row_leaf,
+                                              const int *__restrict__ row_beg, const int *__restrict__ rows,
+                                              const int *__restrict__ col_beg, const int *__restrict__ cols,
+                                              const int64_t *__restrict__ val_beg, const T *__restrict__ vals,
+                                              const double *__restrict__ phi, const double *__restrict__ dphi,
+                                              double *__restrict__ outN, double *__restrict__ outD) {
+  using T2 = typename Vec2<T>::type;
+  const int row = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nrows) return;
+  const int leaf = row_leaf[row];
+  const int ri = row - row_beg[leaf];
+  const int cb = col_beg[leaf], C = col_beg[leaf + 1] - cb;
+  const T2 *v = reinterpret_cast<const T2 *>(vals) + val_beg[leaf] + (int64_t)ri * C;
+  const int *cc = cols + cb;
+  T aN = 0, aD = 0;
+  for (int c = lane; c < C; c += 64) {
+    const T2 x = v[c];
+    const int col = cc[c];
+    aN += x.x * (T)phi[col];
+    aD += x.y * (T)dphi[col];
+  }
+  aN = wave_sum(aN);
+  aD = wave_sum(aD);
+  if (lane == 0) {
+    const int node = rows[row];
+    outN[node] = (double)aN;
+    outD[node] = (double)aD;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P2M at the leaves (generate_multipole_expansions, bem_fma.cc:2064-2114, with
+// the unit multipoles of multipole_integrals, bem_fma.cc:1675-1737, folded in:
+// the quad point strength is sum_j N_j value_j JxW / 4pi).  Ker1 is the normal
+// derivative expansion of phi (MultipoleExpansion::AddNormDer,
+// multipole_expansion.cc:124-181), Ker2 the monopole expansion of dphi/dn
+// (Add, :93-122).  One wavefront per leaf, lane = coefficient (n, m); the
+// leaf's quad points are staged through LDS 64 at a time.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(64) k_p2m(
+    const int *__restrict__ leaves, const int *__restrict__ quad_beg, const int *__restrict__ quad_cnt,
+    const int *__restrict__ qperm, const T *__restrict__ qx, const T *__restrict__ qn,
+    const T *__restrict__ qw, const T *__restrict__ qsh, const int *__restrict__ cdofs, int dpc, int nq,
+    const double *__restrict__ phi, const double *__restrict__ dphi, const double *__restrict__ pmin,
+    const double *__restrict__ delta, int p, int kt, int kp, T *__restrict__ ME) {
+  __shared__ T s_q1[64], s_q2[64], s_rho[64], s_x[64], s_s[64], s_beta[64], s_dr[64], s_da[64], s_db[64];
+  const int b = leaves[blockIdx.x];
+  const int lane = threadIdx.x;
+  const double hd = delta[b] * 0.5;
+  const T c0 = (T)(pmin[3 * b] + hd), c1 = (T)(pmin[3 * b + 1] + hd), c2 = (T)(pmin[3 * b + 2] + hd);
+  int nA = 0, mA = 0, nB = 0, mB = 0;
+  const bool hasA = lane < kt, hasB = lane + 64 < kt;
+  {
+    int k = lane, n = 0;
+    while (hasA && k > n) { k -= n + 1; ++n; }
+    nA = n; mA = k;
+    k = lane + 64; n = 0;
+    while (hasB && k > n) { k -= n + 1; ++n; }
+    nB = n; mB = k;
+  }
+  T a1r[2] = {0, 0}, a1i[2] = {0, 0}, a2r[2] = {0, 0}, a2i[2] = {0, 0};
+  const int beg = quad_beg[b], cnt = quad_cnt[b];
+  for (int base = 0; base < cnt; base += 64) {
+    const int i = base + lane;
+    T q1 = 0, q2 = 0, rho = 1, x = 0, s = 1, beta = 0, dr = 0, da = 0, db = 0;
+    if (i < cnt) {
+      const int g = qperm[beg + i];
+      const int cell = g / nq;
+      T s1 = 0, s2 = 0;
+      for (int j = 0; j < dpc; ++j) {
+        const T N = qsh[(int64_t)g * dpc + j];
+        const int d = cdofs[(int64_t)cell * dpc + j];
+        s1 += N * (T)phi[d];
+        s2 += N * (T)dphi[d];
+      }
+      const T wq = qw[g] / T(4 * kPi);
+      q1 = s1 * wq;
+      q2 = s2 * wq;
+      const T d0 = qx[3 * (int64_t)g] - c0, d1 = qx[3 * (int64_t)g + 1] - c1, d2 = qx[3 * (int64_t)g + 2] - c2;
+      const T nx = qn[3 * (int64_t)g], ny = qn[3 * (int64_t)g + 1], nz = qn[3 * (int64_t)g + 2];
+      rho = dsqrt(d0 * d0 + d1 * d1 + d2 * d2);
+      x = d2 / rho;
+      beta = datan2(d1, d0);
+      dr = (d0 / rho) * nx + (d1 / rho) * ny + (d2 / rho) * nz;
+      const T rxy2 = d0 * d0 + d1 * d1;
+      db = (-d1 / rxy2) * nx + (d0 / rxy2) * ny;
+      s = dsqrt(T(1) - x * x);
+      T sb, cbeta;
+      dsincos(beta, &sb, &cbeta);
+      da = (x * cbeta / rho) * nx + (x * sb / rho) * ny + (-s / rho) * nz;
+    }
+    s_q1[lane] = q1; s_q2[lane] = q2; s_rho[lane] = rho; s_x[lane] = x; s_s[lane] = s;
+    s_beta[lane] = beta; s_dr[lane] = dr; s_da[lane] = da; s_db[lane] = db;
+    __syncthreads();
+    const int cntc = min(64, cnt - base);
+    for (int t = 0; t < cntc; ++t) {
+      const T Q1 = s_q1[t], Q2 = s_q2[t], RHO = s_rho[t], X = s_x[t], S = s_s[t], BETA = s_beta[t];
+      const T DR = s_dr[t], DA = s_da[t], DB = s_db[t];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const bool has = u == 0 ? hasA : hasB;
+        if (!has) continue;
+        const int n = u == 0 ? nA : nB, m = u == 0 ? mA : mB;
+        T pmm = PMM<T>(m);
+        for (int k = 0; k < m; ++k) pmm *= -S;
+        T pp = 0, pc = pmm;
+        for (int k = m + 1; k <= n; ++k) {
+          const T pn = RA<T>(k, m) * X * pc - RB<T>(k, m) * pp;
+          pp = pc;
+          pc = pn;
+        }
+        T rn1 = 1;
+        for (int k = 1; k < n; ++k) rn1 *= RHO;
+        const T rn = n > 0 ? rn1 * RHO : T(1);
+        T sm, cm;
+        dsincos(T(m) * BETA, &sm, &cm);
+        if (Q2 != T(0)) {
+          const T mono = Q2 * (pc * rn);
+          a2r[u] += mono * cm;
+          a2i[u] -= mono * sm;
+        }
+        if (Q1 != T(0)) {
+          const T dPs = ((T(n) * X * pc - RS<T>(n, m) * pp) / (X * X - T(1))) * S;
+          const T A = (T(n) * (n > 0 ? rn1 : T(0)) * pc * DR - rn * dPs * DA) * Q1;
+          const T B = (-T(m) * rn * pc * DB) * Q1;
+          a1r[u] += cm * A + sm * B;
+          a1i[u] += cm * B - sm * A;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  T *M1 = ME + (int64_t)b * 2 * kp, *M2 = M1 + kp;
+  if (hasA) {
+    M1[lane] = a1r[0]; M1[kt + lane] = a1i[0];
+    M2[lane] = a2r[0]; M2[kt + lane] = a2i[0];
+  }
+  if (hasB) {
+    M1[lane + 64] = a1r[1]; M1[kt + lane + 64] = a1i[1];
+    M2[lane + 64] = a2r[1]; M2[kt + lane + 64] = a2i[1];
+  }
+  for (int k = 2 * kt + lane; k < kp; k += 64) {
+    M1[k] = T(0);
+    M2[k] = T(0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// M2M (MultipoleExpansion::Add(child), multipole_expansion.cc:184-261, driven
+// by bem_fma.cc:2274-2339): parent = sum over children of T_octant . child.
+// One block of 2*kp threads per parent; thread = (kernel, output row).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_m2m(const int *__restrict__ parents, const int *__restrict__ nchild,
+                      const int *__restrict__ child, const int *__restrict__ octant,
+                      const T *__restrict__ ops, int kp, T *__restrict__ ME) {
+  extern __shared__ unsigned char smem_raw[];
+  T *sv = reinterpret_cast<T *>(smem_raw);
+  const int P = parents[blockIdx.x];
+  const int t = threadIdx.x, ker = t / kp, row = t - ker * kp;
+  T acc = 0;
+  for (int c = 0; c < nchild[P]; ++c) {
+    const int ch = child[8 * P + c];
+    __syncthreads();
+    sv[t] = ME[(int64_t)ch * 2 * kp + t];
+    __syncthreads();
+    const T *op = ops + (int64_t)octant[ch] * kp * kp;
+    const T *v = sv + ker * kp;
+    for (int k = 0; k < kp; ++k) acc += op[(int64_t)k * kp + row] * v[k];
+  }
+  ME[(int64_t)P * 2 * kp + t] = acc;
+}
+
+// L2L (LocalExpansion::Add(parent), local_expansion.cc:107-176, driven by
+// bem_fma.cc:2600-2602): LE[b] += T_octant . LE[parent].
+template <typename T>
+__global__ void k_l2l(const int *__restrict__ tgts, const int *__restrict__ parent,
+                      const int *__restrict__ octant, const T *__restrict__ ops, int kp, T *__restrict__ LE) {
+  extern __shared__ unsigned char smem_raw[];
+  T *sv = reinterpret_cast<T *>(smem_raw);
+  const int b = tgts[blockIdx.x];
+  const int t = threadIdx.x, ker = t / kp, row = t - ker * kp;
+  sv[t] = LE[(int64_t)parent[b] * 2 * kp + t];
+  __syncthreads();
+  const T *op = ops + (int64_t)octant[b] * kp * kp;
+  const T *v = sv + ker * kp;
+  T acc = 0;
+  for (int k = 0; k < kp; ++k) acc += op[(int64_t)k * kp + row] * v[k];
+  LE[(int64_t)b * 2 * kp + t] += acc;
+}
+
+// ---------------------------------------------------------------------------
+// M2L (LocalExpansion::Add(multipole), local_expansion.cc:179-248, driven by
+// bem_fma.cc:2622-2652), regrouped by translation offset: one block per chunk
+// of <= 64 (target, source) pairs sharing one operator, the operator staged in
+// LDS when it fits, 4 pairs per pass so that each operator element read from
+// LDS feeds 4 FMAs.  Results are accumulated into the targets atomically.
+// ---------------------------------------------------------------------------
+template <typename T, bool OP_LDS>
+__global__ void k_m2l(const int *__restrict__ chunk_op, const int *__restrict__ chunk_beg,
+                      const int *__restrict__ chunk_cnt, const int *__restrict__ tgt,
+                      const int *__restrict__ src, const T *__restrict__ ops, int kp,
+                      const T *__restrict__ ME, T *__restrict__ LE) {
+  extern __shared__ unsigned char smem_raw[];
+  T *sop = reinterpret_cast<T *>(smem_raw);
+  T *sv = sop + (OP_LDS ? kp * kp : 0);
+  const int c = blockIdx.x;
+  const int t = threadIdx.x, nt = blockDim.x, ker = t / kp, row = t - ker * kp;
+  const T *opg = ops + (int64_t)chunk_op[c] * kp * kp;
+  if (OP_LDS)
+    for (int i = t; i < kp * kp; i += nt) sop[i] = opg[i];
+  const T *op = OP_LDS ? sop : opg;
+  const int beg = chunk_beg[c], cnt = chunk_cnt[c];
+  const int K2 = 2 * kp;
+  for (int i0 = 0; i0 < cnt; i0 += 4) {
+    const int np = min(4, cnt - i0);
+    __syncthreads();
+    for (int q = 0; q < 4; ++q) sv[q * K2 + t] = (q < np) ? ME[(int64_t)src[beg + i0 + q] * K2 + t] : T(0);
+    __syncthreads();
+    T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    const T *v0 = sv + ker * kp;
+    for (int k = 0; k < kp; ++k) {
+      const T o = op[k * kp + row];
+      a0 += o * v0[k];
+      a1 += o * v0[K2 + k];
+      a2 += o * v0[2 * K2 + k];
+      a3 += o * v0[3 * K2 + k];
+    }
+    atomicAdd(LE + (int64_t)tgt[beg + i0] * K2 + t, a0);
+    if (np > 1) atomicAdd(LE + (int64_t)tgt[beg + i0 + 1] * K2 + t, a1);
+    if (np > 2) atomicAdd(LE + (int64_t)tgt[beg + i0 + 2] * K2 + t, a2);
+    if (np > 3) atomicAdd(LE + (int64_t)tgt[beg + i0 + 3] * K2 + t, a3);
+  }
+}
+
+// Evaluation of both kernels' expansions at a point: M2P
+// (MultipoleExpansion::Evaluate, multipole_expansion.cc:264-298, rho^(-n-1))
+// or L2P (LocalExpansion::Evaluate, local_expansion.cc:251-287, rho^n).
+template <typename T, bool LOCAL>
+__device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, int kp, T d0, T d1, T d2,
+                                      T &v1, T &v2) {
+  const T rho = dsqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  const T x = d2 / rho;
+  const T beta = datan2(d1, d0);
+  const T s = dsqrt(T(1) - x * x);
+  T sb, cb;
+  dsincos(beta, &sb, &cb);
+  const T step = LOCAL ? rho : T(1) / rho;
+  T rstart = LOCAL ? T(1) : step;
+  T pmm = 1, cm = 1, sm = 0;
+  v1 = 0;
+  v2 = 0;
+  for (int m = 0; m <= p; ++m) {
+    if (m > 0) {
+      pmm *= -PMF<T>(m) * s;
+      const T c2 = cm * cb - sm * sb;
+      sm = sm * cb + cm * sb;
+      cm = c2;
+      rstart *= step;
+    }
+    T pp = 0, pc = pmm, r = rstart;
+    const T f2 = m ? T(2) : T(1);
+    for (int n = m; n <= p; ++n) {
+      if (n > m) {
+        const T pn = RA<T>(n, m) * x * pc - RB<T>(n, m) * pp;
+        pp = pc;
+        pc = pn;
+        r *= step;
+      }
+      const int idx = n * (n + 1) / 2 + m;
+      const T f = pc * r * f2;
+      v1 += (E[idx] * cm - E[kt + idx] * sm) * f;
+      v2 += (E[kp + idx] * cm - E[kp + kt + idx] * sm) * f;
+    }
+  }
+}
+
+// M2P for smaller far blocks (bem_fma.cc:2664-2704): one thread per (pair, node).
+template <typename T>
+__global__ void __launch_bounds__(256) k_m2p(int npairs, const int64_t *__restrict__ work_beg,
+                                             const int *__restrict__ tgt, const int *__restrict__ src,
+                                             const int *__restrict__ node_beg, const int *__restrict__ nperm,
+                                             const double *__restrict__ xyz, const double *__restrict__ pmin,
+                                             const double *__restrict__ delta, int p, int kt, int kp,
+                                             const T *__restrict__ ME, double *__restrict__ outN,
+                                             double *__restrict__ outD) {
+  const int64_t total = work_beg[npairs];
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w >= total) return;
+  int lo = 0, hi = npairs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (work_beg[mid] <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  const int b = tgt[lo], sblk = src[lo];
+  const int node = nperm[node_beg[b] + (int)(w - work_beg[lo])];
+  const double hd = delta[sblk] * 0.5;
+  const T d0 = (T)(xyz[3 * node] - (pmin[3 * sblk] + hd));
+  const T d1 = (T)(xyz[3 * node + 1] - (pmin[3 * sblk + 1] + hd));
+  const T d2 = (T)(xyz[3 * node + 2] - (pmin[3 * sblk + 2] + hd));
+  T v1, v2;
+  eval2<T, false>(ME + (int64_t)sblk * 2 * kp, p, kt, kp, d0, d1, d2, v1, v2);
+  atomicAdd(outN + node, (double)v1);
+  atomicAdd(outD + node, (double)v2);
+}
+
+// L2P at every node of every leaf (bem_fma.cc:2959-2986): one thread per node.
+template <typename T>
+__global__ void __launch_bounds__(256) k_l2p(int nrows, const int *__restrict__ rows,
+                                             const int *__restrict__ row_leaf, const int *__restrict__ leaf_block,
+                                             const double *__restrict__ xyz, const double *__restrict__ pmin,
+                                             const double *__restrict__ delta, int p, int kt, int kp,
+                                             const T *__restrict__ LE, double *__restrict__ outN,
+                                             double *__restrict__ outD) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int node = rows[r];
+  const int b = leaf_block[row_leaf[r]];
+  const double hd = delta[b] * 0.5;
+  const T d0 = (T)(xyz[3 * node] - (pmin[3 * b] + hd));
+  const T d1 = (T)(xyz[3 * node + 1] - (pmin[3 * b + 1] + hd));
+  const T d2 = (T)(xyz[3 * node + 2] - (pmin[3 * b + 2] + hd));
+  T v1, v2;
+  eval2<T, true>(LE + (int64_t)b * 2 * kp, p, kt, kp, d0, d1, d2, v1, v2);
+  outN[node] += (double)v1;
+  outD[node] += (double)v2;
+}
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ a, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// BEMProblem operator pieces (bem_problem.cc:1164-1260) and the constrained
+// rows of ConstrainedOperator::vmult (constrained_matrix.h:91-135).
+// ---------------------------------------------------------------------------
+__global__ void k_vmult_pre(int64_t n, const double *__restrict__ src, const uint8_t *__restrict__ dir,
+                            const double *__restrict__ shift, double *__restrict__ sp, double *__restrict__ sd) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double sh = shift ? *shift : 0.0;
+  sp[i] = dir[i] ? 0.0 : (src[i] - sh);
+  sd[i] = dir[i] ? src[i] : 0.0;
+}
+__global__ void k_vmult_post(int64_t n, const double *__restrict__ N, const double *__restrict__ D,
+                             const double *__restrict__ alpha, const double *__restrict__ sp, double *__restrict__ dst) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dst[i] = -D[i] + N[i] + alpha[i] * sp[i];
+}
+__global__ void k_rhs_pre(int64_t n, const double *__restrict__ g, const uint8_t *__restrict__ dir,
+                          double *__restrict__ sp, double *__restrict__ sd) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  sp[i] = dir[i] ? g[i] : 0.0;
+  sd[i] = dir[i] ? 0.0 : g[i];
+}
+__global__ void k_rhs_post(int64_t n, const double *__restrict__ N, const double *__restrict__ D,
+                           const double *__restrict__ alpha, const double *__restrict__ sp, double *__restrict__ dst) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dst[i] = -(N[i] + alpha[i] * sp[i]) + D[i];
+}
+__global__ void k_shift(int64_t n, double *__restrict__ v, const double *__restrict__ s) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) v[i] -= *s;
+}
+__global__ void k_constrain(int64_t n, const uint8_t *__restrict__ is_c, const int *__restrict__ cptr,
+                            const int *__restrict__ cidx, const double *__restrict__ cval,
+                            const double *__restrict__ src, double *__restrict__ dst) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n || !is_c[i]) return;
+  double v = src[i];
+  for (int k = cptr[i]; k < cptr[i + 1]; ++k) v -= cval[k] * src[cidx[k]];
+  dst[i] = v;
+}
+__global__ void k_set_constrained(int64_t n, const uint8_t *__restrict__ is_c, const double *__restrict__ inh,
+                                  double *__restrict__ rhs) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n && is_c[i]) rhs[i] = inh[i];
+}
+
+// ---------------------------------------------------------------------------
+// GMRES building blocks (FP64).  Dots of w against nv basis vectors: blocks
+// (chunk, j) write partial sums; a second kernel finishes them.
+// ---------------------------------------------------------------------------
+constexpr int kRedThreads = 256;
+constexpr int kRedChunk = kRedThreads * 8;
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+  return r;
+}
+
+__global__ void __launch_bounds__(kRedThreads) k_mdot(int64_t n, int64_t ldv, const double *__restrict__ V,
+                                                      const double *__restrict__ w, double *__restrict__ part) {
+  __shared__ double sh[kRedThreads / 64];
+  const int j = blockIdx.y;
+  const double *v = V + (int64_t)j * ldv;
+  const int64_t beg = (int64_t)blockIdx.x * kRedChunk;
+  const int64_t end = min(n, beg + (int64_t)kRedChunk);
+  double acc = 0;
+  for (int64_t i = beg + threadIdx.x; i < end; i += kRedThreads) acc += v[i] * w[i];
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[(int64_t)j * gridDim.x + blockIdx.x] = r;
+}
+__global__ void k_finish(int nchunk, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double sh[kRedThreads / 64];
+  const int j = blockIdx.x;
+  double acc = 0;
+  for (int i = threadIdx.x; i < nchunk; i += blockDim.x) acc += part[(int64_t)j * nchunk + i];
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[j] = r;
+}
+__global__ void k_sqrt_inplace(double *v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) v[0] = sqrt(v[0]);
+}
+// w -= sum_j V_j h_j
+__global__ void k_gemv_sub(int64_t n, int64_t ldv, int nv, const double *__restrict__ V,
+                           const double *__restrict__ h, double *__restrict__ w) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = w[i];
+  for (int j = 0; j < nv; ++j) acc -= V[(int64_t)j * ldv + i] * h[j];
+  w[i] = acc;
+}
+// x += sum_j V_j y_j
+__global__ void k_gemv_add(int64_t n, int64_t ldv, int nv, const double *__restrict__ V,
+                           const double *__restrict__ y, double *__restrict__ x) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = x[i];
+  for (int j = 0; j < nv; ++j) acc += V[(int64_t)j * ldv + i] * y[j];
+  x[i] = acc;
+}
+__global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + b * y[i];
+}
+__global__ void k_scale_to(int64_t n, const double *__restrict__ x, double s, double *__restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = s * x[i];
+}
+
+// ===========================================================================
+// Host orchestration
+// ===========================================================================
+static inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+template <typename T>
+static void up(DevBuf &b, const std::vector<T> &v) {
+  b.release();
+  const size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
+  HIPCHK(hipMalloc(&b.p, bytes));
+  b.bytes = bytes;
+  if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+static void alloc(DevBuf &b, size_t bytes) {
+  b.release();
+  bytes = std::max<size_t>(bytes, 16);
+  HIPCHK(hipMalloc(&b.p, bytes));
+  b.bytes = bytes;
+}
+template <typename D, typename S>
+static std::vector<D> cast_vec(const std::vector<S> &v) {
+  std::vector<D> o(v.size());
+  for (size_t i = 0; i < v.size(); ++i) o[i] = (D)v[i];
+  return o;
+}
+
+void Engine::init_device(int dev, void *st) {
+  device = dev;
+  HIPCHK(hipSetDevice(dev));
+  if (st) {
+    stream = (hipStream_t)st;
+    own_stream = false;
+  } else {
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    own_stream = true;
+  }
+  upload_legendre_tables();
+}
+
+Engine::~Engine() {
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+void Engine::sync() { HIPCHK(hipStreamSynchronize(stream)); }
+
+void Engine::upload_mesh_and_tree() {
+  HIPCHK(hipSetDevice(device));
+  up(d_xyz, mesh.xyz);
+  up(d_geom, mesh.geom);
+  up(d_cdofs, mesh.cdofs);
+  up(d_dbl_ptr, mesh.dbl_ptr);
+  up(d_dbl_idx, mesh.dbl_idx);
+  up(d_dir, mesh.dir);
+  up(d_pmin, tree.pmin);
+  up(d_delta, tree.delta);
+  up(d_nchild, tree.nchild);
+  up(d_child, tree.child);
+  up(d_octant, tree.octant);
+  up(d_node_beg, tree.node_beg);
+  up(d_node_cnt, tree.node_cnt);
+  up(d_nperm, tree.nperm);
+  up(d_quad_beg, tree.quad_beg);
+  up(d_quad_cnt, tree.quad_cnt);
+  up(d_qperm, tree.qperm);
+  up(d_parent, tree.parent);
+  up(d_alpha, std::vector<double>(n, 0.0));
+  alloc(d_sp, n * 8);
+  alloc(d_sd, n * 8);
+  alloc(d_outN, n * 8);
+  alloc(d_outD, n * 8);
+  alloc(d_in, n * 8);
+  alloc(d_out, n * 8);
+  alloc(d_red, (size_t)(nblk(n, kRedChunk) * 128 + 256) * 8);
+}
+
+void Engine::upload_near_plan() {
+  up(d_row_leaf, near.row_leaf);
+  up(d_row_beg, near.row_beg);
+  up(d_rows, near.rows);
+  up(d_col_beg, near.col_beg);
+  up(d_cols, near.cols);
+  up(d_val_beg, near.val_beg);
+  up(d_ent_beg, near.ent_beg);
+  up(d_ent_cell, near.ent_cell);
+  up(d_ent_mask, near.ent_mask);
+  up(d_ent_near, near.ent_near);
+  up(d_ent_colpos, near.ent_colpos);
+  up(d_work_beg, near.work_beg);
+  up(d_leaf_block, near.leaf_block);
+}
+
+template <typename T>
+static void upload_far_t(Engine &e, FarDev<T> &F) {
+  up(F.m2m, cast_vec<T>(e.far.m2m_ops));
+  up(F.l2l, cast_vec<T>(e.far.l2l_ops));
+  up(F.m2l, cast_vec<T>(e.far.m2l_ops));
+  alloc(F.me, (size_t)e.tree.nb * 2 * e.kp * sizeof(T));
+  alloc(F.le, (size_t)e.tree.nb * 2 * e.kp * sizeof(T));
+  HIPCHK(hipMemset(F.me.p, 0, F.me.bytes));
+  HIPCHK(hipMemset(F.le.p, 0, F.le.bytes));
+}
+
+template <typename T>
+static void upload_quad_t(Engine &e, FarDev<T> &F) {
+  up(F.qx, cast_vec<T>(e.qd.x));
+  up(F.qn, cast_vec<T>(e.qd.n));
+  up(F.qw, cast_vec<T>(e.qd.jxw));
+  up(F.qsh, cast_vec<T>(e.qd.sh));
+}
+
+void Engine::upload_quad(bool want_f32) {
+  upload_quad_t<double>(*this, f64);
+  if (want_f32) upload_quad_t<float>(*this, f32);
+  have_f32 = want_f32;
+}
+
+void Engine::upload_far(bool want_f32) {
+  std::vector<int32_t> par, tg;
+  m2m_off.assign(L + 2, 0);
+  l2l_off.assign(L + 2, 0);
+  for (int l = 0; l <= L; ++l) {
+    m2m_off[l] = (int)par.size();
+    par.insert(par.end(), far.m2m_par[l].begin(), far.m2m_par[l].end());
+    l2l_off[l] = (int)tg.size();
+    tg.insert(tg.end(), far.l2l_tgt[l].begin(), far.l2l_tgt[l].end());
+  }
+  m2m_off[L + 1] = (int)par.size();
+  l2l_off[L + 1] = (int)tg.size();
+  up(d_m2m_par, par);
+  up(d_l2l_tgt, tg);
+  up(d_p2m_leaf, far.p2m_leaf);
+  up(d_m2l_tgt, far.m2l_tgt);
+  up(d_m2l_src, far.m2l_src);
+  up(d_chunk_op, far.chunk_op);
+  up(d_chunk_beg, far.chunk_beg);
+  up(d_chunk_cnt, far.chunk_cnt);
+  up(d_m2p_tgt, far.m2p_tgt);
+  up(d_m2p_src, far.m2p_src);
+  up(d_m2p_work, far.m2p_work_beg);
+  upload_far_t<double>(*this, f64);
+  if (want_f32) upload_far_t<float>(*this, f32);
+}
+
+// Singular rules for every local support point js: Telles in each direction
+// (QTelles<2>(sing_quad_order, support_point), bem_fma.cc:127-131), with the
+// shape values and gradients at the rule points.
+void Engine::assemble_near(bool want_f32) {
+  const int ps = 1 + 3 * dpc;
+  std::vector<std::vector<double>> rules(dpc);
+  int maxq = 0;
+  std::vector<int32_t> cnt(dpc);
+  for (int js = 0; js < dpc; ++js) {
+    double sx, sy;
+    unit_support_point(dpc, js, sx, sy);
+    std::vector<double> x1, w1, x2, w2;
+    telles_rule01(sing_order, sx, x1, w1);
+    telles_rule01(sing_order, sy, x2, w2);
+    Quadrature2D r = tensor_rule(x1, w1, x2, w2);
+    const int nqs = (int)r.w.size();
+    cnt[js] = nqs;
+    maxq = std::max(maxq, nqs);
+    rules[js].resize((size_t)nqs * ps);
+    for (int q = 0; q < nqs; ++q) {
+      double *pt = &rules[js][(size_t)q * ps];
+      pt[0] = r.w[q];
+      lagrange_q(dpc, r.pts[2 * q], r.pts[2 * q + 1], pt + 1, pt + 1 + dpc, pt + 1 + 2 * dpc);
+    }
+  }
+  sing_nq = maxq;
+  std::vector<double> flat((size_t)dpc * maxq * ps, 0.0);
+  for (int js = 0; js < dpc; ++js) std::copy(rules[js].begin(), rules[js].end(), flat.begin() + (size_t)js * maxq * ps);
+  up(d_sing, flat);
+  DevBuf d_scnt;
+  up(d_scnt, cnt);
+
+  auto run = [&](auto tag, auto &F) {
+    using T = decltype(tag);
+    alloc(F.vals, (size_t)std::max<int64_t>(near.nnz, 1) * 2 * sizeof(T));
+    HIPCHK(hipMemsetAsync(F.vals.p, 0, F.vals.bytes, stream));
+    const int64_t total = near.work_beg.back();
+    if (near.nleaf > 0 && total > 0) {
+      const unsigned grid = (unsigned)std::min<int64_t>(nblk(total, 256), 65536);
+      hipLaunchKernelGGL(k_assemble_near<T>, dim3(grid), dim3(256), 0, stream, near.nleaf,
+                         d_work_beg.as<int64_t>(), d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(),
+                         d_ent_beg.as<int>(), d_ent_cell.as<int>(), d_ent_mask.as<uint64_t>(),
+                         d_ent_near.as<uint8_t>(), d_ent_colpos.as<int>(), d_val_beg.as<int64_t>(),
+                         d_cdofs.as<int>(), dpc, nq, d_dbl_ptr.as<int>(), d_dbl_idx.as<int>(), d_xyz.as<double>(),
+                         F.qx.template as<T>(), F.qn.template as<T>(), F.qw.template as<T>(),
+                         F.qsh.template as<T>(), d_geom.as<double>(), d_sing.as<double>(), d_scnt.as<int>(),
+                         maxq * ps, F.vals.template as<T>());
+      HIPCHK(hipGetLastError());
+    }
+  };
+  run(double(0), f64);
+  if (want_f32) run(float(0), f32);
+  sync();
+}
+
+template <typename T>
+static FarDev<T> &far_of(Engine &e);
+template <>
+FarDev<double> &far_of<double>(Engine &e) { return e.f64; }
+template <>
+FarDev<float> &far_of<float>(Engine &e) { return e.f32; }
+
+template <typename T>
+void Engine::generate(const double *phi, const double *dphi) {
+  FarDev<T> &F = far_of<T>(*this);
+  const int nleaf = (int)far.p2m_leaf.size();
+  if (nleaf > 0)
+    hipLaunchKernelGGL(k_p2m<T>, dim3(nleaf), dim3(64), 0, stream, d_p2m_leaf.as<int>(), d_quad_beg.as<int>(),
+                       d_quad_cnt.as<int>(), d_qperm.as<int>(), F.qx.template as<T>(), F.qn.template as<T>(),
+                       F.qw.template as<T>(), F.qsh.template as<T>(), d_cdofs.as<int>(), dpc, nq, phi, dphi,
+                       d_pmin.as<double>(), d_delta.as<double>(), p, kt, kp, F.me.template as<T>());
+  for (int l = L; l >= 1; --l) {
+    const int cnt = m2m_off[l + 1] - m2m_off[l];
+    if (cnt == 0) continue;
+    hipLaunchKernelGGL(k_m2m<T>, dim3(cnt), dim3(2 * kp), 2 * kp * sizeof(T), stream,
+                       d_m2m_par.as<int>() + m2m_off[l], d_nchild.as<int>(), d_child.as<int>(), d_octant.as<int>(),
+                       F.m2m.template as<T>() + (size_t)l * 8 * kp * kp, kp, F.me.template as<T>());
+  }
+  HIPCHK(hipGetLastError());
+}
+
+template <typename T>
+void Engine::products(const double *phi, const double *dphi, double *outN, double *outD) {
+  FarDev<T> &F = far_of<T>(*this);
+  const int nrows = (int)near.rows.size();
+  if (nrows > 0)
+    hipLaunchKernelGGL(k_spmv<T>, dim3(nblk((int64_t)nrows * 64, 256)), dim3(256), 0, stream, nrows,
+                       d_row_leaf.as<int>(), d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(),
+                       d_cols.as<int>(), d_val_beg.as<int64_t>(), F.vals.template as<T>(), phi, dphi, outN, outD);
+  const int64_t nle = (int64_t)tree.nb * 2 * kp;
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(nle, 256), 4096)), dim3(256), 0, stream,
+                     F.le.template as<T>(), nle, T(0));
+  const int nchunk = (int)far.chunk_op.size();
+  if (nchunk > 0) {
+    const size_t op_bytes = (size_t)kp * kp * sizeof(T), v_bytes = (size_t)4 * 2 * kp * sizeof(T);
+    if (op_bytes + v_bytes <= 64 * 1024)
+      hipLaunchKernelGGL((k_m2l<T, true>), dim3(nchunk), dim3(2 * kp), op_bytes + v_bytes, stream,
+                         d_chunk_op.as<int>(), d_chunk_beg.as<int>(), d_chunk_cnt.as<int>(), d_m2l_tgt.as<int>(),
+                         d_m2l_src.as<int>(), F.m2l.template as<T>(), kp, F.me.template as<T>(), F.le.template as<T>());
+    else
+      hipLaunchKernelGGL((k_m2l<T, false>), dim3(nchunk), dim3(2 * kp), v_bytes, stream, d_chunk_op.as<int>(),
+                         d_chunk_beg.as<int>(), d_chunk_cnt.as<int>(), d_m2l_tgt.as<int>(), d_m2l_src.as<int>(),
+                         F.m2l.template as<T>(), kp, F.me.template as<T>(), F.le.template as<T>());
+  }
+  const int npairs = (int)far.m2p_tgt.size();
+  if (npairs > 0) {
+    const int64_t total = far.m2p_work_beg.back();
+    if (total > 0)
+      hipLaunchKernelGGL(k_m2p<T>, dim3(nblk(total, 256)), dim3(256), 0, stream, npairs, d_m2p_work.as<int64_t>(),
+                         d_m2p_tgt.as<int>(), d_m2p_src.as<int>(), d_node_beg.as<int>(), d_nperm.as<int>(),
+                         d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt, kp,
+                         F.me.template as<T>(), outN, outD);
+  }
+  for (int l = 2; l <= L; ++l) {
+    const int cnt = l2l_off[l + 1] - l2l_off[l];
+    if (cnt == 0) continue;
+    hipLaunchKernelGGL(k_l2l<T>, dim3(cnt), dim3(2 * kp), 2 * kp * sizeof(T), stream,
+                       d_l2l_tgt.as<int>() + l2l_off[l], d_parent.as<int>(), d_octant.as<int>(),
+                       F.l2l.template as<T>() + (size_t)l * 8 * kp * kp, kp, F.le.template as<T>());
+  }
+  if (nrows > 0)
+    hipLaunchKernelGGL(k_l2p<T>, dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
+                       d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
+                       d_delta.as<double>(), p, kt, kp, F.le.template as<T>(), outN, outD);
+  HIPCHK(hipGetLastError());
+}
+
+template <typename T>
+void Engine::matvec(const double *phi, const double *dphi, double *outN, double *outD) {
+  generate<T>(phi, dphi);
+  products<T>(phi, dphi, outN, outD);
+}
+
+template void Engine::matvec<double>(const double *, const double *, double *, double *);
+template void Engine::matvec<float>(const double *, const double *, double *, double *);
+template void Engine::generate<double>(const double *, const double *);
+template void Engine::products<double>(const double *, const double *, double *, double *);
+
+double Engine::norm2(const double *v) {
+  const int nchunk = (int)nblk(n, kRedChunk);
+  double *part = d_red.as<double>();
+  double *res = part + (size_t)nchunk * 128;
+  hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, v, v, part);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, stream, nchunk, part, res);
+  double h = 0;
+  HIPCHK(hipMemcpyAsync(&h, res, 8, hipMemcpyDeviceToHost, stream));
+  sync();
+  return std::sqrt(h);
+}
+
+void Engine::vmult(const double *src, double *dst, int precision) {
+  const unsigned g = nblk(n, 256);
+  double *shift = nullptr;
+  double *res = d_red.as<double>() + (size_t)nblk(n, kRedChunk) * 128;
+  if (!have_dirichlet) {
+    // serv_phi -= ||src||_2 (bem_problem.cc:1169-1172)
+    const int nchunk = (int)nblk(n, kRedChunk);
+    hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, src, src, d_red.as<double>());
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, stream, nchunk, d_red.as<double>(), res);
+    hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(64), 0, stream, res);
+    shift = res;
+  }
+  hipLaunchKernelGGL(k_vmult_pre, dim3(g), dim3(256), 0, stream, n, src, d_dir.as<uint8_t>(), shift,
+                     d_sp.as<double>(), d_sd.as<double>());
+  if (precision == PBFMM_MIXED && have_f32)
+    matvec<float>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
+  else
+    matvec<double>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
+  hipLaunchKernelGGL(k_vmult_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_outD.as<double>(),
+                     d_alpha.as<double>(), d_sp.as<double>(), dst);
+  if (!have_dirichlet) {
+    // dst -= ||dst||_2 (bem_problem.cc:1214-1215)
+    const int nchunk = (int)nblk(n, kRedChunk);
+    hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, dst, dst, d_red.as<double>());
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, stream, nchunk, d_red.as<double>(), res);
+    hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(64), 0, stream, res);
+    hipLaunchKernelGGL(k_shift, dim3(g), dim3(256), 0, stream, n, dst, res);
+  }
+  if (have_constraints)
+    hipLaunchKernelGGL(k_constrain, dim3(g), dim3(256), 0, stream, n, d_c_is.as<uint8_t>(), d_c_ptr.as<int>(),
+                       d_c_idx.as<int>(), d_c_val.as<double>(), src, dst);
+  HIPCHK(hipGetLastError());
+}
+
+void Engine::compute_alpha() {
+  // alpha = N.(-1) through the FP64 FMM (bem_problem.cc:1138-1162)
+  const unsigned g = nblk(n, 256);
+  hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d_sp.as<double>(), n, -1.0);
+  hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d_sd.as<double>(), n, 0.0);
+  matvec<double>(d_sp.as<double>(), d_sd.as<double>(), d_alpha.as<double>(), d_outD.as<double>());
+  HIPCHK(hipGetLastError());
+}
+
+void Engine::compute_rhs(const double *gvec, double *rhs) {
+  const unsigned g = nblk(n, 256);
+  hipLaunchKernelGGL(k_rhs_pre, dim3(g), dim3(256), 0, stream, n, gvec, d_dir.as<uint8_t>(), d_sp.as<double>(),
+                     d_sd.as<double>());
+  matvec<double>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
+  hipLaunchKernelGGL(k_rhs_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_outD.as<double>(),
+                     d_alpha.as<double>(), d_sp.as<double>(), rhs);
+  HIPCHK(hipGetLastError());
+}
+
+// Restarted GMRES with classical Gram-Schmidt and one reorthogonalisation
+// (CGS2), FP64 basis and Hessenberg, operator precision as requested.  Solves
+// A x = b from the given x until the Arnoldi residual estimate reaches
+// tol_abs or maxit operator applications have been spent.
+static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int m, int maxit, int precision,
+                      int &it, double &est) {
+  const int64_t n = e.n;
+  const unsigned g = nblk(n, 256);
+  const int nchunk = (int)nblk(n, kRedChunk);
+  double *V = e.d_V.as<double>(), *w = e.d_w.as<double>(), *hd = e.d_h.as<double>();
+  double *part = e.d_red.as<double>();
+  std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1), h(m + 2), y(m);
+  auto dots = [&](int nv, const double *vec, double *out_dev) {
+    hipLaunchKernelGGL(k_mdot, dim3(nchunk, nv), dim3(kRedThreads), 0, e.stream, n, n, V, vec, part);
+    hipLaunchKernelGGL(k_finish, dim3(nv), dim3(kRedThreads), 0, e.stream, nchunk, part, out_dev);
+  };
+  while (it < maxit) {
+    // r = b - A x into V_0
+    e.vmult(x, w, precision);
+    hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, b, -1.0, w);
+    const double beta = e.norm2(w);
+    est = beta;
+    if (beta <= tol_abs) return 0;
+    hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, w, 1.0 / beta, V);
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0] = beta;
+    int k = 0;
+    bool done = false;
+    for (; k < m && it < maxit; ++k) {
+      double *vk1 = V + (int64_t)(k + 1) * n;
+      e.vmult(V + (int64_t)k * n, vk1, precision);
+      ++it;
+      for (int j = 0; j <= k + 1; ++j) h[j] = 0.0;
+      for (int pass = 0; pass < 2; ++pass) {
+        dots(k + 1, vk1, hd);
+        hipLaunchKernelGGL(k_gemv_sub, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, hd, vk1);
+        std::vector<double> hp(k + 1);
+        HIPCHK(hipMemcpyAsync(hp.data(), hd, (k + 1) * 8, hipMemcpyDeviceToHost, e.stream));
+        e.sync();
+        for (int j = 0; j <= k; ++j) h[j] += hp[j];
+      }
+      const double hn = e.norm2(vk1);
+      h[k + 1] = hn;
+      if (hn > 0) hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, vk1, 1.0 / hn, vk1);
+      for (int j = 0; j < k; ++j) {
+        const double a = h[j], bb = h[j + 1];
+        h[j] = cs[j] * a + sn[j] * bb;
+        h[j + 1] = -sn[j] * a + cs[j] * bb;
+      }
+      const double den = std::hypot(h[k], h[k + 1]);
+      cs[k] = h[k] / den;
+      sn[k] = h[k + 1] / den;
+      h[k] = den;
+      for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] = h[j];
+      gv[k + 1] = -sn[k] * gv[k];
+      gv[k] = cs[k] * gv[k];
+      est = std::fabs(gv[k + 1]);
+      if (est <= tol_abs || hn == 0) {
+        ++k;
+        done = true;
+        break;
+      }
+    }
+    for (int i = k - 1; i >= 0; --i) {
+      double s = gv[i];
+      for (int j = i + 1; j < k; ++j) s -= H[(size_t)i * m + j] * y[j];
+      y[i] = s / H[(size_t)i * m + i];
+    }
+    HIPCHK(hipMemcpyAsync(hd, y.data(), k * 8, hipMemcpyHostToDevice, e.stream));
+    hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, V, hd, x);
+    e.sync();
+    if (done) return 0;
+  }
+  return 1;
+}
+
+int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit, int precision, int *iters,
+                  double *relres) {
+  if (restart != gm_restart) {
+    alloc(d_V, (size_t)(restart + 1) * n * 8);
+    alloc(d_w, n * 8);
+    alloc(d_r, n * 8);
+    alloc(d_x, n * 8);
+    alloc(d_h, (size_t)(restart + 2) * 8);
+    gm_restart = restart;
+  }
+  const unsigned g = nblk(n, 256);
+  const double bn = std::max(norm2(b), 1e-300);
+  int it = 0;
+  double est = 0, rr = 1;
+  double *r = d_r.as<double>(), *d = d_x.as<double>();
+  for (int outer = 0; outer < 100 && it < maxit; ++outer) {
+    // true FP64 residual
+    vmult(x, r, PBFMM_FP64);
+    hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, b, -1.0, r);
+    const double rn = norm2(r);
+    rr = rn / bn;
+    if (rr <= tol) break;
+    if (precision == PBFMM_MIXED && have_f32) {
+      // iterative refinement: FP32-operator GMRES on the FP64 residual
+      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d, n, 0.0);
+      const double tin = std::max(tol * bn * 0.5, 1e-5 * rn);
+      gmres_core(*this, r, d, tin, restart, maxit, PBFMM_MIXED, it, est);
+      hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, d, 1.0, x);
+    } else {
+      gmres_core(*this, b, x, tol * bn * 0.5, restart, maxit, PBFMM_FP64, it, est);
+    }
+  }
+  sync();
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  return rr <= tol ? 0 : 1;
+}
+
+}  // namespace pbfmm

@@ -1,0 +1,129 @@
+// Internal declarations of libpbfmm: host-side setup structures and the
+// device-side state shared by the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pbfmm.h"
+
+namespace pbfmm {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kToll = 0.001;  // box adjacency tolerance of bem_fma.cc:1
+
+inline int kt_of(int p) { return (p + 1) * (p + 2) / 2; }
+// padded length of one real expansion vector [Re(0..Kt-1), Im(0..Kt-1)]
+inline int kp_of(int p) { return ((2 * kt_of(p) + 15) / 16) * 16; }
+
+// ---------------------------------------------------------------------------
+// Mesh and regular quadrature data (host, FP64)
+// ---------------------------------------------------------------------------
+struct HostMesh {
+  int64_t n = 0, ncells = 0;
+  int dpc = 4;
+  std::vector<double> xyz;      // 3n
+  std::vector<double> geom;     // ncells*dpc*3 mapping nodes
+  std::vector<int32_t> cdofs;   // ncells*dpc
+  std::vector<int32_t> dbl_ptr, dbl_idx;  // double-node sets, each holding the dof itself
+  std::vector<uint8_t> dir;
+  std::vector<int32_t> d2c_ptr, d2c_idx;  // dof -> cells (ascending)
+};
+
+struct Quadrature2D {
+  std::vector<double> pts, w;  // 2*n, n on the unit square, x fastest
+};
+
+// Gauss-Legendre on [0,1] (Golub-Welsch free Newton iteration), Telles cubic
+// transformation, tensor products; Lagrange Q1/Q2 in FE_Q numbering.
+void gauss_legendre01(int n, std::vector<double> &x, std::vector<double> &w);
+void telles_rule01(int n, double x0, std::vector<double> &x, std::vector<double> &w);
+Quadrature2D tensor_rule(const std::vector<double> &x1, const std::vector<double> &w1,
+                         const std::vector<double> &x2, const std::vector<double> &w2);
+void lagrange_q(int dpc, double xi, double eta, double *N, double *dNx, double *dNy);
+void unit_support_point(int dpc, int j, double &xi, double &eta);
+
+struct QuadData {
+  int nq = 0;
+  std::vector<double> x, n, jxw, sh;  // per global quad id g = cell*nq+q
+};
+void compute_quad_data(const HostMesh &m, const Quadrature2D &rule, QuadData &qd);
+
+// ---------------------------------------------------------------------------
+// Octree with the reference list semantics, in flat arrays
+// ---------------------------------------------------------------------------
+struct Tree {
+  int L = 0;
+  int nb = 0;
+  std::vector<int32_t> level, parent, nchild, child, octant;  // child: 8 per block
+  std::vector<double> pmin, delta;                            // 3*nb, nb
+  std::vector<int32_t> startL, endL;                          // L+1
+  std::vector<int32_t> leaves;                                // childlessList order
+  std::vector<std::vector<int32_t>> dofs_filled;              // per level
+  std::vector<int32_t> nperm, node_beg, node_cnt;
+  std::vector<int32_t> qperm, quad_beg, quad_cnt;
+  std::vector<int32_t> qblock;  // (L+1) x nquad
+  std::vector<int32_t> nsub, sub_beg;
+  std::vector<std::vector<int32_t>> nn, il, nil;  // index sub_beg[b]+s
+  void center(int b, double c[3]) const {
+    for (int k = 0; k < 3; ++k) c[k] = pmin[3 * b + k] + delta[b] / 2.0;
+  }
+};
+void build_tree(const HostMesh &m, const QuadData &qd, int levels, int max_nodes, Tree &t);
+
+// ---------------------------------------------------------------------------
+// Near field: leaf-blocked storage.  All rows (nodes) of one leaf share the
+// same column set, so values are a dense rows x cols block per leaf and the
+// column indices are stored once per leaf.
+// ---------------------------------------------------------------------------
+struct NearPlan {
+  int nleaf = 0;                    // leaves holding nodes
+  std::vector<int32_t> leaf_block;  // block id
+  std::vector<int32_t> row_beg, rows;   // rows (node ids) per leaf
+  std::vector<int32_t> col_beg, cols;   // columns per leaf
+  std::vector<int64_t> val_beg;         // offset of the rows x cols block
+  std::vector<int32_t> ent_beg;         // entries per leaf
+  std::vector<int32_t> ent_cell;
+  std::vector<uint64_t> ent_mask;       // quad points of the cell in this entry
+  std::vector<uint8_t> ent_near;        // 1: leaf near list (may be singular)
+  std::vector<int32_t> ent_colpos;      // dpc per entry
+  std::vector<int64_t> work_beg;        // prefix of rows*entries per leaf
+  std::vector<int32_t> row_leaf;        // leaf-ordered row -> leaf
+  int64_t nnz = 0;
+};
+void build_near_plan(const HostMesh &m, const QuadData &qd, const Tree &t, NearPlan &np);
+
+// ---------------------------------------------------------------------------
+// Far field plan and translation operators
+// ---------------------------------------------------------------------------
+struct FarPlan {
+  int p = 0, kt = 0, kp = 0;
+  // upward
+  std::vector<int32_t> p2m_leaf;              // all childless blocks
+  std::vector<std::vector<int32_t>> m2m_par;  // per child level l: parents at l-1 with children
+  // downward
+  std::vector<std::vector<int32_t>> l2l_tgt;  // per level: blocks of dofs_filled[l], l>=2
+  // M2L pairs sorted by (level, class); chunks of <= 64 pairs of one class
+  std::vector<int32_t> m2l_tgt, m2l_src;
+  std::vector<int32_t> chunk_op, chunk_beg, chunk_cnt;
+  std::vector<int64_t> m2l_per_level;
+  int n_m2l_ops = 0;
+  // M2P pairs (target block, smaller far source block)
+  std::vector<int32_t> m2p_tgt, m2p_src;
+  std::vector<int64_t> m2p_work_beg;  // prefix of node counts
+  // L2P
+  std::vector<int32_t> l2p_leaf;  // childless blocks holding nodes
+  // operators, transposed (column-major: op[k*kp + row]), FP64
+  std::vector<double> m2m_ops;  // (L+1) levels x 8 octants
+  std::vector<double> l2l_ops;  // (L+1) x 8
+  std::vector<double> m2l_ops;  // n_m2l_ops
+};
+void build_far_plan(const Tree &t, int p, FarPlan &fp);
+// R-linear translation operators on [Re, Im] vectors (padded to kp).
+void op_m2m(int p, const double d[3], double *opT);  // d = child - parent
+void op_l2l(int p, const double d[3], double *opT);  // d = parent - child
+void op_m2l(int p, const double d[3], double *opT);  // d = source - target
+
+}  // namespace pbfmm

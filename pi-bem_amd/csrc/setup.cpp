@@ -1,0 +1,611 @@
+// Host-side setup of libpbfmm: quadrature, FE values, the octree and its
+// interaction lists, the near-field work plan and the far-field plan with its
+// translation operators.  Tree semantics follow
+// BEMFMA::generate_octree_blocking (source/bem_fma.cc:3614-4801); the data
+// layout is flat (sorted index ranges and CSR lists) rather than per-block
+// std::set / std::map objects.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <array>
+#include <map>
+#include <numeric>
+#include <stdexcept>
+
+#include "pbfmm_impl.h"
+
+namespace pbfmm {
+
+// ---------------------------------------------------------------------------
+// Quadrature rules
+// ---------------------------------------------------------------------------
+void gauss_legendre01(int n, std::vector<double> &x, std::vector<double> &w) {
+  x.resize(n);
+  w.resize(n);
+  for (int i = 0; i < n; ++i) {
+    long double z = std::cos(kPi * (n - i - 0.25) / (n + 0.5));  // ascending roots
+    long double d = 0;
+    for (int it = 0; it < 100; ++it) {
+      long double pm1 = 1, p0 = z;
+      for (int k = 2; k <= n; ++k) {
+        long double pk = ((2 * k - 1) * z * p0 - (k - 1) * pm1) / k;
+        pm1 = p0;
+        p0 = pk;
+      }
+      if (n == 1) { pm1 = 1; p0 = z; }
+      d = n * (z * p0 - pm1) / (z * z - 1);
+      long double dz = p0 / d;
+      z -= dz;
+      if (std::fabs((double)dz) < 1e-19) break;
+    }
+    x[i] = (double)((z + 1) / 2);
+    w[i] = (double)(1 / ((1 - z * z) * d * d));
+  }
+}
+
+// Telles (1987) cubic transformation on [-1,1] towards eta_bar = 2 x0 - 1.
+void telles_rule01(int n, double x0, std::vector<double> &x, std::vector<double> &w) {
+  std::vector<double> gx, gw;
+  gauss_legendre01(n, gx, gw);
+  const double eb = 2.0 * x0 - 1.0;
+  const double es = eb * eb - 1.0;
+  const double gb = std::cbrt(eb * es + std::fabs(es)) + std::cbrt(eb * es - std::fabs(es)) + eb;
+  const double den = 1.0 + 3.0 * gb * gb;
+  x.clear();
+  w.clear();
+  for (int i = 0; i < n; ++i) {
+    if (std::fabs(gx[i] - x0) <= 1e-10) continue;  // Jacobian vanishes there
+    const double g = 2.0 * gx[i] - 1.0, t = g - gb;
+    x.push_back(0.5 * ((t * t * t + gb * (gb * gb + 3.0)) / den + 1.0));
+    w.push_back(gw[i] * 3.0 * t * t / den);
+  }
+}
+
+Quadrature2D tensor_rule(const std::vector<double> &x1, const std::vector<double> &w1,
+                         const std::vector<double> &x2, const std::vector<double> &w2) {
+  Quadrature2D r;
+  for (size_t j = 0; j < x2.size(); ++j)
+    for (size_t i = 0; i < x1.size(); ++i) {
+      r.pts.push_back(x1[i]);
+      r.pts.push_back(x2[j]);
+      r.w.push_back(w1[i] * w2[j]);
+    }
+  return r;
+}
+
+// FE_Q numbering: vertices (0,0),(1,0),(0,1),(1,1); lines x=0, x=1, y=0, y=1;
+// interior.  1-D factors indexed 0 -> node 0, 1 -> node 1, 2 -> midpoint.
+static const int kQmap[9][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}, {0, 2}, {1, 2}, {2, 0}, {2, 1}, {2, 2}};
+
+void unit_support_point(int dpc, int j, double &xi, double &eta) {
+  static const double c[3] = {0.0, 1.0, 0.5};
+  (void)dpc;
+  xi = c[kQmap[j][0]];
+  eta = c[kQmap[j][1]];
+}
+
+void lagrange_q(int dpc, double xi, double eta, double *N, double *dNx, double *dNy) {
+  double a[3], b[3], da[3], db[3];
+  auto f = [&](double t, double *L, double *dL) {
+    if (dpc == 4) {
+      L[0] = 1 - t; L[1] = t; L[2] = 0;
+      dL[0] = -1; dL[1] = 1; dL[2] = 0;
+    } else {
+      L[0] = (1 - t) * (1 - 2 * t); L[1] = t * (2 * t - 1); L[2] = 4 * t * (1 - t);
+      dL[0] = 4 * t - 3; dL[1] = 4 * t - 1; dL[2] = 4 - 8 * t;
+    }
+  };
+  f(xi, a, da);
+  f(eta, b, db);
+  for (int j = 0; j < dpc; ++j) {
+    const int u = kQmap[j][0], v = kQmap[j][1];
+    if (N) N[j] = a[u] * b[v];
+    if (dNx) dNx[j] = da[u] * b[v];
+    if (dNy) dNy[j] = a[u] * db[v];
+  }
+}
+
+void compute_quad_data(const HostMesh &m, const Quadrature2D &rule, QuadData &qd) {
+  const int nq = (int)rule.w.size(), dpc = m.dpc;
+  qd.nq = nq;
+  const int64_t NQ = m.ncells * nq;
+  qd.x.resize(3 * NQ);
+  qd.n.resize(3 * NQ);
+  qd.jxw.resize(NQ);
+  qd.sh.resize(NQ * dpc);
+  std::vector<double> N(nq * dpc), dx(nq * dpc), dy(nq * dpc);
+  for (int q = 0; q < nq; ++q)
+    lagrange_q(dpc, rule.pts[2 * q], rule.pts[2 * q + 1], &N[q * dpc], &dx[q * dpc], &dy[q * dpc]);
+  for (int64_t c = 0; c < m.ncells; ++c) {
+    const double *G = &m.geom[c * dpc * 3];
+    for (int q = 0; q < nq; ++q) {
+      double x[3] = {0, 0, 0}, a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+      for (int j = 0; j < dpc; ++j)
+        for (int k = 0; k < 3; ++k) {
+          x[k] += N[q * dpc + j] * G[3 * j + k];
+          a[k] += dx[q * dpc + j] * G[3 * j + k];
+          b[k] += dy[q * dpc + j] * G[3 * j + k];
+        }
+      const double cr[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+      const double J = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+      const int64_t g = c * nq + q;
+      for (int k = 0; k < 3; ++k) {
+        qd.x[3 * g + k] = x[k];
+        qd.n[3 * g + k] = cr[k] / J;
+      }
+      qd.jxw[g] = J * rule.w[q];
+      for (int j = 0; j < dpc; ++j) qd.sh[g * dpc + j] = N[q * dpc + j];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Octree (bem_fma.cc:3614-4801)
+// ---------------------------------------------------------------------------
+static inline int child_index(const double pm[3], double dl, const double *x) {
+  if (x[2] <= pm[2] + dl) {
+    if (x[1] <= pm[1] + dl) return (x[0] <= pm[0] + dl) ? 0 : 1;
+    return (x[0] <= pm[0] + dl) ? 3 : 2;
+  }
+  if (x[1] <= pm[1] + dl) return (x[0] <= pm[0] + dl) ? 4 : 5;
+  return (x[0] <= pm[0] + dl) ? 7 : 6;
+}
+static const double kOffs[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                   {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+// Face/edge/corner contact with the absolute tolerance (bem_fma.cc:4323-4357).
+static bool boxes_touch(const Tree &t, int b1, int b2) {
+  const double c1 = t.delta[b1] / 2., c2 = t.delta[b2] / 2.;
+  double P1[3], Q1[3], P2[3], Q2[3];
+  for (int k = 0; k < 3; ++k) {
+    P1[k] = t.pmin[3 * b1 + k];
+    Q1[k] = 2. * c1 + P1[k];
+    P2[k] = t.pmin[3 * b2 + k];
+    Q2[k] = 2. * c2 + P2[k];
+  }
+  auto ov = [&](int k) { return (P1[k] - kToll <= Q2[k]) && (Q1[k] + kToll >= P2[k]); };
+  auto fc = [&](int k) { return (std::fabs(P1[k] - Q2[k]) <= kToll) || (std::fabs(Q1[k] - P2[k]) <= kToll); };
+  return (fc(0) && ov(1) && ov(2)) || (fc(1) && ov(0) && ov(2)) || (fc(2) && ov(0) && ov(1));
+}
+
+static void sort_unique(std::vector<int32_t> &v) {
+  std::sort(v.begin(), v.end());
+  v.erase(std::unique(v.begin(), v.end()), v.end());
+}
+
+void build_tree(const HostMesh &m, const QuadData &qd, int L, int max_nodes, Tree &t) {
+  const int64_t N = m.n, NQ = m.ncells * qd.nq;
+  t = Tree();
+  t.L = L;
+  double maxc = 0;
+  for (int64_t i = 0; i < 3 * N; ++i) maxc = std::max(maxc, std::fabs(m.xyz[i]));
+  t.nperm.resize(N);
+  std::iota(t.nperm.begin(), t.nperm.end(), 0);
+  t.qperm.resize(NQ);
+  std::iota(t.qperm.begin(), t.qperm.end(), 0);
+  t.qblock.assign((size_t)(L + 1) * NQ, 0);
+  t.startL.assign(L + 1, 0);
+  t.endL.assign(L + 1, 0);
+  t.dofs_filled.assign(L + 1, {});
+
+  auto add_block = [&](int lev, int par, const double pm[3], double dl, int nb0, int nc, int qb0, int qc) {
+    const int id = t.nb++;
+    t.level.push_back(lev);
+    t.parent.push_back(par);
+    t.nchild.push_back(0);
+    for (int k = 0; k < 8; ++k) t.child.push_back(-1);
+    t.octant.push_back(0);
+    for (int k = 0; k < 3; ++k) t.pmin.push_back(pm[k]);
+    t.delta.push_back(dl);
+    t.node_beg.push_back(nb0);
+    t.node_cnt.push_back(nc);
+    t.quad_beg.push_back(qb0);
+    t.quad_cnt.push_back(qc);
+    return id;
+  };
+  double delta = 2.2 * maxc;
+  const double rootp[3] = {-1.1 * maxc, -1.1 * maxc, -1.1 * maxc};
+  add_block(0, 0, rootp, delta, 0, (int)N, 0, (int)NQ);
+
+  std::vector<int32_t> parents = {0}, tmp;
+  std::vector<uint8_t> cls;
+  for (int level = 1; level <= L; ++level) {
+    delta /= 2.;
+    for (int P : parents) {
+      const double pm[3] = {t.pmin[3 * P], t.pmin[3 * P + 1], t.pmin[3 * P + 2]};
+      int cn[8] = {0}, cq[8] = {0};
+      // stable partition of the parent's node and quad ranges into octants
+      auto part = [&](std::vector<int32_t> &perm, int beg, int cnt, const double *X, int *counts) {
+        cls.resize(cnt);
+        for (int i = 0; i < cnt; ++i) {
+          cls[i] = (uint8_t)child_index(pm, delta, X + 3 * (int64_t)perm[beg + i]);
+          counts[cls[i]]++;
+        }
+        int off[8];
+        off[0] = 0;
+        for (int c = 1; c < 8; ++c) off[c] = off[c - 1] + counts[c - 1];
+        tmp.resize(cnt);
+        for (int i = 0; i < cnt; ++i) tmp[off[cls[i]]++] = perm[beg + i];
+        std::copy(tmp.begin(), tmp.begin() + cnt, perm.begin() + beg);
+      };
+      part(t.nperm, t.node_beg[P], t.node_cnt[P], m.xyz.data(), cn);
+      part(t.qperm, t.quad_beg[P], t.quad_cnt[P], qd.x.data(), cq);
+      int nb0 = t.node_beg[P], qb0 = t.quad_beg[P];
+      for (int c = 0; c < 8; ++c) {
+        if (cn[c] + cq[c] > 0) {
+          double cp[3];
+          for (int k = 0; k < 3; ++k) cp[k] = pm[k] + (kOffs[c][k] != 0 ? delta : 0.);
+          const int id = add_block(level, P, cp, delta, nb0, cn[c], qb0, cq[c]);
+          t.child[8 * P + t.nchild[P]++] = id;
+          t.octant[id] = c;
+          for (int i = 0; i < cq[c]; ++i) t.qblock[(size_t)level * NQ + t.qperm[qb0 + i]] = id;
+        }
+        nb0 += cn[c];
+        qb0 += cq[c];
+      }
+    }
+    t.startL[level] = t.endL[level - 1] + 1;
+    t.endL[level] = t.nb - 1;
+    std::vector<int32_t> next;
+    for (int b = t.startL[level]; b <= t.endL[level]; ++b) {
+      double bn = 0.0;  // nodes ascending within the range: same summation order as the reference
+      for (int i = 0; i < t.node_cnt[b]; ++i) {
+        const int node = t.nperm[t.node_beg[b] + i];
+        bn += 1.0 / double(m.dbl_ptr[node + 1] - m.dbl_ptr[node]);
+      }
+      if (std::round(bn) <= max_nodes) {
+        t.leaves.push_back(b);
+        for (int i = 0; i < t.quad_cnt[b]; ++i)
+          for (int k = level + 1; k <= L; ++k) t.qblock[(size_t)k * NQ + t.qperm[t.quad_beg[b] + i]] = b;
+      } else {
+        next.push_back(b);
+      }
+      if (bn > 0) t.dofs_filled[level].push_back(b);
+    }
+    parents.swap(next);
+  }
+  for (int b : parents) t.leaves.push_back(b);
+
+  // sublevels: childless blocks below level 1 carry lists down to level L
+  t.nsub.assign(t.nb, 1);
+  for (int b = 0; b < t.nb; ++b)
+    if (t.level[b] >= 2 && t.nchild[b] == 0) t.nsub[b] = L - t.level[b] + 1;
+  t.sub_beg.assign(t.nb + 1, 0);
+  for (int b = 0; b < t.nb; ++b) t.sub_beg[b + 1] = t.sub_beg[b] + t.nsub[b];
+  const int ns = t.sub_beg[t.nb];
+  t.nn.assign(ns, {});
+  t.il.assign(ns, {});
+  t.nil.assign(ns, {});
+
+  // near neighbours (bem_fma.cc:4271-4579)
+  for (int b = t.startL[1]; b <= t.endL[1] && L >= 1; ++b)
+    for (int c = t.startL[1]; c <= t.endL[1]; ++c) t.nn[t.sub_beg[b]].push_back(c);
+  for (int level = 2; level <= L; ++level)
+    for (int b = t.startL[level]; b <= t.endL[level]; ++b) {
+      std::vector<int32_t> &n0 = t.nn[t.sub_beg[b]];
+      n0.push_back(b);
+      for (int pos : t.nn[t.sub_beg[t.parent[b]]]) {
+        if (t.nchild[pos] == 0 && boxes_touch(t, b, pos)) n0.push_back(pos);
+        for (int k = 0; k < t.nchild[pos]; ++k)
+          if (boxes_touch(t, b, t.child[8 * pos + k])) n0.push_back(t.child[8 * pos + k]);
+      }
+      sort_unique(n0);
+      for (int s = 1; s < t.nsub[b]; ++s) {
+        std::vector<int32_t> &ns_ = t.nn[t.sub_beg[b] + s];
+        for (int pos : t.nn[t.sub_beg[b] + s - 1]) {
+          if (t.nchild[pos] == 0) ns_.push_back(pos);
+          for (int k = 0; k < t.nchild[pos]; ++k)
+            if (boxes_touch(t, b, t.child[8 * pos + k])) ns_.push_back(t.child[8 * pos + k]);
+        }
+        sort_unique(ns_);
+      }
+    }
+
+  // interaction lists (bem_fma.cc:4610-4711)
+  for (int b = t.startL[1]; b <= t.endL[1] && L >= 1; ++b)
+    for (int c = t.startL[1]; c <= t.endL[1]; ++c) t.il[t.sub_beg[b]].push_back(c);
+  std::vector<int32_t> cells;
+  for (int level = 2; level <= L; ++level)
+    for (int b = t.startL[level]; b <= t.endL[level]; ++b) {
+      // cells touching any double of any node of b
+      cells.clear();
+      for (int i = 0; i < t.node_cnt[b]; ++i) {
+        const int node = t.nperm[t.node_beg[b] + i];
+        for (int k = m.dbl_ptr[node]; k < m.dbl_ptr[node + 1]; ++k) {
+          const int d = m.dbl_idx[k];
+          for (int c = m.d2c_ptr[d]; c < m.d2c_ptr[d + 1]; ++c) cells.push_back(m.d2c_idx[c]);
+        }
+      }
+      sort_unique(cells);
+      for (int s = 0; s < t.nsub[b]; ++s) {
+        std::vector<int32_t> &il = t.il[t.sub_beg[b] + s];
+        il = t.nn[t.sub_beg[b] + s];
+        const size_t lev = level + s;
+        for (int c : cells)
+          for (int q = 0; q < qd.nq; ++q) il.push_back(t.qblock[lev * NQ + (int64_t)c * qd.nq + q]);
+        sort_unique(il);
+      }
+      for (int s = 0; s < t.nsub[b]; ++s) {
+        const std::vector<int32_t> &il = t.il[t.sub_beg[b] + s];
+        const std::vector<int32_t> &pil = (s == 0) ? t.il[t.sub_beg[t.parent[b]]] : t.il[t.sub_beg[b] + s - 1];
+        std::vector<int32_t> &nl = t.nil[t.sub_beg[b] + s];
+        auto in_il = [&](int x) { return std::binary_search(il.begin(), il.end(), x); };
+        for (int pos : pil) {
+          if (t.nchild[pos] == 0) {
+            if (!in_il(pos)) nl.push_back(pos);
+          } else {
+            for (int k = 0; k < t.nchild[pos]; ++k)
+              if (!in_il(t.child[8 * pos + k])) nl.push_back(t.child[8 * pos + k]);
+          }
+        }
+        sort_unique(nl);
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Near-field plan (direct_integrals, bem_fma.cc:107-1600)
+// ---------------------------------------------------------------------------
+void build_near_plan(const HostMesh &m, const QuadData &qd, const Tree &t, NearPlan &np) {
+  np = NearPlan();
+  if (qd.nq > 64) throw std::runtime_error("quadrature order above 8 is not supported");
+  const int dpc = m.dpc;
+  std::vector<int32_t> qs;
+  auto add_entries = [&](std::vector<int32_t> &q, uint8_t near) {
+    sort_unique(q);
+    size_t i = 0;
+    while (i < q.size()) {
+      const int cell = q[i] / qd.nq;
+      uint64_t mask = 0;
+      while (i < q.size() && q[i] / qd.nq == cell) mask |= (uint64_t)1 << (q[i++] % qd.nq);
+      np.ent_cell.push_back(cell);
+      np.ent_mask.push_back(mask);
+      np.ent_near.push_back(near);
+    }
+  };
+  auto quads_of = [&](int b, std::vector<int32_t> &q) {
+    for (int i = 0; i < t.quad_cnt[b]; ++i) q.push_back(t.qperm[t.quad_beg[b] + i]);
+  };
+  np.row_beg.push_back(0);
+  np.col_beg.push_back(0);
+  np.ent_beg.push_back(0);
+  np.work_beg.push_back(0);
+  std::vector<int32_t> cols;
+  for (int leaf : t.leaves) {
+    if (t.node_cnt[leaf] == 0) continue;
+    const int e0 = (int)np.ent_cell.size();
+    // near quad points: blocks of the deepest intList
+    qs.clear();
+    for (int b : t.il[t.sub_beg[leaf] + t.nsub[leaf] - 1]) quads_of(b, qs);
+    add_entries(qs, 1);
+    // bigger far blocks in the nonIntLists of the leaf and of every ancestor
+    for (int a = leaf; a != 0; a = t.parent[a]) {
+      const int lev = t.level[a];
+      qs.clear();
+      for (int s = 0; s < t.nsub[a]; ++s)
+        for (int b : t.nil[t.sub_beg[a] + s])
+          if (b < t.startL[lev]) quads_of(b, qs);
+      if (!qs.empty()) add_entries(qs, 0);
+    }
+    const int e1 = (int)np.ent_cell.size();
+    cols.clear();
+    for (int e = e0; e < e1; ++e)
+      for (int j = 0; j < dpc; ++j) cols.push_back(m.cdofs[(int64_t)np.ent_cell[e] * dpc + j]);
+    sort_unique(cols);
+    for (int e = e0; e < e1; ++e)
+      for (int j = 0; j < dpc; ++j) {
+        const int d = m.cdofs[(int64_t)np.ent_cell[e] * dpc + j];
+        np.ent_colpos.push_back((int32_t)(std::lower_bound(cols.begin(), cols.end(), d) - cols.begin()));
+      }
+    std::vector<int32_t> rows(t.nperm.begin() + t.node_beg[leaf],
+                              t.nperm.begin() + t.node_beg[leaf] + t.node_cnt[leaf]);
+    std::sort(rows.begin(), rows.end());
+    np.val_beg.push_back(np.nnz);
+    np.nnz += (int64_t)rows.size() * (int64_t)cols.size();
+    for (int r : rows) {
+      np.rows.push_back(r);
+      np.row_leaf.push_back(np.nleaf);
+    }
+    np.cols.insert(np.cols.end(), cols.begin(), cols.end());
+    np.leaf_block.push_back(leaf);
+    np.row_beg.push_back((int32_t)np.rows.size());
+    np.col_beg.push_back((int32_t)np.cols.size());
+    np.ent_beg.push_back(e1);
+    np.work_beg.push_back(np.work_beg.back() + (int64_t)rows.size() * (e1 - e0));
+    np.nleaf++;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Translation operators as real (2Kt x 2Kt) maps on [Re M, Im M]
+// ---------------------------------------------------------------------------
+namespace {
+struct Leg {
+  int nmax;
+  std::vector<double> P;
+  Leg(int nm, double x) : nmax(nm), P((nm + 1) * (nm + 1), 0.0) {
+    const int ld = nm + 1;
+    const double s = std::sqrt(std::max(0.0, 1.0 - x * x));
+    double pmm = 1.0;
+    for (int mm = 0; mm <= nm; ++mm) {
+      if (mm > 0) pmm *= -std::sqrt((2.0 * mm - 1.0) / (2.0 * mm)) * s;
+      P[mm * ld + mm] = pmm;
+      if (mm + 1 <= nm) P[(mm + 1) * ld + mm] = x * std::sqrt(2.0 * mm + 1.0) * pmm;
+      for (int n = mm + 2; n <= nm; ++n)
+        P[n * ld + mm] = ((2.0 * n - 1.0) * x * P[(n - 1) * ld + mm] -
+                          std::sqrt(double((n - 1) * (n - 1) - mm * mm)) * P[(n - 2) * ld + mm]) /
+                         std::sqrt(double(n * n - mm * mm));
+    }
+  }
+  double operator()(int n, int m) const { return P[n * (nmax + 1) + m]; }
+};
+// A_n^m = (-1)^n / sqrt((n-m)!(n+m)!)  (multipole_expansion.h:123-147)
+double Anm(int n, int m) {
+  m = std::abs(m);
+  double f1 = 1, f2 = 1;
+  for (int i = n - m; i > 0; --i) f1 *= i;
+  for (int i = n + m; i > 0; --i) f2 *= i;
+  return ((n & 1) ? -1.0 : 1.0) / std::sqrt(f1 * f2);
+}
+double ire(int k) {  // Re(i^k), k even
+  const int r = ((k % 4) + 4) % 4;
+  return r == 0 ? 1.0 : (r == 2 ? -1.0 : 0.0);
+}
+inline int tri(int n, int m) { return n * (n + 1) / 2 + m; }
+inline int sgn(int x) { return (x > 0) - (x < 0); }
+// out(n,m) += w * a~ where a~ = (x, sig*y) of input (nn, mi): adds the four real
+// entries of the R-linear map into the transposed operator opT[col*kp + row].
+inline void put(double *opT, int kt, int kp, int out, int in, double wr, double wi, int sig) {
+  opT[(size_t)in * kp + out] += wr;
+  opT[(size_t)(kt + in) * kp + out] += -sig * wi;
+  opT[(size_t)in * kp + kt + out] += wi;
+  opT[(size_t)(kt + in) * kp + kt + out] += sig * wr;
+}
+void sph(const double d[3], double &rho, double &ca, double &beta) {
+  rho = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  ca = d[2] / rho;
+  beta = std::atan2(d[1], d[0]);
+}
+}  // namespace
+
+// MultipoleExpansion::Add(child) (multipole_expansion.cc:184-261)
+void op_m2m(int p, const double d[3], double *opT) {
+  const int kt = kt_of(p), kp = kp_of(p);
+  std::fill(opT, opT + (size_t)kp * kp, 0.0);
+  double rho, ca, beta;
+  sph(d, rho, ca, beta);
+  Leg P(p, ca);
+  for (int n = 0; n <= p; ++n)
+    for (int m = 0; m <= n; ++m)
+      for (int nn = 0; nn <= n; ++nn)
+        for (int mm = -nn; mm <= nn; ++mm) {
+          if (std::abs(m - mm) > n - nn) continue;
+          double rf = P(nn, std::abs(mm)) * std::pow(rho, double(nn)) * Anm(nn, mm) * Anm(n - nn, m - mm) / Anm(n, m);
+          rf *= ire(std::abs(m) - std::abs(mm) - std::abs(m - mm));
+          put(opT, kt, kp, tri(n, m), tri(n - nn, std::abs(m - mm)), rf * std::cos(mm * beta),
+              -rf * std::sin(mm * beta), sgn(m - mm));
+        }
+}
+
+// LocalExpansion::Add(parent) (local_expansion.cc:107-176, table local_expansion.h:182-215)
+void op_l2l(int p, const double d[3], double *opT) {
+  const int kt = kt_of(p), kp = kp_of(p);
+  std::fill(opT, opT + (size_t)kp * kp, 0.0);
+  double rho, ca, beta;
+  sph(d, rho, ca, beta);
+  Leg P(p, ca);
+  for (int n = 0; n <= p; ++n)
+    for (int m = 0; m <= n; ++m)
+      for (int nn = n; nn <= p; ++nn) {
+        const double rf0 = std::pow(rho, double(nn - n));
+        for (int mm = -nn; mm <= nn; ++mm) {
+          if (std::abs(mm - m) > nn - n) continue;
+          double c = Anm(nn - n, mm - m) / Anm(nn, mm) * Anm(n, m) * ire(std::abs(mm) - std::abs(mm - m) - std::abs(m)) *
+                     (((nn + n) & 1) ? -1.0 : 1.0);
+          const double rf = P(nn - n, std::abs(mm - m)) * rf0 * c;
+          put(opT, kt, kp, tri(n, m), tri(nn, std::abs(mm)), rf * std::cos((mm - m) * beta),
+              rf * std::sin((mm - m) * beta), sgn(mm));
+        }
+      }
+}
+
+// LocalExpansion::Add(multipole) (local_expansion.cc:179-248, table local_expansion.h:145-180)
+void op_m2l(int p, const double d[3], double *opT) {
+  const int kt = kt_of(p), kp = kp_of(p);
+  std::fill(opT, opT + (size_t)kp * kp, 0.0);
+  double rho, ca, beta;
+  sph(d, rho, ca, beta);
+  Leg P(2 * p, ca);
+  for (int n = 0; n <= p; ++n)
+    for (int m = 0; m <= n; ++m)
+      for (int nn = 0; nn <= p; ++nn) {
+        const double rf0 = std::pow(rho, double(-n - nn - 1));
+        for (int mm = -nn; mm <= nn; ++mm) {
+          double c = Anm(nn, mm) / Anm(n + nn, m - mm) * Anm(n, m) * ire(std::abs(m - mm) - std::abs(m) - std::abs(mm)) /
+                     ((nn & 1) ? -1.0 : 1.0);
+          const double rf = P(n + nn, std::abs(mm - m)) * rf0 * c;
+          put(opT, kt, kp, tri(n, m), tri(nn, std::abs(mm)), rf * std::cos((mm - m) * beta),
+              rf * std::sin((mm - m) * beta), mm < 0 ? -1 : 1);
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// Far-field plan (generate_multipole_expansions bem_fma.cc:1970-2380 and
+// multipole_matr_vect_products bem_fma.cc:2390-3072)
+// ---------------------------------------------------------------------------
+void build_far_plan(const Tree &t, int p, FarPlan &fp) {
+  fp = FarPlan();
+  fp.p = p;
+  fp.kt = kt_of(p);
+  fp.kp = kp_of(p);
+  const int L = t.L, kp = fp.kp;
+  fp.p2m_leaf = t.leaves;
+  fp.m2m_par.assign(L + 1, {});
+  for (int l = 1; l <= L; ++l)
+    for (int b = t.startL[l - 1]; b <= t.endL[l - 1] && t.endL[l] >= t.startL[l]; ++b)
+      if (t.nchild[b] > 0) fp.m2m_par[l].push_back(b);
+  fp.l2l_tgt.assign(L + 1, {});
+  for (int l = 2; l <= L; ++l) fp.l2l_tgt[l] = t.dofs_filled[l];
+  for (int b : t.leaves)
+    if (t.node_cnt[b] > 0) fp.l2p_leaf.push_back(b);
+
+  // M2M / L2L operators per level and octant
+  fp.m2m_ops.assign((size_t)(L + 1) * 8 * kp * kp, 0.0);
+  fp.l2l_ops.assign((size_t)(L + 1) * 8 * kp * kp, 0.0);
+  for (int l = 1; l <= L; ++l) {
+    const double dl = t.delta[0] / std::pow(2.0, l);
+    for (int o = 0; o < 8; ++o) {
+      double d[3];
+      for (int k = 0; k < 3; ++k) d[k] = (kOffs[o][k] != 0 ? 0.5 : -0.5) * dl;
+      op_m2m(p, d, &fp.m2m_ops[((size_t)l * 8 + o) * kp * kp]);
+      const double dn[3] = {-d[0], -d[1], -d[2]};
+      op_l2l(p, dn, &fp.l2l_ops[((size_t)l * 8 + o) * kp * kp]);
+    }
+  }
+
+  // M2L pairs by (level, offset class); M2P pairs
+  fp.m2l_per_level.assign(L + 1, 0);
+  fp.m2p_work_beg.push_back(0);
+  for (int l = 1; l <= L; ++l) {
+    std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> cls;
+    const double dl = t.delta[0] / std::pow(2.0, l);
+    for (int b : t.dofs_filled[l]) {
+      double cb[3];
+      t.center(b, cb);
+      for (int s = 0; s < t.nsub[b]; ++s)
+        for (int src : t.nil[t.sub_beg[b] + s]) {
+          if (src >= t.startL[l] && src <= t.endL[l]) {
+            double cs[3];
+            t.center(src, cs);
+            std::array<int, 3> key;
+            for (int k = 0; k < 3; ++k) key[k] = (int)std::lround((cs[k] - cb[k]) / dl);
+            cls[key].push_back({b, src});
+          } else if (src > t.endL[l]) {
+            fp.m2p_tgt.push_back(b);
+            fp.m2p_src.push_back(src);
+            fp.m2p_work_beg.push_back(fp.m2p_work_beg.back() + t.node_cnt[b]);
+          }
+        }
+    }
+    for (auto &kv : cls) {
+      const int op = fp.n_m2l_ops++;
+      fp.m2l_ops.resize((size_t)fp.n_m2l_ops * kp * kp);
+      const double d[3] = {kv.first[0] * dl, kv.first[1] * dl, kv.first[2] * dl};
+      op_m2l(p, d, &fp.m2l_ops[(size_t)op * kp * kp]);
+      const auto &pairs = kv.second;
+      for (size_t i = 0; i < pairs.size(); i += 64) {
+        fp.chunk_op.push_back(op);
+        fp.chunk_beg.push_back((int)fp.m2l_tgt.size() + (int)i);
+        fp.chunk_cnt.push_back((int)std::min<size_t>(64, pairs.size() - i));
+      }
+      for (auto &pr : pairs) {
+        fp.m2l_tgt.push_back(pr.first);
+        fp.m2l_src.push_back(pr.second);
+      }
+      fp.m2l_per_level[l] += (int64_t)pairs.size();
+    }
+  }
+}
+
+}  // namespace pbfmm

@@ -1,0 +1,229 @@
+"""Host-side mirror of pi-BEM's BEM-FMM interface over libpbfmm (C ABI).
+
+``BEMFMA`` carries the method set of ``BEMFMA<3>`` (include/bem_fma.h:74-202):
+``init_fma``, ``generate_octree_blocking``, ``compute_m2l_flags``,
+``direct_integrals``, ``multipole_integrals``,
+``generate_multipole_expansions``, ``multipole_matr_vect_products``.
+``BEMProblem`` carries the FMA branch of ``BEMProblem<3>``
+(include/bem_problem.h:131-242): ``compute_alpha``, ``vmult``,
+``compute_rhs``, ``solve_system``.  Errors raise ``PbfmmError`` (the reference
+throws ``ExcMessage`` through ``AssertThrow``).  All computation runs in the
+HIP library; nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import PBFMM_FP64, PBFMM_MIXED, Params, PbfmmError, load
+
+__all__ = ["BEMFMA", "BEMProblem", "PBFMM_FP64", "PBFMM_MIXED", "PbfmmError", "load"]
+
+_d = C.POINTER(C.c_double)
+_i64 = C.POINTER(C.c_int64)
+_i32 = C.POINTER(C.c_int32)
+_u8 = C.POINTER(C.c_uint8)
+
+
+def _p(a, t=_d):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+class BEMFMA:
+    """BEMFMA<3> over the C ABI.  Parameters as in BEMFMA::declare_parameters
+    (bem_fma.cc:64-85): num_octree_levels=10, max_nodes_per_leaf=20,
+    trunc_order=6; quadrature orders as handed to init_fma (4, 5)."""
+
+    def __init__(self, trunc_order=6, num_octree_levels=10, max_nodes_per_leaf=20, quad_order=4,
+                 sing_quad_order=5, precision=PBFMM_FP64, device=0, stream=None):
+        self._L = load()
+        prm = Params()
+        self._L.pbfmm_default_params(C.byref(prm))
+        prm.num_octree_levels = num_octree_levels
+        prm.max_nodes_per_leaf = max_nodes_per_leaf
+        prm.trunc_order = trunc_order
+        prm.quad_order = quad_order
+        prm.sing_quad_order = sing_quad_order
+        prm.precision = precision
+        prm.device = device
+        prm.stream = stream
+        self.params = prm
+        h = C.c_void_p()
+        rc = self._L.pbfmm_create(C.byref(prm), C.byref(h))
+        if rc != 0:
+            raise PbfmmError(rc, "pbfmm_create failed (invalid parameters or no HIP device)")
+        self._h = h
+        self.n = 0
+        self.p = trunc_order
+        self.levels = num_octree_levels
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.pbfmm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != 0:
+            msg = self._L.pbfmm_last_error(self._h)
+            raise PbfmmError(rc, msg.decode() if msg else "")
+
+    # ---- BEMFMA<3> --------------------------------------------------------
+    def init_fma(self, mesh, dirichlet_mask):
+        """bem_fma.h:103-108: mesh = pibem_amd.mesh.Mesh (support points,
+        FE_Q cell DoFs, double-node CSR), Dirichlet mask per DoF."""
+        self._keep = [_f64(mesh.support_xyz), _i(mesh.cell_dofs),
+                      None if mesh.dbl_ptr is None else _i(mesh.dbl_ptr),
+                      None if mesh.dbl_idx is None else _i(mesh.dbl_idx),
+                      np.ascontiguousarray(dirichlet_mask, dtype=np.uint8)]
+        xyz, cd, dp, di, dm = self._keep
+        self._chk(self._L.pbfmm_init_fma(self._h, mesh.n_dofs, _p(xyz), mesh.n_cells, mesh.dpc, _p(cd, _i64), None,
+                                         _p(dp, _i64), _p(di, _i64), _p(dm, _u8)))
+        self.n = mesh.n_dofs
+
+    def generate_octree_blocking(self):
+        self._chk(self._L.pbfmm_generate_octree_blocking(self._h))
+
+    def compute_m2l_flags(self):
+        out = np.zeros(self.levels + 1, dtype=np.int64)
+        self._chk(self._L.pbfmm_compute_m2l_flags(self._h, _p(out, _i64)))
+        return out
+
+    def direct_integrals(self):
+        self._chk(self._L.pbfmm_direct_integrals(self._h))
+
+    def multipole_integrals(self):
+        self._chk(self._L.pbfmm_multipole_integrals(self._h))
+
+    def generate_multipole_expansions(self, phi, dphi_dn):
+        self._chk(self._L.pbfmm_generate_multipole_expansions(self._h, _p(_f64(phi)), _p(_f64(dphi_dn))))
+
+    def multipole_matr_vect_products(self, phi, dphi_dn):
+        N, D = np.zeros(self.n), np.zeros(self.n)
+        self._chk(self._L.pbfmm_multipole_matr_vect_products(self._h, _p(_f64(phi)), _p(_f64(dphi_dn)), _p(N),
+                                                             _p(D)))
+        return N, D
+
+    # ---- device-resident entry points -----------------------------------
+    def matvec_dev(self, phi_ptr, dphi_ptr, outN_ptr, outD_ptr, precision=PBFMM_FP64):
+        self._chk(self._L.pbfmm_matvec_dev(self._h, phi_ptr, dphi_ptr, outN_ptr, outD_ptr, precision))
+
+    def gmres_dev(self, rhs_ptr, x_ptr, rel_tol=1e-8, restart=100, max_it=1000, precision=PBFMM_FP64,
+                  check=True):
+        it, rr = C.c_int32(0), C.c_double(0)
+        rc = self._L.pbfmm_gmres_dev(self._h, rhs_ptr, x_ptr, rel_tol, restart, max_it, precision, C.byref(it),
+                                     C.byref(rr))
+        if check:
+            self._chk(rc)
+        return it.value, rr.value
+
+    # ---- introspection ------------------------------------------------------
+    def tree(self):
+        cnt = np.zeros(8, np.int64)
+        self._chk(self._L.pbfmm_tree_counts(self._h, _p(cnt, _i64)))
+        nb, L, nleaf, ns, nil, nnil, nn, _ = (int(x) for x in cnt)
+        t = dict(level=np.zeros(nb, np.int32), parent=np.zeros(nb, np.int64), pmin=np.zeros(3 * nb),
+                 delta=np.zeros(nb), startL=np.zeros(L + 1, np.int64), endL=np.zeros(L + 1, np.int64),
+                 childless=np.zeros(nleaf, np.int64), sub_ptr=np.zeros(nb + 1, np.int64),
+                 il_ptr=np.zeros(ns + 1, np.int64), il_idx=np.zeros(nil, np.int64),
+                 nil_ptr=np.zeros(ns + 1, np.int64), nil_idx=np.zeros(nnil, np.int64),
+                 node_ptr=np.zeros(nb + 1, np.int64), node_idx=np.zeros(nn, np.int64))
+        args = [_p(t["level"], _i32)]
+        for k in ("parent", "pmin", "delta", "startL", "endL", "childless", "sub_ptr", "il_ptr", "il_idx", "nil_ptr",
+                  "nil_idx", "node_ptr", "node_idx"):
+            args.append(_p(t[k], _i64) if t[k].dtype == np.int64 else _p(t[k]))
+        self._chk(self._L.pbfmm_tree_export(self._h, *args))
+        t["pmin"] = t["pmin"].reshape(nb, 3)
+        t["num_blocks"] = nb
+        return t
+
+    def nearfield(self, precision=PBFMM_FP64):
+        cnt = np.zeros(8, np.int64)
+        self._chk(self._L.pbfmm_tree_counts(self._h, _p(cnt, _i64)))
+        nnz = int(cnt[7])
+        rp, col = np.zeros(self.n + 1, np.int64), np.zeros(nnz, np.int64)
+        vN, vD = np.zeros(nnz), np.zeros(nnz)
+        self._chk(self._L.pbfmm_nearfield_export(self._h, precision, _p(rp, _i64), _p(col, _i64), _p(vN), _p(vD)))
+        return rp, col, vN, vD
+
+    def expansions(self, which="me"):
+        nb = self.tree()["num_blocks"]
+        kt = (self.p + 1) * (self.p + 2) // 2
+        a, b = np.zeros(2 * kt * nb), np.zeros(2 * kt * nb)
+        self._chk(self._L.pbfmm_get_expansions(self._h, 0 if which == "me" else 1, _p(a), _p(b)))
+        return (a[0::2] + 1j * a[1::2]).reshape(nb, kt), (b[0::2] + 1j * b[1::2]).reshape(nb, kt)
+
+    def last_timings(self):
+        ms = np.zeros(8)
+        self._chk(self._L.pbfmm_last_timings(self._h, _p(ms)))
+        return ms
+
+
+class BEMProblem:
+    """FMA branch of BEMProblem<3> (bem_problem.cc:1138-1347) over a BEMFMA."""
+
+    def __init__(self, fma: BEMFMA):
+        self.fma = fma
+
+    @property
+    def _h(self):
+        return self.fma._h
+
+    def compute_alpha(self):
+        a = np.zeros(self.fma.n)
+        self.fma._chk(self.fma._L.pbfmm_compute_alpha(self._h, _p(a)))
+        return a
+
+    def set_constraints(self, cons):
+        """cons = (ptr, idx, val, inhom, is_constrained) as produced by
+        pibem_amd.mesh.constraints (compute_constraints, bem_problem.cc:1376-1653)."""
+        if cons is None:
+            self.fma._chk(self.fma._L.pbfmm_set_constraints(self._h, None, None, None, None, None))
+            return
+        ptr, idx, val, inh, isc = cons
+        self._ckeep = [_i(ptr), _i(idx), _f64(val), _f64(inh), np.ascontiguousarray(isc, np.uint8)]
+        a, b, c, d, e = self._ckeep
+        self.fma._chk(self.fma._L.pbfmm_set_constraints(self._h, _p(a, _i64), _p(b, _i64), _p(c), _p(d),
+                                                        _p(e, _u8)))
+
+    def vmult(self, src):
+        dst = np.zeros(self.fma.n)
+        self.fma._chk(self.fma._L.pbfmm_vmult(self._h, _p(_f64(src)), _p(dst)))
+        return dst
+
+    def compute_rhs(self, tmp_rhs):
+        dst = np.zeros(self.fma.n)
+        self.fma._chk(self.fma._L.pbfmm_compute_rhs(self._h, _p(_f64(tmp_rhs)), _p(dst)))
+        return dst
+
+    def solve_system(self, tmp_rhs, tol=1e-8, restart=100, max_it=1000, phi=None, dphi_dn=None):
+        phi = np.zeros(self.fma.n) if phi is None else _f64(phi).copy()
+        dphi = np.zeros(self.fma.n) if dphi_dn is None else _f64(dphi_dn).copy()
+        it, rr = C.c_int32(0), C.c_double(0)
+        self.fma._chk(self.fma._L.pbfmm_solve_system(self._h, _p(_f64(tmp_rhs)), _p(phi), _p(dphi), tol, restart,
+                                                     max_it, C.byref(it), C.byref(rr)))
+        return phi, dphi, it.value, rr.value
+
+
+def setup(problem, precision=PBFMM_FP64, device=0):
+    """The set-up sequence of BEMProblem::solve (bem_problem.cc:1363-1371,
+    upstream wiring): init_fma, octree, direct and multipole integrals."""
+    f = BEMFMA(trunc_order=problem.p, num_octree_levels=problem.levels,
+               max_nodes_per_leaf=problem.max_nodes_per_leaf, quad_order=problem.quad_order,
+               sing_quad_order=problem.sing_quad_order, precision=precision, device=device)
+    f.init_fma(problem.mesh, problem.dirichlet)
+    f.generate_octree_blocking()
+    f.direct_integrals()
+    f.multipole_integrals()
+    return f

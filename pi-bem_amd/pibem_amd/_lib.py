@@ -1,0 +1,80 @@
+"""ctypes binding of libpbfmm.so (include/pbfmm.h).
+
+The library is loaded from pi-bem_amd/lib/ (built in-tree by
+pi-bem_amd/build.py).  There is no fallback: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libpbfmm.so")
+
+PBFMM_FP64 = 0
+PBFMM_MIXED = 1
+
+_d = C.POINTER(C.c_double)
+_i64 = C.POINTER(C.c_int64)
+_i32 = C.POINTER(C.c_int32)
+_u8 = C.POINTER(C.c_uint8)
+_h = C.c_void_p
+
+
+class Params(C.Structure):
+    _fields_ = [("num_octree_levels", C.c_int32), ("max_nodes_per_leaf", C.c_int32), ("trunc_order", C.c_int32),
+                ("quad_order", C.c_int32), ("sing_quad_order", C.c_int32), ("precision", C.c_int32),
+                ("device", C.c_int32), ("stream", C.c_void_p)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/pbfmm.h
+SIGNATURES = {
+    "pbfmm_default_params": (None, [C.POINTER(Params)]),
+    "pbfmm_create": (C.c_int, [C.POINTER(Params), C.POINTER(_h)]),
+    "pbfmm_destroy": (C.c_int, [_h]),
+    "pbfmm_last_error": (C.c_char_p, [_h]),
+    "pbfmm_init_fma": (C.c_int, [_h, C.c_int64, _d, C.c_int64, C.c_int32, _i64, _d, _i64, _i64, _u8]),
+    "pbfmm_generate_octree_blocking": (C.c_int, [_h]),
+    "pbfmm_compute_m2l_flags": (C.c_int, [_h, _i64]),
+    "pbfmm_direct_integrals": (C.c_int, [_h]),
+    "pbfmm_multipole_integrals": (C.c_int, [_h]),
+    "pbfmm_generate_multipole_expansions": (C.c_int, [_h, _d, _d]),
+    "pbfmm_multipole_matr_vect_products": (C.c_int, [_h, _d, _d, _d, _d]),
+    "pbfmm_compute_alpha": (C.c_int, [_h, _d]),
+    "pbfmm_set_constraints": (C.c_int, [_h, _i64, _i64, _d, _d, _u8]),
+    "pbfmm_vmult": (C.c_int, [_h, _d, _d]),
+    "pbfmm_compute_rhs": (C.c_int, [_h, _d, _d]),
+    "pbfmm_solve_system": (C.c_int, [_h, _d, _d, _d, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                     _d]),
+    "pbfmm_matvec_dev": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
+    "pbfmm_gmres_dev": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                                  C.POINTER(C.c_int32), _d]),
+    "pbfmm_tree_counts": (C.c_int, [_h, _i64]),
+    "pbfmm_tree_export": (C.c_int, [_h, _i32, _i64, _d, _d, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
+                                    _i64]),
+    "pbfmm_nearfield_export": (C.c_int, [_h, C.c_int32, _i64, _i64, _d, _d]),
+    "pbfmm_get_expansions": (C.c_int, [_h, C.c_int32, _d, _d]),
+    "pbfmm_last_timings": (C.c_int, [_h, _d]),
+}
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libpbfmm.so not built: {LIB_PATH} (run python pi-bem_amd/build.py)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class PbfmmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pbfmm error {code}: {msg}")
+        self.code = code

@@ -1,0 +1,169 @@
+"""GPU parity of the HIP path against the oracle, called through the C ABI.
+
+Bars:
+* near field (N, D on the shared pattern): identical pattern; FP64 values
+  within 1e-12 relative; the mixed-mode FP32 regular quadrature within 1e-5
+* expansions after generate / products: FP64 within 1e-11 relative per block
+* FMM matvec (multipole_matr_vect_products), alpha, vmult, compute_rhs:
+  FP64 within 1e-10 relative L2 of the oracle (same mesh, order and octree);
+  FP32 operator within 2e-5
+* GMRES solve: true relative residual <= 1e-8 in both precision modes, and
+  the solution within 1e-7 of the oracle's solution
+"""
+import numpy as np
+import pytest
+
+import pibem_amd as P
+from pibem_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+class Case:
+    def __init__(self, pr, oracle_mod, mixed=True):
+        print(f"\n[case] {pr.mesh.name} N={pr.mesh.n_dofs} p={pr.p}", flush=True)
+        self.pr = pr
+        self.o = oracle_mod.OracleFMM(pr).setup()
+        self.f64 = P.setup(pr, P.PBFMM_FP64)
+        self.mix = P.setup(pr, P.PBFMM_MIXED) if mixed else None
+        rng = np.random.default_rng(1234)
+        self.phi = rng.uniform(-1, 1, pr.mesh.n_dofs)
+        self.dphi = rng.uniform(-1, 1, pr.mesh.n_dofs)
+
+
+@pytest.fixture(scope="module")
+def c1(oracle_mod):
+    return Case(M.config_sphere(16, 4), oracle_mod)
+
+
+@pytest.fixture(scope="module")
+def dcube(oracle_mod):
+    return Case(M.config_dcube(8, 5), oracle_mod)
+
+
+def test_nearfield_fp64(c1):
+    rp, col, vN, vD, _ = c1.o.csr()
+    grp, gcol, gN, gD = c1.f64.nearfield(P.PBFMM_FP64)
+    assert np.array_equal(rp, grp) and np.array_equal(col, gcol)
+    assert np.max(np.abs(gN - vN)) <= 1e-12 * np.max(np.abs(vN))
+    assert np.max(np.abs(gD - vD)) <= 1e-12 * np.max(np.abs(vD))
+
+
+def test_nearfield_fp32(c1):
+    rp, col, vN, vD, _ = c1.o.csr()
+    grp, gcol, gN, gD = c1.mix.nearfield(P.PBFMM_MIXED)
+    assert np.array_equal(rp, grp) and np.array_equal(col, gcol)
+    assert rel(gN, vN) <= 1e-5 and rel(gD, vD) <= 1e-5
+
+
+def test_expansions_fp64(c1):
+    c1.o.generate_multipole_expansions(c1.phi, c1.dphi)
+    c1.f64.generate_multipole_expansions(c1.phi, c1.dphi)
+    for a, b in zip(c1.o.expansions("me"), c1.f64.expansions("me")):
+        scale = np.linalg.norm(a, axis=1).max()
+        assert np.max(np.linalg.norm(a - b, axis=1)) <= 1e-11 * scale
+    N0, D0 = c1.o.multipole_matr_vect_products(c1.phi, c1.dphi)
+    N1, D1 = c1.f64.multipole_matr_vect_products(c1.phi, c1.dphi)
+    for a, b in zip(c1.o.expansions("le"), c1.f64.expansions("le")):
+        scale = np.linalg.norm(a, axis=1).max()
+        assert np.max(np.linalg.norm(a - b, axis=1)) <= 1e-11 * scale
+    assert rel(N1, N0) <= 1e-10 and rel(D1, D0) <= 1e-10
+
+
+@pytest.mark.parametrize("which", ["c1", "dcube"])
+def test_matvec_parity(which, request):
+    c = request.getfixturevalue(which)
+    N0, D0 = c.o.matvec(c.phi, c.dphi)
+    c.f64.generate_multipole_expansions(c.phi, c.dphi)
+    N1, D1 = c.f64.multipole_matr_vect_products(c.phi, c.dphi)
+    print(f"  fp64 matvec rel err N {rel(N1, N0):.3e} D {rel(D1, D0):.3e}")
+    assert rel(N1, N0) <= 1e-10 and rel(D1, D0) <= 1e-10
+
+
+def test_matvec_mixed_dev(c1):
+    from pibem_amd import hip
+    phi, dphi = hip.DeviceArray.from_numpy(c1.phi), hip.DeviceArray.from_numpy(c1.dphi)
+    oN, oD = hip.DeviceArray(c1.pr.mesh.n_dofs), hip.DeviceArray(c1.pr.mesh.n_dofs)
+    N0, D0 = c1.o.matvec(c1.phi, c1.dphi)
+    c1.mix.matvec_dev(phi.value, dphi.value, oN.value, oD.value, P.PBFMM_MIXED)
+    hip.synchronize()
+    eN, eD = rel(oN.numpy(), N0), rel(oD.numpy(), D0)
+    print(f"  fp32 matvec rel err N {eN:.3e} D {eD:.3e}")
+    assert eN <= 2e-5 and eD <= 2e-5
+    c1.mix.matvec_dev(phi.value, dphi.value, oN.value, oD.value, P.PBFMM_FP64)
+    hip.synchronize()
+    assert rel(oN.numpy(), N0) <= 1e-10
+
+
+def test_alpha_vmult_rhs(c1):
+    bp = P.BEMProblem(c1.f64)
+    a1 = bp.compute_alpha()
+    a0 = c1.o.compute_alpha()
+    assert rel(a1, a0) <= 1e-10
+    v0 = c1.o.vmult(c1.phi)
+    v1 = bp.vmult(c1.phi)
+    assert rel(v1, v0) <= 1e-10
+    r0 = c1.o.compute_rhs(c1.pr.tmp_rhs)
+    r1 = bp.compute_rhs(c1.pr.tmp_rhs)
+    assert rel(r1, r0) <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [P.PBFMM_FP64, P.PBFMM_MIXED])
+def test_solve_system(c1, precision):
+    f = c1.f64 if precision == P.PBFMM_FP64 else c1.mix
+    bp = P.BEMProblem(f)
+    phi, dphi, it, rr = bp.solve_system(c1.pr.tmp_rhs, tol=1e-8)
+    print(f"  solve precision={precision} iters={it} relres={rr:.3e}")
+    assert rr <= 1e-8
+    ph0, dph0, _, _ = c1.o.solve_system(c1.pr.tmp_rhs, tol=1e-10)
+    neu = c1.pr.dirichlet == 0
+    assert rel(phi[neu], ph0[neu]) <= 1e-7
+    assert rel(dphi[~neu], dph0[~neu]) <= 1e-6
+    x = c1.pr.mesh.support_xyz
+    assert rel(phi[neu], c1.pr.exact.phi(x)[neu]) < 1e-3
+
+
+def test_constraints_double_nodes(dcube):
+    cons = M.constraints(dcube.pr)
+    bp = P.BEMProblem(dcube.f64)
+    bp.set_constraints(cons)
+    phi, dphi, it, rr = bp.solve_system(dcube.pr.tmp_rhs, tol=1e-8)
+    assert rr <= 1e-8
+    ph0, dph0, _, rr0 = dcube.o.solve_system(dcube.pr.tmp_rhs, cons, tol=1e-10)
+    neu = dcube.pr.dirichlet == 0
+    assert rel(phi[neu], ph0[neu]) <= 1e-6
+    assert rel(dphi[~neu], dph0[~neu]) <= 1e-5
+    bp.set_constraints(None)
+
+
+def test_pure_neumann_vmult(oracle_mod):
+    pr = M.config_sphere(8, 4)
+    pr.dirichlet = np.zeros_like(pr.dirichlet)
+    o = oracle_mod.OracleFMM(pr).setup()
+    f = P.setup(pr, P.PBFMM_FP64)
+    x = np.random.default_rng(7).uniform(-1, 1, pr.mesh.n_dofs)
+    o.compute_alpha()
+    bp = P.BEMProblem(f)
+    bp.compute_alpha()
+    assert rel(bp.vmult(x), o.vmult(x)) <= 1e-10
+
+
+def test_q2_spheroid_matvec(oracle_mod):
+    c = Case(M.config_spheroid(8, 6), oracle_mod, mixed=False)
+    N0, D0 = c.o.matvec(c.phi, c.dphi)
+    c.f64.generate_multipole_expansions(c.phi, c.dphi)
+    N1, D1 = c.f64.multipole_matr_vect_products(c.phi, c.dphi)
+    assert rel(N1, N0) <= 1e-10 and rel(D1, D0) <= 1e-10
+
+
+def test_config2_matvec(oracle_mod):
+    c = Case(M.config_sphere(64, 6), oracle_mod, mixed=False)
+    N0, D0 = c.o.matvec(c.phi, c.dphi)
+    c.f64.generate_multipole_expansions(c.phi, c.dphi)
+    N1, D1 = c.f64.multipole_matr_vect_products(c.phi, c.dphi)
+    print(f"  config2 fp64 matvec rel err N {rel(N1, N0):.3e} D {rel(D1, D0):.3e}")
+    assert rel(N1, N0) <= 1e-10 and rel(D1, D0) <= 1e-10

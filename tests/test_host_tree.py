@@ -1,0 +1,56 @@
+"""Host-side octree of the product (pi-bem_amd/csrc/setup.cpp, flat arrays)
+against the oracle's std::set restatement of generate_octree_blocking
+(bem_fma.cc:3614-4801): blocks, levels, node lists, childless list, and every
+intList / nonIntList of every sublevel must be identical (integer work:
+bit-exact).  Runs without a GPU: init_fma and generate_octree_blocking are
+host-only steps of the C ABI."""
+import numpy as np
+import pytest
+
+from pibem_amd import mesh as M
+
+
+def _compare(pr, oracle_mod):
+    import pibem_amd as P
+    o = oracle_mod.OracleFMM(pr)
+    o.generate_octree_blocking()
+    to = o.tree()
+    f = P.BEMFMA(trunc_order=pr.p, num_octree_levels=pr.levels, max_nodes_per_leaf=pr.max_nodes_per_leaf)
+    f.init_fma(pr.mesh, pr.dirichlet)
+    f.generate_octree_blocking()
+    tp = f.tree()
+    assert to["num_blocks"] == tp["num_blocks"]
+    for k in to:
+        if isinstance(to[k], np.ndarray):
+            assert np.array_equal(to[k], tp[k]), k
+    return to
+
+
+@pytest.mark.parametrize("case", ["sphere16", "dcube8", "spheroid12q2", "sphere16_leaf5"])
+def test_tree_matches_oracle(case, oracle_mod):
+    if case == "sphere16":
+        pr = M.config_sphere(16, 4)
+    elif case == "dcube8":
+        pr = M.config_dcube(8, 4)
+    elif case == "spheroid12q2":
+        pr = M.config_spheroid(12, 6)
+    else:
+        pr = M.config_sphere(16, 4)
+        pr.max_nodes_per_leaf = 5
+        pr.levels = 4  # forces full leaves at the deepest level
+    t = _compare(pr, oracle_mod)
+    # every node sits in exactly one childless block
+    nodes = np.concatenate([t["node_idx"][t["node_ptr"][b]:t["node_ptr"][b + 1]] for b in t["childless"]])
+    assert np.array_equal(np.sort(nodes), np.arange(pr.mesh.n_dofs))
+
+
+def test_lists_are_disjoint(oracle_mod):
+    pr = M.config_dcube(8, 4)
+    o = oracle_mod.OracleFMM(pr)
+    o.generate_octree_blocking()
+    t = o.tree()
+    for b in range(t["num_blocks"]):
+        for s in range(t["sub_ptr"][b], t["sub_ptr"][b + 1]):
+            il = set(t["il_idx"][t["il_ptr"][s]:t["il_ptr"][s + 1]].tolist())
+            nil = set(t["nil_idx"][t["nil_ptr"][s]:t["nil_ptr"][s + 1]].tolist())
+            assert not (il & nil)
