@@ -161,9 +161,20 @@ int pbfmm_nearfield_export(pbfmm_handle *h, int32_t precision, int64_t *rowptr, 
  * 1 = local; Kt=(p+1)(p+2)/2 complex coefficients per block and kernel,
  * interleaved (re, im), index n(n+1)/2+m as in multipole_expansion.h:101-117. */
 int pbfmm_get_expansions(pbfmm_handle *h, int32_t which, double *ker1, double *ker2);
-/* Per-phase device times of the last matvec in ms:
+/* Per-phase device times of the last matvec in ms, measured with HIP events
+ * on the handle's stream while phase timing is enabled (it adds one host
+ * synchronisation per matvec):
  * [0] spmv [1] p2m [2] m2m [3] l2l [4] m2l [5] m2p [6] l2p [7] total. */
+int pbfmm_set_phase_timing(pbfmm_handle *h, int32_t enable);
 int pbfmm_last_timings(pbfmm_handle *h, double *ms);
+
+/* Work counts of the current plan, for roofline accounting:
+ * [0] n_dofs [1] near-field values (rows x cols of the leaf blocks)
+ * [2] near-field column entries [3] near-field rows [4] leaves
+ * [5] quad points [6] M2L pairs [7] M2P (pair, node) evaluations
+ * [8] M2M child translations [9] L2L translations [10] num_blocks
+ * [11] trunc_order [12] padded expansion length kp */
+int pbfmm_work_counts(pbfmm_handle *h, int64_t *counts);
 
 #ifdef __cplusplus
 }

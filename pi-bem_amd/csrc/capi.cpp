@@ -536,6 +536,39 @@ int pbfmm_get_expansions(pbfmm_handle *h, int32_t which, double *k1, double *k2)
   });
 }
 
+int pbfmm_set_phase_timing(pbfmm_handle *h, int32_t enable) {
+  return guarded(h, [&]() -> int {
+    h->eng.time_phases = enable != 0;
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_work_counts(pbfmm_handle *h, int64_t *c) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "work_counts")) return rc;
+    if (!c) return fail(h, PBFMM_ERR_ARG, "work_counts: null output");
+    const Engine &e = h->eng;
+    c[0] = e.n;
+    c[1] = e.near.nnz;
+    c[2] = (int64_t)e.near.cols.size();
+    c[3] = (int64_t)e.near.rows.size();
+    c[4] = (int64_t)e.far.p2m_leaf.size();
+    c[5] = e.mesh.ncells * e.nq;
+    c[6] = (int64_t)e.far.m2l_tgt.size();
+    c[7] = e.far.m2p_work_beg.empty() ? 0 : e.far.m2p_work_beg.back();
+    int64_t m2m = 0, l2l = 0;
+    for (int l = 1; l <= e.L; ++l)
+      for (int b : e.far.m2m_par[l]) m2m += e.tree.nchild[b];
+    for (int l = 2; l <= e.L; ++l) l2l += (int64_t)e.far.l2l_tgt[l].size();
+    c[8] = m2m;
+    c[9] = l2l;
+    c[10] = e.tree.nb;
+    c[11] = e.p;
+    c[12] = e.kp;
+    return PBFMM_OK;
+  }, false);
+}
+
 int pbfmm_last_timings(pbfmm_handle *h, double *ms) {
   return guarded(h, [&]() -> int {
     for (int i = 0; i < 8; ++i) ms[i] = h->eng.last_ms[i];

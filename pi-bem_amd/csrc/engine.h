@@ -76,6 +76,9 @@ struct Engine {
   // timings of the last matvec (ms)
   double last_ms[8] = {0};
   bool time_phases = false;
+  hipEvent_t ev[10] = {};
+  void mark(int i);
+  void collect_phase_times();
 
   void init_device(int dev, void *stream);
   ~Engine();

@@ -723,7 +723,35 @@ void Engine::init_device(int dev, void *st) {
 }
 
 Engine::~Engine() {
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+// Phase events on the handle's stream (no synchronisation between kernels):
+// 0 P2M 1 M2M 2 | 3 SpMV 4 fill 5 M2L 6 M2P 7 L2L 8 L2P 9.
+void Engine::mark(int i) {
+  if (!time_phases) return;
+  if (!ev[i]) HIPCHK(hipEventCreate(&ev[i]));
+  HIPCHK(hipEventRecord(ev[i], stream));
+}
+
+void Engine::collect_phase_times() {
+  if (!time_phases) return;
+  HIPCHK(hipEventSynchronize(ev[9]));
+  auto el = [&](int a, int b) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, ev[a], ev[b]));
+    return (double)ms;
+  };
+  last_ms[0] = el(3, 4);  // spmv
+  last_ms[1] = el(0, 1);  // p2m
+  last_ms[2] = el(1, 2);  // m2m
+  last_ms[3] = el(7, 8);  // l2l
+  last_ms[4] = el(5, 6);  // m2l
+  last_ms[5] = el(6, 7);  // m2p
+  last_ms[6] = el(8, 9);  // l2p
+  last_ms[7] = el(0, 9);  // whole matvec
 }
 
 void Engine::sync() { HIPCHK(hipStreamSynchronize(stream)); }
@@ -892,6 +920,7 @@ template <typename T>
 void Engine::generate(const double *phi, const double *dphi) {
   FarDev<T> &F = far_of<T>(*this);
   const int nleaf = (int)far.p2m_leaf.size();
+  mark(0);
   if (nleaf > 0)
     hipLaunchKernelGGL(k_p2m<T>, dim3(nleaf), dim3(64), 0, stream, d_p2m_leaf.as<int>(), d_quad_beg.as<int>(),
                        d_quad_cnt.as<int>(), d_qperm.as<int>(), F.qx.template as<T>(), F.qn.template as<T>(),
@@ -904,6 +933,7 @@ void Engine::generate(const double *phi, const double *dphi) {
                        d_m2m_par.as<int>() + m2m_off[l], d_nchild.as<int>(), d_child.as<int>(), d_octant.as<int>(),
                        F.m2m.template as<T>() + (size_t)l * 8 * kp * kp, kp, F.me.template as<T>());
   }
+  mark(1);
   HIPCHK(hipGetLastError());
 }
 
@@ -911,13 +941,17 @@ template <typename T>
 void Engine::products(const double *phi, const double *dphi, double *outN, double *outD) {
   FarDev<T> &F = far_of<T>(*this);
   const int nrows = (int)near.rows.size();
+  mark(2);
+  mark(3);
   if (nrows > 0)
     hipLaunchKernelGGL(k_spmv<T>, dim3(nblk((int64_t)nrows * 64, 256)), dim3(256), 0, stream, nrows,
                        d_row_leaf.as<int>(), d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(),
                        d_cols.as<int>(), d_val_beg.as<int64_t>(), F.vals.template as<T>(), phi, dphi, outN, outD);
+  mark(4);
   const int64_t nle = (int64_t)tree.nb * 2 * kp;
   hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(nle, 256), 4096)), dim3(256), 0, stream,
                      F.le.template as<T>(), nle, T(0));
+  mark(5);
   const int nchunk = (int)far.chunk_op.size();
   if (nchunk > 0) {
     const size_t op_bytes = (size_t)kp * kp * sizeof(T), v_bytes = (size_t)4 * 2 * kp * sizeof(T);
@@ -930,6 +964,7 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
                          d_chunk_beg.as<int>(), d_chunk_cnt.as<int>(), d_m2l_tgt.as<int>(), d_m2l_src.as<int>(),
                          F.m2l.template as<T>(), kp, F.me.template as<T>(), F.le.template as<T>());
   }
+  mark(6);
   const int npairs = (int)far.m2p_tgt.size();
   if (npairs > 0) {
     const int64_t total = far.m2p_work_beg.back();
@@ -939,6 +974,7 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
                          d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt, kp,
                          F.me.template as<T>(), outN, outD);
   }
+  mark(7);
   for (int l = 2; l <= L; ++l) {
     const int cnt = l2l_off[l + 1] - l2l_off[l];
     if (cnt == 0) continue;
@@ -946,10 +982,12 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
                        d_l2l_tgt.as<int>() + l2l_off[l], d_parent.as<int>(), d_octant.as<int>(),
                        F.l2l.template as<T>() + (size_t)l * 8 * kp * kp, kp, F.le.template as<T>());
   }
+  mark(8);
   if (nrows > 0)
     hipLaunchKernelGGL(k_l2p<T>, dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
                        d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
                        d_delta.as<double>(), p, kt, kp, F.le.template as<T>(), outN, outD);
+  mark(9);
   HIPCHK(hipGetLastError());
 }
 
@@ -957,6 +995,7 @@ template <typename T>
 void Engine::matvec(const double *phi, const double *dphi, double *outN, double *outD) {
   generate<T>(phi, dphi);
   products<T>(phi, dphi, outN, outD);
+  collect_phase_times();
 }
 
 template void Engine::matvec<double>(const double *, const double *, double *, double *);

@@ -164,10 +164,21 @@ class BEMFMA:
         self._chk(self._L.pbfmm_get_expansions(self._h, 0 if which == "me" else 1, _p(a), _p(b)))
         return (a[0::2] + 1j * a[1::2]).reshape(nb, kt), (b[0::2] + 1j * b[1::2]).reshape(nb, kt)
 
+    def set_phase_timing(self, on=True):
+        self._chk(self._L.pbfmm_set_phase_timing(self._h, 1 if on else 0))
+
     def last_timings(self):
+        """Phase times (ms) of the last matvec: spmv, p2m, m2m, l2l, m2l, m2p, l2p, total."""
         ms = np.zeros(8)
         self._chk(self._L.pbfmm_last_timings(self._h, _p(ms)))
-        return ms
+        return dict(zip(("spmv", "p2m", "m2m", "l2l", "m2l", "m2p", "l2p", "total"), ms.tolist()))
+
+    def work_counts(self):
+        c = np.zeros(13, np.int64)
+        self._chk(self._L.pbfmm_work_counts(self._h, _p(c, _i64)))
+        keys = ("n_dofs", "near_values", "near_cols", "near_rows", "leaves", "quad_points", "m2l_pairs",
+                "m2p_evals", "m2m_translations", "l2l_translations", "num_blocks", "trunc_order", "kp")
+        return dict(zip(keys, (int(x) for x in c)))
 
 
 class BEMProblem:

@@ -54,7 +54,9 @@ SIGNATURES = {
                                     _i64]),
     "pbfmm_nearfield_export": (C.c_int, [_h, C.c_int32, _i64, _i64, _d, _d]),
     "pbfmm_get_expansions": (C.c_int, [_h, C.c_int32, _d, _d]),
+    "pbfmm_set_phase_timing": (C.c_int, [_h, C.c_int32]),
     "pbfmm_last_timings": (C.c_int, [_h, _d]),
+    "pbfmm_work_counts": (C.c_int, [_h, _i64]),
 }
 
 _lib = None
