@@ -554,14 +554,10 @@ int pbfmm_work_counts(pbfmm_handle *h, int64_t *c) {
     c[3] = (int64_t)e.near.rows.size();
     c[4] = (int64_t)e.far.p2m_leaf.size();
     c[5] = e.mesh.ncells * e.nq;
-    c[6] = (int64_t)e.far.m2l_tgt.size();
+    c[6] = (int64_t)e.far.m2l.tgt.size();
     c[7] = e.far.m2p_work_beg.empty() ? 0 : e.far.m2p_work_beg.back();
-    int64_t m2m = 0, l2l = 0;
-    for (int l = 1; l <= e.L; ++l)
-      for (int b : e.far.m2m_par[l]) m2m += e.tree.nchild[b];
-    for (int l = 2; l <= e.L; ++l) l2l += (int64_t)e.far.l2l_tgt[l].size();
-    c[8] = m2m;
-    c[9] = l2l;
+    c[8] = (int64_t)e.far.m2m.tgt.size();
+    c[9] = (int64_t)e.far.l2l.tgt.size();
     c[10] = e.tree.nb;
     c[11] = e.p;
     c[12] = e.kp;

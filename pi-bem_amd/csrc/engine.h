@@ -27,6 +27,11 @@ struct HipError : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// Chunk list of one batched translation (see FarPlan::ChunkList) on the device.
+struct ChunkDev {
+  DevBuf op, beg, cnt, tgt, src;
+};
+
 // Per-precision copies of the far-field operators and expansion storage.
 template <typename T>
 struct FarDev {
@@ -56,9 +61,8 @@ struct Engine {
   DevBuf d_pmin, d_delta, d_parent, d_nchild, d_child, d_octant, d_node_beg, d_node_cnt, d_nperm,
       d_quad_beg, d_quad_cnt, d_qperm;
   // device: plans
-  DevBuf d_p2m_leaf, d_m2m_par, d_l2l_tgt, d_m2l_tgt, d_m2l_src, d_chunk_op, d_chunk_beg,
-      d_chunk_cnt, d_m2p_tgt, d_m2p_src, d_m2p_work;
-  std::vector<int> m2m_off, l2l_off;  // per level offsets into d_m2m_par / d_l2l_tgt
+  DevBuf d_p2m_leaf, d_m2p_tgt, d_m2p_src, d_m2p_work;
+  ChunkDev m2m_cl, m2l_cl, l2l_cl;
   DevBuf d_row_leaf, d_row_beg, d_rows, d_col_beg, d_cols, d_val_beg, d_ent_beg, d_ent_cell,
       d_ent_mask, d_ent_near, d_ent_colpos, d_work_beg, d_leaf_block;
   DevBuf d_sing;  // singular rules: per js [w, N(dpc), dNx(dpc), dNy(dpc)] per point

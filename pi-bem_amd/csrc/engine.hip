@@ -262,206 +262,215 @@ __global__ void __launch_bounds__(256) k_spmv(int nrows, const int *__restrict__
 // ---------------------------------------------------------------------------
 // P2M at the leaves (generate_multipole_expansions, bem_fma.cc:2064-2114, with
 // the unit multipoles of multipole_integrals, bem_fma.cc:1675-1737, folded in:
-// the quad point strength is sum_j N_j value_j JxW / 4pi).  Ker1 is the normal
-// derivative expansion of phi (MultipoleExpansion::AddNormDer,
-// multipole_expansion.cc:124-181), Ker2 the monopole expansion of dphi/dn
-// (Add, :93-122).  One wavefront per leaf, lane = coefficient (n, m); the
-// leaf's quad points are staged through LDS 64 at a time.
+// the quad point strength is sum_j N_j value_j JxW / 4pi).  Ker2 (monopole of
+// dphi/dn, MultipoleExpansion::Add, multipole_expansion.cc:93-122) adds
+// q R_n^m(d) with the regular solid harmonic R_n^m(d) = rho^n Pt_n^m(cos a)
+// e^{-i m b}; Ker1 (MultipoleExpansion::AddNormDer, :124-181) adds
+// q (n . grad) R_n^m(d), the same normal derivative the reference forms with
+// the chain rule through (rho, alpha, beta).  Both come from trig-free
+// Cartesian recurrences in d = y - c:
+//   R_m^m = -sqrt((2m-1)/2m) (x - i y) R_{m-1}^{m-1}
+//   R_n^m = ra(n,m) z R_{n-1}^m - rb(n,m) |d|^2 R_{n-2}^m
+// and their directional derivatives (product rule, D(x - i y) = n_x - i n_y,
+// D z = n_z, D |d|^2 = 2 d.n).  One wavefront per leaf, lane = quad point,
+// Kt x 4 accumulators in registers (order P fixed at compile time), one
+// wavefront reduction per coefficient at the end.
 // ---------------------------------------------------------------------------
 template <typename T>
+__device__ __forceinline__ T wave_allsum(T v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename T, int P>
 __global__ void __launch_bounds__(64) k_p2m(
     const int *__restrict__ leaves, const int *__restrict__ quad_beg, const int *__restrict__ quad_cnt,
     const int *__restrict__ qperm, const T *__restrict__ qx, const T *__restrict__ qn,
     const T *__restrict__ qw, const T *__restrict__ qsh, const int *__restrict__ cdofs, int dpc, int nq,
     const double *__restrict__ phi, const double *__restrict__ dphi, const double *__restrict__ pmin,
-    const double *__restrict__ delta, int p, int kt, int kp, T *__restrict__ ME) {
-  __shared__ T s_q1[64], s_q2[64], s_rho[64], s_x[64], s_s[64], s_beta[64], s_dr[64], s_da[64], s_db[64];
+    const double *__restrict__ delta, int kp, T *__restrict__ ME) {
+  constexpr int KT = (P + 1) * (P + 2) / 2;
   const int b = leaves[blockIdx.x];
   const int lane = threadIdx.x;
   const double hd = delta[b] * 0.5;
   const T c0 = (T)(pmin[3 * b] + hd), c1 = (T)(pmin[3 * b + 1] + hd), c2 = (T)(pmin[3 * b + 2] + hd);
-  int nA = 0, mA = 0, nB = 0, mB = 0;
-  const bool hasA = lane < kt, hasB = lane + 64 < kt;
-  {
-    int k = lane, n = 0;
-    while (hasA && k > n) { k -= n + 1; ++n; }
-    nA = n; mA = k;
-    k = lane + 64; n = 0;
-    while (hasB && k > n) { k -= n + 1; ++n; }
-    nB = n; mB = k;
-  }
-  T a1r[2] = {0, 0}, a1i[2] = {0, 0}, a2r[2] = {0, 0}, a2i[2] = {0, 0};
-  const int beg = quad_beg[b], cnt = quad_cnt[b];
-  for (int base = 0; base < cnt; base += 64) {
-    const int i = base + lane;
-    T q1 = 0, q2 = 0, rho = 1, x = 0, s = 1, beta = 0, dr = 0, da = 0, db = 0;
-    if (i < cnt) {
-      const int g = qperm[beg + i];
-      const int cell = g / nq;
-      T s1 = 0, s2 = 0;
-      for (int j = 0; j < dpc; ++j) {
-        const T N = qsh[(int64_t)g * dpc + j];
-        const int d = cdofs[(int64_t)cell * dpc + j];
-        s1 += N * (T)phi[d];
-        s2 += N * (T)dphi[d];
-      }
-      const T wq = qw[g] / T(4 * kPi);
-      q1 = s1 * wq;
-      q2 = s2 * wq;
-      const T d0 = qx[3 * (int64_t)g] - c0, d1 = qx[3 * (int64_t)g + 1] - c1, d2 = qx[3 * (int64_t)g + 2] - c2;
-      const T nx = qn[3 * (int64_t)g], ny = qn[3 * (int64_t)g + 1], nz = qn[3 * (int64_t)g + 2];
-      rho = dsqrt(d0 * d0 + d1 * d1 + d2 * d2);
-      x = d2 / rho;
-      beta = datan2(d1, d0);
-      dr = (d0 / rho) * nx + (d1 / rho) * ny + (d2 / rho) * nz;
-      const T rxy2 = d0 * d0 + d1 * d1;
-      db = (-d1 / rxy2) * nx + (d0 / rxy2) * ny;
-      s = dsqrt(T(1) - x * x);
-      T sb, cbeta;
-      dsincos(beta, &sb, &cbeta);
-      da = (x * cbeta / rho) * nx + (x * sb / rho) * ny + (-s / rho) * nz;
-    }
-    s_q1[lane] = q1; s_q2[lane] = q2; s_rho[lane] = rho; s_x[lane] = x; s_s[lane] = s;
-    s_beta[lane] = beta; s_dr[lane] = dr; s_da[lane] = da; s_db[lane] = db;
-    __syncthreads();
-    const int cntc = min(64, cnt - base);
-    for (int t = 0; t < cntc; ++t) {
-      const T Q1 = s_q1[t], Q2 = s_q2[t], RHO = s_rho[t], X = s_x[t], S = s_s[t], BETA = s_beta[t];
-      const T DR = s_dr[t], DA = s_da[t], DB = s_db[t];
+  T aRr[KT], aRi[KT], aGr[KT], aGi[KT];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const bool has = u == 0 ? hasA : hasB;
-        if (!has) continue;
-        const int n = u == 0 ? nA : nB, m = u == 0 ? mA : mB;
-        T pmm = PMM<T>(m);
-        for (int k = 0; k < m; ++k) pmm *= -S;
-        T pp = 0, pc = pmm;
-        for (int k = m + 1; k <= n; ++k) {
-          const T pn = RA<T>(k, m) * X * pc - RB<T>(k, m) * pp;
-          pp = pc;
-          pc = pn;
-        }
-        T rn1 = 1;
-        for (int k = 1; k < n; ++k) rn1 *= RHO;
-        const T rn = n > 0 ? rn1 * RHO : T(1);
-        T sm, cm;
-        dsincos(T(m) * BETA, &sm, &cm);
-        if (Q2 != T(0)) {
-          const T mono = Q2 * (pc * rn);
-          a2r[u] += mono * cm;
-          a2i[u] -= mono * sm;
-        }
-        if (Q1 != T(0)) {
-          const T dPs = ((T(n) * X * pc - RS<T>(n, m) * pp) / (X * X - T(1))) * S;
-          const T A = (T(n) * (n > 0 ? rn1 : T(0)) * pc * DR - rn * dPs * DA) * Q1;
-          const T B = (-T(m) * rn * pc * DB) * Q1;
-          a1r[u] += cm * A + sm * B;
-          a1i[u] += cm * B - sm * A;
-        }
+  for (int k = 0; k < KT; ++k) aRr[k] = aRi[k] = aGr[k] = aGi[k] = T(0);
+  const int beg = quad_beg[b], cnt = quad_cnt[b];
+  for (int i = lane; i < cnt; i += 64) {
+    const int g = qperm[beg + i];
+    const int cell = g / nq;
+    T s1 = 0, s2 = 0;
+    for (int j = 0; j < dpc; ++j) {
+      const T N = qsh[(int64_t)g * dpc + j];
+      const int d = cdofs[(int64_t)cell * dpc + j];
+      s1 += N * (T)phi[d];
+      s2 += N * (T)dphi[d];
+    }
+    const T wq = qw[g] / T(4 * kPi);
+    const T q1 = s1 * wq, q2 = s2 * wq;
+    const T x = qx[3 * (int64_t)g] - c0, y = qx[3 * (int64_t)g + 1] - c1, z = qx[3 * (int64_t)g + 2] - c2;
+    const T nx = qn[3 * (int64_t)g], ny = qn[3 * (int64_t)g + 1], nz = qn[3 * (int64_t)g + 2];
+    const T r2 = x * x + y * y + z * z;
+    const T dn2 = T(2) * (x * nx + y * ny + z * nz);
+    T Rmr = 1, Rmi = 0, Gmr = 0, Gmi = 0;  // R_m^m, D R_m^m
+#pragma unroll
+    for (int m = 0; m <= P; ++m) {
+      if (m > 0) {
+        const T f = -PMF<T>(m);
+        const T nr = f * (x * Rmr + y * Rmi), ni = f * (x * Rmi - y * Rmr);
+        const T gr = f * ((nx * Rmr + ny * Rmi) + (x * Gmr + y * Gmi));
+        const T gi = f * ((nx * Rmi - ny * Rmr) + (x * Gmi - y * Gmr));
+        Rmr = nr; Rmi = ni; Gmr = gr; Gmi = gi;
+      }
+      T Rcr = Rmr, Rci = Rmi, Gcr = Gmr, Gci = Gmi, Rpr = 0, Rpi = 0, Gpr = 0, Gpi = 0;
+      const int k0 = m * (m + 1) / 2 + m;
+      aRr[k0] += q2 * Rcr; aRi[k0] += q2 * Rci;
+      aGr[k0] += q1 * Gcr; aGi[k0] += q1 * Gci;
+#pragma unroll
+      for (int n = m + 1; n <= P; ++n) {
+        const T a = RA<T>(n, m), bb = RB<T>(n, m);
+        const T Rnr = a * z * Rcr - bb * r2 * Rpr, Rni = a * z * Rci - bb * r2 * Rpi;
+        const T Gnr = a * (nz * Rcr + z * Gcr) - bb * (dn2 * Rpr + r2 * Gpr);
+        const T Gni = a * (nz * Rci + z * Gci) - bb * (dn2 * Rpi + r2 * Gpi);
+        const int k = n * (n + 1) / 2 + m;
+        aRr[k] += q2 * Rnr; aRi[k] += q2 * Rni;
+        aGr[k] += q1 * Gnr; aGi[k] += q1 * Gni;
+        Rpr = Rcr; Rpi = Rci; Gpr = Gcr; Gpi = Gci;
+        Rcr = Rnr; Rci = Rni; Gcr = Gnr; Gci = Gni;
       }
     }
-    __syncthreads();
   }
   T *M1 = ME + (int64_t)b * 2 * kp, *M2 = M1 + kp;
-  if (hasA) {
-    M1[lane] = a1r[0]; M1[kt + lane] = a1i[0];
-    M2[lane] = a2r[0]; M2[kt + lane] = a2i[0];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const T v0 = wave_allsum(aGr[k]), v1 = wave_allsum(aGi[k]);
+    const T v2 = wave_allsum(aRr[k]), v3 = wave_allsum(aRi[k]);
+    if (lane == (k & 63)) {
+      M1[k] = v0; M1[KT + k] = v1;
+      M2[k] = v2; M2[KT + k] = v3;
+    }
   }
-  if (hasB) {
-    M1[lane + 64] = a1r[1]; M1[kt + lane + 64] = a1i[1];
-    M2[lane + 64] = a2r[1]; M2[kt + lane + 64] = a2i[1];
-  }
-  for (int k = 2 * kt + lane; k < kp; k += 64) {
+  for (int k = 2 * KT + lane; k < kp; k += 64) {
     M1[k] = T(0);
     M2[k] = T(0);
   }
 }
 
+// Padded length of one real expansion vector at order P.
+template <int P>
+struct Kp {
+  static constexpr int kt = (P + 1) * (P + 2) / 2;
+  static constexpr int value = ((2 * kt + 15) / 16) * 16;
+};
+
 // ---------------------------------------------------------------------------
-// M2M (MultipoleExpansion::Add(child), multipole_expansion.cc:184-261, driven
-// by bem_fma.cc:2274-2339): parent = sum over children of T_octant . child.
-// One block of 2*kp threads per parent; thread = (kernel, output row).
+// Translations as batched GEMMs on the matrix cores: M2M
+// (MultipoleExpansion::Add(child), multipole_expansion.cc:184-261, driven by
+// bem_fma.cc:2274-2339), M2L (LocalExpansion::Add(multipole),
+// local_expansion.cc:179-248, driven by bem_fma.cc:2622-2652) and L2L
+// (LocalExpansion::Add(parent), local_expansion.cc:107-176, driven by
+// bem_fma.cc:2600-2602) are all OUT[tgt] += T_class . IN[src] with one R-linear
+// KP x KP operator per class (M2M / L2L: level and child octant; M2L: level
+// and integer offset).  Pairs are grouped by class into chunks of 32; a block
+// computes C (KP x 64) = A (KP x KP) . B (KP x 64 source columns, both
+// kernels) with 16x16x4 MFMA tiles (wave w: columns 16w..16w+15, all KP/16
+// row tiles), stages C through LDS and adds it to the targets with coalesced
+// atomics (targets are distinct within a chunk, not across chunks).
+//   f32 (v_mfma_f32_16x16x4f32): lane l holds A[l%16][l/16], B[l/16][l%16],
+//       D[4(l/16)+j][l%16], j < 4
+//   f64 (v_mfma_f64_16x16x4f64): same inputs, D[l/16+4j][l%16]
+// (both layouts checked on gfx950 with an in-bounds probe).
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ void k_m2m(const int *__restrict__ parents, const int *__restrict__ nchild,
-                      const int *__restrict__ child, const int *__restrict__ octant,
-                      const T *__restrict__ ops, int kp, T *__restrict__ ME) {
-  extern __shared__ unsigned char smem_raw[];
-  T *sv = reinterpret_cast<T *>(smem_raw);
-  const int P = parents[blockIdx.x];
-  const int t = threadIdx.x, ker = t / kp, row = t - ker * kp;
-  T acc = 0;
-  for (int c = 0; c < nchild[P]; ++c) {
-    const int ch = child[8 * P + c];
-    __syncthreads();
-    sv[t] = ME[(int64_t)ch * 2 * kp + t];
-    __syncthreads();
-    const T *op = ops + (int64_t)octant[ch] * kp * kp;
-    const T *v = sv + ker * kp;
-    for (int k = 0; k < kp; ++k) acc += op[(int64_t)k * kp + row] * v[k];
+typedef float v4f32 __attribute__((ext_vector_type(4)));
+typedef double v4f64 __attribute__((ext_vector_type(4)));
+template <typename T> struct Mfma;
+template <> struct Mfma<float> {
+  using acc = v4f32;
+  static __device__ __forceinline__ acc mac(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
-  ME[(int64_t)P * 2 * kp + t] = acc;
-}
+  static __device__ __forceinline__ int row(int l, int j) { return 4 * (l >> 4) + j; }
+};
+template <> struct Mfma<double> {
+  using acc = v4f64;
+  static __device__ __forceinline__ acc mac(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int j) { return (l >> 4) + 4 * j; }
+};
 
-// L2L (LocalExpansion::Add(parent), local_expansion.cc:107-176, driven by
-// bem_fma.cc:2600-2602): LE[b] += T_octant . LE[parent].
-template <typename T>
-__global__ void k_l2l(const int *__restrict__ tgts, const int *__restrict__ parent,
-                      const int *__restrict__ octant, const T *__restrict__ ops, int kp, T *__restrict__ LE) {
-  extern __shared__ unsigned char smem_raw[];
-  T *sv = reinterpret_cast<T *>(smem_raw);
-  const int b = tgts[blockIdx.x];
-  const int t = threadIdx.x, ker = t / kp, row = t - ker * kp;
-  sv[t] = LE[(int64_t)parent[b] * 2 * kp + t];
-  __syncthreads();
-  const T *op = ops + (int64_t)octant[b] * kp * kp;
-  const T *v = sv + ker * kp;
-  T acc = 0;
-  for (int k = 0; k < kp; ++k) acc += op[(int64_t)k * kp + row] * v[k];
-  LE[(int64_t)b * 2 * kp + t] += acc;
-}
+static_assert(kM2LChunk == 32, "k_translate maps 2 x 32 pairs onto 64 GEMM columns");
 
-// ---------------------------------------------------------------------------
-// M2L (LocalExpansion::Add(multipole), local_expansion.cc:179-248, driven by
-// bem_fma.cc:2622-2652), regrouped by translation offset: one block per chunk
-// of <= 64 (target, source) pairs sharing one operator, the operator staged in
-// LDS when it fits, 4 pairs per pass so that each operator element read from
-// LDS feeds 4 FMAs.  Results are accumulated into the targets atomically.
-// ---------------------------------------------------------------------------
-template <typename T, bool OP_LDS>
-__global__ void k_m2l(const int *__restrict__ chunk_op, const int *__restrict__ chunk_beg,
-                      const int *__restrict__ chunk_cnt, const int *__restrict__ tgt,
-                      const int *__restrict__ src, const T *__restrict__ ops, int kp,
-                      const T *__restrict__ ME, T *__restrict__ LE) {
+// LDS layout (elements of T): B tile [KP][CS] (k-major, 64 columns padded to
+// CS; reused for C as [64][KP + 1] after the GEMM), then A [KP][AS]
+// (column-major as stored, padded) when staged.
+template <int P, typename T>
+struct TrLayout {
+  static_assert((64 + 16 / (sizeof(T) / 4)) * ((((P + 1) * (P + 2) + 15) / 16) * 16) >=
+                    64 * (((((P + 1) * (P + 2) + 15) / 16) * 16) + 1),
+                "C tile must fit in the B tile");
+  static constexpr int KP = Kp<P>::value;
+  static constexpr int CS = 64 + 16 / (int)(sizeof(T) / 4);  // row stride of the B / C tile
+  static constexpr int AS = KP + 16 / (int)(sizeof(T) / 4);  // column stride of A
+  static constexpr size_t b_bytes = (size_t)KP * CS * sizeof(T);
+  static constexpr size_t a_bytes = (size_t)KP * AS * sizeof(T);
+};
+
+template <typename T, int P, bool A_LDS>
+__global__ void __launch_bounds__(256) k_translate(const int *__restrict__ chunk_op,
+                                                   const int *__restrict__ chunk_beg,
+                                                   const int *__restrict__ chunk_cnt, const int *__restrict__ tgt,
+                                                   const int *__restrict__ src, const T *__restrict__ ops,
+                                                   const T *__restrict__ IN, T *__restrict__ OUT) {
+  using Lay = TrLayout<P, T>;
+  constexpr int KP = Kp<P>::value, K2 = 2 * KP, KT2 = 2 * Kp<P>::kt, NR = KP / 16;
+  constexpr int CS = Lay::CS, AS = Lay::AS;
+  using M = Mfma<T>;
   extern __shared__ unsigned char smem_raw[];
-  T *sop = reinterpret_cast<T *>(smem_raw);
-  T *sv = sop + (OP_LDS ? kp * kp : 0);
+  T *sB = reinterpret_cast<T *>(smem_raw);  // [KP][CS]
+  T *sA = sB + KP * CS;                     // [KP][AS]
+  __shared__ int sT[kM2LChunk];
   const int c = blockIdx.x;
-  const int t = threadIdx.x, nt = blockDim.x, ker = t / kp, row = t - ker * kp;
-  const T *opg = ops + (int64_t)chunk_op[c] * kp * kp;
-  if (OP_LDS)
-    for (int i = t; i < kp * kp; i += nt) sop[i] = opg[i];
-  const T *op = OP_LDS ? sop : opg;
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const T *opg = ops + (int64_t)chunk_op[c] * KP * KP;
   const int beg = chunk_beg[c], cnt = chunk_cnt[c];
-  const int K2 = 2 * kp;
-  for (int i0 = 0; i0 < cnt; i0 += 4) {
-    const int np = min(4, cnt - i0);
-    __syncthreads();
-    for (int q = 0; q < 4; ++q) sv[q * K2 + t] = (q < np) ? ME[(int64_t)src[beg + i0 + q] * K2 + t] : T(0);
-    __syncthreads();
-    T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    const T *v0 = sv + ker * kp;
-    for (int k = 0; k < kp; ++k) {
-      const T o = op[k * kp + row];
-      a0 += o * v0[k];
-      a1 += o * v0[K2 + k];
-      a2 += o * v0[2 * K2 + k];
-      a3 += o * v0[3 * K2 + k];
+  if (A_LDS)
+    for (int i = t; i < KP * KP; i += 256) {
+      const int k = i / KP, r = i - k * KP;
+      sA[k * AS + r] = opg[i];
     }
-    atomicAdd(LE + (int64_t)tgt[beg + i0] * K2 + t, a0);
-    if (np > 1) atomicAdd(LE + (int64_t)tgt[beg + i0 + 1] * K2 + t, a1);
-    if (np > 2) atomicAdd(LE + (int64_t)tgt[beg + i0 + 2] * K2 + t, a2);
-    if (np > 3) atomicAdd(LE + (int64_t)tgt[beg + i0 + 3] * K2 + t, a3);
+  for (int i = t; i < 64 * KP; i += 256) {
+    const int col = i / KP, k = i - col * KP, pr = col >> 1, ker = col & 1;
+    sB[k * CS + col] = pr < cnt ? IN[(int64_t)src[beg + pr] * K2 + ker * KP + k] : T(0);
+  }
+  if (t < kM2LChunk) sT[t] = t < cnt ? tgt[beg + t] : -1;
+  __syncthreads();
+  typename M::acc acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = typename M::acc{0, 0, 0, 0};
+  const int col = 16 * w + (l & 15);
+  const T *bk = sB + (l >> 4) * CS + col;
+  const T *ak = A_LDS ? (sA + (l >> 4) * AS + (l & 15)) : (opg + (l >> 4) * KP + (l & 15));
+  const int astep = A_LDS ? AS : KP;
+#pragma unroll 4
+  for (int k0 = 0; k0 < KP; k0 += 4) {
+    const T b = bk[k0 * CS];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) acc[r] = M::mac(ak[k0 * astep + 16 * r], b, acc[r]);
+  }
+  __syncthreads();  // B tile no longer needed: reuse it for C, column-major [64][KP + 1]
+  T *sC = sB;
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sC[col * (KP + 1) + 16 * r + M::row(l, j)] = acc[r][j];
+  __syncthreads();
+  // coalesced accumulation: consecutive lanes add consecutive rows of one target
+  for (int i = t; i < 2 * cnt * KT2; i += 256) {
+    const int cc = i / KT2, row = i - cc * KT2;
+    atomicAdd(OUT + (int64_t)sT[cc >> 1] * K2 + (cc & 1) * KP + row, sC[cc * (KP + 1) + row]);
   }
 }
 
@@ -682,6 +691,12 @@ __global__ void k_scale_to(int64_t n, const double *__restrict__ x, double s, do
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) y[i] = s * x[i];
 }
+// v /= *nrm when *nrm > 0 (the Arnoldi normalisation, norm kept on the device)
+__global__ void k_scale_inv(int64_t n, double *__restrict__ v, const double *__restrict__ nrm) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const double s = *nrm;
+  if (i < n && s > 0) v[i] /= s;
+}
 
 // ===========================================================================
 // Host orchestration
@@ -729,7 +744,7 @@ Engine::~Engine() {
 }
 
 // Phase events on the handle's stream (no synchronisation between kernels):
-// 0 P2M 1 M2M 2 | 3 SpMV 4 fill 5 M2L 6 M2P 7 L2L 8 L2P 9.
+// 0 P2M 1 M2M 2, 3 SpMV 4 fill 5 M2L 6 M2P 7 L2L 8 L2P 9.
 void Engine::mark(int i) {
   if (!time_phases) return;
   if (!ev[i]) HIPCHK(hipEventCreate(&ev[i]));
@@ -827,26 +842,19 @@ void Engine::upload_quad(bool want_f32) {
   have_f32 = want_f32;
 }
 
+static void upload_chunks(ChunkDev &d, const ChunkList &c) {
+  up(d.op, c.op);
+  up(d.beg, c.beg);
+  up(d.cnt, c.cnt);
+  up(d.tgt, c.tgt);
+  up(d.src, c.src);
+}
+
 void Engine::upload_far(bool want_f32) {
-  std::vector<int32_t> par, tg;
-  m2m_off.assign(L + 2, 0);
-  l2l_off.assign(L + 2, 0);
-  for (int l = 0; l <= L; ++l) {
-    m2m_off[l] = (int)par.size();
-    par.insert(par.end(), far.m2m_par[l].begin(), far.m2m_par[l].end());
-    l2l_off[l] = (int)tg.size();
-    tg.insert(tg.end(), far.l2l_tgt[l].begin(), far.l2l_tgt[l].end());
-  }
-  m2m_off[L + 1] = (int)par.size();
-  l2l_off[L + 1] = (int)tg.size();
-  up(d_m2m_par, par);
-  up(d_l2l_tgt, tg);
   up(d_p2m_leaf, far.p2m_leaf);
-  up(d_m2l_tgt, far.m2l_tgt);
-  up(d_m2l_src, far.m2l_src);
-  up(d_chunk_op, far.chunk_op);
-  up(d_chunk_beg, far.chunk_beg);
-  up(d_chunk_cnt, far.chunk_cnt);
+  upload_chunks(m2m_cl, far.m2m);
+  upload_chunks(m2l_cl, far.m2l);
+  upload_chunks(l2l_cl, far.l2l);
   up(d_m2p_tgt, far.m2p_tgt);
   up(d_m2p_src, far.m2p_src);
   up(d_m2p_work, far.m2p_work_beg);
@@ -916,24 +924,60 @@ FarDev<double> &far_of<double>(Engine &e) { return e.f64; }
 template <>
 FarDev<float> &far_of<float>(Engine &e) { return e.f32; }
 
+// OUT[tgt] += ops[class] . IN[src] for the chunks [c0, c1) of a chunk list.
+template <typename T>
+static void launch_translate(Engine &e, const ChunkDev &cl, int c0, int c1, const T *ops, const T *IN, T *OUT) {
+  const int n = c1 - c0;
+  if (n <= 0) return;
+#define PBFMM_TR(PP)                                                                                          \
+  case PP: {                                                                                                  \
+    using Lay = TrLayout<PP, T>;                                                                              \
+    if (Lay::a_bytes + Lay::b_bytes <= 64 * 1024)                                                             \
+      hipLaunchKernelGGL((k_translate<T, PP, true>), dim3(n), dim3(256), Lay::a_bytes + Lay::b_bytes, e.stream, \
+                         cl.op.as<int>() + c0, cl.beg.as<int>() + c0, cl.cnt.as<int>() + c0, cl.tgt.as<int>(),   \
+                         cl.src.as<int>(), ops, IN, OUT);                                                     \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_translate<T, PP, false>), dim3(n), dim3(256), Lay::b_bytes, e.stream,               \
+                         cl.op.as<int>() + c0, cl.beg.as<int>() + c0, cl.cnt.as<int>() + c0, cl.tgt.as<int>(),   \
+                         cl.src.as<int>(), ops, IN, OUT);                                                     \
+    break;                                                                                                    \
+  }
+  switch (e.p) {
+    PBFMM_TR(1) PBFMM_TR(2) PBFMM_TR(3) PBFMM_TR(4) PBFMM_TR(5)
+    PBFMM_TR(6) PBFMM_TR(7) PBFMM_TR(8) PBFMM_TR(9) PBFMM_TR(10)
+    default: throw std::runtime_error("trunc_order outside 1..10");
+  }
+#undef PBFMM_TR
+}
+
 template <typename T>
 void Engine::generate(const double *phi, const double *dphi) {
   FarDev<T> &F = far_of<T>(*this);
   const int nleaf = (int)far.p2m_leaf.size();
   mark(0);
-  if (nleaf > 0)
-    hipLaunchKernelGGL(k_p2m<T>, dim3(nleaf), dim3(64), 0, stream, d_p2m_leaf.as<int>(), d_quad_beg.as<int>(),
-                       d_quad_cnt.as<int>(), d_qperm.as<int>(), F.qx.template as<T>(), F.qn.template as<T>(),
-                       F.qw.template as<T>(), F.qsh.template as<T>(), d_cdofs.as<int>(), dpc, nq, phi, dphi,
-                       d_pmin.as<double>(), d_delta.as<double>(), p, kt, kp, F.me.template as<T>());
-  for (int l = L; l >= 1; --l) {
-    const int cnt = m2m_off[l + 1] - m2m_off[l];
-    if (cnt == 0) continue;
-    hipLaunchKernelGGL(k_m2m<T>, dim3(cnt), dim3(2 * kp), 2 * kp * sizeof(T), stream,
-                       d_m2m_par.as<int>() + m2m_off[l], d_nchild.as<int>(), d_child.as<int>(), d_octant.as<int>(),
-                       F.m2m.template as<T>() + (size_t)l * 8 * kp * kp, kp, F.me.template as<T>());
+  const int64_t nme = (int64_t)tree.nb * 2 * kp;
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(nme, 256), 4096)), dim3(256), 0, stream,
+                     F.me.template as<T>(), nme, T(0));
+  if (nleaf > 0) {
+#define PBFMM_P2M(PP)                                                                                          \
+  case PP:                                                                                                     \
+    hipLaunchKernelGGL((k_p2m<T, PP>), dim3(nleaf), dim3(64), 0, stream, d_p2m_leaf.as<int>(),                  \
+                       d_quad_beg.as<int>(), d_quad_cnt.as<int>(), d_qperm.as<int>(), F.qx.template as<T>(),   \
+                       F.qn.template as<T>(), F.qw.template as<T>(), F.qsh.template as<T>(), d_cdofs.as<int>(), \
+                       dpc, nq, phi, dphi, d_pmin.as<double>(), d_delta.as<double>(), kp, F.me.template as<T>()); \
+    break;
+    switch (p) {
+      PBFMM_P2M(1) PBFMM_P2M(2) PBFMM_P2M(3) PBFMM_P2M(4) PBFMM_P2M(5)
+      PBFMM_P2M(6) PBFMM_P2M(7) PBFMM_P2M(8) PBFMM_P2M(9) PBFMM_P2M(10)
+      default: throw std::runtime_error("trunc_order outside 1..10");
+    }
+#undef PBFMM_P2M
   }
   mark(1);
+  for (int l = L; l >= 1; --l)
+    launch_translate<T>(*this, m2m_cl, far.m2m.level_off[l], far.m2m.level_off[l + 1], F.m2m.template as<T>(),
+                        F.me.template as<T>(), F.me.template as<T>());
+  mark(2);
   HIPCHK(hipGetLastError());
 }
 
@@ -941,7 +985,6 @@ template <typename T>
 void Engine::products(const double *phi, const double *dphi, double *outN, double *outD) {
   FarDev<T> &F = far_of<T>(*this);
   const int nrows = (int)near.rows.size();
-  mark(2);
   mark(3);
   if (nrows > 0)
     hipLaunchKernelGGL(k_spmv<T>, dim3(nblk((int64_t)nrows * 64, 256)), dim3(256), 0, stream, nrows,
@@ -952,18 +995,8 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
   hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(nle, 256), 4096)), dim3(256), 0, stream,
                      F.le.template as<T>(), nle, T(0));
   mark(5);
-  const int nchunk = (int)far.chunk_op.size();
-  if (nchunk > 0) {
-    const size_t op_bytes = (size_t)kp * kp * sizeof(T), v_bytes = (size_t)4 * 2 * kp * sizeof(T);
-    if (op_bytes + v_bytes <= 64 * 1024)
-      hipLaunchKernelGGL((k_m2l<T, true>), dim3(nchunk), dim3(2 * kp), op_bytes + v_bytes, stream,
-                         d_chunk_op.as<int>(), d_chunk_beg.as<int>(), d_chunk_cnt.as<int>(), d_m2l_tgt.as<int>(),
-                         d_m2l_src.as<int>(), F.m2l.template as<T>(), kp, F.me.template as<T>(), F.le.template as<T>());
-    else
-      hipLaunchKernelGGL((k_m2l<T, false>), dim3(nchunk), dim3(2 * kp), v_bytes, stream, d_chunk_op.as<int>(),
-                         d_chunk_beg.as<int>(), d_chunk_cnt.as<int>(), d_m2l_tgt.as<int>(), d_m2l_src.as<int>(),
-                         F.m2l.template as<T>(), kp, F.me.template as<T>(), F.le.template as<T>());
-  }
+  launch_translate<T>(*this, m2l_cl, 0, (int)far.m2l.op.size(), F.m2l.template as<T>(), F.me.template as<T>(),
+                      F.le.template as<T>());
   mark(6);
   const int npairs = (int)far.m2p_tgt.size();
   if (npairs > 0) {
@@ -975,13 +1008,9 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
                          F.me.template as<T>(), outN, outD);
   }
   mark(7);
-  for (int l = 2; l <= L; ++l) {
-    const int cnt = l2l_off[l + 1] - l2l_off[l];
-    if (cnt == 0) continue;
-    hipLaunchKernelGGL(k_l2l<T>, dim3(cnt), dim3(2 * kp), 2 * kp * sizeof(T), stream,
-                       d_l2l_tgt.as<int>() + l2l_off[l], d_parent.as<int>(), d_octant.as<int>(),
-                       F.l2l.template as<T>() + (size_t)l * 8 * kp * kp, kp, F.le.template as<T>());
-  }
+  for (int l = 2; l <= L; ++l)
+    launch_translate<T>(*this, l2l_cl, far.l2l.level_off[l], far.l2l.level_off[l + 1], F.l2l.template as<T>(),
+                        F.le.template as<T>(), F.le.template as<T>());
   mark(8);
   if (nrows > 0)
     hipLaunchKernelGGL(k_l2p<T>, dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
@@ -1080,6 +1109,7 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
   double *V = e.d_V.as<double>(), *w = e.d_w.as<double>(), *hd = e.d_h.as<double>();
   double *part = e.d_red.as<double>();
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1), h(m + 2), y(m);
+  std::vector<double> hbuf(2 * (m + 2) + 1);
   auto dots = [&](int nv, const double *vec, double *out_dev) {
     hipLaunchKernelGGL(k_mdot, dim3(nchunk, nv), dim3(kRedThreads), 0, e.stream, n, n, V, vec, part);
     hipLaunchKernelGGL(k_finish, dim3(nv), dim3(kRedThreads), 0, e.stream, nchunk, part, out_dev);
@@ -1100,18 +1130,21 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
       double *vk1 = V + (int64_t)(k + 1) * n;
       e.vmult(V + (int64_t)k * n, vk1, precision);
       ++it;
-      for (int j = 0; j <= k + 1; ++j) h[j] = 0.0;
-      for (int pass = 0; pass < 2; ++pass) {
-        dots(k + 1, vk1, hd);
-        hipLaunchKernelGGL(k_gemv_sub, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, hd, vk1);
-        std::vector<double> hp(k + 1);
-        HIPCHK(hipMemcpyAsync(hp.data(), hd, (k + 1) * 8, hipMemcpyDeviceToHost, e.stream));
-        e.sync();
-        for (int j = 0; j <= k; ++j) h[j] += hp[j];
-      }
-      const double hn = e.norm2(vk1);
+      // CGS2: two projections and the norm on the device, one host round trip
+      double *h1 = hd, *h2 = hd + (m + 2), *hn_d = hd + 2 * (m + 2);
+      dots(k + 1, vk1, h1);
+      hipLaunchKernelGGL(k_gemv_sub, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, h1, vk1);
+      dots(k + 1, vk1, h2);
+      hipLaunchKernelGGL(k_gemv_sub, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, h2, vk1);
+      hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, e.stream, n, n, vk1, vk1, part);
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, e.stream, nchunk, part, hn_d);
+      hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(64), 0, e.stream, hn_d);
+      hipLaunchKernelGGL(k_scale_inv, dim3(g), dim3(256), 0, e.stream, n, vk1, hn_d);
+      HIPCHK(hipMemcpyAsync(hbuf.data(), hd, (2 * (m + 2) + 1) * 8, hipMemcpyDeviceToHost, e.stream));
+      e.sync();
+      for (int j = 0; j <= k; ++j) h[j] = hbuf[j] + hbuf[(m + 2) + j];
+      const double hn = hbuf[2 * (m + 2)];
       h[k + 1] = hn;
-      if (hn > 0) hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, vk1, 1.0 / hn, vk1);
       for (int j = 0; j < k; ++j) {
         const double a = h[j], bb = h[j + 1];
         h[j] = cs[j] * a + sn[j] * bb;
@@ -1151,7 +1184,7 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
     alloc(d_w, n * 8);
     alloc(d_r, n * 8);
     alloc(d_x, n * 8);
-    alloc(d_h, (size_t)(restart + 2) * 8);
+    alloc(d_h, (size_t)(2 * (restart + 2) + 2) * 8);
     gm_restart = restart;
   }
   const unsigned g = nblk(n, 256);

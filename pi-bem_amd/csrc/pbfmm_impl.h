@@ -15,6 +15,8 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kToll = 0.001;  // box adjacency tolerance of bem_fma.cc:1
 
 inline int kt_of(int p) { return (p + 1) * (p + 2) / 2; }
+// M2L pairs per GEMM chunk (64 columns of the batched M2L: 2 kernels x 32 pairs)
+constexpr int kM2LChunk = 32;
 // padded length of one real expansion vector [Re(0..Kt-1), Im(0..Kt-1)]
 inline int kp_of(int p) { return ((2 * kt_of(p) + 15) / 16) * 16; }
 
@@ -98,16 +100,21 @@ void build_near_plan(const HostMesh &m, const QuadData &qd, const Tree &t, NearP
 // ---------------------------------------------------------------------------
 // Far field plan and translation operators
 // ---------------------------------------------------------------------------
+// Pairs (tgt, src) of one kind of translation grouped by operator class and
+// cut into chunks of <= kM2LChunk pairs; level_off[l] .. level_off[l+1] are
+// the chunks of level l (M2M: child level; L2L: target level).
+struct ChunkList {
+  std::vector<int32_t> op, beg, cnt;  // per chunk: operator index, first pair, pair count
+  std::vector<int32_t> tgt, src;      // pairs
+  std::vector<int32_t> level_off;     // L+2 entries
+};
+
 struct FarPlan {
   int p = 0, kt = 0, kp = 0;
-  // upward
-  std::vector<int32_t> p2m_leaf;              // all childless blocks
-  std::vector<std::vector<int32_t>> m2m_par;  // per child level l: parents at l-1 with children
-  // downward
-  std::vector<std::vector<int32_t>> l2l_tgt;  // per level: blocks of dofs_filled[l], l>=2
-  // M2L pairs sorted by (level, class); chunks of <= 64 pairs of one class
-  std::vector<int32_t> m2l_tgt, m2l_src;
-  std::vector<int32_t> chunk_op, chunk_beg, chunk_cnt;
+  std::vector<int32_t> p2m_leaf;  // all childless blocks
+  ChunkList m2m;                  // tgt = parent, src = child; op = level*8 + octant
+  ChunkList m2l;                  // op = offset class (all levels)
+  ChunkList l2l;                  // tgt = child, src = parent; op = level*8 + octant
   std::vector<int64_t> m2l_per_level;
   int n_m2l_ops = 0;
   // M2P pairs (target block, smaller far source block)

@@ -534,6 +534,18 @@ void op_m2l(int p, const double d[3], double *opT) {
 // Far-field plan (generate_multipole_expansions bem_fma.cc:1970-2380 and
 // multipole_matr_vect_products bem_fma.cc:2390-3072)
 // ---------------------------------------------------------------------------
+static void add_class(ChunkList &cl, int op, const std::vector<std::pair<int, int>> &pairs) {
+  for (size_t i = 0; i < pairs.size(); i += kM2LChunk) {
+    cl.op.push_back(op);
+    cl.beg.push_back((int)cl.tgt.size() + (int)i);
+    cl.cnt.push_back((int)std::min<size_t>(kM2LChunk, pairs.size() - i));
+  }
+  for (auto &pr : pairs) {
+    cl.tgt.push_back(pr.first);
+    cl.src.push_back(pr.second);
+  }
+}
+
 void build_far_plan(const Tree &t, int p, FarPlan &fp) {
   fp = FarPlan();
   fp.p = p;
@@ -541,12 +553,6 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
   fp.kp = kp_of(p);
   const int L = t.L, kp = fp.kp;
   fp.p2m_leaf = t.leaves;
-  fp.m2m_par.assign(L + 1, {});
-  for (int l = 1; l <= L; ++l)
-    for (int b = t.startL[l - 1]; b <= t.endL[l - 1] && t.endL[l] >= t.startL[l]; ++b)
-      if (t.nchild[b] > 0) fp.m2m_par[l].push_back(b);
-  fp.l2l_tgt.assign(L + 1, {});
-  for (int l = 2; l <= L; ++l) fp.l2l_tgt[l] = t.dofs_filled[l];
   for (int b : t.leaves)
     if (t.node_cnt[b] > 0) fp.l2p_leaf.push_back(b);
 
@@ -563,11 +569,33 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
       op_l2l(p, dn, &fp.l2l_ops[((size_t)l * 8 + o) * kp * kp]);
     }
   }
+  // M2M pairs (bem_fma.cc:2318-2339: every block of level l into its parent)
+  // and L2L pairs (bem_fma.cc:2600-2602: blocks of dofs_filled[l] from their
+  // parent; level 1 takes the root's local expansion, which is zero)
+  fp.m2m.level_off.assign(L + 2, 0);
+  fp.l2l.level_off.assign(L + 2, 0);
+  for (int l = 0; l <= L; ++l) {
+    fp.m2m.level_off[l] = (int)fp.m2m.op.size();
+    fp.l2l.level_off[l] = (int)fp.l2l.op.size();
+    if (l == 0) continue;
+    std::vector<std::pair<int, int>> byoct[8];
+    for (int b = t.startL[l]; b <= t.endL[l]; ++b) byoct[t.octant[b]].push_back({t.parent[b], b});
+    for (int o = 0; o < 8; ++o) add_class(fp.m2m, l * 8 + o, byoct[o]);
+    if (l >= 2) {
+      std::vector<std::pair<int, int>> lo[8];
+      for (int b : t.dofs_filled[l]) lo[t.octant[b]].push_back({b, t.parent[b]});
+      for (int o = 0; o < 8; ++o) add_class(fp.l2l, l * 8 + o, lo[o]);
+    }
+  }
+  fp.m2m.level_off[L + 1] = (int)fp.m2m.op.size();
+  fp.l2l.level_off[L + 1] = (int)fp.l2l.op.size();
 
   // M2L pairs by (level, offset class); M2P pairs
   fp.m2l_per_level.assign(L + 1, 0);
+  fp.m2l.level_off.assign(L + 2, 0);
   fp.m2p_work_beg.push_back(0);
   for (int l = 1; l <= L; ++l) {
+    fp.m2l.level_off[l] = (int)fp.m2l.op.size();
     std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> cls;
     const double dl = t.delta[0] / std::pow(2.0, l);
     for (int b : t.dofs_filled[l]) {
@@ -593,19 +621,11 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
       fp.m2l_ops.resize((size_t)fp.n_m2l_ops * kp * kp);
       const double d[3] = {kv.first[0] * dl, kv.first[1] * dl, kv.first[2] * dl};
       op_m2l(p, d, &fp.m2l_ops[(size_t)op * kp * kp]);
-      const auto &pairs = kv.second;
-      for (size_t i = 0; i < pairs.size(); i += 64) {
-        fp.chunk_op.push_back(op);
-        fp.chunk_beg.push_back((int)fp.m2l_tgt.size() + (int)i);
-        fp.chunk_cnt.push_back((int)std::min<size_t>(64, pairs.size() - i));
-      }
-      for (auto &pr : pairs) {
-        fp.m2l_tgt.push_back(pr.first);
-        fp.m2l_src.push_back(pr.second);
-      }
-      fp.m2l_per_level[l] += (int64_t)pairs.size();
+      add_class(fp.m2l, op, kv.second);
+      fp.m2l_per_level[l] += (int64_t)kv.second.size();
     }
   }
+  fp.m2l.level_off[L + 1] = (int)fp.m2l.op.size();
 }
 
 }  // namespace pbfmm
