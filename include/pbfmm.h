@@ -50,6 +50,7 @@ typedef struct pbfmm_params {
   int32_t precision;          /* PBFMM_FP64 or PBFMM_MIXED (solve_system) */
   int32_t device;             /* HIP device ordinal */
   void *stream;               /* hipStream_t, NULL = library-owned stream */
+  int32_t preconditioner;     /* 0 none, 1 near-field block Jacobi (default) */
 } pbfmm_params;
 
 /* Defaults of bem_fma.cc:64-85 and bem_problem.cc:268-269. */
@@ -113,6 +114,16 @@ int pbfmm_compute_alpha(pbfmm_handle *h, double *alpha);
  * (distribute_rhs, constrained_matrix.h:137-147).  NULL pointers clear them. */
 int pbfmm_set_constraints(pbfmm_handle *h, const int64_t *c_ptr, const int64_t *c_idx,
                           const double *c_val, const double *c_inhom, const uint8_t *is_constrained);
+
+/* BEMFMA<3>::FMA_preconditioner (bem_fma.h:201-202, bem_fma.cc:3078-3411):
+ * the near-field matrix with -D on Dirichlet columns and N on Neumann columns
+ * (init_preconditioner, bem_fma.cc:919-975), alpha added on the diagonal of
+ * unconstrained Neumann rows, constrained rows replaced by their constraint
+ * (with the operator's sign, -c_ij).  The reference factors it with ILU(0);
+ * here every leaf's own rows form a dense block that is inverted (block
+ * Jacobi), applied as right preconditioner by the GMRES of solve_system and
+ * gmres_dev.  alpha NULL uses the handle's alpha (computed if needed). */
+int pbfmm_fma_preconditioner(pbfmm_handle *h, const double *alpha);
 
 /* ConstrainedOperator::vmult over BEMProblem<3>::vmult, FMA branch
  * (constrained_matrix.h:91-135, bem_problem.cc:1164-1217):

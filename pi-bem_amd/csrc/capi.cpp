@@ -66,7 +66,13 @@ void ensure_alpha(pbfmm_handle *h) {
   if (!h->alpha_ready) {
     h->eng.compute_alpha();
     h->alpha_ready = true;
+    h->eng.have_prec = false;
   }
+}
+// FMA_preconditioner as called from solve_system (bem_problem.cc:1302-1304)
+void ensure_prec(pbfmm_handle *h) {
+  ensure_alpha(h);
+  if (h->prm.preconditioner == 1 && !h->eng.have_prec) h->eng.build_preconditioner();
 }
 }  // namespace
 
@@ -82,6 +88,7 @@ void pbfmm_default_params(pbfmm_params *p) {
   p->precision = PBFMM_FP64;
   p->device = 0;
   p->stream = nullptr;
+  p->preconditioner = 1;
 }
 
 int pbfmm_create(const pbfmm_params *p, pbfmm_handle **out) {
@@ -89,7 +96,8 @@ int pbfmm_create(const pbfmm_params *p, pbfmm_handle **out) {
   *out = nullptr;
   if (p->trunc_order < 1 || p->trunc_order > 10 || p->num_octree_levels < 1 || p->num_octree_levels > 20 ||
       p->max_nodes_per_leaf < 1 || p->quad_order < 1 || p->quad_order > 8 || p->sing_quad_order < 1 ||
-      p->sing_quad_order > 16 || (p->precision != PBFMM_FP64 && p->precision != PBFMM_MIXED))
+      p->sing_quad_order > 16 || (p->precision != PBFMM_FP64 && p->precision != PBFMM_MIXED) ||
+      p->preconditioner < 0 || p->preconditioner > 1)
     return PBFMM_ERR_ARG;
   pbfmm_handle *h = nullptr;
   try {
@@ -282,6 +290,7 @@ int pbfmm_set_constraints(pbfmm_handle *h, const int64_t *c_ptr, const int64_t *
     Engine &e = h->eng;
     if (!c_ptr || !is_c) {
       e.have_constraints = false;
+      e.have_prec = false;
       return PBFMM_OK;
     }
     const int64_t n = e.n;
@@ -309,7 +318,26 @@ int pbfmm_set_constraints(pbfmm_handle *h, const int64_t *c_ptr, const int64_t *
     up(e.d_c_val, val.data(), val.size() * 8);
     up(e.d_c_inh, inh.data(), inh.size() * 8);
     up(e.d_c_is, isc.data(), isc.size());
+    e.hc_ptr = ptr;
+    e.hc_idx = idx;
+    e.hc_val = val;
+    e.hc_is = isc;
     e.have_constraints = true;
+    e.have_prec = false;
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_fma_preconditioner(pbfmm_handle *h, const double *alpha) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "fma_preconditioner")) return rc;
+    Engine &e = h->eng;
+    if (alpha) {
+      h2d(e, e.d_alpha.as<double>(), alpha, e.n);
+      h->alpha_ready = true;
+    }
+    ensure_alpha(h);
+    e.build_preconditioner();
     return PBFMM_OK;
   });
 }
@@ -347,7 +375,7 @@ int pbfmm_solve_system(pbfmm_handle *h, const double *tmp_rhs, double *phi, doub
     if (!tmp_rhs || !phi || !dphi) return fail(h, PBFMM_ERR_ARG, "solve_system: null vector");
     if (restart < 1 || restart > 127 || max_it < 1 || !(rel_tol > 0))
       return fail(h, PBFMM_ERR_ARG, "solve_system: need 1 <= restart <= 127, max_it >= 1, rel_tol > 0");
-    ensure_alpha(h);
+    ensure_prec(h);
     Engine &e = h->eng;
     const int64_t n = e.n;
     // rhs (compute_rhs) and distribute_rhs of the constrained rows
@@ -410,7 +438,7 @@ int pbfmm_gmres_dev(pbfmm_handle *h, const double *rhs, double *x, double rel_to
       return fail(h, PBFMM_ERR_ARG, "gmres_dev: need 1 <= restart <= 127, max_it >= 1, rel_tol > 0");
     if (precision == PBFMM_MIXED && !h->eng.have_f32)
       return fail(h, PBFMM_ERR_STATE, "gmres_dev: handle was not created with PBFMM_MIXED");
-    ensure_alpha(h);
+    ensure_prec(h);
     int it = 0;
     double rr = 0;
     h->eng.gmres(rhs, x, rel_tol, restart, max_it, precision, &it, &rr);

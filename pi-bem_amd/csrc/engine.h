@@ -74,6 +74,16 @@ struct Engine {
   DevBuf d_alpha, d_sp, d_sd, d_outN, d_outD, d_in, d_out, d_red;
   DevBuf d_c_ptr, d_c_idx, d_c_val, d_c_inh, d_c_is;
   bool have_constraints = false;
+  std::vector<int32_t> hc_ptr, hc_idx;  // host copies of the constraint rows
+  std::vector<double> hc_val;
+  std::vector<uint8_t> hc_is;
+  // near-field block-Jacobi preconditioner: per leaf the inverse of the
+  // rows x rows block, row-major, at pinv_beg[leaf]
+  bool have_prec = false;
+  std::vector<int64_t> pinv_beg;
+  DevBuf d_pinv_beg, d_pinv, d_ptmp, d_ptmp2;
+  void build_preconditioner();
+  void apply_prec(const double *v, double *z);
   // GMRES workspace
   DevBuf d_V, d_w, d_r, d_x, d_h;
   int gm_restart = 0;

@@ -403,62 +403,53 @@ template <> struct Mfma<double> {
 
 static_assert(kM2LChunk == 32, "k_translate maps 2 x 32 pairs onto 64 GEMM columns");
 
-// LDS layout (elements of T): B tile [KP][CS] (k-major, 64 columns padded to
-// CS; reused for C as [64][KP + 1] after the GEMM), then A [KP][AS]
-// (column-major as stored, padded) when staged.
+// LDS tile (elements of T): B as [64 columns][KP + BP] (a source vector per
+// column, padded so that both the coalesced fill and the MFMA reads are free
+// of bank conflicts), reused after the GEMM for C as [64][KP + 1].
 template <int P, typename T>
 struct TrLayout {
-  static_assert((64 + 16 / (sizeof(T) / 4)) * ((((P + 1) * (P + 2) + 15) / 16) * 16) >=
-                    64 * (((((P + 1) * (P + 2) + 15) / 16) * 16) + 1),
-                "C tile must fit in the B tile");
   static constexpr int KP = Kp<P>::value;
-  static constexpr int CS = 64 + 16 / (int)(sizeof(T) / 4);  // row stride of the B / C tile
-  static constexpr int AS = KP + 16 / (int)(sizeof(T) / 4);  // column stride of A
-  static constexpr size_t b_bytes = (size_t)KP * CS * sizeof(T);
-  static constexpr size_t a_bytes = (size_t)KP * AS * sizeof(T);
+  static constexpr int BP = sizeof(T) == 4 ? 4 : 2;
+  static constexpr int BS = KP + BP;
+  static constexpr size_t bytes = (size_t)64 * BS * sizeof(T);
 };
 
-template <typename T, int P, bool A_LDS>
+template <typename T, int P>
 __global__ void __launch_bounds__(256) k_translate(const int *__restrict__ chunk_op,
                                                    const int *__restrict__ chunk_beg,
                                                    const int *__restrict__ chunk_cnt, const int *__restrict__ tgt,
                                                    const int *__restrict__ src, const T *__restrict__ ops,
                                                    const T *__restrict__ IN, T *__restrict__ OUT) {
   using Lay = TrLayout<P, T>;
-  constexpr int KP = Kp<P>::value, K2 = 2 * KP, KT2 = 2 * Kp<P>::kt, NR = KP / 16;
-  constexpr int CS = Lay::CS, AS = Lay::AS;
+  constexpr int KP = Kp<P>::value, K2 = 2 * KP, KT2 = 2 * Kp<P>::kt, NR = KP / 16, BS = Lay::BS;
   using M = Mfma<T>;
-  extern __shared__ unsigned char smem_raw[];
-  T *sB = reinterpret_cast<T *>(smem_raw);  // [KP][CS]
-  T *sA = sB + KP * CS;                     // [KP][AS]
-  __shared__ int sT[kM2LChunk];
+  __shared__ T sB[64 * BS];
+  __shared__ int sT[kM2LChunk], sS[kM2LChunk];
   const int c = blockIdx.x;
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
-  const T *opg = ops + (int64_t)chunk_op[c] * KP * KP;
+  const T *A = ops + (int64_t)chunk_op[c] * KP * KP;  // column-major, read from L2
   const int beg = chunk_beg[c], cnt = chunk_cnt[c];
-  if (A_LDS)
-    for (int i = t; i < KP * KP; i += 256) {
-      const int k = i / KP, r = i - k * KP;
-      sA[k * AS + r] = opg[i];
-    }
+  if (t < kM2LChunk) {
+    sT[t] = t < cnt ? tgt[beg + t] : -1;
+    sS[t] = t < cnt ? src[beg + t] : -1;
+  }
+  __syncthreads();
   for (int i = t; i < 64 * KP; i += 256) {
     const int col = i / KP, k = i - col * KP, pr = col >> 1, ker = col & 1;
-    sB[k * CS + col] = pr < cnt ? IN[(int64_t)src[beg + pr] * K2 + ker * KP + k] : T(0);
+    sB[col * BS + k] = pr < cnt ? IN[(int64_t)sS[pr] * K2 + ker * KP + k] : T(0);
   }
-  if (t < kM2LChunk) sT[t] = t < cnt ? tgt[beg + t] : -1;
   __syncthreads();
   typename M::acc acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = typename M::acc{0, 0, 0, 0};
   const int col = 16 * w + (l & 15);
-  const T *bk = sB + (l >> 4) * CS + col;
-  const T *ak = A_LDS ? (sA + (l >> 4) * AS + (l & 15)) : (opg + (l >> 4) * KP + (l & 15));
-  const int astep = A_LDS ? AS : KP;
+  const T *bk = sB + col * BS + (l >> 4);
+  const T *ak = A + (l >> 4) * KP + (l & 15);
 #pragma unroll 4
   for (int k0 = 0; k0 < KP; k0 += 4) {
-    const T b = bk[k0 * CS];
+    const T b = bk[k0];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = M::mac(ak[k0 * astep + 16 * r], b, acc[r]);
+    for (int r = 0; r < NR; ++r) acc[r] = M::mac(ak[k0 * KP + 16 * r], b, acc[r]);
   }
   __syncthreads();  // B tile no longer needed: reuse it for C, column-major [64][KP + 1]
   T *sC = sB;
@@ -622,6 +613,56 @@ __global__ void k_set_constrained(int64_t n, const uint8_t *__restrict__ is_c, c
                                   double *__restrict__ rhs) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n && is_c[i]) rhs[i] = inh[i];
+}
+
+// ---------------------------------------------------------------------------
+// Near-field block-Jacobi preconditioner (FMA_preconditioner,
+// bem_fma.cc:3078-3411, restricted to each leaf's own rows): extraction of the
+// rows x rows blocks of the preconditioner matrix and their application.
+// ---------------------------------------------------------------------------
+__global__ void k_extract_blocks(const int *__restrict__ row_beg, const int *__restrict__ rows,
+                                 const int *__restrict__ col_beg, const int *__restrict__ cols,
+                                 const int64_t *__restrict__ val_beg, const double *__restrict__ vals,
+                                 const uint8_t *__restrict__ dir, const double *__restrict__ alpha,
+                                 const int64_t *__restrict__ blk_beg, double *__restrict__ out) {
+  const int leaf = blockIdx.x;
+  const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0;
+  const int cb = col_beg[leaf], C = col_beg[leaf + 1] - cb;
+  for (int idx = threadIdx.x; idx < R * R; idx += blockDim.x) {
+    const int r = idx / R, c = idx - r * R;
+    const int node_r = rows[r0 + r], node_c = rows[r0 + c];
+    int lo = 0, hi = C - 1, pos = -1;
+    while (lo <= hi) {
+      const int mid = (lo + hi) >> 1;
+      const int v = cols[cb + mid];
+      if (v == node_c) { pos = mid; break; }
+      if (v < node_c) lo = mid + 1;
+      else hi = mid - 1;
+    }
+    double v = 0.0;
+    if (pos >= 0) {
+      const double *pr = vals + 2 * (val_beg[leaf] + (int64_t)r * C + pos);
+      v = dir[node_c] ? -pr[1] : pr[0];
+    }
+    if (r == c && !dir[node_r]) v += alpha[node_r];
+    out[blk_beg[leaf] + idx] = v;
+  }
+}
+
+// z[rows of leaf] = Pinv_leaf . v[rows of leaf]; one thread per leaf-ordered row
+__global__ void __launch_bounds__(256) k_bjacobi(int nrows, const int *__restrict__ row_leaf,
+                                                 const int *__restrict__ row_beg, const int *__restrict__ rows,
+                                                 const int64_t *__restrict__ blk_beg,
+                                                 const double *__restrict__ pinv, const double *__restrict__ v,
+                                                 double *__restrict__ z) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nrows) return;
+  const int leaf = row_leaf[g];
+  const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0, r = g - r0;
+  const double *P = pinv + blk_beg[leaf] + (int64_t)r * R;
+  double acc = 0.0;
+  for (int c = 0; c < R; ++c) acc += P[c] * v[rows[r0 + c]];
+  z[rows[g]] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -926,22 +967,16 @@ FarDev<float> &far_of<float>(Engine &e) { return e.f32; }
 
 // OUT[tgt] += ops[class] . IN[src] for the chunks [c0, c1) of a chunk list.
 template <typename T>
-static void launch_translate(Engine &e, const ChunkDev &cl, int c0, int c1, const T *ops, const T *IN, T *OUT) {
+static void launch_translate(Engine &e, hipStream_t st, const ChunkDev &cl, int c0, int c1, const T *ops,
+                             const T *IN, T *OUT) {
   const int n = c1 - c0;
   if (n <= 0) return;
-#define PBFMM_TR(PP)                                                                                          \
-  case PP: {                                                                                                  \
-    using Lay = TrLayout<PP, T>;                                                                              \
-    if (Lay::a_bytes + Lay::b_bytes <= 64 * 1024)                                                             \
-      hipLaunchKernelGGL((k_translate<T, PP, true>), dim3(n), dim3(256), Lay::a_bytes + Lay::b_bytes, e.stream, \
-                         cl.op.as<int>() + c0, cl.beg.as<int>() + c0, cl.cnt.as<int>() + c0, cl.tgt.as<int>(),   \
-                         cl.src.as<int>(), ops, IN, OUT);                                                     \
-    else                                                                                                      \
-      hipLaunchKernelGGL((k_translate<T, PP, false>), dim3(n), dim3(256), Lay::b_bytes, e.stream,               \
-                         cl.op.as<int>() + c0, cl.beg.as<int>() + c0, cl.cnt.as<int>() + c0, cl.tgt.as<int>(),   \
-                         cl.src.as<int>(), ops, IN, OUT);                                                     \
-    break;                                                                                                    \
-  }
+#define PBFMM_TR(PP)                                                                                         \
+  case PP:                                                                                                   \
+    hipLaunchKernelGGL((k_translate<T, PP>), dim3(n), dim3(256), 0, st, cl.op.as<int>() + c0,               \
+                       cl.beg.as<int>() + c0, cl.cnt.as<int>() + c0, cl.tgt.as<int>(), cl.src.as<int>(), ops,  \
+                       IN, OUT);                                                                             \
+    break;
   switch (e.p) {
     PBFMM_TR(1) PBFMM_TR(2) PBFMM_TR(3) PBFMM_TR(4) PBFMM_TR(5)
     PBFMM_TR(6) PBFMM_TR(7) PBFMM_TR(8) PBFMM_TR(9) PBFMM_TR(10)
@@ -975,8 +1010,8 @@ void Engine::generate(const double *phi, const double *dphi) {
   }
   mark(1);
   for (int l = L; l >= 1; --l)
-    launch_translate<T>(*this, m2m_cl, far.m2m.level_off[l], far.m2m.level_off[l + 1], F.m2m.template as<T>(),
-                        F.me.template as<T>(), F.me.template as<T>());
+    launch_translate<T>(*this, stream, m2m_cl, far.m2m.level_off[l], far.m2m.level_off[l + 1],
+                        F.m2m.template as<T>(), F.me.template as<T>(), F.me.template as<T>());
   mark(2);
   HIPCHK(hipGetLastError());
 }
@@ -995,8 +1030,8 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
   hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(nle, 256), 4096)), dim3(256), 0, stream,
                      F.le.template as<T>(), nle, T(0));
   mark(5);
-  launch_translate<T>(*this, m2l_cl, 0, (int)far.m2l.op.size(), F.m2l.template as<T>(), F.me.template as<T>(),
-                      F.le.template as<T>());
+  launch_translate<T>(*this, stream, m2l_cl, 0, (int)far.m2l.op.size(), F.m2l.template as<T>(),
+                      F.me.template as<T>(), F.le.template as<T>());
   mark(6);
   const int npairs = (int)far.m2p_tgt.size();
   if (npairs > 0) {
@@ -1009,8 +1044,8 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
   }
   mark(7);
   for (int l = 2; l <= L; ++l)
-    launch_translate<T>(*this, l2l_cl, far.l2l.level_off[l], far.l2l.level_off[l + 1], F.l2l.template as<T>(),
-                        F.le.template as<T>(), F.le.template as<T>());
+    launch_translate<T>(*this, stream, l2l_cl, far.l2l.level_off[l], far.l2l.level_off[l + 1],
+                        F.l2l.template as<T>(), F.le.template as<T>(), F.le.template as<T>());
   mark(8);
   if (nrows > 0)
     hipLaunchKernelGGL(k_l2p<T>, dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
@@ -1097,6 +1132,101 @@ void Engine::compute_rhs(const double *gvec, double *rhs) {
   HIPCHK(hipGetLastError());
 }
 
+// Gauss-Jordan inverse with partial pivoting of a row-major R x R block in
+// place; returns false when a pivot vanishes.
+static bool invert_block(std::vector<double> &A, int R) {
+  std::vector<double> I((size_t)R * R, 0.0);
+  for (int i = 0; i < R; ++i) I[(size_t)i * R + i] = 1.0;
+  double scale = 0;
+  for (double x : A) scale = std::max(scale, std::fabs(x));
+  for (int c = 0; c < R; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < R; ++r)
+      if (std::fabs(A[(size_t)r * R + c]) > std::fabs(A[(size_t)piv * R + c])) piv = r;
+    if (!(std::fabs(A[(size_t)piv * R + c]) > 1e-13 * scale)) return false;
+    if (piv != c)
+      for (int k = 0; k < R; ++k) {
+        std::swap(A[(size_t)piv * R + k], A[(size_t)c * R + k]);
+        std::swap(I[(size_t)piv * R + k], I[(size_t)c * R + k]);
+      }
+    const double d = 1.0 / A[(size_t)c * R + c];
+    for (int k = 0; k < R; ++k) {
+      A[(size_t)c * R + k] *= d;
+      I[(size_t)c * R + k] *= d;
+    }
+    for (int r = 0; r < R; ++r) {
+      if (r == c) continue;
+      const double f = A[(size_t)r * R + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < R; ++k) {
+        A[(size_t)r * R + k] -= f * A[(size_t)c * R + k];
+        I[(size_t)r * R + k] -= f * I[(size_t)c * R + k];
+      }
+    }
+  }
+  A.swap(I);
+  return true;
+}
+
+void Engine::build_preconditioner() {
+  const int nl = near.nleaf;
+  pinv_beg.assign(nl + 1, 0);
+  for (int l = 0; l < nl; ++l) {
+    const int64_t R = near.row_beg[l + 1] - near.row_beg[l];
+    pinv_beg[l + 1] = pinv_beg[l] + R * R;
+  }
+  up(d_pinv_beg, pinv_beg);
+  alloc(d_pinv, (size_t)std::max<int64_t>(pinv_beg[nl], 1) * 8);
+  if (nl > 0)
+    hipLaunchKernelGGL(k_extract_blocks, dim3(nl), dim3(256), 0, stream, d_row_beg.as<int>(), d_rows.as<int>(),
+                       d_col_beg.as<int>(), d_cols.as<int>(), d_val_beg.as<int64_t>(), f64.vals.as<double>(),
+                       d_dir.as<uint8_t>(), d_alpha.as<double>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>());
+  HIPCHK(hipGetLastError());
+  std::vector<double> blk(pinv_beg[nl]);
+  if (!blk.empty())
+    HIPCHK(hipMemcpyAsync(blk.data(), d_pinv.p, blk.size() * 8, hipMemcpyDeviceToHost, stream));
+  sync();
+  std::vector<int32_t> local(n, -1);
+  std::vector<double> A;
+  for (int l = 0; l < nl; ++l) {
+    const int r0 = near.row_beg[l], R = near.row_beg[l + 1] - r0;
+    for (int r = 0; r < R; ++r) local[near.rows[r0 + r]] = r;
+    A.assign(blk.begin() + pinv_beg[l], blk.begin() + pinv_beg[l + 1]);
+    if (have_constraints)
+      for (int r = 0; r < R; ++r) {
+        const int i = near.rows[r0 + r];
+        if (!hc_is[i]) continue;
+        // constrained row: dst(i) = src(i) - sum c_ij src(j) (constrained_matrix.h:109-121)
+        for (int c = 0; c < R; ++c) A[(size_t)r * R + c] = (c == r) ? 1.0 : 0.0;
+        for (int k = hc_ptr[i]; k < hc_ptr[i + 1]; ++k) {
+          const int j = hc_idx[k];
+          if (j >= 0 && local[j] >= 0 && near.rows[r0 + local[j]] == j) A[(size_t)r * R + local[j]] -= hc_val[k];
+        }
+      }
+    std::vector<double> keep = A;
+    if (!invert_block(A, R)) {
+      // singular block (e.g. unconstrained double nodes): diagonal scaling
+      for (int r = 0; r < R; ++r)
+        for (int c = 0; c < R; ++c) {
+          const double d = keep[(size_t)r * R + r];
+          A[(size_t)r * R + c] = (r == c) ? (d != 0.0 ? 1.0 / d : 1.0) : 0.0;
+        }
+    }
+    std::copy(A.begin(), A.end(), blk.begin() + pinv_beg[l]);
+    for (int r = 0; r < R; ++r) local[near.rows[r0 + r]] = -1;
+  }
+  if (!blk.empty()) HIPCHK(hipMemcpy(d_pinv.p, blk.data(), blk.size() * 8, hipMemcpyHostToDevice));
+  alloc(d_ptmp, n * 8);
+  alloc(d_ptmp2, n * 8);
+  have_prec = true;
+}
+
+void Engine::apply_prec(const double *v, double *z) {
+  const int nrows = (int)near.rows.size();
+  hipLaunchKernelGGL(k_bjacobi, dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_row_leaf.as<int>(),
+                     d_row_beg.as<int>(), d_rows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z);
+}
+
 // Restarted GMRES with classical Gram-Schmidt and one reorthogonalisation
 // (CGS2), FP64 basis and Hessenberg, operator precision as requested.  Solves
 // A x = b from the given x until the Arnoldi residual estimate reaches
@@ -1128,7 +1258,12 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
     bool done = false;
     for (; k < m && it < maxit; ++k) {
       double *vk1 = V + (int64_t)(k + 1) * n;
-      e.vmult(V + (int64_t)k * n, vk1, precision);
+      if (e.have_prec) {
+        e.apply_prec(V + (int64_t)k * n, e.d_ptmp.as<double>());
+        e.vmult(e.d_ptmp.as<double>(), vk1, precision);
+      } else {
+        e.vmult(V + (int64_t)k * n, vk1, precision);
+      }
       ++it;
       // CGS2: two projections and the norm on the device, one host round trip
       double *h1 = hd, *h2 = hd + (m + 2), *hn_d = hd + 2 * (m + 2);
@@ -1170,7 +1305,16 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
       y[i] = s / H[(size_t)i * m + i];
     }
     HIPCHK(hipMemcpyAsync(hd, y.data(), k * 8, hipMemcpyHostToDevice, e.stream));
-    hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, V, hd, x);
+    if (e.have_prec) {
+      // x += M^-1 (V y)
+      double *t = e.d_ptmp2.as<double>(), *z = e.d_ptmp.as<double>();
+      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, e.stream, t, n, 0.0);
+      hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, V, hd, t);
+      e.apply_prec(t, z);
+      hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, z, 1.0, x);
+    } else {
+      hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, V, hd, x);
+    }
     e.sync();
     if (done) return 0;
   }

@@ -44,7 +44,7 @@ class BEMFMA:
     trunc_order=6; quadrature orders as handed to init_fma (4, 5)."""
 
     def __init__(self, trunc_order=6, num_octree_levels=10, max_nodes_per_leaf=20, quad_order=4,
-                 sing_quad_order=5, precision=PBFMM_FP64, device=0, stream=None):
+                 sing_quad_order=5, precision=PBFMM_FP64, device=0, stream=None, preconditioner=1):
         self._L = load()
         prm = Params()
         self._L.pbfmm_default_params(C.byref(prm))
@@ -56,6 +56,7 @@ class BEMFMA:
         prm.precision = precision
         prm.device = device
         prm.stream = stream
+        prm.preconditioner = preconditioner
         self.params = prm
         h = C.c_void_p()
         rc = self._L.pbfmm_create(C.byref(prm), C.byref(h))
@@ -208,6 +209,11 @@ class BEMProblem:
         self.fma._chk(self.fma._L.pbfmm_set_constraints(self._h, _p(a, _i64), _p(b, _i64), _p(c), _p(d),
                                                         _p(e, _u8)))
 
+    def fma_preconditioner(self, alpha=None):
+        """BEMFMA::FMA_preconditioner (bem_fma.cc:3078-3411) as leaf block Jacobi."""
+        a = None if alpha is None else _f64(alpha)
+        self.fma._chk(self.fma._L.pbfmm_fma_preconditioner(self._h, _p(a)))
+
     def vmult(self, src):
         dst = np.zeros(self.fma.n)
         self.fma._chk(self.fma._L.pbfmm_vmult(self._h, _p(_f64(src)), _p(dst)))
@@ -227,12 +233,13 @@ class BEMProblem:
         return phi, dphi, it.value, rr.value
 
 
-def setup(problem, precision=PBFMM_FP64, device=0):
+def setup(problem, precision=PBFMM_FP64, device=0, preconditioner=1):
     """The set-up sequence of BEMProblem::solve (bem_problem.cc:1363-1371,
     upstream wiring): init_fma, octree, direct and multipole integrals."""
     f = BEMFMA(trunc_order=problem.p, num_octree_levels=problem.levels,
                max_nodes_per_leaf=problem.max_nodes_per_leaf, quad_order=problem.quad_order,
-               sing_quad_order=problem.sing_quad_order, precision=precision, device=device)
+               sing_quad_order=problem.sing_quad_order, precision=precision, device=device,
+               preconditioner=preconditioner)
     f.init_fma(problem.mesh, problem.dirichlet)
     f.generate_octree_blocking()
     f.direct_integrals()

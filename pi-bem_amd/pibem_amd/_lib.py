@@ -24,7 +24,7 @@ _h = C.c_void_p
 class Params(C.Structure):
     _fields_ = [("num_octree_levels", C.c_int32), ("max_nodes_per_leaf", C.c_int32), ("trunc_order", C.c_int32),
                 ("quad_order", C.c_int32), ("sing_quad_order", C.c_int32), ("precision", C.c_int32),
-                ("device", C.c_int32), ("stream", C.c_void_p)]
+                ("device", C.c_int32), ("stream", C.c_void_p), ("preconditioner", C.c_int32)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/pbfmm.h
@@ -42,6 +42,7 @@ SIGNATURES = {
     "pbfmm_multipole_matr_vect_products": (C.c_int, [_h, _d, _d, _d, _d]),
     "pbfmm_compute_alpha": (C.c_int, [_h, _d]),
     "pbfmm_set_constraints": (C.c_int, [_h, _i64, _i64, _d, _d, _u8]),
+    "pbfmm_fma_preconditioner": (C.c_int, [_h, _d]),
     "pbfmm_vmult": (C.c_int, [_h, _d, _d]),
     "pbfmm_compute_rhs": (C.c_int, [_h, _d, _d]),
     "pbfmm_solve_system": (C.c_int, [_h, _d, _d, _d, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int32),

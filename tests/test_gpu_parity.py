@@ -127,6 +127,15 @@ def test_solve_system(c1, precision):
     assert rel(phi[neu], c1.pr.exact.phi(x)[neu]) < 1e-3
 
 
+def test_solve_without_preconditioner(c1):
+    f = P.setup(c1.pr, P.PBFMM_FP64, preconditioner=0)
+    phi, dphi, it0, rr = P.BEMProblem(f).solve_system(c1.pr.tmp_rhs, tol=1e-8)
+    assert rr <= 1e-8
+    _, _, it1, _ = P.BEMProblem(c1.f64).solve_system(c1.pr.tmp_rhs, tol=1e-8)
+    print(f"  iterations: no preconditioner {it0}, block Jacobi {it1}")
+    assert it1 < it0
+
+
 def test_constraints_double_nodes(dcube):
     cons = M.constraints(dcube.pr)
     bp = P.BEMProblem(dcube.f64)
