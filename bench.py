@@ -8,11 +8,17 @@ of phi = 1/|x - (2,1,1)|) with the mixed-precision path: FP32 operator
 refinement with an FP64 Arnoldi process and FP64 residuals.  The FP64 GPU path
 is timed beside it, as are FMM matvecs per second, per-phase device times from
 HIP events on the library's stream, and the CPU oracle (the reference's
-algorithm restated in C++, OpenMP) on the host cores.
+algorithm restated in C++, OpenMP) on the host cores.  Configuration 3
+(prolate spheroid (4,1,1), 6 x 128 x 128 Q2 cells, 393,218 DoFs, order 8) is
+measured the same way and reported in the "config3" object.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-config3]
+    python bench.py --mode matvec --config {2,3} --precision {fp32,fp64} --steps K
+    python bench.py --mode copy --mib M --steps K
 
-Prints one JSON line.
+The default mode prints one JSON line.  "matvec" only runs K FMM matvecs (a
+short program to run under rocprofv3); "copy" runs K device-to-device copies
+of M MiB, the calibration for the HBM byte counters.
 """
 from __future__ import annotations
 
@@ -29,56 +35,87 @@ for _p in (os.path.join(ROOT, "pi-bem_amd"), os.path.join(ROOT, "oracle")):
 
 import numpy as np  # noqa: E402
 
-METRIC = json.load(open(os.path.join(ROOT, "BASELINE.json")))["metric"] if os.path.exists(
-    os.path.join(ROOT, "BASELINE.json")) else "GMRES time to rel-res 1e-8, mixed vs FP64 GPU; FMM matvecs/s; SpMV HBM GB/s"
+METRIC = "GMRES time to rel-res 1e-8, mixed vs FP64 GPU; FMM matvecs/s; SpMV HBM GB/s"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFS = 157.3
 FP64_PEAK_TFS = 78.6
+PHASES = ("spmv", "p2m", "m2m", "l2l", "m2l", "m2p", "l2p")
+# kernel names as they appear in traces, per phase and operator precision
+KERNEL = {
+    "spmv": "k_spmv<{t}>", "p2m": "k_p2m<{t}>", "m2m": "k_translate<{t}, {p}, 0>",
+    "m2l": "k_translate<{t}, {p}, 1>", "l2l": "k_translate<{t}, {p}, 2>", "m2p": "k_m2p<{t}>", "l2p": "k_l2p<{t}>",
+}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def spmv_bytes(wc, s_v):
-    """Algorithmic bytes of one fused N/D SpMV on the leaf-blocked layout:
-    (N, D) value pairs, one column index per leaf column, the row's node id,
-    the gathered phi / dphi entries once each and the two outputs."""
-    n = wc["n_dofs"]
-    return wc["near_values"] * 2 * s_v + wc["near_cols"] * 4 + wc["near_rows"] * 4 + 2 * n * 8 + 2 * n * 8
+def work_model(wc, s_v):
+    """Algorithmic bytes and flops of one launch of each phase (s_v: bytes of
+    an operator value, 4 for the FP32 operator, 8 for FP64).
 
-
-def m2l_flops(wc):
-    """8 Kt^2 flops per (target, source, kernel): a complex Kt x Kt map; two kernels."""
-    p = wc["trunc_order"]
+    spmv: the (N, D) value pairs of the leaf blocks, one column index per leaf
+          column, one row id per row, phi and dphi read once, both outputs
+          written once.
+    p2m:  the unit-multipole blocks (2 kernels x kp values per leaf column), one
+          DoF index per column, phi and dphi once, the leaf expansions written.
+    m2m, l2l, m2l: 8 Kt^2 flops per (pair, kernel): a complex Kt x Kt map.
+    m2p, l2p: ~(Kt (8 + 8) + 40) flops per (point, expansion pair) evaluation.
+    """
+    n, p, kp = wc["n_dofs"], wc["trunc_order"], wc["kp"]
     kt = (p + 1) * (p + 2) // 2
-    return wc["m2l_pairs"] * 2 * 8 * kt * kt
+    out = {
+        "spmv": dict(bound="hbm", bytes=wc["near_values"] * 2 * s_v + wc["near_cols"] * 4 + wc["near_rows"] * 4
+                     + 2 * n * 8 + 2 * n * 8),
+        "p2m": dict(bound="hbm", bytes=wc["p2m_cols"] * (2 * kp * s_v + 4) + 2 * n * 8
+                    + wc["leaves"] * 2 * kp * s_v),
+    }
+    for k, pairs in (("m2m", wc["m2m_translations"]), ("l2l", wc["l2l_translations"]), ("m2l", wc["m2l_pairs"])):
+        out[k] = dict(bound="mfma", flops=pairs * 2 * 8 * kt * kt)
+    out["m2p"] = dict(bound="valu", flops=wc["m2p_evals"] * (16 * kt + 40))
+    out["l2p"] = dict(bound="valu", flops=wc["near_rows"] * (16 * kt + 40))
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build runs on a single "
-              f"MI355X; see DESIGN.md for how the path would shard)", file=sys.stderr)
-        sys.exit(2)
+def roofline_of(phase, model, ms, prec, p):
+    t = "float" if prec == "fp32" else "double"
+    m = model[phase]
+    if m["bound"] == "hbm":
+        ach = m["bytes"] / (ms * 1e-3) / 1e9
+        return dict(kernel=KERNEL[phase].format(t=t, p=p), bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
+                     unit="GB/s", frac=ach / HBM_PEAK_GBS, algorithmic_per_launch=m["bytes"], launch_ms=ms)
+    peak = FP32_PEAK_TFS if prec == "fp32" else FP64_PEAK_TFS
+    ach = m["flops"] / (ms * 1e-3) / 1e12
+    return dict(kernel=KERNEL[phase].format(t=t, p=p), bound="mfma" if m["bound"] == "mfma" else "valu",
+                achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak, algorithmic_per_launch=m["flops"],
+                launch_ms=ms)
 
-    import pibem_amd as P
-    from pibem_amd import hip
+
+def traffic_for(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 counter summary of
+    this kernel and workload (profiles/traffic_*.json), or None."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
+        return None
+    for f in sorted(os.listdir(d)):
+        if f.startswith("traffic_") and f.endswith(".json"):
+            try:
+                t = json.load(open(os.path.join(d, f)))
+            except (OSError, ValueError):
+                continue
+            if t.get("kernel") == kernel and t.get("workload") == workload:
+                return t.get("traffic_bytes")
+    return None
+
+
+def make_problem(config):
     from pibem_amd import mesh as M
+    return M.config_sphere(64, 6) if config == 2 else M.config_spheroid(128, 8)
 
-    if hip.device_count() < 1:
-        print("bench.py needs a HIP device", file=sys.stderr)
-        sys.exit(1)
-    pr = M.config_sphere(64, 6)
-    n = pr.mesh.n_dofs
-    log(f"config 2: N={n} cells={pr.mesh.n_cells} p={pr.p}")
-    stream = hip.Stream()
+
+def setup_handle(pr, stream):
+    import pibem_amd as P
     t0 = time.perf_counter()
     f = P.BEMFMA(trunc_order=pr.p, num_octree_levels=pr.levels, max_nodes_per_leaf=pr.max_nodes_per_leaf,
                  quad_order=pr.quad_order, sing_quad_order=pr.sing_quad_order, precision=P.PBFMM_MIXED,
@@ -92,123 +129,194 @@ def main():
     bp = P.BEMProblem(f)
     bp.compute_alpha()
     t3 = time.perf_counter()
-    log(f"setup: octree {t1 - t0:.2f}s direct {t2 - t1:.2f}s multipole+alpha {t3 - t2:.2f}s")
-    wc = f.work_counts()
-    rhs_h = bp.compute_rhs(pr.tmp_rhs)
-    rhs = hip.DeviceArray.from_numpy(rhs_h)
+    return f, bp, dict(octree_s=t1 - t0, direct_s=t2 - t1, multipole_alpha_s=t3 - t2)
+
+
+def time_solves(f, bp, pr, stream, steps, warmup):
+    import pibem_amd as P
+    from pibem_amd import hip
+    n = pr.mesh.n_dofs
+    rhs = hip.DeviceArray.from_numpy(bp.compute_rhs(pr.tmp_rhs))
     x = hip.DeviceArray(n)
     zero = hip.DeviceArray(n)
-
-    def solve(prec):
-        x.copy_from(zero)
-        return f.gmres_dev(rhs.value, x.value, 1e-8, 100, 2000, prec)
-
-    res = {}
-    for name, prec in (("mixed", P.PBFMM_MIXED), ("fp64", P.PBFMM_FP64)):
-        for _ in range(args.warmup):
-            solve(prec)
-        stream.synchronize()
-        hip.synchronize()
-        t = time.perf_counter()
-        its = rrs = None
-        for _ in range(args.steps):
-            its, rrs = solve(prec)
-        stream.synchronize()
-        hip.synchronize()
-        dt = (time.perf_counter() - t) / args.steps
-        res[name] = dict(ms=dt * 1e3, iters=its, relres=rrs)
-        log(f"{name}: {dt * 1e3:.2f} ms per solve, {its} operator applications, relres {rrs:.3e}")
-    # accuracy of the mixed solution against the analytic field
-    sol = x.numpy()
-    solve(P.PBFMM_MIXED)
-    sol = x.numpy()
     xs = pr.mesh.support_xyz
     neu = pr.dirichlet == 0
     phi_ex = pr.exact.phi(xs)
     dn_ex = (pr.exact.grad(xs) * pr.mesh.normals).sum(1)
-    err_phi = float(np.linalg.norm((sol - phi_ex)[neu]) / np.linalg.norm(phi_ex[neu]))
-    err_dn = float(np.linalg.norm((sol - dn_ex)[~neu]) / np.linalg.norm(dn_ex[~neu]))
+    res = {}
+    for name, prec in (("mixed", P.PBFMM_MIXED), ("fp64", P.PBFMM_FP64)):
+        def solve():
+            x.copy_from(zero)
+            return f.gmres_dev(rhs.value, x.value, 1e-8, 100, 2000, prec)
+        for _ in range(warmup):
+            solve()
+        stream.synchronize()
+        t = time.perf_counter()
+        its = rr = None
+        for _ in range(steps):
+            its, rr = solve()
+        stream.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        sol = x.numpy()
+        err_phi = float(np.linalg.norm((sol - phi_ex)[neu]) / np.linalg.norm(phi_ex[neu]))
+        err_dn = float(np.linalg.norm((sol - dn_ex)[~neu]) / np.linalg.norm(dn_ex[~neu])) if (~neu).any() else 0.0
+        res[name] = dict(ms=dt * 1e3, iters=its, relres=rr, err_phi_neumann=err_phi, err_dphi_dirichlet=err_dn)
+        log(f"{name}: {dt * 1e3:.2f} ms per solve, {its} operator applications, relres {rr:.3e}, "
+            f"error vs analytic phi {err_phi:.2e} dphi/dn {err_dn:.2e}")
+    return res
 
-    # FMM matvec throughput and per-phase device times
+
+def time_matvecs(f, n, stream, reps=50, phase_reps=10):
+    """FMM matvec rate (hipEvents on the library stream) and per-phase device
+    times; seeded U(-1,1) inputs (seed 1234)."""
+    import pibem_amd as P
+    from pibem_amd import hip
     rng = np.random.default_rng(1234)
     a = hip.DeviceArray.from_numpy(rng.uniform(-1, 1, n))
     b = hip.DeviceArray.from_numpy(rng.uniform(-1, 1, n))
     oN, oD = hip.DeviceArray(n), hip.DeviceArray(n)
     e0, e1 = hip.Event(), hip.Event()
-    mv = {}
-    phases = {}
+    mv, phases, outs = {}, {}, {}
     for name, prec in (("fp32", P.PBFMM_MIXED), ("fp64", P.PBFMM_FP64)):
-        for _ in range(3):
+        for _ in range(5):
             f.matvec_dev(a.value, b.value, oN.value, oD.value, prec)
-        reps = 50
         e0.record(stream)
         for _ in range(reps):
             f.matvec_dev(a.value, b.value, oN.value, oD.value, prec)
         e1.record(stream)
         e1.synchronize()
         ms = e0.elapsed_ms(e1) / reps
-        mv[name] = 1e3 / ms
+        mv[name] = dict(ms=ms, per_s=1e3 / ms)
+        outs[name] = (oN.numpy(), oD.numpy())
         f.set_phase_timing(True)
         acc = None
-        for _ in range(10):
+        for _ in range(phase_reps):
             f.matvec_dev(a.value, b.value, oN.value, oD.value, prec)
             t = f.last_timings()
             acc = t if acc is None else {k: acc[k] + t[k] for k in t}
         f.set_phase_timing(False)
-        phases[name] = {k: v / 10 for k, v in acc.items()}
+        phases[name] = {k: v / phase_reps for k, v in acc.items()}
         log(f"matvec {name}: {ms:.3f} ms; phases " + " ".join(f"{k}={v:.3f}" for k, v in phases[name].items()))
+    rel = {k: float(np.linalg.norm(outs["fp32"][i] - outs["fp64"][i]) / np.linalg.norm(outs["fp64"][i]))
+           for i, k in enumerate(("N", "D"))}
+    return mv, phases, rel
 
-    spmv = {k: spmv_bytes(wc, 4 if k == "fp32" else 8) / (phases[k]["spmv"] * 1e-3) / 1e9 for k in phases}
-    m2l_tfs = {k: m2l_flops(wc) / (phases[k]["m2l"] * 1e-3) / 1e12 for k in phases}
+
+def measure_config(config, steps, warmup):
+    from pibem_amd import hip
+    pr = make_problem(config)
+    n = pr.mesh.n_dofs
+    log(f"config {config}: N={n} cells={pr.mesh.n_cells} p={pr.p}")
+    stream = hip.Stream()
+    f, bp, setup = setup_handle(pr, stream)
+    log("setup: " + " ".join(f"{k}={v:.2f}" for k, v in setup.items()))
+    wc = f.work_counts()
+    solves = time_solves(f, bp, pr, stream, steps, warmup)
+    mv, phases, rel = time_matvecs(f, n, stream)
+    model = {"fp32": work_model(wc, 4), "fp64": work_model(wc, 8)}
+    roof = {prec: {ph: roofline_of(ph, model[prec], phases[prec][ph], prec, pr.p) for ph in PHASES}
+            for prec in phases}
+    return dict(pr=pr, f=f, stream=stream, wc=wc, setup=setup, solves=solves, matvec=mv, phases=phases,
+                fp32_vs_fp64_matvec=rel, roofline=roof)
+
+
+def cpu_baseline(pr, ops, seconds):
+    """The oracle's FP64 vmult (reference algorithm restated in C++/OpenMP) on
+    the host cores, over a bounded number of calls, scaled to one solve."""
+    import oracle
+    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
+    oracle.set_threads(threads)
+    o = oracle.OracleFMM(pr).setup()
+    o.compute_alpha()
+    v = np.random.default_rng(7).uniform(-1, 1, pr.mesh.n_dofs)
+    t = time.perf_counter()
+    k = 0
+    while k < 3 or (time.perf_counter() - t < seconds and k < 50):
+        o.vmult(v)
+        k += 1
+    per = (time.perf_counter() - t) / k
+    cpu_solve = per * ops
+    log(f"cpu oracle: {per * 1e3:.1f} ms per vmult on {threads} threads -> {cpu_solve:.1f} s per solve")
+    return dict(value=1.0 / cpu_solve, unit="solves/s", cores=threads, kind="port",
+                sample=f"{k} oracle vmult calls at config 2 (FP64 FMM operator, the reference algorithm restated in "
+                       f"C++/OpenMP), {per * 1e3:.1f} ms each, times the {ops} operator applications of the FP64 "
+                       f"GPU solve", ms_per_solve=cpu_solve * 1e3, ms_per_matvec=per * 1e3)
+
+
+def mode_matvec(args):
+    import pibem_amd as P
+    from pibem_amd import hip
+    pr = make_problem(args.config)
+    stream = hip.Stream()
+    f, _, _ = setup_handle(pr, stream)
+    n = pr.mesh.n_dofs
+    rng = np.random.default_rng(1234)
+    a = hip.DeviceArray.from_numpy(rng.uniform(-1, 1, n))
+    b = hip.DeviceArray.from_numpy(rng.uniform(-1, 1, n))
+    oN, oD = hip.DeviceArray(n), hip.DeviceArray(n)
+    prec = P.PBFMM_MIXED if args.precision == "fp32" else P.PBFMM_FP64
+    for _ in range(args.steps):
+        f.matvec_dev(a.value, b.value, oN.value, oD.value, prec)
+    stream.synchronize()
+    log(f"{args.steps} {args.precision} matvecs at config {args.config} done")
+
+
+def mode_copy(args):
+    from pibem_amd import hip
+    nb = args.mib * 1024 * 1024 // 8
+    a = hip.DeviceArray.from_numpy(np.ones(nb))
+    b = hip.DeviceArray(nb)
+    for _ in range(args.steps):
+        b.copy_from(a)
+    hip.synchronize()
+    log(f"{args.steps} device copies of {args.mib} MiB done")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-config3", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--mode", choices=("full", "matvec", "copy"), default="full")
+    ap.add_argument("--config", type=int, choices=(2, 3), default=2)
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--mib", type=int, default=1024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported; this build runs the BEM-FMM path on a "
+              f"single MI355X", file=sys.stderr)
+        sys.exit(2)
+    from pibem_amd import hip
+    if hip.device_count() < 1:
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+    if args.mode == "matvec":
+        return mode_matvec(args)
+    if args.mode == "copy":
+        return mode_copy(args)
+
+    c2 = measure_config(2, args.steps, args.warmup)
+    wc = c2["wc"]
     # dominant kernel of the headline (FP32 operator) matvec
-    ph = phases["fp32"]
-    dom = max(("spmv", "p2m", "m2l"), key=lambda k: ph[k])
-    if dom == "spmv":
-        roof = dict(kernel="k_spmv<float>", bound="hbm", achieved=spmv["fp32"], peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=spmv["fp32"] / HBM_PEAK_GBS, traffic=None,
-                    algorithmic_per_launch=spmv_bytes(wc, 4))
-    elif dom == "m2l":
-        roof = dict(kernel="k_m2l<float>", bound="mfma", achieved=m2l_tfs["fp32"], peak=FP32_PEAK_TFS,
-                    unit="TFLOP/s", frac=m2l_tfs["fp32"] / FP32_PEAK_TFS, traffic=None,
-                    algorithmic_per_launch=m2l_flops(wc))
-    else:
-        p = wc["trunc_order"]
-        kt = (p + 1) * (p + 2) // 2
-        fl = wc["quad_points"] * kt * 2 * 60  # ~60 flops per (point, coefficient, kernel) of the recurrences
-        ach = fl / (ph["p2m"] * 1e-3) / 1e12
-        roof = dict(kernel="k_p2m<float>", bound="mfma", achieved=ach, peak=FP32_PEAK_TFS, unit="TFLOP/s",
-                    frac=ach / FP32_PEAK_TFS, traffic=None, algorithmic_per_launch=fl)
-
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-        threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
-        oracle.set_threads(threads)
-        o = oracle.OracleFMM(pr).setup()
-        o.compute_alpha()
-        v = rng.uniform(-1, 1, n)
-        t = time.perf_counter()
-        k = 0
-        while k < 3 or (time.perf_counter() - t < args.cpu_seconds and k < 50):
-            o.vmult(v)
-            k += 1
-        per = (time.perf_counter() - t) / k
-        ops = res["fp64"]["iters"] + 2
-        cpu_solve = per * ops
-        cpu = dict(value=1.0 / cpu_solve, unit="solves/s", cores=threads, kind="port",
-                   sample=f"{k} oracle vmult calls at config 2 (FP64 FMM operator, reference algorithm restated "
-                          f"in C++/OpenMP), {per * 1e3:.1f} ms each, scaled by the {ops} operator applications of "
-                          f"the FP64 GMRES solve", ms_per_solve=cpu_solve * 1e3, ms_per_matvec=per * 1e3)
-        log(f"cpu oracle: {per * 1e3:.1f} ms per vmult on {threads} threads -> {cpu_solve:.1f} s per solve")
-
+    ph = c2["phases"]["fp32"]
+    dom = max(PHASES, key=lambda k: ph[k])
+    roof = dict(c2["roofline"]["fp32"][dom])
+    roof["traffic"] = traffic_for(roof["kernel"], "config 2")
+    hl = dict(bound=roof["bound"], achieved=roof["achieved"], peak=roof["peak"], unit=roof["unit"],
+              frac=roof["frac"], traffic=roof["traffic"], kernel=roof["kernel"],
+              algorithmic_per_launch=roof["algorithmic_per_launch"], launch_ms=roof["launch_ms"])
+    sp = c2["roofline"]
     out = {
         "metric": METRIC,
-        "value": 1e3 / res["mixed"]["ms"],
+        "value": 1e3 / c2["solves"]["mixed"]["ms"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": res["mixed"]["ms"],
+        "ms_per_step": c2["solves"]["mixed"]["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -216,22 +324,48 @@ def main():
         "data": "synthetic",
         "config": {"workload": "config 2: unit sphere, cube-projected 6x64x64 Q1 cells, 24,578 DoFs, FMM order 6, "
                                "leaf <= 20, GMRES(100) to true rel-res 1e-8, mixed precision (FP32 operator, FP64 "
-                               "Arnoldi/residual/refinement)",
-                   "n_dofs": n, "trunc_order": pr.p, "parallelism": "single GPU"},
-        "fp64_ms_per_solve": res["fp64"]["ms"],
-        "mixed_speedup_vs_fp64": res["fp64"]["ms"] / res["mixed"]["ms"],
-        "operator_applications": {"mixed": res["mixed"]["iters"], "fp64": res["fp64"]["iters"]},
-        "relres": {"mixed": res["mixed"]["relres"], "fp64": res["fp64"]["relres"]},
-        "error_vs_analytic": {"phi_neumann_dofs": err_phi, "dphi_dirichlet_dofs": err_dn},
-        "fmm_matvecs_per_s": mv,
-        "spmv_gbs": spmv,
-        "m2l_tflops": m2l_tfs,
-        "phase_ms": phases,
+                               "singular quadrature, Arnoldi, residual and refinement)",
+                   "n_dofs": wc["n_dofs"], "trunc_order": 6, "parallelism": "single GPU"},
+        "fp64_ms_per_solve": c2["solves"]["fp64"]["ms"],
+        "mixed_speedup_vs_fp64": c2["solves"]["fp64"]["ms"] / c2["solves"]["mixed"]["ms"],
+        "solves": c2["solves"],
+        "fmm_matvecs_per_s": {k: v["per_s"] for k, v in c2["matvec"].items()},
+        "spmv_gbs": {k: sp[k]["spmv"]["achieved"] for k in sp},
+        "m2l_tflops": {k: sp[k]["m2l"]["achieved"] for k in sp},
+        "fp32_vs_fp64_matvec": c2["fp32_vs_fp64_matvec"],
+        "phase_ms": c2["phases"],
+        "phase_roofline": {k: {ph_: {kk: v[kk] for kk in ("kernel", "achieved", "unit", "frac")}
+                               for ph_, v in sp[k].items()} for k in sp},
+        "setup_s": c2["setup"],
         "work_counts": wc,
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "speedup_vs_cpu": (1e3 / res["mixed"]["ms"]) / cpu["value"] if cpu else None,
+        "roofline": hl,
     }
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(c2["pr"], c2["solves"]["fp64"]["iters"] + 2, args.cpu_seconds)
+    out["cpu_baseline"] = cpu
+    out["speedup_vs_cpu"] = out["value"] / cpu["value"] if cpu else None
+    del c2
+    if not args.no_config3:
+        c3 = measure_config(3, max(1, min(args.steps, 3)), 1)
+        r3 = c3["roofline"]
+        out["config3"] = {
+            "workload": "config 3: prolate spheroid (4,1,1), cube-projected 6x128x128 Q2 cells with pole-stretched "
+                        "nodes, 393,218 DoFs, FMM order 8, phi = x + 8 seeded point sources, GMRES(100) to 1e-8",
+            "mixed_ms_per_solve": c3["solves"]["mixed"]["ms"],
+            "fp64_ms_per_solve": c3["solves"]["fp64"]["ms"],
+            "mixed_speedup_vs_fp64": c3["solves"]["fp64"]["ms"] / c3["solves"]["mixed"]["ms"],
+            "solves": c3["solves"],
+            "fmm_matvecs_per_s": {k: v["per_s"] for k, v in c3["matvec"].items()},
+            "spmv_gbs": {k: r3[k]["spmv"]["achieved"] for k in r3},
+            "m2l_tflops": {k: r3[k]["m2l"]["achieved"] for k in r3},
+            "fp32_vs_fp64_matvec": c3["fp32_vs_fp64_matvec"],
+            "phase_ms": c3["phases"],
+            "phase_roofline": {k: {ph_: {kk: v[kk] for kk in ("kernel", "achieved", "unit", "frac")}
+                                   for ph_, v in r3[k].items()} for k in r3},
+            "setup_s": c3["setup"],
+            "work_counts": c3["wc"],
+        }
     print(json.dumps(out))
 
 

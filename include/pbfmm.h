@@ -85,9 +85,10 @@ int pbfmm_compute_m2l_flags(pbfmm_handle *h, int64_t *m2l_count_per_level);
  * near-field N and D matrices on their shared pattern, assembled on the GPU. */
 int pbfmm_direct_integrals(pbfmm_handle *h);
 
-/* BEMFMA<3>::multipole_integrals (bem_fma.h:137, bem_fma.cc:1608-1840).  The
- * per-cell unit multipoles are not stored: leaf quadrature data is staged for
- * the on-the-fly P2M of generate_multipole_expansions. */
+/* BEMFMA<3>::multipole_integrals (bem_fma.h:137, bem_fma.cc:1608-1840): the
+ * translation plans and operators, and the unit multipoles of every leaf
+ * (the reference's elemME per (leaf, cell, shape function), merged here per
+ * (leaf, DoF)), computed on the GPU in FP64. */
 int pbfmm_multipole_integrals(pbfmm_handle *h);
 
 /* BEMFMA<3>::generate_multipole_expansions (bem_fma.h:149-150,
@@ -184,7 +185,8 @@ int pbfmm_last_timings(pbfmm_handle *h, double *ms);
  * [2] near-field column entries [3] near-field rows [4] leaves
  * [5] quad points [6] M2L pairs [7] M2P (pair, node) evaluations
  * [8] M2M child translations [9] L2L translations [10] num_blocks
- * [11] trunc_order [12] padded expansion length kp */
+ * [11] trunc_order [12] padded expansion length kp
+ * [13] unit-multipole columns (sum over leaves of the DoFs feeding P2M) */
 int pbfmm_work_counts(pbfmm_handle *h, int64_t *counts);
 
 #ifdef __cplusplus

@@ -239,7 +239,9 @@ int pbfmm_multipole_integrals(pbfmm_handle *h) {
     if (int rc = need(h, 3, "multipole_integrals")) return rc;
     Engine &e = h->eng;
     build_far_plan(e.tree, e.p, e.far);
+    build_p2m_plan(e.mesh, e.qd, e.tree, e.far);
     e.upload_far(h->prm.precision == PBFMM_MIXED);
+    e.build_unit_multipoles(h->prm.precision == PBFMM_MIXED);
     h->stage = 4;
     return PBFMM_OK;
   });
@@ -589,6 +591,7 @@ int pbfmm_work_counts(pbfmm_handle *h, int64_t *c) {
     c[10] = e.tree.nb;
     c[11] = e.p;
     c[12] = e.kp;
+    c[13] = (int64_t)e.far.p2m_col_dof.size();
     return PBFMM_OK;
   }, false);
 }

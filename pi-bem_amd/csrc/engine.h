@@ -37,8 +37,9 @@ template <typename T>
 struct FarDev {
   DevBuf m2m, l2l, m2l;  // transposed operators
   DevBuf me, le;         // [nb][2 kernels][kp]
-  DevBuf qx, qn, qw, qsh;  // regular quadrature data in T
+  DevBuf qx, qn, qw, qsh;  // regular quadrature data (FP64 copy only)
   DevBuf vals;             // near-field values (N, D) pairs, leaf-blocked
+  DevBuf unit;             // unit multipoles: [column][2 kernels][kp]
 };
 
 struct Engine {
@@ -62,6 +63,7 @@ struct Engine {
       d_quad_beg, d_quad_cnt, d_qperm;
   // device: plans
   DevBuf d_p2m_leaf, d_m2p_tgt, d_m2p_src, d_m2p_work;
+  DevBuf d_p2m_col_beg, d_p2m_col_dof;
   ChunkDev m2m_cl, m2l_cl, l2l_cl;
   DevBuf d_row_leaf, d_row_beg, d_rows, d_col_beg, d_cols, d_val_beg, d_ent_beg, d_ent_cell,
       d_ent_mask, d_ent_near, d_ent_colpos, d_work_beg, d_leaf_block;
@@ -90,7 +92,7 @@ struct Engine {
   // timings of the last matvec (ms)
   double last_ms[8] = {0};
   bool time_phases = false;
-  hipEvent_t ev[10] = {};
+  hipEvent_t ev[11] = {};
   void mark(int i);
   void collect_phase_times();
 
@@ -100,6 +102,7 @@ struct Engine {
   void upload_near_plan();
   void upload_quad(bool want_f32);
   void upload_far(bool want_f32);
+  void build_unit_multipoles(bool want_f32);
   void assemble_near(bool want_f32);
   template <typename T>
   void matvec(const double *phi, const double *dphi, double *outN, double *outD);

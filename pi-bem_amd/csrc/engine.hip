@@ -102,16 +102,18 @@ __device__ __forceinline__ T wave_sum(T v) {
 // ---------------------------------------------------------------------------
 // Near-field assembly (BEMFMA::direct_integrals, bem_fma.cc:670-913 regular
 // and singular, 997-1126 bigger blocks).  One thread per (row, entry): the
-// regular rule over the entry's quad points in T, or the Telles rule over the
-// whole cell in FP64 when the cell holds a double of the row's node.  Results
-// are added into the leaf's rows x cols block of (N, D) pairs.
+// regular rule over the entry's quad points, or the Telles rule over the
+// whole cell when the cell holds a double of the row's node, both in FP64.
+// Results are added into the leaf's rows x cols block of (N, D) pairs; the
+// FP32 operator is this block rounded once (k_round_f32), since in FP32 the
+// difference R = y - x loses ~|x|/|R| ulps and R.n cancels again, which at a
+// radius-4 surface costs 1e-4 of the double-layer product.
 // ---------------------------------------------------------------------------
 // Contraction is off in this kernel: near a singular vertex R = X - x_i is
 // ~1e-5 of |X| and R.n cancels to O(r^2), so the singular double-layer
 // entries follow the last bit of X; evaluating X and R exactly as the
 // reference's FEValues path does (separately rounded products and sums) keeps
 // FP64 entries reproducible to rounding.
-template <typename T>
 __global__ void __launch_bounds__(256) k_assemble_near(
     int nleaf, const int64_t *__restrict__ work_beg, const int *__restrict__ row_beg,
     const int *__restrict__ rows, const int *__restrict__ col_beg, const int *__restrict__ ent_beg,
@@ -119,10 +121,11 @@ __global__ void __launch_bounds__(256) k_assemble_near(
     const uint8_t *__restrict__ ent_near, const int *__restrict__ ent_colpos,
     const int64_t *__restrict__ val_beg, const int *__restrict__ cdofs, int dpc, int nq,
     const int *__restrict__ dbl_ptr, const int *__restrict__ dbl_idx, const double *__restrict__ xyz,
-    const T *__restrict__ qx, const T *__restrict__ qn, const T *__restrict__ qw,
-    const T *__restrict__ qsh, const double *__restrict__ geom, const double *__restrict__ sing,
-    const int *__restrict__ sing_cnt, int sing_stride, T *__restrict__ vals) {
+    const double *__restrict__ qx, const double *__restrict__ qn, const double *__restrict__ qw,
+    const double *__restrict__ qsh, const double *__restrict__ geom, const double *__restrict__ sing,
+    const int *__restrict__ sing_cnt, int sing_stride, double *__restrict__ vals) {
 #pragma clang fp contract(off)
+  using T = double;
   const int64_t total = work_beg[nleaf];
   for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < total;
        w += (int64_t)gridDim.x * blockDim.x) {
@@ -211,70 +214,99 @@ __global__ void __launch_bounds__(256) k_assemble_near(
         }
       }
       for (int j = 0; j < dpc; ++j) {
-        atomicAdd(out + 2 * cp[j], (T)Nj[j]);
-        atomicAdd(out + 2 * cp[j] + 1, (T)Dj[j]);
+        atomicAdd(out + 2 * cp[j], Nj[j]);
+        atomicAdd(out + 2 * cp[j] + 1, Dj[j]);
+      }
+    }
+  }
+}
+
+__global__ void k_round_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+// ---------------------------------------------------------------------------
+// Near-field SpMV, fused N.phi and D.dphi on the shared pattern
+// (prec_neumann_matrix.vmult / prec_dirichlet_matrix.vmult, bem_fma.cc:2408-2409),
+// on the leaf-blocked layout: one workgroup per leaf.  All rows of a leaf share
+// its column set, so the workgroup gathers (phi, dphi) at those columns once
+// into LDS (tiles of kSpmvTile columns) and each wavefront then streams whole
+// rows of (N, D) pairs against the LDS copy: the value block is read once,
+// coalesced, and the gathers are paid per leaf instead of per row.  Leaves are
+// dealt out so that each XCD (workgroups are placed round-robin over the 8
+// XCDs) takes a contiguous range of Morton-ordered leaves, whose columns
+// overlap, into its own L2.
+// ---------------------------------------------------------------------------
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+
+constexpr int kSpmvTile = 2048;
+constexpr int kXcd = 8;
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_spmv(int nleaf, int per_xcd, const int *__restrict__ row_beg,
+                                              const int *__restrict__ rows, const int *__restrict__ col_beg,
+                                              const int *__restrict__ cols, const int64_t *__restrict__ val_beg,
+                                              const T *__restrict__ vals, const double *__restrict__ phi,
+                                              const double *__restrict__ dphi, double *__restrict__ outN,
+                                              double *__restrict__ outD) {
+  using T2 = typename Vec2<T>::type;
+  __shared__ T2 xs[kSpmvTile];
+  const int leaf = (int)(blockIdx.x % kXcd) * per_xcd + (int)(blockIdx.x / kXcd);
+  if (leaf >= nleaf) return;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0;
+  const int cb = col_beg[leaf], C = col_beg[leaf + 1] - cb;
+  const T2 *V = reinterpret_cast<const T2 *>(vals) + val_beg[leaf];
+  for (int c0 = 0; c0 < C; c0 += kSpmvTile) {
+    const int CT = min(kSpmvTile, C - c0);
+    if (c0 > 0) __syncthreads();  // previous tile fully consumed
+    for (int c = t; c < CT; c += 256) {
+      const int col = cols[cb + c0 + c];
+      xs[c] = T2{(T)phi[col], (T)dphi[col]};
+    }
+    __syncthreads();
+    for (int r = w; r < R; r += 4) {
+      const T2 *v = V + (int64_t)r * C + c0;
+      T aN = 0, aD = 0;
+#pragma unroll 4
+      for (int c = lane; c < CT; c += 64) {
+        const T2 a = v[c], x = xs[c];
+        aN += a.x * x.x;
+        aD += a.y * x.y;
+      }
+      aN = wave_sum(aN);
+      aD = wave_sum(aD);
+      if (lane == 0) {
+        const int node = rows[r0 + r];
+        // the wave that owns row r owns it in every tile: plain read-modify-write
+        outN[node] = (c0 == 0 ? 0.0 : outN[node]) + (double)aN;
+        outD[node] = (c0 == 0 ? 0.0 : outD[node]) + (double)aD;
       }
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// Near-field SpMV, fused N.phi and D.dphi on the shared pattern
-// (prec_neumann_matrix.vmult / prec_dirichlet_matrix.vmult, bem_fma.cc:2408-2409).
-// One wavefront per row; the (N, D) pairs of a row are contiguous.
-// ---------------------------------------------------------------------------
-template <typename T> struct Vec2;
-template <> struct Vec2<double> { using type = double2; };
-template <> struct Vec2<float> { using type = float2; };
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_spmv(int nrows, const int *__restrict__ row_leaf,
-                                              const int *__restrict__ row_beg, const int *__restrict__ rows,
-                                              const int *__restrict__ col_beg, const int *__restrict__ cols,
-                                              const int64_t *__restrict__ val_beg, const T *__restrict__ vals,
-                                              const double *__restrict__ phi, const double *__restrict__ dphi,
-                                              double *__restrict__ outN, double *__restrict__ outD) {
-  using T2 = typename Vec2<T>::type;
-  const int row = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= nrows) return;
-  const int leaf = row_leaf[row];
-  const int ri = row - row_beg[leaf];
-  const int cb = col_beg[leaf], C = col_beg[leaf + 1] - cb;
-  const T2 *v = reinterpret_cast<const T2 *>(vals) + val_beg[leaf] + (int64_t)ri * C;
-  const int *cc = cols + cb;
-  T aN = 0, aD = 0;
-  for (int c = lane; c < C; c += 64) {
-    const T2 x = v[c];
-    const int col = cc[c];
-    aN += x.x * (T)phi[col];
-    aD += x.y * (T)dphi[col];
-  }
-  aN = wave_sum(aN);
-  aD = wave_sum(aD);
-  if (lane == 0) {
-    const int node = rows[row];
-    outN[node] = (double)aN;
-    outD[node] = (double)aD;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// P2M at the leaves (generate_multipole_expansions, bem_fma.cc:2064-2114, with
-// the unit multipoles of multipole_integrals, bem_fma.cc:1675-1737, folded in:
-// the quad point strength is sum_j N_j value_j JxW / 4pi).  Ker2 (monopole of
-// dphi/dn, MultipoleExpansion::Add, multipole_expansion.cc:93-122) adds
-// q R_n^m(d) with the regular solid harmonic R_n^m(d) = rho^n Pt_n^m(cos a)
-// e^{-i m b}; Ker1 (MultipoleExpansion::AddNormDer, :124-181) adds
-// q (n . grad) R_n^m(d), the same normal derivative the reference forms with
-// the chain rule through (rho, alpha, beta).  Both come from trig-free
-// Cartesian recurrences in d = y - c:
+// P2M at the leaves (generate_multipole_expansions, bem_fma.cc:2064-2114) as a
+// product with unit multipoles precomputed at set-up (multipole_integrals,
+// bem_fma.cc:1675-1737).  A quad point y with weight w and normal n adds, for
+// each DoF j of its cell, N_j(y) w / 4pi times
+//   Ker1 (MultipoleExpansion::AddNormDer, multipole_expansion.cc:124-181):
+//        (n . grad) R_n^m(y - c)                              [applied to phi]
+//   Ker2 (MultipoleExpansion::Add, multipole_expansion.cc:93-122):
+//        R_n^m(y - c)                                         [applied to dphi]
+// with the regular solid harmonic R_n^m(d) = rho^n Pt_n^m(cos a) e^{-i m b}.
+// Both come from trig-free Cartesian recurrences in d = y - c:
 //   R_m^m = -sqrt((2m-1)/2m) (x - i y) R_{m-1}^{m-1}
 //   R_n^m = ra(n,m) z R_{n-1}^m - rb(n,m) |d|^2 R_{n-2}^m
 // and their directional derivatives (product rule, D(x - i y) = n_x - i n_y,
-// D z = n_z, D |d|^2 = 2 d.n).  One wavefront per leaf, lane = quad point,
-// Kt x 4 accumulators in registers (order P fixed at compile time), one
-// wavefront reduction per coefficient at the end.
+// D z = n_z, D |d|^2 = 2 d.n).  Summed per (leaf, DoF) these give the leaf's
+// unit-multipole block U[column][kernel][kp]; per matvec
+//   ME_leaf = sum_columns (phi, dphi)[dof] * U[column]
+// is a dense, coalesced stream of U.
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T wave_allsum(T v) {
@@ -282,79 +314,119 @@ __device__ __forceinline__ T wave_allsum(T v) {
   return v;
 }
 
-template <typename T, int P>
-__global__ void __launch_bounds__(64) k_p2m(
-    const int *__restrict__ leaves, const int *__restrict__ quad_beg, const int *__restrict__ quad_cnt,
-    const int *__restrict__ qperm, const T *__restrict__ qx, const T *__restrict__ qn,
-    const T *__restrict__ qw, const T *__restrict__ qsh, const int *__restrict__ cdofs, int dpc, int nq,
-    const double *__restrict__ phi, const double *__restrict__ dphi, const double *__restrict__ pmin,
-    const double *__restrict__ delta, int kp, T *__restrict__ ME) {
-  constexpr int KT = (P + 1) * (P + 2) / 2;
-  const int b = leaves[blockIdx.x];
-  const int lane = threadIdx.x;
+// Set-up, FP64: one thread per quad point of the leaves, atomics into U.
+__global__ void __launch_bounds__(256) k_p2m_unit(
+    int npts, int p, int kt, int kp, const int *__restrict__ qpt, const int *__restrict__ qleaf,
+    const int *__restrict__ leaves, const int *__restrict__ qcol, int dpc, const double *__restrict__ qx,
+    const double *__restrict__ qn, const double *__restrict__ qw, const double *__restrict__ qsh,
+    const double *__restrict__ pmin, const double *__restrict__ delta, double *__restrict__ U) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npts) return;
+  const int64_t g = qpt[i];
+  const int b = leaves[qleaf[i]];
   const double hd = delta[b] * 0.5;
-  const T c0 = (T)(pmin[3 * b] + hd), c1 = (T)(pmin[3 * b + 1] + hd), c2 = (T)(pmin[3 * b + 2] + hd);
-  T aRr[KT], aRi[KT], aGr[KT], aGi[KT];
-#pragma unroll
-  for (int k = 0; k < KT; ++k) aRr[k] = aRi[k] = aGr[k] = aGi[k] = T(0);
-  const int beg = quad_beg[b], cnt = quad_cnt[b];
-  for (int i = lane; i < cnt; i += 64) {
-    const int g = qperm[beg + i];
-    const int cell = g / nq;
-    T s1 = 0, s2 = 0;
+  const double x = qx[3 * g] - (pmin[3 * b] + hd), y = qx[3 * g + 1] - (pmin[3 * b + 1] + hd),
+               z = qx[3 * g + 2] - (pmin[3 * b + 2] + hd);
+  const double nx = qn[3 * g], ny = qn[3 * g + 1], nz = qn[3 * g + 2];
+  const double r2 = x * x + y * y + z * z;
+  const double dn2 = 2.0 * (x * nx + y * ny + z * nz);
+  double wj[9];
+  double *col[9];
+  for (int j = 0; j < dpc; ++j) {
+    wj[j] = qsh[g * dpc + j] * qw[g] / (4 * kPi);
+    col[j] = U + (int64_t)qcol[(int64_t)i * dpc + j] * 2 * kp;
+  }
+  auto emit = [&](int k, double Rr, double Ri, double Gr, double Gi) {
     for (int j = 0; j < dpc; ++j) {
-      const T N = qsh[(int64_t)g * dpc + j];
-      const int d = cdofs[(int64_t)cell * dpc + j];
-      s1 += N * (T)phi[d];
-      s2 += N * (T)dphi[d];
+      atomicAdd(col[j] + k, wj[j] * Gr);
+      atomicAdd(col[j] + kt + k, wj[j] * Gi);
+      atomicAdd(col[j] + kp + k, wj[j] * Rr);
+      atomicAdd(col[j] + kp + kt + k, wj[j] * Ri);
     }
-    const T wq = qw[g] / T(4 * kPi);
-    const T q1 = s1 * wq, q2 = s2 * wq;
-    const T x = qx[3 * (int64_t)g] - c0, y = qx[3 * (int64_t)g + 1] - c1, z = qx[3 * (int64_t)g + 2] - c2;
-    const T nx = qn[3 * (int64_t)g], ny = qn[3 * (int64_t)g + 1], nz = qn[3 * (int64_t)g + 2];
-    const T r2 = x * x + y * y + z * z;
-    const T dn2 = T(2) * (x * nx + y * ny + z * nz);
-    T Rmr = 1, Rmi = 0, Gmr = 0, Gmi = 0;  // R_m^m, D R_m^m
+  };
+  double Rmr = 1, Rmi = 0, Gmr = 0, Gmi = 0;  // R_m^m, D R_m^m
+  for (int m = 0; m <= p; ++m) {
+    if (m > 0) {
+      const double f = -PMF<double>(m);
+      const double nr = f * (x * Rmr + y * Rmi), ni = f * (x * Rmi - y * Rmr);
+      const double gr = f * ((nx * Rmr + ny * Rmi) + (x * Gmr + y * Gmi));
+      const double gi = f * ((nx * Rmi - ny * Rmr) + (x * Gmi - y * Gmr));
+      Rmr = nr; Rmi = ni; Gmr = gr; Gmi = gi;
+    }
+    double Rcr = Rmr, Rci = Rmi, Gcr = Gmr, Gci = Gmi, Rpr = 0, Rpi = 0, Gpr = 0, Gpi = 0;
+    emit(m * (m + 1) / 2 + m, Rcr, Rci, Gcr, Gci);
+    for (int n = m + 1; n <= p; ++n) {
+      const double a = RA<double>(n, m), bb = RB<double>(n, m);
+      const double Rnr = a * z * Rcr - bb * r2 * Rpr, Rni = a * z * Rci - bb * r2 * Rpi;
+      const double Gnr = a * (nz * Rcr + z * Gcr) - bb * (dn2 * Rpr + r2 * Gpr);
+      const double Gni = a * (nz * Rci + z * Gci) - bb * (dn2 * Rpi + r2 * Gpi);
+      emit(n * (n + 1) / 2 + m, Rnr, Rni, Gnr, Gni);
+      Rpr = Rcr; Rpi = Rci; Gpr = Gcr; Gpi = Gci;
+      Rcr = Rnr; Rci = Rni; Gcr = Gnr; Gci = Gni;
+    }
+  }
+}
+
+// Per matvec: one workgroup per leaf.  Output o of the leaf's 2 kp
+// coefficients (kernel o / kp) is owned by virtual thread v = ph * 2kp + o,
+// which sums the columns ph, ph + nph, ...; the leaf's (phi, dphi) values are
+// gathered into LDS once per tile of columns.
+constexpr int kP2mTile = 1024;
+template <typename T>
+__global__ void __launch_bounds__(256) k_p2m(int nleaf, const int *__restrict__ leaves,
+                                             const int *__restrict__ col_beg, const int *__restrict__ col_dof,
+                                             const T *__restrict__ U, int kp, const double *__restrict__ phi,
+                                             const double *__restrict__ dphi, T *__restrict__ ME) {
+  using T2 = typename Vec2<T>::type;
+  __shared__ T2 xs[kP2mTile];
+  __shared__ T part[512];
+  const int li = blockIdx.x;
+  const int t = threadIdx.x;
+  const int NO = 2 * kp;                       // <= 288 for p <= 10
+  const int nph = NO <= 256 ? 256 / NO : 1;    // column phases
+  const int nv = nph * NO;                     // virtual threads (<= 512)
+  const int c0 = col_beg[li], C = col_beg[li + 1] - c0;
+  const T *Ub = U + (int64_t)c0 * NO;
+  T acc[2] = {T(0), T(0)};
+  for (int cb = 0; cb < C; cb += kP2mTile) {
+    const int CT = min(kP2mTile, C - cb);
+    if (cb > 0) __syncthreads();
+    for (int c = t; c < CT; c += 256) {
+      const int d = col_dof[c0 + cb + c];
+      xs[c] = T2{(T)phi[d], (T)dphi[d]};
+    }
+    __syncthreads();
 #pragma unroll
-    for (int m = 0; m <= P; ++m) {
-      if (m > 0) {
-        const T f = -PMF<T>(m);
-        const T nr = f * (x * Rmr + y * Rmi), ni = f * (x * Rmi - y * Rmr);
-        const T gr = f * ((nx * Rmr + ny * Rmi) + (x * Gmr + y * Gmi));
-        const T gi = f * ((nx * Rmi - ny * Rmr) + (x * Gmi - y * Gmr));
-        Rmr = nr; Rmi = ni; Gmr = gr; Gmi = gi;
-      }
-      T Rcr = Rmr, Rci = Rmi, Gcr = Gmr, Gci = Gmi, Rpr = 0, Rpi = 0, Gpr = 0, Gpi = 0;
-      const int k0 = m * (m + 1) / 2 + m;
-      aRr[k0] += q2 * Rcr; aRi[k0] += q2 * Rci;
-      aGr[k0] += q1 * Gcr; aGi[k0] += q1 * Gci;
-#pragma unroll
-      for (int n = m + 1; n <= P; ++n) {
-        const T a = RA<T>(n, m), bb = RB<T>(n, m);
-        const T Rnr = a * z * Rcr - bb * r2 * Rpr, Rni = a * z * Rci - bb * r2 * Rpi;
-        const T Gnr = a * (nz * Rcr + z * Gcr) - bb * (dn2 * Rpr + r2 * Gpr);
-        const T Gni = a * (nz * Rci + z * Gci) - bb * (dn2 * Rpi + r2 * Gpi);
-        const int k = n * (n + 1) / 2 + m;
-        aRr[k] += q2 * Rnr; aRi[k] += q2 * Rni;
-        aGr[k] += q1 * Gnr; aGi[k] += q1 * Gni;
-        Rpr = Rcr; Rpi = Rci; Gpr = Gcr; Gpi = Gci;
-        Rcr = Rnr; Rci = Rni; Gcr = Gnr; Gci = Gni;
+    for (int s = 0; s < 2; ++s) {
+      const int v = t + 256 * s;
+      if (v < nv) {
+        const int o = v % NO, ph = v / NO;
+        const bool k2 = o >= kp;
+        const T *u = Ub + (int64_t)cb * NO + o;
+        T a = acc[s];
+        for (int c = ph; c < CT; c += nph) {
+          const T2 xv = xs[c];
+          a += (k2 ? xv.y : xv.x) * u[(int64_t)c * NO];
+        }
+        acc[s] = a;
       }
     }
   }
-  T *M1 = ME + (int64_t)b * 2 * kp, *M2 = M1 + kp;
+  T *out = ME + (int64_t)leaves[li] * NO;
+  if (nph == 1) {
 #pragma unroll
-  for (int k = 0; k < KT; ++k) {
-    const T v0 = wave_allsum(aGr[k]), v1 = wave_allsum(aGi[k]);
-    const T v2 = wave_allsum(aRr[k]), v3 = wave_allsum(aRi[k]);
-    if (lane == (k & 63)) {
-      M1[k] = v0; M1[KT + k] = v1;
-      M2[k] = v2; M2[KT + k] = v3;
+    for (int s = 0; s < 2; ++s) {
+      const int v = t + 256 * s;
+      if (v < NO) out[v] = acc[s];
     }
+    return;
   }
-  for (int k = 2 * KT + lane; k < kp; k += 64) {
-    M1[k] = T(0);
-    M2[k] = T(0);
+  part[t] = acc[0];  // nv <= 256 here
+  __syncthreads();
+  if (t < NO) {
+    T a = part[t];
+    for (int ph = 1; ph < nph; ++ph) a += part[ph * NO + t];
+    out[t] = a;
   }
 }
 
@@ -414,7 +486,9 @@ struct TrLayout {
   static constexpr size_t bytes = (size_t)64 * BS * sizeof(T);
 };
 
-template <typename T, int P>
+// KIND (0 M2M, 1 M2L, 2 L2L) only names the instantiation, so that traces and
+// profiles tell the three passes apart.
+template <typename T, int P, int KIND>
 __global__ void __launch_bounds__(256) k_translate(const int *__restrict__ chunk_op,
                                                    const int *__restrict__ chunk_beg,
                                                    const int *__restrict__ chunk_cnt, const int *__restrict__ tgt,
@@ -785,7 +859,7 @@ Engine::~Engine() {
 }
 
 // Phase events on the handle's stream (no synchronisation between kernels):
-// 0 P2M 1 M2M 2, 3 SpMV 4 fill 5 M2L 6 M2P 7 L2L 8 L2P 9.
+// 10 clear ME 0 P2M 1 M2M 2, 3 SpMV 4 clear LE 5 M2L 6 M2P 7 L2L 8 L2P 9.
 void Engine::mark(int i) {
   if (!time_phases) return;
   if (!ev[i]) HIPCHK(hipEventCreate(&ev[i]));
@@ -807,7 +881,7 @@ void Engine::collect_phase_times() {
   last_ms[4] = el(5, 6);  // m2l
   last_ms[5] = el(6, 7);  // m2p
   last_ms[6] = el(8, 9);  // l2p
-  last_ms[7] = el(0, 9);  // whole matvec
+  last_ms[7] = el(10, 9);  // whole matvec
 }
 
 void Engine::sync() { HIPCHK(hipStreamSynchronize(stream)); }
@@ -869,17 +943,11 @@ static void upload_far_t(Engine &e, FarDev<T> &F) {
   HIPCHK(hipMemset(F.le.p, 0, F.le.bytes));
 }
 
-template <typename T>
-static void upload_quad_t(Engine &e, FarDev<T> &F) {
-  up(F.qx, cast_vec<T>(e.qd.x));
-  up(F.qn, cast_vec<T>(e.qd.n));
-  up(F.qw, cast_vec<T>(e.qd.jxw));
-  up(F.qsh, cast_vec<T>(e.qd.sh));
-}
-
 void Engine::upload_quad(bool want_f32) {
-  upload_quad_t<double>(*this, f64);
-  if (want_f32) upload_quad_t<float>(*this, f32);
+  up(f64.qx, qd.x);
+  up(f64.qn, qd.n);
+  up(f64.qw, qd.jxw);
+  up(f64.qsh, qd.sh);
   have_f32 = want_f32;
 }
 
@@ -893,6 +961,8 @@ static void upload_chunks(ChunkDev &d, const ChunkList &c) {
 
 void Engine::upload_far(bool want_f32) {
   up(d_p2m_leaf, far.p2m_leaf);
+  up(d_p2m_col_beg, far.p2m_col_beg);
+  up(d_p2m_col_dof, far.p2m_col_dof);
   upload_chunks(m2m_cl, far.m2m);
   upload_chunks(m2l_cl, far.m2l);
   upload_chunks(l2l_cl, far.l2l);
@@ -901,6 +971,34 @@ void Engine::upload_far(bool want_f32) {
   up(d_m2p_work, far.m2p_work_beg);
   upload_far_t<double>(*this, f64);
   if (want_f32) upload_far_t<float>(*this, f32);
+}
+
+// Unit multipoles of every leaf (FP64, rounded once for the FP32 operator).
+void Engine::build_unit_multipoles(bool want_f32) {
+  const int64_t ncol = (int64_t)far.p2m_col_dof.size();
+  const int64_t nu = std::max<int64_t>(ncol, 1) * 2 * kp;
+  alloc(f64.unit, (size_t)nu * sizeof(double));
+  HIPCHK(hipMemsetAsync(f64.unit.p, 0, f64.unit.bytes, stream));
+  const int npts = (int)far.p2m_qpt.size();
+  if (npts > 0) {
+    DevBuf qpt, qleaf, qcol;
+    up(qpt, far.p2m_qpt);
+    up(qleaf, far.p2m_qleaf);
+    up(qcol, far.p2m_qcol);
+    hipLaunchKernelGGL(k_p2m_unit, dim3(nblk(npts, 256)), dim3(256), 0, stream, npts, p, kt, kp, qpt.as<int>(),
+                       qleaf.as<int>(), d_p2m_leaf.as<int>(), qcol.as<int>(), dpc, f64.qx.as<double>(),
+                       f64.qn.as<double>(), f64.qw.as<double>(), f64.qsh.as<double>(), d_pmin.as<double>(),
+                       d_delta.as<double>(), f64.unit.as<double>());
+    HIPCHK(hipGetLastError());
+    sync();  // qpt, qleaf, qcol are freed on return
+  }
+  if (want_f32) {
+    alloc(f32.unit, (size_t)nu * sizeof(float));
+    hipLaunchKernelGGL(k_round_f32, dim3((unsigned)std::min<int64_t>(nblk(nu, 256), 8192)), dim3(256), 0, stream, nu,
+                       f64.unit.as<double>(), f32.unit.as<float>());
+    HIPCHK(hipGetLastError());
+  }
+  sync();
 }
 
 // Singular rules for every local support point js: Telles in each direction
@@ -935,26 +1033,27 @@ void Engine::assemble_near(bool want_f32) {
   DevBuf d_scnt;
   up(d_scnt, cnt);
 
-  auto run = [&](auto tag, auto &F) {
-    using T = decltype(tag);
-    alloc(F.vals, (size_t)std::max<int64_t>(near.nnz, 1) * 2 * sizeof(T));
-    HIPCHK(hipMemsetAsync(F.vals.p, 0, F.vals.bytes, stream));
-    const int64_t total = near.work_beg.back();
-    if (near.nleaf > 0 && total > 0) {
-      const unsigned grid = (unsigned)std::min<int64_t>(nblk(total, 256), 65536);
-      hipLaunchKernelGGL(k_assemble_near<T>, dim3(grid), dim3(256), 0, stream, near.nleaf,
-                         d_work_beg.as<int64_t>(), d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(),
-                         d_ent_beg.as<int>(), d_ent_cell.as<int>(), d_ent_mask.as<uint64_t>(),
-                         d_ent_near.as<uint8_t>(), d_ent_colpos.as<int>(), d_val_beg.as<int64_t>(),
-                         d_cdofs.as<int>(), dpc, nq, d_dbl_ptr.as<int>(), d_dbl_idx.as<int>(), d_xyz.as<double>(),
-                         F.qx.template as<T>(), F.qn.template as<T>(), F.qw.template as<T>(),
-                         F.qsh.template as<T>(), d_geom.as<double>(), d_sing.as<double>(), d_scnt.as<int>(),
-                         maxq * ps, F.vals.template as<T>());
-      HIPCHK(hipGetLastError());
-    }
-  };
-  run(double(0), f64);
-  if (want_f32) run(float(0), f32);
+  const int64_t nv = std::max<int64_t>(near.nnz, 1) * 2;
+  alloc(f64.vals, (size_t)nv * sizeof(double));
+  HIPCHK(hipMemsetAsync(f64.vals.p, 0, f64.vals.bytes, stream));
+  const int64_t total = near.work_beg.back();
+  if (near.nleaf > 0 && total > 0) {
+    const unsigned grid = (unsigned)std::min<int64_t>(nblk(total, 256), 65536);
+    hipLaunchKernelGGL(k_assemble_near, dim3(grid), dim3(256), 0, stream, near.nleaf, d_work_beg.as<int64_t>(),
+                       d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_ent_beg.as<int>(),
+                       d_ent_cell.as<int>(), d_ent_mask.as<uint64_t>(), d_ent_near.as<uint8_t>(),
+                       d_ent_colpos.as<int>(), d_val_beg.as<int64_t>(), d_cdofs.as<int>(), dpc, nq,
+                       d_dbl_ptr.as<int>(), d_dbl_idx.as<int>(), d_xyz.as<double>(), f64.qx.as<double>(),
+                       f64.qn.as<double>(), f64.qw.as<double>(), f64.qsh.as<double>(), d_geom.as<double>(),
+                       d_sing.as<double>(), d_scnt.as<int>(), maxq * ps, f64.vals.as<double>());
+    HIPCHK(hipGetLastError());
+  }
+  if (want_f32) {
+    alloc(f32.vals, (size_t)nv * sizeof(float));
+    hipLaunchKernelGGL(k_round_f32, dim3((unsigned)std::min<int64_t>(nblk(nv, 256), 8192)), dim3(256), 0, stream, nv,
+                       f64.vals.as<double>(), f32.vals.as<float>());
+    HIPCHK(hipGetLastError());
+  }
   sync();
 }
 
@@ -966,14 +1065,14 @@ template <>
 FarDev<float> &far_of<float>(Engine &e) { return e.f32; }
 
 // OUT[tgt] += ops[class] . IN[src] for the chunks [c0, c1) of a chunk list.
-template <typename T>
+template <typename T, int KIND>
 static void launch_translate(Engine &e, hipStream_t st, const ChunkDev &cl, int c0, int c1, const T *ops,
                              const T *IN, T *OUT) {
   const int n = c1 - c0;
   if (n <= 0) return;
 #define PBFMM_TR(PP)                                                                                         \
   case PP:                                                                                                   \
-    hipLaunchKernelGGL((k_translate<T, PP>), dim3(n), dim3(256), 0, st, cl.op.as<int>() + c0,               \
+    hipLaunchKernelGGL((k_translate<T, PP, KIND>), dim3(n), dim3(256), 0, st, cl.op.as<int>() + c0,         \
                        cl.beg.as<int>() + c0, cl.cnt.as<int>() + c0, cl.tgt.as<int>(), cl.src.as<int>(), ops,  \
                        IN, OUT);                                                                             \
     break;
@@ -989,28 +1088,18 @@ template <typename T>
 void Engine::generate(const double *phi, const double *dphi) {
   FarDev<T> &F = far_of<T>(*this);
   const int nleaf = (int)far.p2m_leaf.size();
-  mark(0);
+  mark(10);
   const int64_t nme = (int64_t)tree.nb * 2 * kp;
   hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(nme, 256), 4096)), dim3(256), 0, stream,
                      F.me.template as<T>(), nme, T(0));
-  if (nleaf > 0) {
-#define PBFMM_P2M(PP)                                                                                          \
-  case PP:                                                                                                     \
-    hipLaunchKernelGGL((k_p2m<T, PP>), dim3(nleaf), dim3(64), 0, stream, d_p2m_leaf.as<int>(),                  \
-                       d_quad_beg.as<int>(), d_quad_cnt.as<int>(), d_qperm.as<int>(), F.qx.template as<T>(),   \
-                       F.qn.template as<T>(), F.qw.template as<T>(), F.qsh.template as<T>(), d_cdofs.as<int>(), \
-                       dpc, nq, phi, dphi, d_pmin.as<double>(), d_delta.as<double>(), kp, F.me.template as<T>()); \
-    break;
-    switch (p) {
-      PBFMM_P2M(1) PBFMM_P2M(2) PBFMM_P2M(3) PBFMM_P2M(4) PBFMM_P2M(5)
-      PBFMM_P2M(6) PBFMM_P2M(7) PBFMM_P2M(8) PBFMM_P2M(9) PBFMM_P2M(10)
-      default: throw std::runtime_error("trunc_order outside 1..10");
-    }
-#undef PBFMM_P2M
-  }
+  mark(0);
+  if (nleaf > 0)
+    hipLaunchKernelGGL(k_p2m<T>, dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
+                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.unit.template as<T>(), kp, phi, dphi,
+                       F.me.template as<T>());
   mark(1);
   for (int l = L; l >= 1; --l)
-    launch_translate<T>(*this, stream, m2m_cl, far.m2m.level_off[l], far.m2m.level_off[l + 1],
+    launch_translate<T, 0>(*this, stream, m2m_cl, far.m2m.level_off[l], far.m2m.level_off[l + 1],
                         F.m2m.template as<T>(), F.me.template as<T>(), F.me.template as<T>());
   mark(2);
   HIPCHK(hipGetLastError());
@@ -1021,16 +1110,18 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
   FarDev<T> &F = far_of<T>(*this);
   const int nrows = (int)near.rows.size();
   mark(3);
-  if (nrows > 0)
-    hipLaunchKernelGGL(k_spmv<T>, dim3(nblk((int64_t)nrows * 64, 256)), dim3(256), 0, stream, nrows,
-                       d_row_leaf.as<int>(), d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(),
-                       d_cols.as<int>(), d_val_beg.as<int64_t>(), F.vals.template as<T>(), phi, dphi, outN, outD);
+  if (near.nleaf > 0) {
+    const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
+    hipLaunchKernelGGL(k_spmv<T>, dim3(per_xcd * kXcd), dim3(256), 0, stream, near.nleaf, per_xcd,
+                       d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_cols.as<int>(),
+                       d_val_beg.as<int64_t>(), F.vals.template as<T>(), phi, dphi, outN, outD);
+  }
   mark(4);
   const int64_t nle = (int64_t)tree.nb * 2 * kp;
   hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(nle, 256), 4096)), dim3(256), 0, stream,
                      F.le.template as<T>(), nle, T(0));
   mark(5);
-  launch_translate<T>(*this, stream, m2l_cl, 0, (int)far.m2l.op.size(), F.m2l.template as<T>(),
+  launch_translate<T, 1>(*this, stream, m2l_cl, 0, (int)far.m2l.op.size(), F.m2l.template as<T>(),
                       F.me.template as<T>(), F.le.template as<T>());
   mark(6);
   const int npairs = (int)far.m2p_tgt.size();
@@ -1044,7 +1135,7 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
   }
   mark(7);
   for (int l = 2; l <= L; ++l)
-    launch_translate<T>(*this, stream, l2l_cl, far.l2l.level_off[l], far.l2l.level_off[l + 1],
+    launch_translate<T, 2>(*this, stream, l2l_cl, far.l2l.level_off[l], far.l2l.level_off[l + 1],
                         F.l2l.template as<T>(), F.le.template as<T>(), F.le.template as<T>());
   mark(8);
   if (nrows > 0)
@@ -1321,6 +1412,11 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
   return 1;
 }
 
+// Residual reduction asked of one FP32-operator inner solve of the iterative
+// refinement: the FP32 operator differs from the FP64 one by ~1e-6 relative, so
+// an inner solve cannot lower the true residual much further than that.
+constexpr double kInnerReduction = 1e-5;
+
 int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit, int precision, int *iters,
                   double *relres) {
   if (restart != gm_restart) {
@@ -1346,7 +1442,7 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
     if (precision == PBFMM_MIXED && have_f32) {
       // iterative refinement: FP32-operator GMRES on the FP64 residual
       hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d, n, 0.0);
-      const double tin = std::max(tol * bn * 0.5, 1e-5 * rn);
+      const double tin = std::max(tol * bn * 0.5, kInnerReduction * rn);
       gmres_core(*this, r, d, tin, restart, maxit, PBFMM_MIXED, it, est);
       hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, d, 1.0, x);
     } else {

@@ -112,6 +112,12 @@ struct ChunkList {
 struct FarPlan {
   int p = 0, kt = 0, kp = 0;
   std::vector<int32_t> p2m_leaf;  // all childless blocks
+  // P2M as a product with unit multipoles: per leaf (index into p2m_leaf) the
+  // sorted DoFs of the cells that have quad points in the leaf (columns
+  // p2m_col_beg[li] .. p2m_col_beg[li+1]); per quad point of the leaves, in
+  // leaf order, its id, its leaf and the column of each of its cell's DoFs.
+  std::vector<int32_t> p2m_col_beg, p2m_col_dof;
+  std::vector<int32_t> p2m_qpt, p2m_qleaf, p2m_qcol;
   ChunkList m2m;                  // tgt = parent, src = child; op = level*8 + octant
   ChunkList m2l;                  // op = offset class (all levels)
   ChunkList l2l;                  // tgt = child, src = parent; op = level*8 + octant
@@ -128,6 +134,7 @@ struct FarPlan {
   std::vector<double> m2l_ops;  // n_m2l_ops
 };
 void build_far_plan(const Tree &t, int p, FarPlan &fp);
+void build_p2m_plan(const HostMesh &m, const QuadData &qd, const Tree &t, FarPlan &fp);
 // R-linear translation operators on [Re, Im] vectors (padded to kp).
 void op_m2m(int p, const double d[3], double *opT);  // d = child - parent
 void op_l2l(int p, const double d[3], double *opT);  // d = parent - child

@@ -628,4 +628,43 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
   fp.m2l.level_off[L + 1] = (int)fp.m2l.op.size();
 }
 
+// Columns of the unit-multipole blocks (multipole_integrals,
+// bem_fma.cc:1675-1737 keeps one expansion per (block, cell, shape function);
+// here the cells' shape functions are merged per DoF, so a leaf's block has one
+// column per distinct DoF).
+void build_p2m_plan(const HostMesh &m, const QuadData &qd, const Tree &t, FarPlan &fp) {
+  const int dpc = m.dpc, nq = qd.nq;
+  const int nleaf = (int)fp.p2m_leaf.size();
+  fp.p2m_col_beg.assign(nleaf + 1, 0);
+  fp.p2m_col_dof.clear();
+  fp.p2m_qpt.clear();
+  fp.p2m_qleaf.clear();
+  fp.p2m_qcol.clear();
+  std::vector<int32_t> cols;
+  for (int li = 0; li < nleaf; ++li) {
+    const int b = fp.p2m_leaf[li];
+    const int beg = t.quad_beg[b], cnt = t.quad_cnt[b];
+    cols.clear();
+    for (int i = 0; i < cnt; ++i) {
+      const int64_t cell = t.qperm[beg + i] / nq;
+      for (int j = 0; j < dpc; ++j) cols.push_back(m.cdofs[cell * dpc + j]);
+    }
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    const int c0 = (int)fp.p2m_col_dof.size();
+    fp.p2m_col_dof.insert(fp.p2m_col_dof.end(), cols.begin(), cols.end());
+    fp.p2m_col_beg[li + 1] = (int)fp.p2m_col_dof.size();
+    for (int i = 0; i < cnt; ++i) {
+      const int g = t.qperm[beg + i];
+      const int64_t cell = g / nq;
+      fp.p2m_qpt.push_back(g);
+      fp.p2m_qleaf.push_back(li);
+      for (int j = 0; j < dpc; ++j) {
+        const int d = m.cdofs[cell * dpc + j];
+        fp.p2m_qcol.push_back(c0 + (int)(std::lower_bound(cols.begin(), cols.end(), d) - cols.begin()));
+      }
+    }
+  }
+}
+
 }  // namespace pbfmm
