@@ -15,10 +15,13 @@ measured the same way and reported in the "config3" object.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-config3]
     python bench.py --mode matvec --config {2,3} --precision {fp32,fp64} --steps K
     python bench.py --mode copy --mib M --steps K
+    python bench.py --mode solve --config {2,3} --precision {fp32,fp64} --steps K
 
-The default mode prints one JSON line.  "matvec" only runs K FMM matvecs (a
-short program to run under rocprofv3); "copy" runs K device-to-device copies
-of M MiB, the calibration for the HBM byte counters.
+The default mode prints one JSON line.  "matvec" only runs K FMM matvecs
+(separate N and D outputs) followed by K operator applications, and
+"solve" K GMRES solves (fp32: the mixed-precision solve), short programs to
+run under rocprofv3; "copy" runs K device-to-device copies of M MiB, the
+calibration for the HBM byte counters.
 """
 from __future__ import annotations
 
@@ -40,10 +43,13 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFS = 157.3
 FP64_PEAK_TFS = 78.6
 PHASES = ("spmv", "p2m", "m2m", "l2l", "m2l", "m2p", "l2p")
-# kernel names as they appear in traces, per phase and operator precision
+# kernel names as they appear in traces, per phase, operator precision and
+# number of expansions per block (2: N and D separately, 1: the operator's
+# combined N.phi - D.dphi)
 KERNEL = {
-    "spmv": "k_spmv<{t}>", "p2m": "k_p2m<{t}>", "m2m": "k_translate<{t}, {p}, 0>",
-    "m2l": "k_translate<{t}, {p}, 1>", "l2l": "k_translate<{t}, {p}, 2>", "m2p": "k_m2p<{t}>", "l2p": "k_l2p<{t}>",
+    "spmv": "k_spmv<{t}, {k}>", "p2m": "k_p2m<{t}, {k}>", "m2m": "k_translate<{t}, {p}, 0, {k}>",
+    "m2l": "k_translate<{t}, {p}, 1, {k}>", "l2l": "k_translate<{t}, {p}, 2, {k}>", "m2p": "k_m2p<{t}, {k}>",
+    "l2p": "k_l2p<{t}, {k}>",
 }
 
 
@@ -51,43 +57,45 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def work_model(wc, s_v):
+def work_model(wc, s_v, nk):
     """Algorithmic bytes and flops of one launch of each phase (s_v: bytes of
-    an operator value, 4 for the FP32 operator, 8 for FP64).
+    an operator value, 4 for the FP32 operator, 8 for FP64; nk: expansions per
+    block, 2 for separate N and D outputs, 1 for the operator's N.phi - D.dphi).
 
     spmv: the (N, D) value pairs of the leaf blocks, one column index per leaf
-          column, one row id per row, phi and dphi read once, both outputs
+          column, one row id per row, phi and dphi read once, nk outputs
           written once.
     p2m:  the unit-multipole blocks (2 kernels x kp values per leaf column), one
-          DoF index per column, phi and dphi once, the leaf expansions written.
-    m2m, l2l, m2l: 8 Kt^2 flops per (pair, kernel): a complex Kt x Kt map.
-    m2p, l2p: ~(Kt (8 + 8) + 40) flops per (point, expansion pair) evaluation.
+          DoF index per column, phi and dphi once, nk kp values per leaf written.
+    m2m, l2l, m2l: 8 Kt^2 flops per (pair, expansion): a complex Kt x Kt map.
+    m2p, l2p: ~(8 nk Kt + 8 Kt + 40) flops per evaluation point.
     """
     n, p, kp = wc["n_dofs"], wc["trunc_order"], wc["kp"]
     kt = (p + 1) * (p + 2) // 2
     out = {
         "spmv": dict(bound="hbm", bytes=wc["near_values"] * 2 * s_v + wc["near_cols"] * 4 + wc["near_rows"] * 4
-                     + 2 * n * 8 + 2 * n * 8),
+                     + 2 * n * 8 + nk * n * 8),
         "p2m": dict(bound="hbm", bytes=wc["p2m_cols"] * (2 * kp * s_v + 4) + 2 * n * 8
-                    + wc["leaves"] * 2 * kp * s_v),
+                    + wc["leaves"] * nk * kp * s_v),
     }
     for k, pairs in (("m2m", wc["m2m_translations"]), ("l2l", wc["l2l_translations"]), ("m2l", wc["m2l_pairs"])):
-        out[k] = dict(bound="mfma", flops=pairs * 2 * 8 * kt * kt)
-    out["m2p"] = dict(bound="valu", flops=wc["m2p_evals"] * (16 * kt + 40))
-    out["l2p"] = dict(bound="valu", flops=wc["near_rows"] * (16 * kt + 40))
+        out[k] = dict(bound="mfma", flops=pairs * nk * 8 * kt * kt)
+    out["m2p"] = dict(bound="valu", flops=wc["m2p_evals"] * (8 * nk * kt + 8 * kt + 40))
+    out["l2p"] = dict(bound="valu", flops=wc["near_rows"] * (8 * nk * kt + 8 * kt + 40))
     return out
 
 
-def roofline_of(phase, model, ms, prec, p):
+def roofline_of(phase, model, ms, prec, p, nk):
     t = "float" if prec == "fp32" else "double"
     m = model[phase]
+    name = KERNEL[phase].format(t=t, p=p, k=nk)
     if m["bound"] == "hbm":
         ach = m["bytes"] / (ms * 1e-3) / 1e9
-        return dict(kernel=KERNEL[phase].format(t=t, p=p), bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
+        return dict(kernel=name, bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
                      unit="GB/s", frac=ach / HBM_PEAK_GBS, algorithmic_per_launch=m["bytes"], launch_ms=ms)
     peak = FP32_PEAK_TFS if prec == "fp32" else FP64_PEAK_TFS
     ach = m["flops"] / (ms * 1e-3) / 1e12
-    return dict(kernel=KERNEL[phase].format(t=t, p=p), bound="mfma" if m["bound"] == "mfma" else "valu",
+    return dict(kernel=name, bound="mfma" if m["bound"] == "mfma" else "valu",
                 achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak, algorithmic_per_launch=m["flops"],
                 launch_ms=ms)
 
@@ -166,9 +174,47 @@ def time_solves(f, bp, pr, stream, steps, warmup):
     return res
 
 
+def time_phases(f, call, reps):
+    f.set_phase_timing(True)
+    acc = None
+    for _ in range(reps):
+        call()
+        t = f.last_timings()
+        acc = t if acc is None else {k: acc[k] + t[k] for k in t}
+    f.set_phase_timing(False)
+    return {k: v / reps for k, v in acc.items()}
+
+
+def time_operator(f, bp, n, stream, reps=50, phase_reps=10):
+    """BEMProblem::vmult on device vectors (the operator GMRES applies, whose
+    FMM carries one combined expansion per block): rate and per-phase device
+    times, seeded U(-1,1) input (seed 1234)."""
+    import pibem_amd as P
+    from pibem_amd import hip
+    src = hip.DeviceArray.from_numpy(np.random.default_rng(1234).uniform(-1, 1, n))
+    dst = hip.DeviceArray(n)
+    e0, e1 = hip.Event(), hip.Event()
+    rate, phases = {}, {}
+    for name, prec in (("fp32", P.PBFMM_MIXED), ("fp64", P.PBFMM_FP64)):
+        for _ in range(5):
+            bp.vmult_dev(src.value, dst.value, prec)
+        e0.record(stream)
+        for _ in range(reps):
+            bp.vmult_dev(src.value, dst.value, prec)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_ms(e1) / reps
+        rate[name] = dict(ms=ms, per_s=1e3 / ms)
+        phases[name] = time_phases(f, lambda: bp.vmult_dev(src.value, dst.value, prec), phase_reps)
+        log(f"operator {name}: {ms:.3f} ms; phases " + " ".join(f"{k}={v:.3f}" for k, v in phases[name].items()))
+    return rate, phases
+
+
 def time_matvecs(f, n, stream, reps=50, phase_reps=10):
-    """FMM matvec rate (hipEvents on the library stream) and per-phase device
-    times; seeded U(-1,1) inputs (seed 1234)."""
+    """FMM matvec with separate N.phi and D.dphi outputs
+    (multipole_matr_vect_products on device vectors): rate (hipEvents on the
+    library stream) and per-phase device times; seeded U(-1,1) inputs (seed
+    1234)."""
     import pibem_amd as P
     from pibem_amd import hip
     rng = np.random.default_rng(1234)
@@ -188,14 +234,7 @@ def time_matvecs(f, n, stream, reps=50, phase_reps=10):
         ms = e0.elapsed_ms(e1) / reps
         mv[name] = dict(ms=ms, per_s=1e3 / ms)
         outs[name] = (oN.numpy(), oD.numpy())
-        f.set_phase_timing(True)
-        acc = None
-        for _ in range(phase_reps):
-            f.matvec_dev(a.value, b.value, oN.value, oD.value, prec)
-            t = f.last_timings()
-            acc = t if acc is None else {k: acc[k] + t[k] for k in t}
-        f.set_phase_timing(False)
-        phases[name] = {k: v / phase_reps for k, v in acc.items()}
+        phases[name] = time_phases(f, lambda: f.matvec_dev(a.value, b.value, oN.value, oD.value, prec), phase_reps)
         log(f"matvec {name}: {ms:.3f} ms; phases " + " ".join(f"{k}={v:.3f}" for k, v in phases[name].items()))
     rel = {k: float(np.linalg.norm(outs["fp32"][i] - outs["fp64"][i]) / np.linalg.norm(outs["fp64"][i]))
            for i, k in enumerate(("N", "D"))}
@@ -212,12 +251,15 @@ def measure_config(config, steps, warmup):
     log("setup: " + " ".join(f"{k}={v:.2f}" for k, v in setup.items()))
     wc = f.work_counts()
     solves = time_solves(f, bp, pr, stream, steps, warmup)
+    op_rate, op_phases = time_operator(f, bp, n, stream)
     mv, phases, rel = time_matvecs(f, n, stream)
-    model = {"fp32": work_model(wc, 4), "fp64": work_model(wc, 8)}
-    roof = {prec: {ph: roofline_of(ph, model[prec], phases[prec][ph], prec, pr.p) for ph in PHASES}
-            for prec in phases}
-    return dict(pr=pr, f=f, stream=stream, wc=wc, setup=setup, solves=solves, matvec=mv, phases=phases,
-                fp32_vs_fp64_matvec=rel, roofline=roof)
+    roof = {prec: {ph: roofline_of(ph, work_model(wc, 4 if prec == "fp32" else 8, 1), op_phases[prec][ph], prec,
+                                   pr.p, 1) for ph in PHASES} for prec in op_phases}
+    roof2 = {prec: {ph: roofline_of(ph, work_model(wc, 4 if prec == "fp32" else 8, 2), phases[prec][ph], prec,
+                                    pr.p, 2) for ph in PHASES} for prec in phases}
+    return dict(pr=pr, f=f, stream=stream, wc=wc, setup=setup, solves=solves, operator=op_rate,
+                operator_phases=op_phases, matvec=mv, phases=phases, fp32_vs_fp64_matvec=rel, roofline=roof,
+                roofline_matvec=roof2)
 
 
 def cpu_baseline(pr, ops, seconds):
@@ -257,8 +299,31 @@ def mode_matvec(args):
     prec = P.PBFMM_MIXED if args.precision == "fp32" else P.PBFMM_FP64
     for _ in range(args.steps):
         f.matvec_dev(a.value, b.value, oN.value, oD.value, prec)
+    bp = P.BEMProblem(f)
+    for _ in range(args.steps):
+        bp.vmult_dev(a.value, oN.value, prec)
     stream.synchronize()
-    log(f"{args.steps} {args.precision} matvecs at config {args.config} done")
+    log(f"{args.steps} {args.precision} FMM matvecs and {args.steps} operator applications at config "
+        f"{args.config} done")
+
+
+def mode_solve(args):
+    import pibem_amd as P
+    from pibem_amd import hip
+    pr = make_problem(args.config)
+    stream = hip.Stream()
+    f, bp, _ = setup_handle(pr, stream)
+    n = pr.mesh.n_dofs
+    rhs = hip.DeviceArray.from_numpy(bp.compute_rhs(pr.tmp_rhs))
+    x = hip.DeviceArray(n)
+    zero = hip.DeviceArray(n)
+    prec = P.PBFMM_MIXED if args.precision == "fp32" else P.PBFMM_FP64
+    for _ in range(args.steps):
+        x.copy_from(zero)
+        its, rr = f.gmres_dev(rhs.value, x.value, 1e-8, 100, 2000, prec)
+    stream.synchronize()
+    log(f"{args.steps} {args.precision} solves at config {args.config} done: {its} operator applications, "
+        f"relres {rr:.3e}")
 
 
 def mode_copy(args):
@@ -280,7 +345,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config3", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--mode", choices=("full", "matvec", "copy"), default="full")
+    ap.add_argument("--mode", choices=("full", "matvec", "solve", "copy"), default="full")
     ap.add_argument("--config", type=int, choices=(2, 3), default=2)
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--mib", type=int, default=1024)
@@ -297,11 +362,13 @@ def main():
         return mode_matvec(args)
     if args.mode == "copy":
         return mode_copy(args)
+    if args.mode == "solve":
+        return mode_solve(args)
 
     c2 = measure_config(2, args.steps, args.warmup)
     wc = c2["wc"]
-    # dominant kernel of the headline (FP32 operator) matvec
-    ph = c2["phases"]["fp32"]
+    # dominant kernel of the headline solve's FP32 operator
+    ph = c2["operator_phases"]["fp32"]
     dom = max(PHASES, key=lambda k: ph[k])
     roof = dict(c2["roofline"]["fp32"][dom])
     roof["traffic"] = traffic_for(roof["kernel"], "config 2")
@@ -329,11 +396,13 @@ def main():
         "fp64_ms_per_solve": c2["solves"]["fp64"]["ms"],
         "mixed_speedup_vs_fp64": c2["solves"]["fp64"]["ms"] / c2["solves"]["mixed"]["ms"],
         "solves": c2["solves"],
+        "operator_applications_per_s": {k: v["per_s"] for k, v in c2["operator"].items()},
         "fmm_matvecs_per_s": {k: v["per_s"] for k, v in c2["matvec"].items()},
         "spmv_gbs": {k: sp[k]["spmv"]["achieved"] for k in sp},
         "m2l_tflops": {k: sp[k]["m2l"]["achieved"] for k in sp},
         "fp32_vs_fp64_matvec": c2["fp32_vs_fp64_matvec"],
-        "phase_ms": c2["phases"],
+        "operator_phase_ms": c2["operator_phases"],
+        "matvec_phase_ms": c2["phases"],
         "phase_roofline": {k: {ph_: {kk: v[kk] for kk in ("kernel", "achieved", "unit", "frac")}
                                for ph_, v in sp[k].items()} for k in sp},
         "setup_s": c2["setup"],
@@ -356,11 +425,13 @@ def main():
             "fp64_ms_per_solve": c3["solves"]["fp64"]["ms"],
             "mixed_speedup_vs_fp64": c3["solves"]["fp64"]["ms"] / c3["solves"]["mixed"]["ms"],
             "solves": c3["solves"],
+            "operator_applications_per_s": {k: v["per_s"] for k, v in c3["operator"].items()},
             "fmm_matvecs_per_s": {k: v["per_s"] for k, v in c3["matvec"].items()},
             "spmv_gbs": {k: r3[k]["spmv"]["achieved"] for k in r3},
             "m2l_tflops": {k: r3[k]["m2l"]["achieved"] for k in r3},
             "fp32_vs_fp64_matvec": c3["fp32_vs_fp64_matvec"],
-            "phase_ms": c3["phases"],
+            "operator_phase_ms": c3["operator_phases"],
+            "matvec_phase_ms": c3["phases"],
             "phase_roofline": {k: {ph_: {kk: v[kk] for kk in ("kernel", "achieved", "unit", "frac")}
                                    for ph_, v in r3[k].items()} for k in r3},
             "setup_s": c3["setup"],

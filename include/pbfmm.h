@@ -4,7 +4,7 @@
  * Drop-in boundary for pi-BEM's fast multipole path: the BEMFMA<3> method set
  * (include/bem_fma.h:74-202) and the FMA branch of BEMProblem<3>
  * (include/bem_problem.h:131-242), re-expressed with plain pointers and sizes
- * so that a thin C++ adapter can forward to it from the
+ * so that a thin C++ adapter class BEMFMA<3> can forward to it from the
  * existing Driver.  Every call returns 0 on success and a negative status on
  * failure; pbfmm_last_error() then holds the message (the reference reports
  * the same conditions through AssertThrow / ExcMessage exceptions).
@@ -149,6 +149,12 @@ int pbfmm_solve_system(pbfmm_handle *h, const double *tmp_rhs, double *phi, doub
  * PBFMM_MIXED (FP32 operator, FP64 in/out). */
 int pbfmm_matvec_dev(pbfmm_handle *h, const double *d_phi, const double *d_dphi, double *d_outN,
                      double *d_outD, int32_t precision);
+
+/* Device-pointer BEMProblem<3>::vmult (same operator as pbfmm_vmult, with
+ * constraints) on resident vectors: the FMM of N.src - D.src runs with one
+ * combined expansion per block.  precision PBFMM_FP64, or PBFMM_MIXED for the
+ * FP32 operator (FP64 in/out); src and dst must differ. */
+int pbfmm_vmult_dev(pbfmm_handle *h, const double *d_src, double *d_dst, int32_t precision);
 
 /* Device-pointer solve: rhs_dev from pbfmm_compute_rhs semantics already
  * applied (constrained rows set), x_dev initial guess in, solution out. */

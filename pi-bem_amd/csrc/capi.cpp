@@ -254,7 +254,7 @@ int pbfmm_generate_multipole_expansions(pbfmm_handle *h, const double *phi, cons
     Engine &e = h->eng;
     h2d(e, e.d_in.as<double>(), phi, e.n);
     h2d(e, e.d_out.as<double>(), dphi, e.n);
-    e.generate<double>(e.d_in.as<double>(), e.d_out.as<double>());
+    e.generate<double, 2>(e.d_in.as<double>(), e.d_out.as<double>());
     e.sync();
     return PBFMM_OK;
   });
@@ -268,7 +268,7 @@ int pbfmm_multipole_matr_vect_products(pbfmm_handle *h, const double *phi, const
     Engine &e = h->eng;
     h2d(e, e.d_in.as<double>(), phi, e.n);
     h2d(e, e.d_out.as<double>(), dphi, e.n);
-    e.products<double>(e.d_in.as<double>(), e.d_out.as<double>(), e.d_outN.as<double>(), e.d_outD.as<double>());
+    e.products<double, 2>(e.d_in.as<double>(), e.d_out.as<double>(), e.d_outN.as<double>(), e.d_outD.as<double>());
     d2h(e, outN, e.d_outN.as<double>(), e.n);
     d2h(e, outD, e.d_outD.as<double>(), e.n);
     return PBFMM_OK;
@@ -427,6 +427,21 @@ int pbfmm_matvec_dev(pbfmm_handle *h, const double *phi, const double *dphi, dou
     } else {
       e.matvec<double>(phi, dphi, outN, outD);
     }
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_vmult_dev(pbfmm_handle *h, const double *src, double *dst, int32_t precision) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "vmult_dev")) return rc;
+    if (!src || !dst) return fail(h, PBFMM_ERR_ARG, "vmult_dev: null vector");
+    if (src == dst) return fail(h, PBFMM_ERR_ARG, "vmult_dev: src and dst must differ");
+    if (precision != PBFMM_FP64 && precision != PBFMM_MIXED)
+      return fail(h, PBFMM_ERR_ARG, "vmult_dev: unknown precision");
+    if (precision == PBFMM_MIXED && !h->eng.have_f32)
+      return fail(h, PBFMM_ERR_STATE, "vmult_dev: handle was not created with PBFMM_MIXED");
+    ensure_alpha(h);
+    h->eng.vmult(src, dst, precision);
     return PBFMM_OK;
   });
 }

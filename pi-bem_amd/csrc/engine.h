@@ -29,7 +29,7 @@ struct HipError : std::runtime_error {
 
 // Chunk list of one batched translation (see FarPlan::ChunkList) on the device.
 struct ChunkDev {
-  DevBuf op, beg, cnt, tgt, src;
+  DevBuf op, beg, cnt, op1, beg1, cnt1, tgt, src;
 };
 
 // Per-precision copies of the far-field operators and expansion storage.
@@ -106,9 +106,13 @@ struct Engine {
   void assemble_near(bool want_f32);
   template <typename T>
   void matvec(const double *phi, const double *dphi, double *outN, double *outD);
+  // the BEMProblem operator's FMM: out = N.phi - D.dphi with one combined
+  // expansion per block
   template <typename T>
+  void matvec_combined(const double *phi, const double *dphi, double *out);
+  template <typename T, int NK>
   void generate(const double *phi, const double *dphi);
-  template <typename T>
+  template <typename T, int NK>
   void products(const double *phi, const double *dphi, double *outN, double *outD);
   // BEMProblem operator (device vectors of n doubles)
   void vmult(const double *src, double *dst, int precision);

@@ -245,7 +245,9 @@ template <> struct Vec2<float> { using type = float2; };
 constexpr int kSpmvTile = 2048;
 constexpr int kXcd = 8;
 
-template <typename T>
+// NK = 2 writes N.phi to outN and D.dphi to outD; NK = 1 writes the
+// combination N.phi - D.dphi to outN (outD unused).
+template <typename T, int NK>
 __global__ void __launch_bounds__(256) k_spmv(int nleaf, int per_xcd, const int *__restrict__ row_beg,
                                               const int *__restrict__ rows, const int *__restrict__ col_beg,
                                               const int *__restrict__ cols, const int64_t *__restrict__ val_beg,
@@ -282,8 +284,12 @@ __global__ void __launch_bounds__(256) k_spmv(int nleaf, int per_xcd, const int 
       if (lane == 0) {
         const int node = rows[r0 + r];
         // the wave that owns row r owns it in every tile: plain read-modify-write
-        outN[node] = (c0 == 0 ? 0.0 : outN[node]) + (double)aN;
-        outD[node] = (c0 == 0 ? 0.0 : outD[node]) + (double)aD;
+        if constexpr (NK == 2) {
+          outN[node] = (c0 == 0 ? 0.0 : outN[node]) + (double)aN;
+          outD[node] = (c0 == 0 ? 0.0 : outD[node]) + (double)aD;
+        } else {
+          outN[node] = (c0 == 0 ? 0.0 : outN[node]) + ((double)aN - (double)aD);
+        }
       }
     }
   }
@@ -367,27 +373,47 @@ __global__ void __launch_bounds__(256) k_p2m_unit(
   }
 }
 
-// Per matvec: one workgroup per leaf.  Output o of the leaf's 2 kp
-// coefficients (kernel o / kp) is owned by virtual thread v = ph * 2kp + o,
-// which sums the columns ph, ph + nph, ...; the leaf's (phi, dphi) values are
-// gathered into LDS once per tile of columns.
+// Per matvec: one workgroup per leaf.  The leaf's 2 kp outputs are read as
+// NOV 16-byte vectors per column; thread t owns vector o = t % NOV over the
+// columns ph, ph + nph, ... (ph = t / NOV, nph = 256 / NOV column phases), and
+// the phases are summed through LDS at the end.  The leaf's (phi, dphi) values
+// are gathered into LDS once per tile of columns.
 constexpr int kP2mTile = 1024;
-template <typename T>
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { using type = float4; };
+template <> struct Vec16<double> { using type = double2; };
+__device__ __forceinline__ void axpy16(float4 &a, float x, const float4 &u) {
+  a.x += x * u.x; a.y += x * u.y; a.z += x * u.z; a.w += x * u.w;
+}
+__device__ __forceinline__ void axpy16(double2 &a, double x, const double2 &u) {
+  a.x += x * u.x; a.y += x * u.y;
+}
+__device__ __forceinline__ void add16(float4 &a, const float4 &u) { a.x += u.x; a.y += u.y; a.z += u.z; a.w += u.w; }
+__device__ __forceinline__ void add16(double2 &a, const double2 &u) { a.x += u.x; a.y += u.y; }
+
+// NK = 1: one combined expansion per leaf, sum_columns (phi U1 - dphi U2),
+// stored [leaf][kp].
+template <typename T, int NK>
 __global__ void __launch_bounds__(256) k_p2m(int nleaf, const int *__restrict__ leaves,
                                              const int *__restrict__ col_beg, const int *__restrict__ col_dof,
                                              const T *__restrict__ U, int kp, const double *__restrict__ phi,
                                              const double *__restrict__ dphi, T *__restrict__ ME) {
   using T2 = typename Vec2<T>::type;
+  using TV = typename Vec16<T>::type;
+  constexpr int VW = 16 / sizeof(T);
   __shared__ T2 xs[kP2mTile];
-  __shared__ T part[512];
+  __shared__ TV part[256];
   const int li = blockIdx.x;
   const int t = threadIdx.x;
-  const int NO = 2 * kp;                       // <= 288 for p <= 10
-  const int nph = NO <= 256 ? 256 / NO : 1;    // column phases
-  const int nv = nph * NO;                     // virtual threads (<= 512)
+  const int NO = 2 * kp, NOV = NO / VW;  // kp is a multiple of 16: NOV <= 144
+  const int NOUT = NK * kp / VW;          // output vectors
+  const int nph = 256 / NOUT;
+  const int o = t % NOUT, ph = t / NOUT;
+  const bool active = ph < nph;
+  const bool k2 = o * VW >= kp;          // a vector never straddles the two kernels
   const int c0 = col_beg[li], C = col_beg[li + 1] - c0;
-  const T *Ub = U + (int64_t)c0 * NO;
-  T acc[2] = {T(0), T(0)};
+  const TV *Ub = reinterpret_cast<const TV *>(U + (int64_t)c0 * NO) + o;
+  TV acc{};
   for (int cb = 0; cb < C; cb += kP2mTile) {
     const int CT = min(kP2mTile, C - cb);
     if (cb > 0) __syncthreads();
@@ -396,37 +422,26 @@ __global__ void __launch_bounds__(256) k_p2m(int nleaf, const int *__restrict__ 
       xs[c] = T2{(T)phi[d], (T)dphi[d]};
     }
     __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int v = t + 256 * s;
-      if (v < nv) {
-        const int o = v % NO, ph = v / NO;
-        const bool k2 = o >= kp;
-        const T *u = Ub + (int64_t)cb * NO + o;
-        T a = acc[s];
-        for (int c = ph; c < CT; c += nph) {
-          const T2 xv = xs[c];
-          a += (k2 ? xv.y : xv.x) * u[(int64_t)c * NO];
+    if (active) {
+#pragma unroll 4
+      for (int c = ph; c < CT; c += nph) {
+        const T2 xv = xs[c];
+        const TV *u = Ub + (int64_t)(cb + c) * NOV;
+        if constexpr (NK == 2) {
+          axpy16(acc, k2 ? xv.y : xv.x, u[0]);
+        } else {
+          axpy16(acc, xv.x, u[0]);
+          axpy16(acc, -xv.y, u[NOUT]);
         }
-        acc[s] = a;
       }
     }
   }
-  T *out = ME + (int64_t)leaves[li] * NO;
-  if (nph == 1) {
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int v = t + 256 * s;
-      if (v < NO) out[v] = acc[s];
-    }
-    return;
-  }
-  part[t] = acc[0];  // nv <= 256 here
+  part[t] = acc;
   __syncthreads();
-  if (t < NO) {
-    T a = part[t];
-    for (int ph = 1; ph < nph; ++ph) a += part[ph * NO + t];
-    out[t] = a;
+  if (t < NOUT) {
+    TV a = part[t];
+    for (int q = 1; q < nph; ++q) add16(a, part[q * NOUT + t]);
+    reinterpret_cast<TV *>(ME + (int64_t)leaves[li] * NK * kp)[t] = a;
   }
 }
 
@@ -447,9 +462,9 @@ struct Kp {
 // KP x KP operator per class (M2M / L2L: level and child octant; M2L: level
 // and integer offset).  Pairs are grouped by class into chunks of 32; a block
 // computes C (KP x 64) = A (KP x KP) . B (KP x 64 source columns, both
-// kernels) with 16x16x4 MFMA tiles (wave w: columns 16w..16w+15, all KP/16
-// row tiles), stages C through LDS and adds it to the targets with coalesced
-// atomics (targets are distinct within a chunk, not across chunks).
+// kernels) with 16x16x4 MFMA tiles (KP/16 wavefronts, wave w: row tile w,
+// all 64 columns), stages C through LDS and adds it to the targets with
+// coalesced atomics (targets are distinct within a chunk, not across chunks).
 //   f32 (v_mfma_f32_16x16x4f32): lane l holds A[l%16][l/16], B[l/16][l%16],
 //       D[4(l/16)+j][l%16], j < 4
 //   f64 (v_mfma_f64_16x16x4f64): same inputs, D[l/16+4j][l%16]
@@ -473,76 +488,126 @@ template <> struct Mfma<double> {
   static __device__ __forceinline__ int row(int l, int j) { return (l >> 4) + 4 * j; }
 };
 
-static_assert(kM2LChunk == 32, "k_translate maps 2 x 32 pairs onto 64 GEMM columns");
+static_assert(kM2LChunk == 32 && kChunk1 == 64, "k_translate maps 2 x 32 or 1 x 64 pairs onto 64 GEMM columns");
 
-// LDS tile (elements of T): B as [64 columns][KP + BP] (a source vector per
-// column, padded so that both the coalesced fill and the MFMA reads are free
-// of bank conflicts), reused after the GEMM for C as [64][KP + 1].
+// LDS tile (elements of T).  B is [64 columns][BS] (one source vector per
+// column), C is [64 columns][CS], both column-major, sharing one buffer.
+//   FP32: BS = KP + 2, so that the MFMA reads (16 columns x 2 k-rows per half
+//   wavefront) hit 32 distinct banks; filled with 8-byte stores.  CS = KP + 4:
+//   an FP32 MFMA lane holds 4 consecutive rows of one column, stored as one
+//   16-byte write, and 8 such writes cover the 32 banks once.
+//   FP64: BS = KP + 2 (16-byte fill stores), CS = KP + 1.
 template <int P, typename T>
 struct TrLayout {
   static constexpr int KP = Kp<P>::value;
-  static constexpr int BP = sizeof(T) == 4 ? 4 : 2;
-  static constexpr int BS = KP + BP;
-  static constexpr size_t bytes = (size_t)64 * BS * sizeof(T);
+  static constexpr int BS = KP + 2;
+  static constexpr int CS = sizeof(T) == 4 ? KP + 4 : KP + 1;
+  static constexpr int NR = KP / 16;        // 16-row tiles = wavefronts per block
+  static constexpr int NT = 64 * NR;        // threads per block
+  static constexpr int elems = 64 * (BS > CS ? BS : CS);
+  static constexpr size_t bytes = (size_t)elems * sizeof(T);
 };
 
+// Wavefront w owns output rows 16w..16w+15 for all 64 columns (4 column
+// groups), so each operator fragment it loads from L2 feeds 4 MFMAs.  The
+// source vectors of the chunk (NK kp contiguous values per pair) are gathered
+// with 16-byte loads, all issued before the LDS stores.  NK = 2: both
+// kernels' expansions, 32 pairs x 2 columns; NK = 1: one combined expansion,
+// 64 pairs.
 // KIND (0 M2M, 1 M2L, 2 L2L) only names the instantiation, so that traces and
 // profiles tell the three passes apart.
-template <typename T, int P, int KIND>
-__global__ void __launch_bounds__(256) k_translate(const int *__restrict__ chunk_op,
-                                                   const int *__restrict__ chunk_beg,
-                                                   const int *__restrict__ chunk_cnt, const int *__restrict__ tgt,
-                                                   const int *__restrict__ src, const T *__restrict__ ops,
-                                                   const T *__restrict__ IN, T *__restrict__ OUT) {
+template <typename T, int P, int KIND, int NK>
+__global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(const int *__restrict__ chunk_op,
+                                                                  const int *__restrict__ chunk_beg,
+                                                                  const int *__restrict__ chunk_cnt,
+                                                                  const int *__restrict__ tgt,
+                                                                  const int *__restrict__ src,
+                                                                  const T *__restrict__ ops,
+                                                                  const T *__restrict__ IN, T *__restrict__ OUT) {
   using Lay = TrLayout<P, T>;
-  constexpr int KP = Kp<P>::value, K2 = 2 * KP, KT2 = 2 * Kp<P>::kt, NR = KP / 16, BS = Lay::BS;
+  constexpr int KP = Lay::KP, K2 = NK * KP, KT2 = 2 * Kp<P>::kt, BS = Lay::BS, CS = Lay::CS, NT = Lay::NT;
+  constexpr int PC = 64 / NK;                // pairs per chunk
+  static_assert(NT == 4 * KP, "launch bound");
+  static_assert(sizeof(T) == 8 || (BS % 2 == 0 && CS % 4 == 0), "LDS vector alignment");
   using M = Mfma<T>;
-  __shared__ T sB[64 * BS];
-  __shared__ int sT[kM2LChunk], sS[kM2LChunk];
+  using TV = typename Vec16<T>::type;
+  constexpr int V = 16 / sizeof(T);          // elements per 16-byte vector
+  constexpr int NVP = K2 / V;                // vectors per pair (both kernels)
+  constexpr int NV = PC * NVP;               // vectors per chunk
+  constexpr int PER = (NV + NT - 1) / NT;    // vectors per thread
+  __shared__ __align__(16) T sB[Lay::elems];
+  __shared__ int sT[PC], sS[PC];
   const int c = blockIdx.x;
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
   const T *A = ops + (int64_t)chunk_op[c] * KP * KP;  // column-major, read from L2
   const int beg = chunk_beg[c], cnt = chunk_cnt[c];
-  if (t < kM2LChunk) {
+  if (t < PC) {
     sT[t] = t < cnt ? tgt[beg + t] : -1;
     sS[t] = t < cnt ? src[beg + t] : -1;
   }
   __syncthreads();
-  for (int i = t; i < 64 * KP; i += 256) {
-    const int col = i / KP, k = i - col * KP, pr = col >> 1, ker = col & 1;
-    sB[col * BS + k] = pr < cnt ? IN[(int64_t)sS[pr] * K2 + ker * KP + k] : T(0);
+  TV buf[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = t + j * NT;
+    const int pr = i / NVP;
+    buf[j] = TV{};
+    if (i < NV && pr < cnt)
+      buf[j] = *reinterpret_cast<const TV *>(IN + (int64_t)sS[pr] * K2 + (i - pr * NVP) * V);
+  }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int i = t + j * NT;
+    if (i < NV) {
+      const int pr = i / NVP, off = (i - pr * NVP) * V;
+      const int col = NK * pr + (off >= KP), k = off >= KP ? off - KP : off;
+      T *d = sB + col * BS + k;
+      if constexpr (sizeof(T) == 4) {  // 8-byte aligned: two 8-byte stores
+        reinterpret_cast<float2 *>(d)[0] = float2{buf[j].x, buf[j].y};
+        reinterpret_cast<float2 *>(d)[1] = float2{buf[j].z, buf[j].w};
+      } else {
+        *reinterpret_cast<TV *>(d) = buf[j];
+      }
+    }
   }
   __syncthreads();
-  typename M::acc acc[NR];
+  typename M::acc acc[4];
 #pragma unroll
-  for (int r = 0; r < NR; ++r) acc[r] = typename M::acc{0, 0, 0, 0};
-  const int col = 16 * w + (l & 15);
-  const T *bk = sB + col * BS + (l >> 4);
-  const T *ak = A + (l >> 4) * KP + (l & 15);
+  for (int g = 0; g < 4; ++g) acc[g] = typename M::acc{0, 0, 0, 0};
+  const T *ak = A + (l >> 4) * KP + 16 * w + (l & 15);
+  const T *bk = sB + (l & 15) * BS + (l >> 4);
 #pragma unroll 4
   for (int k0 = 0; k0 < KP; k0 += 4) {
-    const T b = bk[k0];
+    const T a = ak[k0 * KP];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = M::mac(ak[k0 * KP + 16 * r], b, acc[r]);
+    for (int g = 0; g < 4; ++g) acc[g] = M::mac(a, bk[g * 16 * BS + k0], acc[g]);
   }
-  __syncthreads();  // B tile no longer needed: reuse it for C, column-major [64][KP + 1]
+  __syncthreads();  // B tile no longer needed: reuse the buffer for C
   T *sC = sB;
+  if constexpr (sizeof(T) == 4) {
 #pragma unroll
-  for (int r = 0; r < NR; ++r)
+    for (int g = 0; g < 4; ++g)
+      *reinterpret_cast<float4 *>(sC + (16 * g + (l & 15)) * CS + 16 * w + 4 * (l >> 4)) =
+          float4{acc[g][0], acc[g][1], acc[g][2], acc[g][3]};
+  } else {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) sC[col * (KP + 1) + 16 * r + M::row(l, j)] = acc[r][j];
+    for (int g = 0; g < 4; ++g)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sC[(16 * g + (l & 15)) * CS + 16 * w + M::row(l, j)] = acc[g][j];
+  }
   __syncthreads();
   // coalesced accumulation: consecutive lanes add consecutive rows of one target
-  for (int i = t; i < 2 * cnt * KT2; i += 256) {
+  for (int i = t; i < NK * cnt * KT2; i += NT) {
     const int cc = i / KT2, row = i - cc * KT2;
-    atomicAdd(OUT + (int64_t)sT[cc >> 1] * K2 + (cc & 1) * KP + row, sC[cc * (KP + 1) + row]);
+    atomicAdd(OUT + (int64_t)sT[cc / NK] * K2 + (cc % NK) * KP + row, sC[cc * CS + row]);
   }
 }
 
 // Evaluation of both kernels' expansions at a point: M2P
 // (MultipoleExpansion::Evaluate, multipole_expansion.cc:264-298, rho^(-n-1))
 // or L2P (LocalExpansion::Evaluate, local_expansion.cc:251-287, rho^n).
-template <typename T, bool LOCAL>
+// NK = 1: the single expansion E (v2 unused).
+template <typename T, bool LOCAL, int NK>
 __device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, int kp, T d0, T d1, T d2,
                                       T &v1, T &v2) {
   const T rho = dsqrt(d0 * d0 + d1 * d1 + d2 * d2);
@@ -576,13 +641,13 @@ __device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, in
       const int idx = n * (n + 1) / 2 + m;
       const T f = pc * r * f2;
       v1 += (E[idx] * cm - E[kt + idx] * sm) * f;
-      v2 += (E[kp + idx] * cm - E[kp + kt + idx] * sm) * f;
+      if constexpr (NK == 2) v2 += (E[kp + idx] * cm - E[kp + kt + idx] * sm) * f;
     }
   }
 }
 
 // M2P for smaller far blocks (bem_fma.cc:2664-2704): one thread per (pair, node).
-template <typename T>
+template <typename T, int NK>
 __global__ void __launch_bounds__(256) k_m2p(int npairs, const int64_t *__restrict__ work_beg,
                                              const int *__restrict__ tgt, const int *__restrict__ src,
                                              const int *__restrict__ node_beg, const int *__restrict__ nperm,
@@ -606,13 +671,13 @@ __global__ void __launch_bounds__(256) k_m2p(int npairs, const int64_t *__restri
   const T d1 = (T)(xyz[3 * node + 1] - (pmin[3 * sblk + 1] + hd));
   const T d2 = (T)(xyz[3 * node + 2] - (pmin[3 * sblk + 2] + hd));
   T v1, v2;
-  eval2<T, false>(ME + (int64_t)sblk * 2 * kp, p, kt, kp, d0, d1, d2, v1, v2);
+  eval2<T, false, NK>(ME + (int64_t)sblk * NK * kp, p, kt, kp, d0, d1, d2, v1, v2);
   atomicAdd(outN + node, (double)v1);
-  atomicAdd(outD + node, (double)v2);
+  if constexpr (NK == 2) atomicAdd(outD + node, (double)v2);
 }
 
 // L2P at every node of every leaf (bem_fma.cc:2959-2986): one thread per node.
-template <typename T>
+template <typename T, int NK>
 __global__ void __launch_bounds__(256) k_l2p(int nrows, const int *__restrict__ rows,
                                              const int *__restrict__ row_leaf, const int *__restrict__ leaf_block,
                                              const double *__restrict__ xyz, const double *__restrict__ pmin,
@@ -628,9 +693,9 @@ __global__ void __launch_bounds__(256) k_l2p(int nrows, const int *__restrict__ 
   const T d1 = (T)(xyz[3 * node + 1] - (pmin[3 * b + 1] + hd));
   const T d2 = (T)(xyz[3 * node + 2] - (pmin[3 * b + 2] + hd));
   T v1, v2;
-  eval2<T, true>(LE + (int64_t)b * 2 * kp, p, kt, kp, d0, d1, d2, v1, v2);
+  eval2<T, true, NK>(LE + (int64_t)b * NK * kp, p, kt, kp, d0, d1, d2, v1, v2);
   outN[node] += (double)v1;
-  outD[node] += (double)v2;
+  if constexpr (NK == 2) outD[node] += (double)v2;
 }
 
 template <typename T>
@@ -651,11 +716,12 @@ __global__ void k_vmult_pre(int64_t n, const double *__restrict__ src, const uin
   sp[i] = dir[i] ? 0.0 : (src[i] - sh);
   sd[i] = dir[i] ? src[i] : 0.0;
 }
-__global__ void k_vmult_post(int64_t n, const double *__restrict__ N, const double *__restrict__ D,
-                             const double *__restrict__ alpha, const double *__restrict__ sp, double *__restrict__ dst) {
+// ND = N.sp - D.sd from the combined-kernel FMM
+__global__ void k_vmult_post(int64_t n, const double *__restrict__ ND, const double *__restrict__ alpha,
+                             const double *__restrict__ sp, double *__restrict__ dst) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  dst[i] = -D[i] + N[i] + alpha[i] * sp[i];
+  dst[i] = ND[i] + alpha[i] * sp[i];
 }
 __global__ void k_rhs_pre(int64_t n, const double *__restrict__ g, const uint8_t *__restrict__ dir,
                           double *__restrict__ sp, double *__restrict__ sd) {
@@ -664,11 +730,12 @@ __global__ void k_rhs_pre(int64_t n, const double *__restrict__ g, const uint8_t
   sp[i] = dir[i] ? g[i] : 0.0;
   sd[i] = dir[i] ? 0.0 : g[i];
 }
-__global__ void k_rhs_post(int64_t n, const double *__restrict__ N, const double *__restrict__ D,
-                           const double *__restrict__ alpha, const double *__restrict__ sp, double *__restrict__ dst) {
+// ND = N.sp - D.sd from the combined-kernel FMM
+__global__ void k_rhs_post(int64_t n, const double *__restrict__ ND, const double *__restrict__ alpha,
+                           const double *__restrict__ sp, double *__restrict__ dst) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  dst[i] = -(N[i] + alpha[i] * sp[i]) + D[i];
+  dst[i] = -(ND[i] + alpha[i] * sp[i]);
 }
 __global__ void k_shift(int64_t n, double *__restrict__ v, const double *__restrict__ s) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -955,6 +1022,9 @@ static void upload_chunks(ChunkDev &d, const ChunkList &c) {
   up(d.op, c.op);
   up(d.beg, c.beg);
   up(d.cnt, c.cnt);
+  up(d.op1, c.op1);
+  up(d.beg1, c.beg1);
+  up(d.cnt1, c.cnt1);
   up(d.tgt, c.tgt);
   up(d.src, c.src);
 }
@@ -1064,17 +1134,19 @@ FarDev<double> &far_of<double>(Engine &e) { return e.f64; }
 template <>
 FarDev<float> &far_of<float>(Engine &e) { return e.f32; }
 
-// OUT[tgt] += ops[class] . IN[src] for the chunks [c0, c1) of a chunk list.
-template <typename T, int KIND>
+// OUT[tgt] += ops[class] . IN[src] for the chunks [c0, c1) of a chunk list
+// (chunks of 32 pairs for NK = 2, of 64 pairs for NK = 1).
+template <typename T, int KIND, int NK>
 static void launch_translate(Engine &e, hipStream_t st, const ChunkDev &cl, int c0, int c1, const T *ops,
                              const T *IN, T *OUT) {
   const int n = c1 - c0;
   if (n <= 0) return;
+  const DevBuf &op = NK == 2 ? cl.op : cl.op1, &beg = NK == 2 ? cl.beg : cl.beg1, &cnt = NK == 2 ? cl.cnt : cl.cnt1;
 #define PBFMM_TR(PP)                                                                                         \
   case PP:                                                                                                   \
-    hipLaunchKernelGGL((k_translate<T, PP, KIND>), dim3(n), dim3(256), 0, st, cl.op.as<int>() + c0,         \
-                       cl.beg.as<int>() + c0, cl.cnt.as<int>() + c0, cl.tgt.as<int>(), cl.src.as<int>(), ops,  \
-                       IN, OUT);                                                                             \
+    hipLaunchKernelGGL((k_translate<T, PP, KIND, NK>), dim3(n), dim3(TrLayout<PP, T>::NT), 0, st,           \
+                       op.as<int>() + c0, beg.as<int>() + c0, cnt.as<int>() + c0, cl.tgt.as<int>(),          \
+                       cl.src.as<int>(), ops, IN, OUT);                                                      \
     break;
   switch (e.p) {
     PBFMM_TR(1) PBFMM_TR(2) PBFMM_TR(3) PBFMM_TR(4) PBFMM_TR(5)
@@ -1084,62 +1156,73 @@ static void launch_translate(Engine &e, hipStream_t st, const ChunkDev &cl, int 
 #undef PBFMM_TR
 }
 
-template <typename T>
+// Chunk range of level l in a chunk list, for the NK-expansion chunking.
+template <int NK>
+static inline int lvl(const ChunkList &c, int l) { return NK == 2 ? c.level_off[l] : c.level_off1[l]; }
+template <int NK>
+static inline int nchunks(const ChunkList &c) { return (int)(NK == 2 ? c.op.size() : c.op1.size()); }
+
+// Upward pass: leaf P2M, then M2M level by level.  NK = 2 keeps the two
+// kernels' expansions ([block][2][kp]); NK = 1 one combined expansion
+// ME1(phi) - ME2(dphi) per block ([block][kp]).
+template <typename T, int NK>
 void Engine::generate(const double *phi, const double *dphi) {
   FarDev<T> &F = far_of<T>(*this);
   const int nleaf = (int)far.p2m_leaf.size();
   mark(10);
-  const int64_t nme = (int64_t)tree.nb * 2 * kp;
+  const int64_t nme = (int64_t)tree.nb * NK * kp;
   hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(nme, 256), 4096)), dim3(256), 0, stream,
                      F.me.template as<T>(), nme, T(0));
   mark(0);
   if (nleaf > 0)
-    hipLaunchKernelGGL(k_p2m<T>, dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
+    hipLaunchKernelGGL((k_p2m<T, NK>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
                        d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.unit.template as<T>(), kp, phi, dphi,
                        F.me.template as<T>());
   mark(1);
   for (int l = L; l >= 1; --l)
-    launch_translate<T, 0>(*this, stream, m2m_cl, far.m2m.level_off[l], far.m2m.level_off[l + 1],
-                        F.m2m.template as<T>(), F.me.template as<T>(), F.me.template as<T>());
+    launch_translate<T, 0, NK>(*this, stream, m2m_cl, lvl<NK>(far.m2m, l), lvl<NK>(far.m2m, l + 1),
+                               F.m2m.template as<T>(), F.me.template as<T>(), F.me.template as<T>());
   mark(2);
   HIPCHK(hipGetLastError());
 }
 
-template <typename T>
+// Downward pass and near field.  NK = 2: outN = N.phi, outD = D.dphi;
+// NK = 1: outN = N.phi - D.dphi (outD unused).
+template <typename T, int NK>
 void Engine::products(const double *phi, const double *dphi, double *outN, double *outD) {
   FarDev<T> &F = far_of<T>(*this);
   const int nrows = (int)near.rows.size();
   mark(3);
   if (near.nleaf > 0) {
     const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
-    hipLaunchKernelGGL(k_spmv<T>, dim3(per_xcd * kXcd), dim3(256), 0, stream, near.nleaf, per_xcd,
+    hipLaunchKernelGGL((k_spmv<T, NK>), dim3(per_xcd * kXcd), dim3(256), 0, stream, near.nleaf, per_xcd,
                        d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_cols.as<int>(),
                        d_val_beg.as<int64_t>(), F.vals.template as<T>(), phi, dphi, outN, outD);
   }
   mark(4);
-  const int64_t nle = (int64_t)tree.nb * 2 * kp;
+  const int64_t nle = (int64_t)tree.nb * NK * kp;
   hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(nle, 256), 4096)), dim3(256), 0, stream,
                      F.le.template as<T>(), nle, T(0));
   mark(5);
-  launch_translate<T, 1>(*this, stream, m2l_cl, 0, (int)far.m2l.op.size(), F.m2l.template as<T>(),
-                      F.me.template as<T>(), F.le.template as<T>());
+  launch_translate<T, 1, NK>(*this, stream, m2l_cl, 0, nchunks<NK>(far.m2l), F.m2l.template as<T>(),
+                             F.me.template as<T>(), F.le.template as<T>());
   mark(6);
   const int npairs = (int)far.m2p_tgt.size();
   if (npairs > 0) {
     const int64_t total = far.m2p_work_beg.back();
     if (total > 0)
-      hipLaunchKernelGGL(k_m2p<T>, dim3(nblk(total, 256)), dim3(256), 0, stream, npairs, d_m2p_work.as<int64_t>(),
-                         d_m2p_tgt.as<int>(), d_m2p_src.as<int>(), d_node_beg.as<int>(), d_nperm.as<int>(),
-                         d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt, kp,
+      hipLaunchKernelGGL((k_m2p<T, NK>), dim3(nblk(total, 256)), dim3(256), 0, stream, npairs,
+                         d_m2p_work.as<int64_t>(), d_m2p_tgt.as<int>(), d_m2p_src.as<int>(), d_node_beg.as<int>(),
+                         d_nperm.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt, kp,
                          F.me.template as<T>(), outN, outD);
   }
   mark(7);
   for (int l = 2; l <= L; ++l)
-    launch_translate<T, 2>(*this, stream, l2l_cl, far.l2l.level_off[l], far.l2l.level_off[l + 1],
-                        F.l2l.template as<T>(), F.le.template as<T>(), F.le.template as<T>());
+    launch_translate<T, 2, NK>(*this, stream, l2l_cl, lvl<NK>(far.l2l, l), lvl<NK>(far.l2l, l + 1),
+                               F.l2l.template as<T>(), F.le.template as<T>(), F.le.template as<T>());
   mark(8);
   if (nrows > 0)
-    hipLaunchKernelGGL(k_l2p<T>, dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
+    hipLaunchKernelGGL((k_l2p<T, NK>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
                        d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
                        d_delta.as<double>(), p, kt, kp, F.le.template as<T>(), outN, outD);
   mark(9);
@@ -1148,15 +1231,24 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
 
 template <typename T>
 void Engine::matvec(const double *phi, const double *dphi, double *outN, double *outD) {
-  generate<T>(phi, dphi);
-  products<T>(phi, dphi, outN, outD);
+  generate<T, 2>(phi, dphi);
+  products<T, 2>(phi, dphi, outN, outD);
+  collect_phase_times();
+}
+
+template <typename T>
+void Engine::matvec_combined(const double *phi, const double *dphi, double *out) {
+  generate<T, 1>(phi, dphi);
+  products<T, 1>(phi, dphi, out, nullptr);
   collect_phase_times();
 }
 
 template void Engine::matvec<double>(const double *, const double *, double *, double *);
 template void Engine::matvec<float>(const double *, const double *, double *, double *);
-template void Engine::generate<double>(const double *, const double *);
-template void Engine::products<double>(const double *, const double *, double *, double *);
+template void Engine::matvec_combined<double>(const double *, const double *, double *);
+template void Engine::matvec_combined<float>(const double *, const double *, double *);
+template void Engine::generate<double, 2>(const double *, const double *);
+template void Engine::products<double, 2>(const double *, const double *, double *, double *);
 
 double Engine::norm2(const double *v) {
   const int nchunk = (int)nblk(n, kRedChunk);
@@ -1185,11 +1277,11 @@ void Engine::vmult(const double *src, double *dst, int precision) {
   hipLaunchKernelGGL(k_vmult_pre, dim3(g), dim3(256), 0, stream, n, src, d_dir.as<uint8_t>(), shift,
                      d_sp.as<double>(), d_sd.as<double>());
   if (precision == PBFMM_MIXED && have_f32)
-    matvec<float>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
+    matvec_combined<float>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>());
   else
-    matvec<double>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
-  hipLaunchKernelGGL(k_vmult_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_outD.as<double>(),
-                     d_alpha.as<double>(), d_sp.as<double>(), dst);
+    matvec_combined<double>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>());
+  hipLaunchKernelGGL(k_vmult_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_alpha.as<double>(),
+                     d_sp.as<double>(), dst);
   if (!have_dirichlet) {
     // dst -= ||dst||_2 (bem_problem.cc:1214-1215)
     const int nchunk = (int)nblk(n, kRedChunk);
@@ -1209,7 +1301,7 @@ void Engine::compute_alpha() {
   const unsigned g = nblk(n, 256);
   hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d_sp.as<double>(), n, -1.0);
   hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d_sd.as<double>(), n, 0.0);
-  matvec<double>(d_sp.as<double>(), d_sd.as<double>(), d_alpha.as<double>(), d_outD.as<double>());
+  matvec_combined<double>(d_sp.as<double>(), d_sd.as<double>(), d_alpha.as<double>());
   HIPCHK(hipGetLastError());
 }
 
@@ -1217,9 +1309,9 @@ void Engine::compute_rhs(const double *gvec, double *rhs) {
   const unsigned g = nblk(n, 256);
   hipLaunchKernelGGL(k_rhs_pre, dim3(g), dim3(256), 0, stream, n, gvec, d_dir.as<uint8_t>(), d_sp.as<double>(),
                      d_sd.as<double>());
-  matvec<double>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
-  hipLaunchKernelGGL(k_rhs_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_outD.as<double>(),
-                     d_alpha.as<double>(), d_sp.as<double>(), rhs);
+  matvec_combined<double>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>());
+  hipLaunchKernelGGL(k_rhs_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_alpha.as<double>(),
+                     d_sp.as<double>(), rhs);
   HIPCHK(hipGetLastError());
 }
 

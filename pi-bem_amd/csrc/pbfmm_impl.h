@@ -15,8 +15,11 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kToll = 0.001;  // box adjacency tolerance of bem_fma.cc:1
 
 inline int kt_of(int p) { return (p + 1) * (p + 2) / 2; }
-// M2L pairs per GEMM chunk (64 columns of the batched M2L: 2 kernels x 32 pairs)
+// Pairs per GEMM chunk of the batched translations (64 GEMM columns): 32 pairs
+// when both kernels' expansions travel (2 columns per pair), 64 pairs when
+// one combined expansion does.
 constexpr int kM2LChunk = 32;
+constexpr int kChunk1 = 64;
 // padded length of one real expansion vector [Re(0..Kt-1), Im(0..Kt-1)]
 inline int kp_of(int p) { return ((2 * kt_of(p) + 15) / 16) * 16; }
 
@@ -101,12 +104,14 @@ void build_near_plan(const HostMesh &m, const QuadData &qd, const Tree &t, NearP
 // Far field plan and translation operators
 // ---------------------------------------------------------------------------
 // Pairs (tgt, src) of one kind of translation grouped by operator class and
-// cut into chunks of <= kM2LChunk pairs; level_off[l] .. level_off[l+1] are
-// the chunks of level l (M2M: child level; L2L: target level).
+// cut into chunks of <= kM2LChunk pairs (op, beg, cnt; level_off[l] ..
+// level_off[l+1] are the chunks of level l, M2M: child level, L2L: target
+// level) and, over the same pairs, into chunks of <= kChunk1 pairs (*1).
 struct ChunkList {
   std::vector<int32_t> op, beg, cnt;  // per chunk: operator index, first pair, pair count
+  std::vector<int32_t> op1, beg1, cnt1;
   std::vector<int32_t> tgt, src;      // pairs
-  std::vector<int32_t> level_off;     // L+2 entries
+  std::vector<int32_t> level_off, level_off1;  // L+2 entries
 };
 
 struct FarPlan {

@@ -540,6 +540,11 @@ static void add_class(ChunkList &cl, int op, const std::vector<std::pair<int, in
     cl.beg.push_back((int)cl.tgt.size() + (int)i);
     cl.cnt.push_back((int)std::min<size_t>(kM2LChunk, pairs.size() - i));
   }
+  for (size_t i = 0; i < pairs.size(); i += kChunk1) {
+    cl.op1.push_back(op);
+    cl.beg1.push_back((int)cl.tgt.size() + (int)i);
+    cl.cnt1.push_back((int)std::min<size_t>(kChunk1, pairs.size() - i));
+  }
   for (auto &pr : pairs) {
     cl.tgt.push_back(pr.first);
     cl.src.push_back(pr.second);
@@ -574,9 +579,13 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
   // parent; level 1 takes the root's local expansion, which is zero)
   fp.m2m.level_off.assign(L + 2, 0);
   fp.l2l.level_off.assign(L + 2, 0);
+  fp.m2m.level_off1.assign(L + 2, 0);
+  fp.l2l.level_off1.assign(L + 2, 0);
   for (int l = 0; l <= L; ++l) {
     fp.m2m.level_off[l] = (int)fp.m2m.op.size();
     fp.l2l.level_off[l] = (int)fp.l2l.op.size();
+    fp.m2m.level_off1[l] = (int)fp.m2m.op1.size();
+    fp.l2l.level_off1[l] = (int)fp.l2l.op1.size();
     if (l == 0) continue;
     std::vector<std::pair<int, int>> byoct[8];
     for (int b = t.startL[l]; b <= t.endL[l]; ++b) byoct[t.octant[b]].push_back({t.parent[b], b});
@@ -589,13 +598,17 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
   }
   fp.m2m.level_off[L + 1] = (int)fp.m2m.op.size();
   fp.l2l.level_off[L + 1] = (int)fp.l2l.op.size();
+  fp.m2m.level_off1[L + 1] = (int)fp.m2m.op1.size();
+  fp.l2l.level_off1[L + 1] = (int)fp.l2l.op1.size();
 
   // M2L pairs by (level, offset class); M2P pairs
   fp.m2l_per_level.assign(L + 1, 0);
   fp.m2l.level_off.assign(L + 2, 0);
+  fp.m2l.level_off1.assign(L + 2, 0);
   fp.m2p_work_beg.push_back(0);
   for (int l = 1; l <= L; ++l) {
     fp.m2l.level_off[l] = (int)fp.m2l.op.size();
+    fp.m2l.level_off1[l] = (int)fp.m2l.op1.size();
     std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> cls;
     const double dl = t.delta[0] / std::pow(2.0, l);
     for (int b : t.dofs_filled[l]) {
@@ -626,6 +639,7 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
     }
   }
   fp.m2l.level_off[L + 1] = (int)fp.m2l.op.size();
+  fp.m2l.level_off1[L + 1] = (int)fp.m2l.op1.size();
 }
 
 // Columns of the unit-multipole blocks (multipole_integrals,

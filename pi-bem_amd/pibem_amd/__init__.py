@@ -220,6 +220,11 @@ class BEMProblem:
         self.fma._chk(self.fma._L.pbfmm_vmult(self._h, _p(_f64(src)), _p(dst)))
         return dst
 
+    def vmult_dev(self, src_ptr, dst_ptr, precision=PBFMM_FP64):
+        """vmult on device vectors (pbfmm_vmult_dev); precision PBFMM_MIXED runs
+        the FP32 operator."""
+        self.fma._chk(self.fma._L.pbfmm_vmult_dev(self._h, src_ptr, dst_ptr, precision))
+
     def compute_rhs(self, tmp_rhs):
         dst = np.zeros(self.fma.n)
         self.fma._chk(self.fma._L.pbfmm_compute_rhs(self._h, _p(_f64(tmp_rhs)), _p(dst)))

@@ -48,6 +48,7 @@ SIGNATURES = {
     "pbfmm_solve_system": (C.c_int, [_h, _d, _d, _d, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                      _d]),
     "pbfmm_matvec_dev": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
+    "pbfmm_vmult_dev": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_int32]),
     "pbfmm_gmres_dev": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_int32,
                                   C.POINTER(C.c_int32), _d]),
     "pbfmm_tree_counts": (C.c_int, [_h, _i64]),

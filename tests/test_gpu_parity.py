@@ -112,6 +112,25 @@ def test_alpha_vmult_rhs(c1):
     assert rel(r1, r0) <= 1e-10
 
 
+def test_vmult_dev(c1):
+    """Device-pointer vmult (combined N.phi - D.dphi expansions) against the
+    oracle's vmult: FP64 to 1e-10, the FP32 operator to 2e-5."""
+    from pibem_amd import hip
+    bp = P.BEMProblem(c1.mix)
+    v0 = c1.o.vmult(c1.phi)
+    src, dst = hip.DeviceArray.from_numpy(c1.phi), hip.DeviceArray(c1.pr.mesh.n_dofs)
+    bp.vmult_dev(src.value, dst.value, P.PBFMM_FP64)
+    hip.synchronize()
+    e64 = rel(dst.numpy(), v0)
+    bp.vmult_dev(src.value, dst.value, P.PBFMM_MIXED)
+    hip.synchronize()
+    e32 = rel(dst.numpy(), v0)
+    print(f"  vmult_dev rel err fp64 {e64:.3e} fp32 operator {e32:.3e}")
+    assert e64 <= 1e-10 and e32 <= 2e-5
+    with pytest.raises(P.PbfmmError):
+        bp.vmult_dev(src.value, src.value, P.PBFMM_FP64)
+
+
 @pytest.mark.parametrize("precision", [P.PBFMM_FP64, P.PBFMM_MIXED])
 def test_solve_system(c1, precision):
     f = c1.f64 if precision == P.PBFMM_FP64 else c1.mix
@@ -176,3 +195,47 @@ def test_config2_matvec(oracle_mod):
     N1, D1 = c.f64.multipole_matr_vect_products(c.phi, c.dphi)
     print(f"  config2 fp64 matvec rel err N {rel(N1, N0):.3e} D {rel(D1, D0):.3e}")
     assert rel(N1, N0) <= 1e-10 and rel(D1, D0) <= 1e-10
+
+
+def test_config3_properties():
+    """Config 3 at full size (393,218 DoFs, Q2, p=8), where the oracle takes a
+    minute per matvec: size-independent properties instead.
+    - the operator's combined-expansion FMM equals N(x.neu) - D(x.dir) +
+      alpha.(x.neu) assembled from the two-output FMM (FP64, 1e-12);
+    - the FP32 operator is within 2e-6 of the FP64 one;
+    - both solves reach true relative residual 1e-8 and the analytic phi to
+      1e-3 on the Neumann DoFs (measured 6e-5)."""
+    from pibem_amd import hip
+    pr = M.config_spheroid(128, 8)
+    n = pr.mesh.n_dofs
+    print(f"\n[case] config 3 N={n}", flush=True)
+    f = P.setup(pr, P.PBFMM_MIXED)
+    bp = P.BEMProblem(f)
+    alpha = bp.compute_alpha()
+    x = np.random.default_rng(1234).uniform(-1, 1, n)
+    neu = pr.dirichlet == 0
+    xs, xd = np.where(neu, x, 0.0), np.where(neu, 0.0, x)
+    a, b = hip.DeviceArray.from_numpy(xs), hip.DeviceArray.from_numpy(xd)
+    oN, oD = hip.DeviceArray(n), hip.DeviceArray(n)
+    f.matvec_dev(a.value, b.value, oN.value, oD.value, P.PBFMM_FP64)
+    src, dst = hip.DeviceArray.from_numpy(x), hip.DeviceArray(n)
+    bp.vmult_dev(src.value, dst.value, P.PBFMM_FP64)
+    hip.synchronize()
+    v64 = dst.numpy()
+    ref = oN.numpy() - oD.numpy() + alpha * xs
+    e_lin = rel(v64, ref)
+    bp.vmult_dev(src.value, dst.value, P.PBFMM_MIXED)
+    hip.synchronize()
+    e32 = rel(dst.numpy(), v64)
+    print(f"  combined vs two-kernel {e_lin:.3e}, fp32 operator vs fp64 {e32:.3e}", flush=True)
+    assert e_lin <= 1e-12 and e32 <= 2e-6
+    rhs = hip.DeviceArray.from_numpy(bp.compute_rhs(pr.tmp_rhs))
+    sol = hip.DeviceArray(n)
+    phi_ex = pr.exact.phi(pr.mesh.support_xyz)
+    for prec in (P.PBFMM_MIXED, P.PBFMM_FP64):
+        sol.copy_from(hip.DeviceArray(n))
+        it, rr = f.gmres_dev(rhs.value, sol.value, 1e-8, 100, 2000, prec)
+        s = sol.numpy()
+        err = rel(s[neu], phi_ex[neu])
+        print(f"  solve precision={prec} iters={it} relres={rr:.3e} err phi {err:.2e}", flush=True)
+        assert rr <= 1e-8 and err <= 1e-3
