@@ -1422,7 +1422,10 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
   double *V = e.d_V.as<double>(), *w = e.d_w.as<double>(), *hd = e.d_h.as<double>();
   double *part = e.d_red.as<double>();
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1), h(m + 2), y(m);
-  std::vector<double> hbuf(2 * (m + 2) + 1);
+  // Hessenberg slots on the device, one per Arnoldi step: [h1 (m+2)][h2 (m+2)][|w|]
+  const int S = 2 * (m + 2) + 1;
+  double *ybuf = hd + (size_t)m * S;
+  std::vector<double> hbuf((size_t)m * S);
   auto dots = [&](int nv, const double *vec, double *out_dev) {
     hipLaunchKernelGGL(k_mdot, dim3(nchunk, nv), dim3(kRedThreads), 0, e.stream, n, n, V, vec, part);
     hipLaunchKernelGGL(k_finish, dim3(nv), dim3(kRedThreads), 0, e.stream, nchunk, part, out_dev);
@@ -1437,7 +1440,7 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
     hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, w, 1.0 / beta, V);
     std::fill(gv.begin(), gv.end(), 0.0);
     gv[0] = beta;
-    int k = 0;
+    int k = 0, k_done = 0, k_sync = 0;  // columns < k_done are reduced; next host sync after step k_sync
     bool done = false;
     for (; k < m && it < maxit; ++k) {
       double *vk1 = V + (int64_t)(k + 1) * n;
@@ -1448,8 +1451,9 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
         e.vmult(V + (int64_t)k * n, vk1, precision);
       }
       ++it;
-      // CGS2: two projections and the norm on the device, one host round trip
-      double *h1 = hd, *h2 = hd + (m + 2), *hn_d = hd + 2 * (m + 2);
+      // CGS2: two projections and the norm, all on the device
+      double *slot = hd + (size_t)k * S;
+      double *h1 = slot, *h2 = slot + (m + 2), *hn_d = slot + 2 * (m + 2);
       dots(k + 1, vk1, h1);
       hipLaunchKernelGGL(k_gemv_sub, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, h1, vk1);
       dots(k + 1, vk1, h2);
@@ -1458,45 +1462,66 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
       hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, e.stream, nchunk, part, hn_d);
       hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(64), 0, e.stream, hn_d);
       hipLaunchKernelGGL(k_scale_inv, dim3(g), dim3(256), 0, e.stream, n, vk1, hn_d);
-      HIPCHK(hipMemcpyAsync(hbuf.data(), hd, (2 * (m + 2) + 1) * 8, hipMemcpyDeviceToHost, e.stream));
+      if (k < k_sync && k + 1 < m && it < maxit) continue;
+      // host round trip: the columns k_done..k, Givens rotations, residual estimate
+      HIPCHK(hipMemcpyAsync(hbuf.data() + (size_t)k_done * S, hd + (size_t)k_done * S,
+                            (size_t)(k + 1 - k_done) * S * 8, hipMemcpyDeviceToHost, e.stream));
       e.sync();
-      for (int j = 0; j <= k; ++j) h[j] = hbuf[j] + hbuf[(m + 2) + j];
-      const double hn = hbuf[2 * (m + 2)];
-      h[k + 1] = hn;
-      for (int j = 0; j < k; ++j) {
-        const double a = h[j], bb = h[j + 1];
-        h[j] = cs[j] * a + sn[j] * bb;
-        h[j + 1] = -sn[j] * a + cs[j] * bb;
+      const double est0 = est;
+      const int j0 = k_done;
+      for (int j = k_done; j <= k; ++j) {
+        const double *hb = hbuf.data() + (size_t)j * S;
+        for (int i = 0; i <= j; ++i) h[i] = hb[i] + hb[(m + 2) + i];
+        const double hn = hb[2 * (m + 2)];
+        h[j + 1] = hn;
+        for (int i = 0; i < j; ++i) {
+          const double a = h[i], bb = h[i + 1];
+          h[i] = cs[i] * a + sn[i] * bb;
+          h[i + 1] = -sn[i] * a + cs[i] * bb;
+        }
+        const double den = std::hypot(h[j], h[j + 1]);
+        cs[j] = h[j] / den;
+        sn[j] = h[j + 1] / den;
+        h[j] = den;
+        for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = h[i];
+        gv[j + 1] = -sn[j] * gv[j];
+        gv[j] = cs[j] * gv[j];
+        est = std::fabs(gv[j + 1]);
+        if (est <= tol_abs || hn == 0) {
+          k = j;  // the Arnoldi steps after j are not used
+          done = true;
+          break;
+        }
       }
-      const double den = std::hypot(h[k], h[k + 1]);
-      cs[k] = h[k] / den;
-      sn[k] = h[k + 1] / den;
-      h[k] = den;
-      for (int j = 0; j <= k; ++j) H[(size_t)j * m + k] = h[j];
-      gv[k + 1] = -sn[k] * gv[k];
-      gv[k] = cs[k] * gv[k];
-      est = std::fabs(gv[k + 1]);
-      if (est <= tol_abs || hn == 0) {
+      if (done) {
         ++k;
-        done = true;
         break;
       }
+      k_done = k + 1;
+      // next sync: half the steps the observed decay predicts, 1..8 ahead
+      int ahead = 1;
+      if (est < est0 && est0 > 0) {
+        const double rate = std::log(est / est0) / (k + 1 - j0);  // per step, < 0
+        const double left = std::log(tol_abs / est) / rate;
+        ahead = (int)std::max(1.0, std::min(8.0, 0.5 * left));
+      }
+      k_sync = k + ahead;
     }
     for (int i = k - 1; i >= 0; --i) {
       double s = gv[i];
       for (int j = i + 1; j < k; ++j) s -= H[(size_t)i * m + j] * y[j];
       y[i] = s / H[(size_t)i * m + i];
     }
-    HIPCHK(hipMemcpyAsync(hd, y.data(), k * 8, hipMemcpyHostToDevice, e.stream));
+    HIPCHK(hipMemcpyAsync(ybuf, y.data(), k * 8, hipMemcpyHostToDevice, e.stream));
     if (e.have_prec) {
       // x += M^-1 (V y)
       double *t = e.d_ptmp2.as<double>(), *z = e.d_ptmp.as<double>();
       hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, e.stream, t, n, 0.0);
-      hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, V, hd, t);
+      hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, V, ybuf, t);
       e.apply_prec(t, z);
       hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, z, 1.0, x);
     } else {
-      hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, V, hd, x);
+      hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, V, ybuf, x);
     }
     e.sync();
     if (done) return 0;
@@ -1516,7 +1541,7 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
     alloc(d_w, n * 8);
     alloc(d_r, n * 8);
     alloc(d_x, n * 8);
-    alloc(d_h, (size_t)(2 * (restart + 2) + 2) * 8);
+    alloc(d_h, ((size_t)restart * (2 * (restart + 2) + 1) + restart + 2) * 8);
     gm_restart = restart;
   }
   const unsigned g = nblk(n, 256);
