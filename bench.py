@@ -59,23 +59,24 @@ def log(msg):
 
 def work_model(wc, s_v, nk):
     """Algorithmic bytes and flops of one launch of each phase (s_v: bytes of
-    an operator value, 4 for the FP32 operator, 8 for FP64; nk: expansions per
-    block, 2 for separate N and D outputs, 1 for the operator's N.phi - D.dphi).
+    an operator value, 4 for the FP32 operator, 8 for FP64; nk: 2 for the FMM
+    matvec with separate N.phi and D.dphi outputs, 1 for the operator GMRES
+    applies, whose near field and unit multipoles hold one value per entry (N
+    on Neumann, -D on Dirichlet columns) and whose expansions are combined).
 
-    spmv: the (N, D) value pairs of the leaf blocks, one column index per leaf
-          column, one row id per row, phi and dphi read once, nk outputs
-          written once.
-    p2m:  the unit-multipole blocks (2 kernels x kp values per leaf column), one
-          DoF index per column, phi and dphi once, nk kp values per leaf written.
+    spmv: nk values per leaf-block entry, one column index per leaf column, one
+          row id per row, nk input vectors read once, nk outputs written once.
+    p2m:  nk kp unit-multipole values per leaf column, one DoF index per
+          column, nk input vectors once, nk kp values per leaf written.
     m2m, l2l, m2l: 8 Kt^2 flops per (pair, expansion): a complex Kt x Kt map.
     m2p, l2p: ~(8 nk Kt + 8 Kt + 40) flops per evaluation point.
     """
     n, p, kp = wc["n_dofs"], wc["trunc_order"], wc["kp"]
     kt = (p + 1) * (p + 2) // 2
     out = {
-        "spmv": dict(bound="hbm", bytes=wc["near_values"] * 2 * s_v + wc["near_cols"] * 4 + wc["near_rows"] * 4
-                     + 2 * n * 8 + nk * n * 8),
-        "p2m": dict(bound="hbm", bytes=wc["p2m_cols"] * (2 * kp * s_v + 4) + 2 * n * 8
+        "spmv": dict(bound="hbm", bytes=wc["near_values"] * nk * s_v + wc["near_cols"] * 4 + wc["near_rows"] * 4
+                     + nk * n * 8 + nk * n * 8),
+        "p2m": dict(bound="hbm", bytes=wc["p2m_cols"] * (nk * kp * s_v + 4) + nk * n * 8
                     + wc["leaves"] * nk * kp * s_v),
     }
     for k, pairs in (("m2m", wc["m2m_translations"]), ("l2l", wc["l2l_translations"]), ("m2l", wc["m2l_pairs"])):

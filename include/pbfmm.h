@@ -192,7 +192,10 @@ int pbfmm_last_timings(pbfmm_handle *h, double *ms);
  * [5] quad points [6] M2L pairs [7] M2P (pair, node) evaluations
  * [8] M2M child translations [9] L2L translations [10] num_blocks
  * [11] trunc_order [12] padded expansion length kp
- * [13] unit-multipole columns (sum over leaves of the DoFs feeding P2M) */
+ * [13] unit-multipole columns (sum over leaves of the DoFs feeding P2M)
+ * [14] [15] near-field SpMV kernels chosen at set-up for FP64 / FP32:
+ *      bit 0 the (N, D) pair storage, bit 1 the operator's single values
+ *      (0 workgroup per leaf, 1 wavefront per leaf) */
 int pbfmm_work_counts(pbfmm_handle *h, int64_t *counts);
 
 #ifdef __cplusplus

@@ -40,6 +40,9 @@ struct FarDev {
   DevBuf qx, qn, qw, qsh;  // regular quadrature data (FP64 copy only)
   DevBuf vals;             // near-field values (N, D) pairs, leaf-blocked
   DevBuf unit;             // unit multipoles: [column][2 kernels][kp]
+  // the operator's single-value copies: N on Neumann / -D on Dirichlet
+  // columns, U1 on Neumann / -U2 on Dirichlet DoFs ([column][kp])
+  DevBuf opvals, opunit;
 };
 
 struct Engine {
@@ -62,7 +65,7 @@ struct Engine {
   DevBuf d_pmin, d_delta, d_parent, d_nchild, d_child, d_octant, d_node_beg, d_node_cnt, d_nperm,
       d_quad_beg, d_quad_cnt, d_qperm;
   // device: plans
-  DevBuf d_p2m_leaf, d_m2p_tgt, d_m2p_src, d_m2p_work;
+  DevBuf d_p2m_leaf, d_m2p_node, d_m2p_isrc;
   DevBuf d_p2m_col_beg, d_p2m_col_dof;
   ChunkDev m2m_cl, m2l_cl, l2l_cl;
   DevBuf d_row_leaf, d_row_beg, d_rows, d_col_beg, d_cols, d_val_beg, d_ent_beg, d_ent_cell,
@@ -106,10 +109,18 @@ struct Engine {
   void assemble_near(bool want_f32);
   template <typename T>
   void matvec(const double *phi, const double *dphi, double *outN, double *outD);
-  // the BEMProblem operator's FMM: out = N.phi - D.dphi with one combined
-  // expansion per block
+  // the BEMProblem operator's FMM: out = N.(x.neu) - D.(x.dir) with one
+  // combined expansion per block and single-value near field / unit multipoles
   template <typename T>
-  void matvec_combined(const double *phi, const double *dphi, double *out);
+  void matvec_combined(const double *x, double *out);
+  // near-field SpMV kernel per precision ([0] FP64, [1] FP32) and storage
+  // (pairs, single values): 0 workgroup per leaf with an LDS copy of the
+  // columns, 1 wavefront per leaf; chosen by timing both at set-up
+  int spmv_variant[2][2] = {{0, 0}, {0, 0}};  // [precision][NK - 1]
+  template <typename T, int NK>
+  void tune_spmv();
+  template <typename T, int NK>
+  void launch_spmv(int variant, const double *phi, const double *dphi, double *outN, double *outD);
   template <typename T, int NK>
   void generate(const double *phi, const double *dphi);
   template <typename T, int NK>

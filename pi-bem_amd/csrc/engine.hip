@@ -47,12 +47,8 @@ template <typename T> __device__ __forceinline__ T PMF(int m);
 template <> __device__ __forceinline__ double PMF<double>(int m) { return c_pmf_d[m]; }
 template <> __device__ __forceinline__ float PMF<float>(int m) { return c_pmf_f[m]; }
 
-__device__ __forceinline__ void dsincos(double a, double *s, double *c) { sincos(a, s, c); }
-__device__ __forceinline__ void dsincos(float a, float *s, float *c) { sincosf(a, s, c); }
 __device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ float dsqrt(float a) { return sqrtf(a); }
-__device__ __forceinline__ double datan2(double y, double x) { return atan2(y, x); }
-__device__ __forceinline__ float datan2(float y, float x) { return atan2f(y, x); }
 
 static void upload_legendre_tables() {
   double ra[kLd * kLd] = {0}, rb[kLd * kLd] = {0}, rs[kLd * kLd] = {0}, pmm[kLd], pmf[kLd];
@@ -229,7 +225,11 @@ __global__ void k_round_f32(int64_t n, const double *__restrict__ a, float *__re
 // ---------------------------------------------------------------------------
 // Near-field SpMV, fused N.phi and D.dphi on the shared pattern
 // (prec_neumann_matrix.vmult / prec_dirichlet_matrix.vmult, bem_fma.cc:2408-2409),
-// on the leaf-blocked layout: one workgroup per leaf.  All rows of a leaf share
+// on the leaf-blocked layout.  Two kernels; set-up times both on the
+// assembled operator and keeps the faster one per precision (many small
+// leaves favour the first, many rows per leaf the second).
+//
+// k_spmv_lds: one workgroup per leaf.  All rows of a leaf share
 // its column set, so the workgroup gathers (phi, dphi) at those columns once
 // into LDS (tiles of kSpmvTile columns) and each wavefront then streams whole
 // rows of (N, D) pairs against the LDS copy: the value block is read once,
@@ -245,51 +245,121 @@ template <> struct Vec2<float> { using type = float2; };
 constexpr int kSpmvTile = 2048;
 constexpr int kXcd = 8;
 
-// NK = 2 writes N.phi to outN and D.dphi to outD; NK = 1 writes the
-// combination N.phi - D.dphi to outN (outD unused).
+// NK = 2: (N, D) pairs per entry, outN = N.phi and outD = D.dphi.  NK = 1:
+// one value per entry (the operator's matrix: N on Neumann columns, -D on
+// Dirichlet columns), outN = M.phi (dphi and outD unused).
+template <typename T, int NK> struct NearEntry { using type = typename Vec2<T>::type; };
+template <typename T> struct NearEntry<T, 1> { using type = T; };
 template <typename T, int NK>
-__global__ void __launch_bounds__(256) k_spmv(int nleaf, int per_xcd, const int *__restrict__ row_beg,
-                                              const int *__restrict__ rows, const int *__restrict__ col_beg,
-                                              const int *__restrict__ cols, const int64_t *__restrict__ val_beg,
-                                              const T *__restrict__ vals, const double *__restrict__ phi,
-                                              const double *__restrict__ dphi, double *__restrict__ outN,
-                                              double *__restrict__ outD) {
-  using T2 = typename Vec2<T>::type;
-  __shared__ T2 xs[kSpmvTile];
+__device__ __forceinline__ typename NearEntry<T, NK>::type near_x(const double *phi, const double *dphi, int col) {
+  if constexpr (NK == 2) return typename NearEntry<T, NK>::type{(T)phi[col], (T)dphi[col]};
+  else return (T)phi[col];
+}
+template <typename T, int NK>
+__device__ __forceinline__ void near_fma(T &aN, T &aD, const typename NearEntry<T, NK>::type &a,
+                                         const typename NearEntry<T, NK>::type &x) {
+  if constexpr (NK == 2) {
+    aN += a.x * x.x;
+    aD += a.y * x.y;
+  } else {
+    aN += a * x;
+  }
+}
+template <int NK>
+__device__ __forceinline__ void near_store(double *outN, double *outD, int node, bool first, double aN, double aD) {
+  outN[node] = (first ? 0.0 : outN[node]) + aN;
+  if constexpr (NK == 2) outD[node] = (first ? 0.0 : outD[node]) + aD;
+}
+
+template <typename T, int NK>
+__global__ void __launch_bounds__(256) k_spmv_lds(int nleaf, int per_xcd, const int *__restrict__ row_beg,
+                                                  const int *__restrict__ rows, const int *__restrict__ col_beg,
+                                                  const int *__restrict__ cols, const int64_t *__restrict__ val_beg,
+                                                  const T *__restrict__ vals, const double *__restrict__ phi,
+                                                  const double *__restrict__ dphi, double *__restrict__ outN,
+                                                  double *__restrict__ outD) {
+  using E = typename NearEntry<T, NK>::type;
+  __shared__ E xs[kSpmvTile];
   const int leaf = (int)(blockIdx.x % kXcd) * per_xcd + (int)(blockIdx.x / kXcd);
   if (leaf >= nleaf) return;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0;
   const int cb = col_beg[leaf], C = col_beg[leaf + 1] - cb;
-  const T2 *V = reinterpret_cast<const T2 *>(vals) + val_beg[leaf];
+  const E *V = reinterpret_cast<const E *>(vals) + val_beg[leaf];
   for (int c0 = 0; c0 < C; c0 += kSpmvTile) {
     const int CT = min(kSpmvTile, C - c0);
     if (c0 > 0) __syncthreads();  // previous tile fully consumed
-    for (int c = t; c < CT; c += 256) {
-      const int col = cols[cb + c0 + c];
-      xs[c] = T2{(T)phi[col], (T)dphi[col]};
-    }
+    for (int c = t; c < CT; c += 256) xs[c] = near_x<T, NK>(phi, dphi, cols[cb + c0 + c]);
     __syncthreads();
     for (int r = w; r < R; r += 4) {
-      const T2 *v = V + (int64_t)r * C + c0;
+      const E *v = V + (int64_t)r * C + c0;
       T aN = 0, aD = 0;
 #pragma unroll 4
-      for (int c = lane; c < CT; c += 64) {
-        const T2 a = v[c], x = xs[c];
-        aN += a.x * x.x;
-        aD += a.y * x.y;
-      }
+      for (int c = lane; c < CT; c += 64) near_fma<T, NK>(aN, aD, v[c], xs[c]);
       aN = wave_sum(aN);
-      aD = wave_sum(aD);
+      if constexpr (NK == 2) aD = wave_sum(aD);
+      // the wave that owns row r owns it in every tile: plain read-modify-write
+      if (lane == 0) near_store<NK>(outN, outD, rows[r0 + r], c0 == 0, (double)aN, (double)aD);
+    }
+  }
+}
+
+// k_spmv_wave: one wavefront per leaf, no LDS and no barrier.  For each tile
+// of 64 x CPL leaf columns (64 bytes of values per lane and row: CPL = 8 / 4
+// for FP32 / FP64 pairs, 16 / 8 for single values) every lane holds the
+// inputs of its CPL columns in registers, and the wave streams two rows at a
+// time against them (2 CPL coalesced loads per lane in flight).  Outputs as
+// k_spmv_lds.
+template <typename T, int NK>
+__global__ void __launch_bounds__(256) k_spmv_wave(int nleaf, const int *__restrict__ row_beg,
+                                                   const int *__restrict__ rows, const int *__restrict__ col_beg,
+                                                   const int *__restrict__ cols, const int64_t *__restrict__ val_beg,
+                                                   const T *__restrict__ vals, const double *__restrict__ phi,
+                                                   const double *__restrict__ dphi, double *__restrict__ outN,
+                                                   double *__restrict__ outD) {
+  using E = typename NearEntry<T, NK>::type;
+  constexpr int CPL = 64 / (int)sizeof(E);
+  constexpr int TILE = 64 * CPL;
+  const int leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (leaf >= nleaf) return;
+  const int lane = threadIdx.x & 63;
+  const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0;
+  const int cb = col_beg[leaf], C = col_beg[leaf + 1] - cb;
+  const E *V = reinterpret_cast<const E *>(vals) + val_beg[leaf];
+  const E zero{};
+  for (int c0 = 0; c0 < C; c0 += TILE) {
+    E x[CPL];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      const int c = c0 + lane + 64 * j;
+      x[j] = c < C ? near_x<T, NK>(phi, dphi, cols[cb + c]) : zero;
+    }
+    for (int r = 0; r < R; r += 2) {
+      const bool two = r + 1 < R;
+      const E *v0 = V + (int64_t)r * C, *v1 = V + (int64_t)(two ? r + 1 : r) * C;
+      E a0[CPL], a1[CPL];
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+        const int c = c0 + lane + 64 * j;
+        a0[j] = c < C ? v0[c] : zero;
+        a1[j] = c < C ? v1[c] : zero;
+      }
+      T aN0 = 0, aD0 = 0, aN1 = 0, aD1 = 0;
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+        near_fma<T, NK>(aN0, aD0, a0[j], x[j]);
+        near_fma<T, NK>(aN1, aD1, a1[j], x[j]);
+      }
+      aN0 = wave_sum(aN0);
+      aN1 = wave_sum(aN1);
+      if constexpr (NK == 2) {
+        aD0 = wave_sum(aD0);
+        aD1 = wave_sum(aD1);
+      }
+      // the wave owns the leaf's rows in every tile: plain read-modify-write
       if (lane == 0) {
-        const int node = rows[r0 + r];
-        // the wave that owns row r owns it in every tile: plain read-modify-write
-        if constexpr (NK == 2) {
-          outN[node] = (c0 == 0 ? 0.0 : outN[node]) + (double)aN;
-          outD[node] = (c0 == 0 ? 0.0 : outD[node]) + (double)aD;
-        } else {
-          outN[node] = (c0 == 0 ? 0.0 : outN[node]) + ((double)aN - (double)aD);
-        }
+        near_store<NK>(outN, outD, rows[r0 + r], c0 == 0, (double)aN0, (double)aD0);
+        if (two) near_store<NK>(outN, outD, rows[r0 + r + 1], c0 == 0, (double)aN1, (double)aD1);
       }
     }
   }
@@ -391,8 +461,10 @@ __device__ __forceinline__ void axpy16(double2 &a, double x, const double2 &u) {
 __device__ __forceinline__ void add16(float4 &a, const float4 &u) { a.x += u.x; a.y += u.y; a.z += u.z; a.w += u.w; }
 __device__ __forceinline__ void add16(double2 &a, const double2 &u) { a.x += u.x; a.y += u.y; }
 
-// NK = 1: one combined expansion per leaf, sum_columns (phi U1 - dphi U2),
-// stored [leaf][kp].
+// NK = 2: U is [column][Ker1 kp][Ker2 kp] and the leaf gets both kernels'
+// expansions from (phi, dphi).  NK = 1: U is the operator's [column][kp]
+// (U1 on Neumann DoFs, -U2 on Dirichlet DoFs) and the leaf gets one combined
+// expansion from phi (dphi unused).  ME is [leaf][NK kp].
 template <typename T, int NK>
 __global__ void __launch_bounds__(256) k_p2m(int nleaf, const int *__restrict__ leaves,
                                              const int *__restrict__ col_beg, const int *__restrict__ col_dof,
@@ -405,12 +477,11 @@ __global__ void __launch_bounds__(256) k_p2m(int nleaf, const int *__restrict__ 
   __shared__ TV part[256];
   const int li = blockIdx.x;
   const int t = threadIdx.x;
-  const int NO = 2 * kp, NOV = NO / VW;  // kp is a multiple of 16: NOV <= 144
-  const int NOUT = NK * kp / VW;          // output vectors
-  const int nph = 256 / NOUT;
-  const int o = t % NOUT, ph = t / NOUT;
+  const int NO = NK * kp, NOV = NO / VW;  // kp is a multiple of 16: NOV <= 144
+  const int nph = 256 / NOV;
+  const int o = t % NOV, ph = t / NOV;
   const bool active = ph < nph;
-  const bool k2 = o * VW >= kp;          // a vector never straddles the two kernels
+  const bool k2 = NK == 2 && o * VW >= kp;  // a vector never straddles the two kernels
   const int c0 = col_beg[li], C = col_beg[li + 1] - c0;
   const TV *Ub = reinterpret_cast<const TV *>(U + (int64_t)c0 * NO) + o;
   TV acc{};
@@ -419,29 +490,50 @@ __global__ void __launch_bounds__(256) k_p2m(int nleaf, const int *__restrict__ 
     if (cb > 0) __syncthreads();
     for (int c = t; c < CT; c += 256) {
       const int d = col_dof[c0 + cb + c];
-      xs[c] = T2{(T)phi[d], (T)dphi[d]};
+      xs[c] = T2{(T)phi[d], NK == 2 ? (T)dphi[d] : T(0)};
     }
     __syncthreads();
     if (active) {
 #pragma unroll 4
       for (int c = ph; c < CT; c += nph) {
         const T2 xv = xs[c];
-        const TV *u = Ub + (int64_t)(cb + c) * NOV;
-        if constexpr (NK == 2) {
-          axpy16(acc, k2 ? xv.y : xv.x, u[0]);
-        } else {
-          axpy16(acc, xv.x, u[0]);
-          axpy16(acc, -xv.y, u[NOUT]);
-        }
+        axpy16(acc, k2 ? xv.y : xv.x, Ub[(int64_t)(cb + c) * NOV]);
       }
     }
   }
   part[t] = acc;
   __syncthreads();
-  if (t < NOUT) {
+  if (t < NOV) {
     TV a = part[t];
-    for (int q = 1; q < nph; ++q) add16(a, part[q * NOUT + t]);
-    reinterpret_cast<TV *>(ME + (int64_t)leaves[li] * NK * kp)[t] = a;
+    for (int q = 1; q < nph; ++q) add16(a, part[q * NOV + t]);
+    reinterpret_cast<TV *>(ME + (int64_t)leaves[li] * NO)[t] = a;
+  }
+}
+
+// The operator's single-value data from the two-kernel data (set-up, FP64):
+// near-field M = N on Neumann columns, -D on Dirichlet columns; unit
+// multipoles U1 on Neumann DoFs, -U2 on Dirichlet DoFs.
+__global__ void k_op_near(int nleaf, const int *__restrict__ row_beg, const int *__restrict__ col_beg,
+                          const int *__restrict__ cols, const int64_t *__restrict__ val_beg,
+                          const uint8_t *__restrict__ dir, const double *__restrict__ vals,
+                          double *__restrict__ op) {
+  const int leaf = blockIdx.x;
+  const int R = row_beg[leaf + 1] - row_beg[leaf];
+  const int cb = col_beg[leaf], C = col_beg[leaf + 1] - cb;
+  const int64_t vb = val_beg[leaf];
+  for (int64_t i = threadIdx.x; i < (int64_t)R * C; i += blockDim.x) {
+    const int c = (int)(i % C);
+    const bool d = dir[cols[cb + c]];
+    op[vb + i] = d ? -vals[2 * (vb + i) + 1] : vals[2 * (vb + i)];
+  }
+}
+__global__ void k_op_unit(int64_t ncol, int kp, const int *__restrict__ col_dof, const uint8_t *__restrict__ dir,
+                          const double *__restrict__ U, double *__restrict__ op) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ncol * kp;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = i / kp;
+    const int o = (int)(i - c * kp);
+    op[i] = dir[col_dof[c]] ? -U[c * 2 * kp + kp + o] : U[c * 2 * kp + o];
   }
 }
 
@@ -610,12 +702,14 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(const int *__res
 template <typename T, bool LOCAL, int NK>
 __device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, int kp, T d0, T d1, T d2,
                                       T &v1, T &v2) {
-  const T rho = dsqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  // spherical angles of d without trigonometric calls: cos(alpha) = z/rho,
+  // sin(alpha) = rho_xy/rho, e^{i beta} = (x + i y)/rho_xy (beta = 0 on the axis)
+  const T rxy2 = d0 * d0 + d1 * d1;
+  const T rho = dsqrt(rxy2 + d2 * d2);
+  const T rxy = dsqrt(rxy2);
   const T x = d2 / rho;
-  const T beta = datan2(d1, d0);
-  const T s = dsqrt(T(1) - x * x);
-  T sb, cb;
-  dsincos(beta, &sb, &cb);
+  const T s = rxy / rho;
+  const T cb = rxy > T(0) ? d0 / rxy : T(1), sb = rxy > T(0) ? d1 / rxy : T(0);
   const T step = LOCAL ? rho : T(1) / rho;
   T rstart = LOCAL ? T(1) : step;
   T pmm = 1, cm = 1, sm = 0;
@@ -646,26 +740,17 @@ __device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, in
   }
 }
 
-// M2P for smaller far blocks (bem_fma.cc:2664-2704): one thread per (pair, node).
+// M2P for smaller far blocks (bem_fma.cc:2664-2704): one thread per
+// (source block, target node) evaluation, listed at set-up.
 template <typename T, int NK>
-__global__ void __launch_bounds__(256) k_m2p(int npairs, const int64_t *__restrict__ work_beg,
-                                             const int *__restrict__ tgt, const int *__restrict__ src,
-                                             const int *__restrict__ node_beg, const int *__restrict__ nperm,
-                                             const double *__restrict__ xyz, const double *__restrict__ pmin,
-                                             const double *__restrict__ delta, int p, int kt, int kp,
-                                             const T *__restrict__ ME, double *__restrict__ outN,
+__global__ void __launch_bounds__(256) k_m2p(int64_t nitems, const int *__restrict__ item_node,
+                                             const int *__restrict__ item_src, const double *__restrict__ xyz,
+                                             const double *__restrict__ pmin, const double *__restrict__ delta, int p,
+                                             int kt, int kp, const T *__restrict__ ME, double *__restrict__ outN,
                                              double *__restrict__ outD) {
-  const int64_t total = work_beg[npairs];
   const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (w >= total) return;
-  int lo = 0, hi = npairs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (work_beg[mid] <= w) lo = mid;
-    else hi = mid - 1;
-  }
-  const int b = tgt[lo], sblk = src[lo];
-  const int node = nperm[node_beg[b] + (int)(w - work_beg[lo])];
+  if (w >= nitems) return;
+  const int node = item_node[w], sblk = item_src[w];
   const double hd = delta[sblk] * 0.5;
   const T d0 = (T)(xyz[3 * node] - (pmin[3 * sblk] + hd));
   const T d1 = (T)(xyz[3 * node + 1] - (pmin[3 * sblk + 1] + hd));
@@ -708,20 +793,23 @@ __global__ void k_fill(T *__restrict__ a, int64_t n, T v) {
 // BEMProblem operator pieces (bem_problem.cc:1164-1260) and the constrained
 // rows of ConstrainedOperator::vmult (constrained_matrix.h:91-135).
 // ---------------------------------------------------------------------------
+// The operator's input: serv_phi (Neumann DoFs, shifted in the pure-Neumann
+// case) and serv_dphi_dn (Dirichlet DoFs) share one vector, since the
+// operator's near field and unit multipoles already pick N or -D per column.
 __global__ void k_vmult_pre(int64_t n, const double *__restrict__ src, const uint8_t *__restrict__ dir,
-                            const double *__restrict__ shift, double *__restrict__ sp, double *__restrict__ sd) {
+                            const double *__restrict__ shift, double *__restrict__ x) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double sh = shift ? *shift : 0.0;
-  sp[i] = dir[i] ? 0.0 : (src[i] - sh);
-  sd[i] = dir[i] ? src[i] : 0.0;
+  x[i] = dir[i] ? src[i] : (src[i] - sh);
 }
-// ND = N.sp - D.sd from the combined-kernel FMM
+// ND = N.sp - D.sd from the operator's FMM, x its input
 __global__ void k_vmult_post(int64_t n, const double *__restrict__ ND, const double *__restrict__ alpha,
-                             const double *__restrict__ sp, double *__restrict__ dst) {
+                             const uint8_t *__restrict__ dir, const double *__restrict__ x,
+                             double *__restrict__ dst) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  dst[i] = ND[i] + alpha[i] * sp[i];
+  dst[i] = ND[i] + (dir[i] ? 0.0 : alpha[i] * x[i]);
 }
 __global__ void k_rhs_pre(int64_t n, const double *__restrict__ g, const uint8_t *__restrict__ dir,
                           double *__restrict__ sp, double *__restrict__ sd) {
@@ -730,12 +818,11 @@ __global__ void k_rhs_pre(int64_t n, const double *__restrict__ g, const uint8_t
   sp[i] = dir[i] ? g[i] : 0.0;
   sd[i] = dir[i] ? 0.0 : g[i];
 }
-// ND = N.sp - D.sd from the combined-kernel FMM
-__global__ void k_rhs_post(int64_t n, const double *__restrict__ ND, const double *__restrict__ alpha,
-                           const double *__restrict__ sp, double *__restrict__ dst) {
+__global__ void k_rhs_post(int64_t n, const double *__restrict__ N, const double *__restrict__ D,
+                           const double *__restrict__ alpha, const double *__restrict__ sp, double *__restrict__ dst) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  dst[i] = -(ND[i] + alpha[i] * sp[i]);
+  dst[i] = -(N[i] + alpha[i] * sp[i]) + D[i];
 }
 __global__ void k_shift(int64_t n, double *__restrict__ v, const double *__restrict__ s) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1036,9 +1123,8 @@ void Engine::upload_far(bool want_f32) {
   upload_chunks(m2m_cl, far.m2m);
   upload_chunks(m2l_cl, far.m2l);
   upload_chunks(l2l_cl, far.l2l);
-  up(d_m2p_tgt, far.m2p_tgt);
-  up(d_m2p_src, far.m2p_src);
-  up(d_m2p_work, far.m2p_work_beg);
+  up(d_m2p_node, far.m2p_item_node);
+  up(d_m2p_isrc, far.m2p_item_src);
   upload_far_t<double>(*this, f64);
   if (want_f32) upload_far_t<float>(*this, f32);
 }
@@ -1062,10 +1148,22 @@ void Engine::build_unit_multipoles(bool want_f32) {
     HIPCHK(hipGetLastError());
     sync();  // qpt, qleaf, qcol are freed on return
   }
+  // the operator's unit multipoles (U1 on Neumann, -U2 on Dirichlet DoFs)
+  const int64_t nou = std::max<int64_t>(ncol, 1) * kp;
+  alloc(f64.opunit, (size_t)nou * sizeof(double));
+  if (ncol > 0) {
+    hipLaunchKernelGGL(k_op_unit, dim3((unsigned)std::min<int64_t>(nblk(ncol * kp, 256), 8192)), dim3(256), 0,
+                       stream, ncol, kp, d_p2m_col_dof.as<int>(), d_dir.as<uint8_t>(), f64.unit.as<double>(),
+                       f64.opunit.as<double>());
+    HIPCHK(hipGetLastError());
+  }
   if (want_f32) {
     alloc(f32.unit, (size_t)nu * sizeof(float));
     hipLaunchKernelGGL(k_round_f32, dim3((unsigned)std::min<int64_t>(nblk(nu, 256), 8192)), dim3(256), 0, stream, nu,
                        f64.unit.as<double>(), f32.unit.as<float>());
+    alloc(f32.opunit, (size_t)nou * sizeof(float));
+    hipLaunchKernelGGL(k_round_f32, dim3((unsigned)std::min<int64_t>(nblk(nou, 256), 8192)), dim3(256), 0, stream,
+                       nou, f64.opunit.as<double>(), f32.opunit.as<float>());
     HIPCHK(hipGetLastError());
   }
   sync();
@@ -1118,11 +1216,29 @@ void Engine::assemble_near(bool want_f32) {
                        d_sing.as<double>(), d_scnt.as<int>(), maxq * ps, f64.vals.as<double>());
     HIPCHK(hipGetLastError());
   }
+  // the operator's single-value near field (N on Neumann, -D on Dirichlet columns)
+  const int64_t nop = std::max<int64_t>(near.nnz, 1);
+  alloc(f64.opvals, (size_t)nop * sizeof(double));
+  if (near.nleaf > 0) {
+    hipLaunchKernelGGL(k_op_near, dim3(near.nleaf), dim3(256), 0, stream, near.nleaf, d_row_beg.as<int>(),
+                       d_col_beg.as<int>(), d_cols.as<int>(), d_val_beg.as<int64_t>(), d_dir.as<uint8_t>(),
+                       f64.vals.as<double>(), f64.opvals.as<double>());
+    HIPCHK(hipGetLastError());
+  }
   if (want_f32) {
     alloc(f32.vals, (size_t)nv * sizeof(float));
     hipLaunchKernelGGL(k_round_f32, dim3((unsigned)std::min<int64_t>(nblk(nv, 256), 8192)), dim3(256), 0, stream, nv,
                        f64.vals.as<double>(), f32.vals.as<float>());
+    alloc(f32.opvals, (size_t)nop * sizeof(float));
+    hipLaunchKernelGGL(k_round_f32, dim3((unsigned)std::min<int64_t>(nblk(nop, 256), 8192)), dim3(256), 0, stream,
+                       nop, f64.opvals.as<double>(), f32.opvals.as<float>());
     HIPCHK(hipGetLastError());
+  }
+  tune_spmv<double, 2>();
+  tune_spmv<double, 1>();
+  if (want_f32) {
+    tune_spmv<float, 2>();
+    tune_spmv<float, 1>();
   }
   sync();
 }
@@ -1156,6 +1272,51 @@ static void launch_translate(Engine &e, hipStream_t st, const ChunkDev &cl, int 
 #undef PBFMM_TR
 }
 
+template <typename T, int NK>
+void Engine::launch_spmv(int variant, const double *phi, const double *dphi, double *outN, double *outD) {
+  const T *vals = (NK == 2 ? far_of<T>(*this).vals : far_of<T>(*this).opvals).template as<T>();
+  if (variant == 0) {
+    const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
+    hipLaunchKernelGGL((k_spmv_lds<T, NK>), dim3(per_xcd * kXcd), dim3(256), 0, stream, near.nleaf, per_xcd,
+                       d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_cols.as<int>(),
+                       d_val_beg.as<int64_t>(), vals, phi, dphi, outN, outD);
+  } else {
+    hipLaunchKernelGGL((k_spmv_wave<T, NK>), dim3((near.nleaf + 3) / 4), dim3(256), 0, stream, near.nleaf,
+                       d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_cols.as<int>(),
+                       d_val_beg.as<int64_t>(), vals, phi, dphi, outN, outD);
+  }
+}
+
+// Times both SpMV kernels on the assembled near field (HIP events, scratch
+// vectors) and keeps the faster one for this precision and storage.
+template <typename T, int NK>
+void Engine::tune_spmv() {
+  if (near.nleaf == 0) return;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 0;
+  int pick = 0;
+  for (int v = 0; v < 2; ++v) {
+    for (int rep = 0; rep < 2; ++rep)
+      launch_spmv<T, NK>(v, d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
+    HIPCHK(hipEventRecord(e0, stream));
+    for (int rep = 0; rep < 5; ++rep)
+      launch_spmv<T, NK>(v, d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
+    HIPCHK(hipEventRecord(e1, stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (v == 0 || ms < best) {
+      best = ms;
+      pick = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  spmv_variant[sizeof(T) == 4 ? 1 : 0][NK - 1] = pick;
+}
+
 // Chunk range of level l in a chunk list, for the NK-expansion chunking.
 template <int NK>
 static inline int lvl(const ChunkList &c, int l) { return NK == 2 ? c.level_off[l] : c.level_off1[l]; }
@@ -1176,8 +1337,8 @@ void Engine::generate(const double *phi, const double *dphi) {
   mark(0);
   if (nleaf > 0)
     hipLaunchKernelGGL((k_p2m<T, NK>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
-                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.unit.template as<T>(), kp, phi, dphi,
-                       F.me.template as<T>());
+                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(),
+                       (NK == 2 ? F.unit : F.opunit).template as<T>(), kp, phi, dphi, F.me.template as<T>());
   mark(1);
   for (int l = L; l >= 1; --l)
     launch_translate<T, 0, NK>(*this, stream, m2m_cl, lvl<NK>(far.m2m, l), lvl<NK>(far.m2m, l + 1),
@@ -1193,12 +1354,7 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
   FarDev<T> &F = far_of<T>(*this);
   const int nrows = (int)near.rows.size();
   mark(3);
-  if (near.nleaf > 0) {
-    const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
-    hipLaunchKernelGGL((k_spmv<T, NK>), dim3(per_xcd * kXcd), dim3(256), 0, stream, near.nleaf, per_xcd,
-                       d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_cols.as<int>(),
-                       d_val_beg.as<int64_t>(), F.vals.template as<T>(), phi, dphi, outN, outD);
-  }
+  if (near.nleaf > 0) launch_spmv<T, NK>(spmv_variant[sizeof(T) == 4 ? 1 : 0][NK - 1], phi, dphi, outN, outD);
   mark(4);
   const int64_t nle = (int64_t)tree.nb * NK * kp;
   hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(nle, 256), 4096)), dim3(256), 0, stream,
@@ -1207,15 +1363,11 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
   launch_translate<T, 1, NK>(*this, stream, m2l_cl, 0, nchunks<NK>(far.m2l), F.m2l.template as<T>(),
                              F.me.template as<T>(), F.le.template as<T>());
   mark(6);
-  const int npairs = (int)far.m2p_tgt.size();
-  if (npairs > 0) {
-    const int64_t total = far.m2p_work_beg.back();
-    if (total > 0)
-      hipLaunchKernelGGL((k_m2p<T, NK>), dim3(nblk(total, 256)), dim3(256), 0, stream, npairs,
-                         d_m2p_work.as<int64_t>(), d_m2p_tgt.as<int>(), d_m2p_src.as<int>(), d_node_beg.as<int>(),
-                         d_nperm.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt, kp,
-                         F.me.template as<T>(), outN, outD);
-  }
+  const int64_t nitems = (int64_t)far.m2p_item_node.size();
+  if (nitems > 0)
+    hipLaunchKernelGGL((k_m2p<T, NK>), dim3((unsigned)nblk(nitems, 256)), dim3(256), 0, stream, nitems,
+                       d_m2p_node.as<int>(), d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
+                       d_delta.as<double>(), p, kt, kp, F.me.template as<T>(), outN, outD);
   mark(7);
   for (int l = 2; l <= L; ++l)
     launch_translate<T, 2, NK>(*this, stream, l2l_cl, lvl<NK>(far.l2l, l), lvl<NK>(far.l2l, l + 1),
@@ -1237,16 +1389,16 @@ void Engine::matvec(const double *phi, const double *dphi, double *outN, double 
 }
 
 template <typename T>
-void Engine::matvec_combined(const double *phi, const double *dphi, double *out) {
-  generate<T, 1>(phi, dphi);
-  products<T, 1>(phi, dphi, out, nullptr);
+void Engine::matvec_combined(const double *x, double *out) {
+  generate<T, 1>(x, nullptr);
+  products<T, 1>(x, nullptr, out, nullptr);
   collect_phase_times();
 }
 
 template void Engine::matvec<double>(const double *, const double *, double *, double *);
 template void Engine::matvec<float>(const double *, const double *, double *, double *);
-template void Engine::matvec_combined<double>(const double *, const double *, double *);
-template void Engine::matvec_combined<float>(const double *, const double *, double *);
+template void Engine::matvec_combined<double>(const double *, double *);
+template void Engine::matvec_combined<float>(const double *, double *);
 template void Engine::generate<double, 2>(const double *, const double *);
 template void Engine::products<double, 2>(const double *, const double *, double *, double *);
 
@@ -1275,13 +1427,13 @@ void Engine::vmult(const double *src, double *dst, int precision) {
     shift = res;
   }
   hipLaunchKernelGGL(k_vmult_pre, dim3(g), dim3(256), 0, stream, n, src, d_dir.as<uint8_t>(), shift,
-                     d_sp.as<double>(), d_sd.as<double>());
+                     d_sp.as<double>());
   if (precision == PBFMM_MIXED && have_f32)
-    matvec_combined<float>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>());
+    matvec_combined<float>(d_sp.as<double>(), d_outN.as<double>());
   else
-    matvec_combined<double>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>());
+    matvec_combined<double>(d_sp.as<double>(), d_outN.as<double>());
   hipLaunchKernelGGL(k_vmult_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_alpha.as<double>(),
-                     d_sp.as<double>(), dst);
+                     d_dir.as<uint8_t>(), d_sp.as<double>(), dst);
   if (!have_dirichlet) {
     // dst -= ||dst||_2 (bem_problem.cc:1214-1215)
     const int nchunk = (int)nblk(n, kRedChunk);
@@ -1301,7 +1453,7 @@ void Engine::compute_alpha() {
   const unsigned g = nblk(n, 256);
   hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d_sp.as<double>(), n, -1.0);
   hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d_sd.as<double>(), n, 0.0);
-  matvec_combined<double>(d_sp.as<double>(), d_sd.as<double>(), d_alpha.as<double>());
+  matvec<double>(d_sp.as<double>(), d_sd.as<double>(), d_alpha.as<double>(), d_outD.as<double>());
   HIPCHK(hipGetLastError());
 }
 
@@ -1309,9 +1461,9 @@ void Engine::compute_rhs(const double *gvec, double *rhs) {
   const unsigned g = nblk(n, 256);
   hipLaunchKernelGGL(k_rhs_pre, dim3(g), dim3(256), 0, stream, n, gvec, d_dir.as<uint8_t>(), d_sp.as<double>(),
                      d_sd.as<double>());
-  matvec_combined<double>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>());
-  hipLaunchKernelGGL(k_rhs_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_alpha.as<double>(),
-                     d_sp.as<double>(), rhs);
+  matvec<double>(d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
+  hipLaunchKernelGGL(k_rhs_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_outD.as<double>(),
+                     d_alpha.as<double>(), d_sp.as<double>(), rhs);
   HIPCHK(hipGetLastError());
 }
 

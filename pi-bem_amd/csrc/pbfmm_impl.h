@@ -131,6 +131,7 @@ struct FarPlan {
   // M2P pairs (target block, smaller far source block)
   std::vector<int32_t> m2p_tgt, m2p_src;
   std::vector<int64_t> m2p_work_beg;  // prefix of node counts
+  std::vector<int32_t> m2p_item_node, m2p_item_src;  // per (pair, target node) evaluation
   // L2P
   std::vector<int32_t> l2p_leaf;  // childless blocks holding nodes
   // operators, transposed (column-major: op[k*kp + row]), FP64

@@ -626,6 +626,10 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
             fp.m2p_tgt.push_back(b);
             fp.m2p_src.push_back(src);
             fp.m2p_work_beg.push_back(fp.m2p_work_beg.back() + t.node_cnt[b]);
+            for (int i = 0; i < t.node_cnt[b]; ++i) {
+              fp.m2p_item_node.push_back(t.nperm[t.node_beg[b] + i]);
+              fp.m2p_item_src.push_back(src);
+            }
           }
         }
     }

@@ -175,11 +175,11 @@ class BEMFMA:
         return dict(zip(("spmv", "p2m", "m2m", "l2l", "m2l", "m2p", "l2p", "total"), ms.tolist()))
 
     def work_counts(self):
-        c = np.zeros(14, np.int64)
+        c = np.zeros(16, np.int64)
         self._chk(self._L.pbfmm_work_counts(self._h, _p(c, _i64)))
         keys = ("n_dofs", "near_values", "near_cols", "near_rows", "leaves", "quad_points", "m2l_pairs",
                 "m2p_evals", "m2m_translations", "l2l_translations", "num_blocks", "trunc_order", "kp",
-                "p2m_cols")
+                "p2m_cols", "spmv_kernel_fp64", "spmv_kernel_fp32")
         return dict(zip(keys, (int(x) for x in c)))
 
 
