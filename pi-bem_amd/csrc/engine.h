@@ -59,6 +59,8 @@ struct Engine {
   NearPlan near;
   FarPlan far;
   bool have_dirichlet = false;
+  bool neumann_mean = false;  // pure Neumann: A0 x + mean(x) 1 (set during GMRES)
+  DevBuf d_ones;
   // device: mesh
   DevBuf d_xyz, d_geom, d_cdofs, d_dbl_ptr, d_dbl_idx, d_dir;
   // device: tree

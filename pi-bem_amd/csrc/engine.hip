@@ -412,7 +412,12 @@ __global__ void __launch_bounds__(256) k_p2m_unit(
     wj[j] = qsh[g * dpc + j] * qw[g] / (4 * kPi);
     col[j] = U + (int64_t)qcol[(int64_t)i * dpc + j] * 2 * kp;
   }
-  auto emit = [&](int k, double Rr, double Ri, double Gr, double Gi) {
+  // level-scaled multipoles: coefficient of degree n divided by delta^n
+  const double inv_delta = 1.0 / delta[b];
+  auto emit = [&](int k, int n, double Rr, double Ri, double Gr, double Gi) {
+    double sc = 1.0;
+    for (int q = 0; q < n; ++q) sc *= inv_delta;
+    Rr *= sc; Ri *= sc; Gr *= sc; Gi *= sc;
     for (int j = 0; j < dpc; ++j) {
       atomicAdd(col[j] + k, wj[j] * Gr);
       atomicAdd(col[j] + kt + k, wj[j] * Gi);
@@ -430,13 +435,13 @@ __global__ void __launch_bounds__(256) k_p2m_unit(
       Rmr = nr; Rmi = ni; Gmr = gr; Gmi = gi;
     }
     double Rcr = Rmr, Rci = Rmi, Gcr = Gmr, Gci = Gmi, Rpr = 0, Rpi = 0, Gpr = 0, Gpi = 0;
-    emit(m * (m + 1) / 2 + m, Rcr, Rci, Gcr, Gci);
+    emit(m * (m + 1) / 2 + m, m, Rcr, Rci, Gcr, Gci);
     for (int n = m + 1; n <= p; ++n) {
       const double a = RA<double>(n, m), bb = RB<double>(n, m);
       const double Rnr = a * z * Rcr - bb * r2 * Rpr, Rni = a * z * Rci - bb * r2 * Rpi;
       const double Gnr = a * (nz * Rcr + z * Gcr) - bb * (dn2 * Rpr + r2 * Gpr);
       const double Gni = a * (nz * Rci + z * Gci) - bb * (dn2 * Rpi + r2 * Gpi);
-      emit(n * (n + 1) / 2 + m, Rnr, Rni, Gnr, Gni);
+      emit(n * (n + 1) / 2 + m, n, Rnr, Rni, Gnr, Gni);
       Rpr = Rcr; Rpi = Rci; Gpr = Gcr; Gpi = Gci;
       Rcr = Rnr; Rci = Rni; Gcr = Gnr; Gci = Gni;
     }
@@ -698,9 +703,11 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(const int *__res
 // Evaluation of both kernels' expansions at a point: M2P
 // (MultipoleExpansion::Evaluate, multipole_expansion.cc:264-298, rho^(-n-1))
 // or L2P (LocalExpansion::Evaluate, local_expansion.cc:251-287, rho^n).
-// NK = 1: the single expansion E (v2 unused).
+// NK = 1: the single expansion E (v2 unused).  E is level-scaled by the box
+// size sc of its block (ME_n / sc^n, LE_n sc^n), so the radial factors are
+// (sc/rho)^n / rho and (rho/sc)^n.
 template <typename T, bool LOCAL, int NK>
-__device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, int kp, T d0, T d1, T d2,
+__device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, int kp, T d0, T d1, T d2, T sc,
                                       T &v1, T &v2) {
   // spherical angles of d without trigonometric calls: cos(alpha) = z/rho,
   // sin(alpha) = rho_xy/rho, e^{i beta} = (x + i y)/rho_xy (beta = 0 on the axis)
@@ -710,8 +717,8 @@ __device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, in
   const T x = d2 / rho;
   const T s = rxy / rho;
   const T cb = rxy > T(0) ? d0 / rxy : T(1), sb = rxy > T(0) ? d1 / rxy : T(0);
-  const T step = LOCAL ? rho : T(1) / rho;
-  T rstart = LOCAL ? T(1) : step;
+  const T step = LOCAL ? rho / sc : sc / rho;
+  T rstart = LOCAL ? T(1) : T(1) / rho;
   T pmm = 1, cm = 1, sm = 0;
   v1 = 0;
   v2 = 0;
@@ -756,7 +763,7 @@ __global__ void __launch_bounds__(256) k_m2p(int64_t nitems, const int *__restri
   const T d1 = (T)(xyz[3 * node + 1] - (pmin[3 * sblk + 1] + hd));
   const T d2 = (T)(xyz[3 * node + 2] - (pmin[3 * sblk + 2] + hd));
   T v1, v2;
-  eval2<T, false, NK>(ME + (int64_t)sblk * NK * kp, p, kt, kp, d0, d1, d2, v1, v2);
+  eval2<T, false, NK>(ME + (int64_t)sblk * NK * kp, p, kt, kp, d0, d1, d2, (T)delta[sblk], v1, v2);
   atomicAdd(outN + node, (double)v1);
   if constexpr (NK == 2) atomicAdd(outD + node, (double)v2);
 }
@@ -778,7 +785,7 @@ __global__ void __launch_bounds__(256) k_l2p(int nrows, const int *__restrict__ 
   const T d1 = (T)(xyz[3 * node + 1] - (pmin[3 * b + 1] + hd));
   const T d2 = (T)(xyz[3 * node + 2] - (pmin[3 * b + 2] + hd));
   T v1, v2;
-  eval2<T, true, NK>(LE + (int64_t)b * NK * kp, p, kt, kp, d0, d1, d2, v1, v2);
+  eval2<T, true, NK>(LE + (int64_t)b * NK * kp, p, kt, kp, d0, d1, d2, (T)delta[b], v1, v2);
   outN[node] += (double)v1;
   if constexpr (NK == 2) outD[node] += (double)v2;
 }
@@ -827,6 +834,11 @@ __global__ void k_rhs_post(int64_t n, const double *__restrict__ N, const double
 __global__ void k_shift(int64_t n, double *__restrict__ v, const double *__restrict__ s) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) v[i] -= *s;
+}
+// v += (*sum) * scale
+__global__ void k_add_scaled_scalar(int64_t n, double *__restrict__ v, const double *__restrict__ sum, double scale) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) v[i] += *sum * scale;
 }
 __global__ void k_constrain(int64_t n, const uint8_t *__restrict__ is_c, const int *__restrict__ cptr,
                             const int *__restrict__ cidx, const double *__restrict__ cval,
@@ -1414,13 +1426,19 @@ double Engine::norm2(const double *v) {
   return std::sqrt(h);
 }
 
+// BEMProblem::vmult.  Without Dirichlet DoFs the reference shifts src and dst
+// by their l2 norms (bem_problem.cc:1169-1172, 1214-1215), which is not a linear
+// map; vmult does the same.  The GMRES solves (neumann_mean) use instead the
+// linear operator A0 x + mean(x) 1, A0 the unshifted one: A0 is singular on
+// constants (alpha = -N.1), the rank-one term removes that and makes the solution
+// the zero-mean one, which is what the reference's comment asks of the shift.
 void Engine::vmult(const double *src, double *dst, int precision) {
   const unsigned g = nblk(n, 256);
   double *shift = nullptr;
   double *res = d_red.as<double>() + (size_t)nblk(n, kRedChunk) * 128;
-  if (!have_dirichlet) {
+  const int nchunk = (int)nblk(n, kRedChunk);
+  if (!have_dirichlet && !neumann_mean) {
     // serv_phi -= ||src||_2 (bem_problem.cc:1169-1172)
-    const int nchunk = (int)nblk(n, kRedChunk);
     hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, src, src, d_red.as<double>());
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, stream, nchunk, d_red.as<double>(), res);
     hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(64), 0, stream, res);
@@ -1435,12 +1453,23 @@ void Engine::vmult(const double *src, double *dst, int precision) {
   hipLaunchKernelGGL(k_vmult_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_alpha.as<double>(),
                      d_dir.as<uint8_t>(), d_sp.as<double>(), dst);
   if (!have_dirichlet) {
-    // dst -= ||dst||_2 (bem_problem.cc:1214-1215)
-    const int nchunk = (int)nblk(n, kRedChunk);
-    hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, dst, dst, d_red.as<double>());
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, stream, nchunk, d_red.as<double>(), res);
-    hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(64), 0, stream, res);
-    hipLaunchKernelGGL(k_shift, dim3(g), dim3(256), 0, stream, n, dst, res);
+    if (neumann_mean) {
+      // dst += (sum src / n) 1
+      if (!d_ones.p) {
+        alloc(d_ones, n * 8);
+        hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d_ones.as<double>(), n, 1.0);
+      }
+      hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, src, d_ones.as<double>(),
+                         d_red.as<double>());
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, stream, nchunk, d_red.as<double>(), res);
+      hipLaunchKernelGGL(k_add_scaled_scalar, dim3(g), dim3(256), 0, stream, n, dst, res, 1.0 / (double)n);
+    } else {
+      // dst -= ||dst||_2 (bem_problem.cc:1214-1215)
+      hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, dst, dst, d_red.as<double>());
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, stream, nchunk, d_red.as<double>(), res);
+      hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(64), 0, stream, res);
+      hipLaunchKernelGGL(k_shift, dim3(g), dim3(256), 0, stream, n, dst, res);
+    }
   }
   if (have_constraints)
     hipLaunchKernelGGL(k_constrain, dim3(g), dim3(256), 0, stream, n, d_c_is.as<uint8_t>(), d_c_ptr.as<int>(),
@@ -1698,6 +1727,12 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
   }
   const unsigned g = nblk(n, 256);
   const double bn = std::max(norm2(b), 1e-300);
+  struct MeanMode {  // linear pure-Neumann operator for the duration of the solve
+    bool &flag;
+    bool old;
+    MeanMode(bool &f, bool v) : flag(f), old(f) { flag = v; }
+    ~MeanMode() { flag = old; }
+  } mean_mode(neumann_mean, !have_dirichlet);
   int it = 0;
   double est = 0, rr = 1;
   double *r = d_r.as<double>(), *d = d_x.as<double>();

@@ -551,12 +551,32 @@ static void add_class(ChunkList &cl, int op, const std::vector<std::pair<int, in
   }
 }
 
+// Degree n of each row of a real expansion vector [Re(0..Kt-1), Im(0..Kt-1),
+// padding] (index n(n+1)/2 + m); padding rows get degree 0.
+static std::vector<int> row_degrees(int p, int kp) {
+  const int kt = kt_of(p);
+  std::vector<int> deg(kp, 0);
+  for (int n = 0; n <= p; ++n)
+    for (int m = 0; m <= n; ++m) {
+      deg[n * (n + 1) / 2 + m] = n;
+      deg[kt + n * (n + 1) / 2 + m] = n;
+    }
+  return deg;
+}
+
+// opT[k*kp + row] *= in_scale^deg(k) * out_scale^deg(row)
+static void scale_op(double *opT, int kp, const std::vector<int> &deg, double in_scale, double out_scale) {
+  for (int k = 0; k < kp; ++k)
+    for (int r = 0; r < kp; ++r) opT[(size_t)k * kp + r] *= std::pow(in_scale, deg[k]) * std::pow(out_scale, deg[r]);
+}
+
 void build_far_plan(const Tree &t, int p, FarPlan &fp) {
   fp = FarPlan();
   fp.p = p;
   fp.kt = kt_of(p);
   fp.kp = kp_of(p);
   const int L = t.L, kp = fp.kp;
+  const std::vector<int> deg = row_degrees(p, kp);
   fp.p2m_leaf = t.leaves;
   for (int b : t.leaves)
     if (t.node_cnt[b] > 0) fp.l2p_leaf.push_back(b);
@@ -572,6 +592,10 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
       op_m2m(p, d, &fp.m2m_ops[((size_t)l * 8 + o) * kp * kp]);
       const double dn[3] = {-d[0], -d[1], -d[2]};
       op_l2l(p, dn, &fp.l2l_ops[((size_t)l * 8 + o) * kp * kp]);
+      // level-scaled expansions: child ME/delta_l^n -> parent ME/delta_{l-1}^n,
+      // parent LE.delta_{l-1}^n -> child LE.delta_l^n
+      scale_op(&fp.m2m_ops[((size_t)l * 8 + o) * kp * kp], kp, deg, dl, 1.0 / (2.0 * dl));
+      scale_op(&fp.l2l_ops[((size_t)l * 8 + o) * kp * kp], kp, deg, 1.0 / (2.0 * dl), dl);
     }
   }
   // M2M pairs (bem_fma.cc:2318-2339: every block of level l into its parent)
@@ -638,6 +662,7 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
       fp.m2l_ops.resize((size_t)fp.n_m2l_ops * kp * kp);
       const double d[3] = {kv.first[0] * dl, kv.first[1] * dl, kv.first[2] * dl};
       op_m2l(p, d, &fp.m2l_ops[(size_t)op * kp * kp]);
+      scale_op(&fp.m2l_ops[(size_t)op * kp * kp], kp, deg, dl, dl);  // ME/delta^n -> LE.delta^n
       add_class(fp.m2l, op, kv.second);
       fp.m2l_per_level[l] += (int64_t)kv.second.size();
     }

@@ -166,6 +166,51 @@ def double_node_cube(n_cells_per_edge: int, half: float = 1.0) -> Mesh:
     return m
 
 
+def wigley_hull(nx: int = 400, nz: int = 100, L: float = 1.0, B: float = 0.1, T: float = 0.0625) -> Mesh:
+    """Closed Wigley-type double body y = +-(B/2)(1 - (2x/L)^2)(1 - (z/T)^2),
+    |x| <= L/2, |z| <= T (the hull of draft T mirrored about z = 0): two Q1
+    sheets of nx x nz cells, one per sign of y, meeting at the stem and stern
+    (x = +-L/2) and at the keel lines (z = +-T), where each sheet keeps its own
+    DoFs, so the DoFs there are double nodes.  Each DoF carries the exact
+    outward normal of its own sheet."""
+    xs = np.linspace(-L / 2, L / 2, nx + 1)
+    zs = np.linspace(-T, T, nz + 1)
+    X, Z = np.meshgrid(xs, zs, indexing="ij")
+    fx = 1.0 - (2.0 * X / L) ** 2
+    fz = 1.0 - (Z / T) ** 2
+    F = 0.5 * B * fx * fz
+    dFx = 0.5 * B * (-8.0 * X / L ** 2) * fz
+    dFz = 0.5 * B * fx * (-2.0 * Z / T ** 2)
+    pts, nrm, cells, cface = [], [], [], []
+    nper = (nx + 1) * (nz + 1)
+    for sheet, sgn in ((0, 1.0), (1, -1.0)):
+        P = np.stack([X, sgn * F, Z], axis=-1).reshape(-1, 3)
+        Nv = np.stack([-dFx, sgn * np.ones_like(F), -dFz], axis=-1).reshape(-1, 3)
+        Nv /= np.linalg.norm(Nv, axis=1)[:, None]
+        pts.append(P)
+        nrm.append(Nv)
+        ids = sheet * nper + np.arange(nper).reshape(nx + 1, nz + 1)
+        a00, a10, a01, a11 = ids[:-1, :-1], ids[1:, :-1], ids[:-1, 1:], ids[1:, 1:]
+        if sgn > 0:  # xi along z, eta along x: x_xi cross x_eta points to +y
+            c = np.stack([a00, a01, a10, a11], axis=-1)
+        else:        # xi along x, eta along z: points to -y
+            c = np.stack([a00, a10, a01, a11], axis=-1)
+        cells.append(c.reshape(-1, 4))
+        cface.append(np.full(nx * nz, sheet, dtype=np.int32))
+    xyz = np.concatenate(pts)
+    m = Mesh(xyz, np.concatenate(cells).astype(np.int64), 4, np.concatenate(nrm), np.concatenate(cface),
+             name=f"wigley{nx}x{nz}")
+    m.dbl_ptr, m.dbl_idx = _double_nodes(xyz)
+    return m
+
+
+def unit_directions(seed: int, n: int) -> np.ndarray:
+    """n seeded directions uniform on the unit sphere."""
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(n, 3))
+    return u / np.linalg.norm(u, axis=1)[:, None]
+
+
 def dirichlet_mask(mesh: Mesh, cell_is_dirichlet: np.ndarray) -> np.ndarray:
     mask = np.zeros(mesh.n_dofs, dtype=np.uint8)
     mask[mesh.cell_dofs[cell_is_dirichlet].ravel()] = 1
@@ -232,6 +277,10 @@ class Problem:
     levels: int = 10
     quad_order: int = 4
     sing_quad_order: int = 5
+    # several right-hand sides of one operator: (n_rhs, N) boundary data and
+    # the exact solution of each
+    rhs_set: np.ndarray = None
+    exact_set: list = None
 
 
 def config_sphere(n: int, p: int, deg: int = 1) -> Problem:
@@ -263,6 +312,19 @@ def config_dcube(n: int, p: int, seed: int = 4) -> Problem:
     y, s = seeded_sources(seed, 8, (1.0, 1.0, 1.0), scale=2.5)
     sol = PointSources(y, s)
     return Problem(m, dmask, sol, bc_vector(m, dmask, sol), p)
+
+
+def config_wigley(nx: int = 400, nz: int = 100, p: int = 8, n_rhs: int = 32, seed: int = 5) -> Problem:
+    """Configuration 5: the Wigley double body (wigley_hull) with n_rhs
+    uniform inflows U_k (seeded directions on the unit sphere): pure Neumann
+    data dphi/dn = -U_k . n, exact interior solution phi_k = -U_k . x + C.
+    tmp_rhs / exact are the first right-hand side; rhs_set / exact_set all."""
+    m = wigley_hull(nx, nz)
+    dmask = np.zeros(m.n_dofs, dtype=np.uint8)
+    dirs = unit_directions(seed, n_rhs)
+    exact_set = [PointSources(np.zeros((0, 3)), [], linear=-u) for u in dirs]
+    rhs_set = np.stack([bc_vector(m, dmask, e) for e in exact_set])
+    return Problem(m, dmask, exact_set[0], rhs_set[0], p, rhs_set=rhs_set, exact_set=exact_set)
 
 
 def constraints(prob: Problem):
