@@ -263,6 +263,101 @@ def measure_config(config, steps, warmup):
                 roofline_matvec=roof2)
 
 
+def measure_multi_rhs(steps):
+    """Configuration 5: the Wigley double body, 32 pure-Neumann right-hand
+    sides solved through the multi-RHS entry point, mixed and FP64."""
+    import pibem_amd as P
+    from pibem_amd import hip
+    from pibem_amd import mesh as M
+    pr = M.config_wigley()
+    n, R = pr.mesh.n_dofs, pr.rhs_set.shape[0]
+    log(f"config 5: N={n} cells={pr.mesh.n_cells} p={pr.p} rhs={R}")
+    stream = hip.Stream()
+    f, bp, setup = setup_handle(pr, stream)
+    cons = M.constraints(pr)
+    bp.set_constraints(cons)
+    rhs = np.stack([bp.compute_rhs(g) for g in pr.rhs_set])
+    rhs[:, cons[4] == 1] = cons[3][cons[4] == 1]
+    drhs, dx = hip.DeviceArray.from_numpy(rhs.ravel()), hip.DeviceArray(n * R)
+    zero = np.zeros(n * R)
+    out = dict(workload="config 5: closed Wigley double body L=1, B=0.1, T=0.0625, 2 x 400 x 100 Q1 cells, "
+                        "81,002 DoFs with stem/keel double nodes, FMM order 8, 32 pure-Neumann right-hand sides "
+                        "-U_k.n (seed 5), each to true rel-res 1e-8",
+               n_dofs=n, n_rhs=R, setup_s=setup)
+    for name, prec in (("mixed", P.PBFMM_MIXED), ("fp64", P.PBFMM_FP64)):
+        dx.upload(zero)
+        f.gmres_dev_multi(drhs.value, dx.value, R, n, 1e-8, 100, 2000, prec)  # warm-up
+        times = []
+        for _ in range(max(1, steps)):
+            dx.upload(zero)
+            stream.synchronize()
+            t = time.perf_counter()
+            its, rr = f.gmres_dev_multi(drhs.value, dx.value, R, n, 1e-8, 100, 2000, prec)
+            stream.synchronize()
+            times.append(time.perf_counter() - t)
+        X = dx.numpy().reshape(R, n)
+        err = max(float(np.linalg.norm((X[r] - X[r].mean()) - (e - e.mean())) / np.linalg.norm(e - e.mean()))
+                  for r, e in enumerate(pr.exact_set[k].phi(pr.mesh.support_xyz) for k in range(R)))
+        ms = min(times) * 1e3
+        out[name] = dict(ms_all_rhs=ms, ms_per_rhs=ms / R, operator_applications=int(its.sum()),
+                         max_relres=float(rr.max()), max_error_vs_analytic=err)
+        log(f"config 5 {name}: {ms:.1f} ms for {R} right-hand sides ({ms / R:.2f} ms each), {int(its.sum())} "
+            f"operator applications, max relres {rr.max():.2e}, max error vs -U.x + C {err:.2e}")
+    out["mixed_speedup_vs_fp64"] = out["fp64"]["ms_all_rhs"] / out["mixed"]["ms_all_rhs"]
+    return out
+
+
+def measure_batch():
+    """Configuration 4: 64 independent problems of uneven size (seed 4), set
+    up and solved one after another (mixed and FP64), each against its own
+    exact solution."""
+    import pibem_amd as P
+    from pibem_amd import hip
+    from pibem_amd import mesh as M
+    t = time.perf_counter()
+    probs = M.config_batch()
+    t_mesh = time.perf_counter() - t
+    ntot = sum(p.mesh.n_dofs for p in probs)
+    log(f"config 4: {len(probs)} problems, {ntot} DoFs in total (meshes {t_mesh:.1f} s)")
+    stream = hip.Stream()
+    acc = dict(setup_s=0.0, mixed_ms=0.0, fp64_ms=0.0, mixed_ops=0, fp64_ops=0, max_error=0.0, max_relres=0.0)
+    for pr in probs:
+        t = time.perf_counter()
+        f, bp, _ = setup_handle(pr, stream)
+        if pr.mesh.dbl_ptr is not None and int(np.diff(pr.mesh.dbl_ptr).max()) > 1:
+            cons = M.constraints(pr)
+            bp.set_constraints(cons)
+        else:
+            cons = None
+        rhs_h = bp.compute_rhs(pr.tmp_rhs)
+        if cons is not None:
+            rhs_h[cons[4] == 1] = cons[3][cons[4] == 1]
+        stream.synchronize()
+        acc["setup_s"] += time.perf_counter() - t
+        n = pr.mesh.n_dofs
+        rhs, x = hip.DeviceArray.from_numpy(rhs_h), hip.DeviceArray(n)
+        neu = pr.dirichlet == 0
+        ex = pr.exact.phi(pr.mesh.support_xyz)
+        for name, prec in (("mixed", P.PBFMM_MIXED), ("fp64", P.PBFMM_FP64)):
+            x.upload(np.zeros(n))
+            stream.synchronize()
+            t = time.perf_counter()
+            it, rr = f.gmres_dev(rhs.value, x.value, 1e-8, 100, 2000, prec)
+            stream.synchronize()
+            acc[name + "_ms"] += (time.perf_counter() - t) * 1e3
+            acc[name + "_ops"] += it
+            acc["max_relres"] = max(acc["max_relres"], rr)
+            s = x.numpy()
+            acc["max_error"] = max(acc["max_error"], float(np.linalg.norm((s - ex)[neu]) / np.linalg.norm(ex[neu])))
+        f.close()
+    log(f"config 4: set-up {acc['setup_s']:.1f} s; solves mixed {acc['mixed_ms']:.0f} ms ({acc['mixed_ops']} "
+        f"operator applications), fp64 {acc['fp64_ms']:.0f} ms ({acc['fp64_ops']}); max relres "
+        f"{acc['max_relres']:.2e}, max error vs analytic phi {acc['max_error']:.2e}")
+    return dict(workload="config 4: 64 problems (seed 4) of 1.5k-25k Q1 DoFs: spheres, spheroids of aspect 1-6, "
+                         "double-node cubes, each with its own 8-source exterior solution, FMM order 6, solved one "
+                         "after another", problems=len(probs), n_dofs_total=ntot, **acc)
+
+
 def cpu_baseline(pr, ops, seconds):
     """The oracle's FP64 vmult (reference algorithm restated in C++/OpenMP) on
     the host cores, over a bounded number of calls, scaled to one solve."""
@@ -345,6 +440,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config3", action="store_true")
+    ap.add_argument("--extra", default="3,4,5", help="further configurations to measure (comma list of 3, 4, 5)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--mode", choices=("full", "matvec", "solve", "copy"), default="full")
     ap.add_argument("--config", type=int, choices=(2, 3), default=2)
@@ -416,7 +512,10 @@ def main():
     out["cpu_baseline"] = cpu
     out["speedup_vs_cpu"] = out["value"] / cpu["value"] if cpu else None
     del c2
-    if not args.no_config3:
+    extra = {int(x) for x in args.extra.split(",") if x.strip()}
+    if args.no_config3:
+        extra.discard(3)
+    if 3 in extra:
         c3 = measure_config(3, max(1, min(args.steps, 3)), 1)
         r3 = c3["roofline"]
         out["config3"] = {
@@ -438,6 +537,11 @@ def main():
             "setup_s": c3["setup"],
             "work_counts": c3["wc"],
         }
+        del c3
+    if 5 in extra:
+        out["config5"] = measure_multi_rhs(args.steps)
+    if 4 in extra:
+        out["config4"] = measure_batch()
     print(json.dumps(out))
 
 

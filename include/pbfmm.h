@@ -162,6 +162,16 @@ int pbfmm_gmres_dev(pbfmm_handle *h, const double *d_rhs, double *d_x, double re
                     int32_t restart, int32_t max_it, int32_t precision, int32_t *iters,
                     double *relres);
 
+/* Several right-hand sides of one operator (device pointers, column r at
+ * d_rhs + r*ld and d_x + r*ld, ld >= n_dofs): each solved as gmres_dev does,
+ * one after another on the handle's stream, sharing the near field, unit
+ * multipoles, translation operators and preconditioner.  iters and relres
+ * have nrhs entries (may be NULL).  No counterpart in the reference, which
+ * solves one right-hand side per BEMProblem::solve. */
+int pbfmm_gmres_dev_multi(pbfmm_handle *h, const double *d_rhs, double *d_x, int32_t nrhs, int64_t ld,
+                          double rel_tol, int32_t restart, int32_t max_it, int32_t precision, int32_t *iters,
+                          double *relres);
+
 /* ---- introspection (tests, profiling) ---------------------------------- */
 /* counts: [0] num_blocks [1] levels [2] n_childless [3] n_sub (sum of
  * sublevels) [4] n_intlist [5] n_nonintlist [6] n_node_entries [7] nnz near */

@@ -466,6 +466,34 @@ int pbfmm_gmres_dev(pbfmm_handle *h, const double *rhs, double *x, double rel_to
   });
 }
 
+int pbfmm_gmres_dev_multi(pbfmm_handle *h, const double *rhs, double *x, int32_t nrhs, int64_t ld, double rel_tol,
+                          int32_t restart, int32_t max_it, int32_t precision, int32_t *iters, double *relres) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "gmres_dev_multi")) return rc;
+    if (!rhs || !x || nrhs < 1 || ld < h->eng.n)
+      return fail(h, PBFMM_ERR_ARG, "gmres_dev_multi: need rhs, x, nrhs >= 1 and ld >= n_dofs");
+    if (restart < 1 || restart > 127 || max_it < 1 || !(rel_tol > 0))
+      return fail(h, PBFMM_ERR_ARG, "gmres_dev_multi: need 1 <= restart <= 127, max_it >= 1, rel_tol > 0");
+    if (precision == PBFMM_MIXED && !h->eng.have_f32)
+      return fail(h, PBFMM_ERR_STATE, "gmres_dev_multi: handle was not created with PBFMM_MIXED");
+    ensure_prec(h);
+    Engine &e = h->eng;
+    int failed = 0;
+    for (int32_t r = 0; r < nrhs; ++r) {
+      int it = 0;
+      double rr = 0;
+      if (e.gmres(rhs + (int64_t)r * ld, x + (int64_t)r * ld, rel_tol, restart, max_it, precision, &it, &rr) != 0)
+        ++failed;
+      if (iters) iters[r] = it;
+      if (relres) relres[r] = rr;
+    }
+    if (failed)
+      return fail(h, PBFMM_ERR_STATE, "gmres_dev_multi: " + std::to_string(failed) + " right-hand side(s) did not "
+                                      "reach rel_tol");
+    return PBFMM_OK;
+  });
+}
+
 int pbfmm_tree_counts(pbfmm_handle *h, int64_t *cnt) {
   return guarded(h, [&]() -> int {
     if (int rc = need(h, 2, "tree_counts")) return rc;

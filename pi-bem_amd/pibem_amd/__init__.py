@@ -130,6 +130,16 @@ class BEMFMA:
         return it.value, rr.value
 
     # ---- introspection ------------------------------------------------------
+    def gmres_dev_multi(self, rhs_ptr, x_ptr, nrhs, ld, rel_tol=1e-8, restart=100, max_it=1000,
+                        precision=PBFMM_FP64):
+        """nrhs device-resident right-hand sides (column r at rhs_ptr + 8 r ld),
+        solved into x_ptr; returns (iterations, relative residuals) per column."""
+        it = np.zeros(nrhs, np.int32)
+        rr = np.zeros(nrhs)
+        self._chk(self._L.pbfmm_gmres_dev_multi(self._h, rhs_ptr, x_ptr, nrhs, ld, rel_tol, restart, max_it,
+                                                 precision, _p(it, _i32), _p(rr)))
+        return it, rr
+
     def tree(self):
         cnt = np.zeros(8, np.int64)
         self._chk(self._L.pbfmm_tree_counts(self._h, _p(cnt, _i64)))

@@ -51,6 +51,8 @@ SIGNATURES = {
     "pbfmm_vmult_dev": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_int32]),
     "pbfmm_gmres_dev": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_int32,
                                   C.POINTER(C.c_int32), _d]),
+    "pbfmm_gmres_dev_multi": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_int32,
+                                        C.c_int32, C.c_int32, _i32, _d]),
     "pbfmm_tree_counts": (C.c_int, [_h, _i64]),
     "pbfmm_tree_export": (C.c_int, [_h, _i32, _i64, _d, _d, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _i64,
                                     _i64]),

@@ -327,6 +327,36 @@ def config_wigley(nx: int = 400, nz: int = 100, p: int = 8, n_rhs: int = 32, see
     return Problem(m, dmask, exact_set[0], rhs_set[0], p, rhs_set=rhs_set, exact_set=exact_set)
 
 
+def config_batch(count: int = 64, p: int = 6, seed: int = 4, n_min: int = 1500, n_max: int = 25000):
+    """Configuration 4: count independent problems of uneven size from one
+    seeded stream: each a sphere, a prolate spheroid of aspect 1..6 or a
+    double-node cube with N drawn uniformly in [n_min, n_max] and rounded to
+    the nearest grid size, Q1, Dirichlet on cells with centroid z > 0, and its
+    own 8-source exterior solution (sources at 1.5x the radii, 2.5x for cubes).
+    Returns a list of Problems."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(count):
+        kind = int(rng.integers(3))
+        target = rng.uniform(n_min, n_max)
+        src_seed = int(rng.integers(1 << 31))
+        if kind == 2:
+            k = max(2, int(round(math.sqrt(target / 6.0))) - 1)
+            m = double_node_cube(k)
+            radii, scale = (1.0, 1.0, 1.0), 2.5
+        else:
+            k = max(2, int(round(math.sqrt((target - 2.0) / 6.0))))
+            aspect = 1.0 if kind == 0 else float(rng.uniform(1.0, 6.0))
+            radii, scale = (aspect, 1.0, 1.0), 1.5
+            m = cube_sphere(k, 1, radii, equiangular=kind == 1)
+        m.name = f"batch{b}-{m.name}"
+        dmask = dirichlet_mask(m, cell_centroids(m)[:, 2] > 0)
+        y, s = seeded_sources(src_seed, 8, radii, scale=scale)
+        sol = PointSources(y, s)
+        out.append(Problem(m, dmask, sol, bc_vector(m, dmask, sol), p))
+    return out
+
+
 def constraints(prob: Problem):
     """Double-node constraints of BEMProblem::compute_constraints
     (bem_problem.cc:1376-1653) for the exact boundary data.
