@@ -73,6 +73,16 @@ int pbfmm_init_fma(pbfmm_handle *h, int64_t n_dofs, const double *support_xyz, i
                    int32_t dpc, const int64_t *cell_dofs, const double *cell_geom,
                    const int64_t *dbl_ptr, const int64_t *dbl_idx, const uint8_t *dirichlet_mask);
 
+/* The double-body operator of BEMProblem<3> (the reference's default problem
+ * type, bem_problem.h:437-445; dense assembly _assemble_system_double_body,
+ * bem_problem.cc:957-1135): every cell also acts through its mirror image about
+ * the plane z = z0, whose quadrature points are reflected, whose normals have
+ * their z component negated and which is always integrated with the regular
+ * rule; collocation points stay on the original surface.  The images become
+ * part of the octree, near field and multipoles.  Call after init_fma and
+ * before generate_octree_blocking; enable = 0 restores the one-body operator. */
+int pbfmm_set_double_body(pbfmm_handle *h, int32_t enable, double z0);
+
 /* BEMFMA<3>::generate_octree_blocking (bem_fma.h:180, bem_fma.cc:3614-4801). */
 int pbfmm_generate_octree_blocking(pbfmm_handle *h);
 

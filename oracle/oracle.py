@@ -41,6 +41,7 @@ def lib():
         L.orc_create.argtypes = [C.c_int] * 5
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_init.argtypes = [C.c_void_p, C.c_int64, _d, C.c_int64, C.c_int, _i64, _d, _i64, _i64, _u8]
+        L.orc_set_double_body.argtypes = [C.c_void_p, C.c_double]
         for f in ("orc_generate_octree_blocking", "orc_direct_integrals", "orc_multipole_integrals"):
             getattr(L, f).argtypes = [C.c_void_p]
         L.orc_generate_multipole_expansions.argtypes = [C.c_void_p, _d, _d]
@@ -199,6 +200,9 @@ class OracleFMM:
         if rc != 0:
             raise RuntimeError("oracle init failed")
         self.n = m.n_dofs
+        z0 = getattr(problem, "mirror_z0", None) if problem is not None else None
+        if z0 is not None:
+            lib().orc_set_double_body(self.h, C.c_double(z0))
 
     def __del__(self):
         if getattr(self, "h", None):

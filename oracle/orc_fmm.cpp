@@ -61,6 +61,7 @@ struct CellME {
 struct Oracle {
   int p = 6, levels = 10, maxnodes = 20, qorder = 4, sqorder = 5;
   int64_t N = 0, ncells = 0;
+  int64_t nreal = 0;  // cells [nreal, ncells) are double-body mirror images, never singular
   int dpc = 4;
   std::vector<double> xyz, geom;
   std::vector<int64_t> cdofs;
@@ -178,6 +179,7 @@ static void generate_octree(Oracle &o) {
   double maxc = 0;
   for (int64_t i = 0; i < o.N; ++i)
     for (int k = 0; k < 3; ++k) maxc = std::max(maxc, std::fabs(o.xyz[3 * i + k]));
+  for (int64_t i = 3 * o.nreal * o.dpc; i < 3 * o.ncells * o.dpc; ++i) maxc = std::max(maxc, std::fabs(o.geom[i]));
 
   o.blocks.clear();
   o.blocks.reserve(1024);
@@ -199,7 +201,8 @@ static void generate_octree(Oracle &o) {
       root.quads[c].push_back(j);
       o.qp2b[c * o.nq + j].push_back(0);
     }
-    for (int j = 0; j < o.dpc; ++j) o.dof2elems[o.cdofs[c * o.dpc + j]].push_back(c);
+    if (c < o.nreal)
+      for (int j = 0; j < o.dpc; ++j) o.dof2elems[o.cdofs[c * o.dpc + j]].push_back(c);
   }
   for (int64_t i = 0; i < o.N; ++i) root.nodes.push_back(i);
   o.blocks.push_back(root);
@@ -404,7 +407,7 @@ static void direct_integrals(Oracle &o) {
       for (auto &cq : dq) {
         const int64_t cell = cq.first;
         int js = -1;
-        for (int j = 0; j < o.dpc; ++j)
+        for (int j = 0; j < o.dpc && cell < o.nreal; ++j)
           if (o.dbl[node].count(o.cdofs[cell * o.dpc + j])) {
             js = j;
             break;
@@ -780,7 +783,7 @@ static void dense_assemble(Oracle &o, int rule, double *Nm, double *Dm) {
     std::fill(dr, dr + o.N, 0.0);
     for (int64_t cell = 0; cell < o.ncells; ++cell) {
       int js = -1;
-      for (int j = 0; j < o.dpc; ++j)
+      for (int j = 0; j < o.dpc && cell < o.nreal; ++j)
         if (o.dbl[i].count(o.cdofs[cell * o.dpc + j])) {
           js = j;
           break;
@@ -904,6 +907,7 @@ int orc_init(void *h, int64_t n, const double *xyz, int64_t ncells, int dpc, con
   if (dpc != 4 && dpc != 9) return -1;
   o.N = n;
   o.ncells = ncells;
+  o.nreal = ncells;
   o.dpc = dpc;
   o.xyz.assign(xyz, xyz + 3 * n);
   o.cdofs.assign(cdofs, cdofs + ncells * dpc);
@@ -924,6 +928,33 @@ int orc_init(void *h, int64_t n, const double *xyz, int64_t ncells, int dpc, con
     o.dir[i] = dirm[i] ? 1.0 : 0.0;
     if (dirm[i]) o.have_dirichlet = true;
   }
+  return 0;
+}
+
+// Double body (BEMProblem::_assemble_system_double_body,
+// bem_problem.cc:957-1135): every real cell also acts through its mirror image
+// about z = z0, with reflected quadrature points, the normal's z component
+// negated and the regular rule only.  The image is the reflected mapping nodes
+// with the reference directions swapped (Q1 {0,2,1,3}, Q2 {0,2,1,3,6,7,4,5,8}),
+// which gives the reflected points with the same shape values, JxW and the
+// reflected normal.  Before generate_octree_blocking.
+int orc_set_double_body(void *h, double z0) {
+  Oracle &o = *(Oracle *)h;
+  static const int perm_q1[4] = {0, 2, 1, 3};
+  static const int perm_q2[9] = {0, 2, 1, 3, 6, 7, 4, 5, 8};
+  const int *perm = o.dpc == 4 ? perm_q1 : perm_q2;
+  const int64_t nr = o.nreal, dpc = o.dpc;
+  o.cdofs.resize(2 * nr * dpc);
+  o.geom.resize(2 * nr * dpc * 3);
+  for (int64_t c = 0; c < nr; ++c)
+    for (int j = 0; j < dpc; ++j) {
+      const int64_t src = c * dpc + perm[j], dst = (nr + c) * dpc + j;
+      o.cdofs[dst] = o.cdofs[src];
+      o.geom[3 * dst] = o.geom[3 * src];
+      o.geom[3 * dst + 1] = o.geom[3 * src + 1];
+      o.geom[3 * dst + 2] = 2.0 * z0 - o.geom[3 * src + 2];
+    }
+  o.ncells = 2 * nr;
   return 0;
 }
 
@@ -1081,7 +1112,10 @@ int orc_solve_system(void *h, const double *tmp_rhs, const int64_t *cptr, const 
 }
 
 int orc_dense_assemble(void *h, int rule, double *Nm, double *Dm) {
-  dense_assemble(*(Oracle *)h, rule, Nm, Dm);
+  Oracle &o = *(Oracle *)h;
+  // the quadrature data come with the octree; build it when not done yet
+  if (o.nq == 0 || (int64_t)o.qw.size() != o.ncells * o.nq) generate_octree(o);
+  dense_assemble(o, rule, Nm, Dm);
   return 0;
 }
 

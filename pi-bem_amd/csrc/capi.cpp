@@ -141,6 +141,7 @@ int pbfmm_init_fma(pbfmm_handle *h, int64_t n_dofs, const double *xyz, int64_t n
     m = HostMesh();
     m.n = n_dofs;
     m.ncells = n_cells;
+    m.ncells_real = n_cells;
     m.dpc = dpc;
     m.xyz.assign(xyz, xyz + 3 * n_dofs);
     m.cdofs.resize(n_cells * dpc);
@@ -190,6 +191,23 @@ int pbfmm_init_fma(pbfmm_handle *h, int64_t n_dofs, const double *xyz, int64_t n
     h->stage = 1;
     h->alpha_ready = false;
     e.have_constraints = false;
+    return PBFMM_OK;
+  }, false);
+}
+
+int pbfmm_set_double_body(pbfmm_handle *h, int32_t enable, double z0) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 1, "set_double_body")) return rc;
+    if (h->stage > 1) return fail(h, PBFMM_ERR_STATE, "set_double_body: call it before generate_octree_blocking");
+    HostMesh &m = h->eng.mesh;
+    m.cdofs.resize(m.ncells_real * m.dpc);
+    m.geom.resize(m.ncells_real * m.dpc * 3);
+    m.ncells = m.ncells_real;
+    m.mirror_z0 = 0.0;
+    if (enable) {
+      if (!std::isfinite(z0)) return fail(h, PBFMM_ERR_ARG, "set_double_body: z0 must be finite");
+      add_mirror_cells(m, z0);
+    }
     return PBFMM_OK;
   }, false);
 }

@@ -28,14 +28,21 @@ inline int kp_of(int p) { return ((2 * kt_of(p) + 15) / 16) * 16; }
 // ---------------------------------------------------------------------------
 struct HostMesh {
   int64_t n = 0, ncells = 0;
+  // cells [ncells_real, ncells) are mirror images of cells [0, ncells_real)
+  // (double body): same DoFs, always integrated with the regular rule
+  int64_t ncells_real = 0;
+  double mirror_z0 = 0.0;
   int dpc = 4;
   std::vector<double> xyz;      // 3n
   std::vector<double> geom;     // ncells*dpc*3 mapping nodes
   std::vector<int32_t> cdofs;   // ncells*dpc
   std::vector<int32_t> dbl_ptr, dbl_idx;  // double-node sets, each holding the dof itself
   std::vector<uint8_t> dir;
-  std::vector<int32_t> d2c_ptr, d2c_idx;  // dof -> cells (ascending)
+  std::vector<int32_t> d2c_ptr, d2c_idx;  // dof -> real cells (ascending)
 };
+// Appends the mirror image of every real cell about the plane z = z0
+// (BEMProblem::_assemble_system_double_body, bem_problem.cc:957-1135).
+void add_mirror_cells(HostMesh &m, double z0);
 
 struct Quadrature2D {
   std::vector<double> pts, w;  // 2*n, n on the unit square, x fastest

@@ -179,6 +179,8 @@ void build_tree(const HostMesh &m, const QuadData &qd, int L, int max_nodes, Tre
   t.L = L;
   double maxc = 0;
   for (int64_t i = 0; i < 3 * N; ++i) maxc = std::max(maxc, std::fabs(m.xyz[i]));
+  // a double body's mirror images must lie in the root box too
+  for (int64_t i = 3 * m.ncells_real * m.dpc; i < 3 * m.ncells * m.dpc; ++i) maxc = std::max(maxc, std::fabs(m.geom[i]));
   t.nperm.resize(N);
   std::iota(t.nperm.begin(), t.nperm.end(), 0);
   t.qperm.resize(NQ);
@@ -360,7 +362,7 @@ void build_near_plan(const HostMesh &m, const QuadData &qd, const Tree &t, NearP
       while (i < q.size() && q[i] / qd.nq == cell) mask |= (uint64_t)1 << (q[i++] % qd.nq);
       np.ent_cell.push_back(cell);
       np.ent_mask.push_back(mask);
-      np.ent_near.push_back(near);
+      np.ent_near.push_back(near && cell < m.ncells_real);  // mirror images are never singular
     }
   };
   auto quads_of = [&](int b, std::vector<int32_t> &q) {
@@ -669,6 +671,31 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
   }
   fp.m2l.level_off[L + 1] = (int)fp.m2l.op.size();
   fp.m2l.level_off1[L + 1] = (int)fp.m2l.op1.size();
+}
+
+// The image of cell c is the reflection x -> (x, y, 2 z0 - z) of its mapping
+// nodes with the two reference directions swapped (local node j of the image
+// is node perm[j] of c): the quadrature points are the reflected ones, the
+// shape function of each DoF is the same at each of them, JxW is unchanged and
+// x_xi cross x_eta of the image is the reflected normal (n_x, n_y, -n_z), which
+// is what the reference integrates (bem_problem.cc:1043-1062).
+void add_mirror_cells(HostMesh &m, double z0) {
+  static const int perm_q1[4] = {0, 2, 1, 3};
+  static const int perm_q2[9] = {0, 2, 1, 3, 6, 7, 4, 5, 8};
+  const int *perm = m.dpc == 4 ? perm_q1 : perm_q2;
+  const int64_t nr = m.ncells_real, dpc = m.dpc;
+  m.cdofs.resize(2 * nr * dpc);
+  m.geom.resize(2 * nr * dpc * 3);
+  for (int64_t c = 0; c < nr; ++c)
+    for (int j = 0; j < dpc; ++j) {
+      const int64_t src = c * dpc + perm[j], dst = (nr + c) * dpc + j;
+      m.cdofs[dst] = m.cdofs[src];
+      m.geom[3 * dst] = m.geom[3 * src];
+      m.geom[3 * dst + 1] = m.geom[3 * src + 1];
+      m.geom[3 * dst + 2] = 2.0 * z0 - m.geom[3 * src + 2];
+    }
+  m.ncells = 2 * nr;
+  m.mirror_z0 = z0;
 }
 
 // Columns of the unit-multipole blocks (multipole_integrals,

@@ -93,6 +93,12 @@ class BEMFMA:
                                          _p(dp, _i64), _p(di, _i64), _p(dm, _u8)))
         self.n = mesh.n_dofs
 
+    def set_double_body(self, enable=True, z0=0.0):
+        """Double-body operator: every cell also acts through its mirror image
+        about z = z0 (BEMProblem::_assemble_system_double_body,
+        bem_problem.cc:957-1135).  Between init_fma and generate_octree_blocking."""
+        self._chk(self._L.pbfmm_set_double_body(self._h, 1 if enable else 0, float(z0)))
+
     def generate_octree_blocking(self):
         self._chk(self._L.pbfmm_generate_octree_blocking(self._h))
 
@@ -257,6 +263,8 @@ def setup(problem, precision=PBFMM_FP64, device=0, preconditioner=1):
                sing_quad_order=problem.sing_quad_order, precision=precision, device=device,
                preconditioner=preconditioner)
     f.init_fma(problem.mesh, problem.dirichlet)
+    if getattr(problem, "mirror_z0", None) is not None:
+        f.set_double_body(True, problem.mirror_z0)
     f.generate_octree_blocking()
     f.direct_integrals()
     f.multipole_integrals()

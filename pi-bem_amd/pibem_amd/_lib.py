@@ -34,6 +34,7 @@ SIGNATURES = {
     "pbfmm_destroy": (C.c_int, [_h]),
     "pbfmm_last_error": (C.c_char_p, [_h]),
     "pbfmm_init_fma": (C.c_int, [_h, C.c_int64, _d, C.c_int64, C.c_int32, _i64, _d, _i64, _i64, _u8]),
+    "pbfmm_set_double_body": (C.c_int, [_h, C.c_int32, C.c_double]),
     "pbfmm_generate_octree_blocking": (C.c_int, [_h]),
     "pbfmm_compute_m2l_flags": (C.c_int, [_h, _i64]),
     "pbfmm_direct_integrals": (C.c_int, [_h]),

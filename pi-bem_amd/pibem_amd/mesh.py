@@ -166,15 +166,18 @@ def double_node_cube(n_cells_per_edge: int, half: float = 1.0) -> Mesh:
     return m
 
 
-def wigley_hull(nx: int = 400, nz: int = 100, L: float = 1.0, B: float = 0.1, T: float = 0.0625) -> Mesh:
+def wigley_hull(nx: int = 400, nz: int = 100, L: float = 1.0, B: float = 0.1, T: float = 0.0625,
+                z_top: float = None) -> Mesh:
     """Closed Wigley-type double body y = +-(B/2)(1 - (2x/L)^2)(1 - (z/T)^2),
     |x| <= L/2, |z| <= T (the hull of draft T mirrored about z = 0): two Q1
     sheets of nx x nz cells, one per sign of y, meeting at the stem and stern
     (x = +-L/2) and at the keel lines (z = +-T), where each sheet keeps its own
     DoFs, so the DoFs there are double nodes.  Each DoF carries the exact
-    outward normal of its own sheet."""
+    outward normal of its own sheet.  z_top = 0 keeps only the submerged half
+    z in [-T, 0] (open at the waterline), the surface a double-body model
+    mirrors about z = 0."""
     xs = np.linspace(-L / 2, L / 2, nx + 1)
-    zs = np.linspace(-T, T, nz + 1)
+    zs = np.linspace(-T, T if z_top is None else z_top, nz + 1)
     X, Z = np.meshgrid(xs, zs, indexing="ij")
     fx = 1.0 - (2.0 * X / L) ** 2
     fz = 1.0 - (Z / T) ** 2
@@ -281,6 +284,8 @@ class Problem:
     # the exact solution of each
     rhs_set: np.ndarray = None
     exact_set: list = None
+    # double body: the mirror plane z = mirror_z0 (None: one body)
+    mirror_z0: float = None
 
 
 def config_sphere(n: int, p: int, deg: int = 1) -> Problem:
@@ -355,6 +360,20 @@ def config_batch(count: int = 64, p: int = 6, seed: int = 4, n_min: int = 1500, 
         sol = PointSources(y, s)
         out.append(Problem(m, dmask, sol, bc_vector(m, dmask, sol), p))
     return out
+
+
+def config_wigley_half(nx: int = 400, nz: int = 50, p: int = 8, n_rhs: int = 4, seed: int = 5) -> Problem:
+    """The submerged half of the Wigley hull (z in [-T, 0]) as a double body
+    mirrored about z = 0, with horizontal inflows U_k (z component dropped, so
+    that the exact solution -U_k . x is even in z): the same problem as
+    config_wigley with 2 nz cells over the full depth."""
+    m = wigley_hull(nx, nz, z_top=0.0)
+    dmask = np.zeros(m.n_dofs, dtype=np.uint8)
+    dirs = unit_directions(seed, n_rhs) * np.array([1.0, 1.0, 0.0])
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    exact_set = [PointSources(np.zeros((0, 3)), [], linear=-u) for u in dirs]
+    rhs_set = np.stack([bc_vector(m, dmask, e) for e in exact_set])
+    return Problem(m, dmask, exact_set[0], rhs_set[0], p, rhs_set=rhs_set, exact_set=exact_set, mirror_z0=0.0)
 
 
 def constraints(prob: Problem):
