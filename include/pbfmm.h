@@ -218,6 +218,55 @@ int pbfmm_last_timings(pbfmm_handle *h, double *ms);
  *      (0 workgroup per leaf, 1 wavefront per leaf) */
 int pbfmm_work_counts(pbfmm_handle *h, int64_t *counts);
 
+/* ---- Post-solve (the step after the solve; real cells only) ----
+ * Vector fields are component-major: v[d * n_dofs + i] is component d at DoF
+ * i, which is the reference's vector DoF numbering after component_wise
+ * renumbering (vec_original_to_sub_wise[n_dofs * d + i], boundary_conditions.cc:226-236).
+ * The L2 projections solve the consistent mass matrix of the scalar element,
+ * built with the QGauss(quad_order) rule of the handle, to a relative
+ * residual of 1e-13 per component; they need init_fma only. */
+
+/* BEMProblem::compute_normals (bem_problem.cc:1997-2079): L2 projection of
+ * the cell normals t1 x t2 / |t1 x t2| onto the vector element (not
+ * normalised afterwards; vector_normals_solution). */
+int pbfmm_compute_normals(pbfmm_handle *h, double *normals);
+/* BEMProblem::compute_gradients (bem_problem.cc:1746-1876): L2 projection of
+ * n * dphi_dn + surface gradient of phi (vector_gradients_solution).  The
+ * result is also kept for pbfmm_hydrodynamic_pressure. */
+int pbfmm_compute_gradients(pbfmm_handle *h, const double *phi, const double *dphi_dn, double *gradients);
+/* BEMProblem::compute_surface_gradients (bem_problem.cc:1878-1995): L2
+ * projection of the surface gradient of tmp_rhs. */
+int pbfmm_compute_surface_gradients(pbfmm_handle *h, const double *tmp_rhs, double *gradients);
+/* Conjugate-gradient steps of the last projection. */
+int pbfmm_projection_iterations(pbfmm_handle *h, int32_t *iters);
+/* BEMProblem::hydrostatic_pressure (bem_problem.cc:2081-2117):
+ * p_i = -density * gravity * (z_i - z0) at every DoF. */
+int pbfmm_hydrostatic_pressure(pbfmm_handle *h, double gravity, double density, double z0, double *pressure);
+/* BEMProblem::hydrodynamic_pressure (bem_problem.cc:2119-2167):
+ * p_i = -density * sum_d u_d (u_d / 2 - W_d), with u the gradients of the last
+ * pbfmm_compute_gradients (PBFMM_ERR_STATE before it) and W the wind function
+ * evaluated at the support points (3 n_dofs, component-major). */
+int pbfmm_hydrodynamic_pressure(pbfmm_handle *h, double density, const double *wind, double *pressure);
+/* Body integrals over the cells with cell_mask[c] != 0 (Body::hasMaterial of
+ * the cell's material id; NULL selects every cell), n_cells entries:
+ * BEMProblem::center_of_pressure (bem_problem.cc:2169-2219): int x p / int p,
+ * center left unchanged when int p is 0;
+ * BEMProblem::pressure_force_and_moment (bem_problem.cc:2221-2272):
+ * force = int p n, moment = int p (cog - x) x n;
+ * BEMProblem::volume_integral (bem_problem.cc:2274-2309): -int x_d n_d. */
+int pbfmm_center_of_pressure(pbfmm_handle *h, const uint8_t *cell_mask, const double *pressure, double *center);
+int pbfmm_pressure_force_and_moment(pbfmm_handle *h, const uint8_t *cell_mask, const double *pressure,
+                                    const double *cog, double *force, double *moment);
+int pbfmm_volume_integral(pbfmm_handle *h, const uint8_t *cell_mask, double *volume);
+/* BEMProblem::free_surface_elevation (bem_problem.cc:2311-2370): for each
+ * (cell, line) pair of cell_lines (line in deal.II numbering 0..3: x = 0,
+ * x = 1, y = 0, y = 1 of the unit cell; the reference visits waterline
+ * boundary lines of body cells in ascending cell, then line, order) and each
+ * of the quad_order Gauss points along it, elevation receives
+ * (x, y, p / (density * gravity)): 3 * quad_order doubles per pair. */
+int pbfmm_free_surface_elevation(pbfmm_handle *h, double gravity, double density, const int32_t *cell_lines,
+                                 int64_t n_lines, const double *pressure, double *elevation);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 HOST_SRC = ["setup.cpp", "capi.cpp"]
-DEV_SRC = ["engine.hip"]
+DEV_SRC = ["engine.hip", "postsolve.hip"]
 HDRS = ["pbfmm_impl.h", "engine.h"]
 
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",

@@ -70,6 +70,47 @@ void ensure_alpha(pbfmm_handle *h) {
     h->eng.have_prec = false;
   }
 }
+// device staging buffer of at least `bytes`, grown on demand
+void *stage(DevBuf &b, size_t bytes) {
+  if (!b.p || b.bytes < bytes) {
+    b.release();
+    bytes = std::max<size_t>(bytes, 16);
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+      b.p = nullptr;
+      throw HipError("device allocation of a post-solve buffer failed");
+    }
+    b.bytes = bytes;
+  }
+  return b.p;
+}
+void h2d_bytes(Engine &e, void *dst, const void *src, size_t bytes) {
+  if (bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, e.stream) != hipSuccess)
+    throw HipError("host to device copy failed");
+}
+// the post-solve data (mass matrix and quadrature tables) on first use
+int ps_ready(pbfmm_handle *h, const char *what) {
+  if (int rc = need(h, 1, what)) return rc;
+  h->eng.ps_setup(h->prm.quad_order);
+  return PBFMM_OK;
+}
+// masked body integrals; mask NULL selects every real cell
+void ps_sums(pbfmm_handle *h, const uint8_t *mask, const double *pressure, const double *cog, double out[13]) {
+  Engine &e = h->eng;
+  const int64_t nc = e.ps.ncells;
+  uint8_t *dm = (uint8_t *)stage(e.ps.mask, (size_t)nc);
+  if (mask) {
+    h2d_bytes(e, dm, mask, (size_t)nc);
+  } else if (hipMemsetAsync(dm, 1, (size_t)nc, e.stream) != hipSuccess) {
+    throw HipError("mask fill failed");
+  }
+  double *dp = (double *)stage(e.ps.in1, (size_t)e.n * 8);
+  if (pressure)
+    h2d(e, dp, pressure, e.n);
+  else if (hipMemsetAsync(dp, 0, (size_t)e.n * 8, e.stream) != hipSuccess)
+    throw HipError("pressure fill failed");
+  const double zero[3] = {0, 0, 0};
+  e.ps_integrals(dm, dp, cog ? cog : zero, out);
+}
 // FMA_preconditioner as called from solve_system (bem_problem.cc:1302-1304)
 void ensure_prec(pbfmm_handle *h) {
   ensure_alpha(h);
@@ -670,6 +711,152 @@ int pbfmm_last_timings(pbfmm_handle *h, double *ms) {
     for (int i = 0; i < 8; ++i) ms[i] = h->eng.last_ms[i];
     return PBFMM_OK;
   }, false);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Post-solve (BEMProblem, bem_problem.cc:1746-2370)
+// ---------------------------------------------------------------------------
+static int ps_project_host(pbfmm_handle *h, int mode, const double *in1, const double *in2, double *out,
+                           const char *what) {
+  return guarded(h, [&]() -> int {
+    if (int rc = ps_ready(h, what)) return rc;
+    if (!out || (mode >= 1 && !in1) || (mode == 1 && !in2))
+      return fail(h, PBFMM_ERR_ARG, std::string(what) + ": null vector");
+    Engine &e = h->eng;
+    double *d1 = nullptr, *d2 = nullptr;
+    if (mode >= 1) {
+      d1 = (double *)stage(e.ps.in1, (size_t)e.n * 8);
+      h2d(e, d1, in1, e.n);
+    }
+    if (mode == 1) {
+      d2 = (double *)stage(e.ps.in2, (size_t)e.n * 8);
+      h2d(e, d2, in2, e.n);
+    }
+    double *dout = (double *)stage(e.ps.out, (size_t)3 * e.n * 8);
+    e.ps_project(mode, d1, d2, dout);
+    if (mode == 1) {
+      double *g = (double *)stage(e.ps.grad, (size_t)3 * e.n * 8);
+      if (hipMemcpyAsync(g, dout, (size_t)3 * e.n * 8, hipMemcpyDeviceToDevice, e.stream) != hipSuccess)
+        throw HipError("gradient copy failed");
+      e.ps.have_grad = true;
+    }
+    d2h(e, out, dout, 3 * e.n);
+    return PBFMM_OK;
+  });
+}
+
+extern "C" {
+
+int pbfmm_compute_normals(pbfmm_handle *h, double *normals) {
+  return ps_project_host(h, 0, nullptr, nullptr, normals, "compute_normals");
+}
+
+int pbfmm_compute_gradients(pbfmm_handle *h, const double *phi, const double *dphi_dn, double *gradients) {
+  return ps_project_host(h, 1, phi, dphi_dn, gradients, "compute_gradients");
+}
+
+int pbfmm_compute_surface_gradients(pbfmm_handle *h, const double *tmp_rhs, double *gradients) {
+  return ps_project_host(h, 2, tmp_rhs, nullptr, gradients, "compute_surface_gradients");
+}
+
+int pbfmm_projection_iterations(pbfmm_handle *h, int32_t *iters) {
+  if (!h || !iters) return PBFMM_ERR_ARG;
+  *iters = h->eng.ps.last_iters;
+  return PBFMM_OK;
+}
+
+int pbfmm_hydrostatic_pressure(pbfmm_handle *h, double gravity, double density, double z0, double *pressure) {
+  return guarded(h, [&]() -> int {
+    if (int rc = ps_ready(h, "hydrostatic_pressure")) return rc;
+    if (!pressure) return fail(h, PBFMM_ERR_ARG, "hydrostatic_pressure: null vector");
+    Engine &e = h->eng;
+    double *dp = (double *)stage(e.ps.out, (size_t)3 * e.n * 8);
+    e.ps_hydrostatic(gravity, density, z0, dp);
+    d2h(e, pressure, dp, e.n);
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_hydrodynamic_pressure(pbfmm_handle *h, double density, const double *wind, double *pressure) {
+  return guarded(h, [&]() -> int {
+    if (int rc = ps_ready(h, "hydrodynamic_pressure")) return rc;
+    if (!wind || !pressure) return fail(h, PBFMM_ERR_ARG, "hydrodynamic_pressure: null vector");
+    Engine &e = h->eng;
+    if (!e.ps.have_grad)
+      return fail(h, PBFMM_ERR_STATE, "hydrodynamic_pressure called before compute_gradients");
+    double *dw = (double *)stage(e.ps.aux, (size_t)3 * e.n * 8);
+    h2d(e, dw, wind, 3 * e.n);
+    double *dp = (double *)stage(e.ps.out, (size_t)3 * e.n * 8);
+    e.ps_hydrodynamic(density, dw, e.ps.grad.as<double>(), dp);
+    d2h(e, pressure, dp, e.n);
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_center_of_pressure(pbfmm_handle *h, const uint8_t *cell_mask, const double *pressure, double *center) {
+  return guarded(h, [&]() -> int {
+    if (int rc = ps_ready(h, "center_of_pressure")) return rc;
+    if (!pressure || !center) return fail(h, PBFMM_ERR_ARG, "center_of_pressure: null argument");
+    double s[13];
+    ps_sums(h, cell_mask, pressure, nullptr, s);
+    if (std::fabs(s[9]) > 0.0)
+      for (int d = 0; d < 3; ++d) center[d] = s[6 + d] / s[9];
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_pressure_force_and_moment(pbfmm_handle *h, const uint8_t *cell_mask, const double *pressure,
+                                    const double *cog, double *force, double *moment) {
+  return guarded(h, [&]() -> int {
+    if (int rc = ps_ready(h, "pressure_force_and_moment")) return rc;
+    if (!pressure || !cog || !force || !moment)
+      return fail(h, PBFMM_ERR_ARG, "pressure_force_and_moment: null argument");
+    double s[13];
+    ps_sums(h, cell_mask, pressure, cog, s);
+    for (int d = 0; d < 3; ++d) {
+      force[d] = s[d];
+      moment[d] = s[3 + d];
+    }
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_volume_integral(pbfmm_handle *h, const uint8_t *cell_mask, double *volume) {
+  return guarded(h, [&]() -> int {
+    if (int rc = ps_ready(h, "volume_integral")) return rc;
+    if (!volume) return fail(h, PBFMM_ERR_ARG, "volume_integral: null argument");
+    double s[13];
+    ps_sums(h, cell_mask, nullptr, nullptr, s);
+    for (int d = 0; d < 3; ++d) volume[d] = s[10 + d];
+    return PBFMM_OK;
+  });
+}
+
+int pbfmm_free_surface_elevation(pbfmm_handle *h, double gravity, double density, const int32_t *cell_lines,
+                                 int64_t n_lines, const double *pressure, double *elevation) {
+  return guarded(h, [&]() -> int {
+    if (int rc = ps_ready(h, "free_surface_elevation")) return rc;
+    if (n_lines < 0 || (n_lines > 0 && (!cell_lines || !pressure || !elevation)))
+      return fail(h, PBFMM_ERR_ARG, "free_surface_elevation: bad arguments");
+    if (!(density * gravity != 0.0)) return fail(h, PBFMM_ERR_ARG, "free_surface_elevation: density * gravity is 0");
+    Engine &e = h->eng;
+    for (int64_t k = 0; k < n_lines; ++k)
+      if (cell_lines[2 * k] < 0 || cell_lines[2 * k] >= e.ps.ncells || cell_lines[2 * k + 1] < 0 ||
+          cell_lines[2 * k + 1] > 3)
+        return fail(h, PBFMM_ERR_ARG, "free_surface_elevation: (cell, line) pair out of range");
+    if (n_lines == 0) return PBFMM_OK;
+    int32_t *df = (int32_t *)stage(e.ps.mask, (size_t)n_lines * 8);
+    h2d_bytes(e, df, cell_lines, (size_t)n_lines * 8);
+    double *dp = (double *)stage(e.ps.in1, (size_t)e.n * 8);
+    h2d(e, dp, pressure, e.n);
+    const int64_t nout = n_lines * e.ps.nq1 * 3;
+    double *dout = (double *)stage(e.ps.out, (size_t)std::max<int64_t>(nout, 3 * e.n) * 8);
+    e.ps_elevation(df, n_lines, dp, density * gravity, dout);
+    d2h(e, elevation, dout, nout);
+    return PBFMM_OK;
+  });
 }
 
 }  // extern "C"

@@ -135,6 +135,35 @@ struct Engine {
             int *iters, double *relres);
   double norm2(const double *v);
   void sync();
+
+  // ---- post-solve (postsolve.hip): BEMProblem::compute_normals/gradients,
+  // pressures and body integrals, over the real cells only ----
+  struct PostSolve {
+    bool ready = false;
+    int nq = 0, nq1 = 0;
+    int64_t ncells = 0, nnz = 0;
+    DevBuf tab, tab1;                       // 2-D rule [w, N, dNx, dNy] per point; 1-D rule [t, w]
+    DevBuf geom, cdofs, xyz;                // real cells; support points
+    DevBuf d2c_ptr, d2c_cell, d2c_loc;      // dof -> (cell, local index), cells ascending
+    DevBuf m_ptr, m_col, m_val, m_diag, m_pos;  // scalar mass matrix (CSR), pos[c][i][j]
+    DevBuf loc;                             // per-cell local matrices / vectors
+    DevBuf b, x, r, z, pv, qv, cg, part;    // CG on three components at once
+    DevBuf in1, in2, out, aux, mask;        // staging of host arrays
+    DevBuf grad;                            // last compute_gradients result (3n)
+    bool have_grad = false;
+    int last_iters = 0;
+  } ps;
+  void ps_setup(int quad_order);
+  // L2 projection onto the vector element: mode 0 cell normals, 1 full
+  // gradient n dphi_dn + surface gradient of phi, 2 surface gradient of in1.
+  // Device vectors; out has 3n entries, component-major.
+  void ps_project(int mode, const double *in1, const double *in2, double *out);
+  void ps_hydrostatic(double gravity, double density, double z0, double *p);
+  void ps_hydrodynamic(double density, const double *wind, const double *grad, double *p);
+  // sums over masked real cells: [0..2] force, [3..5] moment about cog,
+  // [6..8] int x p, [9] int p, [10..12] -int x_d n_d (volume)
+  void ps_integrals(const uint8_t *mask, const double *p, const double cog[3], double out[13]);
+  void ps_elevation(const int32_t *faces, int64_t nfaces, const double *p, double rho_g, double *out);
 };
 
 }  // namespace pbfmm

@@ -6,7 +6,10 @@
 ``generate_multipole_expansions``, ``multipole_matr_vect_products``.
 ``BEMProblem`` carries the FMA branch of ``BEMProblem<3>``
 (include/bem_problem.h:131-242): ``compute_alpha``, ``vmult``,
-``compute_rhs``, ``solve_system``.  Errors raise ``PbfmmError`` (the reference
+``compute_rhs``, ``solve_system``, and the post-solve steps
+(``compute_normals``, ``compute_gradients``, ``compute_surface_gradients``,
+pressures, body integrals and waterline elevation; bem_problem.cc:1746-2370)
+over a ``Body`` (include/Body.h).  Errors raise ``PbfmmError`` (the reference
 throws ``ExcMessage`` through ``AssertThrow``).  All computation runs in the
 HIP library; nothing here computes on the CPU.
 """
@@ -18,7 +21,7 @@ import numpy as np
 
 from ._lib import PBFMM_FP64, PBFMM_MIXED, Params, PbfmmError, load
 
-__all__ = ["BEMFMA", "BEMProblem", "PBFMM_FP64", "PBFMM_MIXED", "PbfmmError", "load"]
+__all__ = ["BEMFMA", "BEMProblem", "Body", "PBFMM_FP64", "PBFMM_MIXED", "PbfmmError", "load"]
 
 _d = C.POINTER(C.c_double)
 _i64 = C.POINTER(C.c_int64)
@@ -92,6 +95,7 @@ class BEMFMA:
         self._chk(self._L.pbfmm_init_fma(self._h, mesh.n_dofs, _p(xyz), mesh.n_cells, mesh.dpc, _p(cd, _i64), None,
                                          _p(dp, _i64), _p(di, _i64), _p(dm, _u8)))
         self.n = mesh.n_dofs
+        self.mesh = mesh
 
     def set_double_body(self, enable=True, z0=0.0):
         """Double-body operator: every cell also acts through its mirror image
@@ -253,6 +257,108 @@ class BEMProblem:
         self.fma._chk(self.fma._L.pbfmm_solve_system(self._h, _p(_f64(tmp_rhs)), _p(phi), _p(dphi), tol, restart,
                                                      max_it, C.byref(it), C.byref(rr)))
         return phi, dphi, it.value, rr.value
+
+    # ---- post-solve (bem_problem.cc:1746-2370) ----
+    def _vec3(self, fn, *args):
+        out = np.zeros(3 * self.fma.n)
+        self.fma._chk(fn(self._h, *args, _p(out)))
+        return out
+
+    def compute_normals(self):
+        """BEMProblem::compute_normals: L2-projected, unnormalised nodal
+        normals, component-major (3 n)."""
+        self.vector_normals_solution = self._vec3(self.fma._L.pbfmm_compute_normals)
+        return self.vector_normals_solution
+
+    def compute_gradients(self, phi, dphi_dn):
+        """BEMProblem::compute_gradients: L2 projection of n dphi_dn + the
+        surface gradient of phi (3 n, component-major)."""
+        self.vector_gradients_solution = self._vec3(self.fma._L.pbfmm_compute_gradients, _p(_f64(phi)),
+                                                    _p(_f64(dphi_dn)))
+        return self.vector_gradients_solution
+
+    def compute_surface_gradients(self, tmp_rhs):
+        """BEMProblem::compute_surface_gradients (3 n, component-major)."""
+        self.vector_surface_gradients_solution = self._vec3(self.fma._L.pbfmm_compute_surface_gradients,
+                                                            _p(_f64(tmp_rhs)))
+        return self.vector_surface_gradients_solution
+
+    def projection_iterations(self):
+        it = C.c_int32(0)
+        self.fma._chk(self.fma._L.pbfmm_projection_iterations(self._h, C.byref(it)))
+        return it.value
+
+    def hydrostatic_pressure(self, gravity, density, z0):
+        p = np.zeros(self.fma.n)
+        self.fma._chk(self.fma._L.pbfmm_hydrostatic_pressure(self._h, float(gravity), float(density), float(z0),
+                                                             _p(p)))
+        return p
+
+    def hydrodynamic_pressure(self, density, wind):
+        """wind: a function of the support points ((n, 3) -> (n, 3)) or its
+        values as a (3 n) component-major array."""
+        w = wind(self.fma.mesh.support_xyz).T.ravel() if callable(wind) else _f64(wind)
+        p = np.zeros(self.fma.n)
+        self.fma._chk(self.fma._L.pbfmm_hydrodynamic_pressure(self._h, float(density), _p(_f64(w)), _p(p)))
+        return p
+
+    def center_of_pressure(self, body, pressure, pressure_center=None):
+        c = np.zeros(3) if pressure_center is None else _f64(pressure_center).copy()
+        m = body.cell_mask(self.fma.mesh)
+        self.fma._chk(self.fma._L.pbfmm_center_of_pressure(self._h, _p(m, _u8), _p(_f64(pressure)), _p(c)))
+        return c
+
+    def pressure_force_and_moment(self, body, pressure):
+        f, mo = np.zeros(3), np.zeros(3)
+        m = body.cell_mask(self.fma.mesh)
+        cog = _f64(body.center_of_gravity)
+        self.fma._chk(self.fma._L.pbfmm_pressure_force_and_moment(self._h, _p(m, _u8), _p(_f64(pressure)),
+                                                                  _p(cog), _p(f), _p(mo)))
+        return f, mo
+
+    def volume_integral(self, body):
+        v = np.zeros(3)
+        m = body.cell_mask(self.fma.mesh)
+        self.fma._chk(self.fma._L.pbfmm_volume_integral(self._h, _p(m, _u8), _p(v)))
+        return v
+
+    def free_surface_elevation(self, gravity, density, body, pressure):
+        """(x, y, p / (density gravity)) at the Gauss points of the body's
+        waterline boundary lines, in the reference's order; (m, 3)."""
+        cl = np.ascontiguousarray(body.waterline_lines(self.fma.mesh), dtype=np.int32)
+        nq = self.fma.params.quad_order
+        out = np.zeros(3 * nq * cl.shape[0])
+        self.fma._chk(self.fma._L.pbfmm_free_surface_elevation(
+            self._h, float(gravity), float(density), cl.ctypes.data_as(_i32), cl.shape[0],
+            _p(_f64(pressure)), _p(out)))
+        return out.reshape(-1, 3)
+
+
+class Body:
+    """include/Body.h: the cells of a body (material ids), its waterline
+    lines (manifold ids), draft and centre of gravity."""
+
+    def __init__(self, material_indices, waterline_indices=(), draft=0.0, center_of_gravity=(0.0, 0.0, 0.0)):
+        self.material_indices = [int(i) for i in material_indices]
+        self.waterline_indices = [int(i) for i in waterline_indices]
+        self.draft = float(draft)
+        self.center_of_gravity = np.asarray(center_of_gravity, dtype=np.float64)
+
+    def cell_mask(self, mesh):
+        """Body::hasMaterial of each cell's material id (Mesh.cell_face)."""
+        return np.ascontiguousarray(np.isin(mesh.cell_face, self.material_indices), dtype=np.uint8)
+
+    def waterline_lines(self, mesh):
+        """(cell, line) pairs of body cells whose line is at the boundary of
+        the surface and carries a waterline manifold id, ascending
+        (bem_problem.cc:2337-2348)."""
+        if mesh.line_ids is None or not self.waterline_indices:
+            return np.zeros((0, 2), dtype=np.int32)
+        from .mesh import boundary_lines
+        sel = (boundary_lines(mesh) & np.isin(mesh.line_ids, self.waterline_indices)
+               & (self.cell_mask(mesh)[:, None] == 1))
+        c, j = np.nonzero(sel)
+        return np.stack([c, j], axis=1).astype(np.int32)
 
 
 def setup(problem, precision=PBFMM_FP64, device=0, preconditioner=1):

@@ -62,6 +62,17 @@ SIGNATURES = {
     "pbfmm_set_phase_timing": (C.c_int, [_h, C.c_int32]),
     "pbfmm_last_timings": (C.c_int, [_h, _d]),
     "pbfmm_work_counts": (C.c_int, [_h, _i64]),
+    "pbfmm_compute_normals": (C.c_int, [_h, _d]),
+    "pbfmm_compute_gradients": (C.c_int, [_h, _d, _d, _d]),
+    "pbfmm_compute_surface_gradients": (C.c_int, [_h, _d, _d]),
+    "pbfmm_projection_iterations": (C.c_int, [_h, C.POINTER(C.c_int32)]),
+    "pbfmm_hydrostatic_pressure": (C.c_int, [_h, C.c_double, C.c_double, C.c_double, _d]),
+    "pbfmm_hydrodynamic_pressure": (C.c_int, [_h, C.c_double, _d, _d]),
+    "pbfmm_center_of_pressure": (C.c_int, [_h, _u8, _d, _d]),
+    "pbfmm_pressure_force_and_moment": (C.c_int, [_h, _u8, _d, _d, _d, _d]),
+    "pbfmm_volume_integral": (C.c_int, [_h, _u8, _d]),
+    "pbfmm_free_surface_elevation": (C.c_int, [_h, C.c_double, C.c_double, C.POINTER(C.c_int32), C.c_int64, _d,
+                                               _d]),
 }
 
 _lib = None

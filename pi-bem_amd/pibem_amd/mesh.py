@@ -48,6 +48,9 @@ class Mesh:
     dbl_ptr: np.ndarray = field(default=None)
     dbl_idx: np.ndarray = field(default=None)
     name: str = ""
+    # manifold id of each cell's four lines (deal.II line numbering: x = 0,
+    # x = 1, y = 0, y = 1 of the unit cell), -1 where none is set
+    line_ids: np.ndarray = field(default=None)
 
     @property
     def n_dofs(self) -> int:
@@ -59,6 +62,35 @@ class Mesh:
 
     def double_sets(self):
         return [set(self.dbl_idx[self.dbl_ptr[i]:self.dbl_ptr[i + 1]].tolist()) for i in range(self.n_dofs)]
+
+
+# vertex DoFs (FE_Q local numbering) at the ends of the four lines of a cell
+CELL_LINES = ((0, 2), (1, 3), (0, 1), (2, 3))
+
+
+def boundary_lines(mesh) -> np.ndarray:
+    """(n_cells, 4) flags of lines that belong to one cell only (deal.II's
+    line->at_boundary() on the surface mesh; double nodes are distinct
+    DoFs, so lines along sharp edges count as boundary lines too)."""
+    cd = mesh.cell_dofs
+    a = np.stack([cd[:, i] for i, _ in CELL_LINES], axis=1)
+    b = np.stack([cd[:, j] for _, j in CELL_LINES], axis=1)
+    lo, hi = np.minimum(a, b).ravel(), np.maximum(a, b).ravel()
+    key = lo.astype(np.int64) * (mesh.n_dofs + 1) + hi
+    _, inv, cnt = np.unique(key, return_inverse=True, return_counts=True)
+    return (cnt[inv] == 1).reshape(-1, 4)
+
+
+def mark_lines(mesh, on_line, line_id: int):
+    """Give line_id to every boundary line whose two end points satisfy
+    on_line ((m, 3) points -> (m,) bools), e.g. a waterline z = z0."""
+    if mesh.line_ids is None:
+        mesh.line_ids = -np.ones((mesh.n_cells, 4), dtype=np.int32)
+    bl = boundary_lines(mesh)
+    for j, (u, v) in enumerate(CELL_LINES):
+        ok = on_line(mesh.support_xyz[mesh.cell_dofs[:, u]]) & on_line(mesh.support_xyz[mesh.cell_dofs[:, v]])
+        mesh.line_ids[bl[:, j] & ok, j] = line_id
+    return mesh
 
 
 def _face_lattice(k: int, face: int, i: np.ndarray, j: np.ndarray) -> np.ndarray:
