@@ -267,6 +267,31 @@ int pbfmm_volume_integral(pbfmm_handle *h, const uint8_t *cell_mask, double *vol
 int pbfmm_free_surface_elevation(pbfmm_handle *h, double gravity, double density, const int32_t *cell_lines,
                                  int64_t n_lines, const double *pressure, double *elevation);
 
+/* ---- Batches of independent problems ----
+ * No counterpart in the reference, which sets up and solves one problem per
+ * run; a batch composes its per-problem steps (BEMFMA set-up,
+ * bem_fma.cc:41-4801; BEMProblem::solve_system, bem_problem.cc:1266-1347) over
+ * many problems as one block-diagonal operator, so that every kernel launch
+ * covers all of them.
+ *
+ * pbfmm_batch_create: parts are handles after init_fma (their octrees and
+ * near/far plans are built here, in parallel on the host, if not built yet);
+ * they must share trunc_order, num_octree_levels, quad orders, precision,
+ * device and element degree, and may not be double bodies or pure Neumann
+ * problems.  The new handle is set up (near field, multipoles) and accepts
+ * every call of the one-problem path on the concatenated vectors (part s at
+ * DoFs dof_off[s] .. dof_off[s+1]); the per-mesh set-up and tree-export calls
+ * return PBFMM_ERR_UNSUPPORTED on it.  The parts stay valid and independent. */
+int pbfmm_batch_create(pbfmm_handle *const *parts, int32_t n_parts, pbfmm_handle **batch);
+/* DoF offsets of the parts: n_parts + 1 entries. */
+int pbfmm_batch_offsets(pbfmm_handle *batch, int64_t *dof_off);
+/* GMRES on device vectors with each part's own true relative residual
+ * ||b_s - A_s x_s|| / ||b_s|| <= rel_tol (written to relres, n_parts
+ * entries): the parts' right-hand sides are scaled to unit norm and the
+ * block-diagonal system is solved once to rel_tol / sqrt(n_parts). */
+int pbfmm_batch_gmres_dev(pbfmm_handle *batch, const double *d_rhs, double *d_x, double rel_tol, int32_t restart,
+                          int32_t max_it, int32_t precision, int32_t *iters, double *relres);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,11 +23,11 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
-HOST_SRC = ["setup.cpp", "capi.cpp"]
+HOST_SRC = ["setup.cpp", "capi.cpp", "forest.cpp"]
 DEV_SRC = ["engine.hip", "postsolve.hip"]
 HDRS = ["pbfmm_impl.h", "engine.h"]
 
-HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread", "-Wall", "-Wno-unused-function",
               f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 DEV_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-munsafe-fp-atomics",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument"]
@@ -58,7 +58,7 @@ def build(verbose: bool = False) -> str:
             print(out)
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, LIB)
     return LIB
 

@@ -3,8 +3,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <atomic>
 #include <exception>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "engine.h"
 
@@ -17,6 +20,7 @@ struct pbfmm_handle {
   int stage = 0;  // 1 init_fma, 2 octree, 3 direct, 4 multipole
   bool alpha_ready = false;
   bool device_ready = false;
+  bool batch = false;  // a forest of parts (pbfmm_batch_create)
 };
 
 namespace {
@@ -61,6 +65,11 @@ void d2h(Engine &e, double *dst, const double *src, int64_t n) {
 }
 int need(pbfmm_handle *h, int stage, const char *what) {
   if (h->stage < stage) return fail(h, PBFMM_ERR_STATE, std::string(what) + " called before its set-up steps");
+  return PBFMM_OK;
+}
+// per-mesh set-up steps and tree exports do not apply to a forest
+int not_batch(pbfmm_handle *h, const char *what) {
+  if (h->batch) return fail(h, PBFMM_ERR_UNSUPPORTED, std::string(what) + " is not available on a batch handle");
   return PBFMM_OK;
 }
 void ensure_alpha(pbfmm_handle *h) {
@@ -171,6 +180,7 @@ int pbfmm_init_fma(pbfmm_handle *h, int64_t n_dofs, const double *xyz, int64_t n
                    const int64_t *cell_dofs, const double *cell_geom, const int64_t *dbl_ptr,
                    const int64_t *dbl_idx, const uint8_t *dirichlet_mask) {
   return guarded(h, [&]() -> int {
+    if (int rc = not_batch(h, "init_fma")) return rc;
     if (n_dofs <= 0 || n_cells <= 0 || !xyz || !cell_dofs || !dirichlet_mask || (dpc != 4 && dpc != 9))
       return fail(h, PBFMM_ERR_ARG, "init_fma: need n_dofs > 0, n_cells > 0, dpc 4 or 9 and non-null arrays");
     if (n_dofs >= (int64_t(1) << 30) || n_cells * dpc >= (int64_t(1) << 31))
@@ -238,6 +248,7 @@ int pbfmm_init_fma(pbfmm_handle *h, int64_t n_dofs, const double *xyz, int64_t n
 
 int pbfmm_set_double_body(pbfmm_handle *h, int32_t enable, double z0) {
   return guarded(h, [&]() -> int {
+    if (int rc = not_batch(h, "set_double_body")) return rc;
     if (int rc = need(h, 1, "set_double_body")) return rc;
     if (h->stage > 1) return fail(h, PBFMM_ERR_STATE, "set_double_body: call it before generate_octree_blocking");
     HostMesh &m = h->eng.mesh;
@@ -255,6 +266,7 @@ int pbfmm_set_double_body(pbfmm_handle *h, int32_t enable, double z0) {
 
 int pbfmm_generate_octree_blocking(pbfmm_handle *h) {
   return guarded(h, [&]() -> int {
+    if (int rc = not_batch(h, "generate_octree_blocking")) return rc;
     if (int rc = need(h, 1, "generate_octree_blocking")) return rc;
     Engine &e = h->eng;
     std::vector<double> gx, gw;
@@ -280,6 +292,7 @@ int pbfmm_compute_m2l_flags(pbfmm_handle *h, int64_t *counts) {
 
 int pbfmm_direct_integrals(pbfmm_handle *h) {
   return guarded(h, [&]() -> int {
+    if (int rc = not_batch(h, "direct_integrals")) return rc;
     if (int rc = need(h, 2, "direct_integrals")) return rc;
     Engine &e = h->eng;
     build_near_plan(e.mesh, e.qd, e.tree, e.near);
@@ -296,10 +309,12 @@ int pbfmm_direct_integrals(pbfmm_handle *h) {
 
 int pbfmm_multipole_integrals(pbfmm_handle *h) {
   return guarded(h, [&]() -> int {
+    if (int rc = not_batch(h, "multipole_integrals")) return rc;
     if (int rc = need(h, 3, "multipole_integrals")) return rc;
     Engine &e = h->eng;
     build_far_plan(e.tree, e.p, e.far);
     build_p2m_plan(e.mesh, e.qd, e.tree, e.far);
+    validate_plans(e);
     e.upload_far(h->prm.precision == PBFMM_MIXED);
     e.build_unit_multipoles(h->prm.precision == PBFMM_MIXED);
     h->stage = 4;
@@ -555,6 +570,7 @@ int pbfmm_gmres_dev_multi(pbfmm_handle *h, const double *rhs, double *x, int32_t
 
 int pbfmm_tree_counts(pbfmm_handle *h, int64_t *cnt) {
   return guarded(h, [&]() -> int {
+    if (int rc = not_batch(h, "tree_counts")) return rc;
     if (int rc = need(h, 2, "tree_counts")) return rc;
     const Tree &t = h->eng.tree;
     cnt[0] = t.nb;
@@ -578,6 +594,7 @@ int pbfmm_tree_export(pbfmm_handle *h, int32_t *level, int64_t *parent, double *
                       int64_t *endL, int64_t *childless, int64_t *sub_ptr, int64_t *il_ptr, int64_t *il_idx,
                       int64_t *nil_ptr, int64_t *nil_idx, int64_t *node_ptr, int64_t *node_idx) {
   return guarded(h, [&]() -> int {
+    if (int rc = not_batch(h, "tree_export")) return rc;
     if (int rc = need(h, 2, "tree_export")) return rc;
     const Tree &t = h->eng.tree;
     for (int l = 0; l <= t.L; ++l) {
@@ -855,6 +872,126 @@ int pbfmm_free_surface_elevation(pbfmm_handle *h, double gravity, double density
     double *dout = (double *)stage(e.ps.out, (size_t)std::max<int64_t>(nout, 3 * e.n) * 8);
     e.ps_elevation(df, n_lines, dp, density * gravity, dout);
     d2h(e, elevation, dout, nout);
+    return PBFMM_OK;
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches of independent problems (a forest of octrees as one operator)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int pbfmm_batch_create(pbfmm_handle *const *parts, int32_t n_parts, pbfmm_handle **out) {
+  if (!out) return PBFMM_ERR_ARG;
+  *out = nullptr;
+  if (!parts || n_parts < 1) return PBFMM_ERR_ARG;
+  for (int32_t s = 0; s < n_parts; ++s)
+    if (!parts[s] || parts[s]->stage < 1 || parts[s]->batch) return PBFMM_ERR_STATE;
+  // host set-up of every part (octree, near-field and far-field plans) on
+  // the host's cores, parts in parallel; nothing here touches the GPU
+  std::atomic<int32_t> next{0};
+  std::vector<std::string> errs(n_parts);
+  auto work = [&]() {
+    for (int32_t s; (s = next++) < n_parts;) {
+      pbfmm_handle *q = parts[s];
+      Engine &e = q->eng;
+      try {
+        if (q->stage < 2) {
+          std::vector<double> gx, gw;
+          gauss_legendre01(q->prm.quad_order, gx, gw);
+          compute_quad_data(e.mesh, tensor_rule(gx, gw, gx, gw), e.qd);
+          e.nq = e.qd.nq;
+          build_tree(e.mesh, e.qd, q->prm.num_octree_levels, q->prm.max_nodes_per_leaf, e.tree);
+          q->stage = 2;
+        }
+        if (q->stage < 4) {
+          build_near_plan(e.mesh, e.qd, e.tree, e.near);
+          build_far_plan(e.tree, e.p, e.far);
+          build_p2m_plan(e.mesh, e.qd, e.tree, e.far);
+        }
+      } catch (const std::exception &ex) {
+        errs[s] = ex.what();
+      }
+    }
+  };
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const unsigned nt = std::min<unsigned>(hw, (unsigned)n_parts);
+  std::vector<std::thread> pool;
+  for (unsigned k = 1; k < nt; ++k) pool.emplace_back(work);
+  work();
+  for (auto &th : pool) th.join();
+  for (int32_t s = 0; s < n_parts; ++s)
+    if (!errs[s].empty()) {
+      parts[s]->err = errs[s];
+      return PBFMM_ERR_ARG;
+    }
+  pbfmm_handle *h = nullptr;
+  try {
+    h = new pbfmm_handle();
+  } catch (...) {
+    return PBFMM_ERR_ALLOC;
+  }
+  h->prm = parts[0]->prm;
+  h->batch = true;
+  std::vector<const Engine *> engs;
+  for (int32_t s = 0; s < n_parts; ++s) engs.push_back(&parts[s]->eng);
+  // host merge and index checks first: nothing reaches the device unless
+  // every index of the merged plans is in range
+  int rc = guarded(h, [&]() -> int {
+    Engine &e = h->eng;
+    for (int32_t s = 0; s < n_parts; ++s)
+      if (parts[s]->prm.precision != h->prm.precision || parts[s]->prm.device != h->prm.device)
+        return fail(h, PBFMM_ERR_ARG, "batch_create: parts differ in precision or device");
+    merge_forest(engs, e, e.seg_off);
+    validate_plans(e);
+    if (e.near.nnz >= (int64_t(1) << 40)) return fail(h, PBFMM_ERR_ARG, "batch_create: near field too large");
+    return PBFMM_OK;
+  }, false);
+  if (rc == PBFMM_OK) rc = guarded(h, [&]() -> int {
+    Engine &e = h->eng;
+    const bool mixed = h->prm.precision == PBFMM_MIXED;
+    e.upload_mesh_and_tree();
+    e.upload_quad(mixed);
+    e.upload_near_plan();
+    e.assemble_near(mixed);
+    e.upload_far(mixed);
+    e.build_unit_multipoles(mixed);
+    e.forest_segments();
+    h->stage = 4;
+    return PBFMM_OK;
+  });
+  if (rc != PBFMM_OK) {
+    if (parts[0]) parts[0]->err = "batch_create: " + h->err;
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return PBFMM_OK;
+}
+
+int pbfmm_batch_offsets(pbfmm_handle *h, int64_t *dof_off) {
+  if (!h || !dof_off) return PBFMM_ERR_ARG;
+  if (!h->batch) return fail(h, PBFMM_ERR_STATE, "batch_offsets: not a batch handle");
+  for (size_t s = 0; s < h->eng.seg_off.size(); ++s) dof_off[s] = h->eng.seg_off[s];
+  return PBFMM_OK;
+}
+
+int pbfmm_batch_gmres_dev(pbfmm_handle *h, const double *d_rhs, double *d_x, double rel_tol, int32_t restart,
+                          int32_t max_it, int32_t precision, int32_t *iters, double *relres) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "batch_gmres_dev")) return rc;
+    if (!h->batch) return fail(h, PBFMM_ERR_STATE, "batch_gmres_dev: not a batch handle");
+    if (!d_rhs || !d_x || restart < 1 || max_it < 1 || !(rel_tol > 0))
+      return fail(h, PBFMM_ERR_ARG, "batch_gmres_dev: bad arguments");
+    if (precision != PBFMM_FP64 && precision != PBFMM_MIXED)
+      return fail(h, PBFMM_ERR_ARG, "batch_gmres_dev: unknown precision");
+    ensure_prec(h);
+    int it = 0;
+    const int bad = h->eng.gmres_forest(d_rhs, d_x, rel_tol, restart, max_it, precision, &it, relres);
+    if (iters) *iters = it;
+    if (bad) return fail(h, PBFMM_ERR_STATE, "batch_gmres_dev: a part did not reach rel_tol");
     return PBFMM_OK;
   });
 }

@@ -136,6 +136,18 @@ struct Engine {
   double norm2(const double *v);
   void sync();
 
+  // ---- forest of independent parts (forest.cpp) ----
+  std::vector<int64_t> seg_off;  // DoF range of each part (n_parts + 1)
+  DevBuf d_seg_off, d_seg, d_segv, d_fb;
+  void forest_segments();        // device copies of seg_off and the DoF -> part map
+  void seg_norms(const double *v, std::vector<double> &out);
+  // GMRES with every part's own relative residual <= tol: each part's
+  // right-hand side scaled to unit norm, one solve of the block-diagonal
+  // system to tol / sqrt(parts), the solution scaled back; relres gets the
+  // true FP64 relative residual of each part
+  int gmres_forest(const double *b, double *x, double tol, int restart, int maxit, int precision, int *iters,
+                   double *relres);
+
   // ---- post-solve (postsolve.hip): BEMProblem::compute_normals/gradients,
   // pressures and body integrals, over the real cells only ----
   struct PostSolve {
@@ -165,5 +177,12 @@ struct Engine {
   void ps_integrals(const uint8_t *mask, const double *p, const double cog[3], double out[13]);
   void ps_elevation(const int32_t *faces, int64_t nfaces, const double *p, double rho_g, double *out);
 };
+
+// Host plans of independent problems merged into one block-diagonal
+// operator (forest.cpp); dof_off receives the DoF range of each part.
+void merge_forest(const std::vector<const Engine *> &parts, Engine &e, std::vector<int64_t> &dof_off);
+// Bounds of every index array of a set-up plan against what it indexes;
+// throws std::runtime_error on the first violation.
+void validate_plans(const Engine &e);
 
 }  // namespace pbfmm

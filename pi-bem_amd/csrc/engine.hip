@@ -979,6 +979,23 @@ __global__ void k_scale_inv(int64_t n, double *__restrict__ v, const double *__r
   if (i < n && s > 0) v[i] /= s;
 }
 
+// Forest segments: sum of squares of each part (one block per part) and a
+// per-part scaling y_i = f[part(i)] x_i.
+__global__ void __launch_bounds__(kRedThreads) k_seg_sumsq(const int64_t *__restrict__ off,
+                                                           const double *__restrict__ v, double *__restrict__ out) {
+  __shared__ double sh[kRedThreads / 64];
+  const int s = blockIdx.x;
+  double acc = 0;
+  for (int64_t i = off[s] + threadIdx.x; i < off[s + 1]; i += kRedThreads) acc += v[i] * v[i];
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[s] = r;
+}
+__global__ void k_seg_scale(int64_t n, const int *__restrict__ seg, const double *__restrict__ f,
+                            const double *__restrict__ x, double *__restrict__ y) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f[seg[i]] * x[i];
+}
+
 // ===========================================================================
 // Host orchestration
 // ===========================================================================
@@ -1757,6 +1774,63 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
   if (iters) *iters = it;
   if (relres) *relres = rr;
   return rr <= tol ? 0 : 1;
+}
+
+void Engine::forest_segments() {
+  const int S = (int)seg_off.size() - 1;
+  std::vector<int32_t> seg(n);
+  for (int s = 0; s < S; ++s)
+    for (int64_t i = seg_off[s]; i < seg_off[s + 1]; ++i) seg[i] = s;
+  up(d_seg_off, seg_off);
+  up(d_seg, seg);
+  alloc(d_segv, (size_t)std::max(S, 1) * 8);
+  alloc(d_fb, (size_t)n * 8);
+}
+
+void Engine::seg_norms(const double *v, std::vector<double> &out) {
+  const int S = (int)seg_off.size() - 1;
+  hipLaunchKernelGGL(k_seg_sumsq, dim3(S), dim3(kRedThreads), 0, stream, d_seg_off.as<int64_t>(), v,
+                     d_segv.as<double>());
+  out.assign(S, 0.0);
+  HIPCHK(hipMemcpyAsync(out.data(), d_segv.p, (size_t)S * 8, hipMemcpyDeviceToHost, stream));
+  sync();
+  for (double &x : out) x = std::sqrt(x);
+}
+
+int Engine::gmres_forest(const double *b, double *x, double tol, int restart, int maxit, int precision,
+                         int *iters, double *relres) {
+  const int S = (int)seg_off.size() - 1;
+  const unsigned g = nblk(n, 256);
+  std::vector<double> bn, f(S), finv(S);
+  seg_norms(b, bn);
+  int nz = 0;
+  for (int s = 0; s < S; ++s) {
+    nz += bn[s] > 0;
+    f[s] = bn[s] > 0 ? 1.0 / bn[s] : 1.0;
+    finv[s] = bn[s] > 0 ? bn[s] : 1.0;
+  }
+  double *fb = d_fb.as<double>(), *sv = d_segv.as<double>();
+  HIPCHK(hipMemcpyAsync(sv, f.data(), (size_t)S * 8, hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_seg_scale, dim3(g), dim3(256), 0, stream, n, d_seg.as<int>(), sv, b, fb);
+  hipLaunchKernelGGL(k_seg_scale, dim3(g), dim3(256), 0, stream, n, d_seg.as<int>(), sv, x, x);
+  sync();
+  int it = 0;
+  gmres(fb, x, tol / std::sqrt((double)std::max(nz, 1)), restart, maxit, precision, &it, nullptr);
+  HIPCHK(hipMemcpyAsync(sv, finv.data(), (size_t)S * 8, hipMemcpyHostToDevice, stream));
+  hipLaunchKernelGGL(k_seg_scale, dim3(g), dim3(256), 0, stream, n, d_seg.as<int>(), sv, x, x);
+  // true residual of every part
+  vmult(x, fb, PBFMM_FP64);
+  hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, b, -1.0, fb);
+  std::vector<double> rn;
+  seg_norms(fb, rn);
+  bool ok = true;
+  for (int s = 0; s < S; ++s) {
+    const double r = bn[s] > 0 ? rn[s] / bn[s] : rn[s];
+    if (relres) relres[s] = r;
+    ok = ok && r <= tol;
+  }
+  if (iters) *iters = it;
+  return ok ? 0 : 1;
 }
 
 }  // namespace pbfmm

@@ -21,7 +21,7 @@ import numpy as np
 
 from ._lib import PBFMM_FP64, PBFMM_MIXED, Params, PbfmmError, load
 
-__all__ = ["BEMFMA", "BEMProblem", "Body", "PBFMM_FP64", "PBFMM_MIXED", "PbfmmError", "load"]
+__all__ = ["BEMFMA", "BatchFMA", "BEMProblem", "Body", "setup_batch", "PBFMM_FP64", "PBFMM_MIXED", "PbfmmError", "load"]
 
 _d = C.POINTER(C.c_double)
 _i64 = C.POINTER(C.c_int64)
@@ -359,6 +359,62 @@ class Body:
                & (self.cell_mask(mesh)[:, None] == 1))
         c, j = np.nonzero(sel)
         return np.stack([c, j], axis=1).astype(np.int32)
+
+
+class BatchFMA(BEMFMA):
+    """Independent problems as one block-diagonal operator
+    (pbfmm_batch_create): the parts' octrees and plans are built in parallel
+    on the host, the near field and multipoles of all parts are set up
+    together, and every call of the one-problem path (matvecs, compute_alpha,
+    vmult, compute_rhs, GMRES) acts on the concatenated vectors, part s at
+    DoFs offsets[s]:offsets[s + 1], with one kernel launch per phase for all
+    parts.  gmres_batch_dev reaches each part's own relative residual."""
+
+    def __init__(self, parts):
+        self._L = load()
+        arr = (C.c_void_p * len(parts))(*[q._h.value for q in parts])
+        h = C.c_void_p()
+        rc = self._L.pbfmm_batch_create(arr, len(parts), C.byref(h))
+        if rc != 0:
+            msg = self._L.pbfmm_last_error(parts[0]._h)
+            raise PbfmmError(rc, msg.decode() if msg else "pbfmm_batch_create failed")
+        self._h = h
+        self.parts = list(parts)
+        self.params = parts[0].params
+        self.p = parts[0].p
+        self.levels = parts[0].levels
+        off = np.zeros(len(parts) + 1, dtype=np.int64)
+        self._chk(self._L.pbfmm_batch_offsets(self._h, _p(off, _i64)))
+        self.offsets = off
+        self.n = int(off[-1])
+
+    def split(self, v):
+        """A concatenated vector as a list of per-part vectors."""
+        return [v[self.offsets[s]:self.offsets[s + 1]] for s in range(len(self.parts))]
+
+    def gmres_batch_dev(self, rhs_ptr, x_ptr, rel_tol=1e-8, restart=100, max_it=1000, precision=PBFMM_FP64,
+                        check=True):
+        """(iterations, per-part true relative residuals)."""
+        it = C.c_int32(0)
+        rr = np.zeros(len(self.parts))
+        rc = self._L.pbfmm_batch_gmres_dev(self._h, rhs_ptr, x_ptr, rel_tol, restart, max_it, precision,
+                                           C.byref(it), _p(rr))
+        if check:
+            self._chk(rc)
+        return it.value, rr
+
+
+def setup_batch(problems, precision=PBFMM_FP64, device=0, preconditioner=1):
+    """A BatchFMA over problems (init_fma per part; the rest of the set-up
+    runs once for the batch)."""
+    parts = []
+    for pr in problems:
+        f = BEMFMA(trunc_order=pr.p, num_octree_levels=pr.levels, max_nodes_per_leaf=pr.max_nodes_per_leaf,
+                   quad_order=pr.quad_order, sing_quad_order=pr.sing_quad_order, precision=precision,
+                   device=device, preconditioner=preconditioner)
+        f.init_fma(pr.mesh, pr.dirichlet)
+        parts.append(f)
+    return BatchFMA(parts)
 
 
 def setup(problem, precision=PBFMM_FP64, device=0, preconditioner=1):

@@ -73,6 +73,10 @@ SIGNATURES = {
     "pbfmm_volume_integral": (C.c_int, [_h, _u8, _d]),
     "pbfmm_free_surface_elevation": (C.c_int, [_h, C.c_double, C.c_double, C.POINTER(C.c_int32), C.c_int64, _d,
                                                _d]),
+    "pbfmm_batch_create": (C.c_int, [C.POINTER(_h), C.c_int32, C.POINTER(_h)]),
+    "pbfmm_batch_offsets": (C.c_int, [_h, _i64]),
+    "pbfmm_batch_gmres_dev": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                                        C.POINTER(C.c_int32), _d]),
 }
 
 _lib = None

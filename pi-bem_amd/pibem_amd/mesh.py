@@ -81,6 +81,20 @@ def boundary_lines(mesh) -> np.ndarray:
     return (cnt[inv] == 1).reshape(-1, 4)
 
 
+def merge_constraints(cons_list, offsets):
+    """Constraint CSRs (ptr, idx, val, inhom, is_constrained) of independent
+    problems as one CSR over the concatenated DoFs (part s from offsets[s])."""
+    ptr, idx, val, inh, isc = [np.zeros(1, dtype=np.int64)], [], [], [], []
+    for (p, i, v, h, c), off in zip(cons_list, offsets):
+        ptr.append(np.asarray(p[1:], dtype=np.int64) + ptr[-1][-1])
+        idx.append(np.asarray(i, dtype=np.int64) + int(off))
+        val.append(np.asarray(v, dtype=np.float64))
+        inh.append(np.asarray(h, dtype=np.float64))
+        isc.append(np.asarray(c, dtype=np.uint8))
+    return (np.concatenate(ptr), np.concatenate(idx) if idx else np.zeros(0, np.int64),
+            np.concatenate(val) if val else np.zeros(0), np.concatenate(inh), np.concatenate(isc))
+
+
 def mark_lines(mesh, on_line, line_id: int):
     """Give line_id to every boundary line whose two end points satisfy
     on_line ((m, 3) points -> (m,) bools), e.g. a waterline z = z0."""

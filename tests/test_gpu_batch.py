@@ -1,0 +1,156 @@
+"""Batches of independent problems on the GPU (pbfmm_batch_create): one
+block-diagonal operator whose every launch covers all parts.
+
+Bars:
+* the batch's FMM matvecs, alpha, vmult (with the parts' double-node
+  constraints) and compute_rhs equal the parts' own, set up and applied one
+  at a time: FP64 within 1e-13, FP32 within 1e-6 (the two may pick different
+  near-field kernels, which sum in different orders);
+* batch GMRES: every part's true relative residual <= 1e-8 in both
+  precisions, and the solution equals the parts' own solves within 1e-6;
+* configuration 4 (64 problems, 902,624 DoFs): every part converges to 1e-8,
+  and phi on the Neumann DoFs matches each part's analytic solution within
+  1e-2 (the FMM order 6 truncation and discretisation error; measured 5.5e-3
+  part by part).
+"""
+import numpy as np
+import pytest
+
+import pibem_amd as P
+from pibem_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+def _constraints(pr):
+    if pr.mesh.dbl_ptr is not None and int(np.diff(pr.mesh.dbl_ptr).max()) > 1:
+        return M.constraints(pr)
+    n = pr.mesh.n_dofs
+    return (np.zeros(n + 1, np.int64), np.zeros(0, np.int64), np.zeros(0), np.zeros(n), np.zeros(n, np.uint8))
+
+
+def _rhs(bp, pr, cons):
+    r = bp.compute_rhs(pr.tmp_rhs)
+    r[cons[4] == 1] = cons[3][cons[4] == 1]  # distribute_rhs, as solve_system does
+    return r
+
+
+@pytest.fixture(scope="module")
+def small():
+    probs = [M.config_sphere(8, 5), M.config_dcube(4, 5), M.config_sphere(12, 5), M.config_dcube(3, 5)]
+    print("\n[case] batch of 4: " + ", ".join(f"{p.mesh.name} N={p.mesh.n_dofs}" for p in probs), flush=True)
+    b = P.setup_batch(probs, P.PBFMM_MIXED)
+    parts = [P.setup(pr, P.PBFMM_MIXED) for pr in probs]
+    return probs, b, parts
+
+
+def test_batch_operator_equals_parts(small):
+    from pibem_amd import hip
+    probs, b, parts = small
+    rng = np.random.default_rng(1234)
+    a = rng.uniform(-1, 1, b.n)
+    d = rng.uniform(-1, 1, b.n)
+    b.generate_multipole_expansions(a, d)
+    N, D = b.multipole_matr_vect_products(a, d)
+    da, dd = hip.DeviceArray.from_numpy(a), hip.DeviceArray.from_numpy(d)
+    oN, oD = hip.DeviceArray(b.n), hip.DeviceArray(b.n)
+    b.matvec_dev(da.value, dd.value, oN.value, oD.value, P.PBFMM_MIXED)
+    hip.synchronize()
+    N32, D32 = oN.numpy(), oD.numpy()
+    for s, f in enumerate(parts):
+        sl = slice(b.offsets[s], b.offsets[s + 1])
+        f.generate_multipole_expansions(a[sl], d[sl])
+        n0, d0 = f.multipole_matr_vect_products(a[sl], d[sl])
+        pa, pd = hip.DeviceArray.from_numpy(a[sl]), hip.DeviceArray.from_numpy(d[sl])
+        qN, qD = hip.DeviceArray(sl.stop - sl.start), hip.DeviceArray(sl.stop - sl.start)
+        f.matvec_dev(pa.value, pd.value, qN.value, qD.value, P.PBFMM_MIXED)
+        hip.synchronize()
+        e64 = max(rel(N[sl], n0), rel(D[sl], d0))
+        e32 = max(rel(N32[sl], qN.numpy()), rel(D32[sl], qD.numpy()))
+        print(f"  part {s}: fp64 {e64:.2e} fp32 {e32:.2e}")
+        assert e64 <= 1e-13 and e32 <= 1e-6
+    bb = P.BEMProblem(b)
+    alpha = bb.compute_alpha()
+    cons = [_constraints(pr) for pr in probs]
+    bb.set_constraints(M.merge_constraints(cons, b.offsets[:-1]))
+    x = rng.uniform(-1, 1, b.n)
+    y = bb.vmult(x)
+    tmp = np.concatenate([pr.tmp_rhs for pr in probs])
+    r = bb.compute_rhs(tmp)
+    for s, f in enumerate(parts):
+        sl = slice(b.offsets[s], b.offsets[s + 1])
+        bp = P.BEMProblem(f)
+        assert rel(alpha[sl], bp.compute_alpha()) <= 1e-13
+        bp.set_constraints(cons[s])
+        assert rel(y[sl], bp.vmult(x[sl])) <= 1e-13
+        assert rel(r[sl], bp.compute_rhs(probs[s].tmp_rhs)) <= 1e-13
+
+
+def test_batch_gmres(small):
+    from pibem_amd import hip
+    probs, b, parts = small
+    bb = P.BEMProblem(b)
+    bb.compute_alpha()
+    cons = [_constraints(pr) for pr in probs]
+    mc = M.merge_constraints(cons, b.offsets[:-1])
+    bb.set_constraints(mc)
+    rhs = bb.compute_rhs(np.concatenate([pr.tmp_rhs for pr in probs]))
+    rhs[mc[4] == 1] = mc[3][mc[4] == 1]
+    dr, dx = hip.DeviceArray.from_numpy(rhs), hip.DeviceArray(b.n)
+    ref = []
+    for s, f in enumerate(parts):
+        bp = P.BEMProblem(f)
+        bp.compute_alpha()
+        bp.set_constraints(cons[s])
+        r1 = hip.DeviceArray.from_numpy(_rhs(bp, probs[s], cons[s]))
+        x1 = hip.DeviceArray(probs[s].mesh.n_dofs)
+        f.gmres_dev(r1.value, x1.value, 1e-10, 100, 1000, P.PBFMM_FP64)
+        ref.append(x1.numpy())
+    for prec in (P.PBFMM_FP64, P.PBFMM_MIXED):
+        dx.upload(np.zeros(b.n))
+        it, rr = b.gmres_batch_dev(dr.value, dx.value, 1e-8, 100, 1000, prec)
+        X = dx.numpy()
+        errs = [rel(X[b.offsets[s]:b.offsets[s + 1]], ref[s]) for s in range(len(parts))]
+        print(f"  precision={prec}: iters {it} relres {np.array2string(rr, precision=2)} vs parts "
+              f"{max(errs):.2e}")
+        assert rr.max() <= 1e-8 and max(errs) <= 1e-6
+
+
+def test_config4_batch():
+    import time
+    from pibem_amd import hip
+    probs = M.config_batch()
+    ntot = sum(p.mesh.n_dofs for p in probs)
+    print(f"\n[case] config 4: {len(probs)} problems, {ntot} DoFs", flush=True)
+    t = time.perf_counter()
+    b = P.setup_batch(probs, P.PBFMM_MIXED)
+    bb = P.BEMProblem(b)
+    bb.compute_alpha()
+    cons = [_constraints(pr) for pr in probs]
+    mc = M.merge_constraints(cons, b.offsets[:-1])
+    bb.set_constraints(mc)
+    rhs = bb.compute_rhs(np.concatenate([pr.tmp_rhs for pr in probs]))
+    rhs[mc[4] == 1] = mc[3][mc[4] == 1]
+    print(f"  set-up {time.perf_counter() - t:.1f} s", flush=True)
+    dr, dx = hip.DeviceArray.from_numpy(rhs), hip.DeviceArray(b.n)
+    for prec in (P.PBFMM_MIXED, P.PBFMM_FP64):
+        dx.upload(np.zeros(b.n))
+        hip.synchronize()
+        t = time.perf_counter()
+        it, rr = b.gmres_batch_dev(dr.value, dx.value, 1e-8, 100, 2000, prec)
+        hip.synchronize()
+        ms = (time.perf_counter() - t) * 1e3
+        X = dx.numpy()
+        err = 0.0
+        for s, pr in enumerate(probs):
+            x = X[b.offsets[s]:b.offsets[s + 1]]
+            ex = pr.exact.phi(pr.mesh.support_xyz)
+            neu = pr.dirichlet == 0
+            err = max(err, rel(x[neu], ex[neu]))
+        print(f"  precision={prec}: {ms:.0f} ms, {it} operator applications, max relres {rr.max():.2e}, "
+              f"max error {err:.2e}", flush=True)
+        assert rr.max() <= 1e-8 and err <= 1e-2
