@@ -287,8 +287,10 @@ int pbfmm_batch_create(pbfmm_handle *const *parts, int32_t n_parts, pbfmm_handle
 int pbfmm_batch_offsets(pbfmm_handle *batch, int64_t *dof_off);
 /* GMRES on device vectors with each part's own true relative residual
  * ||b_s - A_s x_s|| / ||b_s|| <= rel_tol (written to relres, n_parts
- * entries): the parts' right-hand sides are scaled to unit norm and the
- * block-diagonal system is solved once to rel_tol / sqrt(n_parts). */
+ * entries): every part runs its own Arnoldi process (own dot products,
+ * Hessenberg matrix, rotations and stopping test) inside shared basis vectors,
+ * the operator is applied to all parts at once, and a part that has converged
+ * stops taking part; iters counts operator applications. */
 int pbfmm_batch_gmres_dev(pbfmm_handle *batch, const double *d_rhs, double *d_x, double rel_tol, int32_t restart,
                           int32_t max_it, int32_t precision, int32_t *iters, double *relres);
 

@@ -139,12 +139,16 @@ struct Engine {
   // ---- forest of independent parts (forest.cpp) ----
   std::vector<int64_t> seg_off;  // DoF range of each part (n_parts + 1)
   DevBuf d_seg_off, d_seg, d_segv, d_fb;
+  // reduction chunks aligned to the parts: chunk c covers rch_beg[c] ..
+  // rch_beg[c+1], the chunks of part s are sch_off[s] .. sch_off[s+1]
+  int n_rch = 0;
+  DevBuf d_rch_beg, d_sch_off, d_seg_active, d_hs, d_ys;
   void forest_segments();        // device copies of seg_off and the DoF -> part map
   void seg_norms(const double *v, std::vector<double> &out);
-  // GMRES with every part's own relative residual <= tol: each part's
-  // right-hand side scaled to unit norm, one solve of the block-diagonal
-  // system to tol / sqrt(parts), the solution scaled back; relres gets the
-  // true FP64 relative residual of each part
+  // GMRES with every part's own relative residual <= tol: one Arnoldi
+  // process per part inside shared basis vectors (segmented dots, norms and
+  // projections), one operator application for all parts per step; relres
+  // gets the true FP64 relative residual of each part
   int gmres_forest(const double *b, double *x, double tol, int restart, int maxit, int precision, int *iters,
                    double *relres);
 

@@ -996,6 +996,62 @@ __global__ void k_seg_scale(int64_t n, const int *__restrict__ seg, const double
   if (i < n) y[i] = f[seg[i]] * x[i];
 }
 
+// Segmented GMRES building blocks: every part of a forest runs its own
+// Arnoldi process inside shared basis vectors.  Dots over part-aligned chunks
+// (part[j * nch + c]) finished per (part, vector): out[s * ldo + j].
+__global__ void __launch_bounds__(kRedThreads) k_mdot_seg(const int64_t *__restrict__ cbeg, int64_t ldv,
+                                                          const double *__restrict__ V, const double *__restrict__ w,
+                                                          double *__restrict__ part) {
+  __shared__ double sh[kRedThreads / 64];
+  const int c = blockIdx.x, j = blockIdx.y;
+  const double *v = V + (int64_t)j * ldv;
+  double acc = 0;
+  for (int64_t i = cbeg[c] + threadIdx.x; i < cbeg[c + 1]; i += kRedThreads) acc += v[i] * w[i];
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[(int64_t)j * gridDim.x + blockIdx.x] = r;
+}
+__global__ void __launch_bounds__(kRedThreads) k_finish_seg(int nch, const int *__restrict__ sch_off, int ldo,
+                                                            const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double sh[kRedThreads / 64];
+  const int s = blockIdx.x, j = blockIdx.y;
+  double acc = 0;
+  for (int c = sch_off[s] + threadIdx.x; c < sch_off[s + 1]; c += kRedThreads) acc += part[(int64_t)j * nch + c];
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[(int64_t)s * ldo + j] = r;
+}
+// w -= sum_j V_j h[part][j]
+__global__ void k_gemv_sub_seg(int64_t n, int64_t ldv, int nv, const double *__restrict__ V,
+                               const double *__restrict__ h, int ldh, const int *__restrict__ seg,
+                               double *__restrict__ w) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *hs = h + (int64_t)seg[i] * ldh;
+  double acc = w[i];
+  for (int j = 0; j < nv; ++j) acc -= V[(int64_t)j * ldv + i] * hs[j];
+  w[i] = acc;
+}
+// x += sum_j V_j y[part][j]
+__global__ void k_gemv_add_seg(int64_t n, int64_t ldv, int nv, const double *__restrict__ V,
+                               const double *__restrict__ y, int ldy, const int *__restrict__ seg,
+                               double *__restrict__ x) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *ys = y + (int64_t)seg[i] * ldy;
+  double acc = x[i];
+  for (int j = 0; j < nv; ++j) acc += V[(int64_t)j * ldv + i] * ys[j];
+  x[i] = acc;
+}
+// Arnoldi normalisation per part (nrm2 holds squared norms); parts that are
+// no longer active get zero vectors, so they stay inert
+__global__ void k_scale_inv_seg(int64_t n, const int *__restrict__ seg, const double *__restrict__ nrm2,
+                                const uint8_t *__restrict__ active, double *__restrict__ v) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int s = seg[i];
+  const double q = nrm2[s];
+  v[i] = (active[s] && q > 0) ? v[i] / sqrt(q) : 0.0;
+}
+
 // ===========================================================================
 // Host orchestration
 // ===========================================================================
@@ -1785,6 +1841,16 @@ void Engine::forest_segments() {
   up(d_seg, seg);
   alloc(d_segv, (size_t)std::max(S, 1) * 8);
   alloc(d_fb, (size_t)n * 8);
+  std::vector<int64_t> rch{0};
+  std::vector<int32_t> sch{0};
+  for (int s = 0; s < S; ++s) {
+    for (int64_t b = seg_off[s]; b < seg_off[s + 1]; b += kRedChunk) rch.push_back(std::min(b + kRedChunk, seg_off[s + 1]));
+    sch.push_back((int32_t)rch.size() - 1);
+  }
+  n_rch = (int)rch.size() - 1;
+  up(d_rch_beg, rch);
+  up(d_sch_off, sch);
+  alloc(d_seg_active, (size_t)std::max(S, 1));
 }
 
 void Engine::seg_norms(const double *v, std::vector<double> &out) {
@@ -1797,37 +1863,213 @@ void Engine::seg_norms(const double *v, std::vector<double> &out) {
   for (double &x : out) x = std::sqrt(x);
 }
 
+// Segmented restarted GMRES (CGS2, FP64 basis): part s of the forest runs its
+// own Arnoldi process and stops when its residual estimate reaches tol_abs[s];
+// the operator is applied to all parts at once.  Returns 0 when every part
+// reached its tolerance.
+static int gmres_core_seg(Engine &e, const double *b, double *x, const std::vector<double> &tol_abs, int m,
+                          int maxit, int precision, int &it) {
+  const int64_t n = e.n;
+  const int S = (int)e.seg_off.size() - 1, nch = e.n_rch;
+  const unsigned g = nblk(n, 256);
+  double *V = e.d_V.as<double>(), *w = e.d_w.as<double>(), *part = e.d_red.as<double>();
+  const int *seg = e.d_seg.as<int>(), *sch = e.d_sch_off.as<int>();
+  const int64_t *rch = e.d_rch_beg.as<int64_t>();
+  uint8_t *act_d = e.d_seg_active.as<uint8_t>();
+  // Hessenberg slots per step: [h1: S x (m+2)][h2: S x (m+2)][|w|^2: S]
+  const int ldh = m + 2;
+  const size_t SL = (size_t)2 * S * ldh + S;
+  double *hd = e.d_hs.as<double>(), *yd = e.d_ys.as<double>();
+  std::vector<double> hbuf(SL * m);
+  std::vector<double> H((size_t)S * (m + 1) * m), cs((size_t)S * m), sn((size_t)S * m), gv((size_t)S * (m + 1));
+  std::vector<double> est(S), beta, y((size_t)S * m), hcol(m + 2), fsc(S);
+  std::vector<int> ks(S);
+  std::vector<uint8_t> active(S), done(S);
+  auto dots = [&](int nv, const double *Vs, const double *vec, double *out, int ldo) {
+    hipLaunchKernelGGL(k_mdot_seg, dim3(nch, nv), dim3(kRedThreads), 0, e.stream, rch, n, Vs, vec, part);
+    hipLaunchKernelGGL(k_finish_seg, dim3(S, nv), dim3(kRedThreads), 0, e.stream, nch, sch, ldo, part, out);
+  };
+  while (it < maxit) {
+    e.vmult(x, w, precision);
+    hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, b, -1.0, w);
+    e.seg_norms(w, beta);
+    int nact = 0;
+    for (int s = 0; s < S; ++s) {
+      active[s] = beta[s] > tol_abs[s];
+      done[s] = !active[s];
+      nact += active[s];
+      fsc[s] = active[s] ? 1.0 / beta[s] : 0.0;
+      est[s] = beta[s];
+      ks[s] = 0;
+      std::fill(gv.begin() + (size_t)s * (m + 1), gv.begin() + (size_t)(s + 1) * (m + 1), 0.0);
+      gv[(size_t)s * (m + 1)] = beta[s];
+    }
+    if (nact == 0) return 0;
+    HIPCHK(hipMemcpyAsync(e.d_segv.p, fsc.data(), (size_t)S * 8, hipMemcpyHostToDevice, e.stream));
+    HIPCHK(hipMemcpyAsync(act_d, active.data(), (size_t)S, hipMemcpyHostToDevice, e.stream));
+    hipLaunchKernelGGL(k_seg_scale, dim3(g), dim3(256), 0, e.stream, n, seg, e.d_segv.as<double>(), w, V);
+    e.sync();
+    int k = 0, k_done = 0, k_sync = 0;
+    for (; k < m && it < maxit; ++k) {
+      double *vk1 = V + (int64_t)(k + 1) * n;
+      if (e.have_prec) {
+        e.apply_prec(V + (int64_t)k * n, e.d_ptmp.as<double>());
+        e.vmult(e.d_ptmp.as<double>(), vk1, precision);
+      } else {
+        e.vmult(V + (int64_t)k * n, vk1, precision);
+      }
+      ++it;
+      double *slot = hd + (size_t)k * SL;
+      double *h1 = slot, *h2 = slot + (size_t)S * ldh, *hn = slot + (size_t)2 * S * ldh;
+      dots(k + 1, V, vk1, h1, ldh);
+      hipLaunchKernelGGL(k_gemv_sub_seg, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, h1, ldh, seg, vk1);
+      dots(k + 1, V, vk1, h2, ldh);
+      hipLaunchKernelGGL(k_gemv_sub_seg, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, h2, ldh, seg, vk1);
+      dots(1, vk1, vk1, hn, 1);
+      hipLaunchKernelGGL(k_scale_inv_seg, dim3(g), dim3(256), 0, e.stream, n, seg, hn, act_d, vk1);
+      if (k < k_sync && k + 1 < m && it < maxit) continue;
+      HIPCHK(hipMemcpyAsync(hbuf.data() + (size_t)k_done * SL, hd + (size_t)k_done * SL,
+                            (size_t)(k + 1 - k_done) * SL * 8, hipMemcpyDeviceToHost, e.stream));
+      e.sync();
+      bool changed = false;
+      double worst_left = 0;
+      int nleft = 0;
+      for (int s = 0; s < S; ++s) {
+        if (done[s]) continue;
+        const double est0 = est[s];
+        double *Hs = &H[(size_t)s * (m + 1) * m], *c_ = &cs[(size_t)s * m], *s_ = &sn[(size_t)s * m];
+        double *g_ = &gv[(size_t)s * (m + 1)];
+        for (int j = k_done; j <= k; ++j) {
+          const double *hb = hbuf.data() + (size_t)j * SL;
+          for (int i = 0; i <= j; ++i) hcol[i] = hb[(size_t)s * ldh + i] + hb[(size_t)S * ldh + (size_t)s * ldh + i];
+          const double hnv = std::sqrt(std::max(hb[(size_t)2 * S * ldh + s], 0.0));
+          hcol[j + 1] = hnv;
+          for (int i = 0; i < j; ++i) {
+            const double a = hcol[i], bb = hcol[i + 1];
+            hcol[i] = c_[i] * a + s_[i] * bb;
+            hcol[i + 1] = -s_[i] * a + c_[i] * bb;
+          }
+          const double den = std::hypot(hcol[j], hcol[j + 1]);
+          c_[j] = den > 0 ? hcol[j] / den : 1.0;
+          s_[j] = den > 0 ? hcol[j + 1] / den : 0.0;
+          hcol[j] = den;
+          for (int i = 0; i <= j; ++i) Hs[(size_t)i * m + j] = hcol[i];
+          g_[j + 1] = -s_[j] * g_[j];
+          g_[j] = c_[j] * g_[j];
+          est[s] = std::fabs(g_[j + 1]);
+          ks[s] = j + 1;
+          if (est[s] <= tol_abs[s] || hnv == 0) {
+            done[s] = 1;
+            active[s] = 0;
+            changed = true;
+            break;
+          }
+        }
+        if (!done[s]) {
+          ++nleft;
+          if (est[s] < est0 && est0 > 0) {
+            const double rate = std::log(est[s] / est0) / (k + 1 - k_done);
+            const double left = std::log(tol_abs[s] / est[s]) / rate;
+            worst_left = std::max(worst_left, left);
+          } else {
+            worst_left = 2.0;  // no decay observed yet: look again soon
+          }
+        }
+      }
+      if (nleft == 0) {
+        ++k;
+        break;
+      }
+      if (changed) HIPCHK(hipMemcpyAsync(act_d, active.data(), (size_t)S, hipMemcpyHostToDevice, e.stream));
+      k_done = k + 1;
+      k_sync = k + (int)std::max(1.0, std::min(8.0, 0.5 * worst_left));
+    }
+    // least squares per part over its own columns, then x += M^-1 (V y)
+    // columns of each part: those processed up to its convergence, or all
+    // of this cycle's (the last step always synchronises); none for parts
+    // that were converged when the cycle began
+    int kmax = 0;
+    for (int s = 0; s < S; ++s) {
+      const int ku = beta[s] > tol_abs[s] ? ks[s] : 0;
+      const double *Hs = &H[(size_t)s * (m + 1) * m], *g_ = &gv[(size_t)s * (m + 1)];
+      double *ys = &y[(size_t)s * m];
+      std::fill(ys, ys + m, 0.0);
+      for (int i = ku - 1; i >= 0; --i) {
+        double t = g_[i];
+        for (int j = i + 1; j < ku; ++j) t -= Hs[(size_t)i * m + j] * ys[j];
+        ys[i] = Hs[(size_t)i * m + i] != 0 ? t / Hs[(size_t)i * m + i] : 0.0;
+      }
+      kmax = std::max(kmax, ku);
+    }
+    HIPCHK(hipMemcpyAsync(yd, y.data(), y.size() * 8, hipMemcpyHostToDevice, e.stream));
+    if (e.have_prec) {
+      double *t = e.d_ptmp2.as<double>(), *z = e.d_ptmp.as<double>();
+      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, e.stream, t, n, 0.0);
+      hipLaunchKernelGGL(k_gemv_add_seg, dim3(g), dim3(256), 0, e.stream, n, n, kmax, V, yd, m, seg, t);
+      e.apply_prec(t, z);
+      hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, z, 1.0, x);
+    } else {
+      hipLaunchKernelGGL(k_gemv_add_seg, dim3(g), dim3(256), 0, e.stream, n, n, kmax, V, yd, m, seg, x);
+    }
+    e.sync();
+    bool all = true;
+    for (int s = 0; s < S; ++s) all = all && done[s];
+    if (all) return 0;
+  }
+  return 1;
+}
+
 int Engine::gmres_forest(const double *b, double *x, double tol, int restart, int maxit, int precision,
                          int *iters, double *relres) {
   const int S = (int)seg_off.size() - 1;
-  const unsigned g = nblk(n, 256);
-  std::vector<double> bn, f(S), finv(S);
-  seg_norms(b, bn);
-  int nz = 0;
-  for (int s = 0; s < S; ++s) {
-    nz += bn[s] > 0;
-    f[s] = bn[s] > 0 ? 1.0 / bn[s] : 1.0;
-    finv[s] = bn[s] > 0 ? bn[s] : 1.0;
+  if (restart != gm_restart) {
+    alloc(d_V, (size_t)(restart + 1) * n * 8);
+    alloc(d_w, n * 8);
+    alloc(d_r, n * 8);
+    alloc(d_x, n * 8);
+    alloc(d_h, ((size_t)restart * (2 * (restart + 2) + 1) + restart + 2) * 8);
+    gm_restart = restart;
   }
-  double *fb = d_fb.as<double>(), *sv = d_segv.as<double>();
-  HIPCHK(hipMemcpyAsync(sv, f.data(), (size_t)S * 8, hipMemcpyHostToDevice, stream));
-  hipLaunchKernelGGL(k_seg_scale, dim3(g), dim3(256), 0, stream, n, d_seg.as<int>(), sv, b, fb);
-  hipLaunchKernelGGL(k_seg_scale, dim3(g), dim3(256), 0, stream, n, d_seg.as<int>(), sv, x, x);
-  sync();
+  const size_t SL = (size_t)2 * S * (restart + 2) + S;
+  if (d_hs.bytes < SL * restart * 8) alloc(d_hs, SL * restart * 8);
+  if (d_ys.bytes < (size_t)S * restart * 8) alloc(d_ys, (size_t)S * restart * 8);
+  const size_t red = (size_t)n_rch * (restart + 2) * 8;
+  if (d_red.bytes < red) alloc(d_red, red);
+  const unsigned g = nblk(n, 256);
+  std::vector<double> bn, rn, tin(S);
+  seg_norms(b, bn);
   int it = 0;
-  gmres(fb, x, tol / std::sqrt((double)std::max(nz, 1)), restart, maxit, precision, &it, nullptr);
-  HIPCHK(hipMemcpyAsync(sv, finv.data(), (size_t)S * 8, hipMemcpyHostToDevice, stream));
-  hipLaunchKernelGGL(k_seg_scale, dim3(g), dim3(256), 0, stream, n, d_seg.as<int>(), sv, x, x);
-  // true residual of every part
-  vmult(x, fb, PBFMM_FP64);
-  hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, b, -1.0, fb);
-  std::vector<double> rn;
-  seg_norms(fb, rn);
+  double *r = d_r.as<double>(), *d = d_x.as<double>();
+  std::vector<double> rr(S, 1.0);
+  for (int outer = 0; outer < 100 && it < maxit; ++outer) {
+    vmult(x, r, PBFMM_FP64);
+    hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, b, -1.0, r);
+    seg_norms(r, rn);
+    bool ok = true;
+    for (int s = 0; s < S; ++s) {
+      rr[s] = bn[s] > 0 ? rn[s] / bn[s] : rn[s];
+      ok = ok && rr[s] <= tol;
+    }
+    if (ok) break;
+    if (precision == PBFMM_MIXED && have_f32) {
+      // iterative refinement per part: FP32-operator GMRES on the FP64 residual
+      for (int s = 0; s < S; ++s)
+        tin[s] = rr[s] <= tol ? 2.0 * rn[s] : std::max(tol * bn[s] * 0.5, kInnerReduction * rn[s]);
+      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d, n, 0.0);
+      gmres_core_seg(*this, r, d, tin, restart, maxit, PBFMM_MIXED, it);
+      hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, d, 1.0, x);
+    } else {
+      // parts already converged stay inactive (a margin over the FMM's
+      // atomic-order rounding when the core recomputes their residual)
+      for (int s = 0; s < S; ++s) tin[s] = rr[s] <= tol ? 2.0 * rn[s] : tol * bn[s] * 0.5;
+      gmres_core_seg(*this, b, x, tin, restart, maxit, PBFMM_FP64, it);
+    }
+  }
+  sync();
   bool ok = true;
   for (int s = 0; s < S; ++s) {
-    const double r = bn[s] > 0 ? rn[s] / bn[s] : rn[s];
-    if (relres) relres[s] = r;
-    ok = ok && r <= tol;
+    if (relres) relres[s] = rr[s];
+    ok = ok && rr[s] <= tol;
   }
   if (iters) *iters = it;
   return ok ? 0 : 1;

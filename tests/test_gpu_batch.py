@@ -6,8 +6,9 @@ Bars:
   constraints) and compute_rhs equal the parts' own, set up and applied one
   at a time: FP64 within 1e-13, FP32 within 1e-6 (the two may pick different
   near-field kernels, which sum in different orders);
-* batch GMRES: every part's true relative residual <= 1e-8 in both
-  precisions, and the solution equals the parts' own solves within 1e-6;
+* batch GMRES (one Arnoldi process per part inside shared basis vectors):
+  every part's true relative residual <= 1e-8 in both precisions, and the
+  solution equals the parts' own solves within 1e-6;
 * configuration 4 (64 problems, 902,624 DoFs): every part converges to 1e-8,
   and phi on the Neumann DoFs matches each part's analytic solution within
   1e-2 (the FMM order 6 truncation and discretisation error; measured 5.5e-3
