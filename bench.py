@@ -265,7 +265,9 @@ def measure_config(config, steps, warmup):
 
 def measure_multi_rhs(steps):
     """Configuration 5: the Wigley double body, 32 pure-Neumann right-hand
-    sides solved through the multi-RHS entry point, mixed and FP64."""
+    sides solved in lockstep through the multi-RHS entry point (each its own
+    Arnoldi process, every operator application serving all 32), mixed and
+    FP64."""
     import pibem_amd as P
     from pibem_amd import hip
     from pibem_amd import mesh as M
@@ -282,7 +284,7 @@ def measure_multi_rhs(steps):
     zero = np.zeros(n * R)
     out = dict(workload="config 5: closed Wigley double body L=1, B=0.1, T=0.0625, 2 x 400 x 100 Q1 cells, "
                         "81,002 DoFs with stem/keel double nodes, FMM order 8, 32 pure-Neumann right-hand sides "
-                        "-U_k.n (seed 5), each to true rel-res 1e-8",
+                        "-U_k.n (seed 5) solved in lockstep, each to true rel-res 1e-8",
                n_dofs=n, n_rhs=R, setup_s=setup)
     for name, prec in (("mixed", P.PBFMM_MIXED), ("fp64", P.PBFMM_FP64)):
         dx.upload(zero)
@@ -299,10 +301,11 @@ def measure_multi_rhs(steps):
         err = max(float(np.linalg.norm((X[r] - X[r].mean()) - (e - e.mean())) / np.linalg.norm(e - e.mean()))
                   for r, e in enumerate(pr.exact_set[k].phi(pr.mesh.support_xyz) for k in range(R)))
         ms = min(times) * 1e3
-        out[name] = dict(ms_all_rhs=ms, ms_per_rhs=ms / R, operator_applications=int(its.sum()),
-                         max_relres=float(rr.max()), max_error_vs_analytic=err)
-        log(f"config 5 {name}: {ms:.1f} ms for {R} right-hand sides ({ms / R:.2f} ms each), {int(its.sum())} "
-            f"operator applications, max relres {rr.max():.2e}, max error vs -U.x + C {err:.2e}")
+        out[name] = dict(ms_all_rhs=ms, ms_per_rhs=ms / R, arnoldi_steps_all_rhs=int(its.sum()),
+                         max_arnoldi_steps=int(its.max()), max_relres=float(rr.max()), max_error_vs_analytic=err)
+        log(f"config 5 {name}: {ms:.1f} ms for {R} right-hand sides ({ms / R:.2f} ms each), Arnoldi steps "
+            f"{int(its.sum())} in all, at most {int(its.max())} for one; max relres {rr.max():.2e}, max error vs "
+            f"-U.x + C {err:.2e}")
     out["mixed_speedup_vs_fp64"] = out["fp64"]["ms_all_rhs"] / out["mixed"]["ms_all_rhs"]
     return out
 

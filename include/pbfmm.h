@@ -173,11 +173,16 @@ int pbfmm_gmres_dev(pbfmm_handle *h, const double *d_rhs, double *d_x, double re
                     double *relres);
 
 /* Several right-hand sides of one operator (device pointers, column r at
- * d_rhs + r*ld and d_x + r*ld, ld >= n_dofs): each solved as gmres_dev does,
- * one after another on the handle's stream, sharing the near field, unit
- * multipoles, translation operators and preconditioner.  iters and relres
- * have nrhs entries (may be NULL).  No counterpart in the reference, which
- * solves one right-hand side per BEMProblem::solve. */
+ * d_rhs + r*ld and d_x + r*ld, ld >= n_dofs; packed internally when
+ * ld > n_dofs), solved in lockstep: every right-hand side runs its own
+ * Arnoldi process and stopping test (true relative residual <= rel_tol, as
+ * gmres_dev), and every operator application serves all of them, with one
+ * translation launch per level for all right-hand sides (pair lists expanded
+ * over them) and the near field, unit multipoles, operators and
+ * preconditioner shared.  iters gets the Arnoldi steps each right-hand side
+ * used and relres its true relative residual (nrhs entries, may be NULL).  No
+ * counterpart in the reference, which solves one right-hand side per
+ * BEMProblem::solve. */
 int pbfmm_gmres_dev_multi(pbfmm_handle *h, const double *d_rhs, double *d_x, int32_t nrhs, int64_t ld,
                           double rel_tol, int32_t restart, int32_t max_it, int32_t precision, int32_t *iters,
                           double *relres);

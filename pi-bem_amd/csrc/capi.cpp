@@ -552,16 +552,35 @@ int pbfmm_gmres_dev_multi(pbfmm_handle *h, const double *rhs, double *x, int32_t
       return fail(h, PBFMM_ERR_STATE, "gmres_dev_multi: handle was not created with PBFMM_MIXED");
     ensure_prec(h);
     Engine &e = h->eng;
+    const int64_t n = e.n;
+    // lockstep solve on contiguous columns (packed when ld > n_dofs)
+    DevBuf pb, px;
+    const double *B = rhs;
+    double *X = x;
+    if (ld != n) {
+      if (hipMalloc(&pb.p, (size_t)nrhs * n * 8) != hipSuccess || hipMalloc(&px.p, (size_t)nrhs * n * 8) != hipSuccess)
+        throw HipError("gmres_dev_multi: packing buffers");
+      if (hipMemcpy2DAsync(pb.p, n * 8, rhs, ld * 8, n * 8, nrhs, hipMemcpyDeviceToDevice, e.stream) != hipSuccess ||
+          hipMemcpy2DAsync(px.p, n * 8, x, ld * 8, n * 8, nrhs, hipMemcpyDeviceToDevice, e.stream) != hipSuccess)
+        throw HipError("gmres_dev_multi: packing copy");
+      B = pb.as<double>();
+      X = px.as<double>();
+    }
+    std::vector<int> its(nrhs, 0);
+    std::vector<double> rrs(nrhs, 0.0);
+    const int bad = e.gmres_multi(B, X, nrhs, rel_tol, restart, max_it, precision, its.data(), rrs.data());
+    if (ld != n) {
+      if (hipMemcpy2DAsync(x, ld * 8, X, n * 8, n * 8, nrhs, hipMemcpyDeviceToDevice, e.stream) != hipSuccess)
+        throw HipError("gmres_dev_multi: unpacking copy");
+      e.sync();
+    }
     int failed = 0;
     for (int32_t r = 0; r < nrhs; ++r) {
-      int it = 0;
-      double rr = 0;
-      if (e.gmres(rhs + (int64_t)r * ld, x + (int64_t)r * ld, rel_tol, restart, max_it, precision, &it, &rr) != 0)
-        ++failed;
-      if (iters) iters[r] = it;
-      if (relres) relres[r] = rr;
+      if (iters) iters[r] = its[r];
+      if (relres) relres[r] = rrs[r];
+      failed += rrs[r] > rel_tol;
     }
-    if (failed)
+    if (bad || failed)
       return fail(h, PBFMM_ERR_STATE, "gmres_dev_multi: " + std::to_string(failed) + " right-hand side(s) did not "
                                       "reach rel_tol");
     return PBFMM_OK;

@@ -152,6 +152,26 @@ struct Engine {
   int gmres_forest(const double *b, double *x, double tol, int restart, int maxit, int precision, int *iters,
                    double *relres);
 
+  // ---- several right-hand sides of one operator in lockstep ----
+  // Vectors hold R right-hand sides one after another (rhs r at r * n).
+  // Translations run once for all of them over pair lists expanded to
+  // (rhs, pair) with block ids rhs * nb + block, on expansions stored
+  // rhs-major ([rhs][block][kp]); the other phases loop over the rhs.
+  int mr_R = 0, mr_nch = 0;
+  ChunkList mr_m2m_h, mr_m2l_h, mr_l2l_h;
+  ChunkDev mr_m2m, mr_m2l, mr_l2l;
+  DevBuf mr_me64, mr_le64, mr_me32, mr_le32, mr_sp, mr_out;
+  DevBuf mr_seg_off, mr_seg, mr_rch, mr_sch, mr_act, mr_segv;
+  DevBuf mr_V, mr_w, mr_r, mr_d, mr_t1, mr_t2, mr_hs, mr_ys, mr_red;
+  int mr_restart = 0;
+  void multi_setup(int R);
+  template <typename T>
+  void matvec_combined_multi(const double *x, double *out, int R);
+  void vmult_multi(const double *src, double *dst, int R, int precision);
+  void apply_prec_multi(const double *v, double *z, int R);
+  int gmres_multi(const double *b, double *x, int R, double tol, int restart, int maxit, int precision,
+                  int *iters, double *relres);
+
   // ---- post-solve (postsolve.hip): BEMProblem::compute_normals/gradients,
   // pressures and body integrals, over the real cells only ----
   struct PostSolve {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -1863,23 +1864,44 @@ void Engine::seg_norms(const double *v, std::vector<double> &out) {
   for (double &x : out) x = std::sqrt(x);
 }
 
-// Segmented restarted GMRES (CGS2, FP64 basis): part s of the forest runs its
-// own Arnoldi process and stops when its residual estimate reaches tol_abs[s];
-// the operator is applied to all parts at once.  Returns 0 when every part
-// reached its tolerance.
-static int gmres_core_seg(Engine &e, const double *b, double *x, const std::vector<double> &tol_abs, int m,
-                          int maxit, int precision, int &it) {
-  const int64_t n = e.n;
-  const int S = (int)e.seg_off.size() - 1, nch = e.n_rch;
+// Segmented restarted GMRES (CGS2, FP64 basis): part s of a vector of
+// several independent parts (problems of a forest, or right-hand sides of one
+// operator) runs its own Arnoldi process and stops when its residual estimate
+// reaches tol_abs[s]; the operator is applied to all parts at once.
+struct SegCtx {
+  int64_t N = 0;                 // total vector length
+  int S = 0, nch = 0;            // parts; part-aligned reduction chunks
+  const int *seg = nullptr;      // part of each entry
+  const int64_t *seg_off = nullptr, *rch = nullptr;
+  const int *sch = nullptr;
+  uint8_t *act = nullptr;        // per part: still iterating
+  double *segv = nullptr;        // per-part scratch
+  double *V = nullptr, *w = nullptr, *t1 = nullptr, *t2 = nullptr, *part = nullptr, *hd = nullptr,
+         *yd = nullptr;
+  std::function<void(const double *, double *)> op64, op, prec;  // op64: FP64 operator; op: inner
+  std::vector<int> *part_it = nullptr;  // optional: Arnoldi steps each part used
+};
+
+static void seg_norms_ctx(Engine &e, const SegCtx &c, const double *v, std::vector<double> &out) {
+  hipLaunchKernelGGL(k_seg_sumsq, dim3(c.S), dim3(kRedThreads), 0, e.stream, c.seg_off, v, c.segv);
+  out.assign(c.S, 0.0);
+  HIPCHK(hipMemcpyAsync(out.data(), c.segv, (size_t)c.S * 8, hipMemcpyDeviceToHost, e.stream));
+  e.sync();
+  for (double &x : out) x = std::sqrt(x);
+}
+
+static int gmres_core_seg(Engine &e, const SegCtx &c, const double *b, double *x, const std::vector<double> &tol_abs,
+                          int m, int maxit, int &it) {
+  const int64_t n = c.N;
+  const int S = c.S, nch = c.nch;
   const unsigned g = nblk(n, 256);
-  double *V = e.d_V.as<double>(), *w = e.d_w.as<double>(), *part = e.d_red.as<double>();
-  const int *seg = e.d_seg.as<int>(), *sch = e.d_sch_off.as<int>();
-  const int64_t *rch = e.d_rch_beg.as<int64_t>();
-  uint8_t *act_d = e.d_seg_active.as<uint8_t>();
+  double *V = c.V, *w = c.w, *part = c.part, *hd = c.hd, *yd = c.yd;
+  const int *seg = c.seg, *sch = c.sch;
+  const int64_t *rch = c.rch;
+  uint8_t *act_d = c.act;
   // Hessenberg slots per step: [h1: S x (m+2)][h2: S x (m+2)][|w|^2: S]
   const int ldh = m + 2;
   const size_t SL = (size_t)2 * S * ldh + S;
-  double *hd = e.d_hs.as<double>(), *yd = e.d_ys.as<double>();
   std::vector<double> hbuf(SL * m);
   std::vector<double> H((size_t)S * (m + 1) * m), cs((size_t)S * m), sn((size_t)S * m), gv((size_t)S * (m + 1));
   std::vector<double> est(S), beta, y((size_t)S * m), hcol(m + 2), fsc(S);
@@ -1890,9 +1912,9 @@ static int gmres_core_seg(Engine &e, const double *b, double *x, const std::vect
     hipLaunchKernelGGL(k_finish_seg, dim3(S, nv), dim3(kRedThreads), 0, e.stream, nch, sch, ldo, part, out);
   };
   while (it < maxit) {
-    e.vmult(x, w, precision);
+    c.op(x, w);
     hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, b, -1.0, w);
-    e.seg_norms(w, beta);
+    seg_norms_ctx(e, c, w, beta);
     int nact = 0;
     for (int s = 0; s < S; ++s) {
       active[s] = beta[s] > tol_abs[s];
@@ -1905,18 +1927,18 @@ static int gmres_core_seg(Engine &e, const double *b, double *x, const std::vect
       gv[(size_t)s * (m + 1)] = beta[s];
     }
     if (nact == 0) return 0;
-    HIPCHK(hipMemcpyAsync(e.d_segv.p, fsc.data(), (size_t)S * 8, hipMemcpyHostToDevice, e.stream));
+    HIPCHK(hipMemcpyAsync(c.segv, fsc.data(), (size_t)S * 8, hipMemcpyHostToDevice, e.stream));
     HIPCHK(hipMemcpyAsync(act_d, active.data(), (size_t)S, hipMemcpyHostToDevice, e.stream));
-    hipLaunchKernelGGL(k_seg_scale, dim3(g), dim3(256), 0, e.stream, n, seg, e.d_segv.as<double>(), w, V);
+    hipLaunchKernelGGL(k_seg_scale, dim3(g), dim3(256), 0, e.stream, n, seg, c.segv, w, V);
     e.sync();
     int k = 0, k_done = 0, k_sync = 0;
     for (; k < m && it < maxit; ++k) {
       double *vk1 = V + (int64_t)(k + 1) * n;
-      if (e.have_prec) {
-        e.apply_prec(V + (int64_t)k * n, e.d_ptmp.as<double>());
-        e.vmult(e.d_ptmp.as<double>(), vk1, precision);
+      if (c.prec) {
+        c.prec(V + (int64_t)k * n, c.t1);
+        c.op(c.t1, vk1);
       } else {
-        e.vmult(V + (int64_t)k * n, vk1, precision);
+        c.op(V + (int64_t)k * n, vk1);
       }
       ++it;
       double *slot = hd + (size_t)k * SL;
@@ -1972,7 +1994,7 @@ static int gmres_core_seg(Engine &e, const double *b, double *x, const std::vect
             const double left = std::log(tol_abs[s] / est[s]) / rate;
             worst_left = std::max(worst_left, left);
           } else {
-            worst_left = 2.0;  // no decay observed yet: look again soon
+            worst_left = std::max(worst_left, 2.0);  // no decay observed yet: look again soon
           }
         }
       }
@@ -1984,7 +2006,6 @@ static int gmres_core_seg(Engine &e, const double *b, double *x, const std::vect
       k_done = k + 1;
       k_sync = k + (int)std::max(1.0, std::min(8.0, 0.5 * worst_left));
     }
-    // least squares per part over its own columns, then x += M^-1 (V y)
     // columns of each part: those processed up to its convergence, or all
     // of this cycle's (the last step always synchronises); none for parts
     // that were converged when the cycle began
@@ -2000,14 +2021,14 @@ static int gmres_core_seg(Engine &e, const double *b, double *x, const std::vect
         ys[i] = Hs[(size_t)i * m + i] != 0 ? t / Hs[(size_t)i * m + i] : 0.0;
       }
       kmax = std::max(kmax, ku);
+      if (c.part_it) (*c.part_it)[s] += ku;
     }
     HIPCHK(hipMemcpyAsync(yd, y.data(), y.size() * 8, hipMemcpyHostToDevice, e.stream));
-    if (e.have_prec) {
-      double *t = e.d_ptmp2.as<double>(), *z = e.d_ptmp.as<double>();
-      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, e.stream, t, n, 0.0);
-      hipLaunchKernelGGL(k_gemv_add_seg, dim3(g), dim3(256), 0, e.stream, n, n, kmax, V, yd, m, seg, t);
-      e.apply_prec(t, z);
-      hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, z, 1.0, x);
+    if (c.prec) {
+      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, e.stream, c.t2, n, 0.0);
+      hipLaunchKernelGGL(k_gemv_add_seg, dim3(g), dim3(256), 0, e.stream, n, n, kmax, V, yd, m, seg, c.t2);
+      c.prec(c.t2, c.t1);
+      hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, c.t1, 1.0, x);
     } else {
       hipLaunchKernelGGL(k_gemv_add_seg, dim3(g), dim3(256), 0, e.stream, n, n, kmax, V, yd, m, seg, x);
     }
@@ -2017,6 +2038,49 @@ static int gmres_core_seg(Engine &e, const double *b, double *x, const std::vect
     if (all) return 0;
   }
   return 1;
+}
+
+// Iterative refinement around the segmented core: FP64 residual per part;
+// mixed precision solves for the correction with the inner (FP32) operator,
+// each part to max(tol ||b_s|| / 2, kInnerReduction ||r_s||).
+static int gmres_seg_outer(Engine &e, const SegCtx &c, const double *b, double *x, double *r, double *d, double tol,
+                           int restart, int maxit, bool mixed, int *iters, double *relres) {
+  const int S = c.S;
+  const unsigned g = nblk(c.N, 256);
+  std::vector<double> bn, rn, tin(S), rr(S, 1.0);
+  seg_norms_ctx(e, c, b, bn);
+  int it = 0;
+  for (int outer = 0; outer < 100 && it < maxit; ++outer) {
+    c.op64(x, r);
+    hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, c.N, 1.0, b, -1.0, r);
+    seg_norms_ctx(e, c, r, rn);
+    bool ok = true;
+    for (int s = 0; s < S; ++s) {
+      rr[s] = bn[s] > 0 ? rn[s] / bn[s] : rn[s];
+      ok = ok && rr[s] <= tol;
+    }
+    if (ok) break;
+    // parts already converged stay inactive (a margin over the FMM's
+    // atomic-order rounding when the core recomputes their residual)
+    if (mixed) {
+      for (int s = 0; s < S; ++s)
+        tin[s] = rr[s] <= tol ? 2.0 * rn[s] : std::max(tol * bn[s] * 0.5, kInnerReduction * rn[s]);
+      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, e.stream, d, c.N, 0.0);
+      gmres_core_seg(e, c, r, d, tin, restart, maxit, it);
+      hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, c.N, 1.0, d, 1.0, x);
+    } else {
+      for (int s = 0; s < S; ++s) tin[s] = rr[s] <= tol ? 2.0 * rn[s] : tol * bn[s] * 0.5;
+      gmres_core_seg(e, c, b, x, tin, restart, maxit, it);
+    }
+  }
+  e.sync();
+  bool ok = true;
+  for (int s = 0; s < S; ++s) {
+    if (relres) relres[s] = rr[s];
+    ok = ok && rr[s] <= tol;
+  }
+  if (iters) *iters = it;
+  return ok ? 0 : 1;
 }
 
 int Engine::gmres_forest(const double *b, double *x, double tol, int restart, int maxit, int precision,
@@ -2035,44 +2099,239 @@ int Engine::gmres_forest(const double *b, double *x, double tol, int restart, in
   if (d_ys.bytes < (size_t)S * restart * 8) alloc(d_ys, (size_t)S * restart * 8);
   const size_t red = (size_t)n_rch * (restart + 2) * 8;
   if (d_red.bytes < red) alloc(d_red, red);
-  const unsigned g = nblk(n, 256);
-  std::vector<double> bn, rn, tin(S);
-  seg_norms(b, bn);
-  int it = 0;
-  double *r = d_r.as<double>(), *d = d_x.as<double>();
-  std::vector<double> rr(S, 1.0);
-  for (int outer = 0; outer < 100 && it < maxit; ++outer) {
-    vmult(x, r, PBFMM_FP64);
-    hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, b, -1.0, r);
-    seg_norms(r, rn);
-    bool ok = true;
-    for (int s = 0; s < S; ++s) {
-      rr[s] = bn[s] > 0 ? rn[s] / bn[s] : rn[s];
-      ok = ok && rr[s] <= tol;
-    }
-    if (ok) break;
-    if (precision == PBFMM_MIXED && have_f32) {
-      // iterative refinement per part: FP32-operator GMRES on the FP64 residual
-      for (int s = 0; s < S; ++s)
-        tin[s] = rr[s] <= tol ? 2.0 * rn[s] : std::max(tol * bn[s] * 0.5, kInnerReduction * rn[s]);
-      hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d, n, 0.0);
-      gmres_core_seg(*this, r, d, tin, restart, maxit, PBFMM_MIXED, it);
-      hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, d, 1.0, x);
-    } else {
-      // parts already converged stay inactive (a margin over the FMM's
-      // atomic-order rounding when the core recomputes their residual)
-      for (int s = 0; s < S; ++s) tin[s] = rr[s] <= tol ? 2.0 * rn[s] : tol * bn[s] * 0.5;
-      gmres_core_seg(*this, b, x, tin, restart, maxit, PBFMM_FP64, it);
-    }
-  }
-  sync();
-  bool ok = true;
-  for (int s = 0; s < S; ++s) {
-    if (relres) relres[s] = rr[s];
-    ok = ok && rr[s] <= tol;
-  }
-  if (iters) *iters = it;
-  return ok ? 0 : 1;
+  if (!d_ptmp.p) alloc(d_ptmp, n * 8);
+  if (!d_ptmp2.p) alloc(d_ptmp2, n * 8);
+  const bool mixed = precision == PBFMM_MIXED && have_f32;
+  SegCtx c;
+  c.N = n;
+  c.S = S;
+  c.nch = n_rch;
+  c.seg = d_seg.as<int>();
+  c.seg_off = d_seg_off.as<int64_t>();
+  c.rch = d_rch_beg.as<int64_t>();
+  c.sch = d_sch_off.as<int>();
+  c.act = d_seg_active.as<uint8_t>();
+  c.segv = d_segv.as<double>();
+  c.V = d_V.as<double>();
+  c.w = d_w.as<double>();
+  c.t1 = d_ptmp.as<double>();
+  c.t2 = d_ptmp2.as<double>();
+  c.part = d_red.as<double>();
+  c.hd = d_hs.as<double>();
+  c.yd = d_ys.as<double>();
+  c.op64 = [this](const double *u, double *v) { vmult(u, v, PBFMM_FP64); };
+  c.op = [this, mixed](const double *u, double *v) { vmult(u, v, mixed ? PBFMM_MIXED : PBFMM_FP64); };
+  if (have_prec) c.prec = [this](const double *u, double *v) { apply_prec(u, v); };
+  return gmres_seg_outer(*this, c, b, x, d_r.as<double>(), d_x.as<double>(), tol, restart, maxit, mixed, iters,
+                         relres);
 }
+
+// ---------------------------------------------------------------------------
+// Several right-hand sides of one operator in lockstep
+// ---------------------------------------------------------------------------
+void Engine::multi_setup(int R) {
+  if (R == mr_R) return;
+  if (R < 1) throw std::runtime_error("multi_setup: no right-hand sides");
+  const int nb = tree.nb;
+  if ((int64_t)R * nb >= (int64_t(1) << 31)) throw std::runtime_error("too many right-hand sides for this tree");
+  // pairs of each chunk for every rhs (block ids rhs * nb + block), cut
+  // again into chunks of <= kChunk1 pairs of one operator class
+  auto expand = [&](const ChunkList &c, ChunkList &o) {
+    o = ChunkList();
+    const size_t L2 = c.level_off1.size();
+    o.level_off1.assign(L2, 0);
+    o.level_off.assign(L2, 0);
+    for (size_t l = 0; l + 1 < L2; ++l) {
+      o.level_off1[l] = (int)o.op1.size();
+      for (int k = c.level_off1[l]; k < c.level_off1[l + 1]; ++k) {
+        const int cnt = c.cnt1[k], total = cnt * R;
+        for (int placed = 0; placed < total; placed += kChunk1) {
+          const int m = std::min(kChunk1, total - placed);
+          o.op1.push_back(c.op1[k]);
+          o.beg1.push_back((int)o.tgt.size());
+          o.cnt1.push_back(m);
+          for (int q = placed; q < placed + m; ++q) {
+            const int r = q / cnt, pr = c.beg1[k] + q % cnt;
+            o.tgt.push_back(r * nb + c.tgt[pr]);
+            o.src.push_back(r * nb + c.src[pr]);
+          }
+        }
+      }
+    }
+    o.level_off1[L2 - 1] = (int)o.op1.size();
+  };
+  expand(far.m2m, mr_m2m_h);
+  expand(far.m2l, mr_m2l_h);
+  expand(far.l2l, mr_l2l_h);
+  upload_chunks(mr_m2m, mr_m2m_h);
+  upload_chunks(mr_m2l, mr_m2l_h);
+  upload_chunks(mr_l2l, mr_l2l_h);
+  const size_t ne = (size_t)R * nb * kp;
+  alloc(mr_me64, ne * 8);
+  alloc(mr_le64, ne * 8);
+  if (have_f32) {
+    alloc(mr_me32, ne * 4);
+    alloc(mr_le32, ne * 4);
+  }
+  const int64_t N = (int64_t)R * n;
+  alloc(mr_sp, N * 8);
+  alloc(mr_out, N * 8);
+  // parts of the lockstep solve: rhs r is entries r * n .. (r + 1) * n
+  std::vector<int64_t> off(R + 1), rch{0};
+  std::vector<int32_t> sg(N), sch{0};
+  for (int r = 0; r <= R; ++r) off[r] = (int64_t)r * n;
+  for (int r = 0; r < R; ++r) {
+    for (int64_t i = off[r]; i < off[r + 1]; ++i) sg[i] = r;
+    for (int64_t b = off[r]; b < off[r + 1]; b += kRedChunk) rch.push_back(std::min(b + kRedChunk, off[r + 1]));
+    sch.push_back((int32_t)rch.size() - 1);
+  }
+  mr_nch = (int)rch.size() - 1;
+  up(mr_seg_off, off);
+  up(mr_seg, sg);
+  up(mr_rch, rch);
+  up(mr_sch, sch);
+  alloc(mr_act, (size_t)R);
+  alloc(mr_segv, (size_t)R * 8);
+  mr_restart = 0;
+  mr_R = R;
+}
+
+template <typename T>
+void Engine::matvec_combined_multi(const double *x, double *out, int R) {
+  FarDev<T> &F = far_of<T>(*this);
+  T *ME = (sizeof(T) == 4 ? mr_me32 : mr_me64).template as<T>();
+  T *LE = (sizeof(T) == 4 ? mr_le32 : mr_le64).template as<T>();
+  const int nb = tree.nb;
+  const int64_t ne = (int64_t)R * nb * kp, sb = (int64_t)nb * kp;
+  const int nleaf = (int)far.p2m_leaf.size();
+  const int nrows = (int)near.rows.size();
+  const int64_t nitems = (int64_t)far.m2p_item_node.size();
+  // upward pass
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(ne, 256), 4096)), dim3(256), 0, stream, ME,
+                     ne, T(0));
+  for (int r = 0; r < R && nleaf > 0; ++r)
+    hipLaunchKernelGGL((k_p2m<T, 1>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
+                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp,
+                       x + (int64_t)r * n, nullptr, ME + r * sb);
+  for (int l = L; l >= 1; --l)
+    launch_translate<T, 0, 1>(*this, stream, mr_m2m, lvl<1>(mr_m2m_h, l), lvl<1>(mr_m2m_h, l + 1),
+                              F.m2m.template as<T>(), ME, ME);
+  // near field, then the far field accumulated onto it
+  for (int r = 0; r < R && near.nleaf > 0; ++r)
+    launch_spmv<T, 1>(spmv_variant[sizeof(T) == 4 ? 1 : 0][0], x + (int64_t)r * n, nullptr, out + (int64_t)r * n,
+                      nullptr);
+  hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(ne, 256), 4096)), dim3(256), 0, stream, LE,
+                     ne, T(0));
+  launch_translate<T, 1, 1>(*this, stream, mr_m2l, 0, nchunks<1>(mr_m2l_h), F.m2l.template as<T>(), ME, LE);
+  for (int r = 0; r < R && nitems > 0; ++r)
+    hipLaunchKernelGGL((k_m2p<T, 1>), dim3((unsigned)nblk(nitems, 256)), dim3(256), 0, stream, nitems,
+                       d_m2p_node.as<int>(), d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
+                       d_delta.as<double>(), p, kt, kp, ME + r * sb, out + (int64_t)r * n, nullptr);
+  for (int l = 2; l <= L; ++l)
+    launch_translate<T, 2, 1>(*this, stream, mr_l2l, lvl<1>(mr_l2l_h, l), lvl<1>(mr_l2l_h, l + 1),
+                              F.l2l.template as<T>(), LE, LE);
+  for (int r = 0; r < R && nrows > 0; ++r)
+    hipLaunchKernelGGL((k_l2p<T, 1>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
+                       d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
+                       d_delta.as<double>(), p, kt, kp, LE + r * sb, out + (int64_t)r * n, nullptr);
+  HIPCHK(hipGetLastError());
+}
+
+// BEMProblem::vmult for R vectors at once; the pure-Neumann operator is the
+// linear A0 x + mean(x) 1 of each vector (the form GMRES uses).
+void Engine::vmult_multi(const double *src, double *dst, int R, int precision) {
+  const unsigned g = nblk(n, 256);
+  double *sp = mr_sp.as<double>(), *o = mr_out.as<double>();
+  for (int r = 0; r < R; ++r)
+    hipLaunchKernelGGL(k_vmult_pre, dim3(g), dim3(256), 0, stream, n, src + (int64_t)r * n, d_dir.as<uint8_t>(),
+                       nullptr, sp + (int64_t)r * n);
+  if (precision == PBFMM_MIXED && have_f32)
+    matvec_combined_multi<float>(sp, o, R);
+  else
+    matvec_combined_multi<double>(sp, o, R);
+  const int nchunk = (int)nblk(n, kRedChunk);
+  double *res = d_red.as<double>() + (size_t)nchunk * 128;
+  if (!have_dirichlet && !d_ones.p) {
+    alloc(d_ones, n * 8);
+    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d_ones.as<double>(), n, 1.0);
+  }
+  for (int r = 0; r < R; ++r) {
+    const double *s_r = src + (int64_t)r * n;
+    double *d_r = dst + (int64_t)r * n;
+    hipLaunchKernelGGL(k_vmult_post, dim3(g), dim3(256), 0, stream, n, o + (int64_t)r * n, d_alpha.as<double>(),
+                       d_dir.as<uint8_t>(), sp + (int64_t)r * n, d_r);
+    if (!have_dirichlet) {
+      hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, s_r, d_ones.as<double>(),
+                         d_red.as<double>());
+      hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, stream, nchunk, d_red.as<double>(), res);
+      hipLaunchKernelGGL(k_add_scaled_scalar, dim3(g), dim3(256), 0, stream, n, d_r, res, 1.0 / (double)n);
+    }
+    if (have_constraints)
+      hipLaunchKernelGGL(k_constrain, dim3(g), dim3(256), 0, stream, n, d_c_is.as<uint8_t>(), d_c_ptr.as<int>(),
+                         d_c_idx.as<int>(), d_c_val.as<double>(), s_r, d_r);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+void Engine::apply_prec_multi(const double *v, double *z, int R) {
+  for (int r = 0; r < R; ++r) apply_prec(v + (int64_t)r * n, z + (int64_t)r * n);
+}
+
+int Engine::gmres_multi(const double *b, double *x, int R, double tol, int restart, int maxit, int precision,
+                        int *iters, double *relres) {
+  multi_setup(R);
+  const int64_t N = (int64_t)R * n;
+  if (restart != mr_restart) {
+    alloc(mr_V, (size_t)(restart + 1) * N * 8);
+    alloc(mr_w, N * 8);
+    alloc(mr_r, N * 8);
+    alloc(mr_d, N * 8);
+    alloc(mr_t1, N * 8);
+    alloc(mr_t2, N * 8);
+    const size_t SL = (size_t)2 * R * (restart + 2) + R;
+    alloc(mr_hs, SL * restart * 8);
+    alloc(mr_ys, (size_t)R * restart * 8);
+    alloc(mr_red, (size_t)mr_nch * (restart + 2) * 8);
+    mr_restart = restart;
+  }
+  struct MeanMode {  // linear pure-Neumann operator for the duration of the solve
+    bool &flag;
+    bool old;
+    MeanMode(bool &f, bool v) : flag(f), old(f) { flag = v; }
+    ~MeanMode() { flag = old; }
+  } mean_mode(neumann_mean, !have_dirichlet);
+  const bool mixed = precision == PBFMM_MIXED && have_f32;
+  std::vector<int> pit(R, 0);
+  SegCtx c;
+  c.N = N;
+  c.S = R;
+  c.nch = mr_nch;
+  c.seg = mr_seg.as<int>();
+  c.seg_off = mr_seg_off.as<int64_t>();
+  c.rch = mr_rch.as<int64_t>();
+  c.sch = mr_sch.as<int>();
+  c.act = mr_act.as<uint8_t>();
+  c.segv = mr_segv.as<double>();
+  c.V = mr_V.as<double>();
+  c.w = mr_w.as<double>();
+  c.t1 = mr_t1.as<double>();
+  c.t2 = mr_t2.as<double>();
+  c.part = mr_red.as<double>();
+  c.hd = mr_hs.as<double>();
+  c.yd = mr_ys.as<double>();
+  c.part_it = &pit;
+  c.op64 = [this, R](const double *u, double *v) { vmult_multi(u, v, R, PBFMM_FP64); };
+  c.op = [this, R, mixed](const double *u, double *v) { vmult_multi(u, v, R, mixed ? PBFMM_MIXED : PBFMM_FP64); };
+  if (have_prec) c.prec = [this, R](const double *u, double *v) { apply_prec_multi(u, v, R); };
+  int steps = 0;
+  const int rc = gmres_seg_outer(*this, c, b, x, mr_r.as<double>(), mr_d.as<double>(), tol, restart, maxit, mixed,
+                                 &steps, relres);
+  if (iters)
+    for (int r = 0; r < R; ++r) iters[r] = pit[r];
+  return rc;
+}
+
+template void Engine::matvec_combined_multi<double>(const double *, double *, int);
+template void Engine::matvec_combined_multi<float>(const double *, double *, int);
 
 }  // namespace pbfmm

@@ -11,7 +11,8 @@ Bars:
   FMM truncation error at p = 6, 1e-3 after removing the means;
 * full size (81,002 DoFs, p = 8): every right-hand side reaches true relative
   residual 1e-8 in both precision modes, phi matches -U . x + C to 1e-3
-  (measured 6e-5), and the multi-RHS entry point equals one-at-a-time solves.
+  (measured 6e-5), and the lockstep multi-RHS entry point agrees with a
+  single-RHS solve to 1e-7.
 """
 import numpy as np
 import pytest
@@ -123,8 +124,10 @@ def test_config5_batch():
             print(f"  precision={prec} rhs {r}: iters {its[r]} relres {rr[r]:.2e} err {err:.2e}", flush=True)
             assert rr[r] <= 1e-8 and err <= 1e-3
     assert rel(sols[P.PBFMM_MIXED], sols[P.PBFMM_FP64]) <= 1e-6
-    # one right-hand side alone gives the same answer as inside the batch
+    # one right-hand side alone gives the same answer as inside the lockstep
+    # batch (the same Krylov process; reductions and FMM atomics sum in a
+    # different order, so the two agree to the solver's accuracy)
     x1 = hip.DeviceArray(n)
     r1 = hip.DeviceArray.from_numpy(rhs[2])
     f.gmres_dev(r1.value, x1.value, 1e-8, 100, 1000, P.PBFMM_FP64)
-    assert rel(x1.numpy(), sols[P.PBFMM_FP64][2]) <= 1e-12
+    assert rel(x1.numpy(), sols[P.PBFMM_FP64][2]) <= 1e-7
