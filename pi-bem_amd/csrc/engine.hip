@@ -623,7 +623,7 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(const int *__res
                                                                   const T *__restrict__ ops,
                                                                   const T *__restrict__ IN, T *__restrict__ OUT) {
   using Lay = TrLayout<P, T>;
-  constexpr int KP = Lay::KP, K2 = NK * KP, KT2 = 2 * Kp<P>::kt, BS = Lay::BS, CS = Lay::CS, NT = Lay::NT;
+  constexpr int KP = Lay::KP, K2 = NK * KP, BS = Lay::BS, CS = Lay::CS, NT = Lay::NT;
   constexpr int PC = 64 / NK;                // pairs per chunk
   static_assert(NT == 4 * KP, "launch bound");
   static_assert(sizeof(T) == 8 || (BS % 2 == 0 && CS % 4 == 0), "LDS vector alignment");
@@ -635,6 +635,7 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(const int *__res
   constexpr int PER = (NV + NT - 1) / NT;    // vectors per thread
   __shared__ __align__(16) T sB[Lay::elems];
   __shared__ int sT[PC], sS[PC];
+  constexpr int KT2 = 2 * Kp<P>::kt;
   const int c = blockIdx.x;
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
   const T *A = ops + (int64_t)chunk_op[c] * KP * KP;  // column-major, read from L2
@@ -652,6 +653,19 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(const int *__res
     buf[j] = TV{};
     if (i < NV && pr < cnt)
       buf[j] = *reinterpret_cast<const TV *>(IN + (int64_t)sS[pr] * K2 + (i - pr * NVP) * V);
+  }
+  // M2M and L2L launches hold few chunks per level and wait on latency: each
+  // lane loads its operator values for the whole K loop at once after the
+  // gather (row 16 w + (l & 15), columns k0 + (l >> 4) for k0 = 0, 4, ..).
+  // M2L has tens of thousands of chunks in flight and keeps its registers
+  // for occupancy, loading operator values inside the K loop.
+  constexpr bool PRE = KIND != 1;
+  constexpr int NA = PRE ? KP / 4 : 1;
+  T areg[NA];
+  const T *ak = A + (l >> 4) * KP + 16 * w + (l & 15);
+  if constexpr (PRE) {
+#pragma unroll
+    for (int j = 0; j < NA; ++j) areg[j] = ak[(int64_t)(4 * j) * KP];
   }
 #pragma unroll
   for (int j = 0; j < PER; ++j) {
@@ -672,13 +686,21 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(const int *__res
   typename M::acc acc[4];
 #pragma unroll
   for (int g = 0; g < 4; ++g) acc[g] = typename M::acc{0, 0, 0, 0};
-  const T *ak = A + (l >> 4) * KP + 16 * w + (l & 15);
   const T *bk = sB + (l & 15) * BS + (l >> 4);
-#pragma unroll 4
-  for (int k0 = 0; k0 < KP; k0 += 4) {
-    const T a = ak[k0 * KP];
+  if constexpr (PRE) {
 #pragma unroll
-    for (int g = 0; g < 4; ++g) acc[g] = M::mac(a, bk[g * 16 * BS + k0], acc[g]);
+    for (int k0 = 0; k0 < KP; k0 += 4) {
+      const T a = areg[k0 / 4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) acc[g] = M::mac(a, bk[g * 16 * BS + k0], acc[g]);
+    }
+  } else {
+#pragma unroll 4
+    for (int k0 = 0; k0 < KP; k0 += 4) {
+      const T a = ak[k0 * KP];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) acc[g] = M::mac(a, bk[g * 16 * BS + k0], acc[g]);
+    }
   }
   __syncthreads();  // B tile no longer needed: reuse the buffer for C
   T *sC = sB;
@@ -694,7 +716,9 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(const int *__res
       for (int j = 0; j < 4; ++j) sC[(16 * g + (l & 15)) * CS + 16 * w + M::row(l, j)] = acc[g][j];
   }
   __syncthreads();
-  // coalesced accumulation: consecutive lanes add consecutive rows of one target
+  // coalesced accumulation: consecutive lanes add consecutive rows of one
+  // target (the Im rows of m = 0, zero for real densities, included: skipping
+  // them scatters the lanes and measured slower)
   for (int i = t; i < NK * cnt * KT2; i += NT) {
     const int cc = i / KT2, row = i - cc * KT2;
     atomicAdd(OUT + (int64_t)sT[cc / NK] * K2 + (cc % NK) * KP + row, sC[cc * CS + row]);
