@@ -133,6 +133,12 @@ struct Engine {
   void compute_rhs(const double *g, double *rhs);
   int gmres(const double *rhs, double *x, double tol, int restart, int maxit, int precision,
             int *iters, double *relres);
+  // mixed precision: the relaxed scheme (FP64 operator, then FP32, in one
+  // Krylov space) when the FP64 operator is not much dearer than the FP32
+  // one, else iterative refinement; the cost ratio is timed once
+  double op_cost_ratio = 0;
+  DevBuf d_cost;
+  bool mixed_relaxed();
   double norm2(const double *v);
   void sync();
 
