@@ -223,6 +223,14 @@ int pbfmm_last_timings(pbfmm_handle *h, double *ms);
  *      (0 workgroup per leaf, 1 wavefront per leaf) */
 int pbfmm_work_counts(pbfmm_handle *h, int64_t *counts);
 
+/* Mixed-precision solver scheme (no counterpart in the reference, whose
+ * solve is FP64 throughout): 0 (default) chooses by the ratio of the FP64 to
+ * the FP32 operator's cost, timed once on this handle; 1 relaxed (FP64
+ * operator until the residual is below 2e5 * rel_tol * ||b||, then the FP32
+ * operator, in one Krylov space); 2 iterative refinement (FP32-operator GMRES
+ * on FP64 residuals).  Either scheme ends on the true FP64 residual. */
+int pbfmm_set_mixed_scheme(pbfmm_handle *h, int32_t scheme);
+
 /* ---- Post-solve (the step after the solve; real cells only) ----
  * Vector fields are component-major: v[d * n_dofs + i] is component d at DoF
  * i, which is the reference's vector DoF numbering after component_wise

@@ -797,6 +797,13 @@ int pbfmm_compute_surface_gradients(pbfmm_handle *h, const double *tmp_rhs, doub
   return ps_project_host(h, 2, tmp_rhs, nullptr, gradients, "compute_surface_gradients");
 }
 
+int pbfmm_set_mixed_scheme(pbfmm_handle *h, int32_t scheme) {
+  if (!h) return PBFMM_ERR_ARG;
+  if (scheme < 0 || scheme > 2) return fail(h, PBFMM_ERR_ARG, "set_mixed_scheme: scheme must be 0, 1 or 2");
+  h->eng.mixed_scheme = scheme;
+  return PBFMM_OK;
+}
+
 int pbfmm_projection_iterations(pbfmm_handle *h, int32_t *iters) {
   if (!h || !iters) return PBFMM_ERR_ARG;
   *iters = h->eng.ps.last_iters;

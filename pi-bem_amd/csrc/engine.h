@@ -137,6 +137,7 @@ struct Engine {
   // Krylov space) when the FP64 operator is not much dearer than the FP32
   // one, else iterative refinement; the cost ratio is timed once
   double op_cost_ratio = 0;
+  int mixed_scheme = 0;  // 0 chosen by the timed cost ratio, 1 relaxed, 2 refinement
   DevBuf d_cost;
   bool mixed_relaxed();
   double norm2(const double *v);

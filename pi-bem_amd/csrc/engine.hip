@@ -1835,6 +1835,7 @@ constexpr double kRelaxSwitch = 2e5;
 constexpr double kRelaxCostRatio = 1.45;
 
 bool Engine::mixed_relaxed() {
+  if (mixed_scheme != 0) return mixed_scheme == 1;
   if (op_cost_ratio <= 0) {
     // one timed application of each operator on scratch vectors of its own
     const unsigned g = nblk(n, 256);

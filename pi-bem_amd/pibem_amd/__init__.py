@@ -97,6 +97,12 @@ class BEMFMA:
         self.n = mesh.n_dofs
         self.mesh = mesh
 
+    def set_mixed_scheme(self, scheme=0):
+        """Mixed-precision solver scheme: 0 chosen by the timed FP64/FP32
+        operator cost ratio, 1 relaxed (FP64 then FP32 operator in one Krylov
+        space), 2 iterative refinement (pbfmm_set_mixed_scheme)."""
+        self._chk(self._L.pbfmm_set_mixed_scheme(self._h, int(scheme)))
+
     def set_double_body(self, enable=True, z0=0.0):
         """Double-body operator: every cell also acts through its mirror image
         about z = z0 (BEMProblem::_assemble_system_double_body,

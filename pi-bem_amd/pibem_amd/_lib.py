@@ -62,6 +62,7 @@ SIGNATURES = {
     "pbfmm_set_phase_timing": (C.c_int, [_h, C.c_int32]),
     "pbfmm_last_timings": (C.c_int, [_h, _d]),
     "pbfmm_work_counts": (C.c_int, [_h, _i64]),
+    "pbfmm_set_mixed_scheme": (C.c_int, [_h, C.c_int32]),
     "pbfmm_compute_normals": (C.c_int, [_h, _d]),
     "pbfmm_compute_gradients": (C.c_int, [_h, _d, _d, _d]),
     "pbfmm_compute_surface_gradients": (C.c_int, [_h, _d, _d]),

@@ -239,3 +239,30 @@ def test_config3_properties():
         err = rel(s[neu], phi_ex[neu])
         print(f"  solve precision={prec} iters={it} relres={rr:.3e} err phi {err:.2e}", flush=True)
         assert rr <= 1e-8 and err <= 1e-3
+
+
+@pytest.mark.parametrize("scheme", [1, 2], ids=["relaxed", "refinement"])
+def test_mixed_schemes(oracle_mod, scheme):
+    """Both mixed-precision schemes reach true relative residual 1e-8 and
+    the FP64 solution (the scheme is otherwise chosen by timing, so each is
+    forced here), for one right-hand side and for a lockstep batch of two."""
+    from pibem_amd import hip
+    pr = M.config_sphere(16, 4)
+    f = P.setup(pr, P.PBFMM_MIXED)
+    f.set_mixed_scheme(scheme)
+    bp = P.BEMProblem(f)
+    bp.compute_alpha()
+    n = pr.mesh.n_dofs
+    rhs = bp.compute_rhs(pr.tmp_rhs)
+    dr, dx = hip.DeviceArray.from_numpy(rhs), hip.DeviceArray(n)
+    f.gmres_dev(dr.value, dx.value, 1e-10, 100, 1000, P.PBFMM_FP64)
+    ref = dx.numpy()
+    dx.upload(np.zeros(n))
+    it, rr = f.gmres_dev(dr.value, dx.value, 1e-8, 100, 1000, P.PBFMM_MIXED)
+    print(f"\n  scheme {scheme}: {it} FP32/FP64 Arnoldi steps, relres {rr:.2e}, vs FP64 {rel(dx.numpy(), ref):.2e}")
+    assert rr <= 1e-8 and rel(dx.numpy(), ref) <= 1e-6
+    d2, x2 = hip.DeviceArray.from_numpy(np.concatenate([rhs, 2.0 * rhs])), hip.DeviceArray(2 * n)
+    its, rrs = f.gmres_dev_multi(d2.value, x2.value, 2, n, 1e-8, 100, 1000, P.PBFMM_MIXED)
+    X = x2.numpy()
+    assert rrs.max() <= 1e-8 and rel(X[:n], ref) <= 1e-6 and rel(X[n:], 2.0 * ref) <= 1e-6
+    f.close()
