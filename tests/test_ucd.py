@@ -81,3 +81,47 @@ def test_tree_matches_oracle_on_ingested(name, levels, oracle_mod):
     for k in to:
         if isinstance(to[k], np.ndarray):
             assert np.array_equal(to[k], tp[k]), k
+
+
+def test_line_records_and_refinement():
+    """Line records keep their manifold ids on the cell lines they lie on,
+    refinement splits each recorded line in two at the shared edge midpoint
+    (both halves keep the id), and the boundary lines of the DTC ship are the
+    open edges of its patches."""
+    from pibem_amd import mesh as M
+    path = os.path.join(MESHES, "dtc_ship.inp")
+    v, q, mt, sg, si = ucd.read_ucd(path, with_lines=True)
+    assert len(sg) == len(si) > 0
+    m0 = ucd.load(path)
+    # every recorded line that is an edge of some quad is found on a cell line
+    ids0, counts0 = np.unique(m0.line_ids[m0.line_ids >= 0], return_counts=True)
+    assert set(ids0.tolist()) <= set(si.tolist())
+    v1, q1, mt1, sg1, si1 = ucd.refine(v, q, mt, 1, None, sg, si)
+    assert len(sg1) == 2 * len(sg) and np.array_equal(np.repeat(si, 2), si1)
+    # the two halves meet at the old line's midpoint
+    mids = v1[sg1[0::2, 1]]
+    assert np.allclose(mids, 0.5 * (v[sg[:, 0]] + v[sg[:, 1]]))
+    m1 = ucd.load(path, 1)
+    ids1, counts1 = np.unique(m1.line_ids[m1.line_ids >= 0], return_counts=True)
+    assert np.array_equal(ids0, ids1) and np.array_equal(2 * counts0, counts1)
+    # lines at the boundary of the surface: each belongs to one cell only
+    bl = M.boundary_lines(m1)
+    cd = m1.cell_dofs
+    for c, j in zip(*np.nonzero(bl)):
+        a, b = cd[c, M.CELL_LINES[j][0]], cd[c, M.CELL_LINES[j][1]]
+        share = sum(1 for cc in range(len(cd)) for jj, (u, w) in enumerate(M.CELL_LINES)
+                    if {cd[cc, u], cd[cc, w]} == {a, b})
+        assert share == 1
+
+
+def test_mark_lines_waterline():
+    """mark_lines gives an id to the boundary lines whose end points both lie
+    on a predicate's set (the z = 0 waterline of the half Wigley hull)."""
+    from pibem_amd import mesh as M
+    pr = M.config_wigley_half(12, 4, 4, 1)
+    m = M.mark_lines(pr.mesh, lambda x: np.abs(x[:, 2]) < 1e-12, 5)
+    c, j = np.nonzero(m.line_ids == 5)
+    assert c.size == 2 * 12  # one waterline line per column of cells, two sides
+    for cc, jj in zip(c, j):
+        ends = m.support_xyz[m.cell_dofs[cc, list(M.CELL_LINES[jj])]]
+        assert np.all(np.abs(ends[:, 2]) < 1e-12)
