@@ -924,9 +924,9 @@ __global__ void __launch_bounds__(256) k_bjacobi(int nrows, const int *__restric
   if (g >= nrows) return;
   const int leaf = row_leaf[g];
   const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0, r = g - r0;
-  const double *P = pinv + blk_beg[leaf] + (int64_t)r * R;
+  const double *P = pinv + blk_beg[leaf] + r;  // block stored column-major
   double acc = 0.0;
-  for (int c = 0; c < R; ++c) acc += P[c] * v[rows[r0 + c]];
+  for (int c = 0; c < R; ++c) acc += P[(int64_t)c * R] * v[rows[r0 + c]];
   z[rows[g]] = acc;
 }
 
@@ -1674,7 +1674,10 @@ void Engine::build_preconditioner() {
           A[(size_t)r * R + c] = (r == c) ? (d != 0.0 ? 1.0 / d : 1.0) : 0.0;
         }
     }
-    std::copy(A.begin(), A.end(), blk.begin() + pinv_beg[l]);
+    // stored column-major (transposed): the apply kernel's lanes, one per
+    // row, then read consecutive addresses
+    for (int r = 0; r < R; ++r)
+      for (int c = 0; c < R; ++c) blk[pinv_beg[l] + (size_t)c * R + r] = A[(size_t)r * R + c];
     for (int r = 0; r < R; ++r) local[near.rows[r0 + r]] = -1;
   }
   if (!blk.empty()) HIPCHK(hipMemcpy(d_pinv.p, blk.data(), blk.size() * 8, hipMemcpyHostToDevice));
