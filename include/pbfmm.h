@@ -219,8 +219,9 @@ int pbfmm_last_timings(pbfmm_handle *h, double *ms);
  * [11] trunc_order [12] padded expansion length kp
  * [13] unit-multipole columns (sum over leaves of the DoFs feeding P2M)
  * [14] [15] near-field SpMV kernels chosen at set-up for FP64 / FP32:
- *      bit 0 the (N, D) pair storage, bit 1 the operator's single values
- *      (0 workgroup per leaf, 1 wavefront per leaf) */
+ *      kernel of the (N, D) pair storage + 4 * kernel of the operator's
+ *      single values (0 workgroup per leaf, 1 wavefront per leaf with 2 rows
+ *      in flight, 2 with 4 rows) */
 int pbfmm_work_counts(pbfmm_handle *h, int64_t *counts);
 
 /* Mixed-precision solver scheme (no counterpart in the reference, whose

@@ -736,8 +736,8 @@ int pbfmm_work_counts(pbfmm_handle *h, int64_t *c) {
     c[11] = e.p;
     c[12] = e.kp;
     c[13] = (int64_t)e.far.p2m_col_dof.size();
-    c[14] = e.spmv_variant[0][0] + 2 * e.spmv_variant[0][1];
-    c[15] = e.spmv_variant[1][0] + 2 * e.spmv_variant[1][1];
+    c[14] = e.spmv_variant[0][0] + 4 * e.spmv_variant[0][1];
+    c[15] = e.spmv_variant[1][0] + 4 * e.spmv_variant[1][1];
     return PBFMM_OK;
   }, false);
 }

@@ -117,7 +117,8 @@ struct Engine {
   void matvec_combined(const double *x, double *out);
   // near-field SpMV kernel per precision ([0] FP64, [1] FP32) and storage
   // (pairs, single values): 0 workgroup per leaf with an LDS copy of the
-  // columns, 1 wavefront per leaf; chosen by timing both at set-up
+  // columns, 1 wavefront per leaf with 2 rows in flight, 2 with 4 rows;
+  // chosen by timing all three at set-up
   int spmv_variant[2][2] = {{0, 0}, {0, 0}};  // [precision][NK - 1]
   template <typename T, int NK>
   void tune_spmv();

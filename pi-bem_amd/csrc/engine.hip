@@ -305,13 +305,14 @@ __global__ void __launch_bounds__(256) k_spmv_lds(int nleaf, int per_xcd, const 
   }
 }
 
-// k_spmv_wave: one wavefront per leaf, no LDS and no barrier.  For each tile
+// k_spmv_wave: one wavefront per leaf, no LDS and no barrier, RPI rows per
+// iteration.  For each tile
 // of 64 x CPL leaf columns (64 bytes of values per lane and row: CPL = 8 / 4
 // for FP32 / FP64 pairs, 16 / 8 for single values) every lane holds the
 // inputs of its CPL columns in registers, and the wave streams two rows at a
 // time against them (2 CPL coalesced loads per lane in flight).  Outputs as
 // k_spmv_lds.
-template <typename T, int NK>
+template <typename T, int NK, int RPI>
 __global__ void __launch_bounds__(256) k_spmv_wave(int nleaf, const int *__restrict__ row_beg,
                                                    const int *__restrict__ rows, const int *__restrict__ col_beg,
                                                    const int *__restrict__ cols, const int64_t *__restrict__ val_beg,
@@ -335,33 +336,33 @@ __global__ void __launch_bounds__(256) k_spmv_wave(int nleaf, const int *__restr
       const int c = c0 + lane + 64 * j;
       x[j] = c < C ? near_x<T, NK>(phi, dphi, cols[cb + c]) : zero;
     }
-    for (int r = 0; r < R; r += 2) {
-      const bool two = r + 1 < R;
-      const E *v0 = V + (int64_t)r * C, *v1 = V + (int64_t)(two ? r + 1 : r) * C;
-      E a0[CPL], a1[CPL];
+    // RPI rows per iteration: their loads are in flight together
+    for (int r = 0; r < R; r += RPI) {
+      E a[RPI][CPL];
 #pragma unroll
-      for (int j = 0; j < CPL; ++j) {
-        const int c = c0 + lane + 64 * j;
-        a0[j] = c < C ? v0[c] : zero;
-        a1[j] = c < C ? v1[c] : zero;
-      }
-      T aN0 = 0, aD0 = 0, aN1 = 0, aD1 = 0;
+      for (int q = 0; q < RPI; ++q) {
+        const E *vq = V + (int64_t)(r + q < R ? r + q : r) * C;
 #pragma unroll
-      for (int j = 0; j < CPL; ++j) {
-        near_fma<T, NK>(aN0, aD0, a0[j], x[j]);
-        near_fma<T, NK>(aN1, aD1, a1[j], x[j]);
+        for (int j = 0; j < CPL; ++j) {
+          const int c = c0 + lane + 64 * j;
+          a[q][j] = c < C ? vq[c] : zero;
+        }
       }
-      aN0 = wave_sum(aN0);
-      aN1 = wave_sum(aN1);
-      if constexpr (NK == 2) {
-        aD0 = wave_sum(aD0);
-        aD1 = wave_sum(aD1);
+      T aN[RPI], aD[RPI];
+#pragma unroll
+      for (int q = 0; q < RPI; ++q) {
+        aN[q] = 0;
+        aD[q] = 0;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) near_fma<T, NK>(aN[q], aD[q], a[q][j], x[j]);
+        aN[q] = wave_sum(aN[q]);
+        if constexpr (NK == 2) aD[q] = wave_sum(aD[q]);
       }
       // the wave owns the leaf's rows in every tile: plain read-modify-write
-      if (lane == 0) {
-        near_store<NK>(outN, outD, rows[r0 + r], c0 == 0, (double)aN0, (double)aD0);
-        if (two) near_store<NK>(outN, outD, rows[r0 + r + 1], c0 == 0, (double)aN1, (double)aD1);
-      }
+      if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < RPI; ++q)
+          if (r + q < R) near_store<NK>(outN, outD, rows[r0 + r + q], c0 == 0, (double)aN[q], (double)aD[q]);
     }
   }
 }
@@ -1390,15 +1391,20 @@ void Engine::launch_spmv(int variant, const double *phi, const double *dphi, dou
     hipLaunchKernelGGL((k_spmv_lds<T, NK>), dim3(per_xcd * kXcd), dim3(256), 0, stream, near.nleaf, per_xcd,
                        d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_cols.as<int>(),
                        d_val_beg.as<int64_t>(), vals, phi, dphi, outN, outD);
+  } else if (variant == 1) {
+    hipLaunchKernelGGL((k_spmv_wave<T, NK, 2>), dim3((near.nleaf + 3) / 4), dim3(256), 0, stream, near.nleaf,
+                       d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_cols.as<int>(),
+                       d_val_beg.as<int64_t>(), vals, phi, dphi, outN, outD);
   } else {
-    hipLaunchKernelGGL((k_spmv_wave<T, NK>), dim3((near.nleaf + 3) / 4), dim3(256), 0, stream, near.nleaf,
+    hipLaunchKernelGGL((k_spmv_wave<T, NK, 4>), dim3((near.nleaf + 3) / 4), dim3(256), 0, stream, near.nleaf,
                        d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_cols.as<int>(),
                        d_val_beg.as<int64_t>(), vals, phi, dphi, outN, outD);
   }
 }
 
-// Times both SpMV kernels on the assembled near field (HIP events, scratch
-// vectors) and keeps the faster one for this precision and storage.
+// Times the SpMV kernels (workgroup per leaf; wavefront per leaf with 2 or 4
+// rows in flight) on the assembled near field (HIP events, scratch vectors)
+// and keeps the fastest for this precision and storage.
 template <typename T, int NK>
 void Engine::tune_spmv() {
   if (near.nleaf == 0) return;
@@ -1407,7 +1413,7 @@ void Engine::tune_spmv() {
   HIPCHK(hipEventCreate(&e1));
   float best = 0;
   int pick = 0;
-  for (int v = 0; v < 2; ++v) {
+  for (int v = 0; v < 3; ++v) {
     for (int rep = 0; rep < 2; ++rep)
       launch_spmv<T, NK>(v, d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
     HIPCHK(hipEventRecord(e0, stream));
