@@ -375,6 +375,18 @@ def measure_batch():
                 **out)
 
 
+def measure_leaf_tuning():
+    """The leaf-size autotuner (pibem_amd.tune_leaf_size) on configuration
+    2: FP32 operator time per application for each candidate leaf size.  The
+    headline keeps the reference's default of 20."""
+    import pibem_amd as P
+    best, times = P.tune_leaf_size(make_problem(2), precision=P.PBFMM_MIXED)
+    log("leaf-size tuning, config 2, FP32 operator: " + " ".join(f"{k}:{v:.3f}ms" for k, v in times.items())
+        + f" -> {best}")
+    return dict(workload="config 2, FP32 operator application", best=best,
+                ms_per_application={str(k): v for k, v in times.items()}, default=20)
+
+
 def cpu_baseline(pr, ops, seconds):
     """The oracle's FP64 vmult (reference algorithm restated in C++/OpenMP) on
     the host cores, over a bounded number of calls, scaled to one solve."""
@@ -457,6 +469,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config3", action="store_true")
+    ap.add_argument("--no-leaf-tuning", action="store_true")
     ap.add_argument("--extra", default="3,4,5", help="further configurations to measure (comma list of 3, 4, 5)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--mode", choices=("full", "matvec", "solve", "copy"), default="full")
@@ -559,6 +572,8 @@ def main():
         out["config5"] = measure_multi_rhs(args.steps)
     if 4 in extra:
         out["config4"] = measure_batch()
+    if not args.no_leaf_tuning:
+        out["leaf_size_tuning"] = measure_leaf_tuning()
     print(json.dumps(out))
 
 

@@ -16,12 +16,13 @@ HIP library; nothing here computes on the CPU.
 from __future__ import annotations
 
 import ctypes as C
+import time
 
 import numpy as np
 
 from ._lib import PBFMM_FP64, PBFMM_MIXED, Params, PbfmmError, load
 
-__all__ = ["BEMFMA", "BatchFMA", "BEMProblem", "Body", "setup_batch", "PBFMM_FP64", "PBFMM_MIXED", "PbfmmError", "load"]
+__all__ = ["BEMFMA", "BatchFMA", "BEMProblem", "Body", "setup_batch", "tune_leaf_size", "PBFMM_FP64", "PBFMM_MIXED", "PbfmmError", "load"]
 
 _d = C.POINTER(C.c_double)
 _i64 = C.POINTER(C.c_int64)
@@ -408,6 +409,36 @@ class BatchFMA(BEMFMA):
         if check:
             self._chk(rc)
         return it.value, rr
+
+
+def tune_leaf_size(problem, candidates=(12, 16, 20, 24, 32, 48), precision=PBFMM_MIXED, reps=20, device=0):
+    """Leaf-size autotuner: sets the problem up with each candidate
+    max_nodes_per_leaf and times the operator GMRES applies (vmult on device
+    vectors, `reps` applications after two warm-ups, timed by wall clock
+    between device synchronisations) in the given precision.  Returns (best leaf size, {leaf size: ms per
+    application}).  The leaf size moves the near/far split, so results differ
+    within the FMM truncation error; the reference's default is 20."""
+    import dataclasses
+    from . import hip
+    times = {}
+    for leaf in candidates:
+        pr = dataclasses.replace(problem, max_nodes_per_leaf=int(leaf))
+        f = setup(pr, precision, device)
+        bp = BEMProblem(f)
+        bp.compute_alpha()
+        src = hip.DeviceArray.from_numpy(np.random.default_rng(1234).uniform(-1, 1, f.n))
+        dst = hip.DeviceArray(f.n)
+        for _ in range(2):
+            bp.vmult_dev(src.value, dst.value, precision)
+        hip.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            bp.vmult_dev(src.value, dst.value, precision)
+        hip.synchronize()
+        times[int(leaf)] = (time.perf_counter() - t0) * 1e3 / reps
+        f.close()
+    best = min(times, key=times.get)
+    return best, times
 
 
 def setup_batch(problems, precision=PBFMM_FP64, device=0, preconditioner=1):

@@ -266,3 +266,13 @@ def test_mixed_schemes(oracle_mod, scheme):
     X = x2.numpy()
     assert rrs.max() <= 1e-8 and rel(X[:n], ref) <= 1e-6 and rel(X[n:], 2.0 * ref) <= 1e-6
     f.close()
+
+
+def test_tune_leaf_size():
+    """The leaf-size autotuner times each candidate's operator and returns the
+    fastest; every candidate gives a working operator."""
+    pr = M.config_sphere(16, 4)
+    best, times = P.tune_leaf_size(pr, candidates=(12, 20, 32), reps=5)
+    print(f"\n  leaf sizes: {times} -> {best}")
+    assert set(times) == {12, 20, 32} and best in times and all(t > 0 for t in times.values())
+    assert times[best] == min(times.values())
