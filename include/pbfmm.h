@@ -166,6 +166,19 @@ int pbfmm_matvec_dev(pbfmm_handle *h, const double *d_phi, const double *d_dphi,
  * FP32 operator (FP64 in/out); src and dst must differ. */
 int pbfmm_vmult_dev(pbfmm_handle *h, const double *d_src, double *d_dst, int32_t precision);
 
+/* pbfmm_vmult_dev on nrhs columns (device pointers, column r at d_src + r*ld
+ * and d_dst + r*ld, ld >= n_dofs; the two ranges must not overlap).  One
+ * operator application serves all columns: the translation pair lists are
+ * expanded over them, so each level is one launch for every column.  Column
+ * r equals pbfmm_vmult_dev of column r (FP64 to rounding; FP32 within its
+ * rounding, as the atomics sum in another order).  Pure Neumann problems
+ * keep vmult's norm shifts, which apply one column at a time.  The
+ * reference applies its operator to one vector per BEMProblem::vmult
+ * (bem_problem.cc:1164-1217); the design contract's nrhs argument asks for
+ * this form. */
+int pbfmm_vmult_dev_multi(pbfmm_handle *h, const double *d_src, double *d_dst, int32_t nrhs, int64_t ld,
+                          int32_t precision);
+
 /* Device-pointer solve: rhs_dev from pbfmm_compute_rhs semantics already
  * applied (constrained rows set), x_dev initial guess in, solution out. */
 int pbfmm_gmres_dev(pbfmm_handle *h, const double *d_rhs, double *d_x, double rel_tol,
@@ -231,6 +244,14 @@ int pbfmm_work_counts(pbfmm_handle *h, int64_t *counts);
  * operator, in one Krylov space); 2 iterative refinement (FP32-operator GMRES
  * on FP64 residuals).  Either scheme ends on the true FP64 residual. */
 int pbfmm_set_mixed_scheme(pbfmm_handle *h, int32_t scheme);
+
+/* Overrides the near-field SpMV kernel that set-up's autotuner chose for one
+ * precision (PBFMM_FP64 or PBFMM_MIXED for the FP32 values), for both
+ * storages: 0 workgroup per leaf, 1 wavefront per leaf with 2 rows in flight,
+ * 2 with 4 rows.  For tests and tuning; call after
+ * pbfmm_direct_integrals.  No counterpart in the reference (Trilinos
+ * chooses its own SpMV). */
+int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel);
 
 /* ---- Post-solve (the step after the solve; real cells only) ----
  * Vector fields are component-major: v[d * n_dofs + i] is component d at DoF

@@ -521,6 +521,49 @@ int pbfmm_vmult_dev(pbfmm_handle *h, const double *src, double *dst, int32_t pre
   });
 }
 
+int pbfmm_vmult_dev_multi(pbfmm_handle *h, const double *src, double *dst, int32_t nrhs, int64_t ld,
+                          int32_t precision) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "vmult_dev_multi")) return rc;
+    Engine &e = h->eng;
+    const int64_t n = e.n;
+    if (!src || !dst || nrhs < 1 || ld < n)
+      return fail(h, PBFMM_ERR_ARG, "vmult_dev_multi: need src, dst, nrhs >= 1 and ld >= n_dofs");
+    const char *s0 = reinterpret_cast<const char *>(src), *d0 = reinterpret_cast<const char *>(dst);
+    const size_t span = (size_t)((nrhs - 1) * ld + n) * 8;
+    if (s0 < d0 + span && d0 < s0 + span) return fail(h, PBFMM_ERR_ARG, "vmult_dev_multi: src and dst overlap");
+    if (precision != PBFMM_FP64 && precision != PBFMM_MIXED)
+      return fail(h, PBFMM_ERR_ARG, "vmult_dev_multi: unknown precision");
+    if (precision == PBFMM_MIXED && !e.have_f32)
+      return fail(h, PBFMM_ERR_STATE, "vmult_dev_multi: handle was not created with PBFMM_MIXED");
+    ensure_alpha(h);
+    if (!e.have_dirichlet) {
+      // vmult's norm shifts (bem_problem.cc:1169-1172, 1214-1215) act on each column
+      for (int32_t r = 0; r < nrhs; ++r) e.vmult(src + r * ld, dst + r * ld, precision);
+      return PBFMM_OK;
+    }
+    e.multi_setup(nrhs);
+    DevBuf pb, px;
+    const double *S = src;
+    double *D = dst;
+    if (ld != n) {
+      if (hipMalloc(&pb.p, (size_t)nrhs * n * 8) != hipSuccess || hipMalloc(&px.p, (size_t)nrhs * n * 8) != hipSuccess)
+        throw HipError("vmult_dev_multi: packing buffers");
+      if (hipMemcpy2DAsync(pb.p, n * 8, src, ld * 8, n * 8, nrhs, hipMemcpyDeviceToDevice, e.stream) != hipSuccess)
+        throw HipError("vmult_dev_multi: packing copy");
+      S = pb.as<double>();
+      D = px.as<double>();
+    }
+    e.vmult_multi(S, D, nrhs, precision);
+    if (ld != n) {
+      if (hipMemcpy2DAsync(dst, ld * 8, D, n * 8, n * 8, nrhs, hipMemcpyDeviceToDevice, e.stream) != hipSuccess)
+        throw HipError("vmult_dev_multi: unpacking copy");
+      e.sync();  // the packing buffers are freed on return
+    }
+    return PBFMM_OK;
+  });
+}
+
 int pbfmm_gmres_dev(pbfmm_handle *h, const double *rhs, double *x, double rel_tol, int32_t restart, int32_t max_it,
                     int32_t precision, int32_t *iters, double *relres) {
   return guarded(h, [&]() -> int {
@@ -802,6 +845,20 @@ int pbfmm_set_mixed_scheme(pbfmm_handle *h, int32_t scheme) {
   if (scheme < 0 || scheme > 2) return fail(h, PBFMM_ERR_ARG, "set_mixed_scheme: scheme must be 0, 1 or 2");
   h->eng.mixed_scheme = scheme;
   return PBFMM_OK;
+}
+
+int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 3, "set_spmv_kernel")) return rc;
+    if (kernel < 0 || kernel > 2) return fail(h, PBFMM_ERR_ARG, "set_spmv_kernel: kernel must be 0, 1 or 2");
+    if (precision != PBFMM_FP64 && precision != PBFMM_MIXED)
+      return fail(h, PBFMM_ERR_ARG, "set_spmv_kernel: unknown precision");
+    if (precision == PBFMM_MIXED && !h->eng.have_f32)
+      return fail(h, PBFMM_ERR_STATE, "set_spmv_kernel: handle was not created with PBFMM_MIXED");
+    const int pi = precision == PBFMM_MIXED ? 1 : 0;
+    h->eng.spmv_variant[pi][0] = h->eng.spmv_variant[pi][1] = kernel;
+    return PBFMM_OK;
+  });
 }
 
 int pbfmm_projection_iterations(pbfmm_handle *h, int32_t *iters) {

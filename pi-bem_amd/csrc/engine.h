@@ -79,6 +79,7 @@ struct Engine {
   bool have_f32 = false;
   // operator vectors
   DevBuf d_alpha, d_sp, d_sd, d_outN, d_outD, d_in, d_out, d_red;
+  DevBuf d_cgs;  // partial sums of the Arnoldi step's dots (kDotParts per column)
   DevBuf d_c_ptr, d_c_idx, d_c_val, d_c_inh, d_c_is;
   bool have_constraints = false;
   std::vector<int32_t> hc_ptr, hc_idx;  // host copies of the constraint rows
@@ -115,6 +116,16 @@ struct Engine {
   // combined expansion per block and single-value near field / unit multipoles
   template <typename T>
   void matvec_combined(const double *x, double *out);
+  // matvec_combined in fewer launches (used by vmult while phase timing is
+  // off): ME and LE cleared together, L2P applying vmult's epilogue into
+  // post_dst (dst = out + alpha x on Neumann DoFs) and, with side = true, the
+  // SpMV and M2P run inside M2M and L2L level launches
+  template <typename T>
+  void matvec_fused(const double *x, double *out, double *post_dst, bool side);
+  // side fusion per precision ([0] FP64, [1] FP32): -1 until timed on first
+  // use, then 1 if the fused launches measured faster
+  int side_fusion[2] = {-1, -1};
+  bool use_side_fusion(bool f32);
   // near-field SpMV kernel per precision ([0] FP64, [1] FP32) and storage
   // (pairs, single values): 0 workgroup per leaf with an LDS copy of the
   // columns, 1 wavefront per leaf with 2 rows in flight, 2 with 4 rows;
@@ -168,7 +179,7 @@ struct Engine {
   int mr_R = 0, mr_nch = 0;
   ChunkList mr_m2m_h, mr_m2l_h, mr_l2l_h;
   ChunkDev mr_m2m, mr_m2l, mr_l2l;
-  DevBuf mr_me64, mr_le64, mr_me32, mr_le32, mr_sp, mr_out;
+  DevBuf mr_me64, mr_le64, mr_me32, mr_le32, mr_out;
   DevBuf mr_seg_off, mr_seg, mr_rch, mr_sch, mr_act, mr_segv;
   DevBuf mr_V, mr_w, mr_r, mr_d, mr_t1, mr_t2, mr_hs, mr_ys, mr_red;
   int mr_restart = 0;

@@ -90,10 +90,38 @@ static void upload_legendre_tables() {
   HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_pmf_f), pmff, sizeof(pmff)));
 }
 
+// Wavefront sums through DPP lane moves (VALU operand modifiers, no LDS
+// crossbar round trips): quad_perm [1,0,3,2] and [2,3,0,1] sum each quad,
+// row_half_mirror pairs lane i with 7-i and row_mirror lane i with 15-i, so
+// after four adds every lane holds its 16-lane row's sum; the four row sums
+// are read into scalars and added.  Every lane of the wavefront must be
+// active; the sum is returned in every lane.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float read_lane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double read_lane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  v += dpp_move<0x140>(v);  // row_mirror
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 // ---------------------------------------------------------------------------
@@ -226,9 +254,9 @@ __global__ void k_round_f32(int64_t n, const double *__restrict__ a, float *__re
 // ---------------------------------------------------------------------------
 // Near-field SpMV, fused N.phi and D.dphi on the shared pattern
 // (prec_neumann_matrix.vmult / prec_dirichlet_matrix.vmult, bem_fma.cc:2408-2409),
-// on the leaf-blocked layout.  Two kernels; set-up times both on the
-// assembled operator and keeps the faster one per precision (many small
-// leaves favour the first, many rows per leaf the second).
+// on the leaf-blocked layout.  Three kernels (a workgroup per leaf; a
+// wavefront per leaf with 2 or 4 rows in flight); set-up times them on the
+// assembled operator and keeps the fastest per precision and storage.
 //
 // k_spmv_lds: one workgroup per leaf.  All rows of a leaf share
 // its column set, so the workgroup gathers (phi, dphi) at those columns once
@@ -249,7 +277,12 @@ constexpr int kXcd = 8;
 // NK = 2: (N, D) pairs per entry, outN = N.phi and outD = D.dphi.  NK = 1:
 // one value per entry (the operator's matrix: N on Neumann columns, -D on
 // Dirichlet columns), outN = M.phi (dphi and outD unused).
-template <typename T, int NK> struct NearEntry { using type = typename Vec2<T>::type; };
+// (N, D) pairs are clang's native two-lane vectors: register-resident arrays
+// of them stay out of scratch memory.
+template <typename T> struct NativePair;
+template <> struct NativePair<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct NativePair<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <typename T, int NK> struct NearEntry { using type = typename NativePair<T>::type; };
 template <typename T> struct NearEntry<T, 1> { using type = T; };
 template <typename T, int NK>
 __device__ __forceinline__ typename NearEntry<T, NK>::type near_x(const double *phi, const double *dphi, int col) {
@@ -272,27 +305,33 @@ __device__ __forceinline__ void near_store(double *outN, double *outD, int node,
   if constexpr (NK == 2) outD[node] = (first ? 0.0 : outD[node]) + aD;
 }
 
-template <typename T, int NK>
-__global__ void __launch_bounds__(256) k_spmv_lds(int nleaf, int per_xcd, const int *__restrict__ row_beg,
-                                                  const int *__restrict__ rows, const int *__restrict__ col_beg,
-                                                  const int *__restrict__ cols, const int64_t *__restrict__ val_beg,
-                                                  const T *__restrict__ vals, const double *__restrict__ phi,
-                                                  const double *__restrict__ dphi, double *__restrict__ outN,
-                                                  double *__restrict__ outD) {
+// The near field's per-leaf blocks and the vectors of one SpMV.
+template <typename T>
+struct NearArgs {
+  const int *row_beg, *rows, *col_beg, *cols;
+  const int64_t *val_beg;
+  const T *vals;
+  const double *phi, *dphi;
+  double *outN, *outD;
+};
+
+// One leaf by a workgroup of NT threads: its inputs gathered once into LDS
+// (xs, kSpmvTile entries), each wavefront streaming whole rows.
+template <typename T, int NK, int NT>
+__device__ __forceinline__ void spmv_lds_leaf(int leaf, typename NearEntry<T, NK>::type *__restrict__ xs,
+                                              const NearArgs<T> &a) {
   using E = typename NearEntry<T, NK>::type;
-  __shared__ E xs[kSpmvTile];
-  const int leaf = (int)(blockIdx.x % kXcd) * per_xcd + (int)(blockIdx.x / kXcd);
-  if (leaf >= nleaf) return;
+  constexpr int NW = NT / 64;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0;
-  const int cb = col_beg[leaf], C = col_beg[leaf + 1] - cb;
-  const E *V = reinterpret_cast<const E *>(vals) + val_beg[leaf];
+  const int r0 = a.row_beg[leaf], R = a.row_beg[leaf + 1] - r0;
+  const int cb = a.col_beg[leaf], C = a.col_beg[leaf + 1] - cb;
+  const E *V = reinterpret_cast<const E *>(a.vals) + a.val_beg[leaf];
   for (int c0 = 0; c0 < C; c0 += kSpmvTile) {
     const int CT = min(kSpmvTile, C - c0);
     if (c0 > 0) __syncthreads();  // previous tile fully consumed
-    for (int c = t; c < CT; c += 256) xs[c] = near_x<T, NK>(phi, dphi, cols[cb + c0 + c]);
+    for (int c = t; c < CT; c += NT) xs[c] = near_x<T, NK>(a.phi, a.dphi, a.cols[cb + c0 + c]);
     __syncthreads();
-    for (int r = w; r < R; r += 4) {
+    for (int r = w; r < R; r += NW) {
       const E *v = V + (int64_t)r * C + c0;
       T aN = 0, aD = 0;
 #pragma unroll 4
@@ -300,52 +339,55 @@ __global__ void __launch_bounds__(256) k_spmv_lds(int nleaf, int per_xcd, const 
       aN = wave_sum(aN);
       if constexpr (NK == 2) aD = wave_sum(aD);
       // the wave that owns row r owns it in every tile: plain read-modify-write
-      if (lane == 0) near_store<NK>(outN, outD, rows[r0 + r], c0 == 0, (double)aN, (double)aD);
+      if (lane == 0) near_store<NK>(a.outN, a.outD, a.rows[r0 + r], c0 == 0, (double)aN, (double)aD);
     }
   }
 }
 
-// k_spmv_wave: one wavefront per leaf, no LDS and no barrier, RPI rows per
-// iteration.  For each tile
-// of 64 x CPL leaf columns (64 bytes of values per lane and row: CPL = 8 / 4
-// for FP32 / FP64 pairs, 16 / 8 for single values) every lane holds the
-// inputs of its CPL columns in registers, and the wave streams two rows at a
-// time against them (2 CPL coalesced loads per lane in flight).  Outputs as
-// k_spmv_lds.
+// Leaves are dealt out so that each XCD takes a contiguous range of per_xcd
+// leaves (workgroups go round-robin over the 8 XCDs): neighbouring leaves,
+// whose columns overlap, share an L2.
+template <typename T, int NK>
+__global__ void __launch_bounds__(256) k_spmv_lds(int nleaf, int per_xcd, NearArgs<T> a) {
+  using E = typename NearEntry<T, NK>::type;
+  __shared__ E xs[kSpmvTile];
+  const int leaf = (int)(blockIdx.x % kXcd) * per_xcd + (int)(blockIdx.x / kXcd);
+  if (leaf >= nleaf) return;
+  spmv_lds_leaf<T, NK, 256>(leaf, xs, a);
+}
+
+// One leaf by one wavefront, no LDS and no barrier, RPI rows per iteration.
+// For each tile of 64 x CPL leaf columns (64 bytes of values per lane and
+// row: CPL = 8 / 4 for FP32 / FP64 pairs, 16 / 8 for single values) every
+// lane holds the inputs of its CPL columns in registers, and the wave streams
+// RPI rows at a time against them (RPI CPL coalesced loads per lane in
+// flight).  Outputs as spmv_lds_leaf.
 template <typename T, int NK, int RPI>
-__global__ void __launch_bounds__(256) k_spmv_wave(int nleaf, const int *__restrict__ row_beg,
-                                                   const int *__restrict__ rows, const int *__restrict__ col_beg,
-                                                   const int *__restrict__ cols, const int64_t *__restrict__ val_beg,
-                                                   const T *__restrict__ vals, const double *__restrict__ phi,
-                                                   const double *__restrict__ dphi, double *__restrict__ outN,
-                                                   double *__restrict__ outD) {
+__device__ __forceinline__ void spmv_wave_leaf(int leaf, int lane, const NearArgs<T> &a) {
   using E = typename NearEntry<T, NK>::type;
   constexpr int CPL = 64 / (int)sizeof(E);
   constexpr int TILE = 64 * CPL;
-  const int leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (leaf >= nleaf) return;
-  const int lane = threadIdx.x & 63;
-  const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0;
-  const int cb = col_beg[leaf], C = col_beg[leaf + 1] - cb;
-  const E *V = reinterpret_cast<const E *>(vals) + val_beg[leaf];
+  const int r0 = a.row_beg[leaf], R = a.row_beg[leaf + 1] - r0;
+  const int cb = a.col_beg[leaf], C = a.col_beg[leaf + 1] - cb;
+  const E *V = reinterpret_cast<const E *>(a.vals) + a.val_beg[leaf];
   const E zero{};
   for (int c0 = 0; c0 < C; c0 += TILE) {
     E x[CPL];
 #pragma unroll
     for (int j = 0; j < CPL; ++j) {
       const int c = c0 + lane + 64 * j;
-      x[j] = c < C ? near_x<T, NK>(phi, dphi, cols[cb + c]) : zero;
+      x[j] = c < C ? near_x<T, NK>(a.phi, a.dphi, a.cols[cb + c]) : zero;
     }
     // RPI rows per iteration: their loads are in flight together
     for (int r = 0; r < R; r += RPI) {
-      E a[RPI][CPL];
+      E v[RPI][CPL];
 #pragma unroll
       for (int q = 0; q < RPI; ++q) {
         const E *vq = V + (int64_t)(r + q < R ? r + q : r) * C;
 #pragma unroll
         for (int j = 0; j < CPL; ++j) {
           const int c = c0 + lane + 64 * j;
-          a[q][j] = c < C ? vq[c] : zero;
+          v[q][j] = c < C ? vq[c] : zero;
         }
       }
       T aN[RPI], aD[RPI];
@@ -354,7 +396,7 @@ __global__ void __launch_bounds__(256) k_spmv_wave(int nleaf, const int *__restr
         aN[q] = 0;
         aD[q] = 0;
 #pragma unroll
-        for (int j = 0; j < CPL; ++j) near_fma<T, NK>(aN[q], aD[q], a[q][j], x[j]);
+        for (int j = 0; j < CPL; ++j) near_fma<T, NK>(aN[q], aD[q], v[q][j], x[j]);
         aN[q] = wave_sum(aN[q]);
         if constexpr (NK == 2) aD[q] = wave_sum(aD[q]);
       }
@@ -362,9 +404,16 @@ __global__ void __launch_bounds__(256) k_spmv_wave(int nleaf, const int *__restr
       if (lane == 0)
 #pragma unroll
         for (int q = 0; q < RPI; ++q)
-          if (r + q < R) near_store<NK>(outN, outD, rows[r0 + r + q], c0 == 0, (double)aN[q], (double)aD[q]);
+          if (r + q < R) near_store<NK>(a.outN, a.outD, a.rows[r0 + r + q], c0 == 0, (double)aN[q], (double)aD[q]);
     }
   }
+}
+
+template <typename T, int NK, int RPI>
+__global__ void __launch_bounds__(256) k_spmv_wave(int nleaf, NearArgs<T> a) {
+  const int leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (leaf >= nleaf) return;
+  spmv_wave_leaf<T, NK, RPI>(leaf, threadIdx.x & 63, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -615,14 +664,24 @@ struct TrLayout {
 // 64 pairs.
 // KIND (0 M2M, 1 M2L, 2 L2L) only names the instantiation, so that traces and
 // profiles tell the three passes apart.
+template <typename T>
+struct TrArgs {
+  const int *chunk_op, *chunk_beg, *chunk_cnt, *tgt, *src;
+  const T *ops, *IN;
+  T *OUT;
+};
+
 template <typename T, int P, int KIND, int NK>
-__global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(const int *__restrict__ chunk_op,
-                                                                  const int *__restrict__ chunk_beg,
-                                                                  const int *__restrict__ chunk_cnt,
-                                                                  const int *__restrict__ tgt,
-                                                                  const int *__restrict__ src,
-                                                                  const T *__restrict__ ops,
-                                                                  const T *__restrict__ IN, T *__restrict__ OUT) {
+__device__ __forceinline__ void translate_chunk(int c, T *__restrict__ sB, int *__restrict__ sT,
+                                                int *__restrict__ sS, const TrArgs<T> &args) {
+  const int *__restrict__ chunk_op = args.chunk_op;
+  const int *__restrict__ chunk_beg = args.chunk_beg;
+  const int *__restrict__ chunk_cnt = args.chunk_cnt;
+  const int *__restrict__ tgt = args.tgt;
+  const int *__restrict__ src = args.src;
+  const T *__restrict__ ops = args.ops;
+  const T *__restrict__ IN = args.IN;
+  T *__restrict__ OUT = args.OUT;
   using Lay = TrLayout<P, T>;
   constexpr int KP = Lay::KP, K2 = NK * KP, BS = Lay::BS, CS = Lay::CS, NT = Lay::NT;
   constexpr int PC = 64 / NK;                // pairs per chunk
@@ -634,10 +693,7 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(const int *__res
   constexpr int NVP = K2 / V;                // vectors per pair (both kernels)
   constexpr int NV = PC * NVP;               // vectors per chunk
   constexpr int PER = (NV + NT - 1) / NT;    // vectors per thread
-  __shared__ __align__(16) T sB[Lay::elems];
-  __shared__ int sT[PC], sS[PC];
   constexpr int KT2 = 2 * Kp<P>::kt;
-  const int c = blockIdx.x;
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
   const T *A = ops + (int64_t)chunk_op[c] * KP * KP;  // column-major, read from L2
   const int beg = chunk_beg[c], cnt = chunk_cnt[c];
@@ -726,6 +782,13 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(const int *__res
   }
 }
 
+template <typename T, int P, int KIND, int NK>
+__global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(TrArgs<T> a) {
+  __shared__ __align__(16) T sB[TrLayout<P, T>::elems];
+  __shared__ int sT[64 / NK], sS[64 / NK];
+  translate_chunk<T, P, KIND, NK>(blockIdx.x, sB, sT, sS, a);
+}
+
 // Evaluation of both kernels' expansions at a point: M2P
 // (MultipoleExpansion::Evaluate, multipole_expansion.cc:264-298, rho^(-n-1))
 // or L2P (LocalExpansion::Evaluate, local_expansion.cc:251-287, rho^n).
@@ -776,13 +839,11 @@ __device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, in
 // M2P for smaller far blocks (bem_fma.cc:2664-2704): one thread per
 // (source block, target node) evaluation, listed at set-up.
 template <typename T, int NK>
-__global__ void __launch_bounds__(256) k_m2p(int64_t nitems, const int *__restrict__ item_node,
-                                             const int *__restrict__ item_src, const double *__restrict__ xyz,
-                                             const double *__restrict__ pmin, const double *__restrict__ delta, int p,
-                                             int kt, int kp, const T *__restrict__ ME, double *__restrict__ outN,
-                                             double *__restrict__ outD) {
-  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (w >= nitems) return;
+__device__ __forceinline__ void m2p_item(int64_t w, const int *__restrict__ item_node,
+                                         const int *__restrict__ item_src, const double *__restrict__ xyz,
+                                         const double *__restrict__ pmin, const double *__restrict__ delta, int p,
+                                         int kt, int kp, const T *__restrict__ ME, double *__restrict__ outN,
+                                         double *__restrict__ outD) {
   const int node = item_node[w], sblk = item_src[w];
   const double hd = delta[sblk] * 0.5;
   const T d0 = (T)(xyz[3 * node] - (pmin[3 * sblk] + hd));
@@ -792,6 +853,17 @@ __global__ void __launch_bounds__(256) k_m2p(int64_t nitems, const int *__restri
   eval2<T, false, NK>(ME + (int64_t)sblk * NK * kp, p, kt, kp, d0, d1, d2, (T)delta[sblk], v1, v2);
   atomicAdd(outN + node, (double)v1);
   if constexpr (NK == 2) atomicAdd(outD + node, (double)v2);
+}
+
+template <typename T, int NK>
+__global__ void __launch_bounds__(256) k_m2p(int64_t nitems, const int *__restrict__ item_node,
+                                             const int *__restrict__ item_src, const double *__restrict__ xyz,
+                                             const double *__restrict__ pmin, const double *__restrict__ delta, int p,
+                                             int kt, int kp, const T *__restrict__ ME, double *__restrict__ outN,
+                                             double *__restrict__ outD) {
+  const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (w >= nitems) return;
+  m2p_item<T, NK>(w, item_node, item_src, xyz, pmin, delta, p, kt, kp, ME, outN, outD);
 }
 
 // L2P at every node of every leaf (bem_fma.cc:2959-2986): one thread per node.
@@ -816,10 +888,93 @@ __global__ void __launch_bounds__(256) k_l2p(int nrows, const int *__restrict__ 
   if constexpr (NK == 2) outD[node] += (double)v2;
 }
 
+// L2P of the operator pipeline with BEMProblem::vmult's epilogue
+// (k_vmult_post) folded in: dst = out + L2P + alpha x on Neumann DoFs.
+template <typename T>
+__global__ void __launch_bounds__(256) k_l2p_post(int nrows, const int *__restrict__ rows,
+                                                  const int *__restrict__ row_leaf, const int *__restrict__ leaf_block,
+                                                  const double *__restrict__ xyz, const double *__restrict__ pmin,
+                                                  const double *__restrict__ delta, int p, int kt, int kp,
+                                                  const T *__restrict__ LE, const double *__restrict__ out,
+                                                  const double *__restrict__ alpha, const uint8_t *__restrict__ dir,
+                                                  const double *__restrict__ x, double *__restrict__ dst) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const int node = rows[r];
+  const int b = leaf_block[row_leaf[r]];
+  const double hd = delta[b] * 0.5;
+  const T d0 = (T)(xyz[3 * node] - (pmin[3 * b] + hd));
+  const T d1 = (T)(xyz[3 * node + 1] - (pmin[3 * b + 1] + hd));
+  const T d2 = (T)(xyz[3 * node + 2] - (pmin[3 * b + 2] + hd));
+  T v1, v2;
+  eval2<T, true, 1>(LE + (int64_t)b * kp, p, kt, kp, d0, d1, d2, (T)delta[b], v1, v2);
+  const double nd = out[node] + (double)v1;
+  dst[node] = nd + (dir[node] ? 0.0 : alpha[node] * x[node]);
+}
+
 template <typename T>
 __global__ void k_fill(T *__restrict__ a, int64_t n, T v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     a[i] = v;
+}
+
+template <typename T>
+__global__ void k_fill2(T *__restrict__ a, T *__restrict__ b, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 2 * n; i += (int64_t)gridDim.x * blockDim.x)
+    (i < n ? a[i] : b[i - n]) = v;
+}
+
+// Work that shares a launch with one level of M2M or L2L in the operator
+// (single-value) pipeline: the near-field SpMV of leaves [lo, lo + count), or
+// the M2P evaluations [m2p_lo, m2p_hi).
+template <typename T>
+struct SideArgs {
+  NearArgs<T> near;
+  int lo, count, per_xcd;
+  int64_t m2p_lo, m2p_hi;
+  const int *item_node, *item_src;
+  const double *xyz, *pmin, *delta;
+  int p, kt, kp;
+  const T *ME;
+  double *out;
+};
+
+// Horizontal fusion.  The level launches of M2M and L2L hold few chunks
+// (one workgroup each) and wait on latency, leaving most CUs idle; the
+// workgroups after the level's nch chunks run independent work meanwhile:
+// SIDE 0..2 the SpMV (workgroup per leaf; wavefront per leaf with 2 or 4
+// rows in flight), SIDE 3 the M2P evaluations.  The SpMV overwrites its rows
+// and M2P adds to them, so the SpMV goes with an M2M level (before M2L) and
+// M2P with an L2L level (after it).  Each goes whole into one launch: its
+// workgroups are latency-bound themselves, and shares spread over several
+// launches would pay that latency once per launch.  One LDS buffer serves
+// either role.
+template <typename T, int P, int KIND, int SIDE>
+__global__ void __launch_bounds__(4 * Kp<P>::value) k_translate_side(int nch, TrArgs<T> tr, SideArgs<T> sd) {
+  constexpr int NT = 4 * Kp<P>::value;
+  using E = typename NearEntry<T, 1>::type;
+  constexpr size_t TB = TrLayout<P, T>::bytes;
+  constexpr size_t SB = SIDE == 0 ? kSpmvTile * sizeof(E) : 0;
+  __shared__ __align__(16) unsigned char smem[TB > SB ? TB : SB];
+  __shared__ int sT[64], sS[64];
+  if ((int)blockIdx.x < nch) {
+    translate_chunk<T, P, KIND, 1>(blockIdx.x, reinterpret_cast<T *>(smem), sT, sS, tr);
+    return;
+  }
+  const int b = (int)blockIdx.x - nch;
+  if constexpr (SIDE == 0) {
+    // leaves dealt out per XCD as in k_spmv_lds (b and blockIdx.x differ by nch)
+    const int leaf = (b % kXcd) * sd.per_xcd + b / kXcd;
+    if (leaf < sd.count) spmv_lds_leaf<T, 1, NT>(sd.lo + leaf, reinterpret_cast<E *>(smem), sd.near);
+  } else if constexpr (SIDE == 1 || SIDE == 2) {
+    const int leaf = b * (NT / 64) + (int)(threadIdx.x >> 6);
+    if (leaf < sd.count) spmv_wave_leaf<T, 1, SIDE == 1 ? 2 : 4>(sd.lo + leaf, threadIdx.x & 63, sd.near);
+  } else {
+    const int64_t w = sd.m2p_lo + (int64_t)b * NT + threadIdx.x;
+    if (w < sd.m2p_hi)
+      m2p_item<T, 1>(w, sd.item_node, sd.item_src, sd.xyz, sd.pmin, sd.delta, sd.p, sd.kt, sd.kp, sd.ME, sd.out,
+                     nullptr);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1002,6 +1157,63 @@ __global__ void k_scale_to(int64_t n, const double *__restrict__ x, double s, do
 __global__ void k_scale_inv(int64_t n, double *__restrict__ v, const double *__restrict__ nrm) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const double s = *nrm;
+  if (i < n && s > 0) v[i] /= s;
+}
+
+// The Arnoldi step's dot products with a fixed, small number of partial sums
+// per column (P row ranges), which lets the kernels that use the sums finish
+// them themselves instead of in launches of their own: V_j . w for j < nv
+// into part[j * P + range].
+constexpr int kDotParts = 16;   // per column of V . w (consumers re-read nv * 16 partials)
+constexpr int kNormParts = 64;  // of |w|^2 (one wavefront finishes them)
+__global__ void __launch_bounds__(kRedThreads) k_mdot_parts(int64_t n, int64_t ldv, const double *__restrict__ V,
+                                                            const double *__restrict__ w, double *__restrict__ part) {
+  __shared__ double sh[kRedThreads / 64];
+  const int j = blockIdx.y, P = (int)gridDim.x;
+  const double *v = V + (int64_t)j * ldv;
+  const int64_t len = (n + P - 1) / P;
+  const int64_t beg = (int64_t)blockIdx.x * len, end = min(n, beg + len);
+  double acc = 0;
+#pragma unroll 4
+  for (int64_t i = beg + threadIdx.x; i < end; i += kRedThreads) acc += v[i] * w[i];
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[(int64_t)j * P + blockIdx.x] = r;
+}
+// w -= sum_j V_j h_j, h_j the sum of column j's P partials, summed by every
+// block in the same order; block 0 also stores h (a Hessenberg column).
+// nv <= 128 (GMRES restart <= 127).
+__global__ void __launch_bounds__(256) k_gemv_sub_fin(int64_t n, int64_t ldv, int nv, const double *__restrict__ V,
+                                                      const double *__restrict__ part, int P,
+                                                      double *__restrict__ h_out, double *__restrict__ w) {
+  __shared__ double hs[128];
+  for (int j = threadIdx.x; j < nv; j += blockDim.x) {
+    double s = 0;
+    for (int c = 0; c < P; ++c) s += part[(int64_t)j * P + c];
+    hs[j] = s;
+    if (blockIdx.x == 0) h_out[j] = s;
+  }
+  __syncthreads();
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = w[i];
+  for (int j = 0; j < nv; ++j) acc -= V[(int64_t)j * ldv + i] * hs[j];
+  w[i] = acc;
+}
+// v /= |v| with |v|^2 the sum of P <= 64 partials (one wavefront per block
+// finishes them, in the same order in every block); block 0 stores |v|.
+__global__ void __launch_bounds__(256) k_scale_fin(int64_t n, double *__restrict__ v, const double *__restrict__ part,
+                                                   int P, double *__restrict__ nrm_out) {
+  __shared__ double s_nrm;
+  if (threadIdx.x < 64) {
+    const double q = wave_sum((int)threadIdx.x < P ? part[threadIdx.x] : 0.0);
+    if (threadIdx.x == 0) {
+      s_nrm = sqrt(q);
+      if (blockIdx.x == 0) *nrm_out = s_nrm;
+    }
+  }
+  __syncthreads();
+  const double s = s_nrm;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n && s > 0) v[i] /= s;
 }
 
@@ -1369,11 +1581,11 @@ static void launch_translate(Engine &e, hipStream_t st, const ChunkDev &cl, int 
   const int n = c1 - c0;
   if (n <= 0) return;
   const DevBuf &op = NK == 2 ? cl.op : cl.op1, &beg = NK == 2 ? cl.beg : cl.beg1, &cnt = NK == 2 ? cl.cnt : cl.cnt1;
+  const TrArgs<T> tr{op.as<int>() + c0, beg.as<int>() + c0, cnt.as<int>() + c0, cl.tgt.as<int>(), cl.src.as<int>(),
+                     ops, IN, OUT};
 #define PBFMM_TR(PP)                                                                                         \
   case PP:                                                                                                   \
-    hipLaunchKernelGGL((k_translate<T, PP, KIND, NK>), dim3(n), dim3(TrLayout<PP, T>::NT), 0, st,           \
-                       op.as<int>() + c0, beg.as<int>() + c0, cnt.as<int>() + c0, cl.tgt.as<int>(),          \
-                       cl.src.as<int>(), ops, IN, OUT);                                                      \
+    hipLaunchKernelGGL((k_translate<T, PP, KIND, NK>), dim3(n), dim3(TrLayout<PP, T>::NT), 0, st, tr);        \
     break;
   switch (e.p) {
     PBFMM_TR(1) PBFMM_TR(2) PBFMM_TR(3) PBFMM_TR(4) PBFMM_TR(5)
@@ -1383,22 +1595,57 @@ static void launch_translate(Engine &e, hipStream_t st, const ChunkDev &cl, int 
 #undef PBFMM_TR
 }
 
+// Workgroups of the side work for a launch of NT threads.
+template <int SIDE, typename T>
+static int side_blocks(const SideArgs<T> &sd, int nt) {
+  if (SIDE == 0) return sd.count > 0 ? sd.per_xcd * kXcd : 0;
+  if (SIDE == 1 || SIDE == 2) return (sd.count + nt / 64 - 1) / (nt / 64);
+  return (int)((sd.m2p_hi - sd.m2p_lo + nt - 1) / nt);
+}
+
+// One level of the single-value pipeline's M2M (KIND 0) or L2L (KIND 2),
+// chunks [c0, c1), with side work sd in the same launch (k_translate_side).
+template <typename T, int KIND, int SIDE>
+static void launch_translate_side(Engine &e, const ChunkDev &cl, int c0, int c1, const T *ops, const T *IN, T *OUT,
+                                  const SideArgs<T> &sd) {
+  const int nch = std::max(0, c1 - c0);
+  TrArgs<T> tr{};
+  if (nch > 0)
+    tr = TrArgs<T>{cl.op1.as<int>() + c0, cl.beg1.as<int>() + c0, cl.cnt1.as<int>() + c0, cl.tgt.as<int>(),
+                   cl.src.as<int>(), ops, IN, OUT};
+#define PBFMM_TRS(PP)                                                                                        \
+  case PP: {                                                                                                 \
+    constexpr int NT = TrLayout<PP, T>::NT;                                                                  \
+    const int nb = nch + side_blocks<SIDE>(sd, NT);                                                         \
+    if (nb > 0)                                                                                              \
+      hipLaunchKernelGGL((k_translate_side<T, PP, KIND, SIDE>), dim3(nb), dim3(NT), 0, e.stream, nch, tr, sd); \
+    break;                                                                                                   \
+  }
+  switch (e.p) {
+    PBFMM_TRS(1) PBFMM_TRS(2) PBFMM_TRS(3) PBFMM_TRS(4) PBFMM_TRS(5)
+    PBFMM_TRS(6) PBFMM_TRS(7) PBFMM_TRS(8) PBFMM_TRS(9) PBFMM_TRS(10)
+    default: throw std::runtime_error("trunc_order outside 1..10");
+  }
+#undef PBFMM_TRS
+}
+
+template <typename T, int NK>
+static NearArgs<T> near_args(Engine &e, const double *phi, const double *dphi, double *outN, double *outD) {
+  const T *vals = (NK == 2 ? far_of<T>(e).vals : far_of<T>(e).opvals).template as<T>();
+  return NearArgs<T>{e.d_row_beg.as<int>(), e.d_rows.as<int>(), e.d_col_beg.as<int>(), e.d_cols.as<int>(),
+                     e.d_val_beg.as<int64_t>(), vals, phi, dphi, outN, outD};
+}
+
 template <typename T, int NK>
 void Engine::launch_spmv(int variant, const double *phi, const double *dphi, double *outN, double *outD) {
-  const T *vals = (NK == 2 ? far_of<T>(*this).vals : far_of<T>(*this).opvals).template as<T>();
+  const NearArgs<T> a = near_args<T, NK>(*this, phi, dphi, outN, outD);
   if (variant == 0) {
     const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
-    hipLaunchKernelGGL((k_spmv_lds<T, NK>), dim3(per_xcd * kXcd), dim3(256), 0, stream, near.nleaf, per_xcd,
-                       d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_cols.as<int>(),
-                       d_val_beg.as<int64_t>(), vals, phi, dphi, outN, outD);
+    hipLaunchKernelGGL((k_spmv_lds<T, NK>), dim3(per_xcd * kXcd), dim3(256), 0, stream, near.nleaf, per_xcd, a);
   } else if (variant == 1) {
-    hipLaunchKernelGGL((k_spmv_wave<T, NK, 2>), dim3((near.nleaf + 3) / 4), dim3(256), 0, stream, near.nleaf,
-                       d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_cols.as<int>(),
-                       d_val_beg.as<int64_t>(), vals, phi, dphi, outN, outD);
+    hipLaunchKernelGGL((k_spmv_wave<T, NK, 2>), dim3((near.nleaf + 3) / 4), dim3(256), 0, stream, near.nleaf, a);
   } else {
-    hipLaunchKernelGGL((k_spmv_wave<T, NK, 4>), dim3((near.nleaf + 3) / 4), dim3(256), 0, stream, near.nleaf,
-                       d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_cols.as<int>(),
-                       d_val_beg.as<int64_t>(), vals, phi, dphi, outN, outD);
+    hipLaunchKernelGGL((k_spmv_wave<T, NK, 4>), dim3((near.nleaf + 3) / 4), dim3(256), 0, stream, near.nleaf, a);
   }
 }
 
@@ -1511,6 +1758,95 @@ void Engine::matvec_combined(const double *x, double *out) {
   collect_phase_times();
 }
 
+// The single-value pipeline in fewer launches: ME and LE cleared in one
+// launch; P2M; M2M levels L..1; M2L; L2L levels 2..L; L2P, which also
+// applies vmult's epilogue into post_dst (every DoF is a row of one leaf).
+// With side = true the deepest non-empty M2M level also runs the whole SpMV
+// and the deepest non-empty L2L level all M2P evaluations
+// (k_translate_side); otherwise they have launches of their own.  The same
+// arithmetic as generate + products + k_vmult_post.  Phase timing runs that
+// unfused sequence instead, so that each phase can be timed.
+template <typename T>
+void Engine::matvec_fused(const double *x, double *out, double *post_dst, bool side) {
+  FarDev<T> &F = far_of<T>(*this);
+  const int nleaf = (int)far.p2m_leaf.size();
+  const int64_t ne = (int64_t)tree.nb * kp;
+  T *me = F.me.template as<T>(), *le = F.le.template as<T>();
+  hipLaunchKernelGGL(k_fill2<T>, dim3((unsigned)std::min<int64_t>(nblk(2 * ne, 256), 4096)), dim3(256), 0, stream, me,
+                     le, ne, T(0));
+  if (nleaf > 0)
+    hipLaunchKernelGGL((k_p2m<T, 1>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
+                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp, x, nullptr, me);
+  const int var = spmv_variant[sizeof(T) == 4 ? 1 : 0][0];
+  const int nl = near.nleaf;
+  SideArgs<T> sd{};
+  sd.near = near_args<T, 1>(*this, x, nullptr, out, nullptr);
+  bool near_done = nl == 0 || !side;
+  if (!side && nl > 0) launch_spmv<T, 1>(var, x, nullptr, out, nullptr);
+  for (int l = L; l >= 1; --l) {
+    const int c0 = lvl<1>(far.m2m, l), c1 = lvl<1>(far.m2m, l + 1);
+    if (c1 <= c0) continue;
+    if (near_done) {
+      launch_translate<T, 0, 1>(*this, stream, m2m_cl, c0, c1, F.m2m.template as<T>(), me, me);
+      continue;
+    }
+    sd.lo = 0;
+    sd.count = near_done ? 0 : nl;
+    sd.per_xcd = (sd.count + kXcd - 1) / kXcd;
+    const T *ops = F.m2m.template as<T>();
+    if (var == 0)
+      launch_translate_side<T, 0, 0>(*this, m2m_cl, c0, c1, ops, me, me, sd);
+    else if (var == 1)
+      launch_translate_side<T, 0, 1>(*this, m2m_cl, c0, c1, ops, me, me, sd);
+    else
+      launch_translate_side<T, 0, 2>(*this, m2m_cl, c0, c1, ops, me, me, sd);
+    near_done = true;
+  }
+  if (!near_done) launch_spmv<T, 1>(var, x, nullptr, out, nullptr);
+  launch_translate<T, 1, 1>(*this, stream, m2l_cl, 0, nchunks<1>(far.m2l), F.m2l.template as<T>(), me, le);
+  const int64_t ni = (int64_t)far.m2p_item_node.size();
+  sd.count = 0;
+  sd.item_node = d_m2p_node.as<int>();
+  sd.item_src = d_m2p_isrc.as<int>();
+  sd.xyz = d_xyz.as<double>();
+  sd.pmin = d_pmin.as<double>();
+  sd.delta = d_delta.as<double>();
+  sd.p = p;
+  sd.kt = kt;
+  sd.kp = kp;
+  sd.ME = me;
+  sd.out = out;
+  int deepest = 0;  // deepest L2L level with chunks: it also runs M2P
+  for (int l = 2; side && l <= L; ++l)
+    if (lvl<1>(far.l2l, l + 1) > lvl<1>(far.l2l, l)) deepest = l;
+  for (int l = 2; l <= L; ++l) {
+    const int c0 = lvl<1>(far.l2l, l), c1 = lvl<1>(far.l2l, l + 1);
+    if (c1 <= c0) continue;
+    if (l != deepest) {
+      launch_translate<T, 2, 1>(*this, stream, l2l_cl, c0, c1, F.l2l.template as<T>(), le, le);
+      continue;
+    }
+    sd.m2p_lo = 0;
+    sd.m2p_hi = ni;
+    launch_translate_side<T, 2, 3>(*this, l2l_cl, c0, c1, F.l2l.template as<T>(), le, le, sd);
+  }
+  if (deepest == 0 && ni > 0)
+    hipLaunchKernelGGL((k_m2p<T, 1>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, stream, ni, d_m2p_node.as<int>(),
+                       d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt,
+                       kp, me, out, nullptr);
+  const int nrows = (int)near.rows.size();
+  if (post_dst)
+    hipLaunchKernelGGL((k_l2p_post<T>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
+                       d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
+                       d_delta.as<double>(), p, kt, kp, le, out, d_alpha.as<double>(), d_dir.as<uint8_t>(), x,
+                       post_dst);
+  else if (nrows > 0)
+    hipLaunchKernelGGL((k_l2p<T, 1>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
+                       d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
+                       d_delta.as<double>(), p, kt, kp, le, out, nullptr);
+  HIPCHK(hipGetLastError());
+}
+
 template void Engine::matvec<double>(const double *, const double *, double *, double *);
 template void Engine::matvec<float>(const double *, const double *, double *, double *);
 template void Engine::matvec_combined<double>(const double *, double *);
@@ -1528,6 +1864,41 @@ double Engine::norm2(const double *v) {
   HIPCHK(hipMemcpyAsync(&h, res, 8, hipMemcpyDeviceToHost, stream));
   sync();
   return std::sqrt(h);
+}
+
+// Side fusion pays while the M2M / L2L level launches are latency-bound and
+// the SpMV and M2P are short beside them (tens of thousands of DoFs); on
+// large trees the fused workgroups only crowd each other.  Timed once per
+// precision, on the first operator application: three applications each
+// way on scratch vectors, after one warm-up.
+bool Engine::use_side_fusion(bool f32) {
+  int &mode = side_fusion[f32 ? 1 : 0];
+  if (mode >= 0) return mode == 1;
+  const unsigned g = nblk(n, 256);
+  if (d_cost.bytes < (size_t)2 * n * 8) alloc(d_cost, (size_t)2 * n * 8);
+  double *u = d_cost.as<double>(), *v = d_cost.as<double>() + n;
+  hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, u, n, 1.0);
+  hipEvent_t t0, t1;
+  HIPCHK(hipEventCreate(&t0));
+  HIPCHK(hipEventCreate(&t1));
+  float ms[2] = {0, 0};
+  for (int s = 0; s < 2; ++s)
+    for (int rep = 0; rep < 4; ++rep) {
+      if (rep == 1) HIPCHK(hipEventRecord(t0, stream));
+      if (f32)
+        matvec_fused<float>(u, d_outN.as<double>(), v, s == 1);
+      else
+        matvec_fused<double>(u, d_outN.as<double>(), v, s == 1);
+      if (rep == 3) {
+        HIPCHK(hipEventRecord(t1, stream));
+        HIPCHK(hipEventSynchronize(t1));
+        HIPCHK(hipEventElapsedTime(&ms[s], t0, t1));
+      }
+    }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  mode = ms[1] < ms[0] ? 1 : 0;
+  return mode == 1;
 }
 
 // BEMProblem::vmult.  Without Dirichlet DoFs the reference shifts src and dst
@@ -1548,14 +1919,29 @@ void Engine::vmult(const double *src, double *dst, int precision) {
     hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(64), 0, stream, res);
     shift = res;
   }
-  hipLaunchKernelGGL(k_vmult_pre, dim3(g), dim3(256), 0, stream, n, src, d_dir.as<uint8_t>(), shift,
-                     d_sp.as<double>());
-  if (precision == PBFMM_MIXED && have_f32)
-    matvec_combined<float>(d_sp.as<double>(), d_outN.as<double>());
-  else
-    matvec_combined<double>(d_sp.as<double>(), d_outN.as<double>());
-  hipLaunchKernelGGL(k_vmult_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_alpha.as<double>(),
-                     d_dir.as<uint8_t>(), d_sp.as<double>(), dst);
+  // without the shift the operator's input is src itself
+  const double *xin = src;
+  if (shift) {
+    hipLaunchKernelGGL(k_vmult_pre, dim3(g), dim3(256), 0, stream, n, src, d_dir.as<uint8_t>(), shift,
+                       d_sp.as<double>());
+    xin = d_sp.as<double>();
+  }
+  const bool f32 = precision == PBFMM_MIXED && have_f32;
+  if (!time_phases && (int64_t)near.rows.size() == n) {
+    // fewer launches, L2P applying the epilogue (every DoF is one leaf row)
+    const bool side = use_side_fusion(f32);
+    if (f32)
+      matvec_fused<float>(xin, d_outN.as<double>(), dst, side);
+    else
+      matvec_fused<double>(xin, d_outN.as<double>(), dst, side);
+  } else {
+    if (f32)
+      matvec_combined<float>(xin, d_outN.as<double>());
+    else
+      matvec_combined<double>(xin, d_outN.as<double>());
+    hipLaunchKernelGGL(k_vmult_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_alpha.as<double>(),
+                       d_dir.as<uint8_t>(), xin, dst);
+  }
   if (!have_dirichlet) {
     if (neumann_mean) {
       // dst += (sum src / n) 1
@@ -1706,18 +2092,14 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
                       int &it, double &est, double switch_abs = -1.0) {
   const int64_t n = e.n;
   const unsigned g = nblk(n, 256);
-  const int nchunk = (int)nblk(n, kRedChunk);
   double *V = e.d_V.as<double>(), *w = e.d_w.as<double>(), *hd = e.d_h.as<double>();
-  double *part = e.d_red.as<double>();
+  if (!e.d_cgs.p) alloc(e.d_cgs, (size_t)kDotParts * 128 * 8);
+  double *cg = e.d_cgs.as<double>();
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1), h(m + 2), y(m);
   // Hessenberg slots on the device, one per Arnoldi step: [h1 (m+2)][h2 (m+2)][|w|]
   const int S = 2 * (m + 2) + 1;
   double *ybuf = hd + (size_t)m * S;
   std::vector<double> hbuf((size_t)m * S);
-  auto dots = [&](int nv, const double *vec, double *out_dev) {
-    hipLaunchKernelGGL(k_mdot, dim3(nchunk, nv), dim3(kRedThreads), 0, e.stream, n, n, V, vec, part);
-    hipLaunchKernelGGL(k_finish, dim3(nv), dim3(kRedThreads), 0, e.stream, nchunk, part, out_dev);
-  };
   // relaxed mixed precision (switch_abs > 0): FP64 operator while the
   // residual estimate is above switch_abs, the FP32 operator below it, in one
   // Krylov space; the cycle's starting residual always in FP64
@@ -1745,17 +2127,16 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
         e.vmult(V + (int64_t)k * n, vk1, pk);
       }
       ++it;
-      // CGS2: two projections and the norm, all on the device
+      // CGS2: two projections and the norm, all on the device, six launches:
+      // the projections and the scaling finish the dot products they use
       double *slot = hd + (size_t)k * S;
       double *h1 = slot, *h2 = slot + (m + 2), *hn_d = slot + 2 * (m + 2);
-      dots(k + 1, vk1, h1);
-      hipLaunchKernelGGL(k_gemv_sub, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, h1, vk1);
-      dots(k + 1, vk1, h2);
-      hipLaunchKernelGGL(k_gemv_sub, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, h2, vk1);
-      hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, e.stream, n, n, vk1, vk1, part);
-      hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, e.stream, nchunk, part, hn_d);
-      hipLaunchKernelGGL(k_sqrt_inplace, dim3(1), dim3(64), 0, e.stream, hn_d);
-      hipLaunchKernelGGL(k_scale_inv, dim3(g), dim3(256), 0, e.stream, n, vk1, hn_d);
+      for (double *hj : {h1, h2}) {
+        hipLaunchKernelGGL(k_mdot_parts, dim3(kDotParts, k + 1), dim3(kRedThreads), 0, e.stream, n, n, V, vk1, cg);
+        hipLaunchKernelGGL(k_gemv_sub_fin, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, cg, kDotParts, hj, vk1);
+      }
+      hipLaunchKernelGGL(k_mdot_parts, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, vk1, vk1, cg);
+      hipLaunchKernelGGL(k_scale_fin, dim3(g), dim3(256), 0, e.stream, n, vk1, cg, kNormParts, hn_d);
       if (k < k_sync && k + 1 < m && it < maxit) continue;
       // host round trip: the columns k_done..k, Givens rotations, residual estimate
       HIPCHK(hipMemcpyAsync(hbuf.data() + (size_t)k_done * S, hd + (size_t)k_done * S,
@@ -2288,7 +2669,6 @@ void Engine::multi_setup(int R) {
     alloc(mr_le32, ne * 4);
   }
   const int64_t N = (int64_t)R * n;
-  alloc(mr_sp, N * 8);
   alloc(mr_out, N * 8);
   // parts of the lockstep solve: rhs r is entries r * n .. (r + 1) * n
   std::vector<int64_t> off(R + 1), rch{0};
@@ -2355,10 +2735,9 @@ void Engine::matvec_combined_multi(const double *x, double *out, int R) {
 // linear A0 x + mean(x) 1 of each vector (the form GMRES uses).
 void Engine::vmult_multi(const double *src, double *dst, int R, int precision) {
   const unsigned g = nblk(n, 256);
-  double *sp = mr_sp.as<double>(), *o = mr_out.as<double>();
-  for (int r = 0; r < R; ++r)
-    hipLaunchKernelGGL(k_vmult_pre, dim3(g), dim3(256), 0, stream, n, src + (int64_t)r * n, d_dir.as<uint8_t>(),
-                       nullptr, sp + (int64_t)r * n);
+  // no shift here: the operator's input is src itself
+  const double *sp = src;
+  double *o = mr_out.as<double>();
   if (precision == PBFMM_MIXED && have_f32)
     matvec_combined_multi<float>(sp, o, R);
   else

@@ -104,6 +104,12 @@ class BEMFMA:
         space), 2 iterative refinement (pbfmm_set_mixed_scheme)."""
         self._chk(self._L.pbfmm_set_mixed_scheme(self._h, int(scheme)))
 
+    def set_spmv_kernel(self, precision, kernel):
+        """Overrides the autotuned near-field SpMV kernel for one precision: 0
+        workgroup per leaf, 1 / 2 wavefront per leaf with 2 / 4 rows in
+        flight (pbfmm_set_spmv_kernel)."""
+        self._chk(self._L.pbfmm_set_spmv_kernel(self._h, int(precision), int(kernel)))
+
     def set_double_body(self, enable=True, z0=0.0):
         """Double-body operator: every cell also acts through its mirror image
         about z = z0 (BEMProblem::_assemble_system_double_body,
@@ -251,6 +257,12 @@ class BEMProblem:
         """vmult on device vectors (pbfmm_vmult_dev); precision PBFMM_MIXED runs
         the FP32 operator."""
         self.fma._chk(self.fma._L.pbfmm_vmult_dev(self._h, src_ptr, dst_ptr, precision))
+
+    def vmult_dev_multi(self, src_ptr, dst_ptr, nrhs, ld=None, precision=PBFMM_FP64):
+        """vmult_dev on nrhs device columns (column r at ptr + 8*r*ld) in one
+        operator application (pbfmm_vmult_dev_multi)."""
+        ld = self.fma.n if ld is None else int(ld)
+        self.fma._chk(self.fma._L.pbfmm_vmult_dev_multi(self._h, src_ptr, dst_ptr, int(nrhs), ld, precision))
 
     def compute_rhs(self, tmp_rhs):
         dst = np.zeros(self.fma.n)

@@ -50,6 +50,7 @@ SIGNATURES = {
                                      _d]),
     "pbfmm_matvec_dev": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
     "pbfmm_vmult_dev": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_int32]),
+    "pbfmm_vmult_dev_multi": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, C.c_int32]),
     "pbfmm_gmres_dev": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_int32,
                                   C.POINTER(C.c_int32), _d]),
     "pbfmm_gmres_dev_multi": (C.c_int, [_h, C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_int32,
@@ -63,6 +64,7 @@ SIGNATURES = {
     "pbfmm_last_timings": (C.c_int, [_h, _d]),
     "pbfmm_work_counts": (C.c_int, [_h, _i64]),
     "pbfmm_set_mixed_scheme": (C.c_int, [_h, C.c_int32]),
+    "pbfmm_set_spmv_kernel": (C.c_int, [_h, C.c_int32, C.c_int32]),
     "pbfmm_compute_normals": (C.c_int, [_h, _d]),
     "pbfmm_compute_gradients": (C.c_int, [_h, _d, _d, _d]),
     "pbfmm_compute_surface_gradients": (C.c_int, [_h, _d, _d]),

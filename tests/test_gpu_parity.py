@@ -276,3 +276,72 @@ def test_tune_leaf_size():
     print(f"\n  leaf sizes: {times} -> {best}")
     assert set(times) == {12, 20, 32} and best in times and all(t > 0 for t in times.values())
     assert times[best] == min(times.values())
+
+
+@pytest.mark.parametrize("name", ["sphere", "wigley"])
+def test_vmult_dev_multi(name):
+    """pbfmm_vmult_dev_multi on three columns with a padded leading dimension
+    equals pbfmm_vmult_dev column by column: FP64 to 1e-13, FP32 to 1e-6
+    (translation atomics sum in another order).  The sphere (mixed data)
+    takes the lockstep path; the Wigley hull (pure Neumann) keeps vmult's
+    per-column norm shifts."""
+    from pibem_amd import hip
+    pr = M.config_sphere(16, 4) if name == "sphere" else M.config_wigley(40, 10, 6, 4)
+    f = P.setup(pr, P.PBFMM_MIXED)
+    bp = P.BEMProblem(f)
+    bp.compute_alpha()
+    if name == "wigley":
+        bp.set_constraints(M.constraints(pr))
+    n, R, ld = pr.mesh.n_dofs, 3, pr.mesh.n_dofs + 5
+    X = np.random.default_rng(7).uniform(-1, 1, (R, ld))
+    ds, dd = hip.DeviceArray.from_numpy(X.ravel()), hip.DeviceArray(R * ld)
+    col_s, col_d = hip.DeviceArray(n), hip.DeviceArray(n)
+    for prec, bar in ((P.PBFMM_FP64, 1e-13), (P.PBFMM_MIXED, 1e-6)):
+        dd.upload(np.zeros(R * ld))
+        bp.vmult_dev_multi(ds.value, dd.value, R, ld, prec)
+        hip.synchronize()  # the handle's stream
+        Y = dd.numpy().reshape(R, ld)
+        for r in range(R):
+            col_s.upload(X[r, :n])
+            bp.vmult_dev(col_s.value, col_d.value, prec)
+            hip.synchronize()
+            e = rel(Y[r, :n], col_d.numpy())
+            print(f"\n  {name} precision={prec} column {r}: {e:.2e}", end="")
+            assert e <= bar
+        assert not Y[:, n:].any()  # padding untouched
+    f.close()
+
+
+@pytest.mark.parametrize("name", ["sphere_p4", "sphere_p6", "spheroid_p8"])
+def test_fused_operator_pipeline(name):
+    """The operator's fused launches (SpMV shares beside the M2M levels, M2P
+    shares beside the L2L levels) against the phase-timed sequence, which
+    runs every phase in its own launch, for each of the three SpMV kernels:
+    FP64 within 1e-14, FP32 within 1e-6 (atomics sum in another order)."""
+    from pibem_amd import hip
+    pr = {"sphere_p4": lambda: M.config_sphere(16, 4), "sphere_p6": lambda: M.config_sphere(24, 6),
+          "spheroid_p8": lambda: M.config_spheroid(24, 8)}[name]()
+    f = P.setup(pr, P.PBFMM_MIXED)
+    bp = P.BEMProblem(f)
+    bp.compute_alpha()
+    n = pr.mesh.n_dofs
+    src = hip.DeviceArray.from_numpy(np.random.default_rng(11).uniform(-1, 1, n))
+    dst = hip.DeviceArray(n)
+    for prec, bar in ((P.PBFMM_FP64, 1e-14), (P.PBFMM_MIXED, 1e-6)):
+        ref = None
+        for kernel in (0, 1, 2):
+            f.set_spmv_kernel(prec, kernel)
+            f.set_phase_timing(True)
+            bp.vmult_dev(src.value, dst.value, prec)
+            hip.synchronize()
+            y0 = dst.numpy()
+            f.set_phase_timing(False)
+            bp.vmult_dev(src.value, dst.value, prec)
+            hip.synchronize()
+            y1 = dst.numpy()
+            ref = y0 if ref is None else ref
+            e, e0 = rel(y1, y0), rel(y0, ref)
+            print(f"\n  {name} N={n} precision={prec} kernel {kernel}: fused vs phases {e:.2e}, vs kernel 0 {e0:.2e}",
+                  end="")
+            assert e <= bar and e0 <= bar
+    f.close()
