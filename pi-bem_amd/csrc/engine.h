@@ -80,6 +80,12 @@ struct Engine {
   // operator vectors
   DevBuf d_alpha, d_sp, d_sd, d_outN, d_outD, d_in, d_out, d_red;
   DevBuf d_cgs;  // partial sums of the Arnoldi step's dots (kDotParts per column)
+  int max_leaf_rows = 1 << 30;  // largest leaf block of the preconditioner
+  // coarsest levels whose expansions are read (MEs: M2L and M2P sources; LEs:
+  // M2L targets); M2M into coarser levels and L2L out of them are skipped in
+  // the operator pipeline.  -1 until computed from the plan.
+  int me_lo = -1, le_lo = -1;
+  void consumer_levels();
   DevBuf d_c_ptr, d_c_idx, d_c_val, d_c_inh, d_c_is;
   bool have_constraints = false;
   std::vector<int32_t> hc_ptr, hc_idx;  // host copies of the constraint rows

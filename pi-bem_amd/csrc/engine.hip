@@ -1085,6 +1085,27 @@ __global__ void __launch_bounds__(256) k_bjacobi(int nrows, const int *__restric
   for (int c = 0; c < R; ++c) acc += P[(int64_t)c * R] * v[rows[r0 + c]];
   z[rows[g]] = acc;
 }
+// The same with one wavefront per leaf (leaves of at most 64 rows): lane r
+// gathers v at row r once, and every lane then reads its block row with R
+// independent coalesced loads, the inputs broadcast from their lanes, so the
+// gathers are no longer on each step of the chain.  Same summation order as
+// k_bjacobi.
+__global__ void __launch_bounds__(256) k_bjacobi_wave(int nleaf, const int *__restrict__ row_beg,
+                                                      const int *__restrict__ rows,
+                                                      const int64_t *__restrict__ blk_beg,
+                                                      const double *__restrict__ pinv, const double *__restrict__ v,
+                                                      double *__restrict__ z) {
+  const int leaf = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+  if (leaf >= nleaf) return;
+  const int lane = threadIdx.x & 63;
+  const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0;
+  const double xv = lane < R ? v[rows[r0 + lane]] : 0.0;
+  const double *P = pinv + blk_beg[leaf] + (lane < R ? lane : 0);  // column-major block
+  double acc = 0.0;
+#pragma unroll 8
+  for (int c = 0; c < R; ++c) acc += (lane < R ? P[(int64_t)c * R] : 0.0) * read_lane(xv, c);
+  if (lane < R) z[rows[r0 + lane]] = acc;
+}
 
 // ---------------------------------------------------------------------------
 // GMRES building blocks (FP64).  Dots of w against nv basis vectors: blocks
@@ -1440,6 +1461,7 @@ static void upload_chunks(ChunkDev &d, const ChunkList &c) {
 }
 
 void Engine::upload_far(bool want_f32) {
+  me_lo = le_lo = -1;  // recomputed from this plan on first use
   up(d_p2m_leaf, far.p2m_leaf);
   up(d_p2m_col_beg, far.p2m_col_beg);
   up(d_p2m_col_dof, far.p2m_col_dof);
@@ -1758,9 +1780,26 @@ void Engine::matvec_combined(const double *x, double *out) {
   collect_phase_times();
 }
 
+// The coarsest tree levels whose expansions anything reads.  MEs at level k
+// are read by M2L (pairs of level k) and M2P (source blocks of level k); LEs
+// at level k are nonzero only from the coarsest M2L level down.  M2M into a
+// level above me_lo computes MEs nobody reads, and L2L out of a level above
+// le_lo adds zeros (levels 0 and 1 on closed surfaces: their blocks all
+// touch).
+void Engine::consumer_levels() {
+  me_lo = le_lo = L + 1;
+  for (int l = 1; l <= L; ++l)
+    if (lvl<1>(far.m2l, l + 1) > lvl<1>(far.m2l, l)) {
+      me_lo = std::min(me_lo, l);
+      le_lo = std::min(le_lo, l);
+    }
+  for (int b : far.m2p_src) me_lo = std::min(me_lo, (int)tree.level[b]);
+}
+
 // The single-value pipeline in fewer launches: ME and LE cleared in one
-// launch; P2M; M2M levels L..1; M2L; L2L levels 2..L; L2P, which also
-// applies vmult's epilogue into post_dst (every DoF is a row of one leaf).
+// launch; P2M; M2M levels L..me_lo + 1; M2L; L2L levels le_lo + 1..L (at
+// least 2); L2P, which also applies vmult's epilogue into post_dst (every DoF
+// is a row of one leaf).
 // With side = true the deepest non-empty M2M level also runs the whole SpMV
 // and the deepest non-empty L2L level all M2P evaluations
 // (k_translate_side); otherwise they have launches of their own.  The same
@@ -1781,9 +1820,10 @@ void Engine::matvec_fused(const double *x, double *out, double *post_dst, bool s
   const int nl = near.nleaf;
   SideArgs<T> sd{};
   sd.near = near_args<T, 1>(*this, x, nullptr, out, nullptr);
+  if (me_lo < 0) consumer_levels();
   bool near_done = nl == 0 || !side;
   if (!side && nl > 0) launch_spmv<T, 1>(var, x, nullptr, out, nullptr);
-  for (int l = L; l >= 1; --l) {
+  for (int l = L; l > me_lo; --l) {  // M2M from level l into l - 1
     const int c0 = lvl<1>(far.m2m, l), c1 = lvl<1>(far.m2m, l + 1);
     if (c1 <= c0) continue;
     if (near_done) {
@@ -1816,10 +1856,11 @@ void Engine::matvec_fused(const double *x, double *out, double *post_dst, bool s
   sd.kp = kp;
   sd.ME = me;
   sd.out = out;
+  const int l2l_lo = std::max(2, le_lo + 1);  // L2L from level l - 1 into l
   int deepest = 0;  // deepest L2L level with chunks: it also runs M2P
-  for (int l = 2; side && l <= L; ++l)
+  for (int l = l2l_lo; side && l <= L; ++l)
     if (lvl<1>(far.l2l, l + 1) > lvl<1>(far.l2l, l)) deepest = l;
-  for (int l = 2; l <= L; ++l) {
+  for (int l = l2l_lo; l <= L; ++l) {
     const int c0 = lvl<1>(far.l2l, l), c1 = lvl<1>(far.l2l, l + 1);
     if (c1 <= c0) continue;
     if (l != deepest) {
@@ -2025,9 +2066,11 @@ static bool invert_block(std::vector<double> &A, int R) {
 void Engine::build_preconditioner() {
   const int nl = near.nleaf;
   pinv_beg.assign(nl + 1, 0);
+  max_leaf_rows = 0;
   for (int l = 0; l < nl; ++l) {
     const int64_t R = near.row_beg[l + 1] - near.row_beg[l];
     pinv_beg[l + 1] = pinv_beg[l] + R * R;
+    max_leaf_rows = std::max(max_leaf_rows, (int)R);
   }
   up(d_pinv_beg, pinv_beg);
   alloc(d_pinv, (size_t)std::max<int64_t>(pinv_beg[nl], 1) * 8);
@@ -2080,6 +2123,11 @@ void Engine::build_preconditioner() {
 
 void Engine::apply_prec(const double *v, double *z) {
   const int nrows = (int)near.rows.size();
+  if (max_leaf_rows <= 64) {
+    hipLaunchKernelGGL(k_bjacobi_wave, dim3(nblk(near.nleaf, 4)), dim3(256), 0, stream, near.nleaf,
+                       d_row_beg.as<int>(), d_rows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z);
+    return;
+  }
   hipLaunchKernelGGL(k_bjacobi, dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_row_leaf.as<int>(),
                      d_row_beg.as<int>(), d_rows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z);
 }
@@ -2707,7 +2755,8 @@ void Engine::matvec_combined_multi(const double *x, double *out, int R) {
     hipLaunchKernelGGL((k_p2m<T, 1>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
                        d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp,
                        x + (int64_t)r * n, nullptr, ME + r * sb);
-  for (int l = L; l >= 1; --l)
+  if (me_lo < 0) consumer_levels();
+  for (int l = L; l > me_lo; --l)
     launch_translate<T, 0, 1>(*this, stream, mr_m2m, lvl<1>(mr_m2m_h, l), lvl<1>(mr_m2m_h, l + 1),
                               F.m2m.template as<T>(), ME, ME);
   // near field, then the far field accumulated onto it
@@ -2721,7 +2770,7 @@ void Engine::matvec_combined_multi(const double *x, double *out, int R) {
     hipLaunchKernelGGL((k_m2p<T, 1>), dim3((unsigned)nblk(nitems, 256)), dim3(256), 0, stream, nitems,
                        d_m2p_node.as<int>(), d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
                        d_delta.as<double>(), p, kt, kp, ME + r * sb, out + (int64_t)r * n, nullptr);
-  for (int l = 2; l <= L; ++l)
+  for (int l = std::max(2, le_lo + 1); l <= L; ++l)
     launch_translate<T, 2, 1>(*this, stream, mr_l2l, lvl<1>(mr_l2l_h, l), lvl<1>(mr_l2l_h, l + 1),
                               F.l2l.template as<T>(), LE, LE);
   for (int r = 0; r < R && nrows > 0; ++r)
