@@ -252,13 +252,15 @@ def measure_config(config, steps, warmup):
     log("setup: " + " ".join(f"{k}={v:.2f}" for k, v in setup.items()))
     wc = f.work_counts()
     solves = time_solves(f, bp, pr, stream, steps, warmup)
+    tuning = f.tuning_state()
+    log("tuned choices: " + " ".join(f"{k}={v}" for k, v in tuning.items()))
     op_rate, op_phases = time_operator(f, bp, n, stream)
     mv, phases, rel = time_matvecs(f, n, stream)
     roof = {prec: {ph: roofline_of(ph, work_model(wc, 4 if prec == "fp32" else 8, 1), op_phases[prec][ph], prec,
                                    pr.p, 1) for ph in PHASES} for prec in op_phases}
     roof2 = {prec: {ph: roofline_of(ph, work_model(wc, 4 if prec == "fp32" else 8, 2), phases[prec][ph], prec,
                                     pr.p, 2) for ph in PHASES} for prec in phases}
-    return dict(pr=pr, f=f, stream=stream, wc=wc, setup=setup, solves=solves, operator=op_rate,
+    return dict(pr=pr, f=f, stream=stream, wc=wc, setup=setup, solves=solves, tuning=tuning, operator=op_rate,
                 operator_phases=op_phases, matvec=mv, phases=phases, fp32_vs_fp64_matvec=rel, roofline=roof,
                 roofline_matvec=roof2)
 
@@ -534,6 +536,7 @@ def main():
                                for ph_, v in sp[k].items()} for k in sp},
         "setup_s": c2["setup"],
         "work_counts": wc,
+        "tuned_choices": c2["tuning"],
         "roofline": hl,
     }
     cpu = None
@@ -566,6 +569,7 @@ def main():
                                    for ph_, v in r3[k].items()} for k in r3},
             "setup_s": c3["setup"],
             "work_counts": c3["wc"],
+            "tuned_choices": c3["tuning"],
         }
         del c3
     if 5 in extra:

@@ -861,6 +861,29 @@ int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel) {
   });
 }
 
+int pbfmm_set_side_fusion(pbfmm_handle *h, int32_t precision, int32_t mode) {
+  if (!h) return PBFMM_ERR_ARG;
+  if (mode < -1 || mode > 2) return fail(h, PBFMM_ERR_ARG, "set_side_fusion: mode must be -1, 0, 1 or 2");
+  if (precision != PBFMM_FP64 && precision != PBFMM_MIXED)
+    return fail(h, PBFMM_ERR_ARG, "set_side_fusion: unknown precision");
+  h->eng.side_fusion[precision == PBFMM_MIXED ? 1 : 0] = mode;
+  return PBFMM_OK;
+}
+
+int pbfmm_tuning_state(pbfmm_handle *h, int32_t *state) {
+  if (!h || !state) return PBFMM_ERR_ARG;
+  const Engine &e = h->eng;
+  const bool timed = h->stage >= 3;
+  state[0] = timed ? e.spmv_variant[0][0] : -1;
+  state[1] = timed && e.have_f32 ? e.spmv_variant[1][0] : -1;
+  state[2] = e.side_fusion[0];
+  state[3] = e.side_fusion[1];
+  state[4] = e.mixed_scheme != 0 ? e.mixed_scheme
+                                 : (e.op_cost_ratio > 0 ? (e.op_cost_ratio < kRelaxCostRatio ? 1 : 2) : -1);
+  state[5] = e.op_cost_ratio > 0 ? (int32_t)(e.op_cost_ratio * 1000.0 + 0.5) : -1;
+  return PBFMM_OK;
+}
+
 int pbfmm_projection_iterations(pbfmm_handle *h, int32_t *iters) {
   if (!h || !iters) return PBFMM_ERR_ARG;
   *iters = h->eng.ps.last_iters;

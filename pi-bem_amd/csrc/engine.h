@@ -45,6 +45,14 @@ struct FarDev {
   DevBuf opvals, opunit;
 };
 
+// Mixed precision: the relaxed scheme spends its first steps (about a
+// third) on the FP64 operator and saves the restarts of refinement;
+// refinement runs almost all steps on the FP32 operator.  Measured: relaxed
+// wins while the FP64 operator costs less than ~1.5x the FP32 one (24,578
+// DoFs, order 6: 1.23x, 10.5 ms against 12.2 ms) and loses above it (393,218
+// DoFs, order 8: 1.56x, 160.8 ms against 156.6 ms).
+constexpr double kRelaxCostRatio = 1.45;
+
 struct Engine {
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -124,14 +132,15 @@ struct Engine {
   void matvec_combined(const double *x, double *out);
   // matvec_combined in fewer launches (used by vmult while phase timing is
   // off): ME and LE cleared together, L2P applying vmult's epilogue into
-  // post_dst (dst = out + alpha x on Neumann DoFs) and, with side = true, the
-  // SpMV and M2P run inside M2M and L2L level launches
+  // post_dst (dst = out + alpha x on Neumann DoFs); side 1 runs the SpMV
+  // inside an M2M level launch and M2P inside an L2L level launch, side 2
+  // the SpMV inside an M2M level launch and M2P inside the M2L launch
   template <typename T>
-  void matvec_fused(const double *x, double *out, double *post_dst, bool side);
-  // side fusion per precision ([0] FP64, [1] FP32): -1 until timed on first
-  // use, then 1 if the fused launches measured faster
+  void matvec_fused(const double *x, double *out, double *post_dst, int side);
+  // side fusion mode per precision ([0] FP64, [1] FP32): -1 until timed on
+  // first use, then the fastest of modes 0, 1, 2
   int side_fusion[2] = {-1, -1};
-  bool use_side_fusion(bool f32);
+  int use_side_fusion(bool f32);
   // near-field SpMV kernel per precision ([0] FP64, [1] FP32) and storage
   // (pairs, single values): 0 workgroup per leaf with an LDS copy of the
   // columns, 1 wavefront per leaf with 2 rows in flight, 2 with 4 rows;

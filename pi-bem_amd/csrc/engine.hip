@@ -1800,13 +1800,14 @@ void Engine::consumer_levels() {
 // launch; P2M; M2M levels L..me_lo + 1; M2L; L2L levels le_lo + 1..L (at
 // least 2); L2P, which also applies vmult's epilogue into post_dst (every DoF
 // is a row of one leaf).
-// With side = true the deepest non-empty M2M level also runs the whole SpMV
-// and the deepest non-empty L2L level all M2P evaluations
-// (k_translate_side); otherwise they have launches of their own.  The same
+// With side 1 or 2 the deepest non-empty M2M level also runs the whole SpMV,
+// and all M2P evaluations run beside the deepest non-empty L2L level (1) or
+// beside M2L (2) (k_translate_side); with side 0 they have launches of their
+// own.  The same
 // arithmetic as generate + products + k_vmult_post.  Phase timing runs that
 // unfused sequence instead, so that each phase can be timed.
 template <typename T>
-void Engine::matvec_fused(const double *x, double *out, double *post_dst, bool side) {
+void Engine::matvec_fused(const double *x, double *out, double *post_dst, int side) {
   FarDev<T> &F = far_of<T>(*this);
   const int nleaf = (int)far.p2m_leaf.size();
   const int64_t ne = (int64_t)tree.nb * kp;
@@ -1843,7 +1844,6 @@ void Engine::matvec_fused(const double *x, double *out, double *post_dst, bool s
     near_done = true;
   }
   if (!near_done) launch_spmv<T, 1>(var, x, nullptr, out, nullptr);
-  launch_translate<T, 1, 1>(*this, stream, m2l_cl, 0, nchunks<1>(far.m2l), F.m2l.template as<T>(), me, le);
   const int64_t ni = (int64_t)far.m2p_item_node.size();
   sd.count = 0;
   sd.item_node = d_m2p_node.as<int>();
@@ -1856,9 +1856,18 @@ void Engine::matvec_fused(const double *x, double *out, double *post_dst, bool s
   sd.kp = kp;
   sd.ME = me;
   sd.out = out;
+  sd.m2p_lo = 0;
+  sd.m2p_hi = ni;
+  bool m2p_done = ni == 0;
+  if (side == 2 && !m2p_done) {  // M2P beside M2L
+    launch_translate_side<T, 1, 3>(*this, m2l_cl, 0, nchunks<1>(far.m2l), F.m2l.template as<T>(), me, le, sd);
+    m2p_done = true;
+  } else {
+    launch_translate<T, 1, 1>(*this, stream, m2l_cl, 0, nchunks<1>(far.m2l), F.m2l.template as<T>(), me, le);
+  }
   const int l2l_lo = std::max(2, le_lo + 1);  // L2L from level l - 1 into l
   int deepest = 0;  // deepest L2L level with chunks: it also runs M2P
-  for (int l = l2l_lo; side && l <= L; ++l)
+  for (int l = l2l_lo; side == 1 && !m2p_done && l <= L; ++l)
     if (lvl<1>(far.l2l, l + 1) > lvl<1>(far.l2l, l)) deepest = l;
   for (int l = l2l_lo; l <= L; ++l) {
     const int c0 = lvl<1>(far.l2l, l), c1 = lvl<1>(far.l2l, l + 1);
@@ -1867,11 +1876,10 @@ void Engine::matvec_fused(const double *x, double *out, double *post_dst, bool s
       launch_translate<T, 2, 1>(*this, stream, l2l_cl, c0, c1, F.l2l.template as<T>(), le, le);
       continue;
     }
-    sd.m2p_lo = 0;
-    sd.m2p_hi = ni;
     launch_translate_side<T, 2, 3>(*this, l2l_cl, c0, c1, F.l2l.template as<T>(), le, le, sd);
+    m2p_done = true;
   }
-  if (deepest == 0 && ni > 0)
+  if (!m2p_done)
     hipLaunchKernelGGL((k_m2p<T, 1>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, stream, ni, d_m2p_node.as<int>(),
                        d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt,
                        kp, me, out, nullptr);
@@ -1909,12 +1917,14 @@ double Engine::norm2(const double *v) {
 
 // Side fusion pays while the M2M / L2L level launches are latency-bound and
 // the SpMV and M2P are short beside them (tens of thousands of DoFs); on
-// large trees the fused workgroups only crowd each other.  Timed once per
-// precision, on the first operator application: three applications each
-// way on scratch vectors, after one warm-up.
-bool Engine::use_side_fusion(bool f32) {
+// large trees the fused workgroups only crowd each other.  Modes: 0 none,
+// 1 SpMV beside an M2M level and M2P beside an L2L level, 2 SpMV beside an
+// M2M level and M2P beside M2L.  Timed once per precision, on the first
+// operator application: three applications each way on scratch vectors,
+// after one warm-up.
+int Engine::use_side_fusion(bool f32) {
   int &mode = side_fusion[f32 ? 1 : 0];
-  if (mode >= 0) return mode == 1;
+  if (mode >= 0) return mode;
   const unsigned g = nblk(n, 256);
   if (d_cost.bytes < (size_t)2 * n * 8) alloc(d_cost, (size_t)2 * n * 8);
   double *u = d_cost.as<double>(), *v = d_cost.as<double>() + n;
@@ -1922,14 +1932,14 @@ bool Engine::use_side_fusion(bool f32) {
   hipEvent_t t0, t1;
   HIPCHK(hipEventCreate(&t0));
   HIPCHK(hipEventCreate(&t1));
-  float ms[2] = {0, 0};
-  for (int s = 0; s < 2; ++s)
+  float ms[3] = {0, 0, 0};
+  for (int s = 0; s < 3; ++s)
     for (int rep = 0; rep < 4; ++rep) {
       if (rep == 1) HIPCHK(hipEventRecord(t0, stream));
       if (f32)
-        matvec_fused<float>(u, d_outN.as<double>(), v, s == 1);
+        matvec_fused<float>(u, d_outN.as<double>(), v, s);
       else
-        matvec_fused<double>(u, d_outN.as<double>(), v, s == 1);
+        matvec_fused<double>(u, d_outN.as<double>(), v, s);
       if (rep == 3) {
         HIPCHK(hipEventRecord(t1, stream));
         HIPCHK(hipEventSynchronize(t1));
@@ -1938,8 +1948,10 @@ bool Engine::use_side_fusion(bool f32) {
     }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
-  mode = ms[1] < ms[0] ? 1 : 0;
-  return mode == 1;
+  mode = 0;
+  for (int s = 1; s < 3; ++s)
+    if (ms[s] < ms[mode]) mode = s;
+  return mode;
 }
 
 // BEMProblem::vmult.  Without Dirichlet DoFs the reference shifts src and dst
@@ -1970,7 +1982,7 @@ void Engine::vmult(const double *src, double *dst, int precision) {
   const bool f32 = precision == PBFMM_MIXED && have_f32;
   if (!time_phases && (int64_t)near.rows.size() == n) {
     // fewer launches, L2P applying the epilogue (every DoF is one leaf row)
-    const bool side = use_side_fusion(f32);
+    const int side = use_side_fusion(f32);
     if (f32)
       matvec_fused<float>(xin, d_outN.as<double>(), dst, side);
     else
@@ -2264,13 +2276,6 @@ constexpr double kInnerReduction = 1e-5;
 // ~tol ||b|| / ||r_k||, so FP32 steps are safe once ||r_k|| <= tol ||b|| / eps,
 // taken here with a factor-5 margin on eps = 1e-6
 constexpr double kRelaxSwitch = 2e5;
-// The relaxed scheme spends its first steps (about a third) on the FP64
-// operator and saves the restarts of refinement; refinement runs almost all
-// steps on the FP32 operator.  Measured: relaxed wins while the FP64 operator
-// costs less than ~1.5x the FP32 one (24,578 DoFs, order 6: 1.23x, 10.5 ms
-// against 12.2 ms) and loses above it (393,218 DoFs, order 8: 1.56x, 160.8
-// ms against 156.6 ms).
-constexpr double kRelaxCostRatio = 1.45;
 
 bool Engine::mixed_relaxed() {
   if (mixed_scheme != 0) return mixed_scheme == 1;

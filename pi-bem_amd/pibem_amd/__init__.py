@@ -110,6 +110,22 @@ class BEMFMA:
         flight (pbfmm_set_spmv_kernel)."""
         self._chk(self._L.pbfmm_set_spmv_kernel(self._h, int(precision), int(kernel)))
 
+    def set_side_fusion(self, precision, mode):
+        """Overrides the timed side-fusion mode of the operator's launches:
+        0 none, 1 SpMV beside an M2M level and M2P beside an L2L level, 2 M2P
+        beside M2L instead; -1 times them again (pbfmm_set_side_fusion)."""
+        self._chk(self._L.pbfmm_set_side_fusion(self._h, int(precision), int(mode)))
+
+    def tuning_state(self):
+        """Choices the handle's timings made so far (-1 = not timed yet):
+        SpMV kernels, side fusion modes, mixed scheme, FP64/FP32 operator
+        cost ratio (pbfmm_tuning_state)."""
+        st = np.zeros(6, np.int32)
+        self._chk(self._L.pbfmm_tuning_state(self._h, _p(st, _i32)))
+        return dict(spmv_kernel_fp64=int(st[0]), spmv_kernel_fp32=int(st[1]), side_fusion_fp64=int(st[2]),
+                    side_fusion_fp32=int(st[3]), mixed_scheme=int(st[4]),
+                    op_cost_ratio=(st[5] / 1000.0 if st[5] >= 0 else None))
+
     def set_double_body(self, enable=True, z0=0.0):
         """Double-body operator: every cell also acts through its mirror image
         about z = z0 (BEMProblem::_assemble_system_double_body,

@@ -65,6 +65,8 @@ SIGNATURES = {
     "pbfmm_work_counts": (C.c_int, [_h, _i64]),
     "pbfmm_set_mixed_scheme": (C.c_int, [_h, C.c_int32]),
     "pbfmm_set_spmv_kernel": (C.c_int, [_h, C.c_int32, C.c_int32]),
+    "pbfmm_tuning_state": (C.c_int, [_h, _i32]),
+    "pbfmm_set_side_fusion": (C.c_int, [_h, C.c_int32, C.c_int32]),
     "pbfmm_compute_normals": (C.c_int, [_h, _d]),
     "pbfmm_compute_gradients": (C.c_int, [_h, _d, _d, _d]),
     "pbfmm_compute_surface_gradients": (C.c_int, [_h, _d, _d]),

@@ -314,10 +314,12 @@ def test_vmult_dev_multi(name):
 
 @pytest.mark.parametrize("name", ["sphere_p4", "sphere_p6", "spheroid_p8"])
 def test_fused_operator_pipeline(name):
-    """The operator's fused launches (SpMV shares beside the M2M levels, M2P
-    shares beside the L2L levels) against the phase-timed sequence, which
-    runs every phase in its own launch, for each of the three SpMV kernels:
-    FP64 within 1e-14, FP32 within 1e-6 (atomics sum in another order)."""
+    """The operator in fewer launches (SpMV beside an M2M level, M2P beside an
+    L2L level or beside M2L, L2P applying vmult's epilogue, levels without
+    readers skipped) against the phase-timed sequence, which runs every phase
+    in its own launch, for each of the three SpMV kernels and each side-fusion
+    mode: FP64 within 1e-14, FP32 within 1e-6 (atomics sum in another
+    order)."""
     from pibem_amd import hip
     pr = {"sphere_p4": lambda: M.config_sphere(16, 4), "sphere_p6": lambda: M.config_sphere(24, 6),
           "spheroid_p8": lambda: M.config_spheroid(24, 8)}[name]()
@@ -336,12 +338,19 @@ def test_fused_operator_pipeline(name):
             hip.synchronize()
             y0 = dst.numpy()
             f.set_phase_timing(False)
-            bp.vmult_dev(src.value, dst.value, prec)
-            hip.synchronize()
-            y1 = dst.numpy()
             ref = y0 if ref is None else ref
-            e, e0 = rel(y1, y0), rel(y0, ref)
-            print(f"\n  {name} N={n} precision={prec} kernel {kernel}: fused vs phases {e:.2e}, vs kernel 0 {e0:.2e}",
-                  end="")
-            assert e <= bar and e0 <= bar
+            for mode in (0, 1, 2):
+                f.set_side_fusion(prec, mode)
+                bp.vmult_dev(src.value, dst.value, prec)
+                hip.synchronize()
+                y1 = dst.numpy()
+                e, e0 = rel(y1, y0), rel(y0, ref)
+                print(f"\n  {name} N={n} precision={prec} kernel {kernel} side {mode}: vs phases {e:.2e}, "
+                      f"vs kernel 0 {e0:.2e}", end="")
+                assert e <= bar and e0 <= bar
+        f.set_side_fusion(prec, -1)
+        bp.vmult_dev(src.value, dst.value, prec)
+        hip.synchronize()
+        assert rel(dst.numpy(), ref) <= bar
+        assert f.tuning_state()["side_fusion_fp32" if prec == P.PBFMM_MIXED else "side_fusion_fp64"] in (0, 1, 2)
     f.close()
