@@ -255,9 +255,9 @@ int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel);
 
 /* Overrides the timed side-fusion mode of the operator's launches for one
  * precision (0 none, 1 SpMV beside an M2M level and M2P beside an L2L level,
- * 2 SpMV beside an M2M level and M2P beside M2L; -1 times them again on the
- * next application).  For tests and tuning.  No counterpart in the
- * reference. */
+ * 2 SpMV beside an M2M level and M2P beside M2L, 3 both beside M2L; -1 times
+ * them again on the next application).  For tests and tuning.  No
+ * counterpart in the reference. */
 int pbfmm_set_side_fusion(pbfmm_handle *h, int32_t precision, int32_t mode);
 
 /* The choices this handle's timings made so far (-1 = not timed yet):
@@ -265,7 +265,7 @@ int pbfmm_set_side_fusion(pbfmm_handle *h, int32_t precision, int32_t mode);
  *         (as pbfmm_set_spmv_kernel)
  * [2] [3] side fusion of the operator's launches, FP64 / FP32: 0 none,
  *         1 SpMV beside an M2M level and M2P beside an L2L level, 2 SpMV
- *         beside an M2M level and M2P beside M2L
+ *         beside an M2M level and M2P beside M2L, 3 both beside M2L
  * [4]     mixed-precision scheme: 1 relaxed, 2 iterative refinement
  * [5]     timed FP64 / FP32 operator cost ratio x 1000
  * No counterpart in the reference. */

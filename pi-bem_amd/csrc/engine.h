@@ -130,15 +130,16 @@ struct Engine {
   // combined expansion per block and single-value near field / unit multipoles
   template <typename T>
   void matvec_combined(const double *x, double *out);
-  // matvec_combined in fewer launches (used by vmult while phase timing is
-  // off): ME and LE cleared together, L2P applying vmult's epilogue into
-  // post_dst (dst = out + alpha x on Neumann DoFs); side 1 runs the SpMV
-  // inside an M2M level launch and M2P inside an L2L level launch, side 2
-  // the SpMV inside an M2M level launch and M2P inside the M2L launch
+  // BEMProblem::vmult's FMM and epilogue in fewer launches (used by vmult
+  // while phase timing is off): dst = out + m2p + L2P + alpha x, with the
+  // SpMV and M2P run beside translation launches per side mode (0 none,
+  // 1 SpMV beside an M2M level and M2P beside an L2L level, 2 SpMV beside an
+  // M2M level and M2P beside M2L, 3 both beside M2L)
   template <typename T>
-  void matvec_fused(const double *x, double *out, double *post_dst, int side);
+  void matvec_fused(const double *x, double *out, double *dst, int side);
   // side fusion mode per precision ([0] FP64, [1] FP32): -1 until timed on
-  // first use, then the fastest of modes 0, 1, 2
+  // first use, then the fastest of the kSideModes modes
+  static constexpr int kSideModes = 4;
   int side_fusion[2] = {-1, -1};
   int use_side_fusion(bool f32);
   // near-field SpMV kernel per precision ([0] FP64, [1] FP32) and storage

@@ -889,15 +889,17 @@ __global__ void __launch_bounds__(256) k_l2p(int nrows, const int *__restrict__ 
 }
 
 // L2P of the operator pipeline with BEMProblem::vmult's epilogue
-// (k_vmult_post) folded in: dst = out + L2P + alpha x on Neumann DoFs.
+// (k_vmult_post) folded in: dst = out + m2p + L2P + alpha x on Neumann DoFs
+// (out the near field, m2p the M2P sums).
 template <typename T>
 __global__ void __launch_bounds__(256) k_l2p_post(int nrows, const int *__restrict__ rows,
                                                   const int *__restrict__ row_leaf, const int *__restrict__ leaf_block,
                                                   const double *__restrict__ xyz, const double *__restrict__ pmin,
                                                   const double *__restrict__ delta, int p, int kt, int kp,
                                                   const T *__restrict__ LE, const double *__restrict__ out,
-                                                  const double *__restrict__ alpha, const uint8_t *__restrict__ dir,
-                                                  const double *__restrict__ x, double *__restrict__ dst) {
+                                                  const double *__restrict__ m2p, const double *__restrict__ alpha,
+                                                  const uint8_t *__restrict__ dir, const double *__restrict__ x,
+                                                  double *__restrict__ dst) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   const int node = rows[r];
@@ -908,7 +910,7 @@ __global__ void __launch_bounds__(256) k_l2p_post(int nrows, const int *__restri
   const T d2 = (T)(xyz[3 * node + 2] - (pmin[3 * b + 2] + hd));
   T v1, v2;
   eval2<T, true, 1>(LE + (int64_t)b * kp, p, kt, kp, d0, d1, d2, (T)delta[b], v1, v2);
-  const double nd = out[node] + (double)v1;
+  const double nd = (out[node] + m2p[node]) + (double)v1;
   dst[node] = nd + (dir[node] ? 0.0 : alpha[node] * x[node]);
 }
 
@@ -918,62 +920,81 @@ __global__ void k_fill(T *__restrict__ a, int64_t n, T v) {
     a[i] = v;
 }
 
+// Clears the operator pipeline's expansions (ME and LE, n_e entries each)
+// and its M2P accumulator (n_a doubles) in one launch.
 template <typename T>
-__global__ void k_fill2(T *__restrict__ a, T *__restrict__ b, int64_t n, T v) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 2 * n; i += (int64_t)gridDim.x * blockDim.x)
-    (i < n ? a[i] : b[i - n]) = v;
+__global__ void k_clear_operator(T *__restrict__ me, T *__restrict__ le, int64_t ne, double *__restrict__ acc,
+                                 int64_t na) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 2 * ne + na;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < ne)
+      me[i] = T(0);
+    else if (i < 2 * ne)
+      le[i - ne] = T(0);
+    else
+      acc[i - 2 * ne] = 0.0;
+  }
 }
 
-// Work that shares a launch with one level of M2M or L2L in the operator
-// (single-value) pipeline: the near-field SpMV of leaves [lo, lo + count), or
-// the M2P evaluations [m2p_lo, m2p_hi).
+// Work that shares a launch with M2M, M2L or L2L chunks in the operator
+// (single-value) pipeline: the near-field SpMV of leaves [lo, lo + count) in
+// spmv_blocks workgroups, and/or the M2P evaluations [m2p_lo, m2p_hi), added
+// into their own accumulator m2p_out.
 template <typename T>
 struct SideArgs {
   NearArgs<T> near;
-  int lo, count, per_xcd;
+  int lo, count, per_xcd, spmv_blocks;
   int64_t m2p_lo, m2p_hi;
   const int *item_node, *item_src;
   const double *xyz, *pmin, *delta;
   int p, kt, kp;
   const T *ME;
-  double *out;
+  double *m2p_out;
 };
 
-// Horizontal fusion.  The level launches of M2M and L2L hold few chunks
-// (one workgroup each) and wait on latency, leaving most CUs idle; the
-// workgroups after the level's nch chunks run independent work meanwhile:
-// SIDE 0..2 the SpMV (workgroup per leaf; wavefront per leaf with 2 or 4
-// rows in flight), SIDE 3 the M2P evaluations.  The SpMV overwrites its rows
-// and M2P adds to them, so the SpMV goes with an M2M level (before M2L) and
-// M2P with an L2L level (after it).  Each goes whole into one launch: its
+// Horizontal fusion.  M2M and L2L level launches hold few chunks (one
+// workgroup each) and wait on latency, leaving most CUs idle, and M2L's
+// workgroups are latency-bound too.  The workgroups after a launch's nch
+// chunks run independent work meanwhile: the SpMV (SPMV 0 workgroup per
+// leaf, 1 and 2 wavefront per leaf with 2 or 4 rows in flight, -1 none) and
+// then, with M2P set, the M2P evaluations.  The SpMV writes the rows of out
+// and M2P adds into a separate accumulator, so neither waits for the other;
+// both need only the final MEs.  Each goes whole into one launch: its
 // workgroups are latency-bound themselves, and shares spread over several
 // launches would pay that latency once per launch.  One LDS buffer serves
-// either role.
-template <typename T, int P, int KIND, int SIDE>
+// every role.
+template <typename T, int P, int KIND, int SPMV, bool M2P>
 __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate_side(int nch, TrArgs<T> tr, SideArgs<T> sd) {
   constexpr int NT = 4 * Kp<P>::value;
   using E = typename NearEntry<T, 1>::type;
   constexpr size_t TB = TrLayout<P, T>::bytes;
-  constexpr size_t SB = SIDE == 0 ? kSpmvTile * sizeof(E) : 0;
+  constexpr size_t SB = SPMV == 0 ? kSpmvTile * sizeof(E) : 0;
   __shared__ __align__(16) unsigned char smem[TB > SB ? TB : SB];
   __shared__ int sT[64], sS[64];
   if ((int)blockIdx.x < nch) {
     translate_chunk<T, P, KIND, 1>(blockIdx.x, reinterpret_cast<T *>(smem), sT, sS, tr);
     return;
   }
-  const int b = (int)blockIdx.x - nch;
-  if constexpr (SIDE == 0) {
-    // leaves dealt out per XCD as in k_spmv_lds (b and blockIdx.x differ by nch)
-    const int leaf = (b % kXcd) * sd.per_xcd + b / kXcd;
-    if (leaf < sd.count) spmv_lds_leaf<T, 1, NT>(sd.lo + leaf, reinterpret_cast<E *>(smem), sd.near);
-  } else if constexpr (SIDE == 1 || SIDE == 2) {
-    const int leaf = b * (NT / 64) + (int)(threadIdx.x >> 6);
-    if (leaf < sd.count) spmv_wave_leaf<T, 1, SIDE == 1 ? 2 : 4>(sd.lo + leaf, threadIdx.x & 63, sd.near);
-  } else {
+  int b = (int)blockIdx.x - nch;
+  if constexpr (SPMV >= 0) {
+    if (b < sd.spmv_blocks) {
+      if constexpr (SPMV == 0) {
+        // leaves dealt out per XCD as in k_spmv_lds (b and blockIdx.x differ by nch)
+        const int leaf = (b % kXcd) * sd.per_xcd + b / kXcd;
+        if (leaf < sd.count) spmv_lds_leaf<T, 1, NT>(sd.lo + leaf, reinterpret_cast<E *>(smem), sd.near);
+      } else {
+        const int leaf = b * (NT / 64) + (int)(threadIdx.x >> 6);
+        if (leaf < sd.count) spmv_wave_leaf<T, 1, SPMV == 1 ? 2 : 4>(sd.lo + leaf, threadIdx.x & 63, sd.near);
+      }
+      return;
+    }
+    b -= sd.spmv_blocks;
+  }
+  if constexpr (M2P) {
     const int64_t w = sd.m2p_lo + (int64_t)b * NT + threadIdx.x;
     if (w < sd.m2p_hi)
-      m2p_item<T, 1>(w, sd.item_node, sd.item_src, sd.xyz, sd.pmin, sd.delta, sd.p, sd.kt, sd.kp, sd.ME, sd.out,
-                     nullptr);
+      m2p_item<T, 1>(w, sd.item_node, sd.item_src, sd.xyz, sd.pmin, sd.delta, sd.p, sd.kt, sd.kp, sd.ME,
+                     sd.m2p_out, nullptr);
   }
 }
 
@@ -1617,31 +1638,35 @@ static void launch_translate(Engine &e, hipStream_t st, const ChunkDev &cl, int 
 #undef PBFMM_TR
 }
 
-// Workgroups of the side work for a launch of NT threads.
-template <int SIDE, typename T>
-static int side_blocks(const SideArgs<T> &sd, int nt) {
-  if (SIDE == 0) return sd.count > 0 ? sd.per_xcd * kXcd : 0;
-  if (SIDE == 1 || SIDE == 2) return (sd.count + nt / 64 - 1) / (nt / 64);
-  return (int)((sd.m2p_hi - sd.m2p_lo + nt - 1) / nt);
+// Workgroups of the SpMV side work for a launch of NT threads.
+template <int SPMV, typename T>
+static int spmv_side_blocks(const SideArgs<T> &sd, int nt) {
+  if (SPMV < 0 || sd.count <= 0) return 0;
+  if (SPMV == 0) return sd.per_xcd * kXcd;
+  return (sd.count + nt / 64 - 1) / (nt / 64);
 }
 
-// One level of the single-value pipeline's M2M (KIND 0) or L2L (KIND 2),
-// chunks [c0, c1), with side work sd in the same launch (k_translate_side).
-template <typename T, int KIND, int SIDE>
+// Chunks [c0, c1) of an M2M (KIND 0), M2L (1) or L2L (2) chunk list of the
+// single-value pipeline, with side work sd in the same launch
+// (k_translate_side).
+template <typename T, int KIND, int SPMV, bool M2P>
 static void launch_translate_side(Engine &e, const ChunkDev &cl, int c0, int c1, const T *ops, const T *IN, T *OUT,
-                                  const SideArgs<T> &sd) {
+                                  SideArgs<T> sd) {
   const int nch = std::max(0, c1 - c0);
   TrArgs<T> tr{};
   if (nch > 0)
     tr = TrArgs<T>{cl.op1.as<int>() + c0, cl.beg1.as<int>() + c0, cl.cnt1.as<int>() + c0, cl.tgt.as<int>(),
                    cl.src.as<int>(), ops, IN, OUT};
-#define PBFMM_TRS(PP)                                                                                        \
-  case PP: {                                                                                                 \
-    constexpr int NT = TrLayout<PP, T>::NT;                                                                  \
-    const int nb = nch + side_blocks<SIDE>(sd, NT);                                                         \
-    if (nb > 0)                                                                                              \
-      hipLaunchKernelGGL((k_translate_side<T, PP, KIND, SIDE>), dim3(nb), dim3(NT), 0, e.stream, nch, tr, sd); \
-    break;                                                                                                   \
+#define PBFMM_TRS(PP)                                                                                         \
+  case PP: {                                                                                                  \
+    constexpr int NT = TrLayout<PP, T>::NT;                                                                   \
+    sd.spmv_blocks = spmv_side_blocks<SPMV>(sd, NT);                                                          \
+    const int64_t mb = M2P ? (sd.m2p_hi - sd.m2p_lo + NT - 1) / NT : 0;                                      \
+    const int64_t nb = nch + sd.spmv_blocks + mb;                                                             \
+    if (nb > 0)                                                                                               \
+      hipLaunchKernelGGL((k_translate_side<T, PP, KIND, SPMV, M2P>), dim3((unsigned)nb), dim3(NT), 0, e.stream, \
+                         nch, tr, sd);                                                                        \
+    break;                                                                                                    \
   }
   switch (e.p) {
     PBFMM_TRS(1) PBFMM_TRS(2) PBFMM_TRS(3) PBFMM_TRS(4) PBFMM_TRS(5)
@@ -1796,56 +1821,44 @@ void Engine::consumer_levels() {
   for (int b : far.m2p_src) me_lo = std::min(me_lo, (int)tree.level[b]);
 }
 
-// The single-value pipeline in fewer launches: ME and LE cleared in one
-// launch; P2M; M2M levels L..me_lo + 1; M2L; L2L levels le_lo + 1..L (at
-// least 2); L2P, which also applies vmult's epilogue into post_dst (every DoF
-// is a row of one leaf).
-// With side 1 or 2 the deepest non-empty M2M level also runs the whole SpMV,
-// and all M2P evaluations run beside the deepest non-empty L2L level (1) or
-// beside M2L (2) (k_translate_side); with side 0 they have launches of their
-// own.  The same
-// arithmetic as generate + products + k_vmult_post.  Phase timing runs that
+// The single-value pipeline in fewer launches, for vmult: ME, LE and the M2P
+// accumulator cleared in one launch; P2M; M2M levels L..me_lo + 1; M2L; L2L
+// levels le_lo + 1..L (at least 2); L2P, which also adds the M2P
+// accumulator and applies vmult's epilogue, dst = out + m2p + L2P + alpha x
+// on Neumann DoFs (every DoF is a row of one leaf).  The SpMV writes out and
+// M2P adds into the accumulator (d_outD, unused by this pipeline), so the two
+// can run beside any launch after P2M (SpMV) or after M2M (M2P), as side
+// work (k_translate_side):
+//   side 0  both in launches of their own
+//   side 1  SpMV beside the deepest M2M level, M2P beside the deepest L2L level
+//   side 2  SpMV beside the deepest M2M level, M2P beside M2L
+//   side 3  both beside M2L
+// The same arithmetic as generate + products + k_vmult_post, with M2P's sum
+// added to the SpMV's once instead of term by term.  Phase timing runs that
 // unfused sequence instead, so that each phase can be timed.
 template <typename T>
-void Engine::matvec_fused(const double *x, double *out, double *post_dst, int side) {
+void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   FarDev<T> &F = far_of<T>(*this);
   const int nleaf = (int)far.p2m_leaf.size();
   const int64_t ne = (int64_t)tree.nb * kp;
   T *me = F.me.template as<T>(), *le = F.le.template as<T>();
-  hipLaunchKernelGGL(k_fill2<T>, dim3((unsigned)std::min<int64_t>(nblk(2 * ne, 256), 4096)), dim3(256), 0, stream, me,
-                     le, ne, T(0));
+  double *acc = d_outD.as<double>();
+  hipLaunchKernelGGL(k_clear_operator<T>, dim3((unsigned)std::min<int64_t>(nblk(2 * ne + n, 256), 4096)), dim3(256),
+                     0, stream, me, le, ne, acc, n);
   if (nleaf > 0)
     hipLaunchKernelGGL((k_p2m<T, 1>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
                        d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp, x, nullptr, me);
+  if (me_lo < 0) consumer_levels();
   const int var = spmv_variant[sizeof(T) == 4 ? 1 : 0][0];
   const int nl = near.nleaf;
+  const int64_t ni = (int64_t)far.m2p_item_node.size();
   SideArgs<T> sd{};
   sd.near = near_args<T, 1>(*this, x, nullptr, out, nullptr);
-  if (me_lo < 0) consumer_levels();
-  bool near_done = nl == 0 || !side;
-  if (!side && nl > 0) launch_spmv<T, 1>(var, x, nullptr, out, nullptr);
-  for (int l = L; l > me_lo; --l) {  // M2M from level l into l - 1
-    const int c0 = lvl<1>(far.m2m, l), c1 = lvl<1>(far.m2m, l + 1);
-    if (c1 <= c0) continue;
-    if (near_done) {
-      launch_translate<T, 0, 1>(*this, stream, m2m_cl, c0, c1, F.m2m.template as<T>(), me, me);
-      continue;
-    }
-    sd.lo = 0;
-    sd.count = near_done ? 0 : nl;
-    sd.per_xcd = (sd.count + kXcd - 1) / kXcd;
-    const T *ops = F.m2m.template as<T>();
-    if (var == 0)
-      launch_translate_side<T, 0, 0>(*this, m2m_cl, c0, c1, ops, me, me, sd);
-    else if (var == 1)
-      launch_translate_side<T, 0, 1>(*this, m2m_cl, c0, c1, ops, me, me, sd);
-    else
-      launch_translate_side<T, 0, 2>(*this, m2m_cl, c0, c1, ops, me, me, sd);
-    near_done = true;
-  }
-  if (!near_done) launch_spmv<T, 1>(var, x, nullptr, out, nullptr);
-  const int64_t ni = (int64_t)far.m2p_item_node.size();
-  sd.count = 0;
+  sd.lo = 0;
+  sd.count = nl;
+  sd.per_xcd = (nl + kXcd - 1) / kXcd;
+  sd.m2p_lo = 0;
+  sd.m2p_hi = ni;
   sd.item_node = d_m2p_node.as<int>();
   sd.item_src = d_m2p_isrc.as<int>();
   sd.xyz = d_xyz.as<double>();
@@ -1855,44 +1868,73 @@ void Engine::matvec_fused(const double *x, double *out, double *post_dst, int si
   sd.kt = kt;
   sd.kp = kp;
   sd.ME = me;
-  sd.out = out;
-  sd.m2p_lo = 0;
-  sd.m2p_hi = ni;
-  bool m2p_done = ni == 0;
-  if (side == 2 && !m2p_done) {  // M2P beside M2L
-    launch_translate_side<T, 1, 3>(*this, m2l_cl, 0, nchunks<1>(far.m2l), F.m2l.template as<T>(), me, le, sd);
-    m2p_done = true;
-  } else {
-    launch_translate<T, 1, 1>(*this, stream, m2l_cl, 0, nchunks<1>(far.m2l), F.m2l.template as<T>(), me, le);
+  sd.m2p_out = acc;
+  bool near_done = nl == 0, m2p_done = ni == 0;
+  if (side == 0 && !near_done) {
+    launch_spmv<T, 1>(var, x, nullptr, out, nullptr);
+    near_done = true;
   }
-  const int l2l_lo = std::max(2, le_lo + 1);  // L2L from level l - 1 into l
-  int deepest = 0;  // deepest L2L level with chunks: it also runs M2P
-  for (int l = l2l_lo; side == 1 && !m2p_done && l <= L; ++l)
-    if (lvl<1>(far.l2l, l + 1) > lvl<1>(far.l2l, l)) deepest = l;
-  for (int l = l2l_lo; l <= L; ++l) {
-    const int c0 = lvl<1>(far.l2l, l), c1 = lvl<1>(far.l2l, l + 1);
+  // M2M, the deepest launched level carrying the SpMV (sides 1, 2)
+  for (int l = L; l > me_lo; --l) {  // M2M from level l into l - 1
+    const int c0 = lvl<1>(far.m2m, l), c1 = lvl<1>(far.m2m, l + 1);
     if (c1 <= c0) continue;
-    if (l != deepest) {
-      launch_translate<T, 2, 1>(*this, stream, l2l_cl, c0, c1, F.l2l.template as<T>(), le, le);
+    const T *ops = F.m2m.template as<T>();
+    if (near_done || side == 3) {
+      launch_translate<T, 0, 1>(*this, stream, m2m_cl, c0, c1, ops, me, me);
       continue;
     }
-    launch_translate_side<T, 2, 3>(*this, l2l_cl, c0, c1, F.l2l.template as<T>(), le, le, sd);
+    if (var == 0)
+      launch_translate_side<T, 0, 0, false>(*this, m2m_cl, c0, c1, ops, me, me, sd);
+    else if (var == 1)
+      launch_translate_side<T, 0, 1, false>(*this, m2m_cl, c0, c1, ops, me, me, sd);
+    else
+      launch_translate_side<T, 0, 2, false>(*this, m2m_cl, c0, c1, ops, me, me, sd);
+    near_done = true;
+  }
+  // M2L, carrying M2P (sides 2, 3) and the SpMV (side 3, or whatever no M2M
+  // launch took)
+  const int c1 = nchunks<1>(far.m2l);
+  const T *mops = F.m2l.template as<T>();
+  const bool m2p_here = !m2p_done && side >= 2, near_here = !near_done;
+  if (near_here && var == 0)
+    m2p_here ? launch_translate_side<T, 1, 0, true>(*this, m2l_cl, 0, c1, mops, me, le, sd)
+             : launch_translate_side<T, 1, 0, false>(*this, m2l_cl, 0, c1, mops, me, le, sd);
+  else if (near_here && var == 1)
+    m2p_here ? launch_translate_side<T, 1, 1, true>(*this, m2l_cl, 0, c1, mops, me, le, sd)
+             : launch_translate_side<T, 1, 1, false>(*this, m2l_cl, 0, c1, mops, me, le, sd);
+  else if (near_here)
+    m2p_here ? launch_translate_side<T, 1, 2, true>(*this, m2l_cl, 0, c1, mops, me, le, sd)
+             : launch_translate_side<T, 1, 2, false>(*this, m2l_cl, 0, c1, mops, me, le, sd);
+  else if (m2p_here)
+    launch_translate_side<T, 1, -1, true>(*this, m2l_cl, 0, c1, mops, me, le, sd);
+  else
+    launch_translate<T, 1, 1>(*this, stream, m2l_cl, 0, c1, mops, me, le);
+  m2p_done = m2p_done || m2p_here;
+  // L2L, the deepest launched level carrying M2P (side 1)
+  const int l2l_lo = std::max(2, le_lo + 1);  // L2L from level l - 1 into l
+  int deepest = 0;
+  for (int l = l2l_lo; !m2p_done && side == 1 && l <= L; ++l)
+    if (lvl<1>(far.l2l, l + 1) > lvl<1>(far.l2l, l)) deepest = l;
+  for (int l = l2l_lo; l <= L; ++l) {
+    const int c0 = lvl<1>(far.l2l, l), cc1 = lvl<1>(far.l2l, l + 1);
+    if (cc1 <= c0) continue;
+    if (l != deepest) {
+      launch_translate<T, 2, 1>(*this, stream, l2l_cl, c0, cc1, F.l2l.template as<T>(), le, le);
+      continue;
+    }
+    launch_translate_side<T, 2, -1, true>(*this, l2l_cl, c0, cc1, F.l2l.template as<T>(), le, le, sd);
     m2p_done = true;
   }
   if (!m2p_done)
     hipLaunchKernelGGL((k_m2p<T, 1>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, stream, ni, d_m2p_node.as<int>(),
                        d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt,
-                       kp, me, out, nullptr);
+                       kp, me, acc, nullptr);
   const int nrows = (int)near.rows.size();
-  if (post_dst)
+  if (nrows > 0)
     hipLaunchKernelGGL((k_l2p_post<T>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
                        d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), p, kt, kp, le, out, d_alpha.as<double>(), d_dir.as<uint8_t>(), x,
-                       post_dst);
-  else if (nrows > 0)
-    hipLaunchKernelGGL((k_l2p<T, 1>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
-                       d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), p, kt, kp, le, out, nullptr);
+                       d_delta.as<double>(), p, kt, kp, le, out, acc, d_alpha.as<double>(), d_dir.as<uint8_t>(), x,
+                       dst);
   HIPCHK(hipGetLastError());
 }
 
@@ -1917,11 +1959,10 @@ double Engine::norm2(const double *v) {
 
 // Side fusion pays while the M2M / L2L level launches are latency-bound and
 // the SpMV and M2P are short beside them (tens of thousands of DoFs); on
-// large trees the fused workgroups only crowd each other.  Modes: 0 none,
-// 1 SpMV beside an M2M level and M2P beside an L2L level, 2 SpMV beside an
-// M2M level and M2P beside M2L.  Timed once per precision, on the first
-// operator application: three applications each way on scratch vectors,
-// after one warm-up.
+// large trees the fused workgroups only crowd each other.  The modes are
+// matvec_fused's side 0..3.  Timed once per precision, on the first operator
+// application: five applications each way on scratch vectors, after one
+// warm-up; a fused mode is kept only if it beats the unfused one by 2%.
 int Engine::use_side_fusion(bool f32) {
   int &mode = side_fusion[f32 ? 1 : 0];
   if (mode >= 0) return mode;
@@ -1932,15 +1973,15 @@ int Engine::use_side_fusion(bool f32) {
   hipEvent_t t0, t1;
   HIPCHK(hipEventCreate(&t0));
   HIPCHK(hipEventCreate(&t1));
-  float ms[3] = {0, 0, 0};
-  for (int s = 0; s < 3; ++s)
-    for (int rep = 0; rep < 4; ++rep) {
+  float ms[kSideModes] = {};
+  for (int s = 0; s < kSideModes; ++s)
+    for (int rep = 0; rep < 6; ++rep) {
       if (rep == 1) HIPCHK(hipEventRecord(t0, stream));
       if (f32)
         matvec_fused<float>(u, d_outN.as<double>(), v, s);
       else
         matvec_fused<double>(u, d_outN.as<double>(), v, s);
-      if (rep == 3) {
+      if (rep == 5) {
         HIPCHK(hipEventRecord(t1, stream));
         HIPCHK(hipEventSynchronize(t1));
         HIPCHK(hipEventElapsedTime(&ms[s], t0, t1));
@@ -1948,9 +1989,10 @@ int Engine::use_side_fusion(bool f32) {
     }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
-  mode = 0;
-  for (int s = 1; s < 3; ++s)
-    if (ms[s] < ms[mode]) mode = s;
+  int best = 0;
+  for (int s = 1; s < kSideModes; ++s)
+    if (ms[s] < ms[best]) best = s;
+  mode = ms[best] < 0.98f * ms[0] ? best : 0;
   return mode;
 }
 

@@ -113,7 +113,8 @@ class BEMFMA:
     def set_side_fusion(self, precision, mode):
         """Overrides the timed side-fusion mode of the operator's launches:
         0 none, 1 SpMV beside an M2M level and M2P beside an L2L level, 2 M2P
-        beside M2L instead; -1 times them again (pbfmm_set_side_fusion)."""
+        beside M2L instead, 3 both beside M2L; -1 times them again
+        (pbfmm_set_side_fusion)."""
         self._chk(self._L.pbfmm_set_side_fusion(self._h, int(precision), int(mode)))
 
     def tuning_state(self):
