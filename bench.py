@@ -427,8 +427,12 @@ def mode_matvec(args):
     for _ in range(args.steps):
         f.matvec_dev(a.value, b.value, oN.value, oD.value, prec)
     bp = P.BEMProblem(f)
+    # phase-timed, as the operator phases bench.py reports: every phase in its
+    # own launch (the solve's fused launches are profiled with --mode solve)
+    f.set_phase_timing(True)
     for _ in range(args.steps):
         bp.vmult_dev(a.value, oN.value, prec)
+    f.set_phase_timing(False)
     stream.synchronize()
     log(f"{args.steps} {args.precision} FMM matvecs and {args.steps} operator applications at config "
         f"{args.config} done")

@@ -213,8 +213,10 @@ int pbfmm_tree_export(pbfmm_handle *h, int32_t *level, int64_t *parent, double *
  * the FP64 or the mixed-mode FP32 values, widened). */
 int pbfmm_nearfield_export(pbfmm_handle *h, int32_t precision, int64_t *rowptr, int64_t *col,
                            double *valN, double *valD);
-/* Expansions of every block after the last FP64 matvec: which 0 = multipole,
- * 1 = local; Kt=(p+1)(p+2)/2 complex coefficients per block and kernel,
+/* Expansions of every block after the last FP64
+ * pbfmm_generate_multipole_expansions / pbfmm_multipole_matr_vect_products
+ * (the device entry points skip levels whose expansions nothing reads):
+ * which 0 = multipole, 1 = local; Kt=(p+1)(p+2)/2 complex coefficients per block and kernel,
  * interleaved (re, im), index n(n+1)/2+m as in multipole_expansion.h:101-117. */
 int pbfmm_get_expansions(pbfmm_handle *h, int32_t which, double *ker1, double *ker2);
 /* Per-phase device times of the last matvec in ms, measured with HIP events

@@ -1791,11 +1791,48 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
   HIPCHK(hipGetLastError());
 }
 
+// The two-output FMM for device vectors, alpha and the right-hand side.
+// While phase timing is off it runs in fewer launches: ME and LE cleared in
+// one launch, and M2M / L2L skipped on the levels whose expansions nothing
+// reads (consumer_levels).  The reference API's generate + products calls
+// keep every level, since pbfmm_get_expansions exports them all.
 template <typename T>
 void Engine::matvec(const double *phi, const double *dphi, double *outN, double *outD) {
-  generate<T, 2>(phi, dphi);
-  products<T, 2>(phi, dphi, outN, outD);
-  collect_phase_times();
+  if (time_phases) {
+    generate<T, 2>(phi, dphi);
+    products<T, 2>(phi, dphi, outN, outD);
+    collect_phase_times();
+    return;
+  }
+  FarDev<T> &F = far_of<T>(*this);
+  const int64_t ne = (int64_t)tree.nb * 2 * kp;
+  T *me = F.me.template as<T>(), *le = F.le.template as<T>();
+  hipLaunchKernelGGL(k_clear_operator<T>, dim3((unsigned)std::min<int64_t>(nblk(2 * ne, 256), 4096)), dim3(256), 0,
+                     stream, me, le, ne, nullptr, (int64_t)0);
+  const int nleaf = (int)far.p2m_leaf.size();
+  if (nleaf > 0)
+    hipLaunchKernelGGL((k_p2m<T, 2>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
+                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.unit.template as<T>(), kp, phi, dphi, me);
+  if (me_lo < 0) consumer_levels();
+  for (int l = L; l > me_lo; --l)
+    launch_translate<T, 0, 2>(*this, stream, m2m_cl, lvl<2>(far.m2m, l), lvl<2>(far.m2m, l + 1),
+                              F.m2m.template as<T>(), me, me);
+  if (near.nleaf > 0) launch_spmv<T, 2>(spmv_variant[sizeof(T) == 4 ? 1 : 0][1], phi, dphi, outN, outD);
+  launch_translate<T, 1, 2>(*this, stream, m2l_cl, 0, nchunks<2>(far.m2l), F.m2l.template as<T>(), me, le);
+  const int64_t ni = (int64_t)far.m2p_item_node.size();
+  if (ni > 0)
+    hipLaunchKernelGGL((k_m2p<T, 2>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, stream, ni, d_m2p_node.as<int>(),
+                       d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt,
+                       kp, me, outN, outD);
+  for (int l = std::max(2, le_lo + 1); l <= L; ++l)
+    launch_translate<T, 2, 2>(*this, stream, l2l_cl, lvl<2>(far.l2l, l), lvl<2>(far.l2l, l + 1),
+                              F.l2l.template as<T>(), le, le);
+  const int nrows = (int)near.rows.size();
+  if (nrows > 0)
+    hipLaunchKernelGGL((k_l2p<T, 2>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
+                       d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
+                       d_delta.as<double>(), p, kt, kp, le, outN, outD);
+  HIPCHK(hipGetLastError());
 }
 
 template <typename T>

@@ -354,3 +354,35 @@ def test_fused_operator_pipeline(name):
         assert rel(dst.numpy(), ref) <= bar
         assert f.tuning_state()["side_fusion_fp32" if prec == P.PBFMM_MIXED else "side_fusion_fp64"] in (0, 1, 2, 3)
     f.close()
+
+
+def test_matvec_dev_fast_path():
+    """pbfmm_matvec_dev (ME and LE cleared in one launch, M2M / L2L skipped on
+    levels whose expansions nothing reads) against the reference API's
+    generate + products on the same handle, which run every level: FP64
+    within 1e-14; FP32 against its phase-timed run (every level) within
+    1e-6."""
+    from pibem_amd import hip
+    pr = M.config_sphere(24, 6)
+    f = P.setup(pr, P.PBFMM_MIXED)
+    n = pr.mesh.n_dofs
+    rng = np.random.default_rng(5)
+    a, b = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
+    f.generate_multipole_expansions(a, b)
+    N0, D0 = f.multipole_matr_vect_products(a, b)
+    da, db = hip.DeviceArray.from_numpy(a), hip.DeviceArray.from_numpy(b)
+    oN, oD = hip.DeviceArray(n), hip.DeviceArray(n)
+    f.matvec_dev(da.value, db.value, oN.value, oD.value, P.PBFMM_FP64)
+    hip.synchronize()
+    e64 = max(rel(oN.numpy(), N0), rel(oD.numpy(), D0))
+    f.set_phase_timing(True)
+    f.matvec_dev(da.value, db.value, oN.value, oD.value, P.PBFMM_MIXED)
+    hip.synchronize()
+    n32, d32 = oN.numpy(), oD.numpy()
+    f.set_phase_timing(False)
+    f.matvec_dev(da.value, db.value, oN.value, oD.value, P.PBFMM_MIXED)
+    hip.synchronize()
+    e32 = max(rel(oN.numpy(), n32), rel(oD.numpy(), d32))
+    print(f"\n  N={n}: fp64 fast vs reference API {e64:.2e}, fp32 fast vs phase-timed {e32:.2e}")
+    assert e64 <= 1e-14 and e32 <= 1e-6
+    f.close()
