@@ -89,6 +89,11 @@ struct Engine {
   DevBuf d_alpha, d_sp, d_sd, d_outN, d_outD, d_in, d_out, d_red;
   DevBuf d_cgs;  // partial sums of the Arnoldi step's dots (kDotParts per column)
   int max_leaf_rows = 1 << 30;  // largest leaf block of the preconditioner
+  // pinned host copy of the Hessenberg slots (asynchronous device-to-host
+  // copies) and the event that marks a copy done
+  double *h_pinned = nullptr;
+  size_t h_pinned_n = 0;
+  hipEvent_t h_ready = nullptr;
   // coarsest levels whose expansions are read (MEs: M2L and M2P sources; LEs:
   // M2L targets); M2M into coarser levels and L2L out of them are skipped in
   // the operator pipeline.  -1 until computed from the plan.

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <functional>
 #include <cmath>
 #include <cstring>
@@ -1374,6 +1375,8 @@ void Engine::init_device(int dev, void *st) {
 Engine::~Engine() {
   for (auto &e : ev)
     if (e) (void)hipEventDestroy(e);
+  if (h_ready) (void)hipEventDestroy(h_ready);
+  if (h_pinned) (void)hipHostFree(h_pinned);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
@@ -2227,6 +2230,11 @@ void Engine::apply_prec(const double *v, double *z) {
 // (CGS2), FP64 basis and Hessenberg, operator precision as requested.  Solves
 // A x = b from the given x until the Arnoldi residual estimate reaches
 // tol_abs or maxit operator applications have been spent.
+// A step is enqueued ahead of a host round trip only while Arnoldi steps
+// take less than this (ms): the idle time a round trip leaves is tens of
+// microseconds, an unused step costs a whole operator application.
+constexpr double kSpeculateMs = 0.5;
+
 static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int m, int maxit, int precision,
                       int &it, double &est, double switch_abs = -1.0) {
   const int64_t n = e.n;
@@ -2238,12 +2246,42 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
   // Hessenberg slots on the device, one per Arnoldi step: [h1 (m+2)][h2 (m+2)][|w|]
   const int S = 2 * (m + 2) + 1;
   double *ybuf = hd + (size_t)m * S;
-  std::vector<double> hbuf((size_t)m * S);
+  if (e.h_pinned_n < (size_t)m * S) {
+    if (e.h_pinned) HIPCHK(hipHostFree(e.h_pinned));
+    e.h_pinned = nullptr;
+    e.h_pinned_n = 0;
+    HIPCHK(hipHostMalloc((void **)&e.h_pinned, (size_t)m * S * 8, hipHostMallocDefault));
+    e.h_pinned_n = (size_t)m * S;
+  }
+  if (!e.h_ready) HIPCHK(hipEventCreateWithFlags(&e.h_ready, hipEventDisableTiming));
+  double *hbuf = e.h_pinned;
   // relaxed mixed precision (switch_abs > 0): FP64 operator while the
   // residual estimate is above switch_abs, the FP32 operator below it, in one
   // Krylov space; the cycle's starting residual always in FP64
   const bool relaxed = switch_abs > 0 && precision == PBFMM_MIXED;
   auto step_prec = [&]() { return relaxed ? (est <= switch_abs ? PBFMM_MIXED : PBFMM_FP64) : precision; };
+  // Arnoldi step k: V_k+1 from V_k, its Hessenberg column into slot k
+  auto enqueue_step = [&](int k) {
+    double *vk1 = V + (int64_t)(k + 1) * n;
+    const int pk = step_prec();
+    if (e.have_prec) {
+      e.apply_prec(V + (int64_t)k * n, e.d_ptmp.as<double>());
+      e.vmult(e.d_ptmp.as<double>(), vk1, pk);
+    } else {
+      e.vmult(V + (int64_t)k * n, vk1, pk);
+    }
+    ++it;
+    // CGS2: two projections and the norm, all on the device, six launches:
+    // the projections and the scaling finish the dot products they use
+    double *slot = hd + (size_t)k * S;
+    double *h1 = slot, *h2 = slot + (m + 2), *hn_d = slot + 2 * (m + 2);
+    for (double *hj : {h1, h2}) {
+      hipLaunchKernelGGL(k_mdot_parts, dim3(kDotParts, k + 1), dim3(kRedThreads), 0, e.stream, n, n, V, vk1, cg);
+      hipLaunchKernelGGL(k_gemv_sub_fin, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, cg, kDotParts, hj, vk1);
+    }
+    hipLaunchKernelGGL(k_mdot_parts, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, vk1, vk1, cg);
+    hipLaunchKernelGGL(k_scale_fin, dim3(g), dim3(256), 0, e.stream, n, vk1, cg, kNormParts, hn_d);
+  };
   while (it < maxit) {
     // r = b - A x into V_0
     e.vmult(x, w, relaxed ? PBFMM_FP64 : precision);
@@ -2254,37 +2292,30 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
     hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, w, 1.0 / beta, V);
     std::fill(gv.begin(), gv.end(), 0.0);
     gv[0] = beta;
-    int k = 0, k_done = 0, k_sync = 0;  // columns < k_done are reduced; next host sync after step k_sync
-    bool done = false;
-    for (; k < m && it < maxit; ++k) {
-      double *vk1 = V + (int64_t)(k + 1) * n;
-      const int pk = step_prec();
-      if (e.have_prec) {
-        e.apply_prec(V + (int64_t)k * n, e.d_ptmp.as<double>());
-        e.vmult(e.d_ptmp.as<double>(), vk1, pk);
-      } else {
-        e.vmult(V + (int64_t)k * n, vk1, pk);
-      }
-      ++it;
-      // CGS2: two projections and the norm, all on the device, six launches:
-      // the projections and the scaling finish the dot products they use
-      double *slot = hd + (size_t)k * S;
-      double *h1 = slot, *h2 = slot + (m + 2), *hn_d = slot + 2 * (m + 2);
-      for (double *hj : {h1, h2}) {
-        hipLaunchKernelGGL(k_mdot_parts, dim3(kDotParts, k + 1), dim3(kRedThreads), 0, e.stream, n, n, V, vk1, cg);
-        hipLaunchKernelGGL(k_gemv_sub_fin, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, cg, kDotParts, hj, vk1);
-      }
-      hipLaunchKernelGGL(k_mdot_parts, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, vk1, vk1, cg);
-      hipLaunchKernelGGL(k_scale_fin, dim3(g), dim3(256), 0, e.stream, n, vk1, cg, kNormParts, hn_d);
-      if (k < k_sync && k + 1 < m && it < maxit) continue;
-      // host round trip: the columns k_done..k, Givens rotations, residual estimate
-      HIPCHK(hipMemcpyAsync(hbuf.data() + (size_t)k_done * S, hd + (size_t)k_done * S,
-                            (size_t)(k + 1 - k_done) * S * 8, hipMemcpyDeviceToHost, e.stream));
-      e.sync();
+    // kq: steps enqueued; k_done: columns reduced on the host; the next host
+    // round trip comes after step k_sync.  A round trip copies the new columns
+    // to pinned memory and, unless convergence is predicted for them, enqueues
+    // the next step before it waits, so that the GPU works while the host
+    // reduces (the step is unused if they do converge).
+    int kq = 0, k_done = 0, k_sync = 0, k = 0;
+    bool done = false, speculate = true;
+    auto t_trip = std::chrono::steady_clock::now();
+    while (true) {
+      if (kq < m && it < maxit) enqueue_step(kq++);
+      const bool more = kq < m && it < maxit;
+      if (kq - 1 < k_sync && more) continue;
+      if (k_done == kq) break;  // every enqueued column reduced and no step left
+      // host round trip: the columns k_done..kq-1, Givens rotations, residual estimate
+      const int kk = kq;
+      HIPCHK(hipMemcpyAsync(hbuf + (size_t)k_done * S, hd + (size_t)k_done * S, (size_t)(kk - k_done) * S * 8,
+                            hipMemcpyDeviceToHost, e.stream));
+      HIPCHK(hipEventRecord(e.h_ready, e.stream));
+      if (more && speculate) enqueue_step(kq++);
+      HIPCHK(hipEventSynchronize(e.h_ready));
       const double est0 = est;
       const int j0 = k_done;
-      for (int j = k_done; j <= k; ++j) {
-        const double *hb = hbuf.data() + (size_t)j * S;
+      for (int j = k_done; j < kk; ++j) {
+        const double *hb = hbuf + (size_t)j * S;
         for (int i = 0; i <= j; ++i) h[i] = hb[i] + hb[(m + 2) + i];
         const double hn = hb[2 * (m + 2)];
         h[j + 1] = hn;
@@ -2302,27 +2333,36 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
         gv[j] = cs[j] * gv[j];
         est = std::fabs(gv[j + 1]);
         if (est <= tol_abs || hn == 0) {
-          k = j;  // the Arnoldi steps after j are not used
+          k = j + 1;  // the Arnoldi steps after j are not used
           done = true;
           break;
         }
       }
-      if (done) {
-        ++k;
-        break;
-      }
-      k_done = k + 1;
-      // next sync: half the steps the observed decay predicts to the next
-      // event (the precision switch, then convergence), 1..8 ahead
+      if (done) break;
+      // host time per step since the previous round trip
+      const auto now = std::chrono::steady_clock::now();
+      const double step_ms = std::chrono::duration<double, std::milli>(now - t_trip).count() / (kk - j0);
+      t_trip = now;
+      k_done = kk;
+      k = kk;
+      // next round trip: half the steps the observed decay predicts to the
+      // next event (the precision switch, then convergence), 1..8 ahead.  A
+      // step is enqueued ahead of a round trip only when convergence is not
+      // expected there and steps are short (an unused step of a large
+      // problem costs more than the round trip's idle time saves)
       int ahead = 1;
+      speculate = step_ms < kSpeculateMs;
       if (est < est0 && est0 > 0) {
-        const double rate = std::log(est / est0) / (k + 1 - j0);  // per step, < 0
+        const double rate = std::log(est / est0) / (kk - j0);  // per step, < 0
         const double goal = (relaxed && est > switch_abs) ? switch_abs : tol_abs;
         const double left = std::log(goal / est) / rate;
         ahead = (int)std::max(1.0, std::min(8.0, 0.5 * left));
+        speculate = speculate && left > ahead + 1.5;
       }
-      k_sync = k + ahead;
+      k_sync = kk - 1 + ahead;
     }
+    // k: the columns that enter the update (all reduced ones, or up to the
+    // converged step)
     for (int i = k - 1; i >= 0; --i) {
       double s = gv[i];
       for (int j = i + 1; j < k; ++j) s -= H[(size_t)i * m + j] * y[j];
