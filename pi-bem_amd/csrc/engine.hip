@@ -2235,8 +2235,10 @@ void Engine::apply_prec(const double *v, double *z) {
 // microseconds, an unused step costs a whole operator application.
 constexpr double kSpeculateMs = 0.5;
 
+// r0 (device) with norm r0n, when given, is the residual b - A x of the
+// starting x: the first cycle then skips its operator application.
 static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int m, int maxit, int precision,
-                      int &it, double &est, double switch_abs = -1.0) {
+                      int &it, double &est, double switch_abs = -1.0, const double *r0 = nullptr, double r0n = 0) {
   const int64_t n = e.n;
   const unsigned g = nblk(n, 256);
   double *V = e.d_V.as<double>(), *w = e.d_w.as<double>(), *hd = e.d_h.as<double>();
@@ -2284,12 +2286,19 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
   };
   while (it < maxit) {
     // r = b - A x into V_0
-    e.vmult(x, w, relaxed ? PBFMM_FP64 : precision);
-    hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, b, -1.0, w);
-    const double beta = e.norm2(w);
+    const double *rv = w;
+    double beta = r0n;
+    if (r0) {
+      rv = r0;
+      r0 = nullptr;
+    } else {
+      e.vmult(x, w, relaxed ? PBFMM_FP64 : precision);
+      hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, b, -1.0, w);
+      beta = e.norm2(w);
+    }
     est = beta;
     if (beta <= tol_abs) return 0;
-    hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, w, 1.0 / beta, V);
+    hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, rv, 1.0 / beta, V);
     std::fill(gv.begin(), gv.end(), 0.0);
     gv[0] = beta;
     // kq: steps enqueued; k_done: columns reduced on the host; the next host
@@ -2439,7 +2448,21 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
     gm_restart = restart;
   }
   const unsigned g = nblk(n, 256);
-  const double bn = std::max(norm2(b), 1e-300);
+  // |b| and |x| in one round trip: a zero starting guess needs no operator
+  // application for its residual
+  double sq[2] = {0, 0};
+  {
+    const int nchunk = (int)nblk(n, kRedChunk);
+    double *part = d_red.as<double>(), *res = part + (size_t)nchunk * 128;
+    hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, b, b, part);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, stream, nchunk, part, res);
+    hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, x, x, part);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, stream, nchunk, part, res + 1);
+    HIPCHK(hipMemcpyAsync(sq, res, 16, hipMemcpyDeviceToHost, stream));
+    sync();
+  }
+  const double bn = std::max(std::sqrt(sq[0]), 1e-300);
+  bool x_zero = sq[1] == 0.0;
   struct MeanMode {  // linear pure-Neumann operator for the duration of the solve
     bool &flag;
     bool old;
@@ -2450,10 +2473,16 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
   double est = 0, rr = 1;
   double *r = d_r.as<double>(), *d = d_x.as<double>();
   for (int outer = 0; outer < 100 && it < maxit; ++outer) {
-    // true FP64 residual
-    vmult(x, r, PBFMM_FP64);
-    hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, b, -1.0, r);
-    const double rn = norm2(r);
+    // true FP64 residual (b itself for a zero starting guess)
+    double rn = bn;
+    if (x_zero) {
+      HIPCHK(hipMemcpyAsync(r, b, (size_t)n * 8, hipMemcpyDeviceToDevice, stream));
+      x_zero = false;
+    } else {
+      vmult(x, r, PBFMM_FP64);
+      hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, b, -1.0, r);
+      rn = norm2(r);
+    }
     rr = rn / bn;
     if (rr <= tol) break;
     if (precision == PBFMM_MIXED && have_f32) {
@@ -2465,15 +2494,15 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
         // step may carry grows as the residual shrinks); the outer FP64
         // residual catches what the FP32 steps leave
         gmres_core(*this, r, d, tol * bn * 0.5, restart, maxit, PBFMM_MIXED, it, est,
-                   std::min(rn, kRelaxSwitch * tol * bn));
+                   std::min(rn, kRelaxSwitch * tol * bn), r, rn);
       } else {
         // iterative refinement: FP32-operator GMRES on the FP64 residual
         const double tin = std::max(tol * bn * 0.5, kInnerReduction * rn);
-        gmres_core(*this, r, d, tin, restart, maxit, PBFMM_MIXED, it, est);
+        gmres_core(*this, r, d, tin, restart, maxit, PBFMM_MIXED, it, est, -1.0, r, rn);
       }
       hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, d, 1.0, x);
     } else {
-      gmres_core(*this, b, x, tol * bn * 0.5, restart, maxit, PBFMM_FP64, it, est);
+      gmres_core(*this, b, x, tol * bn * 0.5, restart, maxit, PBFMM_FP64, it, est, -1.0, r, rn);
     }
   }
   sync();
