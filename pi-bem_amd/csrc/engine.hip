@@ -783,11 +783,26 @@ __device__ __forceinline__ void translate_chunk(int c, T *__restrict__ sB, int *
   }
 }
 
+// Chunks of one class are consecutive in a chunk list and share their
+// operator.  Workgroups go round-robin over the 8 XCDs, so workgroup w takes
+// chunk (w % 8) * per + w / 8: each XCD works through a contiguous run of
+// chunks and finds their operators in its own L2.  The launch has 8 * per
+// workgroups (per = ceil(nch / 8)); those past the last chunk return.
+constexpr bool kXcdChunks = true;
+__device__ __forceinline__ int xcd_chunk(int w, int nch) {
+  if (!kXcdChunks) return w;
+  const int per = (nch + kXcd - 1) / kXcd;
+  return (w % kXcd) * per + w / kXcd;
+}
+static inline int xcd_blocks(int nch) { return kXcdChunks ? (nch + kXcd - 1) / kXcd * kXcd : nch; }
+
 template <typename T, int P, int KIND, int NK>
-__global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(TrArgs<T> a) {
+__global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(TrArgs<T> a, int nch) {
   __shared__ __align__(16) T sB[TrLayout<P, T>::elems];
   __shared__ int sT[64 / NK], sS[64 / NK];
-  translate_chunk<T, P, KIND, NK>(blockIdx.x, sB, sT, sS, a);
+  const int c = xcd_chunk(blockIdx.x, nch);
+  if (c >= nch) return;
+  translate_chunk<T, P, KIND, NK>(c, sB, sT, sS, a);
 }
 
 // Evaluation of both kernels' expansions at a point: M2P
@@ -972,15 +987,18 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate_side(int nch, Tr
   constexpr size_t SB = SPMV == 0 ? kSpmvTile * sizeof(E) : 0;
   __shared__ __align__(16) unsigned char smem[TB > SB ? TB : SB];
   __shared__ int sT[64], sS[64];
-  if ((int)blockIdx.x < nch) {
-    translate_chunk<T, P, KIND, 1>(blockIdx.x, reinterpret_cast<T *>(smem), sT, sS, tr);
+  const int npad = kXcdChunks ? (nch + kXcd - 1) / kXcd * kXcd : nch;  // xcd_blocks(nch)
+  if ((int)blockIdx.x < npad) {
+    const int c = xcd_chunk(blockIdx.x, nch);
+    if (c < nch) translate_chunk<T, P, KIND, 1>(c, reinterpret_cast<T *>(smem), sT, sS, tr);
     return;
   }
-  int b = (int)blockIdx.x - nch;
+  int b = (int)blockIdx.x - npad;
   if constexpr (SPMV >= 0) {
     if (b < sd.spmv_blocks) {
       if constexpr (SPMV == 0) {
-        // leaves dealt out per XCD as in k_spmv_lds (b and blockIdx.x differ by nch)
+        // leaves dealt out per XCD as in k_spmv_lds (b and blockIdx.x differ by a
+        // multiple of 8, so b % 8 is the XCD)
         const int leaf = (b % kXcd) * sd.per_xcd + b / kXcd;
         if (leaf < sd.count) spmv_lds_leaf<T, 1, NT>(sd.lo + leaf, reinterpret_cast<E *>(smem), sd.near);
       } else {
@@ -1631,7 +1649,8 @@ static void launch_translate(Engine &e, hipStream_t st, const ChunkDev &cl, int 
                      ops, IN, OUT};
 #define PBFMM_TR(PP)                                                                                         \
   case PP:                                                                                                   \
-    hipLaunchKernelGGL((k_translate<T, PP, KIND, NK>), dim3(n), dim3(TrLayout<PP, T>::NT), 0, st, tr);        \
+    hipLaunchKernelGGL((k_translate<T, PP, KIND, NK>), dim3(xcd_blocks(n)), dim3(TrLayout<PP, T>::NT), 0, st, tr, \
+                       n);                                                                                   \
     break;
   switch (e.p) {
     PBFMM_TR(1) PBFMM_TR(2) PBFMM_TR(3) PBFMM_TR(4) PBFMM_TR(5)
@@ -1665,7 +1684,7 @@ static void launch_translate_side(Engine &e, const ChunkDev &cl, int c0, int c1,
     constexpr int NT = TrLayout<PP, T>::NT;                                                                   \
     sd.spmv_blocks = spmv_side_blocks<SPMV>(sd, NT);                                                          \
     const int64_t mb = M2P ? (sd.m2p_hi - sd.m2p_lo + NT - 1) / NT : 0;                                      \
-    const int64_t nb = nch + sd.spmv_blocks + mb;                                                             \
+    const int64_t nb = xcd_blocks(nch) + sd.spmv_blocks + mb;                                                 \
     if (nb > 0)                                                                                               \
       hipLaunchKernelGGL((k_translate_side<T, PP, KIND, SPMV, M2P>), dim3((unsigned)nb), dim3(NT), 0, e.stream, \
                          nch, tr, sd);                                                                        \
