@@ -2587,8 +2587,11 @@ static void seg_norms_ctx(Engine &e, const SegCtx &c, const double *v, std::vect
   for (double &x : out) x = std::sqrt(x);
 }
 
+// r0 (device) with per-part norms r0n, when given, is the residual b - A x
+// of the starting x: the first cycle then skips its operator application.
 static int gmres_core_seg(Engine &e, const SegCtx &c, const double *b, double *x, const std::vector<double> &tol_abs,
-                          int m, int maxit, int &it, const std::vector<double> *sw = nullptr) {
+                          int m, int maxit, int &it, const std::vector<double> *sw = nullptr,
+                          const double *r0 = nullptr, const std::vector<double> *r0n = nullptr) {
   const int64_t n = c.N;
   const int S = c.S, nch = c.nch;
   const unsigned g = nblk(n, 256);
@@ -2618,9 +2621,16 @@ static int gmres_core_seg(Engine &e, const SegCtx &c, const double *b, double *x
     return true;
   };
   while (it < maxit) {
-    (sw ? c.op64 : c.op)(x, w);
-    hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, b, -1.0, w);
-    seg_norms_ctx(e, c, w, beta);
+    const double *rv = w;
+    if (r0) {
+      rv = r0;
+      beta = *r0n;
+      r0 = nullptr;
+    } else {
+      (sw ? c.op64 : c.op)(x, w);
+      hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, b, -1.0, w);
+      seg_norms_ctx(e, c, w, beta);
+    }
     int nact = 0;
     for (int s = 0; s < S; ++s) {
       active[s] = beta[s] > tol_abs[s];
@@ -2635,7 +2645,7 @@ static int gmres_core_seg(Engine &e, const SegCtx &c, const double *b, double *x
     if (nact == 0) return 0;
     HIPCHK(hipMemcpyAsync(c.segv, fsc.data(), (size_t)S * 8, hipMemcpyHostToDevice, e.stream));
     HIPCHK(hipMemcpyAsync(act_d, active.data(), (size_t)S, hipMemcpyHostToDevice, e.stream));
-    hipLaunchKernelGGL(k_seg_scale, dim3(g), dim3(256), 0, e.stream, n, seg, c.segv, w, V);
+    hipLaunchKernelGGL(k_seg_scale, dim3(g), dim3(256), 0, e.stream, n, seg, c.segv, rv, V);
     e.sync();
     int k = 0, k_done = 0, k_sync = 0;
     for (; k < m && it < maxit; ++k) {
@@ -2755,13 +2765,23 @@ static int gmres_seg_outer(Engine &e, const SegCtx &c, const double *b, double *
                            int restart, int maxit, bool mixed, int *iters, double *relres) {
   const int S = c.S;
   const unsigned g = nblk(c.N, 256);
-  std::vector<double> bn, rn, tin(S), rr(S, 1.0);
+  std::vector<double> bn, rn, xn, tin(S), rr(S, 1.0);
   seg_norms_ctx(e, c, b, bn);
+  // a zero starting guess needs no operator application for its residual
+  seg_norms_ctx(e, c, x, xn);
+  bool x_zero = true;
+  for (double v : xn) x_zero = x_zero && v == 0.0;
   int it = 0;
   for (int outer = 0; outer < 100 && it < maxit; ++outer) {
-    c.op64(x, r);
-    hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, c.N, 1.0, b, -1.0, r);
-    seg_norms_ctx(e, c, r, rn);
+    if (x_zero) {
+      HIPCHK(hipMemcpyAsync(r, b, (size_t)c.N * 8, hipMemcpyDeviceToDevice, e.stream));
+      rn = bn;
+      x_zero = false;
+    } else {
+      c.op64(x, r);
+      hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, c.N, 1.0, b, -1.0, r);
+      seg_norms_ctx(e, c, r, rn);
+    }
     bool ok = true;
     for (int s = 0; s < S; ++s) {
       rr[s] = bn[s] > 0 ? rn[s] / bn[s] : rn[s];
@@ -2781,11 +2801,11 @@ static int gmres_seg_outer(Engine &e, const SegCtx &c, const double *b, double *
         sw[s] = std::min(rn[s], kRelaxSwitch * tol * bn[s]);
       }
       hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, e.stream, d, c.N, 0.0);
-      gmres_core_seg(e, c, r, d, tin, restart, maxit, it, relaxed ? &sw : nullptr);
+      gmres_core_seg(e, c, r, d, tin, restart, maxit, it, relaxed ? &sw : nullptr, r, &rn);
       hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, c.N, 1.0, d, 1.0, x);
     } else {
       for (int s = 0; s < S; ++s) tin[s] = rr[s] <= tol ? 2.0 * rn[s] : tol * bn[s] * 0.5;
-      gmres_core_seg(e, c, b, x, tin, restart, maxit, it);
+      gmres_core_seg(e, c, b, x, tin, restart, maxit, it, nullptr, r, &rn);
     }
   }
   e.sync();
