@@ -811,9 +811,14 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(TrArgs<T> a, int
 // NK = 1: the single expansion E (v2 unused).  E is level-scaled by the box
 // size sc of its block (ME_n / sc^n, LE_n sc^n), so the radial factors are
 // (sc/rho)^n / rho and (rho/sc)^n.
-template <typename T, bool LOCAL, int NK>
-__device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, int kp, T d0, T d1, T d2, T sc,
-                                      T &v1, T &v2) {
+// P > 0 fixes the order at compile time (loops unrolled, the expansion's
+// loads scheduled ahead of the recurrences); P = 0 takes p, kt, kp at run time.
+template <typename T, bool LOCAL, int NK, int P = 0>
+__device__ __forceinline__ void eval2(const T *__restrict__ E, int p_rt, int kt_rt, int kp_rt, T d0, T d1, T d2,
+                                      T sc, T &v1, T &v2) {
+  const int p = P > 0 ? P : p_rt;
+  const int kt = P > 0 ? (P + 1) * (P + 2) / 2 : kt_rt;
+  const int kp = P > 0 ? Kp<P>::value : kp_rt;
   // spherical angles of d without trigonometric calls: cos(alpha) = z/rho,
   // sin(alpha) = rho_xy/rho, e^{i beta} = (x + i y)/rho_xy (beta = 0 on the axis)
   const T rxy2 = d0 * d0 + d1 * d1;
@@ -827,6 +832,7 @@ __device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, in
   T pmm = 1, cm = 1, sm = 0;
   v1 = 0;
   v2 = 0;
+#pragma unroll
   for (int m = 0; m <= p; ++m) {
     if (m > 0) {
       pmm *= -PMF<T>(m) * s;
@@ -837,6 +843,7 @@ __device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, in
     }
     T pp = 0, pc = pmm, r = rstart;
     const T f2 = m ? T(2) : T(1);
+#pragma unroll
     for (int n = m; n <= p; ++n) {
       if (n > m) {
         const T pn = RA<T>(n, m) * x * pc - RB<T>(n, m) * pp;
@@ -854,7 +861,7 @@ __device__ __forceinline__ void eval2(const T *__restrict__ E, int p, int kt, in
 
 // M2P for smaller far blocks (bem_fma.cc:2664-2704): one thread per
 // (source block, target node) evaluation, listed at set-up.
-template <typename T, int NK>
+template <typename T, int NK, int P = 0>
 __device__ __forceinline__ void m2p_item(int64_t w, const int *__restrict__ item_node,
                                          const int *__restrict__ item_src, const double *__restrict__ xyz,
                                          const double *__restrict__ pmin, const double *__restrict__ delta, int p,
@@ -866,12 +873,12 @@ __device__ __forceinline__ void m2p_item(int64_t w, const int *__restrict__ item
   const T d1 = (T)(xyz[3 * node + 1] - (pmin[3 * sblk + 1] + hd));
   const T d2 = (T)(xyz[3 * node + 2] - (pmin[3 * sblk + 2] + hd));
   T v1, v2;
-  eval2<T, false, NK>(ME + (int64_t)sblk * NK * kp, p, kt, kp, d0, d1, d2, (T)delta[sblk], v1, v2);
+  eval2<T, false, NK, P>(ME + (int64_t)sblk * NK * kp, p, kt, kp, d0, d1, d2, (T)delta[sblk], v1, v2);
   atomicAdd(outN + node, (double)v1);
   if constexpr (NK == 2) atomicAdd(outD + node, (double)v2);
 }
 
-template <typename T, int NK>
+template <typename T, int NK, int P>
 __global__ void __launch_bounds__(256) k_m2p(int64_t nitems, const int *__restrict__ item_node,
                                              const int *__restrict__ item_src, const double *__restrict__ xyz,
                                              const double *__restrict__ pmin, const double *__restrict__ delta, int p,
@@ -879,11 +886,11 @@ __global__ void __launch_bounds__(256) k_m2p(int64_t nitems, const int *__restri
                                              double *__restrict__ outD) {
   const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (w >= nitems) return;
-  m2p_item<T, NK>(w, item_node, item_src, xyz, pmin, delta, p, kt, kp, ME, outN, outD);
+  m2p_item<T, NK, P>(w, item_node, item_src, xyz, pmin, delta, p, kt, kp, ME, outN, outD);
 }
 
 // L2P at every node of every leaf (bem_fma.cc:2959-2986): one thread per node.
-template <typename T, int NK>
+template <typename T, int NK, int P>
 __global__ void __launch_bounds__(256) k_l2p(int nrows, const int *__restrict__ rows,
                                              const int *__restrict__ row_leaf, const int *__restrict__ leaf_block,
                                              const double *__restrict__ xyz, const double *__restrict__ pmin,
@@ -899,7 +906,7 @@ __global__ void __launch_bounds__(256) k_l2p(int nrows, const int *__restrict__ 
   const T d1 = (T)(xyz[3 * node + 1] - (pmin[3 * b + 1] + hd));
   const T d2 = (T)(xyz[3 * node + 2] - (pmin[3 * b + 2] + hd));
   T v1, v2;
-  eval2<T, true, NK>(LE + (int64_t)b * NK * kp, p, kt, kp, d0, d1, d2, (T)delta[b], v1, v2);
+  eval2<T, true, NK, P>(LE + (int64_t)b * NK * kp, p, kt, kp, d0, d1, d2, (T)delta[b], v1, v2);
   outN[node] += (double)v1;
   if constexpr (NK == 2) outD[node] += (double)v2;
 }
@@ -907,7 +914,7 @@ __global__ void __launch_bounds__(256) k_l2p(int nrows, const int *__restrict__ 
 // L2P of the operator pipeline with BEMProblem::vmult's epilogue
 // (k_vmult_post) folded in: dst = out + m2p + L2P + alpha x on Neumann DoFs
 // (out the near field, m2p the M2P sums).
-template <typename T>
+template <typename T, int P>
 __global__ void __launch_bounds__(256) k_l2p_post(int nrows, const int *__restrict__ rows,
                                                   const int *__restrict__ row_leaf, const int *__restrict__ leaf_block,
                                                   const double *__restrict__ xyz, const double *__restrict__ pmin,
@@ -925,7 +932,7 @@ __global__ void __launch_bounds__(256) k_l2p_post(int nrows, const int *__restri
   const T d1 = (T)(xyz[3 * node + 1] - (pmin[3 * b + 1] + hd));
   const T d2 = (T)(xyz[3 * node + 2] - (pmin[3 * b + 2] + hd));
   T v1, v2;
-  eval2<T, true, 1>(LE + (int64_t)b * kp, p, kt, kp, d0, d1, d2, (T)delta[b], v1, v2);
+  eval2<T, true, 1, P>(LE + (int64_t)b * kp, p, kt, kp, d0, d1, d2, (T)delta[b], v1, v2);
   const double nd = (out[node] + m2p[node]) + (double)v1;
   dst[node] = nd + (dir[node] ? 0.0 : alpha[node] * x[node]);
 }
@@ -1012,7 +1019,7 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate_side(int nch, Tr
   if constexpr (M2P) {
     const int64_t w = sd.m2p_lo + (int64_t)b * NT + threadIdx.x;
     if (w < sd.m2p_hi)
-      m2p_item<T, 1>(w, sd.item_node, sd.item_src, sd.xyz, sd.pmin, sd.delta, sd.p, sd.kt, sd.kp, sd.ME,
+      m2p_item<T, 1, P>(w, sd.item_node, sd.item_src, sd.xyz, sd.pmin, sd.delta, sd.p, sd.kt, sd.kp, sd.ME,
                      sd.m2p_out, nullptr);
   }
 }
@@ -1637,6 +1644,22 @@ FarDev<double> &far_of<double>(Engine &e) { return e.f64; }
 template <>
 FarDev<float> &far_of<float>(Engine &e) { return e.f32; }
 
+// Runs STMT with PP the truncation order p (1..10) as a compile-time constant.
+#define PBFMM_BY_ORDER(p, STMT)                                              \
+  switch (p) {                                                               \
+    case 1: { constexpr int PP = 1; STMT; } break;                           \
+    case 2: { constexpr int PP = 2; STMT; } break;                           \
+    case 3: { constexpr int PP = 3; STMT; } break;                           \
+    case 4: { constexpr int PP = 4; STMT; } break;                           \
+    case 5: { constexpr int PP = 5; STMT; } break;                           \
+    case 6: { constexpr int PP = 6; STMT; } break;                           \
+    case 7: { constexpr int PP = 7; STMT; } break;                           \
+    case 8: { constexpr int PP = 8; STMT; } break;                           \
+    case 9: { constexpr int PP = 9; STMT; } break;                           \
+    case 10: { constexpr int PP = 10; STMT; } break;                         \
+    default: throw std::runtime_error("trunc_order outside 1..10");          \
+  }
+
 // OUT[tgt] += ops[class] . IN[src] for the chunks [c0, c1) of a chunk list
 // (chunks of 32 pairs for NK = 2, of 64 pairs for NK = 1).
 template <typename T, int KIND, int NK>
@@ -1797,18 +1820,18 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
   mark(6);
   const int64_t nitems = (int64_t)far.m2p_item_node.size();
   if (nitems > 0)
-    hipLaunchKernelGGL((k_m2p<T, NK>), dim3((unsigned)nblk(nitems, 256)), dim3(256), 0, stream, nitems,
+    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_m2p<T, NK, PP>), dim3((unsigned)nblk(nitems, 256)), dim3(256), 0, stream, nitems,
                        d_m2p_node.as<int>(), d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), p, kt, kp, F.me.template as<T>(), outN, outD);
+                       d_delta.as<double>(), p, kt, kp, F.me.template as<T>(), outN, outD));
   mark(7);
   for (int l = 2; l <= L; ++l)
     launch_translate<T, 2, NK>(*this, stream, l2l_cl, lvl<NK>(far.l2l, l), lvl<NK>(far.l2l, l + 1),
                                F.l2l.template as<T>(), F.le.template as<T>(), F.le.template as<T>());
   mark(8);
   if (nrows > 0)
-    hipLaunchKernelGGL((k_l2p<T, NK>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
+    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_l2p<T, NK, PP>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
                        d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), p, kt, kp, F.le.template as<T>(), outN, outD);
+                       d_delta.as<double>(), p, kt, kp, F.le.template as<T>(), outN, outD));
   mark(9);
   HIPCHK(hipGetLastError());
 }
@@ -1843,17 +1866,17 @@ void Engine::matvec(const double *phi, const double *dphi, double *outN, double 
   launch_translate<T, 1, 2>(*this, stream, m2l_cl, 0, nchunks<2>(far.m2l), F.m2l.template as<T>(), me, le);
   const int64_t ni = (int64_t)far.m2p_item_node.size();
   if (ni > 0)
-    hipLaunchKernelGGL((k_m2p<T, 2>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, stream, ni, d_m2p_node.as<int>(),
+    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_m2p<T, 2, PP>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, stream, ni, d_m2p_node.as<int>(),
                        d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt,
-                       kp, me, outN, outD);
+                       kp, me, outN, outD));
   for (int l = std::max(2, le_lo + 1); l <= L; ++l)
     launch_translate<T, 2, 2>(*this, stream, l2l_cl, lvl<2>(far.l2l, l), lvl<2>(far.l2l, l + 1),
                               F.l2l.template as<T>(), le, le);
   const int nrows = (int)near.rows.size();
   if (nrows > 0)
-    hipLaunchKernelGGL((k_l2p<T, 2>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
+    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_l2p<T, 2, PP>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
                        d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), p, kt, kp, le, outN, outD);
+                       d_delta.as<double>(), p, kt, kp, le, outN, outD));
   HIPCHK(hipGetLastError());
 }
 
@@ -1985,15 +2008,15 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
     m2p_done = true;
   }
   if (!m2p_done)
-    hipLaunchKernelGGL((k_m2p<T, 1>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, stream, ni, d_m2p_node.as<int>(),
+    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_m2p<T, 1, PP>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, stream, ni, d_m2p_node.as<int>(),
                        d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt,
-                       kp, me, acc, nullptr);
+                       kp, me, acc, nullptr));
   const int nrows = (int)near.rows.size();
   if (nrows > 0)
-    hipLaunchKernelGGL((k_l2p_post<T>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
+    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_l2p_post<T, PP>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
                        d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
                        d_delta.as<double>(), p, kt, kp, le, out, acc, d_alpha.as<double>(), d_dir.as<uint8_t>(), x,
-                       dst);
+                       dst));
   HIPCHK(hipGetLastError());
 }
 
@@ -2959,16 +2982,16 @@ void Engine::matvec_combined_multi(const double *x, double *out, int R) {
                      ne, T(0));
   launch_translate<T, 1, 1>(*this, stream, mr_m2l, 0, nchunks<1>(mr_m2l_h), F.m2l.template as<T>(), ME, LE);
   for (int r = 0; r < R && nitems > 0; ++r)
-    hipLaunchKernelGGL((k_m2p<T, 1>), dim3((unsigned)nblk(nitems, 256)), dim3(256), 0, stream, nitems,
+    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_m2p<T, 1, PP>), dim3((unsigned)nblk(nitems, 256)), dim3(256), 0, stream, nitems,
                        d_m2p_node.as<int>(), d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), p, kt, kp, ME + r * sb, out + (int64_t)r * n, nullptr);
+                       d_delta.as<double>(), p, kt, kp, ME + r * sb, out + (int64_t)r * n, nullptr));
   for (int l = std::max(2, le_lo + 1); l <= L; ++l)
     launch_translate<T, 2, 1>(*this, stream, mr_l2l, lvl<1>(mr_l2l_h, l), lvl<1>(mr_l2l_h, l + 1),
                               F.l2l.template as<T>(), LE, LE);
   for (int r = 0; r < R && nrows > 0; ++r)
-    hipLaunchKernelGGL((k_l2p<T, 1>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
+    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_l2p<T, 1, PP>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
                        d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), p, kt, kp, LE + r * sb, out + (int64_t)r * n, nullptr);
+                       d_delta.as<double>(), p, kt, kp, LE + r * sb, out + (int64_t)r * n, nullptr));
   HIPCHK(hipGetLastError());
 }
 
