@@ -2043,7 +2043,7 @@ double Engine::norm2(const double *v) {
 // the SpMV and M2P are short beside them (tens of thousands of DoFs); on
 // large trees the fused workgroups only crowd each other.  The modes are
 // matvec_fused's side 0..3.  Timed once per precision, on the first operator
-// application: five applications each way on scratch vectors, after one
+// application: ten applications each way on scratch vectors, after one
 // warm-up; a fused mode is kept only if it beats the unfused one by 2%.
 int Engine::use_side_fusion(bool f32) {
   int &mode = side_fusion[f32 ? 1 : 0];
@@ -2057,13 +2057,13 @@ int Engine::use_side_fusion(bool f32) {
   HIPCHK(hipEventCreate(&t1));
   float ms[kSideModes] = {};
   for (int s = 0; s < kSideModes; ++s)
-    for (int rep = 0; rep < 6; ++rep) {
+    for (int rep = 0; rep < 11; ++rep) {
       if (rep == 1) HIPCHK(hipEventRecord(t0, stream));
       if (f32)
         matvec_fused<float>(u, d_outN.as<double>(), v, s);
       else
         matvec_fused<double>(u, d_outN.as<double>(), v, s);
-      if (rep == 5) {
+      if (rep == 10) {
         HIPCHK(hipEventRecord(t1, stream));
         HIPCHK(hipEventSynchronize(t1));
         HIPCHK(hipEventElapsedTime(&ms[s], t0, t1));

@@ -386,3 +386,41 @@ def test_matvec_dev_fast_path():
     print(f"\n  N={n}: fp64 fast vs reference API {e64:.2e}, fp32 fast vs phase-timed {e32:.2e}")
     assert e64 <= 1e-14 and e32 <= 1e-6
     f.close()
+
+
+@pytest.mark.parametrize("name", ["batch", "double_body"])
+def test_fused_operator_modes_on_batches_and_images(name):
+    """Every side-fusion mode of the operator against its phase-timed run on
+    a batch handle (two problems, one block-diagonal operator) and on a
+    double body (mirror cells in the tree): FP64 within 1e-14, FP32 within
+    1e-6."""
+    import dataclasses
+    from pibem_amd import hip
+    if name == "batch":
+        probs = [M.config_sphere(8, 5), M.config_dcube(4, 5)]
+        f = P.setup_batch(probs, P.PBFMM_MIXED)
+    else:
+        pr = M.config_sphere(12, 6)
+        m = dataclasses.replace(pr.mesh, support_xyz=pr.mesh.support_xyz + np.array([0.3, -0.2, -2.5]))
+        sol = M.PointSources([[0.3, -0.2, 1.0]], [1.0])
+        f = P.setup(M.Problem(m, pr.dirichlet, sol, M.bc_vector(m, pr.dirichlet, sol), 6, mirror_z0=0.0),
+                    P.PBFMM_MIXED)
+    bp = P.BEMProblem(f)
+    bp.compute_alpha()
+    n = f.n
+    src = hip.DeviceArray.from_numpy(np.random.default_rng(3).uniform(-1, 1, n))
+    dst = hip.DeviceArray(n)
+    for prec, bar in ((P.PBFMM_FP64, 1e-14), (P.PBFMM_MIXED, 1e-6)):
+        f.set_phase_timing(True)
+        bp.vmult_dev(src.value, dst.value, prec)
+        hip.synchronize()
+        y0 = dst.numpy()
+        f.set_phase_timing(False)
+        for mode in (0, 1, 2, 3):
+            f.set_side_fusion(prec, mode)
+            bp.vmult_dev(src.value, dst.value, prec)
+            hip.synchronize()
+            e = rel(dst.numpy(), y0)
+            print(f"\n  {name} N={n} precision={prec} side {mode}: {e:.2e}", end="")
+            assert e <= bar
+    f.close()
