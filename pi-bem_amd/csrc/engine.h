@@ -155,7 +155,7 @@ struct Engine {
   template <typename T, int NK>
   void tune_spmv();
   template <typename T, int NK>
-  void launch_spmv(int variant, const double *phi, const double *dphi, double *outN, double *outD);
+  void launch_spmv(int variant, const double *phi, const double *dphi, double *outN, double *outD, int R = 1);
   template <typename T, int NK>
   void generate(const double *phi, const double *dphi);
   template <typename T, int NK>

@@ -314,7 +314,18 @@ struct NearArgs {
   const T *vals;
   const double *phi, *dphi;
   double *outN, *outD;
+  int64_t stride;  // between right-hand sides (blockIdx.y) of the vectors
 };
+// The vectors of right-hand side blockIdx.y.
+template <typename T>
+__device__ __forceinline__ NearArgs<T> near_rhs(NearArgs<T> a) {
+  const int64_t o = (int64_t)blockIdx.y * a.stride;
+  a.phi += o;
+  if (a.dphi) a.dphi += o;
+  a.outN += o;
+  if (a.outD) a.outD += o;
+  return a;
+}
 
 // One leaf by a workgroup of NT threads: its inputs gathered once into LDS
 // (xs, kSpmvTile entries), each wavefront streaming whole rows.
@@ -354,7 +365,7 @@ __global__ void __launch_bounds__(256) k_spmv_lds(int nleaf, int per_xcd, NearAr
   __shared__ E xs[kSpmvTile];
   const int leaf = (int)(blockIdx.x % kXcd) * per_xcd + (int)(blockIdx.x / kXcd);
   if (leaf >= nleaf) return;
-  spmv_lds_leaf<T, NK, 256>(leaf, xs, a);
+  spmv_lds_leaf<T, NK, 256>(leaf, xs, near_rhs(a));
 }
 
 // One leaf by one wavefront, no LDS and no barrier, RPI rows per iteration.
@@ -414,7 +425,7 @@ template <typename T, int NK, int RPI>
 __global__ void __launch_bounds__(256) k_spmv_wave(int nleaf, NearArgs<T> a) {
   const int leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (leaf >= nleaf) return;
-  spmv_wave_leaf<T, NK, RPI>(leaf, threadIdx.x & 63, a);
+  spmv_wave_leaf<T, NK, RPI>(leaf, threadIdx.x & 63, near_rhs(a));
 }
 
 // ---------------------------------------------------------------------------
@@ -522,11 +533,16 @@ __device__ __forceinline__ void add16(double2 &a, const double2 &u) { a.x += u.x
 // expansions from (phi, dphi).  NK = 1: U is the operator's [column][kp]
 // (U1 on Neumann DoFs, -U2 on Dirichlet DoFs) and the leaf gets one combined
 // expansion from phi (dphi unused).  ME is [leaf][NK kp].
+// Right-hand side blockIdx.y reads phi (dphi) + y vs and writes ME + y es.
 template <typename T, int NK>
 __global__ void __launch_bounds__(256) k_p2m(int nleaf, const int *__restrict__ leaves,
                                              const int *__restrict__ col_beg, const int *__restrict__ col_dof,
                                              const T *__restrict__ U, int kp, const double *__restrict__ phi,
-                                             const double *__restrict__ dphi, T *__restrict__ ME) {
+                                             const double *__restrict__ dphi, T *__restrict__ ME, int64_t vs,
+                                             int64_t es) {
+  phi += (int64_t)blockIdx.y * vs;
+  if (dphi) dphi += (int64_t)blockIdx.y * vs;
+  ME += (int64_t)blockIdx.y * es;
   using T2 = typename Vec2<T>::type;
   using TV = typename Vec16<T>::type;
   constexpr int VW = 16 / sizeof(T);
@@ -878,15 +894,18 @@ __device__ __forceinline__ void m2p_item(int64_t w, const int *__restrict__ item
   if constexpr (NK == 2) atomicAdd(outD + node, (double)v2);
 }
 
+// Right-hand side blockIdx.y reads ME + y es and adds into outN (outD) + y os.
 template <typename T, int NK, int P>
 __global__ void __launch_bounds__(256) k_m2p(int64_t nitems, const int *__restrict__ item_node,
                                              const int *__restrict__ item_src, const double *__restrict__ xyz,
                                              const double *__restrict__ pmin, const double *__restrict__ delta, int p,
                                              int kt, int kp, const T *__restrict__ ME, double *__restrict__ outN,
-                                             double *__restrict__ outD) {
+                                             double *__restrict__ outD, int64_t es, int64_t os) {
   const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (w >= nitems) return;
-  m2p_item<T, NK, P>(w, item_node, item_src, xyz, pmin, delta, p, kt, kp, ME, outN, outD);
+  const int64_t y = blockIdx.y;
+  m2p_item<T, NK, P>(w, item_node, item_src, xyz, pmin, delta, p, kt, kp, ME + y * es, outN + y * os,
+                     outD ? outD + y * os : nullptr);
 }
 
 // L2P at every node of every leaf (bem_fma.cc:2959-2986): one thread per node.
@@ -896,9 +915,13 @@ __global__ void __launch_bounds__(256) k_l2p(int nrows, const int *__restrict__ 
                                              const double *__restrict__ xyz, const double *__restrict__ pmin,
                                              const double *__restrict__ delta, int p, int kt, int kp,
                                              const T *__restrict__ LE, double *__restrict__ outN,
-                                             double *__restrict__ outD) {
+                                             double *__restrict__ outD, int64_t es, int64_t os) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
+  // right-hand side blockIdx.y: LE + y es, outN (outD) + y os
+  LE += (int64_t)blockIdx.y * es;
+  outN += (int64_t)blockIdx.y * os;
+  if (outD) outD += (int64_t)blockIdx.y * os;
   const int node = rows[r];
   const int b = leaf_block[row_leaf[r]];
   const double hd = delta[b] * 0.5;
@@ -1725,19 +1748,20 @@ template <typename T, int NK>
 static NearArgs<T> near_args(Engine &e, const double *phi, const double *dphi, double *outN, double *outD) {
   const T *vals = (NK == 2 ? far_of<T>(e).vals : far_of<T>(e).opvals).template as<T>();
   return NearArgs<T>{e.d_row_beg.as<int>(), e.d_rows.as<int>(), e.d_col_beg.as<int>(), e.d_cols.as<int>(),
-                     e.d_val_beg.as<int64_t>(), vals, phi, dphi, outN, outD};
+                     e.d_val_beg.as<int64_t>(), vals, phi, dphi, outN, outD, 0};
 }
 
 template <typename T, int NK>
-void Engine::launch_spmv(int variant, const double *phi, const double *dphi, double *outN, double *outD) {
-  const NearArgs<T> a = near_args<T, NK>(*this, phi, dphi, outN, outD);
+void Engine::launch_spmv(int variant, const double *phi, const double *dphi, double *outN, double *outD, int R) {
+  NearArgs<T> a = near_args<T, NK>(*this, phi, dphi, outN, outD);
+  a.stride = n;  // R right-hand sides, n apart (blockIdx.y)
   if (variant == 0) {
     const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
-    hipLaunchKernelGGL((k_spmv_lds<T, NK>), dim3(per_xcd * kXcd), dim3(256), 0, stream, near.nleaf, per_xcd, a);
+    hipLaunchKernelGGL((k_spmv_lds<T, NK>), dim3(per_xcd * kXcd, R), dim3(256), 0, stream, near.nleaf, per_xcd, a);
   } else if (variant == 1) {
-    hipLaunchKernelGGL((k_spmv_wave<T, NK, 2>), dim3((near.nleaf + 3) / 4), dim3(256), 0, stream, near.nleaf, a);
+    hipLaunchKernelGGL((k_spmv_wave<T, NK, 2>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, stream, near.nleaf, a);
   } else {
-    hipLaunchKernelGGL((k_spmv_wave<T, NK, 4>), dim3((near.nleaf + 3) / 4), dim3(256), 0, stream, near.nleaf, a);
+    hipLaunchKernelGGL((k_spmv_wave<T, NK, 4>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, stream, near.nleaf, a);
   }
 }
 
@@ -1793,7 +1817,8 @@ void Engine::generate(const double *phi, const double *dphi) {
   if (nleaf > 0)
     hipLaunchKernelGGL((k_p2m<T, NK>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
                        d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(),
-                       (NK == 2 ? F.unit : F.opunit).template as<T>(), kp, phi, dphi, F.me.template as<T>());
+                       (NK == 2 ? F.unit : F.opunit).template as<T>(), kp, phi, dphi, F.me.template as<T>(),
+                       (int64_t)0, (int64_t)0);
   mark(1);
   for (int l = L; l >= 1; --l)
     launch_translate<T, 0, NK>(*this, stream, m2m_cl, lvl<NK>(far.m2m, l), lvl<NK>(far.m2m, l + 1),
@@ -1822,7 +1847,7 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
   if (nitems > 0)
     PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_m2p<T, NK, PP>), dim3((unsigned)nblk(nitems, 256)), dim3(256), 0, stream, nitems,
                        d_m2p_node.as<int>(), d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), p, kt, kp, F.me.template as<T>(), outN, outD));
+                       d_delta.as<double>(), p, kt, kp, F.me.template as<T>(), outN, outD, (int64_t)0, (int64_t)0));
   mark(7);
   for (int l = 2; l <= L; ++l)
     launch_translate<T, 2, NK>(*this, stream, l2l_cl, lvl<NK>(far.l2l, l), lvl<NK>(far.l2l, l + 1),
@@ -1831,7 +1856,7 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
   if (nrows > 0)
     PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_l2p<T, NK, PP>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
                        d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), p, kt, kp, F.le.template as<T>(), outN, outD));
+                       d_delta.as<double>(), p, kt, kp, F.le.template as<T>(), outN, outD, (int64_t)0, (int64_t)0));
   mark(9);
   HIPCHK(hipGetLastError());
 }
@@ -1857,7 +1882,8 @@ void Engine::matvec(const double *phi, const double *dphi, double *outN, double 
   const int nleaf = (int)far.p2m_leaf.size();
   if (nleaf > 0)
     hipLaunchKernelGGL((k_p2m<T, 2>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
-                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.unit.template as<T>(), kp, phi, dphi, me);
+                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.unit.template as<T>(), kp, phi, dphi, me,
+                       (int64_t)0, (int64_t)0);
   if (me_lo < 0) consumer_levels();
   for (int l = L; l > me_lo; --l)
     launch_translate<T, 0, 2>(*this, stream, m2m_cl, lvl<2>(far.m2m, l), lvl<2>(far.m2m, l + 1),
@@ -1868,7 +1894,7 @@ void Engine::matvec(const double *phi, const double *dphi, double *outN, double 
   if (ni > 0)
     PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_m2p<T, 2, PP>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, stream, ni, d_m2p_node.as<int>(),
                        d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt,
-                       kp, me, outN, outD));
+                       kp, me, outN, outD, (int64_t)0, (int64_t)0));
   for (int l = std::max(2, le_lo + 1); l <= L; ++l)
     launch_translate<T, 2, 2>(*this, stream, l2l_cl, lvl<2>(far.l2l, l), lvl<2>(far.l2l, l + 1),
                               F.l2l.template as<T>(), le, le);
@@ -1876,7 +1902,7 @@ void Engine::matvec(const double *phi, const double *dphi, double *outN, double 
   if (nrows > 0)
     PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_l2p<T, 2, PP>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
                        d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), p, kt, kp, le, outN, outD));
+                       d_delta.as<double>(), p, kt, kp, le, outN, outD, (int64_t)0, (int64_t)0));
   HIPCHK(hipGetLastError());
 }
 
@@ -1929,7 +1955,8 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
                      0, stream, me, le, ne, acc, n);
   if (nleaf > 0)
     hipLaunchKernelGGL((k_p2m<T, 1>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
-                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp, x, nullptr, me);
+                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp, x, nullptr, me,
+                       (int64_t)0, (int64_t)0);
   if (me_lo < 0) consumer_levels();
   const int var = spmv_variant[sizeof(T) == 4 ? 1 : 0][0];
   const int nl = near.nleaf;
@@ -2010,7 +2037,7 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   if (!m2p_done)
     PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_m2p<T, 1, PP>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, stream, ni, d_m2p_node.as<int>(),
                        d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt,
-                       kp, me, acc, nullptr));
+                       kp, me, acc, nullptr, (int64_t)0, (int64_t)0));
   const int nrows = (int)near.rows.size();
   if (nrows > 0)
     PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_l2p_post<T, PP>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
@@ -2966,32 +2993,32 @@ void Engine::matvec_combined_multi(const double *x, double *out, int R) {
   // upward pass
   hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(ne, 256), 4096)), dim3(256), 0, stream, ME,
                      ne, T(0));
-  for (int r = 0; r < R && nleaf > 0; ++r)
-    hipLaunchKernelGGL((k_p2m<T, 1>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
-                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp,
-                       x + (int64_t)r * n, nullptr, ME + r * sb);
+  if (nleaf > 0)  // every right-hand side in one launch (blockIdx.y)
+    hipLaunchKernelGGL((k_p2m<T, 1>), dim3(nleaf, R), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
+                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp, x, nullptr,
+                       ME, (int64_t)n, sb);
   if (me_lo < 0) consumer_levels();
   for (int l = L; l > me_lo; --l)
     launch_translate<T, 0, 1>(*this, stream, mr_m2m, lvl<1>(mr_m2m_h, l), lvl<1>(mr_m2m_h, l + 1),
                               F.m2m.template as<T>(), ME, ME);
   // near field, then the far field accumulated onto it
-  for (int r = 0; r < R && near.nleaf > 0; ++r)
-    launch_spmv<T, 1>(spmv_variant[sizeof(T) == 4 ? 1 : 0][0], x + (int64_t)r * n, nullptr, out + (int64_t)r * n,
-                      nullptr);
+  if (near.nleaf > 0) launch_spmv<T, 1>(spmv_variant[sizeof(T) == 4 ? 1 : 0][0], x, nullptr, out, nullptr, R);
   hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(ne, 256), 4096)), dim3(256), 0, stream, LE,
                      ne, T(0));
   launch_translate<T, 1, 1>(*this, stream, mr_m2l, 0, nchunks<1>(mr_m2l_h), F.m2l.template as<T>(), ME, LE);
-  for (int r = 0; r < R && nitems > 0; ++r)
-    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_m2p<T, 1, PP>), dim3((unsigned)nblk(nitems, 256)), dim3(256), 0, stream, nitems,
-                       d_m2p_node.as<int>(), d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), p, kt, kp, ME + r * sb, out + (int64_t)r * n, nullptr));
+  if (nitems > 0)  // every right-hand side in one launch (blockIdx.y)
+    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_m2p<T, 1, PP>), dim3((unsigned)nblk(nitems, 256), R), dim3(256), 0, stream,
+                                         nitems, d_m2p_node.as<int>(), d_m2p_isrc.as<int>(), d_xyz.as<double>(),
+                                         d_pmin.as<double>(), d_delta.as<double>(), p, kt, kp, ME, out, nullptr, sb,
+                                         (int64_t)n));
   for (int l = std::max(2, le_lo + 1); l <= L; ++l)
     launch_translate<T, 2, 1>(*this, stream, mr_l2l, lvl<1>(mr_l2l_h, l), lvl<1>(mr_l2l_h, l + 1),
                               F.l2l.template as<T>(), LE, LE);
-  for (int r = 0; r < R && nrows > 0; ++r)
-    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_l2p<T, 1, PP>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
-                       d_row_leaf.as<int>(), d_leaf_block.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), p, kt, kp, LE + r * sb, out + (int64_t)r * n, nullptr));
+  if (nrows > 0)  // every right-hand side in one launch (blockIdx.y)
+    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_l2p<T, 1, PP>), dim3(nblk(nrows, 256), R), dim3(256), 0, stream, nrows,
+                                         d_rows.as<int>(), d_row_leaf.as<int>(), d_leaf_block.as<int>(),
+                                         d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt, kp,
+                                         LE, out, nullptr, sb, (int64_t)n));
   HIPCHK(hipGetLastError());
 }
 
