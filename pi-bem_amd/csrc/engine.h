@@ -110,7 +110,8 @@ struct Engine {
   std::vector<int64_t> pinv_beg;
   DevBuf d_pinv_beg, d_pinv, d_ptmp, d_ptmp2;
   void build_preconditioner();
-  void apply_prec(const double *v, double *z);
+  void apply_prec(const double *v, double *z, int R = 1);  // R right-hand sides, n apart
+  DevBuf d_rhs_sum;  // per right-hand side sums of the lockstep operator's mean term
   // GMRES workspace
   DevBuf d_V, d_w, d_r, d_x, d_h;
   int gm_restart = 0;

@@ -1062,11 +1062,16 @@ __global__ void k_vmult_pre(int64_t n, const double *__restrict__ src, const uin
   x[i] = dir[i] ? src[i] : (src[i] - sh);
 }
 // ND = N.sp - D.sd from the operator's FMM, x its input
+// (right-hand side blockIdx.y: ND, x and dst + y stride)
 __global__ void k_vmult_post(int64_t n, const double *__restrict__ ND, const double *__restrict__ alpha,
                              const uint8_t *__restrict__ dir, const double *__restrict__ x,
-                             double *__restrict__ dst) {
+                             double *__restrict__ dst, int64_t stride) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int64_t o = (int64_t)blockIdx.y * stride;
+  ND += o;
+  x += o;
+  dst += o;
   dst[i] = ND[i] + (dir[i] ? 0.0 : alpha[i] * x[i]);
 }
 __global__ void k_rhs_pre(int64_t n, const double *__restrict__ g, const uint8_t *__restrict__ dir,
@@ -1087,15 +1092,20 @@ __global__ void k_shift(int64_t n, double *__restrict__ v, const double *__restr
   if (i < n) v[i] -= *s;
 }
 // v += (*sum) * scale
-__global__ void k_add_scaled_scalar(int64_t n, double *__restrict__ v, const double *__restrict__ sum, double scale) {
+// v += sum scale; right-hand side blockIdx.y: v + y stride, sum[y]
+__global__ void k_add_scaled_scalar(int64_t n, double *__restrict__ v, const double *__restrict__ sum, double scale,
+                                    int64_t stride) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) v[i] += *sum * scale;
+  if (i < n) v[(int64_t)blockIdx.y * stride + i] += sum[blockIdx.y] * scale;
 }
+// (right-hand side blockIdx.y: src and dst + y stride)
 __global__ void k_constrain(int64_t n, const uint8_t *__restrict__ is_c, const int *__restrict__ cptr,
                             const int *__restrict__ cidx, const double *__restrict__ cval,
-                            const double *__restrict__ src, double *__restrict__ dst) {
+                            const double *__restrict__ src, double *__restrict__ dst, int64_t stride) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n || !is_c[i]) return;
+  src += (int64_t)blockIdx.y * stride;
+  dst += (int64_t)blockIdx.y * stride;
   double v = src[i];
   for (int k = cptr[i]; k < cptr[i + 1]; ++k) v -= cval[k] * src[cidx[k]];
   dst[i] = v;
@@ -1160,13 +1170,16 @@ __global__ void __launch_bounds__(256) k_bjacobi(int nrows, const int *__restric
 // independent coalesced loads, the inputs broadcast from their lanes, so the
 // gathers are no longer on each step of the chain.  Same summation order as
 // k_bjacobi.
+// (right-hand side blockIdx.y: v and z + y stride)
 __global__ void __launch_bounds__(256) k_bjacobi_wave(int nleaf, const int *__restrict__ row_beg,
                                                       const int *__restrict__ rows,
                                                       const int64_t *__restrict__ blk_beg,
                                                       const double *__restrict__ pinv, const double *__restrict__ v,
-                                                      double *__restrict__ z) {
+                                                      double *__restrict__ z, int64_t stride) {
   const int leaf = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
   if (leaf >= nleaf) return;
+  v += (int64_t)blockIdx.y * stride;
+  z += (int64_t)blockIdx.y * stride;
   const int lane = threadIdx.x & 63;
   const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0;
   const double xv = lane < R ? v[rows[r0 + lane]] : 0.0;
@@ -1318,6 +1331,16 @@ __global__ void __launch_bounds__(kRedThreads) k_seg_sumsq(const int64_t *__rest
   for (int64_t i = off[s] + threadIdx.x; i < off[s + 1]; i += kRedThreads) acc += v[i] * v[i];
   const double r = block_sum(acc, sh);
   if (threadIdx.x == 0) out[s] = r;
+}
+// out[b] = sum of v[b n .. b n + n), one workgroup per right-hand side b
+__global__ void __launch_bounds__(kRedThreads) k_rhs_sums(int64_t n, const double *__restrict__ v,
+                                                          double *__restrict__ out) {
+  __shared__ double sh[kRedThreads / 64];
+  const double *vb = v + (int64_t)blockIdx.x * n;
+  double acc = 0;
+  for (int64_t i = threadIdx.x; i < n; i += kRedThreads) acc += vb[i];
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = r;
 }
 __global__ void k_seg_scale(int64_t n, const int *__restrict__ seg, const double *__restrict__ f,
                             const double *__restrict__ x, double *__restrict__ y) {
@@ -2144,7 +2167,7 @@ void Engine::vmult(const double *src, double *dst, int precision) {
     else
       matvec_combined<double>(xin, d_outN.as<double>());
     hipLaunchKernelGGL(k_vmult_post, dim3(g), dim3(256), 0, stream, n, d_outN.as<double>(), d_alpha.as<double>(),
-                       d_dir.as<uint8_t>(), xin, dst);
+                       d_dir.as<uint8_t>(), xin, dst, (int64_t)0);
   }
   if (!have_dirichlet) {
     if (neumann_mean) {
@@ -2156,7 +2179,8 @@ void Engine::vmult(const double *src, double *dst, int precision) {
       hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, src, d_ones.as<double>(),
                          d_red.as<double>());
       hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, stream, nchunk, d_red.as<double>(), res);
-      hipLaunchKernelGGL(k_add_scaled_scalar, dim3(g), dim3(256), 0, stream, n, dst, res, 1.0 / (double)n);
+      hipLaunchKernelGGL(k_add_scaled_scalar, dim3(g), dim3(256), 0, stream, n, dst, res, 1.0 / (double)n,
+                         (int64_t)0);
     } else {
       // dst -= ||dst||_2 (bem_problem.cc:1214-1215)
       hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, dst, dst, d_red.as<double>());
@@ -2167,7 +2191,7 @@ void Engine::vmult(const double *src, double *dst, int precision) {
   }
   if (have_constraints)
     hipLaunchKernelGGL(k_constrain, dim3(g), dim3(256), 0, stream, n, d_c_is.as<uint8_t>(), d_c_ptr.as<int>(),
-                       d_c_idx.as<int>(), d_c_val.as<double>(), src, dst);
+                       d_c_idx.as<int>(), d_c_val.as<double>(), src, dst, (int64_t)0);
   HIPCHK(hipGetLastError());
 }
 
@@ -2284,13 +2308,15 @@ void Engine::build_preconditioner() {
   have_prec = true;
 }
 
-void Engine::apply_prec(const double *v, double *z) {
+void Engine::apply_prec(const double *v, double *z, int R) {
   const int nrows = (int)near.rows.size();
-  if (max_leaf_rows <= 64) {
-    hipLaunchKernelGGL(k_bjacobi_wave, dim3(nblk(near.nleaf, 4)), dim3(256), 0, stream, near.nleaf,
-                       d_row_beg.as<int>(), d_rows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z);
+  if (max_leaf_rows <= 64) {  // R right-hand sides, n apart, in one launch
+    hipLaunchKernelGGL(k_bjacobi_wave, dim3(nblk(near.nleaf, 4), R), dim3(256), 0, stream, near.nleaf,
+                       d_row_beg.as<int>(), d_rows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z,
+                       (int64_t)n);
     return;
   }
+  for (int r = 1; r < R; ++r) apply_prec(v + (int64_t)r * n, z + (int64_t)r * n, 1);
   hipLaunchKernelGGL(k_bjacobi, dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_row_leaf.as<int>(),
                      d_row_beg.as<int>(), d_rows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z);
 }
@@ -3033,33 +3059,22 @@ void Engine::vmult_multi(const double *src, double *dst, int R, int precision) {
     matvec_combined_multi<float>(sp, o, R);
   else
     matvec_combined_multi<double>(sp, o, R);
-  const int nchunk = (int)nblk(n, kRedChunk);
-  double *res = d_red.as<double>() + (size_t)nchunk * 128;
-  if (!have_dirichlet && !d_ones.p) {
-    alloc(d_ones, n * 8);
-    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, stream, d_ones.as<double>(), n, 1.0);
+  // every right-hand side in one launch per step (blockIdx.y)
+  hipLaunchKernelGGL(k_vmult_post, dim3(g, R), dim3(256), 0, stream, n, o, d_alpha.as<double>(), d_dir.as<uint8_t>(),
+                     sp, dst, (int64_t)n);
+  if (!have_dirichlet) {
+    if (d_rhs_sum.bytes < (size_t)R * 8) alloc(d_rhs_sum, (size_t)R * 8);
+    hipLaunchKernelGGL(k_rhs_sums, dim3(R), dim3(kRedThreads), 0, stream, n, src, d_rhs_sum.as<double>());
+    hipLaunchKernelGGL(k_add_scaled_scalar, dim3(g, R), dim3(256), 0, stream, n, dst, d_rhs_sum.as<double>(),
+                       1.0 / (double)n, (int64_t)n);
   }
-  for (int r = 0; r < R; ++r) {
-    const double *s_r = src + (int64_t)r * n;
-    double *d_r = dst + (int64_t)r * n;
-    hipLaunchKernelGGL(k_vmult_post, dim3(g), dim3(256), 0, stream, n, o + (int64_t)r * n, d_alpha.as<double>(),
-                       d_dir.as<uint8_t>(), sp + (int64_t)r * n, d_r);
-    if (!have_dirichlet) {
-      hipLaunchKernelGGL(k_mdot, dim3(nchunk, 1), dim3(kRedThreads), 0, stream, n, n, s_r, d_ones.as<double>(),
-                         d_red.as<double>());
-      hipLaunchKernelGGL(k_finish, dim3(1), dim3(kRedThreads), 0, stream, nchunk, d_red.as<double>(), res);
-      hipLaunchKernelGGL(k_add_scaled_scalar, dim3(g), dim3(256), 0, stream, n, d_r, res, 1.0 / (double)n);
-    }
-    if (have_constraints)
-      hipLaunchKernelGGL(k_constrain, dim3(g), dim3(256), 0, stream, n, d_c_is.as<uint8_t>(), d_c_ptr.as<int>(),
-                         d_c_idx.as<int>(), d_c_val.as<double>(), s_r, d_r);
-  }
+  if (have_constraints)
+    hipLaunchKernelGGL(k_constrain, dim3(g, R), dim3(256), 0, stream, n, d_c_is.as<uint8_t>(), d_c_ptr.as<int>(),
+                       d_c_idx.as<int>(), d_c_val.as<double>(), src, dst, (int64_t)n);
   HIPCHK(hipGetLastError());
 }
 
-void Engine::apply_prec_multi(const double *v, double *z, int R) {
-  for (int r = 0; r < R; ++r) apply_prec(v + (int64_t)r * n, z + (int64_t)r * n);
-}
+void Engine::apply_prec_multi(const double *v, double *z, int R) { apply_prec(v, z, R); }
 
 int Engine::gmres_multi(const double *b, double *x, int R, double tol, int restart, int maxit, int precision,
                         int *iters, double *relres) {
