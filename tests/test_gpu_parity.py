@@ -424,3 +424,28 @@ def test_fused_operator_modes_on_batches_and_images(name):
             print(f"\n  {name} N={n} precision={prec} side {mode}: {e:.2e}", end="")
             assert e <= bar
     f.close()
+
+
+@pytest.mark.parametrize("p", [1, 2, 10])
+def test_operator_extreme_orders(oracle_mod, p):
+    """The operator (fused launches, per-order M2P/L2P kernels) at the lowest
+    and highest truncation orders against the oracle's vmult: FP64 within
+    1e-10, FP32 within 2e-5."""
+    from pibem_amd import hip
+    pr = M.config_sphere(16, p)
+    n = pr.mesh.n_dofs
+    o = oracle_mod.OracleFMM(pr).setup()
+    o.compute_alpha()
+    f = P.setup(pr, P.PBFMM_MIXED)
+    bp = P.BEMProblem(f)
+    bp.compute_alpha()
+    x = np.random.default_rng(p).uniform(-1, 1, n)
+    y0 = o.vmult(x)
+    src, dst = hip.DeviceArray.from_numpy(x), hip.DeviceArray(n)
+    for prec, bar in ((P.PBFMM_FP64, 1e-10), (P.PBFMM_MIXED, 2e-5)):
+        bp.vmult_dev(src.value, dst.value, prec)
+        hip.synchronize()
+        e = rel(dst.numpy(), y0)
+        print(f"\n  p={p} N={n} precision={prec}: {e:.2e}", end="")
+        assert e <= bar
+    f.close()
