@@ -131,9 +131,11 @@ int pbfmm_set_constraints(pbfmm_handle *h, const int64_t *c_ptr, const int64_t *
  * (init_preconditioner, bem_fma.cc:919-975), alpha added on the diagonal of
  * unconstrained Neumann rows, constrained rows replaced by their constraint
  * (with the operator's sign, -c_ij).  The reference factors it with ILU(0);
- * here every leaf's own rows form a dense block that is inverted (block
- * Jacobi), applied as right preconditioner by the GMRES of solve_system and
- * gmres_dev.  alpha NULL uses the handle's alpha (computed if needed). */
+ * here the rows of the leaves that share a parent block (at most 48 rows per
+ * block, larger parents cut into even parts) form dense blocks that are
+ * inverted (block Jacobi), applied as right preconditioner by the GMRES of
+ * solve_system and gmres_dev.  alpha NULL uses the handle's alpha (computed
+ * if needed). */
 int pbfmm_fma_preconditioner(pbfmm_handle *h, const double *alpha);
 
 /* ConstrainedOperator::vmult over BEMProblem<3>::vmult, FMA branch

@@ -88,7 +88,7 @@ struct Engine {
   // operator vectors
   DevBuf d_alpha, d_sp, d_sd, d_outN, d_outD, d_in, d_out, d_red;
   DevBuf d_cgs;  // partial sums of the Arnoldi step's dots (kDotParts per column)
-  int max_leaf_rows = 1 << 30;  // largest leaf block of the preconditioner
+  int max_leaf_rows = 1 << 30;  // largest block of the preconditioner
   // pinned host copy of the Hessenberg slots (asynchronous device-to-host
   // copies) and the event that marks a copy done
   double *h_pinned = nullptr;
@@ -104,11 +104,21 @@ struct Engine {
   std::vector<int32_t> hc_ptr, hc_idx;  // host copies of the constraint rows
   std::vector<double> hc_val;
   std::vector<uint8_t> hc_is;
-  // near-field block-Jacobi preconditioner: per leaf the inverse of the
-  // rows x rows block, row-major, at pinv_beg[leaf]
+  // near-field block-Jacobi preconditioner: blocks of the near leaves that
+  // share a parent block (at most kPrecRows rows, leaf after leaf; a larger
+  // leaf alone), per block the inverse of its rows x rows near-field block,
+  // column-major, at pinv_beg[block]; prow_beg / prows / prow_blk: the
+  // blocks' row ranges, rows in block order, block of each of them
+  // Row cap of a preconditioner block (k_bjacobi_wg takes up to 64, a lane
+  // per row).  Measured GMRES to 1e-8, mixed / FP64: 24,578 DoFs (order 6)
+  // 6.73 / 7.83 ms at 48 rows against 7.08 / 8.18 ms at 64; 393,218 DoFs
+  // (Q2, order 8) 112.9 / 148.3 ms against 110.4 / 144.1 ms.
+  static constexpr int kPrecRows = 48;
   bool have_prec = false;
+  int nprec = 0;
   std::vector<int64_t> pinv_beg;
-  DevBuf d_pinv_beg, d_pinv, d_ptmp, d_ptmp2;
+  std::vector<int32_t> prow_beg, prows;
+  DevBuf d_pinv_beg, d_pinv, d_ptmp, d_ptmp2, d_prow_beg, d_prows, d_prow_blk;
   void build_preconditioner();
   void apply_prec(const double *v, double *z, int R = 1);  // R right-hand sides, n apart
   DevBuf d_rhs_sum;  // per right-hand side sums of the lockstep operator's mean term

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <thread>
 
 #include "engine.h"
 
@@ -1118,39 +1119,48 @@ __global__ void k_set_constrained(int64_t n, const uint8_t *__restrict__ is_c, c
 
 // ---------------------------------------------------------------------------
 // Near-field block-Jacobi preconditioner (FMA_preconditioner,
-// bem_fma.cc:3078-3411, restricted to each leaf's own rows): extraction of the
-// rows x rows blocks of the preconditioner matrix and their application.
+// bem_fma.cc:3078-3411, restricted to blocks of the near leaves that share a
+// parent block): extraction of the rows x rows blocks of the preconditioner
+// matrix and their application.
 // ---------------------------------------------------------------------------
-__global__ void k_extract_blocks(const int *__restrict__ row_beg, const int *__restrict__ rows,
-                                 const int *__restrict__ col_beg, const int *__restrict__ cols,
-                                 const int64_t *__restrict__ val_beg, const double *__restrict__ vals,
-                                 const uint8_t *__restrict__ dir, const double *__restrict__ alpha,
-                                 const int64_t *__restrict__ blk_beg, double *__restrict__ out) {
-  const int leaf = blockIdx.x;
-  const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0;
-  const int cb = col_beg[leaf], C = col_beg[leaf + 1] - cb;
+// Block g of the preconditioner (several leaves): row r of block g is
+// the leaf-ordered near row pos[r] (its leaf row_leaf[pos]); an entry is the
+// operator's near value when the column is in that leaf's column list, else
+// zero.
+__global__ void k_extract_groups(const int *__restrict__ grp_beg, const int *__restrict__ pos,
+                                 const int *__restrict__ row_leaf, const int *__restrict__ row_beg,
+                                 const int *__restrict__ rows, const int *__restrict__ col_beg,
+                                 const int *__restrict__ cols, const int64_t *__restrict__ val_beg,
+                                 const double *__restrict__ vals, const uint8_t *__restrict__ dir,
+                                 const double *__restrict__ alpha, const int64_t *__restrict__ blk_beg,
+                                 double *__restrict__ out) {
+  const int g = blockIdx.x;
+  const int g0 = grp_beg[g], R = grp_beg[g + 1] - g0;
   for (int idx = threadIdx.x; idx < R * R; idx += blockDim.x) {
     const int r = idx / R, c = idx - r * R;
-    const int node_r = rows[r0 + r], node_c = rows[r0 + c];
-    int lo = 0, hi = C - 1, pos = -1;
+    const int pr = pos[g0 + r], leaf = row_leaf[pr];
+    const int node_r = rows[pr], node_c = rows[pos[g0 + c]];
+    const int cb = col_beg[leaf], C = col_beg[leaf + 1] - cb;
+    int lo = 0, hi = C - 1, at = -1;
     while (lo <= hi) {
       const int mid = (lo + hi) >> 1;
       const int v = cols[cb + mid];
-      if (v == node_c) { pos = mid; break; }
+      if (v == node_c) { at = mid; break; }
       if (v < node_c) lo = mid + 1;
       else hi = mid - 1;
     }
     double v = 0.0;
-    if (pos >= 0) {
-      const double *pr = vals + 2 * (val_beg[leaf] + (int64_t)r * C + pos);
-      v = dir[node_c] ? -pr[1] : pr[0];
+    if (at >= 0) {
+      const double *q = vals + 2 * (val_beg[leaf] + (int64_t)(pr - row_beg[leaf]) * C + at);
+      v = dir[node_c] ? -q[1] : q[0];
     }
     if (r == c && !dir[node_r]) v += alpha[node_r];
-    out[blk_beg[leaf] + idx] = v;
+    out[blk_beg[g] + idx] = v;
   }
 }
 
-// z[rows of leaf] = Pinv_leaf . v[rows of leaf]; one thread per leaf-ordered row
+// z[rows of block] = Pinv_block . v[rows of block]; one thread per block-ordered
+// row (row_leaf: its block)
 __global__ void __launch_bounds__(256) k_bjacobi(int nrows, const int *__restrict__ row_leaf,
                                                  const int *__restrict__ row_beg, const int *__restrict__ rows,
                                                  const int64_t *__restrict__ blk_beg,
@@ -1165,29 +1175,31 @@ __global__ void __launch_bounds__(256) k_bjacobi(int nrows, const int *__restric
   for (int c = 0; c < R; ++c) acc += P[(int64_t)c * R] * v[rows[r0 + c]];
   z[rows[g]] = acc;
 }
-// The same with one wavefront per leaf (leaves of at most 64 rows): lane r
-// gathers v at row r once, and every lane then reads its block row with R
-// independent coalesced loads, the inputs broadcast from their lanes, so the
-// gathers are no longer on each step of the chain.  Same summation order as
-// k_bjacobi.
+// The same with one workgroup of four wavefronts per block (blocks of at
+// most 64 rows): every wavefront gathers the block's inputs once (lane c: v
+// at row c) and sums the columns c = w, w + 4, ... of all rows (lane r: row
+// r), reading them with independent coalesced loads and the inputs broadcast
+// from their lanes; the four partial sums meet in LDS.  Load chains are R/4
+// long, and there are four times the wavefronts in flight.
 // (right-hand side blockIdx.y: v and z + y stride)
-__global__ void __launch_bounds__(256) k_bjacobi_wave(int nleaf, const int *__restrict__ row_beg,
-                                                      const int *__restrict__ rows,
-                                                      const int64_t *__restrict__ blk_beg,
-                                                      const double *__restrict__ pinv, const double *__restrict__ v,
-                                                      double *__restrict__ z, int64_t stride) {
-  const int leaf = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
-  if (leaf >= nleaf) return;
+__global__ void __launch_bounds__(256) k_bjacobi_wg(const int *__restrict__ row_beg, const int *__restrict__ rows,
+                                                    const int64_t *__restrict__ blk_beg,
+                                                    const double *__restrict__ pinv, const double *__restrict__ v,
+                                                    double *__restrict__ z, int64_t stride) {
+  __shared__ double part[4][64];
+  const int blk = blockIdx.x;
   v += (int64_t)blockIdx.y * stride;
   z += (int64_t)blockIdx.y * stride;
-  const int lane = threadIdx.x & 63;
-  const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0;
+  const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+  const int r0 = row_beg[blk], R = row_beg[blk + 1] - r0;
   const double xv = lane < R ? v[rows[r0 + lane]] : 0.0;
-  const double *P = pinv + blk_beg[leaf] + (lane < R ? lane : 0);  // column-major block
+  const double *P = pinv + blk_beg[blk] + (lane < R ? lane : 0);  // column-major block
   double acc = 0.0;
-#pragma unroll 8
-  for (int c = 0; c < R; ++c) acc += (lane < R ? P[(int64_t)c * R] : 0.0) * read_lane(xv, c);
-  if (lane < R) z[rows[r0 + lane]] = acc;
+#pragma unroll 4
+  for (int c = w; c < R; c += 4) acc += (lane < R ? P[(int64_t)c * R] : 0.0) * read_lane(xv, c);
+  part[w][lane] = acc;
+  __syncthreads();
+  if (w == 0 && lane < R) z[rows[r0 + lane]] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2251,56 +2263,106 @@ static bool invert_block(std::vector<double> &A, int R) {
 }
 
 void Engine::build_preconditioner() {
+  // blocks: near leaves with the same parent block, in near-leaf order; a
+  // parent of more than kPrecRows rows is cut into ceil(rows / kPrecRows)
+  // blocks of about equal rows
   const int nl = near.nleaf;
-  pinv_beg.assign(nl + 1, 0);
+  std::vector<int> order(nl);
+  for (int l = 0; l < nl; ++l) order[l] = l;
+  auto parent_of = [&](int l) { return tree.parent[near.leaf_block[l]]; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return parent_of(a) < parent_of(b); });
+  std::vector<int32_t> pos;  // block-ordered row -> leaf-ordered near row
+  prow_beg.assign(1, 0);
+  prows.clear();
+  for (int i = 0; i < nl;) {
+    const int par = parent_of(order[i]);
+    int total = 0;
+    for (int k = i; k < nl && parent_of(order[k]) == par; ++k)
+      total += near.row_beg[order[k] + 1] - near.row_beg[order[k]];
+    const int parts = (total + kPrecRows - 1) / kPrecRows, target = (total + parts - 1) / parts;
+    int rows_in = 0;
+    for (; i < nl && parent_of(order[i]) == par; ++i) {
+      const int l = order[i], R = near.row_beg[l + 1] - near.row_beg[l];
+      if (rows_in > 0 && (rows_in + R > kPrecRows || rows_in >= target)) {
+        prow_beg.push_back((int32_t)pos.size());
+        rows_in = 0;
+      }
+      for (int r = near.row_beg[l]; r < near.row_beg[l + 1]; ++r) {
+        pos.push_back(r);
+        prows.push_back(near.rows[r]);
+      }
+      rows_in += R;
+    }
+    if (rows_in > 0) prow_beg.push_back((int32_t)pos.size());
+  }
+  nprec = (int)prow_beg.size() - 1;
+  pinv_beg.assign(nprec + 1, 0);
   max_leaf_rows = 0;
-  for (int l = 0; l < nl; ++l) {
-    const int64_t R = near.row_beg[l + 1] - near.row_beg[l];
-    pinv_beg[l + 1] = pinv_beg[l] + R * R;
+  std::vector<int32_t> prow_blk(prows.size());
+  for (int g = 0; g < nprec; ++g) {
+    const int64_t R = prow_beg[g + 1] - prow_beg[g];
+    pinv_beg[g + 1] = pinv_beg[g] + R * R;
     max_leaf_rows = std::max(max_leaf_rows, (int)R);
+    for (int r = prow_beg[g]; r < prow_beg[g + 1]; ++r) prow_blk[r] = g;
   }
   up(d_pinv_beg, pinv_beg);
-  alloc(d_pinv, (size_t)std::max<int64_t>(pinv_beg[nl], 1) * 8);
-  if (nl > 0)
-    hipLaunchKernelGGL(k_extract_blocks, dim3(nl), dim3(256), 0, stream, d_row_beg.as<int>(), d_rows.as<int>(),
-                       d_col_beg.as<int>(), d_cols.as<int>(), d_val_beg.as<int64_t>(), f64.vals.as<double>(),
-                       d_dir.as<uint8_t>(), d_alpha.as<double>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>());
-  HIPCHK(hipGetLastError());
-  std::vector<double> blk(pinv_beg[nl]);
-  if (!blk.empty())
-    HIPCHK(hipMemcpyAsync(blk.data(), d_pinv.p, blk.size() * 8, hipMemcpyDeviceToHost, stream));
-  sync();
-  std::vector<int32_t> local(n, -1);
-  std::vector<double> A;
-  for (int l = 0; l < nl; ++l) {
-    const int r0 = near.row_beg[l], R = near.row_beg[l + 1] - r0;
-    for (int r = 0; r < R; ++r) local[near.rows[r0 + r]] = r;
-    A.assign(blk.begin() + pinv_beg[l], blk.begin() + pinv_beg[l + 1]);
-    if (have_constraints)
-      for (int r = 0; r < R; ++r) {
-        const int i = near.rows[r0 + r];
-        if (!hc_is[i]) continue;
-        // constrained row: dst(i) = src(i) - sum c_ij src(j) (constrained_matrix.h:109-121)
-        for (int c = 0; c < R; ++c) A[(size_t)r * R + c] = (c == r) ? 1.0 : 0.0;
-        for (int k = hc_ptr[i]; k < hc_ptr[i + 1]; ++k) {
-          const int j = hc_idx[k];
-          if (j >= 0 && local[j] >= 0 && near.rows[r0 + local[j]] == j) A[(size_t)r * R + local[j]] -= hc_val[k];
+  up(d_prow_beg, prow_beg);
+  up(d_prows, prows);
+  up(d_prow_blk, prow_blk);
+  alloc(d_pinv, (size_t)std::max<int64_t>(pinv_beg[nprec], 1) * 8);
+  if (nprec > 0) {
+    DevBuf d_pos;
+    up(d_pos, pos);
+    hipLaunchKernelGGL(k_extract_groups, dim3(nprec), dim3(256), 0, stream, d_prow_beg.as<int>(), d_pos.as<int>(),
+                       d_row_leaf.as<int>(), d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(),
+                       d_cols.as<int>(), d_val_beg.as<int64_t>(), f64.vals.as<double>(), d_dir.as<uint8_t>(),
+                       d_alpha.as<double>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>());
+    HIPCHK(hipGetLastError());
+    sync();  // d_pos is freed on return
+  }
+  std::vector<double> blk(pinv_beg[nprec]);
+  if (!blk.empty()) HIPCHK(hipMemcpy(blk.data(), d_pinv.p, blk.size() * 8, hipMemcpyDeviceToHost));
+  // invert the blocks on host threads
+  auto invert_range = [&](int g_lo, int g_hi) {
+    std::vector<double> A;
+    for (int g = g_lo; g < g_hi; ++g) {
+      const int r0 = prow_beg[g], R = prow_beg[g + 1] - r0;
+      A.assign(blk.begin() + pinv_beg[g], blk.begin() + pinv_beg[g + 1]);
+      if (have_constraints)
+        for (int r = 0; r < R; ++r) {
+          const int i = prows[r0 + r];
+          if (!hc_is[i]) continue;
+          // constrained row: dst(i) = src(i) - sum c_ij src(j) (constrained_matrix.h:109-121)
+          for (int c = 0; c < R; ++c) A[(size_t)r * R + c] = (c == r) ? 1.0 : 0.0;
+          for (int k = hc_ptr[i]; k < hc_ptr[i + 1]; ++k) {
+            const int j = hc_idx[k];
+            for (int c = 0; c < R; ++c)
+              if (prows[r0 + c] == j) A[(size_t)r * R + c] -= hc_val[k];
+          }
         }
+      std::vector<double> keep = A;
+      if (!invert_block(A, R)) {
+        // singular block (e.g. unconstrained double nodes): diagonal scaling
+        for (int r = 0; r < R; ++r)
+          for (int c = 0; c < R; ++c) {
+            const double d = keep[(size_t)r * R + r];
+            A[(size_t)r * R + c] = (r == c) ? (d != 0.0 ? 1.0 / d : 1.0) : 0.0;
+          }
       }
-    std::vector<double> keep = A;
-    if (!invert_block(A, R)) {
-      // singular block (e.g. unconstrained double nodes): diagonal scaling
+      // stored column-major (transposed): the apply kernels' lanes, one per
+      // row, then read consecutive addresses
       for (int r = 0; r < R; ++r)
-        for (int c = 0; c < R; ++c) {
-          const double d = keep[(size_t)r * R + r];
-          A[(size_t)r * R + c] = (r == c) ? (d != 0.0 ? 1.0 / d : 1.0) : 0.0;
-        }
+        for (int c = 0; c < R; ++c) blk[pinv_beg[g] + (size_t)c * R + r] = A[(size_t)r * R + c];
     }
-    // stored column-major (transposed): the apply kernel's lanes, one per
-    // row, then read consecutive addresses
-    for (int r = 0; r < R; ++r)
-      for (int c = 0; c < R; ++c) blk[pinv_beg[l] + (size_t)c * R + r] = A[(size_t)r * R + c];
-    for (int r = 0; r < R; ++r) local[near.rows[r0 + r]] = -1;
+  };
+  const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (nprec < 64 || nth == 1) {
+    invert_range(0, nprec);
+  } else {
+    std::vector<std::thread> th;
+    for (int k = 0; k < nth; ++k)
+      th.emplace_back(invert_range, (int)((int64_t)nprec * k / nth), (int)((int64_t)nprec * (k + 1) / nth));
+    for (auto &x : th) x.join();
   }
   if (!blk.empty()) HIPCHK(hipMemcpy(d_pinv.p, blk.data(), blk.size() * 8, hipMemcpyHostToDevice));
   alloc(d_ptmp, n * 8);
@@ -2309,16 +2371,16 @@ void Engine::build_preconditioner() {
 }
 
 void Engine::apply_prec(const double *v, double *z, int R) {
-  const int nrows = (int)near.rows.size();
+  const int nrows = (int)prows.size();
   if (max_leaf_rows <= 64) {  // R right-hand sides, n apart, in one launch
-    hipLaunchKernelGGL(k_bjacobi_wave, dim3(nblk(near.nleaf, 4), R), dim3(256), 0, stream, near.nleaf,
-                       d_row_beg.as<int>(), d_rows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z,
-                       (int64_t)n);
+    if (nprec > 0)
+      hipLaunchKernelGGL(k_bjacobi_wg, dim3(nprec, R), dim3(256), 0, stream, d_prow_beg.as<int>(), d_prows.as<int>(),
+                         d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z, (int64_t)n);
     return;
   }
   for (int r = 1; r < R; ++r) apply_prec(v + (int64_t)r * n, z + (int64_t)r * n, 1);
-  hipLaunchKernelGGL(k_bjacobi, dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_row_leaf.as<int>(),
-                     d_row_beg.as<int>(), d_rows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z);
+  hipLaunchKernelGGL(k_bjacobi, dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_prow_blk.as<int>(),
+                     d_prow_beg.as<int>(), d_prows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z);
 }
 
 // Restarted GMRES with classical Gram-Schmidt and one reorthogonalisation

@@ -261,7 +261,8 @@ class BEMProblem:
                                                         _p(e, _u8)))
 
     def fma_preconditioner(self, alpha=None):
-        """BEMFMA::FMA_preconditioner (bem_fma.cc:3078-3411) as leaf block Jacobi."""
+        """BEMFMA::FMA_preconditioner (bem_fma.cc:3078-3411) as block Jacobi over
+        the rows of the leaves that share a parent block."""
         a = None if alpha is None else _f64(alpha)
         self.fma._chk(self.fma._L.pbfmm_fma_preconditioner(self._h, _p(a)))
 
