@@ -110,9 +110,10 @@ struct Engine {
   // column-major, at pinv_beg[block]; prow_beg / prows / prow_blk: the
   // blocks' row ranges, rows in block order, block of each of them
   // Row cap of a preconditioner block (k_bjacobi_wg takes up to 64, a lane
-  // per row).  Measured GMRES to 1e-8, mixed / FP64: 24,578 DoFs (order 6)
-  // 6.73 / 7.83 ms at 48 rows against 7.08 / 8.18 ms at 64; 393,218 DoFs
-  // (Q2, order 8) 112.9 / 148.3 ms against 110.4 / 144.1 ms.
+  // per row).  Measured GMRES to 1e-8, mixed / FP64, at 48 / 64 / 40 rows:
+  // 24,578 DoFs (order 6) 6.43 / 6.54 / 6.71 ms and 7.54 / 7.72 / 7.92 ms;
+  // 393,218 DoFs (Q2, order 8) 115.0 / 113.5 / 119.5 ms and 150.4 / 145.7 /
+  // 152.8 ms.
   static constexpr int kPrecRows = 48;
   bool have_prec = false;
   int nprec = 0;
