@@ -737,10 +737,10 @@ int pbfmm_get_expansions(pbfmm_handle *h, int32_t which, double *k1, double *k2)
     std::vector<double> v((size_t)nb * 2 * kp);
     const DevBuf &src = which == 0 ? e.f64.me : e.f64.le;
     if (hipMemcpy(v.data(), src.p, v.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) throw HipError("copy failed");
-    // the device keeps level-scaled expansions (ME_n / delta^n, LE_n delta^n)
+    // the device keeps level-scaled expansions (ME_n / delta^n, LE_n delta^(n+1))
     for (int b = 0; b < nb; ++b)
       for (int nn = 0; nn <= e.p; ++nn) {
-        const double sc = std::pow(e.tree.delta[b], which == 0 ? nn : -nn);
+        const double sc = std::pow(e.tree.delta[b], which == 0 ? nn : -(nn + 1));
         for (int m = 0; m <= nn; ++m) {
           const int i = nn * (nn + 1) / 2 + m;
           k1[(size_t)b * 2 * kt + 2 * i] = v[(size_t)b * 2 * kp + i] * sc;

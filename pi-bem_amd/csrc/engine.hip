@@ -625,8 +625,8 @@ struct Kp {
 // local_expansion.cc:179-248, driven by bem_fma.cc:2622-2652) and L2L
 // (LocalExpansion::Add(parent), local_expansion.cc:107-176, driven by
 // bem_fma.cc:2600-2602) are all OUT[tgt] += T_class . IN[src] with one R-linear
-// KP x KP operator per class (M2M / L2L: level and child octant; M2L: level
-// and integer offset).  Pairs are grouped by class into chunks of 32; a block
+// KP x KP operator per class (M2M / L2L: level and child octant; M2L:
+// integer offset, all levels).  Pairs are grouped by class into chunks of 32; a block
 // computes C (KP x 64) = A (KP x KP) . B (KP x 64 source columns, both
 // kernels) with 16x16x4 MFMA tiles (KP/16 wavefronts, wave w: row tile w,
 // all 64 columns), stages C through LDS and adds it to the targets with
@@ -826,8 +826,8 @@ __global__ void __launch_bounds__(4 * Kp<P>::value) k_translate(TrArgs<T> a, int
 // (MultipoleExpansion::Evaluate, multipole_expansion.cc:264-298, rho^(-n-1))
 // or L2P (LocalExpansion::Evaluate, local_expansion.cc:251-287, rho^n).
 // NK = 1: the single expansion E (v2 unused).  E is level-scaled by the box
-// size sc of its block (ME_n / sc^n, LE_n sc^n), so the radial factors are
-// (sc/rho)^n / rho and (rho/sc)^n.
+// size sc of its block (ME_n / sc^n, LE_n sc^(n+1)), so the radial factors
+// are (sc/rho)^n / rho and (rho/sc)^n / sc.
 // P > 0 fixes the order at compile time (loops unrolled, the expansion's
 // loads scheduled ahead of the recurrences); P = 0 takes p, kt, kp at run time.
 template <typename T, bool LOCAL, int NK, int P = 0>
@@ -845,7 +845,7 @@ __device__ __forceinline__ void eval2(const T *__restrict__ E, int p_rt, int kt_
   const T s = rxy / rho;
   const T cb = rxy > T(0) ? d0 / rxy : T(1), sb = rxy > T(0) ? d1 / rxy : T(0);
   const T step = LOCAL ? rho / sc : sc / rho;
-  T rstart = LOCAL ? T(1) : T(1) / rho;
+  T rstart = T(1) / (LOCAL ? sc : rho);
   T pmm = 1, cm = 1, sm = 0;
   v1 = 0;
   v2 = 0;
@@ -1957,7 +1957,7 @@ void Engine::matvec_combined(const double *x, double *out) {
 void Engine::consumer_levels() {
   me_lo = le_lo = L + 1;
   for (int l = 1; l <= L; ++l)
-    if (lvl<1>(far.m2l, l + 1) > lvl<1>(far.m2l, l)) {
+    if (far.m2l_per_level[l] > 0) {
       me_lo = std::min(me_lo, l);
       le_lo = std::min(le_lo, l);
     }

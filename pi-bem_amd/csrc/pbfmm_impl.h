@@ -143,12 +143,13 @@ struct FarPlan {
   std::vector<int32_t> l2p_leaf;  // childless blocks holding nodes
   // operators, transposed (column-major: op[k*kp + row]), FP64, acting on
   // level-scaled expansions: a block of level l with box size delta_l stores
-  // ME_n^m / delta_l^n and LE_n^m * delta_l^n, so that every operator entry is
-  // O(1) in box units (the unscaled entries of a p = 8 M2L between boxes of
-  // size 2e-3 reach 1e40, past FP32's range)
+  // ME_n^m / delta_l^n and LE_n^m * delta_l^(n+1), so that every operator
+  // entry is O(1) in box units (the unscaled entries of a p = 8 M2L between
+  // boxes of size 2e-3 reach 1e40, past FP32's range) and the M2L operator of
+  // an offset class is the same at every level
   std::vector<double> m2m_ops;  // (L+1) levels x 8 octants
   std::vector<double> l2l_ops;  // (L+1) x 8
-  std::vector<double> m2l_ops;  // n_m2l_ops
+  std::vector<double> m2l_ops;  // n_m2l_ops: one per offset class (unit box)
 };
 void build_far_plan(const Tree &t, int p, FarPlan &fp);
 void build_p2m_plan(const HostMesh &m, const QuadData &qd, const Tree &t, FarPlan &fp);

@@ -595,9 +595,11 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
       const double dn[3] = {-d[0], -d[1], -d[2]};
       op_l2l(p, dn, &fp.l2l_ops[((size_t)l * 8 + o) * kp * kp]);
       // level-scaled expansions: child ME/delta_l^n -> parent ME/delta_{l-1}^n,
-      // parent LE.delta_{l-1}^n -> child LE.delta_l^n
+      // parent LE.delta_{l-1}^(n+1) -> child LE.delta_l^(n+1)
       scale_op(&fp.m2m_ops[((size_t)l * 8 + o) * kp * kp], kp, deg, dl, 1.0 / (2.0 * dl));
       scale_op(&fp.l2l_ops[((size_t)l * 8 + o) * kp * kp], kp, deg, 1.0 / (2.0 * dl), dl);
+      // the extra power of the box size: child delta / parent delta
+      for (size_t i = 0; i < (size_t)kp * kp; ++i) fp.l2l_ops[((size_t)l * 8 + o) * kp * kp + i] *= 0.5;
     }
   }
   // M2M pairs (bem_fma.cc:2318-2339: every block of level l into its parent)
@@ -627,15 +629,19 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
   fp.m2m.level_off1[L + 1] = (int)fp.m2m.op1.size();
   fp.l2l.level_off1[L + 1] = (int)fp.l2l.op1.size();
 
-  // M2L pairs by (level, offset class); M2P pairs
+  // M2L pairs by offset class over all levels; M2P pairs.  With ME_n /
+  // delta^n and LE_n delta^(n+1) the M2L operator of a class is the same at
+  // every level (the unit-box one: the Laplace M2L between boxes of size delta
+  // scales as delta^-(n+nn+1)), so a class's pairs of all levels share one
+  // operator and their chunks; the chunk list has no per-level ranges (all
+  // chunks are listed under level 1) and the per-level pair counts are in
+  // m2l_per_level.
   fp.m2l_per_level.assign(L + 1, 0);
   fp.m2l.level_off.assign(L + 2, 0);
   fp.m2l.level_off1.assign(L + 2, 0);
   fp.m2p_work_beg.push_back(0);
+  std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> cls;
   for (int l = 1; l <= L; ++l) {
-    fp.m2l.level_off[l] = (int)fp.m2l.op.size();
-    fp.m2l.level_off1[l] = (int)fp.m2l.op1.size();
-    std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> cls;
     const double dl = t.delta[0] / std::pow(2.0, l);
     for (int b : t.dofs_filled[l]) {
       double cb[3];
@@ -648,6 +654,7 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
             std::array<int, 3> key;
             for (int k = 0; k < 3; ++k) key[k] = (int)std::lround((cs[k] - cb[k]) / dl);
             cls[key].push_back({b, src});
+            ++fp.m2l_per_level[l];
           } else if (src > t.endL[l]) {
             fp.m2p_tgt.push_back(b);
             fp.m2p_src.push_back(src);
@@ -659,18 +666,18 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
           }
         }
     }
-    for (auto &kv : cls) {
-      const int op = fp.n_m2l_ops++;
-      fp.m2l_ops.resize((size_t)fp.n_m2l_ops * kp * kp);
-      const double d[3] = {kv.first[0] * dl, kv.first[1] * dl, kv.first[2] * dl};
-      op_m2l(p, d, &fp.m2l_ops[(size_t)op * kp * kp]);
-      scale_op(&fp.m2l_ops[(size_t)op * kp * kp], kp, deg, dl, dl);  // ME/delta^n -> LE.delta^n
-      add_class(fp.m2l, op, kv.second);
-      fp.m2l_per_level[l] += (int64_t)kv.second.size();
-    }
   }
-  fp.m2l.level_off[L + 1] = (int)fp.m2l.op.size();
-  fp.m2l.level_off1[L + 1] = (int)fp.m2l.op1.size();
+  for (auto &kv : cls) {
+    const int op = fp.n_m2l_ops++;
+    fp.m2l_ops.resize((size_t)fp.n_m2l_ops * kp * kp);
+    const double d[3] = {(double)kv.first[0], (double)kv.first[1], (double)kv.first[2]};
+    op_m2l(p, d, &fp.m2l_ops[(size_t)op * kp * kp]);  // unit box
+    add_class(fp.m2l, op, kv.second);
+  }
+  for (int l = 2; l <= L + 1; ++l) {
+    fp.m2l.level_off[l] = (int)fp.m2l.op.size();
+    fp.m2l.level_off1[l] = (int)fp.m2l.op1.size();
+  }
 }
 
 // The image of cell c is the reflection x -> (x, y, 2 z0 - z) of its mapping
