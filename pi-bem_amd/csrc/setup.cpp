@@ -629,19 +629,21 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
   fp.m2m.level_off1[L + 1] = (int)fp.m2m.op1.size();
   fp.l2l.level_off1[L + 1] = (int)fp.l2l.op1.size();
 
-  // M2L pairs by offset class over all levels; M2P pairs.  With ME_n /
-  // delta^n and LE_n delta^(n+1) the M2L operator of a class is the same at
-  // every level (the unit-box one: the Laplace M2L between boxes of size delta
-  // scales as delta^-(n+nn+1)), so a class's pairs of all levels share one
-  // operator and their chunks; the chunk list has no per-level ranges (all
-  // chunks are listed under level 1) and the per-level pair counts are in
-  // m2l_per_level.
+  // M2L pairs by (level, offset class); M2P pairs.  With ME_n / delta^n and
+  // LE_n delta^(n+1) the M2L operator of an offset class is the same at every
+  // level (the unit-box one: the Laplace M2L between boxes of size delta
+  // scales as delta^-(n+nn+1)), so the classes of all levels share one
+  // operator per offset.  Chunks stay per (level, class), level after level:
+  // a workgroup range then gathers sources of one level.
   fp.m2l_per_level.assign(L + 1, 0);
   fp.m2l.level_off.assign(L + 2, 0);
   fp.m2l.level_off1.assign(L + 2, 0);
   fp.m2p_work_beg.push_back(0);
-  std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> cls;
+  std::map<std::array<int, 3>, int> op_of;
   for (int l = 1; l <= L; ++l) {
+    fp.m2l.level_off[l] = (int)fp.m2l.op.size();
+    fp.m2l.level_off1[l] = (int)fp.m2l.op1.size();
+    std::map<std::array<int, 3>, std::vector<std::pair<int, int>>> cls;
     const double dl = t.delta[0] / std::pow(2.0, l);
     for (int b : t.dofs_filled[l]) {
       double cb[3];
@@ -654,7 +656,6 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
             std::array<int, 3> key;
             for (int k = 0; k < 3; ++k) key[k] = (int)std::lround((cs[k] - cb[k]) / dl);
             cls[key].push_back({b, src});
-            ++fp.m2l_per_level[l];
           } else if (src > t.endL[l]) {
             fp.m2p_tgt.push_back(b);
             fp.m2p_src.push_back(src);
@@ -666,18 +667,21 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
           }
         }
     }
+    for (auto &kv : cls) {
+      auto it = op_of.find(kv.first);
+      if (it == op_of.end()) {
+        const int op = fp.n_m2l_ops++;
+        fp.m2l_ops.resize((size_t)fp.n_m2l_ops * kp * kp);
+        const double d[3] = {(double)kv.first[0], (double)kv.first[1], (double)kv.first[2]};
+        op_m2l(p, d, &fp.m2l_ops[(size_t)op * kp * kp]);  // unit box
+        it = op_of.emplace(kv.first, op).first;
+      }
+      add_class(fp.m2l, it->second, kv.second);
+      fp.m2l_per_level[l] += (int64_t)kv.second.size();
+    }
   }
-  for (auto &kv : cls) {
-    const int op = fp.n_m2l_ops++;
-    fp.m2l_ops.resize((size_t)fp.n_m2l_ops * kp * kp);
-    const double d[3] = {(double)kv.first[0], (double)kv.first[1], (double)kv.first[2]};
-    op_m2l(p, d, &fp.m2l_ops[(size_t)op * kp * kp]);  // unit box
-    add_class(fp.m2l, op, kv.second);
-  }
-  for (int l = 2; l <= L + 1; ++l) {
-    fp.m2l.level_off[l] = (int)fp.m2l.op.size();
-    fp.m2l.level_off1[l] = (int)fp.m2l.op1.size();
-  }
+  fp.m2l.level_off[L + 1] = (int)fp.m2l.op.size();
+  fp.m2l.level_off1[L + 1] = (int)fp.m2l.op1.size();
 }
 
 // The image of cell c is the reflection x -> (x, y, 2 z0 - z) of its mapping
