@@ -535,12 +535,36 @@ __device__ __forceinline__ void add16(double2 &a, const double2 &u) { a.x += u.x
 // (U1 on Neumann DoFs, -U2 on Dirichlet DoFs) and the leaf gets one combined
 // expansion from phi (dphi unused).  ME is [leaf][NK kp].
 // Right-hand side blockIdx.y reads phi (dphi) + y vs and writes ME + y es.
+// Workgroups past the last leaf (operator pipeline only: cl.leaf set) clear,
+// for the same application, the MEs of the blocks that are not leaves, every
+// LE and the M2P accumulator; the leaves' MEs are stored whole by their own
+// workgroups.
+template <typename T>
+struct P2mClear {
+  const uint8_t *leaf = nullptr;  // per block: 1 for a leaf
+  T *le = nullptr;
+  double *acc = nullptr;
+  int64_t ne = 0, na = 0;  // expansion values (blocks x kp), accumulator length
+};
 template <typename T, int NK>
 __global__ void __launch_bounds__(256) k_p2m(int nleaf, const int *__restrict__ leaves,
                                              const int *__restrict__ col_beg, const int *__restrict__ col_dof,
                                              const T *__restrict__ U, int kp, const double *__restrict__ phi,
                                              const double *__restrict__ dphi, T *__restrict__ ME, int64_t vs,
-                                             int64_t es) {
+                                             int64_t es, P2mClear<T> cl = P2mClear<T>()) {
+  if ((int)blockIdx.x >= nleaf) {  // clearing workgroups (gridDim.y is 1 here)
+    const int64_t nt = (int64_t)(gridDim.x - nleaf) * blockDim.x;
+    for (int64_t i = (int64_t)(blockIdx.x - nleaf) * blockDim.x + threadIdx.x; i < 2 * cl.ne + cl.na; i += nt) {
+      if (i < cl.ne) {
+        if (!cl.leaf[i / kp]) ME[i] = T(0);
+      } else if (i < 2 * cl.ne) {
+        cl.le[i - cl.ne] = T(0);
+      } else {
+        cl.acc[i - 2 * cl.ne] = 0.0;
+      }
+    }
+    return;
+  }
   phi += (int64_t)blockIdx.y * vs;
   if (dphi) dphi += (int64_t)blockIdx.y * vs;
   ME += (int64_t)blockIdx.y * es;
@@ -1568,6 +1592,11 @@ static void upload_chunks(ChunkDev &d, const ChunkList &c) {
 }
 
 void Engine::upload_far(bool want_f32) {
+  {
+    std::vector<uint8_t> is_leaf(std::max(tree.nb, 1), 0);
+    for (int b : far.p2m_leaf) is_leaf[b] = 1;
+    up(d_is_leaf, is_leaf);
+  }
   me_lo = le_lo = -1;  // recomputed from this plan on first use
   up(d_p2m_leaf, far.p2m_leaf);
   up(d_p2m_col_beg, far.p2m_col_beg);
@@ -1964,8 +1993,9 @@ void Engine::consumer_levels() {
   for (int b : far.m2p_src) me_lo = std::min(me_lo, (int)tree.level[b]);
 }
 
-// The single-value pipeline in fewer launches, for vmult: ME, LE and the M2P
-// accumulator cleared in one launch; P2M; M2M levels L..me_lo + 1; M2L; L2L
+// The single-value pipeline in fewer launches, for vmult: P2M, whose extra
+// workgroups clear the other MEs, LE and the M2P accumulator; M2M levels
+// L..me_lo + 1; M2L; L2L
 // levels le_lo + 1..L (at least 2); L2P, which also adds the M2P
 // accumulator and applies vmult's epilogue, dst = out + m2p + L2P + alpha x
 // on Neumann DoFs (every DoF is a row of one leaf).  The SpMV writes out and
@@ -1986,12 +2016,21 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   const int64_t ne = (int64_t)tree.nb * kp;
   T *me = F.me.template as<T>(), *le = F.le.template as<T>();
   double *acc = d_outD.as<double>();
-  hipLaunchKernelGGL(k_clear_operator<T>, dim3((unsigned)std::min<int64_t>(nblk(2 * ne + n, 256), 4096)), dim3(256),
-                     0, stream, me, le, ne, acc, n);
-  if (nleaf > 0)
-    hipLaunchKernelGGL((k_p2m<T, 1>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
+  if (nleaf > 0) {  // P2M, with clearing workgroups after the leaves' ones
+    P2mClear<T> cl;
+    cl.leaf = d_is_leaf.as<uint8_t>();
+    cl.le = le;
+    cl.acc = acc;
+    cl.ne = ne;
+    cl.na = n;
+    const unsigned nclear = (unsigned)std::min<int64_t>(nblk(2 * ne + n, 256), 1024);
+    hipLaunchKernelGGL((k_p2m<T, 1>), dim3(nleaf + nclear), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
                        d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp, x, nullptr, me,
-                       (int64_t)0, (int64_t)0);
+                       (int64_t)0, (int64_t)0, cl);
+  } else {
+    hipLaunchKernelGGL(k_clear_operator<T>, dim3((unsigned)std::min<int64_t>(nblk(2 * ne + n, 256), 4096)),
+                       dim3(256), 0, stream, me, le, ne, acc, n);
+  }
   if (me_lo < 0) consumer_levels();
   const int var = spmv_variant[sizeof(T) == 4 ? 1 : 0][0];
   const int nl = near.nleaf;
