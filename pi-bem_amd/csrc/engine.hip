@@ -535,10 +535,10 @@ __device__ __forceinline__ void add16(double2 &a, const double2 &u) { a.x += u.x
 // (U1 on Neumann DoFs, -U2 on Dirichlet DoFs) and the leaf gets one combined
 // expansion from phi (dphi unused).  ME is [leaf][NK kp].
 // Right-hand side blockIdx.y reads phi (dphi) + y vs and writes ME + y es.
-// Workgroups past the last leaf (operator pipeline only: cl.leaf set) clear,
-// for the same application, the MEs of the blocks that are not leaves, every
-// LE and the M2P accumulator; the leaves' MEs are stored whole by their own
-// workgroups.
+// Workgroups past the last leaf (fused pipelines, cl.leaf set) clear, for
+// the same application, the MEs of the blocks that are not leaves, every LE
+// and the M2P accumulator (if any); the leaves' MEs are stored whole by their
+// own workgroups.
 template <typename T>
 struct P2mClear {
   const uint8_t *leaf = nullptr;  // per block: 1 for a leaf
@@ -556,7 +556,7 @@ __global__ void __launch_bounds__(256) k_p2m(int nleaf, const int *__restrict__ 
     const int64_t nt = (int64_t)(gridDim.x - nleaf) * blockDim.x;
     for (int64_t i = (int64_t)(blockIdx.x - nleaf) * blockDim.x + threadIdx.x; i < 2 * cl.ne + cl.na; i += nt) {
       if (i < cl.ne) {
-        if (!cl.leaf[i / kp]) ME[i] = T(0);
+        if (!cl.leaf[i / (NK * kp)]) ME[i] = T(0);
       } else if (i < 2 * cl.ne) {
         cl.le[i - cl.ne] = T(0);
       } else {
@@ -1941,13 +1941,20 @@ void Engine::matvec(const double *phi, const double *dphi, double *outN, double 
   FarDev<T> &F = far_of<T>(*this);
   const int64_t ne = (int64_t)tree.nb * 2 * kp;
   T *me = F.me.template as<T>(), *le = F.le.template as<T>();
-  hipLaunchKernelGGL(k_clear_operator<T>, dim3((unsigned)std::min<int64_t>(nblk(2 * ne, 256), 4096)), dim3(256), 0,
-                     stream, me, le, ne, nullptr, (int64_t)0);
   const int nleaf = (int)far.p2m_leaf.size();
-  if (nleaf > 0)
-    hipLaunchKernelGGL((k_p2m<T, 2>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
+  if (nleaf > 0) {  // P2M, with clearing workgroups after the leaves' ones
+    P2mClear<T> cl;
+    cl.leaf = d_is_leaf.as<uint8_t>();
+    cl.le = le;
+    cl.ne = ne;
+    const unsigned nclear = (unsigned)std::min<int64_t>(nblk(2 * ne, 256), 1024);
+    hipLaunchKernelGGL((k_p2m<T, 2>), dim3(nleaf + nclear), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
                        d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.unit.template as<T>(), kp, phi, dphi, me,
-                       (int64_t)0, (int64_t)0);
+                       (int64_t)0, (int64_t)0, cl);
+  } else {
+    hipLaunchKernelGGL(k_clear_operator<T>, dim3((unsigned)std::min<int64_t>(nblk(2 * ne, 256), 4096)), dim3(256),
+                       0, stream, me, le, ne, nullptr, (int64_t)0);
+  }
   if (me_lo < 0) consumer_levels();
   for (int l = L; l > me_lo; --l)
     launch_translate<T, 0, 2>(*this, stream, m2m_cl, lvl<2>(far.m2m, l), lvl<2>(far.m2m, l + 1),
