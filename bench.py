@@ -1,18 +1,24 @@
 """Benchmark of the BEM-FMM GMRES solve on one MI355X.
 
-A step is one GMRES(100) solve of configuration 2 to true relative residual
-1e-8 (unit sphere from a 6-face cube with 64 x 64 Q1 cells per face, 24,578
-DoFs, FMM order 6, leaf <= 20 nodes, mixed Dirichlet (z > 0) / Neumann data
-of phi = 1/|x - (2,1,1)|) with the mixed-precision path: FP32 operator
-(near-field SpMV, P2M, M2M, M2L, L2L, M2P, L2P) inside FP64 iterative
-refinement with an FP64 Arnoldi process and FP64 residuals.  The FP64 GPU path
-is timed beside it, as are FMM matvecs per second, per-phase device times from
-HIP events on the library's stream, and the CPU oracle (the reference's
-algorithm restated in C++, OpenMP) on the host cores.  Configuration 3
-(prolate spheroid (4,1,1), 6 x 128 x 128 Q2 cells, 393,218 DoFs, order 8) is
-measured the same way and reported in the "config3" object.
+A step is one GMRES(100) solve of configuration 3 to true relative residual
+1e-8 with the mixed-precision path: prolate spheroid (4,1,1) from a 6-face
+cube grid with 128 x 128 Q2 cells per face (98,304 cells, 393,218 DoFs;
+equiangular face coordinates c -> tan(pi c / 4) before the projection),
+FMM order 8, leaf <= 20 nodes, Dirichlet on cells with centroid z > 0 and
+Neumann elsewhere, data from phi = x + 8 seeded exterior point sources at
+1.5x the surface (seed 3), singular cells integrated by the Duffy split of the
+reference's dense solve.  Mixed precision: FP32 operator (near-field SpMV,
+P2M, M2M, M2L, L2L, M2P, L2P) inside an FP64 outer loop with FP64 Arnoldi,
+FP64 residuals and FP64 singular quadrature.  The FP64 GPU path is timed
+beside it, as are FMM matvecs per second, per-phase device times from HIP
+events on the library's stream, and the CPU oracle (the reference's algorithm
+restated in C++, OpenMP) on the host cores.  Configuration 2 (unit sphere,
+6 x 64 x 64 Q1 cells, 24,578 DoFs, order 6), the 64-problem batch
+(configuration 4) and the 32 right-hand sides of the Wigley hull
+(configuration 5) are measured the same way and reported in the "config2",
+"config4" and "config5" objects.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-config3]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--extra 2,4,5]
     python bench.py --mode matvec --config {2,3} --precision {fp32,fp64} --steps K
     python bench.py --mode copy --mib M --steps K
     python bench.py --mode solve --config {2,3} --precision {fp32,fp64} --steps K
@@ -128,7 +134,7 @@ def setup_handle(pr, stream):
     t0 = time.perf_counter()
     f = P.BEMFMA(trunc_order=pr.p, num_octree_levels=pr.levels, max_nodes_per_leaf=pr.max_nodes_per_leaf,
                  quad_order=pr.quad_order, sing_quad_order=pr.sing_quad_order, precision=P.PBFMM_MIXED,
-                 stream=stream.h)
+                 stream=stream.h, sing_rule=pr.sing_rule)
     f.init_fma(pr.mesh, pr.dirichlet)
     f.generate_octree_blocking()
     t1 = time.perf_counter()
@@ -240,6 +246,76 @@ def time_matvecs(f, n, stream, reps=50, phase_reps=10):
     rel = {k: float(np.linalg.norm(outs["fp32"][i] - outs["fp64"][i]) / np.linalg.norm(outs["fp64"][i]))
            for i, k in enumerate(("N", "D"))}
     return mv, phases, rel
+
+
+WORKLOAD = {
+    2: "config 2: unit sphere, cube-projected 6x64x64 Q1 cells, 24,578 DoFs, FMM order 6, leaf <= 20, "
+       "phi = 1/|x - (2,1,1)|, Duffy-split singular rule, GMRES(100) to true rel-res 1e-8",
+    3: "config 3: prolate spheroid (4,1,1), 6x128x128 Q2 cells (98,304) on an equiangular cube map "
+       "(c -> tan(pi c/4) before projection), 393,218 DoFs, FMM order 8, leaf <= 20, phi = x + 8 seeded point "
+       "sources (seed 3), Duffy-split singular rule, GMRES(100) to true rel-res 1e-8, mixed precision (FP32 "
+       "operator; FP64 singular quadrature, Arnoldi, residual and outer loop)",
+}
+
+
+def config_summary(c, workload):
+    """The per-configuration object: solves, rates, phases, rooflines."""
+    r = c["roofline"]
+    return {
+        "workload": workload,
+        "n_dofs": c["wc"]["n_dofs"],
+        "mixed_ms_per_solve": c["solves"]["mixed"]["ms"],
+        "fp64_ms_per_solve": c["solves"]["fp64"]["ms"],
+        "mixed_speedup_vs_fp64": c["solves"]["fp64"]["ms"] / c["solves"]["mixed"]["ms"],
+        "solves": c["solves"],
+        "operator_applications_per_s": {k: v["per_s"] for k, v in c["operator"].items()},
+        "fmm_matvecs_per_s": {k: v["per_s"] for k, v in c["matvec"].items()},
+        "spmv_gbs": {k: r[k]["spmv"]["achieved"] for k in r},
+        "m2l_tflops": {k: r[k]["m2l"]["achieved"] for k in r},
+        "fp32_vs_fp64_matvec": c["fp32_vs_fp64_matvec"],
+        "operator_phase_ms": c["operator_phases"],
+        "matvec_phase_ms": c["phases"],
+        "phase_roofline": {k: {ph_: {kk: v[kk] for kk in ("kernel", "achieved", "unit", "frac")}
+                               for ph_, v in r[k].items()} for k in r},
+        "setup_s": c["setup"],
+        "work_counts": c["wc"],
+        "tuned_choices": c["tuning"],
+    }
+
+
+def headline(c, args):
+    """The JSON line's contract fields for configuration c (mixed solve),
+    with the roofline of the dominant kernel of its FP32 operator."""
+    ph = c["operator_phases"]["fp32"]
+    dom = max(PHASES, key=lambda k: ph[k])
+    roof = dict(c["roofline"]["fp32"][dom])
+    roof["traffic"] = traffic_for(roof["kernel"], "config 3")
+    hl = dict(bound=roof["bound"], achieved=roof["achieved"], peak=roof["peak"], unit=roof["unit"],
+              frac=roof["frac"], traffic=roof["traffic"], kernel=roof["kernel"],
+              algorithmic_per_launch=roof["algorithmic_per_launch"], launch_ms=roof["launch_ms"])
+    wc = c["wc"]
+    out = {
+        "metric": METRIC,
+        "value": 1e3 / c["solves"]["mixed"]["ms"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": c["solves"]["mixed"]["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD[3], "n_dofs": wc["n_dofs"], "trunc_order": wc["trunc_order"],
+                   "parallelism": "single GPU"},
+        "roofline": hl,
+    }
+    summ = config_summary(c, WORKLOAD[3])
+    for k in ("workload", "n_dofs"):
+        summ.pop(k)
+    out.update(summ)
+    return out
 
 
 def measure_config(config, steps, warmup):
@@ -389,27 +465,55 @@ def measure_leaf_tuning():
                 ms_per_application={str(k): v for k, v in times.items()}, default=20)
 
 
-def cpu_baseline(pr, ops, seconds):
-    """The oracle's FP64 vmult (reference algorithm restated in C++/OpenMP) on
-    the host cores, over a bounded number of calls, scaled to one solve."""
+def host_cpu():
+    """CPU model, logical CPUs and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return dict(model=model, nproc=os.cpu_count() or 1, available=avail)
+
+
+def cpu_baseline(pr, ops, label):
+    """The oracle's FP64 vmult (the reference algorithm restated in
+    C++/OpenMP: the same octree and lists, naive O(p^4) translations,
+    per-(leaf, cell) unit multipoles) on the host cores.  One call is the
+    sample (at configuration 3 about 25-50 s of CPU time on 8-16 cores); the
+    per-solve figure scales it by the operator applications of the FP64 GPU
+    solve, since a whole CPU solve would take ~30 min.  Threads: every CPU
+    this process may use, capped by OMP_NUM_THREADS when set (the GPU host
+    allots each GPU a share of its cores)."""
     import oracle
-    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
+    cpu = host_cpu()
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(cpu["available"], cap) if cap > 0 else cpu["available"]
     oracle.set_threads(threads)
+    t = time.perf_counter()
     o = oracle.OracleFMM(pr).setup()
     o.compute_alpha()
+    setup_s = time.perf_counter() - t
     v = np.random.default_rng(7).uniform(-1, 1, pr.mesh.n_dofs)
     t = time.perf_counter()
-    k = 0
-    while k < 3 or (time.perf_counter() - t < seconds and k < 50):
-        o.vmult(v)
-        k += 1
-    per = (time.perf_counter() - t) / k
+    o.vmult(v)
+    per = time.perf_counter() - t
     cpu_solve = per * ops
-    log(f"cpu oracle: {per * 1e3:.1f} ms per vmult on {threads} threads -> {cpu_solve:.1f} s per solve")
+    log(f"cpu oracle ({cpu['model']}, {threads} of {cpu['nproc']} CPUs): set-up {setup_s:.1f} s, {per:.2f} s per "
+        f"vmult -> {cpu_solve:.0f} s per solve ({ops} applications)")
     return dict(value=1.0 / cpu_solve, unit="solves/s", cores=threads, kind="port",
-                sample=f"{k} oracle vmult calls at config 2 (FP64 FMM operator, the reference algorithm restated in "
-                       f"C++/OpenMP), {per * 1e3:.1f} ms each, times the {ops} operator applications of the FP64 "
-                       f"GPU solve", ms_per_solve=cpu_solve * 1e3, ms_per_matvec=per * 1e3)
+                sample=f"one oracle vmult at {label} (FP64 FMM operator, the reference algorithm restated in "
+                       f"C++/OpenMP) timed: {per:.2f} s; scaled by the {ops} operator applications of the FP64 GPU "
+                       f"solve (extrapolated: a whole CPU solve is not run)",
+                s_per_solve=cpu_solve, s_per_vmult=per, setup_s=setup_s, cpu_model=cpu["model"],
+                nproc=cpu["nproc"], cpus_available=cpu["available"])
 
 
 def mode_matvec(args):
@@ -474,12 +578,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-config3", action="store_true")
     ap.add_argument("--no-leaf-tuning", action="store_true")
-    ap.add_argument("--extra", default="3,4,5", help="further configurations to measure (comma list of 3, 4, 5)")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--extra", default="2,4,5", help="further configurations to measure (comma list of 2, 4, 5)")
     ap.add_argument("--mode", choices=("full", "matvec", "solve", "copy"), default="full")
-    ap.add_argument("--config", type=int, choices=(2, 3), default=2)
+    ap.add_argument("--config", type=int, choices=(2, 3), default=3)
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--mib", type=int, default=1024)
     args = ap.parse_args()
@@ -498,84 +600,19 @@ def main():
     if args.mode == "solve":
         return mode_solve(args)
 
-    c2 = measure_config(2, args.steps, args.warmup)
-    wc = c2["wc"]
-    # dominant kernel of the headline solve's FP32 operator
-    ph = c2["operator_phases"]["fp32"]
-    dom = max(PHASES, key=lambda k: ph[k])
-    roof = dict(c2["roofline"]["fp32"][dom])
-    roof["traffic"] = traffic_for(roof["kernel"], "config 2")
-    hl = dict(bound=roof["bound"], achieved=roof["achieved"], peak=roof["peak"], unit=roof["unit"],
-              frac=roof["frac"], traffic=roof["traffic"], kernel=roof["kernel"],
-              algorithmic_per_launch=roof["algorithmic_per_launch"], launch_ms=roof["launch_ms"])
-    sp = c2["roofline"]
-    out = {
-        "metric": METRIC,
-        "value": 1e3 / c2["solves"]["mixed"]["ms"],
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": c2["solves"]["mixed"]["ms"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": "config 2: unit sphere, cube-projected 6x64x64 Q1 cells, 24,578 DoFs, FMM order 6, "
-                               "leaf <= 20, GMRES(100) to true rel-res 1e-8, mixed precision (FP32 operator, FP64 "
-                               "singular quadrature, Arnoldi, residual and refinement)",
-                   "n_dofs": wc["n_dofs"], "trunc_order": 6, "parallelism": "single GPU"},
-        "fp64_ms_per_solve": c2["solves"]["fp64"]["ms"],
-        "mixed_speedup_vs_fp64": c2["solves"]["fp64"]["ms"] / c2["solves"]["mixed"]["ms"],
-        "solves": c2["solves"],
-        "operator_applications_per_s": {k: v["per_s"] for k, v in c2["operator"].items()},
-        "fmm_matvecs_per_s": {k: v["per_s"] for k, v in c2["matvec"].items()},
-        "spmv_gbs": {k: sp[k]["spmv"]["achieved"] for k in sp},
-        "m2l_tflops": {k: sp[k]["m2l"]["achieved"] for k in sp},
-        "fp32_vs_fp64_matvec": c2["fp32_vs_fp64_matvec"],
-        "operator_phase_ms": c2["operator_phases"],
-        "matvec_phase_ms": c2["phases"],
-        "phase_roofline": {k: {ph_: {kk: v[kk] for kk in ("kernel", "achieved", "unit", "frac")}
-                               for ph_, v in sp[k].items()} for k in sp},
-        "setup_s": c2["setup"],
-        "work_counts": wc,
-        "tuned_choices": c2["tuning"],
-        "roofline": hl,
-    }
+    c3 = measure_config(3, args.steps, args.warmup)
+    out = headline(c3, args)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(c2["pr"], c2["solves"]["fp64"]["iters"] + 2, args.cpu_seconds)
+        cpu = cpu_baseline(c3["pr"], c3["solves"]["fp64"]["iters"] + 2, "config 3")
     out["cpu_baseline"] = cpu
     out["speedup_vs_cpu"] = out["value"] / cpu["value"] if cpu else None
-    del c2
+    del c3
     extra = {int(x) for x in args.extra.split(",") if x.strip()}
-    if args.no_config3:
-        extra.discard(3)
-    if 3 in extra:
-        c3 = measure_config(3, max(1, min(args.steps, 3)), 1)
-        r3 = c3["roofline"]
-        out["config3"] = {
-            "workload": "config 3: prolate spheroid (4,1,1), cube-projected 6x128x128 Q2 cells with pole-stretched "
-                        "nodes, 393,218 DoFs, FMM order 8, phi = x + 8 seeded point sources, GMRES(100) to 1e-8",
-            "mixed_ms_per_solve": c3["solves"]["mixed"]["ms"],
-            "fp64_ms_per_solve": c3["solves"]["fp64"]["ms"],
-            "mixed_speedup_vs_fp64": c3["solves"]["fp64"]["ms"] / c3["solves"]["mixed"]["ms"],
-            "solves": c3["solves"],
-            "operator_applications_per_s": {k: v["per_s"] for k, v in c3["operator"].items()},
-            "fmm_matvecs_per_s": {k: v["per_s"] for k, v in c3["matvec"].items()},
-            "spmv_gbs": {k: r3[k]["spmv"]["achieved"] for k in r3},
-            "m2l_tflops": {k: r3[k]["m2l"]["achieved"] for k in r3},
-            "fp32_vs_fp64_matvec": c3["fp32_vs_fp64_matvec"],
-            "operator_phase_ms": c3["operator_phases"],
-            "matvec_phase_ms": c3["phases"],
-            "phase_roofline": {k: {ph_: {kk: v[kk] for kk in ("kernel", "achieved", "unit", "frac")}
-                                   for ph_, v in r3[k].items()} for k in r3},
-            "setup_s": c3["setup"],
-            "work_counts": c3["wc"],
-            "tuned_choices": c3["tuning"],
-        }
-        del c3
+    if 2 in extra:
+        c2 = measure_config(2, args.steps, args.warmup)
+        out["config2"] = config_summary(c2, WORKLOAD[2])
+        del c2
     if 5 in extra:
         out["config5"] = measure_multi_rhs(args.steps)
     if 4 in extra:

@@ -36,6 +36,15 @@ extern "C" {
                          singular quadrature, alpha, Arnoldi, residual and
                          iterative refinement */
 
+/* Singular quadrature of the near field (cells holding the collocation node
+ * or one of its double nodes). */
+#define PBFMM_SING_TELLES 0 /* QTelles<2>(sing_quad_order, support point): the
+                               rule of BEMFMA::direct_integrals (bem_fma.cc:127-131) */
+#define PBFMM_SING_DUFFY 1  /* QSplit<2>(QDuffy(sing_quad_order, 1), support point):
+                               the rule of the dense assembly,
+                               BEMProblem::get_singular_quadrature
+                               (bem_problem.cc:213-229) */
+
 typedef struct pbfmm_handle pbfmm_handle;
 
 /* Parameters of BEMFMA<3>::declare_parameters (bem_fma.cc:64-85) and of the
@@ -46,11 +55,13 @@ typedef struct pbfmm_params {
                                  Childless Block", default 20 */
   int32_t trunc_order;        /* "FMA Truncation Order", default 6, 1..10 */
   int32_t quad_order;         /* QGauss order of the regular rule, default 4 */
-  int32_t sing_quad_order;    /* QTelles order of the singular rule, default 5 */
+  int32_t sing_quad_order;    /* Gauss order of the singular rule, default 5 */
   int32_t precision;          /* PBFMM_FP64 or PBFMM_MIXED (solve_system) */
   int32_t device;             /* HIP device ordinal */
   void *stream;               /* hipStream_t, NULL = library-owned stream */
   int32_t preconditioner;     /* 0 none, 1 near-field block Jacobi (default) */
+  int32_t sing_rule;          /* PBFMM_SING_TELLES (default, as BEMFMA) or
+                                 PBFMM_SING_DUFFY (as the dense solve) */
 } pbfmm_params;
 
 /* Defaults of bem_fma.cc:64-85 and bem_problem.cc:268-269. */
@@ -87,8 +98,11 @@ int pbfmm_set_double_body(pbfmm_handle *h, int32_t enable, double z0);
 int pbfmm_generate_octree_blocking(pbfmm_handle *h);
 
 /* BEMFMA<3>::compute_m2l_flags (bem_fma.h:173, body bem_fma.cc:1846-1956,
- * commented out in the reference).  Single process: writes the number of M2L
- * translations per level, levels+1 entries, into m2l_count_per_level. */
+ * commented out in the reference).  Single process: writes the reference's
+ * m2l_operations_per_level, levels+1 entries, into m2l_count_per_level: per
+ * level, the nonIntList entries of all sublevels summed over the blocks that
+ * hold nodes (same-level entries are M2L translations, finer ones M2P
+ * evaluations, coarser ones direct integrals).  Needs the octree only. */
 int pbfmm_compute_m2l_flags(pbfmm_handle *h, int64_t *m2l_count_per_level);
 
 /* BEMFMA<3>::direct_integrals (bem_fma.h:127, bem_fma.cc:107-1600): the
@@ -182,7 +196,9 @@ int pbfmm_vmult_dev_multi(pbfmm_handle *h, const double *d_src, double *d_dst, i
                           int32_t precision);
 
 /* Device-pointer solve: rhs_dev from pbfmm_compute_rhs semantics already
- * applied (constrained rows set), x_dev initial guess in, solution out. */
+ * applied (constrained rows set), x_dev initial guess in, solution out.  The
+ * two vectors may not overlap (PBFMM_ERR_ARG): the solve reads rhs again
+ * after it has updated x. */
 int pbfmm_gmres_dev(pbfmm_handle *h, const double *d_rhs, double *d_x, double rel_tol,
                     int32_t restart, int32_t max_it, int32_t precision, int32_t *iters,
                     double *relres);
@@ -194,7 +210,7 @@ int pbfmm_gmres_dev(pbfmm_handle *h, const double *d_rhs, double *d_x, double re
  * gmres_dev), and every operator application serves all of them, with one
  * translation launch per level for all right-hand sides (pair lists expanded
  * over them) and the near field, unit multipoles, operators and
- * preconditioner shared.  iters gets the Arnoldi steps each right-hand side
+ * preconditioner shared; the rhs and x ranges may not overlap.  iters gets the Arnoldi steps each right-hand side
  * used and relres its true relative residual (nrhs entries, may be NULL).  No
  * counterpart in the reference, which solves one right-hand side per
  * BEMProblem::solve. */

@@ -42,6 +42,7 @@ def lib():
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_init.argtypes = [C.c_void_p, C.c_int64, _d, C.c_int64, C.c_int, _i64, _d, _i64, _i64, _u8]
         L.orc_set_double_body.argtypes = [C.c_void_p, C.c_double]
+        L.orc_set_sing_rule.argtypes = [C.c_void_p, C.c_int]
         for f in ("orc_generate_octree_blocking", "orc_direct_integrals", "orc_multipole_integrals"):
             getattr(L, f).argtypes = [C.c_void_p]
         L.orc_generate_multipole_expansions.argtypes = [C.c_void_p, _d, _d]
@@ -182,15 +183,19 @@ class OracleFMM:
     """CPU restatement of BEMFMA<3> + the FMA branch of BEMProblem<3>."""
 
     def __init__(self, problem=None, *, p=6, levels=10, max_nodes=20, quad_order=4, sing_order=5, mesh=None,
-                 dirichlet=None):
+                 dirichlet=None, sing_rule=0):
         if problem is not None:
             mesh, dirichlet = problem.mesh, problem.dirichlet
             p, levels, max_nodes = problem.p, problem.levels, problem.max_nodes_per_leaf
             quad_order, sing_order = problem.quad_order, problem.sing_quad_order
+            sing_rule = problem.sing_rule
         self.p = p
         self.levels = levels
         self.mesh = mesh
         self.h = lib().orc_create(p, levels, max_nodes, quad_order, sing_order)
+        self.sing_rule = sing_rule
+        if lib().orc_set_sing_rule(self.h, int(sing_rule)) != 0:
+            raise ValueError(f"singular rule {sing_rule}")
         m = mesh
         self._keep = [_f64(m.support_xyz), _i(m.cell_dofs), _i(m.dbl_ptr), _i(m.dbl_idx),
                       np.ascontiguousarray(dirichlet, dtype=np.uint8)]
@@ -299,7 +304,10 @@ class OracleFMM:
                                C.byref(it), C.byref(rr))
         return phi, dphi, it.value, rr.value
 
-    def dense(self, rule=0):
+    def dense(self, rule=None):
+        """Dense collocation matrices (bem_problem.cc:477-950); rule None
+        takes this oracle's near-field rule (0 Telles, 1 Duffy split)."""
+        rule = self.sing_rule if rule is None else rule
         Nm, Dm = np.zeros((self.n, self.n)), np.zeros((self.n, self.n))
         lib().orc_dense_assemble(self.h, rule, _p(Nm), _p(Dm))
         return Nm, Dm

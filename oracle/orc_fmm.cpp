@@ -8,21 +8,27 @@
 //
 // Parity status per function:
 //   expansions                     pinned (reference goldens, orc_math.h)
-//   octree + lists                 pinned by invariants in tests/test_oracle_tree.py
+//   octree + lists                 pinned by brute force in tests/test_oracle_tree.py
 //                                  (partition of nodes/quad points, leaf rule,
-//                                  list disjointness, near+far+bigger cover
-//                                  every quad point exactly once per node)
+//                                  M2L pairs non-adjacent under TOLL, double-node
+//                                  intList augmentation, near + far + bigger
+//                                  cover every quad point exactly once per node)
 //   direct_integrals               pinned: regular part against brute force
-//                                  kernel sums; singular part against the
-//                                  closed-form corner integral and the
-//                                  flat-panel identity; whole operator
-//                                  against the dense assembly (FMM error only)
+//                                  kernel sums; singular rules against closed
+//                                  forms at corner, edge and centre points, the
+//                                  Telles cubic's defining identities and the
+//                                  Duffy split's polynomial exactness
+//                                  (tests/test_oracle_quadrature.py); whole
+//                                  operator against the dense assembly
 //   multipole_matr_vect_products   pinned against the dense collocation
-//                                  matvec (truncation error decays with p)
+//                                  matvec with the same singular rule
+//                                  (truncation error decays with p)
 //   vmult / rhs / alpha / GMRES    pinned against the analytic harmonic
 //                                  solution (Green's identity residual)
-//   QTelles node set of deal.II    parity unpinned (deal.II not vendored;
-//                                  Telles 1987 restated)
+//   deal.II's QTelles / QSplit(QDuffy) as such: not vendored; restated from
+//                                  Telles 1987 and Duffy 1982 and pinned by the
+//                                  identities above (bit-exact node order of a
+//                                  deal.II build stays unpinned)
 #include <omp.h>
 
 #include <algorithm>
@@ -60,6 +66,7 @@ struct CellME {
 
 struct Oracle {
   int p = 6, levels = 10, maxnodes = 20, qorder = 4, sqorder = 5;
+  int sing_rule = 0;  // near field: 0 Telles (bem_fma.cc:127-131), 1 Duffy split (bem_problem.cc:213-229)
   int64_t N = 0, ncells = 0;
   int64_t nreal = 0;  // cells [nreal, ncells) are double-body mirror images, never singular
   int dpc = 4;
@@ -414,7 +421,7 @@ static void direct_integrals(Oracle &o) {
           }
         double Nj[9], Dj[9];
         if (js < 0) integrate_regular(o, node, cell, cq.second, Nj, Dj);
-        else integrate_singular(o, node, cell, js, 0, Nj, Dj);
+        else integrate_singular(o, node, cell, js, o.sing_rule, Nj, Dj);
         add_row(r, cell, Nj, Dj);
       }
     }
@@ -900,6 +907,14 @@ void *orc_create(int p, int levels, int maxnodes, int qorder, int sqorder) {
   return o;
 }
 void orc_destroy(void *h) { delete (Oracle *)h; }
+// The near field's singular rule (before direct_integrals): 0 QTelles as
+// BEMFMA::direct_integrals (bem_fma.cc:127-131), 1 QSplit(QDuffy) as
+// BEMProblem::get_singular_quadrature (bem_problem.cc:213-229).
+int orc_set_sing_rule(void *h, int rule) {
+  if (rule != 0 && rule != 1) return -1;
+  ((Oracle *)h)->sing_rule = rule;
+  return 0;
+}
 
 int orc_init(void *h, int64_t n, const double *xyz, int64_t ncells, int dpc, const int64_t *cdofs,
              const double *geom, const int64_t *dptr, const int64_t *didx, const uint8_t *dirm) {

@@ -140,6 +140,7 @@ void pbfmm_default_params(pbfmm_params *p) {
   p->device = 0;
   p->stream = nullptr;
   p->preconditioner = 1;
+  p->sing_rule = PBFMM_SING_TELLES;
 }
 
 int pbfmm_create(const pbfmm_params *p, pbfmm_handle **out) {
@@ -148,7 +149,8 @@ int pbfmm_create(const pbfmm_params *p, pbfmm_handle **out) {
   if (p->trunc_order < 1 || p->trunc_order > 10 || p->num_octree_levels < 1 || p->num_octree_levels > 20 ||
       p->max_nodes_per_leaf < 1 || p->quad_order < 1 || p->quad_order > 8 || p->sing_quad_order < 1 ||
       p->sing_quad_order > 16 || (p->precision != PBFMM_FP64 && p->precision != PBFMM_MIXED) ||
-      p->preconditioner < 0 || p->preconditioner > 1)
+      p->preconditioner < 0 || p->preconditioner > 1 ||
+      (p->sing_rule != PBFMM_SING_TELLES && p->sing_rule != PBFMM_SING_DUFFY))
     return PBFMM_ERR_ARG;
   pbfmm_handle *h = nullptr;
   try {
@@ -159,6 +161,7 @@ int pbfmm_create(const pbfmm_params *p, pbfmm_handle **out) {
     h->eng.kp = kp_of(p->trunc_order);
     h->eng.L = p->num_octree_levels;
     h->eng.sing_order = p->sing_quad_order;
+    h->eng.sing_rule = p->sing_rule;
   } catch (...) {
     delete h;
     return PBFMM_ERR_HIP;
@@ -281,13 +284,25 @@ int pbfmm_generate_octree_blocking(pbfmm_handle *h) {
   }, false);
 }
 
+// The census of compute_m2l_flags (bem_fma.cc:1846-1956): per level, the
+// nonIntList entries of every sublevel summed over the blocks that hold
+// nodes (dofs_filled_blocks), which the reference splits across MPI ranks;
+// one process here, so no split.
 int pbfmm_compute_m2l_flags(pbfmm_handle *h, int64_t *counts) {
   return guarded(h, [&]() -> int {
-    if (int rc = need(h, 4, "compute_m2l_flags")) return rc;
+    if (int rc = not_batch(h, "compute_m2l_flags")) return rc;
+    if (int rc = need(h, 2, "compute_m2l_flags")) return rc;
     if (!counts) return fail(h, PBFMM_ERR_ARG, "compute_m2l_flags: null output");
-    for (int l = 0; l <= h->eng.L; ++l) counts[l] = h->eng.far.m2l_per_level[l];
+    const Tree &t = h->eng.tree;
+    for (int l = 0; l <= t.L; ++l) {
+      int64_t c = 0;
+      if (l >= 1)
+        for (int32_t b : t.dofs_filled[l])
+          for (int s = 0; s < t.nsub[b]; ++s) c += (int64_t)t.nil[t.sub_beg[b] + s].size();
+      counts[l] = c;
+    }
     return PBFMM_OK;
-  });
+  }, false);
 }
 
 int pbfmm_direct_integrals(pbfmm_handle *h) {
@@ -564,11 +579,21 @@ int pbfmm_vmult_dev_multi(pbfmm_handle *h, const double *src, double *dst, int32
   });
 }
 
+// Byte ranges [a, a + span) and [b, b + span) share memory.
+static bool ranges_overlap(const void *a, const void *b, size_t span) {
+  const char *p = static_cast<const char *>(a), *q = static_cast<const char *>(b);
+  return p < q + span && q < p + span;
+}
+
 int pbfmm_gmres_dev(pbfmm_handle *h, const double *rhs, double *x, double rel_tol, int32_t restart, int32_t max_it,
                     int32_t precision, int32_t *iters, double *relres) {
   return guarded(h, [&]() -> int {
     if (int rc = need(h, 4, "gmres_dev")) return rc;
     if (!rhs || !x) return fail(h, PBFMM_ERR_ARG, "gmres_dev: null vector");
+    // the solve reads rhs again after updating x (restart residuals, the
+    // outer FP64 residual), so the two may not share memory
+    if (ranges_overlap(rhs, x, (size_t)h->eng.n * 8))
+      return fail(h, PBFMM_ERR_ARG, "gmres_dev: rhs and x overlap");
     if (restart < 1 || restart > 127 || max_it < 1 || !(rel_tol > 0))
       return fail(h, PBFMM_ERR_ARG, "gmres_dev: need 1 <= restart <= 127, max_it >= 1, rel_tol > 0");
     if (precision == PBFMM_MIXED && !h->eng.have_f32)
@@ -589,6 +614,8 @@ int pbfmm_gmres_dev_multi(pbfmm_handle *h, const double *rhs, double *x, int32_t
     if (int rc = need(h, 4, "gmres_dev_multi")) return rc;
     if (!rhs || !x || nrhs < 1 || ld < h->eng.n)
       return fail(h, PBFMM_ERR_ARG, "gmres_dev_multi: need rhs, x, nrhs >= 1 and ld >= n_dofs");
+    if (ranges_overlap(rhs, x, (size_t)((nrhs - 1) * ld + h->eng.n) * 8))
+      return fail(h, PBFMM_ERR_ARG, "gmres_dev_multi: rhs and x overlap");
     if (restart < 1 || restart > 127 || max_it < 1 || !(rel_tol > 0))
       return fail(h, PBFMM_ERR_ARG, "gmres_dev_multi: need 1 <= restart <= 127, max_it >= 1, rel_tol > 0");
     if (precision == PBFMM_MIXED && !h->eng.have_f32)
@@ -1092,6 +1119,8 @@ int pbfmm_batch_gmres_dev(pbfmm_handle *h, const double *d_rhs, double *d_x, dou
     if (!h->batch) return fail(h, PBFMM_ERR_STATE, "batch_gmres_dev: not a batch handle");
     if (!d_rhs || !d_x || restart < 1 || max_it < 1 || !(rel_tol > 0))
       return fail(h, PBFMM_ERR_ARG, "batch_gmres_dev: bad arguments");
+    if (ranges_overlap(d_rhs, d_x, (size_t)h->eng.n * 8))
+      return fail(h, PBFMM_ERR_ARG, "batch_gmres_dev: rhs and x overlap");
     if (precision != PBFMM_FP64 && precision != PBFMM_MIXED)
       return fail(h, PBFMM_ERR_ARG, "batch_gmres_dev: unknown precision");
     ensure_prec(h);

@@ -59,6 +59,7 @@ struct Engine {
   int device = 0;
   int p = 6, kt = 0, kp = 0, L = 10;
   int dpc = 4, nq = 16, sing_order = 5;
+  int sing_rule = 0;  // 0 Telles tensor rule, 1 Duffy split (PBFMM_SING_*)
   int64_t n = 0;
   // host plans
   HostMesh mesh;

@@ -129,8 +129,9 @@ __device__ __forceinline__ T wave_sum(T v) {
 // ---------------------------------------------------------------------------
 // Near-field assembly (BEMFMA::direct_integrals, bem_fma.cc:670-913 regular
 // and singular, 997-1126 bigger blocks).  One thread per (row, entry): the
-// regular rule over the entry's quad points, or the Telles rule over the
-// whole cell when the cell holds a double of the row's node, both in FP64.
+// regular rule over the entry's quad points, or the singular rule (Telles or
+// Duffy split, chosen at set-up) over the whole cell when the cell holds a
+// double of the row's node, both in FP64.
 // Results are added into the leaf's rows x cols block of (N, D) pairs; the
 // FP32 operator is this block rounded once (k_round_f32), since in FP32 the
 // difference R = y - x loses ~|x|/|R| ulps and R.n cancels again, which at a
@@ -1650,9 +1651,11 @@ void Engine::build_unit_multipoles(bool want_f32) {
   sync();
 }
 
-// Singular rules for every local support point js: Telles in each direction
-// (QTelles<2>(sing_quad_order, support_point), bem_fma.cc:127-131), with the
-// shape values and gradients at the rule points.
+// Singular rules for every local support point js, with the shape values and
+// gradients at the rule points: Telles in each direction
+// (QTelles<2>(sing_quad_order, support_point), bem_fma.cc:127-131) or the
+// Duffy split (QSplit<2>(QDuffy(sing_quad_order, 1), support_point),
+// bem_problem.cc:213-229).
 void Engine::assemble_near(bool want_f32) {
   const int ps = 1 + 3 * dpc;
   std::vector<std::vector<double>> rules(dpc);
@@ -1661,10 +1664,15 @@ void Engine::assemble_near(bool want_f32) {
   for (int js = 0; js < dpc; ++js) {
     double sx, sy;
     unit_support_point(dpc, js, sx, sy);
-    std::vector<double> x1, w1, x2, w2;
-    telles_rule01(sing_order, sx, x1, w1);
-    telles_rule01(sing_order, sy, x2, w2);
-    Quadrature2D r = tensor_rule(x1, w1, x2, w2);
+    Quadrature2D r;
+    if (sing_rule == 1) {
+      r = duffy_split_rule(sing_order, sx, sy);
+    } else {
+      std::vector<double> x1, w1, x2, w2;
+      telles_rule01(sing_order, sx, x1, w1);
+      telles_rule01(sing_order, sy, x2, w2);
+      r = tensor_rule(x1, w1, x2, w2);
+    }
     const int nqs = (int)r.w.size();
     cnt[js] = nqs;
     maxq = std::max(maxq, nqs);

@@ -72,7 +72,7 @@ void merge_forest(const std::vector<const Engine *> &parts, Engine &e, std::vect
   const Engine &a = *parts[0];
   for (const Engine *q : parts) {
     if (q->p != a.p || q->L != a.L || q->mesh.dpc != a.mesh.dpc || q->qd.nq != a.qd.nq ||
-        q->sing_order != a.sing_order)
+        q->sing_order != a.sing_order || q->sing_rule != a.sing_rule)
       throw std::runtime_error("forest: parts differ in order, levels, element or quadrature");
     if (q->mesh.ncells != q->mesh.ncells_real)
       throw std::runtime_error("forest: double-body parts are not supported");
@@ -88,6 +88,7 @@ void merge_forest(const std::vector<const Engine *> &parts, Engine &e, std::vect
   e.dpc = dpc;
   e.nq = nq;
   e.sing_order = a.sing_order;
+  e.sing_rule = a.sing_rule;
   e.have_dirichlet = true;
   HostMesh &m = e.mesh;
   m = HostMesh();

@@ -49,11 +49,13 @@ struct Quadrature2D {
 };
 
 // Gauss-Legendre on [0,1] (Golub-Welsch free Newton iteration), Telles cubic
-// transformation, tensor products; Lagrange Q1/Q2 in FE_Q numbering.
+// transformation, tensor products, the Duffy split rule; Lagrange Q1/Q2 in
+// FE_Q numbering.
 void gauss_legendre01(int n, std::vector<double> &x, std::vector<double> &w);
 void telles_rule01(int n, double x0, std::vector<double> &x, std::vector<double> &w);
 Quadrature2D tensor_rule(const std::vector<double> &x1, const std::vector<double> &w1,
                          const std::vector<double> &x2, const std::vector<double> &w2);
+Quadrature2D duffy_split_rule(int n, double sx, double sy);
 void lagrange_q(int dpc, double xi, double eta, double *N, double *dNx, double *dNy);
 void unit_support_point(int dpc, int j, double &xi, double &eta);
 

@@ -73,6 +73,34 @@ Quadrature2D tensor_rule(const std::vector<double> &x1, const std::vector<double
   return r;
 }
 
+// QSplit<2>(QDuffy(n, 1), s), the dense assembly's singular rule
+// (bem_problem.cc:213-229; deal.II's QSplit and QDuffy, written here from
+// Duffy 1982).  The unit square is cut into the triangles (s, a, b) over its
+// four sides a-b, in deal.II's face order x=0, x=1, y=0, y=1; a side through s
+// spans no area and is skipped.  Each triangle takes the Duffy collapse of an
+// n x n Gauss rule on [0,1]^2, radial coordinate u (fastest) and angular v:
+//   (u, v) -> s + u (a - s) + u v (b - a),  weight w_u w_v u |det(a-s, b-s)|,
+// whose factor u cancels the 1/r of the Laplace kernels at s.
+Quadrature2D duffy_split_rule(int n, double sx, double sy) {
+  std::vector<double> g, gw;
+  gauss_legendre01(n, g, gw);
+  static const double side[4][4] = {{0, 0, 0, 1}, {1, 0, 1, 1}, {0, 0, 1, 0}, {0, 1, 1, 1}};
+  Quadrature2D r;
+  for (const auto &sd : side) {
+    const double ax = sd[0] - sx, ay = sd[1] - sy, bx = sd[2] - sx, by = sd[3] - sy;
+    const double area2 = std::fabs(ax * by - ay * bx);
+    if (area2 < 1e-12) continue;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const double u = g[i], v = g[j];
+        r.pts.push_back(sx + u * ax + u * v * (bx - ax));
+        r.pts.push_back(sy + u * ay + u * v * (by - ay));
+        r.w.push_back(gw[i] * gw[j] * u * area2);
+      }
+  }
+  return r;
+}
+
 // FE_Q numbering: vertices (0,0),(1,0),(0,1),(1,1); lines x=0, x=1, y=0, y=1;
 // interior.  1-D factors indexed 0 -> node 0, 1 -> node 1, 2 -> midpoint.
 static const int kQmap[9][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}, {0, 2}, {1, 2}, {2, 0}, {2, 1}, {2, 2}};

@@ -20,9 +20,10 @@ import time
 
 import numpy as np
 
-from ._lib import PBFMM_FP64, PBFMM_MIXED, Params, PbfmmError, load
+from ._lib import PBFMM_FP64, PBFMM_MIXED, PBFMM_SING_DUFFY, PBFMM_SING_TELLES, Params, PbfmmError, load
 
-__all__ = ["BEMFMA", "BatchFMA", "BEMProblem", "Body", "setup_batch", "tune_leaf_size", "PBFMM_FP64", "PBFMM_MIXED", "PbfmmError", "load"]
+__all__ = ["BEMFMA", "BatchFMA", "BEMProblem", "Body", "setup_batch", "tune_leaf_size", "PBFMM_FP64", "PBFMM_MIXED",
+           "PBFMM_SING_TELLES", "PBFMM_SING_DUFFY", "PbfmmError", "load"]
 
 _d = C.POINTER(C.c_double)
 _i64 = C.POINTER(C.c_int64)
@@ -45,10 +46,14 @@ def _i(a):
 class BEMFMA:
     """BEMFMA<3> over the C ABI.  Parameters as in BEMFMA::declare_parameters
     (bem_fma.cc:64-85): num_octree_levels=10, max_nodes_per_leaf=20,
-    trunc_order=6; quadrature orders as handed to init_fma (4, 5)."""
+    trunc_order=6; quadrature orders as handed to init_fma (4, 5).
+    sing_rule selects the singular rule: PBFMM_SING_TELLES (BEMFMA's QTelles,
+    bem_fma.cc:127-131) or PBFMM_SING_DUFFY (the dense assembly's
+    QSplit(QDuffy), bem_problem.cc:213-229)."""
 
     def __init__(self, trunc_order=6, num_octree_levels=10, max_nodes_per_leaf=20, quad_order=4,
-                 sing_quad_order=5, precision=PBFMM_FP64, device=0, stream=None, preconditioner=1):
+                 sing_quad_order=5, precision=PBFMM_FP64, device=0, stream=None, preconditioner=1,
+                 sing_rule=PBFMM_SING_TELLES):
         self._L = load()
         prm = Params()
         self._L.pbfmm_default_params(C.byref(prm))
@@ -61,6 +66,7 @@ class BEMFMA:
         prm.device = device
         prm.stream = stream
         prm.preconditioner = preconditioner
+        prm.sing_rule = sing_rule
         self.params = prm
         h = C.c_void_p()
         rc = self._L.pbfmm_create(C.byref(prm), C.byref(h))
@@ -478,7 +484,7 @@ def setup_batch(problems, precision=PBFMM_FP64, device=0, preconditioner=1):
     for pr in problems:
         f = BEMFMA(trunc_order=pr.p, num_octree_levels=pr.levels, max_nodes_per_leaf=pr.max_nodes_per_leaf,
                    quad_order=pr.quad_order, sing_quad_order=pr.sing_quad_order, precision=precision,
-                   device=device, preconditioner=preconditioner)
+                   device=device, preconditioner=preconditioner, sing_rule=pr.sing_rule)
         f.init_fma(pr.mesh, pr.dirichlet)
         parts.append(f)
     return BatchFMA(parts)
@@ -490,7 +496,7 @@ def setup(problem, precision=PBFMM_FP64, device=0, preconditioner=1):
     f = BEMFMA(trunc_order=problem.p, num_octree_levels=problem.levels,
                max_nodes_per_leaf=problem.max_nodes_per_leaf, quad_order=problem.quad_order,
                sing_quad_order=problem.sing_quad_order, precision=precision, device=device,
-               preconditioner=preconditioner)
+               preconditioner=preconditioner, sing_rule=problem.sing_rule)
     f.init_fma(problem.mesh, problem.dirichlet)
     if getattr(problem, "mirror_z0", None) is not None:
         f.set_double_body(True, problem.mirror_z0)

@@ -13,6 +13,8 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libpbfmm.so")
 
 PBFMM_FP64 = 0
 PBFMM_MIXED = 1
+PBFMM_SING_TELLES = 0
+PBFMM_SING_DUFFY = 1
 
 _d = C.POINTER(C.c_double)
 _i64 = C.POINTER(C.c_int64)
@@ -24,7 +26,8 @@ _h = C.c_void_p
 class Params(C.Structure):
     _fields_ = [("num_octree_levels", C.c_int32), ("max_nodes_per_leaf", C.c_int32), ("trunc_order", C.c_int32),
                 ("quad_order", C.c_int32), ("sing_quad_order", C.c_int32), ("precision", C.c_int32),
-                ("device", C.c_int32), ("stream", C.c_void_p), ("preconditioner", C.c_int32)]
+                ("device", C.c_int32), ("stream", C.c_void_p), ("preconditioner", C.c_int32),
+                ("sing_rule", C.c_int32)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/pbfmm.h

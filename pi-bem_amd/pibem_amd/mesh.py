@@ -326,6 +326,9 @@ class Problem:
     levels: int = 10
     quad_order: int = 4
     sing_quad_order: int = 5
+    # singular rule: 1 the Duffy split of the reference's dense solve
+    # (bem_problem.cc:213-229), 0 the Telles rule of its BEMFMA (bem_fma.cc:127-131)
+    sing_rule: int = 1
     # several right-hand sides of one operator: (n_rhs, N) boundary data and
     # the exact solution of each
     rhs_set: np.ndarray = None

@@ -131,6 +131,41 @@ def test_vmult_dev(c1):
         bp.vmult_dev(src.value, src.value, P.PBFMM_FP64)
 
 
+@pytest.mark.parametrize("which", ["c1", "dcube"])
+def test_m2l_census_matches_plan(which, request):
+    """The same-level entries of the nonIntLists that compute_m2l_flags
+    counts are the M2L translations the GPU plan launches (work count 6),
+    and the census covers every nonIntList entry of the handle's tree."""
+    c = request.getfixturevalue(which)
+    f = c.f64
+    census = f.compute_m2l_flags()
+    t = f.tree()
+    same = total = 0
+    for b in range(1, t["num_blocks"]):
+        if t["node_ptr"][b + 1] == t["node_ptr"][b]:
+            continue
+        for s in range(t["sub_ptr"][b], t["sub_ptr"][b + 1]):
+            lst = t["nil_idx"][t["nil_ptr"][s]:t["nil_ptr"][s + 1]]
+            total += len(lst)
+            same += int((t["level"][lst] == t["level"][b]).sum())
+    assert census.sum() == total
+    assert same == f.work_counts()["m2l_pairs"]
+
+
+def test_gmres_dev_rejects_overlap(c1):
+    """The device solves read rhs again after updating x, so rhs and x that
+    share memory are refused (PBFMM_ERR_ARG) before any work is done."""
+    from pibem_amd import hip
+    n = c1.pr.mesh.n_dofs
+    buf = hip.DeviceArray(2 * n)
+    for x_ptr in (buf.value, buf.value + 8 * (n // 2)):
+        with pytest.raises(P.PbfmmError) as ei:
+            c1.f64.gmres_dev(buf.value, x_ptr)
+        assert ei.value.code == -1
+        with pytest.raises(P.PbfmmError):
+            c1.f64.gmres_dev_multi(buf.value, x_ptr, 1, n)
+
+
 @pytest.mark.parametrize("precision", [P.PBFMM_FP64, P.PBFMM_MIXED])
 def test_solve_system(c1, precision):
     f = c1.f64 if precision == P.PBFMM_FP64 else c1.mix
@@ -226,8 +261,10 @@ def test_config3_properties():
     - the operator's combined-expansion FMM equals N(x.neu) - D(x.dir) +
       alpha.(x.neu) assembled from the two-output FMM (FP64, 1e-12);
     - the FP32 operator is within 2e-6 of the FP64 one;
-    - both solves reach true relative residual 1e-8 and the analytic phi to
-      1e-3 on the Neumann DoFs (measured 6e-5)."""
+    - both solves reach true relative residual 1e-8, the analytic phi to
+      1e-3 on the Neumann DoFs (measured 6e-5) and the analytic dphi/dn to
+      1e-2 on the Dirichlet DoFs (with the Duffy split; the Telles rule had
+      left 0.133 there)."""
     from pibem_amd import hip
     pr = M.config_spheroid(128, 8)
     n = pr.mesh.n_dofs
@@ -254,14 +291,19 @@ def test_config3_properties():
     assert e_lin <= 1e-12 and e32 <= 2e-6
     rhs = hip.DeviceArray.from_numpy(bp.compute_rhs(pr.tmp_rhs))
     sol = hip.DeviceArray(n)
-    phi_ex = pr.exact.phi(pr.mesh.support_xyz)
+    xyz = pr.mesh.support_xyz
+    phi_ex = pr.exact.phi(xyz)
+    dn_ex = (pr.exact.grad(xyz) * pr.mesh.normals).sum(1)
+    assert pr.sing_rule == P.PBFMM_SING_DUFFY
     for prec in (P.PBFMM_MIXED, P.PBFMM_FP64):
         sol.copy_from(hip.DeviceArray(n))
         it, rr = f.gmres_dev(rhs.value, sol.value, 1e-8, 100, 2000, prec)
         s = sol.numpy()
         err = rel(s[neu], phi_ex[neu])
-        print(f"  solve precision={prec} iters={it} relres={rr:.3e} err phi {err:.2e}", flush=True)
-        assert rr <= 1e-8 and err <= 1e-3
+        err_dn = rel(s[~neu], dn_ex[~neu])
+        print(f"  solve precision={prec} iters={it} relres={rr:.3e} err phi {err:.2e} dphi/dn {err_dn:.2e}",
+              flush=True)
+        assert rr <= 1e-8 and err <= 1e-3 and err_dn <= 1e-2
 
 
 @pytest.mark.parametrize("scheme", [1, 2], ids=["relaxed", "refinement"])

@@ -68,7 +68,7 @@ def test_wigley_solve_vs_dense(small):
     cons = M.constraints(pr)
     ptr, idx, val, inh, isc = cons
     # dense pure-Neumann system: (N + diag(alpha)) x + mean(x) 1 = D g, double-node rows x_i - x_j = 0
-    Nm, Dm = o.dense(0)
+    Nm, Dm = o.dense()
     alpha = -Nm.sum(axis=1)
     A = Nm + np.diag(alpha) + np.ones((n, n)) / n
     for i in np.nonzero(isc)[0]:

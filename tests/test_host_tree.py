@@ -54,3 +54,32 @@ def test_lists_are_disjoint(oracle_mod):
             il = set(t["il_idx"][t["il_ptr"][s]:t["il_ptr"][s + 1]].tolist())
             nil = set(t["nil_idx"][t["nil_ptr"][s]:t["nil_ptr"][s + 1]].tolist())
             assert not (il & nil)
+
+
+@pytest.mark.parametrize("case", ["sphere16", "spheroid12q2"])
+def test_m2l_census(case, oracle_mod):
+    """compute_m2l_flags (bem_fma.cc:1846-1956): per level, the nonIntList
+    entries of every sublevel over the blocks holding nodes, counted here from
+    the oracle's exported tree; the same-level part is the M2L pair count."""
+    import pibem_amd as P
+    pr = M.config_sphere(16, 4) if case == "sphere16" else M.config_spheroid(12, 6)
+    o = oracle_mod.OracleFMM(pr)
+    o.generate_octree_blocking()
+    t = o.tree()
+    f = P.BEMFMA(trunc_order=pr.p, num_octree_levels=pr.levels, max_nodes_per_leaf=pr.max_nodes_per_leaf)
+    f.init_fma(pr.mesh, pr.dirichlet)
+    with pytest.raises(P.PbfmmError):
+        f.compute_m2l_flags()  # before the octree
+    f.generate_octree_blocking()
+    got = f.compute_m2l_flags()
+    want = np.zeros(pr.levels + 1, np.int64)
+    same = 0
+    for b in range(1, t["num_blocks"]):
+        if t["node_ptr"][b + 1] == t["node_ptr"][b]:
+            continue
+        for s in range(t["sub_ptr"][b], t["sub_ptr"][b + 1]):
+            lst = t["nil_idx"][t["nil_ptr"][s]:t["nil_ptr"][s + 1]]
+            want[t["level"][b]] += len(lst)
+            same += int((t["level"][lst] == t["level"][b]).sum())
+    assert np.array_equal(got, want)
+    assert got.sum() > same > 0

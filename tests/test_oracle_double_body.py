@@ -43,9 +43,9 @@ def _union(pr):
 def test_dense_double_body_folds_union(oracle_mod):
     pr = _below()
     n = pr.mesh.n_dofs
-    Nd, Dd = oracle_mod.OracleFMM(pr).dense(0)
+    Nd, Dd = oracle_mod.OracleFMM(pr).dense()
     ou = oracle_mod.OracleFMM(_union(pr))
-    Nu, Du = ou.dense(0)
+    Nu, Du = ou.dense()
     Nf = Nu[:n, :n] + Nu[:n, n:]
     Df = Du[:n, :n] + Du[:n, n:]
     assert np.max(np.abs(Nd - Nf)) <= 1e-13 * np.max(np.abs(Nf))
@@ -69,7 +69,7 @@ def test_fma_double_body_vs_dense(oracle_mod):
         pr = _below(8)
         pr.p = p
         o = oracle_mod.OracleFMM(pr).setup()
-        Nd, Dd = o.dense(0)
+        Nd, Dd = o.dense()
         rng = np.random.default_rng(5)
         a, b = rng.uniform(-1, 1, pr.mesh.n_dofs), rng.uniform(-1, 1, pr.mesh.n_dofs)
         N1, D1 = o.matvec(a, b)

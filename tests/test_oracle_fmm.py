@@ -13,7 +13,7 @@ def c1(oracle_mod):
     pr = M.config_sphere(16, 4)
     o = oracle_mod.OracleFMM(pr)
     o.generate_octree_blocking()
-    Nm, Dm = o.dense(0)
+    Nm, Dm = o.dense()
     return pr, Nm, Dm
 
 
