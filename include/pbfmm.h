@@ -273,19 +273,24 @@ int pbfmm_set_mixed_scheme(pbfmm_handle *h, int32_t scheme);
  * chooses its own SpMV). */
 int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel);
 
-/* Overrides the timed side-fusion mode of the operator's launches for one
- * precision (0 none, 1 SpMV beside an M2M level and M2P beside an L2L level,
- * 2 SpMV beside an M2M level and M2P beside M2L, 3 both beside M2L; -1 times
- * them again on the next application).  For tests and tuning.  No
- * counterpart in the reference. */
+/* Overrides the timed launch plan of the operator for one precision
+ * (0 none, 1 SpMV beside an M2M level and M2P beside an L2L level,
+ * 2 SpMV beside an M2M level and M2P beside M2L, 3 both beside M2L; 4 and 5:
+ * plans 0 and 1 with M2L storing each pair's result into a slot of its own,
+ * summed per target by a second kernel, instead of adding it to the target
+ * with atomics; 8 and 12: plans 0 and 4 with the SpMV on a second stream of
+ * the library, beside the whole FMM; -1 times them all again on the next
+ * application).  For tests and tuning.  No counterpart in the reference. */
 int pbfmm_set_side_fusion(pbfmm_handle *h, int32_t precision, int32_t mode);
 
 /* The choices this handle's timings made so far (-1 = not timed yet):
  * [0] [1] near-field SpMV kernel of the operator's single values, FP64 / FP32
  *         (as pbfmm_set_spmv_kernel)
- * [2] [3] side fusion of the operator's launches, FP64 / FP32: 0 none,
+ * [2] [3] launch plan of the operator, FP64 / FP32: 0 none,
  *         1 SpMV beside an M2M level and M2P beside an L2L level, 2 SpMV
- *         beside an M2M level and M2P beside M2L, 3 both beside M2L
+ *         beside an M2M level and M2P beside M2L, 3 both beside M2L, 4 and 5
+ *         plans 0 and 1 with M2L into per-pair slots summed per target, 8
+ *         and 12 plans 0 and 4 with the SpMV on a second stream
  * [4]     mixed-precision scheme: 1 relaxed, 2 iterative refinement
  * [5]     timed FP64 / FP32 operator cost ratio x 1000
  * No counterpart in the reference. */

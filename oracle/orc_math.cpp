@@ -277,38 +277,31 @@ void tensor2(const std::vector<double> &x1, const std::vector<double> &w1,
     }
 }
 
+// QSplit<2>(QDuffy(n, 1), s) (bem_problem.cc:213-229), restated from
+// deal.II's construction (Duffy 1982): for each face f of the unit square in
+// deal.II order (x=0, x=1, y=0, y=1) the triangle (s, v0, v1) of its two
+// vertices, skipped when it has no area (s on the face); on it the Duffy
+// rule of the n x n Gauss product (radial q0 fastest): reference point
+// (q0 (1 - q1), q0 q1), weight q0 w0 w1, mapped affinely by
+// x = s + (v0 - s) xk + (v1 - s) yk with weight |det| times it.
 void duffy_split(int n, double sx, double sy, std::vector<double> &pts, std::vector<double> &wts) {
   std::vector<double> gx, gw;
   gauss01(n, gx, gw);
-  // boundary of the square split at the corners and at S when S lies on it
-  std::vector<std::pair<double, double>> ring = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
-  std::vector<std::pair<double, double>> b;
-  for (int k = 0; k < 4; ++k) {
-    auto P = ring[k], Q = ring[(k + 1) % 4];
-    b.push_back(P);
-    // S strictly inside segment PQ?
-    double cr = (Q.first - P.first) * (sy - P.second) - (Q.second - P.second) * (sx - P.first);
-    double t = ((sx - P.first) * (Q.first - P.first) + (sy - P.second) * (Q.second - P.second));
-    double L2 = (Q.first - P.first) * (Q.first - P.first) + (Q.second - P.second) * (Q.second - P.second);
-    if (std::fabs(cr) < 1e-14 && t > 1e-14 && t < L2 - 1e-14) b.push_back({sx, sy});
-  }
+  static const double face[4][2][2] = {{{0, 0}, {0, 1}}, {{1, 0}, {1, 1}}, {{0, 0}, {1, 0}}, {{0, 1}, {1, 1}}};
   pts.clear();
   wts.clear();
-  for (size_t k = 0; k < b.size(); ++k) {
-    auto A = b[k], B = b[(k + 1) % b.size()];
-    auto on_s = [&](std::pair<double, double> P) {
-      return std::fabs(P.first - sx) < 1e-14 && std::fabs(P.second - sy) < 1e-14;
-    };
-    if (on_s(A) || on_s(B)) continue;
-    double ax = A.first - sx, ay = A.second - sy, bx = B.first - A.first, by = B.second - A.second;
-    double det = std::fabs(ax * by - ay * bx);
-    if (det < 1e-14) continue;
+  for (const auto &f : face) {
+    const double a0 = f[0][0] - sx, a1 = f[0][1] - sy, b0 = f[1][0] - sx, b1 = f[1][1] - sy;
+    const double J = std::fabs(a0 * b1 - b0 * a1);
+    if (J < 1e-12) continue;
     for (int j = 0; j < n; ++j)
       for (int i = 0; i < n; ++i) {
-        double u = gx[i], v = gx[j];
-        pts.push_back(sx + u * ax + u * v * bx);
-        pts.push_back(sy + u * ay + u * v * by);
-        wts.push_back(gw[i] * gw[j] * u * det);
+        const double q0 = gx[i], q1 = gx[j];
+        const double xk = q0 * (1.0 - q1), yk = q0 * q1;
+        const double wd = q0 * (gw[i] * gw[j]);
+        pts.push_back(sx + (a0 * xk + b0 * yk));
+        pts.push_back(sy + (a1 * xk + b1 * yk));
+        wts.push_back(J * wd);
       }
   }
 }

@@ -32,6 +32,7 @@ struct ChunkDev {
   DevBuf op, beg, cnt, op1, beg1, cnt1, tgt, src;
 };
 
+
 // Per-precision copies of the far-field operators and expansion storage.
 template <typename T>
 struct FarDev {
@@ -43,6 +44,7 @@ struct FarDev {
   // the operator's single-value copies: N on Neumann / -D on Dirichlet
   // columns, U1 on Neumann / -U2 on Dirichlet DoFs ([column][kp])
   DevBuf opvals, opunit;
+  DevBuf slots;  // slot M2L: [pair][2 Kt], on first use
 };
 
 // Mixed precision: the relaxed scheme spends its first steps (about a
@@ -157,8 +159,26 @@ struct Engine {
   template <typename T>
   void matvec_fused(const double *x, double *out, double *dst, int side);
   // side fusion mode per precision ([0] FP64, [1] FP32): -1 until timed on
-  // first use, then the fastest of the kSideModes modes
-  static constexpr int kSideModes = 4;
+  // first use, then the fastest of the plans kPlans
+  // Plans are bit sets: bits 0-1 the side placement above (0..3); bit 2
+  // (plans 4, 5 with placements 0, 1) M2L into per-pair slots and a
+  // per-target sum (k_translate<., ., 3, 1> + k_m2l_reduce) instead of atomic
+  // adds into the targets; bit 3 (plans 8, 12 with placement 0) the SpMV on a
+  // second stream beside the whole FMM, joined before L2P.
+  static constexpr int kPlans[] = {0, 1, 2, 3, 4, 5, 8, 12};
+  static bool valid_plan(int m) {
+    for (int v : kPlans)
+      if (v == m) return true;
+    return false;
+  }
+  hipStream_t side_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  DevBuf d_red_tgt, d_red_beg, d_red_pair;  // M2L pairs per target (ascending)
+  int n_red_tgt = 0;
+  template <typename T>
+  T *m2l_slots();
+  template <typename T>
+  void launch_m2l_slots(const T *ops, const T *IN, T *LE);
   int side_fusion[2] = {-1, -1};
   int use_side_fusion(bool f32);
   // near-field SpMV kernel per precision ([0] FP64, [1] FP32) and storage
@@ -169,7 +189,8 @@ struct Engine {
   template <typename T, int NK>
   void tune_spmv();
   template <typename T, int NK>
-  void launch_spmv(int variant, const double *phi, const double *dphi, double *outN, double *outD, int R = 1);
+  void launch_spmv(int variant, const double *phi, const double *dphi, double *outN, double *outD, int R = 1,
+                   hipStream_t st = nullptr);
   template <typename T, int NK>
   void generate(const double *phi, const double *dphi);
   template <typename T, int NK>

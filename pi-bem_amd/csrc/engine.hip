@@ -706,7 +706,9 @@ struct TrLayout {
 // kernels' expansions, 32 pairs x 2 columns; NK = 1: one combined expansion,
 // 64 pairs.
 // KIND (0 M2M, 1 M2L, 2 L2L) only names the instantiation, so that traces and
-// profiles tell the three passes apart.
+// profiles tell the three passes apart; KIND 3 is M2L that stores each pair's
+// result into its own slot instead of adding it to the target (summed per
+// target by k_m2l_reduce).
 template <typename T>
 struct TrArgs {
   const int *chunk_op, *chunk_beg, *chunk_cnt, *tgt, *src;
@@ -816,13 +818,62 @@ __device__ __forceinline__ void translate_chunk(int c, T *__restrict__ sB, int *
       for (int j = 0; j < 4; ++j) sC[(16 * g + (l & 15)) * CS + 16 * w + M::row(l, j)] = acc[g][j];
   }
   __syncthreads();
-  // coalesced accumulation: consecutive lanes add consecutive rows of one
-  // target (the Im rows of m = 0, zero for real densities, included: skipping
-  // them scatters the lanes and measured slower)
-  for (int i = t; i < NK * cnt * KT2; i += NT) {
-    const int cc = i / KT2, row = i - cc * KT2;
-    atomicAdd(OUT + (int64_t)sT[cc / NK] * K2 + (cc % NK) * KP + row, sC[cc * CS + row]);
+  if constexpr (KIND == 3) {
+    // M2L into per-pair slots: pair beg + cc at OUT + (beg + cc) * KT2, the
+    // chunk's results one contiguous, coalesced run of plain stores
+    static_assert(NK == 1, "slot M2L is the single-expansion operator's");
+    T *o = OUT + (int64_t)beg * KT2;
+    for (int i = t; i < cnt * KT2; i += NT) {
+      const int cc = i / KT2, row = i - cc * KT2;
+      o[i] = sC[cc * CS + row];
+    }
+  } else {
+    // coalesced accumulation: consecutive lanes add consecutive rows of one
+    // target (the Im rows of m = 0, zero for real densities, included:
+    // skipping them scatters the lanes and measured slower)
+    for (int i = t; i < NK * cnt * KT2; i += NT) {
+      const int cc = i / KT2, row = i - cc * KT2;
+      atomicAdd(OUT + (int64_t)sT[cc / NK] * K2 + (cc % NK) * KP + row, sC[cc * CS + row]);
+    }
   }
+}
+
+// The slot M2L's sums (KIND 3): one wavefront per target adds the slots of
+// its pairs (ascending pair index, a fixed order) and stores the target's
+// local expansion, which nothing else has written since it was cleared.
+// Lanes take two consecutive coefficients each (2 Kt <= 132 at p <= 10).
+template <typename T>
+__global__ void __launch_bounds__(256) k_m2l_reduce(int ntgt, const int *__restrict__ tgt,
+                                                    const int *__restrict__ beg, const int *__restrict__ pair,
+                                                    int kt2, int kp, const T *__restrict__ slots, T *__restrict__ LE) {
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+  if (w >= ntgt) return;
+  const int k0 = beg[w], k1 = beg[w + 1], row = 2 * l;
+  if (row >= kt2) return;
+  T a0 = 0, a1 = 0;
+  int k = k0;
+  for (; k + 4 <= k1; k += 4) {
+    T v[4][2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const T *sq = slots + (int64_t)pair[k + q] * kt2 + row;
+      v[q][0] = sq[0];
+      v[q][1] = sq[1];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a0 += v[q][0];
+      a1 += v[q][1];
+    }
+  }
+  for (; k < k1; ++k) {
+    const T *sq = slots + (int64_t)pair[k] * kt2 + row;
+    a0 += sq[0];
+    a1 += sq[1];
+  }
+  T *o = LE + (int64_t)tgt[w] * kp + row;
+  o[0] = a0;
+  o[1] = a1;
 }
 
 // Chunks of one class are consecutive in a chunk list and share their
@@ -1486,6 +1537,9 @@ Engine::~Engine() {
   if (h_ready) (void)hipEventDestroy(h_ready);
   if (h_pinned) (void)hipHostFree(h_pinned);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
+  if (side_stream) (void)hipStreamDestroy(side_stream);
 }
 
 // Phase events on the handle's stream (no synchronisation between kernels):
@@ -1824,16 +1878,18 @@ static NearArgs<T> near_args(Engine &e, const double *phi, const double *dphi, d
 }
 
 template <typename T, int NK>
-void Engine::launch_spmv(int variant, const double *phi, const double *dphi, double *outN, double *outD, int R) {
+void Engine::launch_spmv(int variant, const double *phi, const double *dphi, double *outN, double *outD, int R,
+                         hipStream_t st) {
   NearArgs<T> a = near_args<T, NK>(*this, phi, dphi, outN, outD);
   a.stride = n;  // R right-hand sides, n apart (blockIdx.y)
+  if (!st) st = stream;
   if (variant == 0) {
     const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
-    hipLaunchKernelGGL((k_spmv_lds<T, NK>), dim3(per_xcd * kXcd, R), dim3(256), 0, stream, near.nleaf, per_xcd, a);
+    hipLaunchKernelGGL((k_spmv_lds<T, NK>), dim3(per_xcd * kXcd, R), dim3(256), 0, st, near.nleaf, per_xcd, a);
   } else if (variant == 1) {
-    hipLaunchKernelGGL((k_spmv_wave<T, NK, 2>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, stream, near.nleaf, a);
+    hipLaunchKernelGGL((k_spmv_wave<T, NK, 2>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, st, near.nleaf, a);
   } else {
-    hipLaunchKernelGGL((k_spmv_wave<T, NK, 4>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, stream, near.nleaf, a);
+    hipLaunchKernelGGL((k_spmv_wave<T, NK, 4>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, st, near.nleaf, a);
   }
 }
 
@@ -1912,8 +1968,11 @@ void Engine::products(const double *phi, const double *dphi, double *outN, doubl
   hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)std::min<int64_t>(nblk(nle, 256), 4096)), dim3(256), 0, stream,
                      F.le.template as<T>(), nle, T(0));
   mark(5);
-  launch_translate<T, 1, NK>(*this, stream, m2l_cl, 0, nchunks<NK>(far.m2l), F.m2l.template as<T>(),
-                             F.me.template as<T>(), F.le.template as<T>());
+  if (NK == 1 && side_fusion[sizeof(T) == 4 ? 1 : 0] >= 0 && (side_fusion[sizeof(T) == 4 ? 1 : 0] & 4))
+    launch_m2l_slots<T>(F.m2l.template as<T>(), F.me.template as<T>(), F.le.template as<T>());
+  else
+    launch_translate<T, 1, NK>(*this, stream, m2l_cl, 0, nchunks<NK>(far.m2l), F.m2l.template as<T>(),
+                               F.me.template as<T>(), F.le.template as<T>());
   mark(6);
   const int64_t nitems = (int64_t)far.m2p_item_node.size();
   if (nitems > 0)
@@ -2008,6 +2067,46 @@ void Engine::consumer_levels() {
   for (int b : far.m2p_src) me_lo = std::min(me_lo, (int)tree.level[b]);
 }
 
+// The slot M2L's buffers on first use: per target the pair indices of the
+// M2L chunk list (ascending), and [pair][2 Kt] slots of the precision.
+template <typename T>
+T *Engine::m2l_slots() {
+  FarDev<T> &F = far_of<T>(*this);
+  const int64_t npair = (int64_t)far.m2l.tgt.size();
+  if (!d_red_beg.p) {
+    std::vector<int32_t> cnt(std::max(tree.nb, 1), 0);
+    for (int32_t t : far.m2l.tgt) ++cnt[t];
+    std::vector<int32_t> tg, beg{0};
+    std::vector<int32_t> pos(cnt.size(), -1);
+    for (int b = 0; b < (int)cnt.size(); ++b)
+      if (cnt[b] > 0) {
+        pos[b] = (int32_t)tg.size();
+        tg.push_back(b);
+        beg.push_back(beg.back() + cnt[b]);
+      }
+    std::vector<int32_t> pair(npair), fill(beg.begin(), beg.end() - 1);
+    for (int64_t i = 0; i < npair; ++i) pair[fill[pos[far.m2l.tgt[i]]]++] = (int32_t)i;
+    up(d_red_tgt, tg);
+    up(d_red_beg, beg);
+    up(d_red_pair, pair);
+    n_red_tgt = (int)tg.size();
+  }
+  const size_t bytes = (size_t)std::max<int64_t>(npair, 1) * 2 * kt * sizeof(T);
+  if (F.slots.bytes < bytes) alloc(F.slots, bytes);
+  return F.slots.template as<T>();
+}
+
+template <typename T>
+void Engine::launch_m2l_slots(const T *ops, const T *IN, T *LE) {
+  T *slots = m2l_slots<T>();
+  launch_translate<T, 3, 1>(*this, stream, m2l_cl, 0, nchunks<1>(far.m2l), ops, IN, slots);
+  if (n_red_tgt > 0)
+    hipLaunchKernelGGL(k_m2l_reduce<T>, dim3((unsigned)((n_red_tgt + 3) / 4)), dim3(256), 0, stream, n_red_tgt,
+                       d_red_tgt.as<int>(), d_red_beg.as<int>(), d_red_pair.as<int>(), 2 * kt, kp,
+                       (const T *)slots, LE);
+  HIPCHK(hipGetLastError());
+}
+
 // The single-value pipeline in fewer launches, for vmult: P2M, whose extra
 // workgroups clear the other MEs, LE and the M2P accumulator; M2M levels
 // L..me_lo + 1; M2L; L2L
@@ -2027,6 +2126,9 @@ void Engine::consumer_levels() {
 template <typename T>
 void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   FarDev<T> &F = far_of<T>(*this);
+  const bool async = (side & 8) != 0;  // SpMV on the second stream
+  const bool slot = (side & 4) != 0;   // M2L into per-pair slots, summed per target
+  side &= 3;
   const int nleaf = (int)far.p2m_leaf.size();
   const int64_t ne = (int64_t)tree.nb * kp;
   T *me = F.me.template as<T>(), *le = F.le.template as<T>();
@@ -2068,6 +2170,21 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   sd.ME = me;
   sd.m2p_out = acc;
   bool near_done = nl == 0, m2p_done = ni == 0;
+  bool join = false;
+  if (async && !near_done) {
+    // the SpMV reads only x, ready when this application starts, and writes
+    // out, which only the final L2P reads: it runs beside the whole FMM
+    if (!side_stream) {
+      HIPCHK(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    }
+    HIPCHK(hipEventRecord(ev_fork, stream));
+    HIPCHK(hipStreamWaitEvent(side_stream, ev_fork, 0));
+    launch_spmv<T, 1>(var, x, nullptr, out, nullptr, 1, side_stream);
+    HIPCHK(hipEventRecord(ev_join, side_stream));
+    near_done = join = true;
+  }
   if (side == 0 && !near_done) {
     launch_spmv<T, 1>(var, x, nullptr, out, nullptr);
     near_done = true;
@@ -2094,7 +2211,11 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   const int c1 = nchunks<1>(far.m2l);
   const T *mops = F.m2l.template as<T>();
   const bool m2p_here = !m2p_done && side >= 2, near_here = !near_done;
-  if (near_here && var == 0)
+  if (slot) {
+    if (near_here) launch_spmv<T, 1>(var, x, nullptr, out, nullptr);
+    near_done = true;
+    launch_m2l_slots<T>(mops, me, le);
+  } else if (near_here && var == 0)
     m2p_here ? launch_translate_side<T, 1, 0, true>(*this, m2l_cl, 0, c1, mops, me, le, sd)
              : launch_translate_side<T, 1, 0, false>(*this, m2l_cl, 0, c1, mops, me, le, sd);
   else if (near_here && var == 1)
@@ -2127,6 +2248,7 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
     PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_m2p<T, 1, PP>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, stream, ni, d_m2p_node.as<int>(),
                        d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt,
                        kp, me, acc, nullptr, (int64_t)0, (int64_t)0));
+  if (join) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
   const int nrows = (int)near.rows.size();
   if (nrows > 0)
     PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_l2p_post<T, PP>), dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_rows.as<int>(),
@@ -2171,14 +2293,15 @@ int Engine::use_side_fusion(bool f32) {
   hipEvent_t t0, t1;
   HIPCHK(hipEventCreate(&t0));
   HIPCHK(hipEventCreate(&t1));
-  float ms[kSideModes] = {};
-  for (int s = 0; s < kSideModes; ++s)
+  constexpr int np = (int)(sizeof(kPlans) / sizeof(kPlans[0]));
+  float ms[np] = {};
+  for (int s = 0; s < np; ++s)
     for (int rep = 0; rep < 11; ++rep) {
       if (rep == 1) HIPCHK(hipEventRecord(t0, stream));
       if (f32)
-        matvec_fused<float>(u, d_outN.as<double>(), v, s);
+        matvec_fused<float>(u, d_outN.as<double>(), v, kPlans[s]);
       else
-        matvec_fused<double>(u, d_outN.as<double>(), v, s);
+        matvec_fused<double>(u, d_outN.as<double>(), v, kPlans[s]);
       if (rep == 10) {
         HIPCHK(hipEventRecord(t1, stream));
         HIPCHK(hipEventSynchronize(t1));
@@ -2188,9 +2311,9 @@ int Engine::use_side_fusion(bool f32) {
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   int best = 0;
-  for (int s = 1; s < kSideModes; ++s)
+  for (int s = 1; s < np; ++s)
     if (ms[s] < ms[best]) best = s;
-  mode = ms[best] < 0.98f * ms[0] ? best : 0;
+  mode = ms[best] < 0.98f * ms[0] ? kPlans[best] : 0;
   return mode;
 }
 

@@ -75,27 +75,30 @@ Quadrature2D tensor_rule(const std::vector<double> &x1, const std::vector<double
 
 // QSplit<2>(QDuffy(n, 1), s), the dense assembly's singular rule
 // (bem_problem.cc:213-229; deal.II's QSplit and QDuffy, written here from
-// Duffy 1982).  The unit square is cut into the triangles (s, a, b) over its
-// four sides a-b, in deal.II's face order x=0, x=1, y=0, y=1; a side through s
-// spans no area and is skipped.  Each triangle takes the Duffy collapse of an
-// n x n Gauss rule on [0,1]^2, radial coordinate u (fastest) and angular v:
-//   (u, v) -> s + u (a - s) + u v (b - a),  weight w_u w_v u |det(a-s, b-s)|,
-// whose factor u cancels the 1/r of the Laplace kernels at s.
+// Duffy 1982).  The unit square is cut into the triangles (s, v0, v1) over
+// its four sides, in deal.II's face order x=0, x=1, y=0, y=1; a side through
+// s spans no area and is skipped.  Each triangle takes the Duffy collapse of
+// an n x n Gauss rule on [0,1]^2, radial coordinate q0 (fastest) and angular
+// q1: reference point (q0 (1 - q1), q0 q1) with weight q0 w0 w1, whose factor
+// q0 cancels the 1/r of the Laplace kernels at s, mapped affinely with the
+// triangle's |det|.  Evaluated in deal.II's order of operations, so that the
+// points and weights match its rule to the last bit.
 Quadrature2D duffy_split_rule(int n, double sx, double sy) {
   std::vector<double> g, gw;
   gauss_legendre01(n, g, gw);
   static const double side[4][4] = {{0, 0, 0, 1}, {1, 0, 1, 1}, {0, 0, 1, 0}, {0, 1, 1, 1}};
   Quadrature2D r;
   for (const auto &sd : side) {
-    const double ax = sd[0] - sx, ay = sd[1] - sy, bx = sd[2] - sx, by = sd[3] - sy;
-    const double area2 = std::fabs(ax * by - ay * bx);
-    if (area2 < 1e-12) continue;
+    const double a0 = sd[0] - sx, a1 = sd[1] - sy, b0 = sd[2] - sx, b1 = sd[3] - sy;
+    const double det = std::fabs(a0 * b1 - b0 * a1);
+    if (det < 1e-12) continue;
     for (int j = 0; j < n; ++j)
       for (int i = 0; i < n; ++i) {
-        const double u = g[i], v = g[j];
-        r.pts.push_back(sx + u * ax + u * v * (bx - ax));
-        r.pts.push_back(sy + u * ay + u * v * (by - ay));
-        r.w.push_back(gw[i] * gw[j] * u * area2);
+        const double q0 = g[i], q1 = g[j];
+        const double xk = q0 * (1.0 - q1), yk = q0 * q1;
+        r.pts.push_back(sx + (a0 * xk + b0 * yk));
+        r.pts.push_back(sy + (a1 * xk + b1 * yk));
+        r.w.push_back(det * (q0 * (gw[i] * gw[j])));
       }
   }
   return r;

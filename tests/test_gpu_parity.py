@@ -18,6 +18,9 @@ from pibem_amd import mesh as M
 
 pytestmark = pytest.mark.gpu
 
+# the operator's launch plans (pbfmm_set_side_fusion)
+PLANS = (0, 1, 2, 3, 4, 5, 8, 12)
+
 
 def rel(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
@@ -262,9 +265,12 @@ def test_config3_properties():
       alpha.(x.neu) assembled from the two-output FMM (FP64, 1e-12);
     - the FP32 operator is within 2e-6 of the FP64 one;
     - both solves reach true relative residual 1e-8, the analytic phi to
-      1e-3 on the Neumann DoFs (measured 6e-5) and the analytic dphi/dn to
-      1e-2 on the Dirichlet DoFs (with the Duffy split; the Telles rule had
-      left 0.133 there)."""
+      1e-4 on the Neumann DoFs (measured 7.0e-6; 6.1e-5 with the Telles rule)
+      and the analytic dphi/dn to 2e-2 on the Dirichlet DoFs (measured 1.42e-2;
+      0.133 with the Telles rule).  That is the accuracy of the reference's
+      dense solve with the same rules (QGauss(4), QSplit(QDuffy(5, 1))):
+      1.36e-2 with dense matrices at 6 x 8 x 8 cells, limited by the 5-point
+      singular rule on curved Q2 cells (9.2e-3 with sing_quad_order 8)."""
     from pibem_amd import hip
     pr = M.config_spheroid(128, 8)
     n = pr.mesh.n_dofs
@@ -303,7 +309,7 @@ def test_config3_properties():
         err_dn = rel(s[~neu], dn_ex[~neu])
         print(f"  solve precision={prec} iters={it} relres={rr:.3e} err phi {err:.2e} dphi/dn {err_dn:.2e}",
               flush=True)
-        assert rr <= 1e-8 and err <= 1e-3 and err_dn <= 1e-2
+        assert rr <= 1e-8 and err <= 1e-4 and err_dn <= 2e-2
 
 
 @pytest.mark.parametrize("scheme", [1, 2], ids=["relaxed", "refinement"])
@@ -382,9 +388,10 @@ def test_fused_operator_pipeline(name):
     """The operator in fewer launches (SpMV beside an M2M level or M2L, M2P
     beside an L2L level or M2L, L2P applying vmult's epilogue, levels without
     readers skipped) against the phase-timed sequence, which runs every phase
-    in its own launch, for each of the three SpMV kernels and each side-fusion
-    mode: FP64 within 1e-14, FP32 within 1e-6 (atomics sum in another
-    order)."""
+    in its own launch, for each of the three SpMV kernels and each launch
+    plan: M2L adding into its targets (plans 0-3) or into per-pair slots
+    summed per target (plans 4, 5), the SpMV on a second stream (plans 8, 12):
+    FP64 within 1e-14, FP32 within 1e-6 (sums in another order)."""
     from pibem_amd import hip
     pr = {"sphere_p4": lambda: M.config_sphere(16, 4), "sphere_p6": lambda: M.config_sphere(24, 6),
           "spheroid_p8": lambda: M.config_spheroid(24, 8)}[name]()
@@ -404,7 +411,7 @@ def test_fused_operator_pipeline(name):
             y0 = dst.numpy()
             f.set_phase_timing(False)
             ref = y0 if ref is None else ref
-            for mode in (0, 1, 2, 3):
+            for mode in PLANS:
                 f.set_side_fusion(prec, mode)
                 bp.vmult_dev(src.value, dst.value, prec)
                 hip.synchronize()
@@ -417,7 +424,7 @@ def test_fused_operator_pipeline(name):
         bp.vmult_dev(src.value, dst.value, prec)
         hip.synchronize()
         assert rel(dst.numpy(), ref) <= bar
-        assert f.tuning_state()["side_fusion_fp32" if prec == P.PBFMM_MIXED else "side_fusion_fp64"] in (0, 1, 2, 3)
+        assert f.tuning_state()["side_fusion_fp32" if prec == P.PBFMM_MIXED else "side_fusion_fp64"] in PLANS
     f.close()
 
 
@@ -481,7 +488,7 @@ def test_fused_operator_modes_on_batches_and_images(name):
         hip.synchronize()
         y0 = dst.numpy()
         f.set_phase_timing(False)
-        for mode in (0, 1, 2, 3):
+        for mode in PLANS:
             f.set_side_fusion(prec, mode)
             bp.vmult_dev(src.value, dst.value, prec)
             hip.synchronize()
