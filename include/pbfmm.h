@@ -336,6 +336,21 @@ int pbfmm_center_of_pressure(pbfmm_handle *h, const uint8_t *cell_mask, const do
 int pbfmm_pressure_force_and_moment(pbfmm_handle *h, const uint8_t *cell_mask, const double *pressure,
                                     const double *cog, double *force, double *moment);
 int pbfmm_volume_integral(pbfmm_handle *h, const uint8_t *cell_mask, double *volume);
+/* pbfmm_pressure_force_and_moment without any device allocation.  No
+ * counterpart in the reference: for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops).  Device pointers: d_cell_mask
+ * (n_cells bytes, NULL selects every real cell), d_pressure (n_dofs) and
+ * d_out (6 doubles: force, then moment about cog); cog is a host array of 3.
+ * The per-workgroup partial sums live in a fixed device array of the library
+ * (6.8 MB on every GPU it runs on, held until the process ends), which limits
+ * the mesh to 16,777,216 cells (PBFMM_ERR_ARG beyond).  The handle's
+ * post-solve data must exist (any post-solve call above sets it up;
+ * PBFMM_ERR_STATE before).  Asynchronous on the handle's stream: d_out holds
+ * the result once that stream has reached this call.  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU (other streams, other
+ * host threads) race; calls in order on one stream do not. */
+int pbfmm_pressure_force_and_moment_noalloc(pbfmm_handle *h, const uint8_t *d_cell_mask, const double *d_pressure,
+                                            const double *cog, double *d_out);
 /* BEMProblem::free_surface_elevation (bem_problem.cc:2311-2370): for each
  * (cell, line) pair of cell_lines (line in deal.II numbering 0..3: x = 0,
  * x = 1, y = 0, y = 1 of the unit cell; the reference visits waterline

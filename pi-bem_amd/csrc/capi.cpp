@@ -974,6 +974,24 @@ int pbfmm_pressure_force_and_moment(pbfmm_handle *h, const uint8_t *cell_mask, c
   });
 }
 
+int pbfmm_pressure_force_and_moment_noalloc(pbfmm_handle *h, const uint8_t *d_cell_mask, const double *d_pressure,
+                                            const double *cog, double *d_out) {
+  return guarded(h, [&]() -> int {
+    if (int rc = not_batch(h, "pressure_force_and_moment_noalloc")) return rc;
+    if (int rc = need(h, 1, "pressure_force_and_moment_noalloc")) return rc;
+    Engine &e = h->eng;
+    if (!e.ps.ready)
+      return fail(h, PBFMM_ERR_STATE,
+                  "pressure_force_and_moment_noalloc: post-solve data not set up (any post-solve call sets it up)");
+    if (!d_pressure || !cog || !d_out) return fail(h, PBFMM_ERR_ARG, "pressure_force_and_moment_noalloc: null argument");
+    if (e.ps.ncells > ps_scratch_max_cells())
+      return fail(h, PBFMM_ERR_ARG, "pressure_force_and_moment_noalloc: more than " +
+                                        std::to_string(ps_scratch_max_cells()) + " cells");
+    e.ps_force_moment_noalloc(d_cell_mask, d_pressure, cog, d_out);
+    return PBFMM_OK;
+  });
+}
+
 int pbfmm_volume_integral(pbfmm_handle *h, const uint8_t *cell_mask, double *volume) {
   return guarded(h, [&]() -> int {
     if (int rc = ps_ready(h, "volume_integral")) return rc;

@@ -274,8 +274,16 @@ struct Engine {
   // sums over masked real cells: [0..2] force, [3..5] moment about cog,
   // [6..8] int x p, [9] int p, [10..12] -int x_d n_d (volume)
   void ps_integrals(const uint8_t *mask, const double *p, const double cog[3], double out[13]);
+  // force and moment of ps_integrals into out[0..5] (device), partial sums in
+  // the device's fixed scratch array: no allocation (mask NULL: all cells)
+  void ps_force_moment_noalloc(const uint8_t *mask, const double *p, const double cog[3], double *out);
   void ps_elevation(const int32_t *faces, int64_t nfaces, const double *p, double rho_g, double *out);
 };
+
+// Fixed device scratch of the allocation-free body integrals (postsolve.hip):
+// its address on device dev is looked up once, with dev current.
+void ps_scratch_init(int dev);
+int64_t ps_scratch_max_cells();
 
 // Host plans of independent problems merged into one block-diagonal
 // operator (forest.cpp); dof_off receives the DoF range of each part.

@@ -1529,6 +1529,7 @@ void Engine::init_device(int dev, void *st) {
     own_stream = true;
   }
   upload_legendre_tables();
+  ps_scratch_init(dev);
 }
 
 Engine::~Engine() {

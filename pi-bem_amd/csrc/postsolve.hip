@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -275,6 +276,9 @@ __global__ void k_ps_finish(int nchunk, int s0, const double *__restrict__ part,
   const double r = ps_block_sum(acc, sh);
   if (threadIdx.x == 0) cg[s] = r;
 }
+__global__ void k_fill_zero6(double *out) {
+  if (threadIdx.x < 6) out[threadIdx.x] = 0.0;
+}
 __global__ void k_ps_alpha(double *cg) {
   const int d = threadIdx.x;
   if (d < 3) cg[kAL + d] = cg[kPQ + d] > 0.0 ? cg[kRZ + d] / cg[kPQ + d] : 0.0;
@@ -344,7 +348,8 @@ __global__ void k_ps_hydrodynamic(int64_t n, const double *__restrict__ grad, co
 
 // Body integrals over the masked real cells (center_of_pressure 2169-2219,
 // pressure_force_and_moment 2221-2272, volume_integral 2274-2309):
-// per-block partial sums of the 13 quantities, finished by k_ps_finish.
+// per-block partial sums of the 13 quantities, finished by k_ps_finish
+// (mask NULL: every cell).
 template <int DPC>
 __global__ void __launch_bounds__(kPsThreads)
     k_ps_integrals(int64_t ncells, int nq, const double *__restrict__ tab, const double *__restrict__ geom,
@@ -354,7 +359,7 @@ __global__ void __launch_bounds__(kPsThreads)
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   double acc[13];
   for (int k = 0; k < 13; ++k) acc[k] = 0.0;
-  if (c < ncells && mask[c]) {
+  if (c < ncells && (!mask || mask[c])) {
     const double *G = geom + c * DPC * 3;
     double pj[DPC];
     for (int j = 0; j < DPC; ++j) pj[j] = pres[cdofs[c * DPC + j]];
@@ -630,6 +635,68 @@ void Engine::ps_integrals(const uint8_t *mask, const double *p, const double cog
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(out, res, 13 * 8, hipMemcpyDeviceToHost, stream));
   sync();
+}
+
+// ---------------------------------------------------------------------------
+// The allocation-free body integrals (pbfmm_pressure_force_and_moment_noalloc)
+// keep their per-block partial sums in this fixed device array instead of
+// the handle's post-solve buffers: 13 sums per workgroup of kPsThreads cells,
+// kScratchBlocks workgroups at most (16,777,216 cells, 6.8 MB per GPU, held
+// until the process ends).  Each GPU holds its own copy at its own address,
+// looked up once per device at set-up (ps_scratch_init); host code passes
+// that address, never the array's name, to kernels.
+// ---------------------------------------------------------------------------
+constexpr int kScratchBlocks = 65536;
+constexpr int kScratchSums = 13;
+static __device__ __align__(256) double scratch[(size_t)kScratchSums * kScratchBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+int64_t ps_scratch_max_cells() { return (int64_t)kScratchBlocks * kPsThreads; }
+
+// Called by Engine::init_device with the device current.
+void ps_scratch_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device ordinal out of range for the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[dev]) return;
+  void *ptr = nullptr;
+  HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) throw HipError("hipGetSymbolAddress returned a null address for the scratch array");
+  g_scratch[dev] = static_cast<double *>(ptr);
+}
+
+// force (out[0..2]) and moment (out[3..5]) over the masked real cells, all
+// device pointers, partial sums in the device's scratch array; no allocation.
+void Engine::ps_force_moment_noalloc(const uint8_t *mask, const double *p, const double cog[3], double *out) {
+  if (!ps.ready) throw std::runtime_error("post-solve data not set up");
+  double *part = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    part = (device >= 0 && device < kMaxDevices) ? g_scratch[device] : nullptr;
+  }
+  if (!part) throw std::runtime_error("scratch array of this device not looked up");
+  const unsigned gc = ps_blocks(ps.ncells, kPsThreads);
+  if (gc > (unsigned)kScratchBlocks) throw std::runtime_error("more cells than the scratch array holds");
+  if (gc == 0) {
+    hipLaunchKernelGGL(k_fill_zero6, dim3(1), dim3(64), 0, stream, out);
+  } else {
+    if (mesh.dpc == 4)
+      hipLaunchKernelGGL((k_ps_integrals<4>), dim3(gc), dim3(kPsThreads), 0, stream, ps.ncells, ps.nq,
+                         ps.tab.as<double>(), ps.geom.as<double>(), ps.cdofs.as<int32_t>(), mask, p, cog[0], cog[1],
+                         cog[2], part);
+    else
+      hipLaunchKernelGGL((k_ps_integrals<9>), dim3(gc), dim3(kPsThreads), 0, stream, ps.ncells, ps.nq,
+                         ps.tab.as<double>(), ps.geom.as<double>(), ps.cdofs.as<int32_t>(), mask, p, cog[0], cog[1],
+                         cog[2], part);
+    // sums 0..5 (force, moment) straight into out; every partial read was
+    // written by this launch (stride gc), so nothing needs clearing
+    hipLaunchKernelGGL(k_ps_finish, dim3(6), dim3(kPsThreads), 0, stream, (int)gc, 0, part, out);
+  }
+  HIPCHK(hipGetLastError());
 }
 
 void Engine::ps_elevation(const int32_t *faces, int64_t nfaces, const double *p, double rho_g, double *out) {

@@ -359,6 +359,15 @@ class BEMProblem:
                                                                   _p(cog), _p(f), _p(mo)))
         return f, mo
 
+    def pressure_force_and_moment_noalloc(self, mask_ptr, pressure_ptr, cog, out_ptr):
+        """pbfmm_pressure_force_and_moment_noalloc: device pointers (mask may be
+        None for every real cell; out receives force then moment, 6 doubles),
+        partial sums in the library's fixed device array, no allocation;
+        asynchronous on the handle's stream."""
+        c = _f64(cog)
+        self.fma._chk(self.fma._L.pbfmm_pressure_force_and_moment_noalloc(self._h, mask_ptr, pressure_ptr, _p(c),
+                                                                       out_ptr))
+
     def volume_integral(self, body):
         v = np.zeros(3)
         m = body.cell_mask(self.fma.mesh)

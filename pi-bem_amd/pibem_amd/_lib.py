@@ -79,6 +79,7 @@ SIGNATURES = {
     "pbfmm_center_of_pressure": (C.c_int, [_h, _u8, _d, _d]),
     "pbfmm_pressure_force_and_moment": (C.c_int, [_h, _u8, _d, _d, _d, _d]),
     "pbfmm_volume_integral": (C.c_int, [_h, _u8, _d]),
+    "pbfmm_pressure_force_and_moment_noalloc": (C.c_int, [_h, C.c_void_p, C.c_void_p, _d, C.c_void_p]),
     "pbfmm_free_surface_elevation": (C.c_int, [_h, C.c_double, C.c_double, C.POINTER(C.c_int32), C.c_int64, _d,
                                                _d]),
     "pbfmm_batch_create": (C.c_int, [C.POINTER(_h), C.c_int32, C.POINTER(_h)]),
