@@ -48,15 +48,17 @@ METRIC = "GMRES time to rel-res 1e-8, mixed vs FP64 GPU; FMM matvecs/s; SpMV HBM
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFS = 157.3
 FP64_PEAK_TFS = 78.6
-PHASES = ("spmv", "p2m", "m2m", "l2l", "m2l", "m2p", "l2p")
+PHASES = ("spmv", "p2m", "m2m", "l2l", "m2l", "m2p", "l2p", "m2l_sum")
 # kernel names as they appear in traces, per phase, operator precision and
 # number of expansions per block (2: N and D separately, 1: the operator's
 # combined N.phi - D.dphi)
 KERNEL = {
     "spmv": "k_spmv<{t}, {k}>", "p2m": "k_p2m<{t}, {k}>", "m2m": "k_translate<{t}, {p}, 0, {k}>",
     "m2l": "k_translate<{t}, {p}, 1, {k}>", "l2l": "k_translate<{t}, {p}, 2, {k}>", "m2p": "k_m2p<{t}, {k}>",
-    "l2p": "k_l2p<{t}, {k}>",
+    "l2p": "k_l2p<{t}, {k}>", "m2l_sum": "k_m2l_reduce<{t}>",
 }
+# M2L into per-pair slots (the slot plans; its per-target sum is m2l_sum)
+KERNEL_M2L_SLOT = "k_translate<{t}, {p}, 3, {k}>"
 
 
 def log(msg):
@@ -76,6 +78,9 @@ def work_model(wc, s_v, nk):
           column, nk input vectors once, nk kp values per leaf written.
     m2m, l2l, m2l: 8 Kt^2 flops per (pair, expansion): a complex Kt x Kt map.
     m2p, l2p: ~(8 nk Kt + 8 Kt + 40) flops per evaluation point.
+    m2l_sum (slot plans): every pair's 2 Kt slot values read once and its
+          4-byte pair index; the targets' writes (at most the expansion
+          array) are left out.
     """
     n, p, kp = wc["n_dofs"], wc["trunc_order"], wc["kp"]
     kt = (p + 1) * (p + 2) // 2
@@ -88,14 +93,18 @@ def work_model(wc, s_v, nk):
     for k, pairs in (("m2m", wc["m2m_translations"]), ("l2l", wc["l2l_translations"]), ("m2l", wc["m2l_pairs"])):
         out[k] = dict(bound="mfma", flops=pairs * nk * 8 * kt * kt)
     out["m2p"] = dict(bound="valu", flops=wc["m2p_evals"] * (8 * nk * kt + 8 * kt + 40))
+    out["m2l_sum"] = dict(bound="hbm", bytes=wc["m2l_pairs"] * (2 * kt * s_v + 4))
     out["l2p"] = dict(bound="valu", flops=wc["near_rows"] * (8 * nk * kt + 8 * kt + 40))
     return out
 
 
-def roofline_of(phase, model, ms, prec, p, nk):
+def roofline_of(phase, model, ms, prec, p, nk, slot=False):
     t = "float" if prec == "fp32" else "double"
     m = model[phase]
-    name = KERNEL[phase].format(t=t, p=p, k=nk)
+    name = (KERNEL_M2L_SLOT if (phase == "m2l" and slot) else KERNEL[phase]).format(t=t, p=p, k=nk)
+    if ms <= 0:
+        return dict(kernel=name, bound=m["bound"], achieved=0.0, peak=None, unit=None, frac=0.0,
+                    algorithmic_per_launch=None, launch_ms=0.0)
     if m["bound"] == "hbm":
         ach = m["bytes"] / (ms * 1e-3) / 1e9
         return dict(kernel=name, bound="hbm", achieved=ach, peak=HBM_PEAK_GBS,
@@ -333,7 +342,8 @@ def measure_config(config, steps, warmup):
     op_rate, op_phases = time_operator(f, bp, n, stream)
     mv, phases, rel = time_matvecs(f, n, stream)
     roof = {prec: {ph: roofline_of(ph, work_model(wc, 4 if prec == "fp32" else 8, 1), op_phases[prec][ph], prec,
-                                   pr.p, 1) for ph in PHASES} for prec in op_phases}
+                                   pr.p, 1, slot=op_phases[prec]["m2l_sum"] > 0) for ph in PHASES}
+            for prec in op_phases}
     roof2 = {prec: {ph: roofline_of(ph, work_model(wc, 4 if prec == "fp32" else 8, 2), phases[prec][ph], prec,
                                     pr.p, 2) for ph in PHASES} for prec in phases}
     return dict(pr=pr, f=f, stream=stream, wc=wc, setup=setup, solves=solves, tuning=tuning, operator=op_rate,

@@ -140,6 +140,27 @@ int pbfmm_compute_alpha(pbfmm_handle *h, double *alpha);
 int pbfmm_set_constraints(pbfmm_handle *h, const int64_t *c_ptr, const int64_t *c_idx,
                           const double *c_val, const double *c_inhom, const uint8_t *is_constrained);
 
+/* BEMProblem::compute_constraints (bem_problem.cc:1376-1653), one process,
+ * no hanging nodes: builds the double-node constraint rows from tmp_rhs (the
+ * boundary data, host, n_dofs) and installs them as pbfmm_set_constraints
+ * does.  For each set of coincident DoFs the first Dirichlet DoF (else the
+ * first DoF) is the master i; Neumann partners of a Neumann master get
+ * phi(it) = phi(i), Neumann partners of a Dirichlet master take tmp_rhs(i) as
+ * inhomogeneity, and Dirichlet partners of a Dirichlet master get both normal
+ * derivatives from the L2-projected normals and surface gradients of tmp_rhs
+ * (pbfmm_compute_normals, pbfmm_compute_surface_gradients) when
+ * continuous_gradient is set ("Continuos gradient across edges", default
+ * true, bem_problem.cc:277).  The reference's parallel-normal test never
+ * fires (it divides n_i.n_it by itself, bem_problem.cc:1490), and the
+ * formula then divides by 1 - (n_i.n_it)^2 = 0; here projected normals within
+ * 1e-4 of parallel give dphi_dn(it) = sign(n_i.n_it) dphi_dn(i) instead. */
+int pbfmm_compute_constraints(pbfmm_handle *h, const double *tmp_rhs, int32_t continuous_gradient);
+/* The installed constraint rows, in pbfmm_set_constraints' layout (c_ptr
+ * n_dofs + 1, c_idx / c_val nnz, c_inhom / is_constrained n_dofs): with only
+ * nnz non-NULL, writes the number of entries. */
+int pbfmm_get_constraints(pbfmm_handle *h, int64_t *nnz, int64_t *c_ptr, int64_t *c_idx, double *c_val,
+                          double *c_inhom, uint8_t *is_constrained);
+
 /* BEMFMA<3>::FMA_preconditioner (bem_fma.h:201-202, bem_fma.cc:3078-3411):
  * the near-field matrix with -D on Dirichlet columns and N on Neumann columns
  * (init_preconditioner, bem_fma.cc:919-975), alpha added on the diagonal of
@@ -240,7 +261,9 @@ int pbfmm_get_expansions(pbfmm_handle *h, int32_t which, double *ker1, double *k
 /* Per-phase device times of the last matvec in ms, measured with HIP events
  * on the handle's stream while phase timing is enabled (it adds one host
  * synchronisation per matvec):
- * [0] spmv [1] p2m [2] m2m [3] l2l [4] m2l [5] m2p [6] l2p [7] total. */
+ * [0] spmv [1] p2m [2] m2m [3] l2l [4] m2l [5] m2p [6] l2p [7] total
+ * [8] the per-target sum of the slot M2L (0 when M2L adds into its targets;
+ *     [4] then holds the translations alone): 9 doubles. */
 int pbfmm_set_phase_timing(pbfmm_handle *h, int32_t enable);
 int pbfmm_last_timings(pbfmm_handle *h, double *ms);
 
@@ -256,6 +279,15 @@ int pbfmm_last_timings(pbfmm_handle *h, double *ms);
  *      single values (0 workgroup per leaf, 1 wavefront per leaf with 2 rows
  *      in flight, 2 with 4 rows) */
 int pbfmm_work_counts(pbfmm_handle *h, int64_t *counts);
+
+/* The block-Jacobi preconditioner of pbfmm_fma_preconditioner before
+ * inversion (the matrix FMA_preconditioner assembles, bem_fma.cc:3078-3411,
+ * restricted to its diagonal blocks), for tests: sizes[0] blocks, [1] rows in
+ * all, [2] values in all; with row_beg (blocks + 1), rows and values non-NULL
+ * also block b's rows rows[row_beg[b] .. row_beg[b + 1]) and its R x R
+ * matrix, row-major, in values (blocks one after another).  Builds the
+ * preconditioner if needed. */
+int pbfmm_preconditioner_blocks(pbfmm_handle *h, int64_t *sizes, int32_t *row_beg, int32_t *rows, double *values);
 
 /* Mixed-precision solver scheme (no counterpart in the reference, whose
  * solve is FP64 throughout): 0 (default) chooses by the ratio of the FP64 to

@@ -137,6 +137,66 @@ class PostSolve:
     def compute_surface_gradients(self, g):
         return self._project(self._surf_grad(g))
 
+    # -- double-node constraints -------------------------------------------
+    def compute_constraints(self, tmp_rhs, dirichlet, continuous_gradient=True):
+        """BEMProblem::compute_constraints (bem_problem.cc:1376-1653) with
+        this module's projected normals and surface gradients of tmp_rhs
+        (bem_problem.cc:1385-1388).  For every double-node set, in DoF
+        order, the master is the set's first Dirichlet DoF, else its smallest
+        (bem_problem.cc:1405-1416).  Partners of a Neumann master get the
+        entry (master, 1) (bem_problem.cc:1567-1575); a Neumann partner of a
+        Dirichlet master takes tmp_rhs[master] as inhomogeneity
+        (bem_problem.cc:1558-1562); a Dirichlet pair with continuous
+        gradients sets both normal derivatives from
+        (s_it.n_i + (s_i.n_it)(n_i.n_it)) / (1 - (n_i.n_it)^2) and its
+        mirror (bem_problem.cc:1499-1556), the last partner's value staying
+        on the master.  The reference's parallel-normal test divides the dot
+        product by itself (bem_problem.cc:1490-1491) and never fires; here,
+        as in the product, normals within 1e-4 of parallel (|cos| > 1 - 1e-4,
+        where that formula divides by nearly zero) give the entry
+        (master, sign cos).  Returns (ptr, idx, val, inhom, is_constrained)
+        with component-major projections as in the product."""
+        m = self.mesh
+        n = m.n_dofs
+        nrm = self.compute_normals().reshape(3, n).T
+        sg = self.compute_surface_gradients(np.asarray(tmp_rhs, dtype=np.float64)).reshape(3, n).T
+        dsets = m.double_sets()
+        entries = [[] for _ in range(n)]
+        inhom = np.zeros(n)
+        isc = np.zeros(n, dtype=np.uint8)
+        for i in range(n):
+            doubles = sorted(dsets[i])
+            first = next((d for d in doubles if dirichlet[d] == 1), doubles[0])
+            if i != first:
+                continue
+            for it in doubles:
+                if it == i:
+                    continue
+                if dirichlet[i] == 0:
+                    isc[it] = 1
+                    entries[it].append((i, 1.0))
+                elif dirichlet[it] == 0:
+                    isc[it] = 1
+                    inhom[it] = tmp_rhs[i]
+                else:
+                    nn = float(nrm[i] @ nrm[it])
+                    c = nn / (np.linalg.norm(nrm[i]) * np.linalg.norm(nrm[it]))
+                    if 1.0 - abs(c) < 1e-4:
+                        isc[it] = 1
+                        entries[it].append((i, 1.0 if c > 0 else -1.0))
+                    elif continuous_gradient:
+                        a = float(sg[it] @ nrm[i])   # s_it . n_i
+                        b = float(sg[i] @ nrm[it])   # s_i . n_it
+                        isc[i] = 1
+                        inhom[i] = (a + b * nn) / (1.0 - nn * nn)
+                        isc[it] = 1
+                        inhom[it] = (b + a * nn) / (1.0 - nn * nn)
+        ptr = np.zeros(n + 1, dtype=np.int64)
+        ptr[1:] = np.cumsum([len(r) for r in entries])
+        idx = np.array([e[0] for r in entries for e in r], dtype=np.int64)
+        val = np.array([e[1] for r in entries for e in r], dtype=np.float64)
+        return ptr, idx, val, inhom, isc
+
     # -- pressures --------------------------------------------------------
     def hydrostatic_pressure(self, gravity, density, z0):
         return -density * gravity * (self.mesh.support_xyz[:, 2] - z0)

@@ -375,16 +375,12 @@ int pbfmm_compute_alpha(pbfmm_handle *h, double *alpha) {
   });
 }
 
-int pbfmm_set_constraints(pbfmm_handle *h, const int64_t *c_ptr, const int64_t *c_idx, const double *c_val,
-                          const double *c_inhom, const uint8_t *is_c) {
-  return guarded(h, [&]() -> int {
-    if (int rc = need(h, 1, "set_constraints")) return rc;
+// Installs constraint rows (CSR over the DoFs, see pbfmm_set_constraints) in
+// the handle: host copies for the preconditioner, device copies for vmult.
+static int install_constraints(pbfmm_handle *h, const int64_t *c_ptr, const int64_t *c_idx, const double *c_val,
+                               const double *c_inhom, const uint8_t *is_c) {
+  {
     Engine &e = h->eng;
-    if (!c_ptr || !is_c) {
-      e.have_constraints = false;
-      e.have_prec = false;
-      return PBFMM_OK;
-    }
     const int64_t n = e.n;
     std::vector<int32_t> ptr(n + 1), idx;
     std::vector<double> val, inh(n, 0.0);
@@ -414,8 +410,132 @@ int pbfmm_set_constraints(pbfmm_handle *h, const int64_t *c_ptr, const int64_t *
     e.hc_idx = idx;
     e.hc_val = val;
     e.hc_is = isc;
+    e.hc_inh = inh;
     e.have_constraints = true;
     e.have_prec = false;
+    return PBFMM_OK;
+  }
+}
+
+int pbfmm_set_constraints(pbfmm_handle *h, const int64_t *c_ptr, const int64_t *c_idx, const double *c_val,
+                          const double *c_inhom, const uint8_t *is_c) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 1, "set_constraints")) return rc;
+    Engine &e = h->eng;
+    if (!c_ptr || !is_c) {
+      e.have_constraints = false;
+      e.have_prec = false;
+      return PBFMM_OK;
+    }
+    return install_constraints(h, c_ptr, c_idx, c_val, c_inhom, is_c);
+  });
+}
+
+// BEMProblem::compute_constraints (bem_problem.cc:1376-1653) for one process
+// and meshes without hanging nodes: the double-node rows from the projected
+// normals and the projected surface gradients of tmp_rhs.
+int pbfmm_compute_constraints(pbfmm_handle *h, const double *tmp_rhs, int32_t continuous_gradient) {
+  return guarded(h, [&]() -> int {
+    if (int rc = not_batch(h, "compute_constraints")) return rc;
+    if (int rc = ps_ready(h, "compute_constraints")) return rc;
+    if (!tmp_rhs) return fail(h, PBFMM_ERR_ARG, "compute_constraints: null tmp_rhs");
+    Engine &e = h->eng;
+    const int64_t n = e.n;
+    // vector_normals_solution and vector_surface_gradients_solution
+    // (compute_normals, compute_surface_gradients; bem_problem.cc:1385),
+    // component-major
+    std::vector<double> nrm((size_t)3 * n), sg((size_t)3 * n);
+    double *dout = (double *)stage(e.ps.out, (size_t)3 * n * 8);
+    e.ps_project(0, nullptr, nullptr, dout);
+    d2h(e, nrm.data(), dout, 3 * n);
+    double *d1 = (double *)stage(e.ps.in1, (size_t)n * 8);
+    h2d(e, d1, tmp_rhs, n);
+    e.ps_project(2, d1, nullptr, dout);
+    d2h(e, sg.data(), dout, 3 * n);
+    auto dot = [&](const std::vector<double> &a, int64_t i, const std::vector<double> &b, int64_t j) {
+      return a[i] * b[j] + a[n + i] * b[n + j] + a[2 * n + i] * b[2 * n + j];
+    };
+    const HostMesh &m = e.mesh;
+    std::vector<std::vector<std::pair<int64_t, double>>> ent(n);
+    std::vector<double> inh(n, 0.0);
+    std::vector<uint8_t> isc(n, 0);
+    for (int64_t i = 0; i < n; ++i) {
+      const int32_t *d0 = m.dbl_idx.data() + m.dbl_ptr[i], *d1e = m.dbl_idx.data() + m.dbl_ptr[i + 1];
+      // the first Dirichlet DoF of the set, else its first DoF (ascending)
+      int64_t first = *d0;
+      for (const int32_t *d = d0; d < d1e; ++d)
+        if (m.dir[*d]) {
+          first = *d;
+          break;
+        }
+      if (i != first) continue;
+      for (const int32_t *d = d0; d < d1e; ++d) {
+        const int64_t it = *d;
+        if (it == i) continue;
+        if (!m.dir[i]) {  // Neumann set: phi(it) = phi(i)
+          isc[it] = 1;
+          ent[it].push_back({i, 1.0});
+        } else if (!m.dir[it]) {  // the Neumann double takes the Dirichlet value
+          isc[it] = 1;
+          inh[it] = tmp_rhs[i];
+        } else {
+          // Dirichlet-Dirichlet.  The reference's parallel-normal test
+          // divides the dot product by itself (bem_problem.cc:1490), so it
+          // never fires and parallel normals divide by 1 - (n.n)^2 = 0; the
+          // intended test is made here: normals within 1e-4 of parallel
+          // give normal derivatives equal up to the sign of n_i.n_it.
+          const double nn = dot(nrm, i, nrm, it);
+          const double ni = std::sqrt(dot(nrm, i, nrm, i)), nt = std::sqrt(dot(nrm, it, nrm, it));
+          const double cosang = (ni > 0 && nt > 0) ? nn / (ni * nt) : 1.0;
+          if (1.0 - std::fabs(cosang) < 1e-4) {
+            isc[it] = 1;
+            ent[it].push_back({i, cosang > 0 ? 1.0 : -1.0});
+          } else if (continuous_gradient) {
+            // both normal derivatives from the surface gradients
+            // (bem_problem.cc:1499-1557)
+            const double s_it_n_i = dot(sg, it, nrm, i), s_i_n_it = dot(sg, i, nrm, it);
+            const double den = 1.0 - nn * nn;
+            isc[i] = 1;
+            inh[i] = (s_it_n_i + s_i_n_it * nn) / den;
+            isc[it] = 1;
+            inh[it] = (s_i_n_it + s_it_n_i * nn) / den;
+          }
+        }
+      }
+    }
+    std::vector<int64_t> ptr(n + 1, 0), idx;
+    std::vector<double> val;
+    for (int64_t i = 0; i < n; ++i) {
+      for (auto &pv : ent[i]) {
+        idx.push_back(pv.first);
+        val.push_back(pv.second);
+      }
+      ptr[i + 1] = (int64_t)idx.size();
+    }
+    return install_constraints(h, ptr.data(), idx.data(), val.data(), inh.data(), isc.data());
+  });
+}
+
+int pbfmm_get_constraints(pbfmm_handle *h, int64_t *nnz, int64_t *c_ptr, int64_t *c_idx, double *c_val,
+                          double *c_inhom, uint8_t *is_c) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 1, "get_constraints")) return rc;
+    if (!nnz) return fail(h, PBFMM_ERR_ARG, "get_constraints: null nnz");
+    Engine &e = h->eng;
+    const int64_t n = e.n;
+    *nnz = e.have_constraints ? (int64_t)e.hc_idx.size() : 0;
+    if (!c_ptr && !c_idx && !c_val && !c_inhom && !is_c) return PBFMM_OK;
+    if (!c_ptr || !c_inhom || !is_c || (*nnz > 0 && (!c_idx || !c_val)))
+      return fail(h, PBFMM_ERR_ARG, "get_constraints: null output");
+    for (int64_t i = 0; i <= n; ++i) c_ptr[i] = e.have_constraints ? e.hc_ptr[i] : 0;
+    for (int64_t k = 0; k < *nnz; ++k) {
+      c_idx[k] = e.hc_idx[k];
+      c_val[k] = e.hc_val[k];
+    }
+    for (int64_t i = 0; i < n; ++i) {
+      c_inhom[i] = e.have_constraints ? e.hc_inh[i] : 0.0;
+      is_c[i] = e.have_constraints ? e.hc_is[i] : 0;
+    }
     return PBFMM_OK;
   });
 }
@@ -814,7 +934,8 @@ int pbfmm_work_counts(pbfmm_handle *h, int64_t *c) {
 
 int pbfmm_last_timings(pbfmm_handle *h, double *ms) {
   return guarded(h, [&]() -> int {
-    for (int i = 0; i < 8; ++i) ms[i] = h->eng.last_ms[i];
+    if (!ms) return fail(h, PBFMM_ERR_ARG, "last_timings: null output");
+    for (int i = 0; i < 9; ++i) ms[i] = h->eng.last_ms[i];
     return PBFMM_OK;
   }, false);
 }
@@ -865,6 +986,27 @@ int pbfmm_compute_gradients(pbfmm_handle *h, const double *phi, const double *dp
 
 int pbfmm_compute_surface_gradients(pbfmm_handle *h, const double *tmp_rhs, double *gradients) {
   return ps_project_host(h, 2, tmp_rhs, nullptr, gradients, "compute_surface_gradients");
+}
+
+int pbfmm_preconditioner_blocks(pbfmm_handle *h, int64_t *sizes, int32_t *row_beg, int32_t *rows, double *values) {
+  return guarded(h, [&]() -> int {
+    if (int rc = need(h, 4, "preconditioner_blocks")) return rc;
+    if (!sizes) return fail(h, PBFMM_ERR_ARG, "preconditioner_blocks: null sizes");
+    if (h->prm.preconditioner != 1) return fail(h, PBFMM_ERR_STATE, "preconditioner_blocks: no preconditioner");
+    ensure_prec(h);
+    Engine &e = h->eng;
+    sizes[0] = e.nprec;
+    sizes[1] = (int64_t)e.prows.size();
+    sizes[2] = e.pinv_beg.empty() ? 0 : e.pinv_beg[e.nprec];
+    if (!row_beg && !rows && !values) return PBFMM_OK;
+    if (!row_beg || !rows || !values) return fail(h, PBFMM_ERR_ARG, "preconditioner_blocks: null output");
+    std::copy(e.prow_beg.begin(), e.prow_beg.end(), row_beg);
+    std::copy(e.prows.begin(), e.prows.end(), rows);
+    std::vector<double> blk;
+    e.prec_blocks(blk);
+    std::copy(blk.begin(), blk.end(), values);
+    return PBFMM_OK;
+  });
 }
 
 int pbfmm_set_mixed_scheme(pbfmm_handle *h, int32_t scheme) {

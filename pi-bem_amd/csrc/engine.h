@@ -106,7 +106,7 @@ struct Engine {
   DevBuf d_c_ptr, d_c_idx, d_c_val, d_c_inh, d_c_is;
   bool have_constraints = false;
   std::vector<int32_t> hc_ptr, hc_idx;  // host copies of the constraint rows
-  std::vector<double> hc_val;
+  std::vector<double> hc_val, hc_inh;
   std::vector<uint8_t> hc_is;
   // near-field block-Jacobi preconditioner: blocks of the near leaves that
   // share a parent block (at most kPrecRows rows, leaf after leaf; a larger
@@ -125,15 +125,18 @@ struct Engine {
   std::vector<int32_t> prow_beg, prows;
   DevBuf d_pinv_beg, d_pinv, d_ptmp, d_ptmp2, d_prow_beg, d_prows, d_prow_blk;
   void build_preconditioner();
+  std::vector<int32_t> prec_pos;  // block-ordered row -> leaf-ordered near row
+  void prec_blocks(std::vector<double> &blk);
   void apply_prec(const double *v, double *z, int R = 1);  // R right-hand sides, n apart
   DevBuf d_rhs_sum;  // per right-hand side sums of the lockstep operator's mean term
   // GMRES workspace
   DevBuf d_V, d_w, d_r, d_x, d_h;
   int gm_restart = 0;
   // timings of the last matvec (ms)
-  double last_ms[8] = {0};
+  double last_ms[9] = {0};
   bool time_phases = false;
-  hipEvent_t ev[11] = {};
+  hipEvent_t ev[12] = {};
+  bool slot_marked = false;  // the last phase-timed M2L ran as slots + sum (event 11 between)
   void mark(int i);
   void collect_phase_times();
 

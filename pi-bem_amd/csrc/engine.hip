@@ -455,7 +455,8 @@ __device__ __forceinline__ T wave_allsum(T v) {
   return v;
 }
 
-// Set-up, FP64: one thread per quad point of the leaves, atomics into U.
+// Set-up, FP64: one thread per quad point of the leaves, atomics into U
+// (for leaves whose unit multipoles do not fit in LDS).
 __global__ void __launch_bounds__(256) k_p2m_unit(
     int npts, int p, int kt, int kp, const int *__restrict__ qpt, const int *__restrict__ qleaf,
     const int *__restrict__ leaves, const int *__restrict__ qcol, int dpc, const double *__restrict__ qx,
@@ -511,6 +512,79 @@ __global__ void __launch_bounds__(256) k_p2m_unit(
       Rcr = Rnr; Rci = Rni; Gcr = Gnr; Gci = Gni;
     }
   }
+}
+
+// Set-up, FP64, one workgroup per leaf whose unit multipoles fit in LDS: the
+// block [column][2 kp] of the leaf is summed in LDS (atomic adds there) and
+// written once with plain stores, since no other leaf owns those columns.
+// Work items are (quad point, m): the item runs the m-recurrence to R_m^m
+// and then the n-recurrence of column m of the harmonics, as k_p2m_unit
+// does for all m of its point.
+__global__ void __launch_bounds__(256) k_p2m_unit_leaf(
+    int p, int kt, int kp, const int *__restrict__ leaf_ids, const int *__restrict__ leaves,
+    const int *__restrict__ qbeg, const int *__restrict__ col_beg, const int *__restrict__ qpt,
+    const int *__restrict__ qcol, int dpc, const double *__restrict__ qx, const double *__restrict__ qn,
+    const double *__restrict__ qw, const double *__restrict__ qsh, const double *__restrict__ pmin,
+    const double *__restrict__ delta, double *__restrict__ U) {
+  extern __shared__ double sU[];
+  const int li = leaf_ids[blockIdx.x];
+  const int b = leaves[li];
+  const int c0 = col_beg[li], nc = col_beg[li + 1] - c0;
+  const int q0 = qbeg[li], nq = qbeg[li + 1] - q0;
+  const int K2 = 2 * kp;
+  for (int i = threadIdx.x; i < nc * K2; i += blockDim.x) sU[i] = 0.0;
+  __syncthreads();
+  const double hd = delta[b] * 0.5, inv_delta = 1.0 / delta[b];
+  const double cx = pmin[3 * b] + hd, cy = pmin[3 * b + 1] + hd, cz = pmin[3 * b + 2] + hd;
+  for (int item = threadIdx.x; item < nq * (p + 1); item += blockDim.x) {
+    const int i = q0 + item / (p + 1), m = item % (p + 1);
+    const int64_t g = qpt[i];
+    const double x = qx[3 * g] - cx, y = qx[3 * g + 1] - cy, z = qx[3 * g + 2] - cz;
+    const double nx = qn[3 * g], ny = qn[3 * g + 1], nz = qn[3 * g + 2];
+    const double r2 = x * x + y * y + z * z;
+    const double dn2 = 2.0 * (x * nx + y * ny + z * nz);
+    double wj[9];
+    int cj[9];
+    for (int j = 0; j < dpc; ++j) {
+      wj[j] = qsh[g * dpc + j] * qw[g] / (4 * kPi);
+      cj[j] = (qcol[(int64_t)i * dpc + j] - c0) * K2;
+    }
+    double Rmr = 1, Rmi = 0, Gmr = 0, Gmi = 0;  // R_m^m, D R_m^m
+    double sc = 1.0;                            // 1 / delta^n
+    for (int mm = 1; mm <= m; ++mm) {
+      const double f = -PMF<double>(mm);
+      const double nr = f * (x * Rmr + y * Rmi), ni = f * (x * Rmi - y * Rmr);
+      const double gr = f * ((nx * Rmr + ny * Rmi) + (x * Gmr + y * Gmi));
+      const double gi = f * ((nx * Rmi - ny * Rmr) + (x * Gmi - y * Gmr));
+      Rmr = nr; Rmi = ni; Gmr = gr; Gmi = gi;
+      sc *= inv_delta;
+    }
+    auto emit = [&](int k, double s, double Rr, double Ri, double Gr, double Gi) {
+      Rr *= s; Ri *= s; Gr *= s; Gi *= s;
+      for (int j = 0; j < dpc; ++j) {
+        double *col = sU + cj[j];
+        atomicAdd(col + k, wj[j] * Gr);
+        atomicAdd(col + kt + k, wj[j] * Gi);
+        atomicAdd(col + kp + k, wj[j] * Rr);
+        atomicAdd(col + kp + kt + k, wj[j] * Ri);
+      }
+    };
+    double Rcr = Rmr, Rci = Rmi, Gcr = Gmr, Gci = Gmi, Rpr = 0, Rpi = 0, Gpr = 0, Gpi = 0;
+    emit(m * (m + 1) / 2 + m, sc, Rcr, Rci, Gcr, Gci);
+    for (int n = m + 1; n <= p; ++n) {
+      const double a = RA<double>(n, m), bb = RB<double>(n, m);
+      const double Rnr = a * z * Rcr - bb * r2 * Rpr, Rni = a * z * Rci - bb * r2 * Rpi;
+      const double Gnr = a * (nz * Rcr + z * Gcr) - bb * (dn2 * Rpr + r2 * Gpr);
+      const double Gni = a * (nz * Rci + z * Gci) - bb * (dn2 * Rpi + r2 * Gpi);
+      sc *= inv_delta;
+      emit(n * (n + 1) / 2 + m, sc, Rnr, Rni, Gnr, Gni);
+      Rpr = Rcr; Rpi = Rci; Gpr = Gcr; Gpi = Gci;
+      Rcr = Rnr; Rci = Rni; Gcr = Gnr; Gci = Gni;
+    }
+  }
+  __syncthreads();
+  double *out = U + (int64_t)c0 * K2;
+  for (int i = threadIdx.x; i < nc * K2; i += blockDim.x) out[i] = sU[i];
 }
 
 // Per matvec: one workgroup per leaf.  The leaf's 2 kp outputs are read as
@@ -1544,7 +1618,8 @@ Engine::~Engine() {
 }
 
 // Phase events on the handle's stream (no synchronisation between kernels):
-// 10 clear ME 0 P2M 1 M2M 2, 3 SpMV 4 clear LE 5 M2L 6 M2P 7 L2L 8 L2P 9.
+// 10 clear ME 0 P2M 1 M2M 2, 3 SpMV 4 clear LE 5 M2L [11 slot sum] 6 M2P 7
+// L2L 8 L2P 9.
 void Engine::mark(int i) {
   if (!time_phases) return;
   if (!ev[i]) HIPCHK(hipEventCreate(&ev[i]));
@@ -1563,7 +1638,9 @@ void Engine::collect_phase_times() {
   last_ms[1] = el(0, 1);  // p2m
   last_ms[2] = el(1, 2);  // m2m
   last_ms[3] = el(7, 8);  // l2l
-  last_ms[4] = el(5, 6);  // m2l
+  last_ms[4] = slot_marked ? el(5, 11) : el(5, 6);  // m2l (translations)
+  last_ms[8] = slot_marked ? el(11, 6) : 0.0;       // m2l slot sums
+  slot_marked = false;
   last_ms[5] = el(6, 7);  // m2p
   last_ms[6] = el(8, 9);  // l2p
   last_ms[7] = el(10, 9);  // whole matvec
@@ -1674,16 +1751,57 @@ void Engine::build_unit_multipoles(bool want_f32) {
   HIPCHK(hipMemsetAsync(f64.unit.p, 0, f64.unit.bytes, stream));
   const int npts = (int)far.p2m_qpt.size();
   if (npts > 0) {
-    DevBuf qpt, qleaf, qcol;
+    // leaves whose [column][2 kp] block fits in LDS: one workgroup each, sums
+    // in LDS; the others (none on the test meshes) by global atomics
+    const int nleaf = (int)far.p2m_leaf.size();
+    const size_t col_bytes = (size_t)2 * kp * sizeof(double);
+    constexpr size_t kLdsMax = 160 * 1024 - 1024;
+    std::vector<int32_t> qbeg(nleaf + 1, 0), fit, big_qpt, big_qleaf, big_qcol;
+    for (int i = 0; i < npts; ++i) qbeg[far.p2m_qleaf[i] + 1]++;
+    for (int li = 0; li < nleaf; ++li) qbeg[li + 1] += qbeg[li];
+    size_t max_bytes = 0;
+    for (int li = 0; li < nleaf; ++li) {
+      const size_t bytes = (size_t)(far.p2m_col_beg[li + 1] - far.p2m_col_beg[li]) * col_bytes;
+      if (bytes <= kLdsMax) {
+        if (qbeg[li + 1] > qbeg[li]) fit.push_back(li);
+        max_bytes = std::max(max_bytes, bytes);
+      } else {
+        for (int i = qbeg[li]; i < qbeg[li + 1]; ++i) {
+          big_qpt.push_back(far.p2m_qpt[i]);
+          big_qleaf.push_back(li);
+          for (int j = 0; j < dpc; ++j) big_qcol.push_back(far.p2m_qcol[(size_t)i * dpc + j]);
+        }
+      }
+    }
+    DevBuf qpt, qcol, dqbeg, dfit;
     up(qpt, far.p2m_qpt);
-    up(qleaf, far.p2m_qleaf);
     up(qcol, far.p2m_qcol);
-    hipLaunchKernelGGL(k_p2m_unit, dim3(nblk(npts, 256)), dim3(256), 0, stream, npts, p, kt, kp, qpt.as<int>(),
-                       qleaf.as<int>(), d_p2m_leaf.as<int>(), qcol.as<int>(), dpc, f64.qx.as<double>(),
-                       f64.qn.as<double>(), f64.qw.as<double>(), f64.qsh.as<double>(), d_pmin.as<double>(),
-                       d_delta.as<double>(), f64.unit.as<double>());
-    HIPCHK(hipGetLastError());
-    sync();  // qpt, qleaf, qcol are freed on return
+    up(dqbeg, qbeg);
+    up(dfit, fit);
+    if (!fit.empty()) {
+      HIPCHK(hipFuncSetAttribute((const void *)k_p2m_unit_leaf, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)std::max<size_t>(max_bytes, 16)));
+      hipLaunchKernelGGL(k_p2m_unit_leaf, dim3((unsigned)fit.size()), dim3(256), max_bytes, stream, p, kt, kp,
+                         dfit.as<int>(), d_p2m_leaf.as<int>(), dqbeg.as<int>(), d_p2m_col_beg.as<int>(),
+                         qpt.as<int>(), qcol.as<int>(), dpc, f64.qx.as<double>(), f64.qn.as<double>(),
+                         f64.qw.as<double>(), f64.qsh.as<double>(), d_pmin.as<double>(), d_delta.as<double>(),
+                         f64.unit.as<double>());
+      HIPCHK(hipGetLastError());
+    }
+    if (!big_qpt.empty()) {
+      DevBuf bqpt, bqleaf, bqcol;
+      up(bqpt, big_qpt);
+      up(bqleaf, big_qleaf);
+      up(bqcol, big_qcol);
+      const int nb = (int)big_qpt.size();
+      hipLaunchKernelGGL(k_p2m_unit, dim3(nblk(nb, 256)), dim3(256), 0, stream, nb, p, kt, kp, bqpt.as<int>(),
+                         bqleaf.as<int>(), d_p2m_leaf.as<int>(), bqcol.as<int>(), dpc, f64.qx.as<double>(),
+                         f64.qn.as<double>(), f64.qw.as<double>(), f64.qsh.as<double>(), d_pmin.as<double>(),
+                         d_delta.as<double>(), f64.unit.as<double>());
+      HIPCHK(hipGetLastError());
+      sync();  // the index copies are freed on return
+    }
+    sync();  // qpt, qcol, dqbeg, dfit are freed on return
   }
   // the operator's unit multipoles (U1 on Neumann, -U2 on Dirichlet DoFs)
   const int64_t nou = std::max<int64_t>(ncol, 1) * kp;
@@ -2101,6 +2219,8 @@ template <typename T>
 void Engine::launch_m2l_slots(const T *ops, const T *IN, T *LE) {
   T *slots = m2l_slots<T>();
   launch_translate<T, 3, 1>(*this, stream, m2l_cl, 0, nchunks<1>(far.m2l), ops, IN, slots);
+  mark(11);
+  slot_marked = time_phases;
   if (n_red_tgt > 0)
     hipLaunchKernelGGL(k_m2l_reduce<T>, dim3((unsigned)((n_red_tgt + 3) / 4)), dim3(256), 0, stream, n_red_tgt,
                        d_red_tgt.as<int>(), d_red_beg.as<int>(), d_red_pair.as<int>(), 2 * kt, kp,
@@ -2440,6 +2560,42 @@ static bool invert_block(std::vector<double> &A, int R) {
   return true;
 }
 
+// The preconditioner's blocks before inversion, row-major, block g at
+// pinv_beg[g] over the rows prows[prow_beg[g] ..): the near field with -D in
+// Dirichlet and N in Neumann columns, alpha on the diagonal of Neumann rows
+// (k_extract_groups, on the GPU from the assembled FP64 near field), and
+// constrained rows replaced by their constraint row src(i) - sum c_ij src(j)
+// (constrained_matrix.h:109-121), on the host.
+void Engine::prec_blocks(std::vector<double> &blk) {
+  blk.assign(pinv_beg.empty() ? 0 : pinv_beg[nprec], 0.0);
+  if (nprec <= 0) return;
+  DevBuf d_pos, d_raw;
+  up(d_pos, prec_pos);
+  alloc(d_raw, std::max<size_t>(blk.size(), 1) * 8);
+  hipLaunchKernelGGL(k_extract_groups, dim3(nprec), dim3(256), 0, stream, d_prow_beg.as<int>(), d_pos.as<int>(),
+                     d_row_leaf.as<int>(), d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(),
+                     d_cols.as<int>(), d_val_beg.as<int64_t>(), f64.vals.as<double>(), d_dir.as<uint8_t>(),
+                     d_alpha.as<double>(), d_pinv_beg.as<int64_t>(), d_raw.as<double>());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(blk.data(), d_raw.p, blk.size() * 8, hipMemcpyDeviceToHost, stream));
+  sync();  // d_pos and d_raw are freed on return
+  if (!have_constraints) return;
+  for (int g = 0; g < nprec; ++g) {
+    const int r0 = prow_beg[g], R = prow_beg[g + 1] - r0;
+    double *A = blk.data() + pinv_beg[g];
+    for (int r = 0; r < R; ++r) {
+      const int i = prows[r0 + r];
+      if (!hc_is[i]) continue;
+      for (int c = 0; c < R; ++c) A[(size_t)r * R + c] = (c == r) ? 1.0 : 0.0;
+      for (int k = hc_ptr[i]; k < hc_ptr[i + 1]; ++k) {
+        const int j = hc_idx[k];
+        for (int c = 0; c < R; ++c)
+          if (prows[r0 + c] == j) A[(size_t)r * R + c] -= hc_val[k];
+      }
+    }
+  }
+}
+
 void Engine::build_preconditioner() {
   // blocks: near leaves with the same parent block, in near-leaf order; a
   // parent of more than kPrecRows rows is cut into ceil(rows / kPrecRows)
@@ -2487,37 +2643,16 @@ void Engine::build_preconditioner() {
   up(d_prow_beg, prow_beg);
   up(d_prows, prows);
   up(d_prow_blk, prow_blk);
+  prec_pos = pos;
   alloc(d_pinv, (size_t)std::max<int64_t>(pinv_beg[nprec], 1) * 8);
-  if (nprec > 0) {
-    DevBuf d_pos;
-    up(d_pos, pos);
-    hipLaunchKernelGGL(k_extract_groups, dim3(nprec), dim3(256), 0, stream, d_prow_beg.as<int>(), d_pos.as<int>(),
-                       d_row_leaf.as<int>(), d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(),
-                       d_cols.as<int>(), d_val_beg.as<int64_t>(), f64.vals.as<double>(), d_dir.as<uint8_t>(),
-                       d_alpha.as<double>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>());
-    HIPCHK(hipGetLastError());
-    sync();  // d_pos is freed on return
-  }
-  std::vector<double> blk(pinv_beg[nprec]);
-  if (!blk.empty()) HIPCHK(hipMemcpy(blk.data(), d_pinv.p, blk.size() * 8, hipMemcpyDeviceToHost));
+  std::vector<double> blk;
+  prec_blocks(blk);
   // invert the blocks on host threads
   auto invert_range = [&](int g_lo, int g_hi) {
     std::vector<double> A;
     for (int g = g_lo; g < g_hi; ++g) {
-      const int r0 = prow_beg[g], R = prow_beg[g + 1] - r0;
+      const int R = prow_beg[g + 1] - prow_beg[g];
       A.assign(blk.begin() + pinv_beg[g], blk.begin() + pinv_beg[g + 1]);
-      if (have_constraints)
-        for (int r = 0; r < R; ++r) {
-          const int i = prows[r0 + r];
-          if (!hc_is[i]) continue;
-          // constrained row: dst(i) = src(i) - sum c_ij src(j) (constrained_matrix.h:109-121)
-          for (int c = 0; c < R; ++c) A[(size_t)r * R + c] = (c == r) ? 1.0 : 0.0;
-          for (int k = hc_ptr[i]; k < hc_ptr[i + 1]; ++k) {
-            const int j = hc_idx[k];
-            for (int c = 0; c < R; ++c)
-              if (prows[r0 + c] == j) A[(size_t)r * R + c] -= hc_val[k];
-          }
-        }
       std::vector<double> keep = A;
       if (!invert_block(A, R)) {
         // singular block (e.g. unconstrained double nodes): diagonal scaling

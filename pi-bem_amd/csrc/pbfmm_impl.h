@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -15,6 +16,10 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kToll = 0.001;  // box adjacency tolerance of bem_fma.cc:1
 
 inline int kt_of(int p) { return (p + 1) * (p + 2) / 2; }
+
+// Runs body(i) for i in [0, n) on up to 16 host threads, `grain` indices at
+// a time (dynamic); the caller's results must not depend on the order.
+void parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t)> &body);
 // Pairs per GEMM chunk of the batched translations (64 GEMM columns): 32 pairs
 // when both kernels' expansions travel (2 columns per pair), 64 pairs when
 // one combined expansion does.

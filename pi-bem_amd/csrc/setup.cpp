@@ -5,6 +5,10 @@
 // layout is flat (sorted index ranges and CSR lists) rather than per-block
 // std::set / std::map objects.
 #include <algorithm>
+#include <atomic>
+#include <exception>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <array>
@@ -313,8 +317,11 @@ void build_tree(const HostMesh &m, const QuadData &qd, int L, int max_nodes, Tre
   // near neighbours (bem_fma.cc:4271-4579)
   for (int b = t.startL[1]; b <= t.endL[1] && L >= 1; ++b)
     for (int c = t.startL[1]; c <= t.endL[1]; ++c) t.nn[t.sub_beg[b]].push_back(c);
+  // every block of a level needs only its parent's lists: the blocks of a
+  // level run on host threads, level after level
   for (int level = 2; level <= L; ++level)
-    for (int b = t.startL[level]; b <= t.endL[level]; ++b) {
+    parallel_for(t.endL[level] - t.startL[level] + 1, 64, [&](int64_t kb) {
+      const int b = t.startL[level] + (int)kb;
       std::vector<int32_t> &n0 = t.nn[t.sub_beg[b]];
       n0.push_back(b);
       for (int pos : t.nn[t.sub_beg[t.parent[b]]]) {
@@ -332,16 +339,16 @@ void build_tree(const HostMesh &m, const QuadData &qd, int L, int max_nodes, Tre
         }
         sort_unique(ns_);
       }
-    }
+    });
 
   // interaction lists (bem_fma.cc:4610-4711)
   for (int b = t.startL[1]; b <= t.endL[1] && L >= 1; ++b)
     for (int c = t.startL[1]; c <= t.endL[1]; ++c) t.il[t.sub_beg[b]].push_back(c);
-  std::vector<int32_t> cells;
   for (int level = 2; level <= L; ++level)
-    for (int b = t.startL[level]; b <= t.endL[level]; ++b) {
+    parallel_for(t.endL[level] - t.startL[level] + 1, 64, [&](int64_t kb) {
+      const int b = t.startL[level] + (int)kb;
       // cells touching any double of any node of b
-      cells.clear();
+      std::vector<int32_t> cells;
       for (int i = 0; i < t.node_cnt[b]; ++i) {
         const int node = t.nperm[t.node_beg[b] + i];
         for (int k = m.dbl_ptr[node]; k < m.dbl_ptr[node + 1]; ++k) {
@@ -373,15 +380,50 @@ void build_tree(const HostMesh &m, const QuadData &qd, int L, int max_nodes, Tre
         }
         sort_unique(nl);
       }
-    }
+    });
 }
 
 // ---------------------------------------------------------------------------
 // Near-field plan (direct_integrals, bem_fma.cc:107-1600)
 // ---------------------------------------------------------------------------
-void build_near_plan(const HostMesh &m, const QuadData &qd, const Tree &t, NearPlan &np) {
-  np = NearPlan();
-  if (qd.nq > 64) throw std::runtime_error("quadrature order above 8 is not supported");
+void parallel_for(int64_t n, int64_t grain, const std::function<void(int64_t)> &body) {
+  if (n <= 0) return;
+  grain = std::max<int64_t>(grain, 1);
+  const int nth = (int)std::min<int64_t>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())),
+                                         (n + grain - 1) / grain);
+  std::atomic<int64_t> next{0};
+  std::exception_ptr err;
+  std::mutex mu;
+  auto work = [&]() {
+    try {
+      for (;;) {
+        const int64_t b = next.fetch_add(grain);
+        if (b >= n) break;
+        for (int64_t i = b; i < std::min(n, b + grain); ++i) body(i);
+      }
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(mu);
+      if (!err) err = std::current_exception();
+      next.store(n);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int k = 1; k < nth; ++k) th.emplace_back(work);
+  work();
+  for (auto &x : th) x.join();
+  if (err) std::rethrow_exception(err);
+}
+
+// The near field of one leaf: its entries (cell, quad-point mask, near
+// flag), the column of each entry's DoFs and its sorted rows and columns.
+namespace {
+struct LeafNear {
+  std::vector<int32_t> ent_cell, ent_colpos, rows, cols;
+  std::vector<uint64_t> ent_mask;
+  std::vector<uint8_t> ent_near;
+};
+
+void leaf_near(const HostMesh &m, const QuadData &qd, const Tree &t, int leaf, LeafNear &o) {
   const int dpc = m.dpc;
   std::vector<int32_t> qs;
   auto add_entries = [&](std::vector<int32_t> &q, uint8_t near) {
@@ -391,61 +433,90 @@ void build_near_plan(const HostMesh &m, const QuadData &qd, const Tree &t, NearP
       const int cell = q[i] / qd.nq;
       uint64_t mask = 0;
       while (i < q.size() && q[i] / qd.nq == cell) mask |= (uint64_t)1 << (q[i++] % qd.nq);
-      np.ent_cell.push_back(cell);
-      np.ent_mask.push_back(mask);
-      np.ent_near.push_back(near && cell < m.ncells_real);  // mirror images are never singular
+      o.ent_cell.push_back(cell);
+      o.ent_mask.push_back(mask);
+      o.ent_near.push_back(near && cell < m.ncells_real);  // mirror images are never singular
     }
   };
   auto quads_of = [&](int b, std::vector<int32_t> &q) {
     for (int i = 0; i < t.quad_cnt[b]; ++i) q.push_back(t.qperm[t.quad_beg[b] + i]);
   };
+  // near quad points: blocks of the deepest intList
+  for (int b : t.il[t.sub_beg[leaf] + t.nsub[leaf] - 1]) quads_of(b, qs);
+  add_entries(qs, 1);
+  // bigger far blocks in the nonIntLists of the leaf and of every ancestor
+  for (int a = leaf; a != 0; a = t.parent[a]) {
+    const int lev = t.level[a];
+    qs.clear();
+    for (int s = 0; s < t.nsub[a]; ++s)
+      for (int b : t.nil[t.sub_beg[a] + s])
+        if (b < t.startL[lev]) quads_of(b, qs);
+    if (!qs.empty()) add_entries(qs, 0);
+  }
+  const size_t ne = o.ent_cell.size();
+  for (size_t e = 0; e < ne; ++e)
+    for (int j = 0; j < dpc; ++j) o.cols.push_back(m.cdofs[(int64_t)o.ent_cell[e] * dpc + j]);
+  sort_unique(o.cols);
+  o.ent_colpos.reserve(ne * dpc);
+  for (size_t e = 0; e < ne; ++e)
+    for (int j = 0; j < dpc; ++j) {
+      const int d = m.cdofs[(int64_t)o.ent_cell[e] * dpc + j];
+      o.ent_colpos.push_back((int32_t)(std::lower_bound(o.cols.begin(), o.cols.end(), d) - o.cols.begin()));
+    }
+  o.rows.assign(t.nperm.begin() + t.node_beg[leaf], t.nperm.begin() + t.node_beg[leaf] + t.node_cnt[leaf]);
+  std::sort(o.rows.begin(), o.rows.end());
+}
+}  // namespace
+
+// Leaves are independent: each is built on a host thread, then they are
+// concatenated in leaf order, so the plan does not depend on the threads.
+void build_near_plan(const HostMesh &m, const QuadData &qd, const Tree &t, NearPlan &np) {
+  np = NearPlan();
+  if (qd.nq > 64) throw std::runtime_error("quadrature order above 8 is not supported");
+  std::vector<int> leaves;
+  for (int leaf : t.leaves)
+    if (t.node_cnt[leaf] > 0) leaves.push_back(leaf);
+  std::vector<LeafNear> part(leaves.size());
+  parallel_for((int64_t)leaves.size(), 64, [&](int64_t k) { leaf_near(m, qd, t, leaves[k], part[k]); });
+  size_t ne = 0, nr = 0, nc = 0;
+  for (const auto &o : part) {
+    ne += o.ent_cell.size();
+    nr += o.rows.size();
+    nc += o.cols.size();
+  }
+  np.ent_cell.reserve(ne);
+  np.ent_mask.reserve(ne);
+  np.ent_near.reserve(ne);
+  np.ent_colpos.reserve(ne * m.dpc);
+  np.rows.reserve(nr);
+  np.row_leaf.reserve(nr);
+  np.cols.reserve(nc);
   np.row_beg.push_back(0);
   np.col_beg.push_back(0);
   np.ent_beg.push_back(0);
   np.work_beg.push_back(0);
-  std::vector<int32_t> cols;
-  for (int leaf : t.leaves) {
-    if (t.node_cnt[leaf] == 0) continue;
-    const int e0 = (int)np.ent_cell.size();
-    // near quad points: blocks of the deepest intList
-    qs.clear();
-    for (int b : t.il[t.sub_beg[leaf] + t.nsub[leaf] - 1]) quads_of(b, qs);
-    add_entries(qs, 1);
-    // bigger far blocks in the nonIntLists of the leaf and of every ancestor
-    for (int a = leaf; a != 0; a = t.parent[a]) {
-      const int lev = t.level[a];
-      qs.clear();
-      for (int s = 0; s < t.nsub[a]; ++s)
-        for (int b : t.nil[t.sub_beg[a] + s])
-          if (b < t.startL[lev]) quads_of(b, qs);
-      if (!qs.empty()) add_entries(qs, 0);
-    }
-    const int e1 = (int)np.ent_cell.size();
-    cols.clear();
-    for (int e = e0; e < e1; ++e)
-      for (int j = 0; j < dpc; ++j) cols.push_back(m.cdofs[(int64_t)np.ent_cell[e] * dpc + j]);
-    sort_unique(cols);
-    for (int e = e0; e < e1; ++e)
-      for (int j = 0; j < dpc; ++j) {
-        const int d = m.cdofs[(int64_t)np.ent_cell[e] * dpc + j];
-        np.ent_colpos.push_back((int32_t)(std::lower_bound(cols.begin(), cols.end(), d) - cols.begin()));
-      }
-    std::vector<int32_t> rows(t.nperm.begin() + t.node_beg[leaf],
-                              t.nperm.begin() + t.node_beg[leaf] + t.node_cnt[leaf]);
-    std::sort(rows.begin(), rows.end());
+  for (size_t k = 0; k < leaves.size(); ++k) {
+    LeafNear &o = part[k];
+    const int64_t e0 = (int64_t)np.ent_cell.size();
+    np.ent_cell.insert(np.ent_cell.end(), o.ent_cell.begin(), o.ent_cell.end());
+    np.ent_mask.insert(np.ent_mask.end(), o.ent_mask.begin(), o.ent_mask.end());
+    np.ent_near.insert(np.ent_near.end(), o.ent_near.begin(), o.ent_near.end());
+    np.ent_colpos.insert(np.ent_colpos.end(), o.ent_colpos.begin(), o.ent_colpos.end());
+    const int64_t e1 = (int64_t)np.ent_cell.size();
     np.val_beg.push_back(np.nnz);
-    np.nnz += (int64_t)rows.size() * (int64_t)cols.size();
-    for (int r : rows) {
+    np.nnz += (int64_t)o.rows.size() * (int64_t)o.cols.size();
+    for (int r : o.rows) {
       np.rows.push_back(r);
       np.row_leaf.push_back(np.nleaf);
     }
-    np.cols.insert(np.cols.end(), cols.begin(), cols.end());
-    np.leaf_block.push_back(leaf);
+    np.cols.insert(np.cols.end(), o.cols.begin(), o.cols.end());
+    np.leaf_block.push_back(leaves[k]);
     np.row_beg.push_back((int32_t)np.rows.size());
     np.col_beg.push_back((int32_t)np.cols.size());
-    np.ent_beg.push_back(e1);
-    np.work_beg.push_back(np.work_beg.back() + (int64_t)rows.size() * (e1 - e0));
+    np.ent_beg.push_back((int32_t)e1);
+    np.work_beg.push_back(np.work_beg.back() + (int64_t)o.rows.size() * (e1 - e0));
     np.nleaf++;
+    o = LeafNear();  // free as we go
   }
 }
 
@@ -703,14 +774,21 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
       auto it = op_of.find(kv.first);
       if (it == op_of.end()) {
         const int op = fp.n_m2l_ops++;
-        fp.m2l_ops.resize((size_t)fp.n_m2l_ops * kp * kp);
-        const double d[3] = {(double)kv.first[0], (double)kv.first[1], (double)kv.first[2]};
-        op_m2l(p, d, &fp.m2l_ops[(size_t)op * kp * kp]);  // unit box
         it = op_of.emplace(kv.first, op).first;
       }
       add_class(fp.m2l, it->second, kv.second);
       fp.m2l_per_level[l] += (int64_t)kv.second.size();
     }
+  }
+  // the operators of the offset classes (unit box), on host threads
+  {
+    std::vector<std::array<int, 3>> key(fp.n_m2l_ops);
+    for (auto &kv : op_of) key[kv.second] = kv.first;
+    fp.m2l_ops.assign((size_t)fp.n_m2l_ops * kp * kp, 0.0);
+    parallel_for(fp.n_m2l_ops, 1, [&](int64_t op) {
+      const double d[3] = {(double)key[op][0], (double)key[op][1], (double)key[op][2]};
+      op_m2l(p, d, &fp.m2l_ops[(size_t)op * kp * kp]);
+    });
   }
   fp.m2l.level_off[L + 1] = (int)fp.m2l.op.size();
   fp.m2l.level_off1[L + 1] = (int)fp.m2l.op1.size();
@@ -774,36 +852,50 @@ void add_mirror_cells(HostMesh &m, double z0) {
 void build_p2m_plan(const HostMesh &m, const QuadData &qd, const Tree &t, FarPlan &fp) {
   const int dpc = m.dpc, nq = qd.nq;
   const int nleaf = (int)fp.p2m_leaf.size();
-  fp.p2m_col_beg.assign(nleaf + 1, 0);
-  fp.p2m_col_dof.clear();
-  fp.p2m_qpt.clear();
-  fp.p2m_qleaf.clear();
-  fp.p2m_qcol.clear();
-  std::vector<int32_t> cols;
-  for (int li = 0; li < nleaf; ++li) {
+  // per leaf (host threads): its sorted DoF columns; then the offsets; then
+  // every quad point's entries at their final places (the same layout as a
+  // sequential pass over the leaves)
+  std::vector<std::vector<int32_t>> cols(nleaf);
+  parallel_for(nleaf, 64, [&](int64_t li) {
     const int b = fp.p2m_leaf[li];
     const int beg = t.quad_beg[b], cnt = t.quad_cnt[b];
-    cols.clear();
+    std::vector<int32_t> &c = cols[li];
+    c.reserve((size_t)cnt * dpc);
     for (int i = 0; i < cnt; ++i) {
       const int64_t cell = t.qperm[beg + i] / nq;
-      for (int j = 0; j < dpc; ++j) cols.push_back(m.cdofs[cell * dpc + j]);
+      for (int j = 0; j < dpc; ++j) c.push_back(m.cdofs[cell * dpc + j]);
     }
-    std::sort(cols.begin(), cols.end());
-    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
-    const int c0 = (int)fp.p2m_col_dof.size();
-    fp.p2m_col_dof.insert(fp.p2m_col_dof.end(), cols.begin(), cols.end());
-    fp.p2m_col_beg[li + 1] = (int)fp.p2m_col_dof.size();
+    std::sort(c.begin(), c.end());
+    c.erase(std::unique(c.begin(), c.end()), c.end());
+  });
+  fp.p2m_col_beg.assign(nleaf + 1, 0);
+  std::vector<int64_t> qbeg(nleaf + 1, 0);
+  for (int li = 0; li < nleaf; ++li) {
+    fp.p2m_col_beg[li + 1] = fp.p2m_col_beg[li] + (int)cols[li].size();
+    qbeg[li + 1] = qbeg[li] + t.quad_cnt[fp.p2m_leaf[li]];
+  }
+  fp.p2m_col_dof.assign(fp.p2m_col_beg[nleaf], 0);
+  fp.p2m_qpt.assign(qbeg[nleaf], 0);
+  fp.p2m_qleaf.assign(qbeg[nleaf], 0);
+  fp.p2m_qcol.assign((size_t)qbeg[nleaf] * dpc, 0);
+  parallel_for(nleaf, 64, [&](int64_t li) {
+    const int b = fp.p2m_leaf[li];
+    const int beg = t.quad_beg[b], cnt = t.quad_cnt[b];
+    const std::vector<int32_t> &c = cols[li];
+    const int c0 = fp.p2m_col_beg[li];
+    std::copy(c.begin(), c.end(), fp.p2m_col_dof.begin() + c0);
     for (int i = 0; i < cnt; ++i) {
+      const int64_t q = qbeg[li] + i;
       const int g = t.qperm[beg + i];
       const int64_t cell = g / nq;
-      fp.p2m_qpt.push_back(g);
-      fp.p2m_qleaf.push_back(li);
+      fp.p2m_qpt[q] = g;
+      fp.p2m_qleaf[q] = (int32_t)li;
       for (int j = 0; j < dpc; ++j) {
         const int d = m.cdofs[cell * dpc + j];
-        fp.p2m_qcol.push_back(c0 + (int)(std::lower_bound(cols.begin(), cols.end(), d) - cols.begin()));
+        fp.p2m_qcol[q * dpc + j] = c0 + (int)(std::lower_bound(c.begin(), c.end(), d) - c.begin());
       }
     }
-  }
+  });
 }
 
 }  // namespace pbfmm

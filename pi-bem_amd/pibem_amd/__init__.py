@@ -225,10 +225,11 @@ class BEMFMA:
         self._chk(self._L.pbfmm_set_phase_timing(self._h, 1 if on else 0))
 
     def last_timings(self):
-        """Phase times (ms) of the last matvec: spmv, p2m, m2m, l2l, m2l, m2p, l2p, total."""
-        ms = np.zeros(8)
+        """Phase times (ms) of the last matvec: spmv, p2m, m2m, l2l, m2l, m2p, l2p, total, and
+        m2l_sum, the per-target sum of the slot M2L (0 when M2L adds into its targets)."""
+        ms = np.zeros(9)
         self._chk(self._L.pbfmm_last_timings(self._h, _p(ms)))
-        return dict(zip(("spmv", "p2m", "m2m", "l2l", "m2l", "m2p", "l2p", "total"), ms.tolist()))
+        return dict(zip(("spmv", "p2m", "m2m", "l2l", "m2l", "m2p", "l2p", "total", "m2l_sum"), ms.tolist()))
 
     def work_counts(self):
         c = np.zeros(16, np.int64)
@@ -266,11 +267,46 @@ class BEMProblem:
         self.fma._chk(self.fma._L.pbfmm_set_constraints(self._h, _p(a, _i64), _p(b, _i64), _p(c), _p(d),
                                                         _p(e, _u8)))
 
+    def compute_constraints(self, tmp_rhs, continuous_gradient=True):
+        """BEMProblem::compute_constraints (bem_problem.cc:1376-1653): the
+        double-node rows from the projected normals and surface gradients of
+        tmp_rhs, installed on the handle; returns them as
+        (ptr, idx, val, inhom, is_constrained)."""
+        self.fma._chk(self.fma._L.pbfmm_compute_constraints(self._h, _p(_f64(tmp_rhs)), 1 if continuous_gradient else 0))
+        return self.constraints()
+
+    def constraints(self):
+        """The installed constraint rows (pbfmm_get_constraints)."""
+        n = self.fma.n
+        nnz = C.c_int64(0)
+        self.fma._chk(self.fma._L.pbfmm_get_constraints(self._h, C.byref(nnz), None, None, None, None, None))
+        ptr, idx, val = np.zeros(n + 1, np.int64), np.zeros(nnz.value, np.int64), np.zeros(nnz.value)
+        inh, isc = np.zeros(n), np.zeros(n, np.uint8)
+        self.fma._chk(self.fma._L.pbfmm_get_constraints(self._h, C.byref(nnz), _p(ptr, _i64), _p(idx, _i64), _p(val),
+                                                        _p(inh), _p(isc, _u8)))
+        return ptr, idx, val, inh, isc
+
     def fma_preconditioner(self, alpha=None):
         """BEMFMA::FMA_preconditioner (bem_fma.cc:3078-3411) as block Jacobi over
         the rows of the leaves that share a parent block."""
         a = None if alpha is None else _f64(alpha)
         self.fma._chk(self.fma._L.pbfmm_fma_preconditioner(self._h, _p(a)))
+
+    def preconditioner_blocks(self):
+        """The block-Jacobi preconditioner before inversion
+        (pbfmm_preconditioner_blocks): a list of (rows, R x R matrix)."""
+        sz = np.zeros(3, np.int64)
+        self.fma._chk(self.fma._L.pbfmm_preconditioner_blocks(self._h, _p(sz, _i64), None, None, None))
+        nb, nr, nv = (int(x) for x in sz)
+        rb, rows, vals = np.zeros(nb + 1, np.int32), np.zeros(nr, np.int32), np.zeros(nv)
+        self.fma._chk(self.fma._L.pbfmm_preconditioner_blocks(self._h, _p(sz, _i64), _p(rb, _i32), _p(rows, _i32),
+                                                              _p(vals)))
+        out, off = [], 0
+        for b in range(nb):
+            r = rows[rb[b]:rb[b + 1]]
+            out.append((r, vals[off:off + len(r) ** 2].reshape(len(r), len(r))))
+            off += len(r) ** 2
+        return out
 
     def vmult(self, src):
         dst = np.zeros(self.fma.n)

@@ -47,6 +47,8 @@ SIGNATURES = {
     "pbfmm_compute_alpha": (C.c_int, [_h, _d]),
     "pbfmm_set_constraints": (C.c_int, [_h, _i64, _i64, _d, _d, _u8]),
     "pbfmm_fma_preconditioner": (C.c_int, [_h, _d]),
+    "pbfmm_compute_constraints": (C.c_int, [_h, _d, C.c_int32]),
+    "pbfmm_get_constraints": (C.c_int, [_h, _i64, _i64, _i64, _d, _d, _u8]),
     "pbfmm_vmult": (C.c_int, [_h, _d, _d]),
     "pbfmm_compute_rhs": (C.c_int, [_h, _d, _d]),
     "pbfmm_solve_system": (C.c_int, [_h, _d, _d, _d, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
@@ -67,6 +69,7 @@ SIGNATURES = {
     "pbfmm_last_timings": (C.c_int, [_h, _d]),
     "pbfmm_work_counts": (C.c_int, [_h, _i64]),
     "pbfmm_set_mixed_scheme": (C.c_int, [_h, C.c_int32]),
+    "pbfmm_preconditioner_blocks": (C.c_int, [_h, _i64, _i32, _i32, _d]),
     "pbfmm_set_spmv_kernel": (C.c_int, [_h, C.c_int32, C.c_int32]),
     "pbfmm_tuning_state": (C.c_int, [_h, _i32]),
     "pbfmm_set_side_fusion": (C.c_int, [_h, C.c_int32, C.c_int32]),

@@ -212,6 +212,17 @@ def double_node_cube(n_cells_per_edge: int, half: float = 1.0) -> Mesh:
     return m
 
 
+def linear_map(mesh: Mesh, A) -> Mesh:
+    """The mesh mapped by the linear map x -> A x: flat faces stay flat, the
+    normals become A^-T n, normalised; the DoFs and double sets are kept."""
+    A = np.asarray(A, dtype=np.float64)
+    nrm = mesh.normals @ np.linalg.inv(A)
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    m = Mesh(mesh.support_xyz @ A.T, mesh.cell_dofs, mesh.dpc, nrm, mesh.cell_face, mesh.dbl_ptr, mesh.dbl_idx,
+             name=f"{mesh.name}-mapped", line_ids=mesh.line_ids)
+    return m
+
+
 def wigley_hull(nx: int = 400, nz: int = 100, L: float = 1.0, B: float = 0.1, T: float = 0.0625,
                 z_top: float = None) -> Mesh:
     """Closed Wigley-type double body y = +-(B/2)(1 - (2x/L)^2)(1 - (z/T)^2),

@@ -521,3 +521,46 @@ def test_operator_extreme_orders(oracle_mod, p):
         print(f"\n  p={p} N={n} precision={prec}: {e:.2e}", end="")
         assert e <= bar
     f.close()
+
+
+@pytest.mark.parametrize("which", ["c1", "dcube"])
+def test_preconditioner_blocks_vs_oracle(which, request, oracle_mod):
+    """The block-Jacobi preconditioner before inversion against the oracle's
+    restatement of FMA_preconditioner's matrix (init_preconditioner,
+    bem_fma.cc:919-975: -D in Dirichlet and N in Neumann columns; alpha on
+    the diagonal of unconstrained Neumann rows; constrained rows replaced by
+    their constraint, with the operator's sign -c_ij, bem_fma.cc:3194-3206 and
+    constrained_matrix.h:109-121): every entry within 1e-10 of the largest.
+    Every entry of a block (rows of sibling leaves) is in the near field."""
+    c = request.getfixturevalue(which)
+    pr = c.pr
+    bp = P.BEMProblem(c.f64)
+    cons = M.constraints(pr) if which == "dcube" else None
+    bp.set_constraints(cons)
+    bp.compute_alpha()
+    blocks = bp.preconditioner_blocks()
+    rp, col, vN, vD, vP = c.o.csr()
+    alpha = c.o.compute_alpha()
+    dirm = pr.dirichlet == 1
+    covered = np.zeros(pr.mesh.n_dofs, int)
+    scale = np.abs(vP).max()
+    worst = 0.0
+    for rows, A in blocks:
+        covered[rows] += 1
+        for r, i in enumerate(rows):
+            if cons is not None and cons[4][i]:
+                want = (rows == i).astype(float)
+                for k in range(cons[0][i], cons[0][i + 1]):
+                    want[rows == cons[1][k]] -= cons[2][k]
+            else:
+                cols, vals = col[rp[i]:rp[i + 1]], vP[rp[i]:rp[i + 1]]
+                pos = np.searchsorted(cols, rows)
+                assert np.all(pos < len(cols)) and np.array_equal(cols[pos], rows), "block entry outside the near field"
+                want = vals[pos].copy()
+                if not dirm[i]:
+                    want[r] += alpha[i]
+            worst = max(worst, np.abs(A[r] - want).max() / scale)
+    print(f"\n  {which}: {len(blocks)} blocks, max rel diff {worst:.2e}")
+    assert np.all(covered == 1)
+    assert worst <= 1e-10
+    bp.set_constraints(None)

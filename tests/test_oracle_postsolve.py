@@ -144,3 +144,66 @@ def test_q2_warped_patch_exact():
     assert np.abs(s - [a[0], a[1], 0.0]).max() < 1e-12
     # area of the warped square: int (1 + 0.2 v)(1) - (0.2 u)(0.2 u) du dv = 1.1 - 0.04/3
     assert abs(-ps.volume_integral(np.ones(m.n_cells))[2] / 0.3 - (1.1 - 0.04 / 3)) < 1e-13
+
+
+# -- double-node constraints (BEMProblem::compute_constraints) --------------
+SHEAR = np.array([[1.0, 0.3, 0.0], [0.0, 1.0, 0.25], [0.15, 0.0, 1.0]])
+
+
+def _rowdict(cons):
+    ptr, idx, val, inh, isc = cons
+    return {int(i): (sorted(zip(idx[ptr[i]:ptr[i + 1]].tolist(), val[ptr[i]:ptr[i + 1]].tolist())), inh[i])
+            for i in np.nonzero(isc)[0]}
+
+
+@pytest.mark.parametrize("shear", [False, True], ids=["cube", "sheared"])
+def test_constraints_linear_field_exact(shear):
+    """phi = a.x with Dirichlet data everywhere: the projections are exact on
+    flat faces, so both normal derivatives of every Dirichlet pair equal
+    a.n_i and a.n_it.  On the sheared cube n_i.n_it != 0, so a dropped
+    (s_i.n_it)(n_i.n_it) term, a swapped pair of indices or a wrong
+    denominator fails."""
+    m = M.linear_map(M.double_node_cube(4), SHEAR if shear else np.eye(3))
+    a = np.array([0.3, -0.5, 0.8])
+    dmask = np.ones(m.n_dofs, dtype=np.uint8)
+    got = _rowdict(PostSolve(m).compute_constraints(m.support_xyz @ a, dmask))
+    dsets = m.double_sets()
+    want = {i for i in range(m.n_dofs) if len(dsets[i]) > 1}
+    assert set(got) == want
+    nn = [m.normals[i] @ m.normals[j] for i in want for j in dsets[i] if j != i]
+    assert (max(np.abs(nn)) > 0.1) == shear
+    err = max(abs(inh - a @ m.normals[i]) for i, (ent, inh) in got.items())
+    assert all(not ent for ent, _ in got.values()) and err < 1e-12
+
+
+def test_constraints_mixed_rows():
+    """Dirichlet on the upper cells of the cube, seeded 8-source field: the
+    row pattern is fixed by the double sets and the Dirichlet mask alone
+    (equal to the rows of the exact-gradient constraints), Neumann partners
+    read (master, 1), Neumann partners of a Dirichlet master carry the
+    master's tmp_rhs, and the Dirichlet pairs whose cells hold Dirichlet data
+    only approach the exact normal derivatives as the mesh is refined."""
+    errs = []
+    for n in (4, 8):
+        pr = M.config_dcube(n, 4)
+        m = pr.mesh
+        got = _rowdict(PostSolve(m).compute_constraints(pr.tmp_rhs, pr.dirichlet))
+        want = _rowdict(M.constraints(pr))
+        assert set(got) == set(want)
+        dsets = m.double_sets()
+        clean = np.ones(m.n_dofs, bool)      # every cell around the DoF is all-Dirichlet
+        for c in m.cell_dofs:
+            if not pr.dirichlet[c].all():
+                clean[c] = False
+        dn = (pr.exact.grad(m.support_xyz) * m.normals).sum(1)
+        e = 0.0
+        for i, (ent, inh) in got.items():
+            assert [k for k, _ in ent] == [k for k, _ in want[i][0]]
+            assert all(v == 1.0 for _, v in ent)
+            master = min(k for k in dsets[i] if pr.dirichlet[k]) if pr.dirichlet[list(dsets[i])].any() else None
+            if pr.dirichlet[i] == 0 and master is not None:
+                assert inh == pr.tmp_rhs[master]
+            if pr.dirichlet[i] and all(clean[k] for k in dsets[i]):
+                e = max(e, abs(inh - dn[i]))
+        errs.append(e / np.abs(dn).max())
+    assert errs[1] < 0.6 * errs[0] and errs[1] < 0.2, errs
