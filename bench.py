@@ -52,13 +52,34 @@ PHASES = ("spmv", "p2m", "m2m", "l2l", "m2l", "m2p", "l2p", "m2l_sum")
 # kernel names as they appear in traces, per phase, operator precision and
 # number of expansions per block (2: N and D separately, 1: the operator's
 # combined N.phi - D.dphi)
+SPMV_KERNEL = {0: "k_spmv_lds<{t}, {k}>", 1: "k_spmv_wave<{t}, {k}, 2>", 2: "k_spmv_wave<{t}, {k}, 4>"}
 KERNEL = {
-    "spmv": "k_spmv<{t}, {k}>", "p2m": "k_p2m<{t}, {k}>", "m2m": "k_translate<{t}, {p}, 0, {k}>",
-    "m2l": "k_translate<{t}, {p}, 1, {k}>", "l2l": "k_translate<{t}, {p}, 2, {k}>", "m2p": "k_m2p<{t}, {k}>",
-    "l2p": "k_l2p<{t}, {k}>", "m2l_sum": "k_m2l_reduce<{t}>",
+    "p2m": "k_p2m<{t}, {k}>", "m2m": "k_translate<{t}, {p}, 0, {k}>", "m2l": "k_translate<{t}, {p}, 1, {k}>",
+    "l2l": "k_translate<{t}, {p}, 2, {k}>", "m2p": "k_m2p<{t}, {k}, {p}>", "m2l_sum": "k_m2l_reduce<{t}>",
 }
 # M2L into per-pair slots (the slot plans; its per-target sum is m2l_sum)
 KERNEL_M2L_SLOT = "k_translate<{t}, {p}, 3, {k}>"
+
+
+def kernel_name(phase, prec, p, nk, tuning):
+    """The trace name of the kernel that runs a phase under the tuned
+    choices (SpMV variant, M2L plan: bit 4 = slots, bit 1 = M2P/L2P side
+    fusion; the operator's L2P is fused with the post-processing)."""
+    t = "float" if prec == "fp32" else "double"
+    plan = int(tuning.get(f"side_fusion_{prec}", 0)) if nk == 1 else 0
+    if phase == "spmv":
+        fmt = (SPMV_KERNEL[int(tuning.get(f"spmv_kernel_{prec}", 0))] if nk == 1
+               else "k_spmv_lds/k_spmv_wave<{t}, {k}> (variant timed for the matvec)")
+    elif phase == "l2p":
+        fmt = "k_l2p_post<{t}, {p}>" if nk == 1 else "k_l2p<{t}, {k}, {p}>"
+    elif phase == "m2l" and plan & 4:
+        fmt = KERNEL_M2L_SLOT
+    else:
+        fmt = KERNEL[phase]
+    name = fmt.format(t=t, p=p, k=nk)
+    if plan & 1 and phase in ("m2l", "m2p", "l2p"):
+        name += " (side-fused plan: k_translate_side)"
+    return name
 
 
 def log(msg):
@@ -98,10 +119,8 @@ def work_model(wc, s_v, nk):
     return out
 
 
-def roofline_of(phase, model, ms, prec, p, nk, slot=False):
-    t = "float" if prec == "fp32" else "double"
+def roofline_of(phase, model, ms, prec, name):
     m = model[phase]
-    name = (KERNEL_M2L_SLOT if (phase == "m2l" and slot) else KERNEL[phase]).format(t=t, p=p, k=nk)
     if ms <= 0:
         return dict(kernel=name, bound=m["bound"], achieved=0.0, peak=None, unit=None, frac=0.0,
                     algorithmic_per_launch=None, launch_ms=0.0)
@@ -342,10 +361,10 @@ def measure_config(config, steps, warmup):
     op_rate, op_phases = time_operator(f, bp, n, stream)
     mv, phases, rel = time_matvecs(f, n, stream)
     roof = {prec: {ph: roofline_of(ph, work_model(wc, 4 if prec == "fp32" else 8, 1), op_phases[prec][ph], prec,
-                                   pr.p, 1, slot=op_phases[prec]["m2l_sum"] > 0) for ph in PHASES}
+                                   kernel_name(ph, prec, pr.p, 1, tuning)) for ph in PHASES}
             for prec in op_phases}
     roof2 = {prec: {ph: roofline_of(ph, work_model(wc, 4 if prec == "fp32" else 8, 2), phases[prec][ph], prec,
-                                    pr.p, 2) for ph in PHASES} for prec in phases}
+                                    kernel_name(ph, prec, pr.p, 2, tuning)) for ph in PHASES} for prec in phases}
     return dict(pr=pr, f=f, stream=stream, wc=wc, setup=setup, solves=solves, tuning=tuning, operator=op_rate,
                 operator_phases=op_phases, matvec=mv, phases=phases, fp32_vs_fp64_matvec=rel, roofline=roof,
                 roofline_matvec=roof2)

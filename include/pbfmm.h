@@ -297,6 +297,16 @@ int pbfmm_preconditioner_blocks(pbfmm_handle *h, int64_t *sizes, int32_t *row_be
  * on FP64 residuals).  Either scheme ends on the true FP64 residual. */
 int pbfmm_set_mixed_scheme(pbfmm_handle *h, int32_t scheme);
 
+/* The two thresholds of the mixed-precision schemes (no counterpart in the
+ * reference): inner_reduction, a fixed residual reduction asked of each
+ * FP32-operator solve of the iterative refinement (an inner solve then stops
+ * at max(rel_tol ||b|| / 2, inner_reduction ||r||); by default the reduction
+ * still needed is split into the fewest equal stages of at most 1e-5), and
+ * relax_switch, the relaxed scheme's switch to the FP32 operator once the
+ * residual is below relax_switch * rel_tol * ||b|| (default 2e5).  A value
+ * <= 0 restores the default.  For tuning. */
+int pbfmm_set_mixed_tolerances(pbfmm_handle *h, double inner_reduction, double relax_switch);
+
 /* Overrides the near-field SpMV kernel that set-up's autotuner chose for one
  * precision (PBFMM_FP64 or PBFMM_MIXED for the FP32 values), for both
  * storages: 0 workgroup per leaf, 1 wavefront per leaf with 2 rows in flight,

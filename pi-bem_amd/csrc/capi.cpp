@@ -1016,6 +1016,15 @@ int pbfmm_set_mixed_scheme(pbfmm_handle *h, int32_t scheme) {
   return PBFMM_OK;
 }
 
+int pbfmm_set_mixed_tolerances(pbfmm_handle *h, double inner_reduction, double relax_switch) {
+  if (!h) return PBFMM_ERR_ARG;
+  if (!(inner_reduction < 1.0) || std::isnan(relax_switch))
+    return fail(h, PBFMM_ERR_ARG, "set_mixed_tolerances: inner_reduction must be below 1");
+  h->eng.inner_reduction = inner_reduction > 0 ? inner_reduction : 0.0;
+  h->eng.relax_switch = relax_switch > 0 ? relax_switch : 0.0;
+  return PBFMM_OK;
+}
+
 int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel) {
   return guarded(h, [&]() -> int {
     if (int rc = need(h, 3, "set_spmv_kernel")) return rc;

@@ -209,6 +209,10 @@ struct Engine {
   // one, else iterative refinement; the cost ratio is timed once
   double op_cost_ratio = 0;
   int mixed_scheme = 0;  // 0 chosen by the timed cost ratio, 1 relaxed, 2 refinement
+  double inner_reduction = 0;  // refinement's inner residual reduction (0: the default)
+  double relax_switch = 0;     // relaxed scheme's FP32 switch over tol ||b|| (0: the default)
+  double inner_target(double tol, double bn, double rn) const;
+  double relax_sw() const;
   DevBuf d_cost;
   bool mixed_relaxed();
   double norm2(const double *v);

@@ -110,6 +110,11 @@ class BEMFMA:
         space), 2 iterative refinement (pbfmm_set_mixed_scheme)."""
         self._chk(self._L.pbfmm_set_mixed_scheme(self._h, int(scheme)))
 
+    def set_mixed_tolerances(self, inner_reduction=0.0, relax_switch=0.0):
+        """Refinement's inner residual reduction and the relaxed scheme's
+        FP32 switch (pbfmm_set_mixed_tolerances; 0 = the defaults)."""
+        self._chk(self._L.pbfmm_set_mixed_tolerances(self._h, float(inner_reduction), float(relax_switch)))
+
     def set_spmv_kernel(self, precision, kernel):
         """Overrides the autotuned near-field SpMV kernel for one precision: 0
         workgroup per leaf, 1 / 2 wavefront per leaf with 2 / 4 rows in
