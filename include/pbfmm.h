@@ -310,7 +310,8 @@ int pbfmm_set_mixed_tolerances(pbfmm_handle *h, double inner_reduction, double r
 /* Overrides the near-field SpMV kernel that set-up's autotuner chose for one
  * precision (PBFMM_FP64 or PBFMM_MIXED for the FP32 values), for both
  * storages: 0 workgroup per leaf, 1 wavefront per leaf with 2 rows in flight,
- * 2 with 4 rows.  For tests and tuning; call after
+ * 2 with 4 rows, 3 with 8 rows summed by one exchange tree.  For tests and
+ * tuning; call after
  * pbfmm_direct_integrals.  No counterpart in the reference (Trilinos
  * chooses its own SpMV). */
 int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel);

@@ -1028,7 +1028,7 @@ int pbfmm_set_mixed_tolerances(pbfmm_handle *h, double inner_reduction, double r
 int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel) {
   return guarded(h, [&]() -> int {
     if (int rc = need(h, 3, "set_spmv_kernel")) return rc;
-    if (kernel < 0 || kernel > 2) return fail(h, PBFMM_ERR_ARG, "set_spmv_kernel: kernel must be 0, 1 or 2");
+    if (kernel < 0 || kernel > 3) return fail(h, PBFMM_ERR_ARG, "set_spmv_kernel: kernel must be 0, 1, 2 or 3");
     if (precision != PBFMM_FP64 && precision != PBFMM_MIXED)
       return fail(h, PBFMM_ERR_ARG, "set_spmv_kernel: unknown precision");
     if (precision == PBFMM_MIXED && !h->eng.have_f32)

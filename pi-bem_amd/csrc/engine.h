@@ -176,7 +176,9 @@ struct Engine {
   }
   hipStream_t side_stream = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  DevBuf d_red_tgt, d_red_beg, d_red_pair;  // M2L pairs per target (ascending)
+  // slot M2L: targets, their slot ranges (CSR) and each pair's slot, the
+  // slots of one target consecutive (its pairs in ascending order)
+  DevBuf d_red_tgt, d_red_beg, d_slot_pos;
   int n_red_tgt = 0;
   template <typename T>
   T *m2l_slots();

@@ -423,6 +423,90 @@ __device__ __forceinline__ void spmv_wave_leaf(int leaf, int lane, const NearArg
   }
 }
 
+// Eight row partials per lane (acc[q]: row q) to one row sum per 8-lane
+// group: three exchange stages halve the partials a lane keeps (lane bit 5
+// picks rows 4..7 or 0..3, bit 4 the upper or lower pair, bit 3 one row), so
+// lane group g = lane >> 3 ends with row g's partials of the wave, and three
+// more stages sum the group.  Ten shuffles in all, against eight full
+// wavefront sums row by row.
+template <typename T>
+__device__ __forceinline__ T rows8_sum(T (&acc)[8], int lane) {
+  {
+    const bool b = (lane >> 5) & 1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const T mine = b ? acc[j + 4] : acc[j], send = b ? acc[j] : acc[j + 4];
+      acc[j] = mine + __shfl_xor(send, 32, 64);
+    }
+  }
+  {
+    const bool b = (lane >> 4) & 1;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const T mine = b ? acc[j + 2] : acc[j], send = b ? acc[j] : acc[j + 2];
+      acc[j] = mine + __shfl_xor(send, 16, 64);
+    }
+  }
+  const bool b = (lane >> 3) & 1;
+  const T mine = b ? acc[1] : acc[0], send = b ? acc[0] : acc[1];
+  T s = mine + __shfl_xor(send, 8, 64);
+  s += __shfl_xor(s, 4, 64);
+  s += __shfl_xor(s, 2, 64);
+  s += __shfl_xor(s, 1, 64);
+  return s;
+}
+
+// One leaf by one wavefront, eight rows at a time: every lane keeps the
+// inputs of its columns (c0 + lane + 64 j, the column tiles of
+// spmv_wave_leaf) and accumulates eight row partials over only the column
+// vectors the leaf fills; rows8_sum then leaves row r + (lane >> 3) with the
+// lanes of group lane >> 3, and lane 8 g stores it.
+template <typename T, int NK>
+__device__ __forceinline__ void spmv_wave8_leaf(int leaf, int lane, const NearArgs<T> &a) {
+  using E = typename NearEntry<T, NK>::type;
+  constexpr int CPL = 64 / (int)sizeof(E);
+  constexpr int TILE = 64 * CPL;
+  const int r0 = a.row_beg[leaf], R = a.row_beg[leaf + 1] - r0;
+  const int cb = a.col_beg[leaf], C = a.col_beg[leaf + 1] - cb;
+  const E *V = reinterpret_cast<const E *>(a.vals) + a.val_beg[leaf];
+  const E zero{};
+  for (int c0 = 0; c0 < C; c0 += TILE) {
+    const int J = min(CPL, (C - c0 + 63) / 64);  // column vectors of this tile (uniform)
+    E x[CPL];
+#pragma unroll
+    for (int j = 0; j < CPL; ++j) {
+      const int c = c0 + lane + 64 * j;
+      x[j] = (j < J && c < C) ? near_x<T, NK>(a.phi, a.dphi, a.cols[cb + c]) : zero;
+    }
+    for (int r = 0; r < R; r += 8) {
+      T aN[8], aD[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        aN[q] = 0;
+        aD[q] = 0;
+        if (r + q < R) {
+          const E *vq = V + (int64_t)(r + q) * C + c0 + lane;
+#pragma unroll
+          for (int j = 0; j < CPL; ++j)
+            if (j < J && c0 + lane + 64 * j < C) near_fma<T, NK>(aN[q], aD[q], vq[64 * j], x[j]);
+        }
+      }
+      const T sN = rows8_sum(aN, lane);
+      T sD = 0;
+      if constexpr (NK == 2) sD = rows8_sum(aD, lane);
+      const int row = r + (lane >> 3);
+      if ((lane & 7) == 0 && row < R) near_store<NK>(a.outN, a.outD, a.rows[r0 + row], c0 == 0, (double)sN, (double)sD);
+    }
+  }
+}
+
+template <typename T, int NK>
+__global__ void __launch_bounds__(256) k_spmv_wave8(int nleaf, NearArgs<T> a) {
+  const int leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (leaf >= nleaf) return;
+  spmv_wave8_leaf<T, NK>(leaf, threadIdx.x & 63, near_rhs(a));
+}
+
 template <typename T, int NK, int RPI>
 __global__ void __launch_bounds__(256) k_spmv_wave(int nleaf, NearArgs<T> a) {
   const int leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -603,6 +687,17 @@ __device__ __forceinline__ void axpy16(double2 &a, double x, const double2 &u) {
   a.x += x * u.x; a.y += x * u.y;
 }
 __device__ __forceinline__ void add16(float4 &a, const float4 &u) { a.x += u.x; a.y += u.y; a.z += u.z; a.w += u.w; }
+// lanes 0..31 add the values of lanes 32..63
+__device__ __forceinline__ void add_upper_half(float4 &a) {
+  a.x += __shfl_down(a.x, 32);
+  a.y += __shfl_down(a.y, 32);
+  a.z += __shfl_down(a.z, 32);
+  a.w += __shfl_down(a.w, 32);
+}
+__device__ __forceinline__ void add_upper_half(double2 &a) {
+  a.x += __shfl_down(a.x, 32);
+  a.y += __shfl_down(a.y, 32);
+}
 __device__ __forceinline__ void add16(double2 &a, const double2 &u) { a.x += u.x; a.y += u.y; }
 
 // NK = 2: U is [column][Ker1 kp][Ker2 kp] and the leaf gets both kernels'
@@ -893,13 +988,14 @@ __device__ __forceinline__ void translate_chunk(int c, T *__restrict__ sB, int *
   }
   __syncthreads();
   if constexpr (KIND == 3) {
-    // M2L into per-pair slots: pair beg + cc at OUT + (beg + cc) * KT2, the
-    // chunk's results one contiguous, coalesced run of plain stores
+    // M2L into per-pair slots: pair beg + cc into its slot sT[cc] (tgt holds
+    // the slot positions here) at OUT + sT[cc] * KP, whole: the padding rows
+    // are zero, so every slot is whole, aligned cache lines, written by
+    // consecutive lanes
     static_assert(NK == 1, "slot M2L is the single-expansion operator's");
-    T *o = OUT + (int64_t)beg * KT2;
-    for (int i = t; i < cnt * KT2; i += NT) {
-      const int cc = i / KT2, row = i - cc * KT2;
-      o[i] = sC[cc * CS + row];
+    for (int i = t; i < cnt * KP; i += NT) {
+      const int cc = i / KP, row = i - cc * KP;
+      OUT[(int64_t)sT[cc] * KP + row] = sC[cc * CS + row];
     }
   } else {
     // coalesced accumulation: consecutive lanes add consecutive rows of one
@@ -912,42 +1008,41 @@ __device__ __forceinline__ void translate_chunk(int c, T *__restrict__ sB, int *
   }
 }
 
-// The slot M2L's sums (KIND 3): one wavefront per target adds the slots of
-// its pairs (ascending pair index, a fixed order) and stores the target's
-// local expansion, which nothing else has written since it was cleared.
-// Lanes take two consecutive coefficients each (2 Kt <= 132 at p <= 10).
-template <typename T>
+// The slot M2L's sums (KIND 3): one wavefront per target adds its slots,
+// consecutive and kp values apart (aligned, whole cache lines), and stores
+// the target's local expansion, which nothing else has written since it was
+// cleared.  A lane loads 16 bytes of a slot, so a slot takes kp / V lanes (V
+// values per 16 bytes): SP = 2 slots at a time in the two half-waves when
+// that fits (FP32, kp <= 128), else one.  Each half adds every SP-th slot
+// from its first, eight loads in flight; the halves meet at the end.  The
+// order of the additions is fixed, so the sums do not vary from run to run.
+template <typename T, int SP>
 __global__ void __launch_bounds__(256) k_m2l_reduce(int ntgt, const int *__restrict__ tgt,
-                                                    const int *__restrict__ beg, const int *__restrict__ pair,
-                                                    int kt2, int kp, const T *__restrict__ slots, T *__restrict__ LE) {
+                                                    const int *__restrict__ beg, int kp,
+                                                    const T *__restrict__ slots, T *__restrict__ LE) {
+  using TV = typename Vec16<T>::type;
+  constexpr int V = 16 / sizeof(T), HS = 64 / SP, U = 8;
   const int w = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
-  if (w >= ntgt) return;
-  const int k0 = beg[w], k1 = beg[w + 1], row = 2 * l;
-  if (row >= kt2) return;
-  T a0 = 0, a1 = 0;
-  int k = k0;
-  for (; k + 4 <= k1; k += 4) {
-    T v[4][2];
+  if (w >= ntgt) return;  // uniform per wavefront
+  const int h = l / HS, li = l - h * HS;
+  const bool act = li * V < kp;
+  const int k1 = beg[w + 1];
+  const T *s = slots + li * V;
+  TV a = {};
+  int k = beg[w] + h;
+  for (; k + (U - 1) * SP < k1; k += U * SP) {
+    TV v[U];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const T *sq = slots + (int64_t)pair[k + q] * kt2 + row;
-      v[q][0] = sq[0];
-      v[q][1] = sq[1];
-    }
+    for (int q = 0; q < U; ++q)
+      if (act) v[q] = *reinterpret_cast<const TV *>(s + (int64_t)(k + q * SP) * kp);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      a0 += v[q][0];
-      a1 += v[q][1];
-    }
+    for (int q = 0; q < U; ++q)
+      if (act) add16(a, v[q]);
   }
-  for (; k < k1; ++k) {
-    const T *sq = slots + (int64_t)pair[k] * kt2 + row;
-    a0 += sq[0];
-    a1 += sq[1];
-  }
-  T *o = LE + (int64_t)tgt[w] * kp + row;
-  o[0] = a0;
-  o[1] = a1;
+  for (; k < k1; k += SP)
+    if (act) add16(a, *reinterpret_cast<const TV *>(s + (int64_t)k * kp));
+  if constexpr (SP == 2) add_upper_half(a);
+  if (h == 0 && act) *reinterpret_cast<TV *>(LE + (int64_t)tgt[w] * kp + li * V) = a;
 }
 
 // Chunks of one class are consecutive in a chunk list and share their
@@ -1932,11 +2027,14 @@ FarDev<float> &far_of<float>(Engine &e) { return e.f32; }
 // (chunks of 32 pairs for NK = 2, of 64 pairs for NK = 1).
 template <typename T, int KIND, int NK>
 static void launch_translate(Engine &e, hipStream_t st, const ChunkDev &cl, int c0, int c1, const T *ops,
-                             const T *IN, T *OUT) {
+                             const T *IN, T *OUT, const int *slot_pos = nullptr) {
   const int n = c1 - c0;
   if (n <= 0) return;
   const DevBuf &op = NK == 2 ? cl.op : cl.op1, &beg = NK == 2 ? cl.beg : cl.beg1, &cnt = NK == 2 ? cl.cnt : cl.cnt1;
-  const TrArgs<T> tr{op.as<int>() + c0, beg.as<int>() + c0, cnt.as<int>() + c0, cl.tgt.as<int>(), cl.src.as<int>(),
+  // the slot M2L (KIND 3) reads each pair's slot where the others read its target
+  const int *tgt = KIND == 3 ? slot_pos : cl.tgt.as<int>();
+  if (!tgt) throw std::runtime_error("slot M2L without slot positions");
+  const TrArgs<T> tr{op.as<int>() + c0, beg.as<int>() + c0, cnt.as<int>() + c0, tgt, cl.src.as<int>(),
                      ops, IN, OUT};
 #define PBFMM_TR(PP)                                                                                         \
   case PP:                                                                                                   \
@@ -2007,8 +2105,10 @@ void Engine::launch_spmv(int variant, const double *phi, const double *dphi, dou
     hipLaunchKernelGGL((k_spmv_lds<T, NK>), dim3(per_xcd * kXcd, R), dim3(256), 0, st, near.nleaf, per_xcd, a);
   } else if (variant == 1) {
     hipLaunchKernelGGL((k_spmv_wave<T, NK, 2>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, st, near.nleaf, a);
-  } else {
+  } else if (variant == 2) {
     hipLaunchKernelGGL((k_spmv_wave<T, NK, 4>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, st, near.nleaf, a);
+  } else {
+    hipLaunchKernelGGL((k_spmv_wave8<T, NK>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, st, near.nleaf, a);
   }
 }
 
@@ -2023,7 +2123,7 @@ void Engine::tune_spmv() {
   HIPCHK(hipEventCreate(&e1));
   float best = 0;
   int pick = 0;
-  for (int v = 0; v < 3; ++v) {
+  for (int v = 0; v < 4; ++v) {
     for (int rep = 0; rep < 2; ++rep)
       launch_spmv<T, NK>(v, d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
     HIPCHK(hipEventRecord(e0, stream));
@@ -2203,14 +2303,14 @@ T *Engine::m2l_slots() {
         tg.push_back(b);
         beg.push_back(beg.back() + cnt[b]);
       }
-    std::vector<int32_t> pair(npair), fill(beg.begin(), beg.end() - 1);
-    for (int64_t i = 0; i < npair; ++i) pair[fill[pos[far.m2l.tgt[i]]]++] = (int32_t)i;
+    std::vector<int32_t> slot(npair), fill(beg.begin(), beg.end() - 1);
+    for (int64_t i = 0; i < npair; ++i) slot[i] = fill[pos[far.m2l.tgt[i]]]++;
     up(d_red_tgt, tg);
     up(d_red_beg, beg);
-    up(d_red_pair, pair);
+    up(d_slot_pos, slot);
     n_red_tgt = (int)tg.size();
   }
-  const size_t bytes = (size_t)std::max<int64_t>(npair, 1) * 2 * kt * sizeof(T);
+  const size_t bytes = (size_t)std::max<int64_t>(npair, 1) * kp * sizeof(T);
   if (F.slots.bytes < bytes) alloc(F.slots, bytes);
   return F.slots.template as<T>();
 }
@@ -2218,13 +2318,18 @@ T *Engine::m2l_slots() {
 template <typename T>
 void Engine::launch_m2l_slots(const T *ops, const T *IN, T *LE) {
   T *slots = m2l_slots<T>();
-  launch_translate<T, 3, 1>(*this, stream, m2l_cl, 0, nchunks<1>(far.m2l), ops, IN, slots);
+  launch_translate<T, 3, 1>(*this, stream, m2l_cl, 0, nchunks<1>(far.m2l), ops, IN, slots, d_slot_pos.as<int>());
   mark(11);
   slot_marked = time_phases;
-  if (n_red_tgt > 0)
-    hipLaunchKernelGGL(k_m2l_reduce<T>, dim3((unsigned)((n_red_tgt + 3) / 4)), dim3(256), 0, stream, n_red_tgt,
-                       d_red_tgt.as<int>(), d_red_beg.as<int>(), d_red_pair.as<int>(), 2 * kt, kp,
-                       (const T *)slots, LE);
+  if (n_red_tgt > 0) {
+    const dim3 grid((unsigned)((n_red_tgt + 3) / 4));
+    if (sizeof(T) == 4 && kp <= 128)
+      hipLaunchKernelGGL((k_m2l_reduce<T, 2>), grid, dim3(256), 0, stream, n_red_tgt, d_red_tgt.as<int>(),
+                         d_red_beg.as<int>(), kp, (const T *)slots, LE);
+    else
+      hipLaunchKernelGGL((k_m2l_reduce<T, 1>), grid, dim3(256), 0, stream, n_red_tgt, d_red_tgt.as<int>(),
+                         d_red_beg.as<int>(), kp, (const T *)slots, LE);
+  }
   HIPCHK(hipGetLastError());
 }
 

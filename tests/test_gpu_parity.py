@@ -403,7 +403,7 @@ def test_fused_operator_pipeline(name):
     dst = hip.DeviceArray(n)
     for prec, bar in ((P.PBFMM_FP64, 1e-14), (P.PBFMM_MIXED, 1e-6)):
         ref = None
-        for kernel in (0, 1, 2):
+        for kernel in (0, 1, 2, 3):
             f.set_spmv_kernel(prec, kernel)
             f.set_phase_timing(True)
             bp.vmult_dev(src.value, dst.value, prec)
@@ -425,6 +425,35 @@ def test_fused_operator_pipeline(name):
         hip.synchronize()
         assert rel(dst.numpy(), ref) <= bar
         assert f.tuning_state()["side_fusion_fp32" if prec == P.PBFMM_MIXED else "side_fusion_fp64"] in PLANS
+    f.close()
+
+
+@pytest.mark.parametrize("name", ["sphere_p6", "spheroid_q2"])
+def test_matvec_spmv_kernels(name):
+    """The two-output FMM matvec (N.phi and D.dphi apart) with each near-field
+    SpMV kernel: leaf per workgroup, leaf per wavefront with 2, 4 or 8 rows in
+    flight (the last summing eight rows in one exchange tree), and leaves of
+    up to 40 rows (several row groups): FP64 within 1e-14 of kernel 0, FP32
+    within 1e-6."""
+    from pibem_amd import hip
+    pr = {"sphere_p6": lambda: M.config_sphere(24, 6), "spheroid_q2": lambda: M.config_spheroid(8, 6)}[name]()
+    pr.max_nodes_per_leaf = 40
+    f = P.setup(pr, P.PBFMM_MIXED)
+    n = pr.mesh.n_dofs
+    rng = np.random.default_rng(5)
+    a, b = hip.DeviceArray.from_numpy(rng.uniform(-1, 1, n)), hip.DeviceArray.from_numpy(rng.uniform(-1, 1, n))
+    oN, oD = hip.DeviceArray(n), hip.DeviceArray(n)
+    for prec, bar in ((P.PBFMM_FP64, 1e-14), (P.PBFMM_MIXED, 1e-6)):
+        ref = None
+        for kernel in (0, 1, 2, 3):
+            f.set_spmv_kernel(prec, kernel)
+            f.matvec_dev(a.value, b.value, oN.value, oD.value, prec)
+            hip.synchronize()
+            out = (oN.numpy(), oD.numpy())
+            ref = out if ref is None else ref
+            e = max(rel(out[0], ref[0]), rel(out[1], ref[1]))
+            print(f"\n  {name} N={n} precision={prec} kernel {kernel}: vs kernel 0 {e:.2e}", end="")
+            assert e <= bar
     f.close()
 
 
