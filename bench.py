@@ -358,6 +358,8 @@ def measure_config(config, steps, warmup):
     wc = f.work_counts()
     solves = time_solves(f, bp, pr, stream, steps, warmup)
     tuning = f.tuning_state()
+    g = f.graph_timings()
+    tuning["graph_step_ms"] = {"fp64_direct": g[0], "fp64_graph": g[1], "fp32_direct": g[2], "fp32_graph": g[3]}
     log("tuned choices: " + " ".join(f"{k}={v}" for k, v in tuning.items()))
     op_rate, op_phases = time_operator(f, bp, n, stream)
     mv, phases, rel = time_matvecs(f, n, stream)

@@ -336,8 +336,24 @@ int pbfmm_set_side_fusion(pbfmm_handle *h, int32_t precision, int32_t mode);
  *         and 12 plans 0 and 4 with the SpMV on a second stream
  * [4]     mixed-precision scheme: 1 relaxed, 2 iterative refinement
  * [5]     timed FP64 / FP32 operator cost ratio x 1000
- * No counterpart in the reference. */
+ * [6] [7] the GMRES step's operator, FP64 / FP32 steps: 1 replayed as a HIP
+ *         graph, 0 launched directly (as pbfmm_set_graph_mode)
+ * state holds 8 values.  No counterpart in the reference. */
 int pbfmm_tuning_state(pbfmm_handle *h, int32_t *state);
+
+/* HIP graphs for the GMRES step's operator (no counterpart in the
+ * reference): the preconditioner and the operator application of one Arnoldi
+ * step of pbfmm_gmres_dev / pbfmm_solve_system run from a fixed input into a
+ * fixed output, so the launches can be captured once and replayed.  -1
+ * (default): the first steps of each precision capture the graph and time
+ * three replays against three direct runs, and the faster is kept; 0 direct
+ * launches only; 1 graph replays.  A graph is captured again whenever a
+ * choice or buffer it holds changes.  Batch handles and many right-hand
+ * sides launch directly. */
+int pbfmm_set_graph_mode(pbfmm_handle *h, int32_t mode);
+/* The timing behind the graph choice, ms per step: [0] [1] FP64 direct /
+ * graph, [2] [3] FP32 direct / graph (0 until timed). */
+int pbfmm_graph_timings(pbfmm_handle *h, float *ms);
 
 /* ---- Post-solve (the step after the solve; real cells only) ----
  * Vector fields are component-major: v[d * n_dofs + i] is component d at DoF

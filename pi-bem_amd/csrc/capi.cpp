@@ -382,6 +382,7 @@ static int install_constraints(pbfmm_handle *h, const int64_t *c_ptr, const int6
   {
     Engine &e = h->eng;
     const int64_t n = e.n;
+    ++e.constraint_version;  // a captured operator holds the old rows
     std::vector<int32_t> ptr(n + 1), idx;
     std::vector<double> val, inh(n, 0.0);
     std::vector<uint8_t> isc(is_c, is_c + n);
@@ -1060,6 +1061,24 @@ int pbfmm_tuning_state(pbfmm_handle *h, int32_t *state) {
   state[4] = e.mixed_scheme != 0 ? e.mixed_scheme
                                  : (e.op_cost_ratio > 0 ? (e.op_cost_ratio < kRelaxCostRatio ? 1 : 2) : -1);
   state[5] = e.op_cost_ratio > 0 ? (int32_t)(e.op_cost_ratio * 1000.0 + 0.5) : -1;
+  state[6] = e.graph_enabled ? e.graph_use[0] : 0;
+  state[7] = e.graph_enabled ? e.graph_use[1] : 0;
+  return PBFMM_OK;
+}
+
+int pbfmm_set_graph_mode(pbfmm_handle *h, int32_t mode) {
+  if (!h) return PBFMM_ERR_ARG;
+  if (mode < -1 || mode > 1) return fail(h, PBFMM_ERR_ARG, "set_graph_mode: mode must be -1, 0 or 1");
+  Engine &e = h->eng;
+  e.graph_enabled = mode != 0;
+  e.graph_use[0] = e.graph_use[1] = mode;
+  return PBFMM_OK;
+}
+
+int pbfmm_graph_timings(pbfmm_handle *h, float *ms) {
+  if (!h || !ms) return PBFMM_ERR_ARG;
+  for (int f = 0; f < 2; ++f)
+    for (int k = 0; k < 2; ++k) ms[2 * f + k] = h->eng.graph_ms[f][k];
   return PBFMM_OK;
 }
 

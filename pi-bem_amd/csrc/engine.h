@@ -185,6 +185,22 @@ struct Engine {
   template <typename T>
   void launch_m2l_slots(const T *ops, const T *IN, T *LE);
   int side_fusion[2] = {-1, -1};
+  // One Arnoldi step's operator (preconditioner + vmult, step_op) captured
+  // into a HIP graph per step precision ([0] FP64, [1] FP32) on the fixed
+  // buffers d_vn (the step's input, the last basis vector) -> d_wk, used when
+  // replaying it beat direct launches (graph_use: -1 not timed, 0 direct,
+  // 1 graph), recaptured when a choice or buffer it holds changes.
+  int graph_use[2] = {-1, -1};
+  bool graph_enabled = true;  // pbfmm_set_graph_mode 0 turns graphs off
+  hipGraphExec_t op_graph[2] = {nullptr, nullptr};
+  std::vector<int64_t> op_graph_key[2];
+  DevBuf d_vn, d_wk;
+  float graph_ms[2][2] = {{0, 0}, {0, 0}};  // timed ms per step: direct, graph
+  int64_t constraint_version = 0;
+  void step_op(const double *vin, double *w, int precision);
+  void step_op_graph(int precision);
+  std::vector<int64_t> graph_key(int precision) const;
+  void drop_graphs();
   int use_side_fusion(bool f32);
   // near-field SpMV kernel per precision ([0] FP64, [1] FP32) and storage
   // (pairs, single values): 0 workgroup per leaf with an LDS copy of the

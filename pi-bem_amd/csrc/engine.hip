@@ -1577,6 +1577,28 @@ __global__ void __launch_bounds__(256) k_scale_fin(int64_t n, double *__restrict
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n && s > 0) v[i] /= s;
 }
+// k_scale_fin from w into two places: the new basis vector and the next
+// step's fixed input (the graph-replayed operator's)
+__global__ void __launch_bounds__(256) k_scale_fin2(int64_t n, const double *__restrict__ w, double *__restrict__ v,
+                                                    double *__restrict__ v2, const double *__restrict__ part, int P,
+                                                    double *__restrict__ nrm_out) {
+  __shared__ double s_nrm;
+  if (threadIdx.x < 64) {
+    const double q = wave_sum((int)threadIdx.x < P ? part[threadIdx.x] : 0.0);
+    if (threadIdx.x == 0) {
+      s_nrm = sqrt(q);
+      if (blockIdx.x == 0) *nrm_out = s_nrm;
+    }
+  }
+  __syncthreads();
+  const double s = s_nrm;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) {
+    const double x = s > 0 ? w[i] / s : w[i];
+    v[i] = x;
+    v2[i] = x;
+  }
+}
 
 // Forest segments: sum of squares of each part (one block per part) and a
 // per-part scaling y_i = f[part(i)] x_i.
@@ -1666,12 +1688,17 @@ __global__ void k_scale_inv_seg(int64_t n, const int *__restrict__ seg, const do
 // ===========================================================================
 static inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
+// Device allocations made so far by this library (a captured operator graph
+// holds the addresses it was captured with: one captured before the last
+// allocation is captured again).
+static int64_t g_alloc_epoch = 0;
 template <typename T>
 static void up(DevBuf &b, const std::vector<T> &v) {
   b.release();
   const size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
   HIPCHK(hipMalloc(&b.p, bytes));
   b.bytes = bytes;
+  ++g_alloc_epoch;
   if (!v.empty()) HIPCHK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
 }
 static void alloc(DevBuf &b, size_t bytes) {
@@ -1679,6 +1706,7 @@ static void alloc(DevBuf &b, size_t bytes) {
   bytes = std::max<size_t>(bytes, 16);
   HIPCHK(hipMalloc(&b.p, bytes));
   b.bytes = bytes;
+  ++g_alloc_epoch;
 }
 template <typename D, typename S>
 static std::vector<D> cast_vec(const std::vector<S> &v) {
@@ -1707,9 +1735,18 @@ Engine::~Engine() {
   if (h_ready) (void)hipEventDestroy(h_ready);
   if (h_pinned) (void)hipHostFree(h_pinned);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
+  drop_graphs();
   if (ev_fork) (void)hipEventDestroy(ev_fork);
   if (ev_join) (void)hipEventDestroy(ev_join);
   if (side_stream) (void)hipStreamDestroy(side_stream);
+}
+
+void Engine::drop_graphs() {
+  for (int f = 0; f < 2; ++f) {
+    if (op_graph[f]) (void)hipGraphExecDestroy(op_graph[f]);
+    op_graph[f] = nullptr;
+    op_graph_key[f].clear();
+  }
 }
 
 // Phase events on the handle's stream (no synchronisation between kernels):
@@ -2801,6 +2838,88 @@ void Engine::apply_prec(const double *v, double *z, int R) {
                      d_prow_beg.as<int>(), d_prows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z);
 }
 
+// One Arnoldi step's operator: w = A M^-1 vin (right preconditioning).
+void Engine::step_op(const double *vin, double *w, int precision) {
+  if (have_prec) {
+    apply_prec(vin, d_ptmp.as<double>());
+    vmult(d_ptmp.as<double>(), w, precision);
+  } else {
+    vmult(vin, w, precision);
+  }
+}
+
+// What a captured step holds: the step precision, the tuned choices its
+// launches follow, the operator's form and every device address (through the
+// allocation count).
+std::vector<int64_t> Engine::graph_key(int precision) const {
+  const int f = precision == PBFMM_MIXED && have_f32 ? 1 : 0;
+  return {precision, side_fusion[f], spmv_variant[f][0], have_prec ? 1 : 0, nprec, have_constraints ? 1 : 0,
+          constraint_version, neumann_mean ? 1 : 0, have_dirichlet ? 1 : 0, g_alloc_epoch};
+}
+
+// step_op(d_vn -> d_wk) as a graph replay when that was measured faster.
+// The first call per step precision runs it directly (the side-fusion and
+// SpMV choices time themselves there), the next captures it after one more
+// direct run (allocations on first use happen outside the capture) and times
+// three direct runs against three replays; from then on the faster one runs,
+// recaptured whenever graph_key changes.
+void Engine::step_op_graph(int precision) {
+  const int f = precision == PBFMM_MIXED && have_f32 ? 1 : 0;
+  double *vin = d_vn.as<double>(), *w = d_wk.as<double>();
+  if (!graph_enabled || graph_use[f] == 0 || side_fusion[f] < 0) {
+    step_op(vin, w, precision);
+    return;
+  }
+  if (!op_graph[f] || op_graph_key[f] != graph_key(precision)) {
+    if (op_graph[f]) (void)hipGraphExecDestroy(op_graph[f]);
+    op_graph[f] = nullptr;
+    step_op(vin, w, precision);  // allocations on first use, outside the capture
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      step_op(vin, w, precision);
+    } catch (...) {
+      (void)hipStreamEndCapture(stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(stream, &g));
+    const hipError_t ie = hipGraphInstantiate(&op_graph[f], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      op_graph[f] = nullptr;
+      throw HipError("operator graph instantiation failed");
+    }
+    op_graph_key[f] = graph_key(precision);
+  }
+  if (graph_use[f] < 0) {
+    hipEvent_t t0, t1, t2;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    HIPCHK(hipEventCreate(&t2));
+    HIPCHK(hipGraphLaunch(op_graph[f], stream));  // warm
+    HIPCHK(hipEventRecord(t0, stream));
+    for (int r = 0; r < 3; ++r) step_op(vin, w, precision);
+    HIPCHK(hipEventRecord(t1, stream));
+    for (int r = 0; r < 3; ++r) HIPCHK(hipGraphLaunch(op_graph[f], stream));
+    HIPCHK(hipEventRecord(t2, stream));
+    HIPCHK(hipEventSynchronize(t2));
+    float md = 0, mg = 0;
+    HIPCHK(hipEventElapsedTime(&md, t0, t1));
+    HIPCHK(hipEventElapsedTime(&mg, t1, t2));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipEventDestroy(t2);
+    graph_use[f] = mg < 0.98f * md ? 1 : 0;
+    graph_ms[f][0] = md / 3;
+    graph_ms[f][1] = mg / 3;
+  }
+  if (graph_use[f] == 1)
+    HIPCHK(hipGraphLaunch(op_graph[f], stream));
+  else
+    step_op(vin, w, precision);
+}
+
 // Restarted GMRES with classical Gram-Schmidt and one reorthogonalisation
 // (CGS2), FP64 basis and Hessenberg, operator precision as requested.  Solves
 // A x = b from the given x until the Arnoldi residual estimate reaches
@@ -2837,15 +2956,24 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
   // Krylov space; the cycle's starting residual always in FP64
   const bool relaxed = switch_abs > 0 && precision == PBFMM_MIXED;
   auto step_prec = [&]() { return relaxed ? (est <= switch_abs ? PBFMM_MIXED : PBFMM_FP64) : precision; };
+  // the step's operator from a fixed input (a copy of the last basis vector)
+  // into a fixed output, so that it can be replayed as a graph
+  // (Engine::step_op_graph); the fused vmult is the one a graph can hold
+  const bool graphs = e.graph_enabled && !e.time_phases && (int64_t)e.near.rows.size() == n;
+  if (graphs) {
+    if (e.d_vn.bytes < (size_t)n * 8) alloc(e.d_vn, (size_t)n * 8);
+    if (e.d_wk.bytes < (size_t)n * 8) alloc(e.d_wk, (size_t)n * 8);
+  }
   // Arnoldi step k: V_k+1 from V_k, its Hessenberg column into slot k
   auto enqueue_step = [&](int k) {
     double *vk1 = V + (int64_t)(k + 1) * n;
     const int pk = step_prec();
-    if (e.have_prec) {
-      e.apply_prec(V + (int64_t)k * n, e.d_ptmp.as<double>());
-      e.vmult(e.d_ptmp.as<double>(), vk1, pk);
+    double *wv = vk1;  // the new vector before its normalisation
+    if (graphs) {
+      e.step_op_graph(pk);  // d_vn (= V_k) -> d_wk
+      wv = e.d_wk.as<double>();
     } else {
-      e.vmult(V + (int64_t)k * n, vk1, pk);
+      e.step_op(V + (int64_t)k * n, vk1, pk);
     }
     ++it;
     // CGS2: two projections and the norm, all on the device, six launches:
@@ -2853,11 +2981,15 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
     double *slot = hd + (size_t)k * S;
     double *h1 = slot, *h2 = slot + (m + 2), *hn_d = slot + 2 * (m + 2);
     for (double *hj : {h1, h2}) {
-      hipLaunchKernelGGL(k_mdot_parts, dim3(kDotParts, k + 1), dim3(kRedThreads), 0, e.stream, n, n, V, vk1, cg);
-      hipLaunchKernelGGL(k_gemv_sub_fin, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, cg, kDotParts, hj, vk1);
+      hipLaunchKernelGGL(k_mdot_parts, dim3(kDotParts, k + 1), dim3(kRedThreads), 0, e.stream, n, n, V, wv, cg);
+      hipLaunchKernelGGL(k_gemv_sub_fin, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, cg, kDotParts, hj, wv);
     }
-    hipLaunchKernelGGL(k_mdot_parts, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, vk1, vk1, cg);
-    hipLaunchKernelGGL(k_scale_fin, dim3(g), dim3(256), 0, e.stream, n, vk1, cg, kNormParts, hn_d);
+    hipLaunchKernelGGL(k_mdot_parts, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, wv, wv, cg);
+    if (graphs)
+      hipLaunchKernelGGL(k_scale_fin2, dim3(g), dim3(256), 0, e.stream, n, wv, vk1, e.d_vn.as<double>(), cg,
+                         kNormParts, hn_d);
+    else
+      hipLaunchKernelGGL(k_scale_fin, dim3(g), dim3(256), 0, e.stream, n, vk1, cg, kNormParts, hn_d);
   };
   while (it < maxit) {
     // r = b - A x into V_0
@@ -2874,6 +3006,7 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
     est = beta;
     if (beta <= tol_abs) return 0;
     hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, rv, 1.0 / beta, V);
+    if (graphs) hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, rv, 1.0 / beta, e.d_vn.as<double>());
     std::fill(gv.begin(), gv.end(), 0.0);
     gv[0] = beta;
     // kq: steps enqueued; k_done: columns reduced on the host; the next host

@@ -132,11 +132,24 @@ class BEMFMA:
         """Choices the handle's timings made so far (-1 = not timed yet):
         SpMV kernels, side fusion modes, mixed scheme, FP64/FP32 operator
         cost ratio (pbfmm_tuning_state)."""
-        st = np.zeros(6, np.int32)
+        st = np.zeros(8, np.int32)
         self._chk(self._L.pbfmm_tuning_state(self._h, _p(st, _i32)))
         return dict(spmv_kernel_fp64=int(st[0]), spmv_kernel_fp32=int(st[1]), side_fusion_fp64=int(st[2]),
                     side_fusion_fp32=int(st[3]), mixed_scheme=int(st[4]),
-                    op_cost_ratio=(st[5] / 1000.0 if st[5] >= 0 else None))
+                    op_cost_ratio=(st[5] / 1000.0 if st[5] >= 0 else None), graph_fp64=int(st[6]),
+                    graph_fp32=int(st[7]))
+
+    def set_graph_mode(self, mode=-1):
+        """GMRES step operator as a HIP graph replay: -1 timed choice, 0 off,
+        1 on (pbfmm_set_graph_mode)."""
+        self._chk(self._L.pbfmm_set_graph_mode(self._h, int(mode)))
+
+    def graph_timings(self):
+        """ms per step behind the graph choice: FP64 direct, graph, FP32
+        direct, graph (pbfmm_graph_timings)."""
+        ms = np.zeros(4, np.float32)
+        self._chk(self._L.pbfmm_graph_timings(self._h, ms.ctypes.data_as(C.POINTER(C.c_float))))
+        return ms.tolist()
 
     def set_double_body(self, enable=True, z0=0.0):
         """Double-body operator: every cell also acts through its mirror image
