@@ -4,12 +4,16 @@ A step is one GMRES(100) solve of configuration 3 to true relative residual
 1e-8 with the mixed-precision path: prolate spheroid (4,1,1) from a 6-face
 cube grid with 128 x 128 Q2 cells per face (98,304 cells, 393,218 DoFs;
 equiangular face coordinates c -> tan(pi c / 4) before the projection),
-FMM order 8, leaf <= 20 nodes, Dirichlet on cells with centroid z > 0 and
+FMM order 8, Dirichlet on cells with centroid z > 0 and
 Neumann elsewhere, data from phi = x + 8 seeded exterior point sources at
 1.5x the surface (seed 3), singular cells integrated by the Duffy split of the
 reference's dense solve.  Mixed precision: FP32 operator (near-field SpMV,
 P2M, M2M, M2L, L2L, M2P, L2P) inside an FP64 outer loop with FP64 Arnoldi,
-FP64 residuals and FP64 singular quadrature.  The FP64 GPU path is timed
+FP64 residuals and FP64 singular quadrature.  The leaf size (max nodes per
+leaf box) is the one the project's autotuner picks (pibem_amd.tune_leaf_size:
+whole solves timed at leaf sizes on a ladder from 8 to 160, walking from the
+reference's default of 20 until the best is bracketed), for the mixed solve
+and, separately, for the FP64 solve.  The FP64 GPU path is timed
 beside it, as are FMM matvecs per second, per-phase device times from HIP
 events on the library's stream, and the CPU oracle (the reference's algorithm
 restated in C++, OpenMP) on the host cores.  Configuration 2 (unit sphere,
@@ -19,9 +23,9 @@ restated in C++, OpenMP) on the host cores.  Configuration 2 (unit sphere,
 "config4" and "config5" objects.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--extra 2,4,5]
-    python bench.py --mode matvec --config {2,3} --precision {fp32,fp64} --steps K
+    python bench.py --mode matvec --config {2,3} --precision {fp32,fp64} --steps K [--leaf L]
     python bench.py --mode copy --mib M --steps K
-    python bench.py --mode solve --config {2,3} --precision {fp32,fp64} --steps K
+    python bench.py --mode solve --config {2,3} --precision {fp32,fp64} --steps K [--leaf L]
 
 The default mode prints one JSON line.  "matvec" only runs K FMM matvecs
 (separate N and D outputs) followed by K operator applications, and
@@ -153,9 +157,30 @@ def traffic_for(kernel, workload):
     return None
 
 
-def make_problem(config):
+def make_problem(config, leaf=None):
+    """Configuration 2 or 3; leaf: max_nodes_per_leaf (None: the reference's
+    default of 20)."""
+    import dataclasses
     from pibem_amd import mesh as M
-    return M.config_sphere(64, 6) if config == 2 else M.config_spheroid(128, 8)
+    pr = M.config_sphere(64, 6) if config == 2 else M.config_spheroid(128, 8)
+    return dataclasses.replace(pr, max_nodes_per_leaf=int(leaf)) if leaf else pr
+
+
+def tune_leaves(config):
+    """The leaf-size autotuner (pibem_amd.tune_leaf_size: whole solves timed,
+    a walk over the ladder 8..160 from the reference's 20 until the best is
+    bracketed) for the mixed and the FP64 solve of a configuration."""
+    import pibem_amd as P
+    pr = make_problem(config)
+    out = {}
+    for name, prec in (("mixed", P.PBFMM_MIXED), ("fp64", P.PBFMM_FP64)):
+        best, times, info, br = P.tune_leaf_size(pr, precision=prec, solves=2)
+        log(f"config {config} leaf tuning, {name}: " + " ".join(f"{k}:{v:.2f}ms" for k, v in sorted(times.items()))
+            + f" -> {best}" + ("" if br else " (not bracketed)"))
+        out[name] = dict(best=best, bracketed=br, ms_per_solve={str(k): v for k, v in sorted(times.items())},
+                         steps={str(k): info[k][0] for k in sorted(info)},
+                         setup_s={str(k): info[k][2] for k in sorted(info)})
+    return out
 
 
 def setup_handle(pr, stream):
@@ -278,24 +303,37 @@ def time_matvecs(f, n, stream, reps=50, phase_reps=10):
 
 
 WORKLOAD = {
-    2: "config 2: unit sphere, cube-projected 6x64x64 Q1 cells, 24,578 DoFs, FMM order 6, leaf <= 20, "
-       "phi = 1/|x - (2,1,1)|, Duffy-split singular rule, GMRES(100) to true rel-res 1e-8",
+    2: "config 2: unit sphere, cube-projected 6x64x64 Q1 cells, 24,578 DoFs, FMM order 6, "
+       "phi = 1/|x - (2,1,1)|, Duffy-split singular rule, GMRES(100) to true rel-res 1e-8, leaf size (max nodes per "
+       "leaf box) chosen by the project's autotuner from timed solves",
     3: "config 3: prolate spheroid (4,1,1), 6x128x128 Q2 cells (98,304) on an equiangular cube map "
-       "(c -> tan(pi c/4) before projection), 393,218 DoFs, FMM order 8, leaf <= 20, phi = x + 8 seeded point "
+       "(c -> tan(pi c/4) before projection), 393,218 DoFs, FMM order 8, phi = x + 8 seeded point "
        "sources (seed 3), Duffy-split singular rule, GMRES(100) to true rel-res 1e-8, mixed precision (FP32 "
-       "operator; FP64 singular quadrature, Arnoldi, residual and outer loop)",
+       "operator; FP64 singular quadrature, Arnoldi, residual and outer loop), leaf size (max nodes per leaf box) "
+       "chosen by the project's autotuner from timed solves",
 }
 
 
-def config_summary(c, workload):
-    """The per-configuration object: solves, rates, phases, rooflines."""
+def config_summary(c, workload, leaves=None):
+    """The per-configuration object: solves, rates, phases, rooflines.  With
+    the leaf tuner's results (leaves), the FP64 solve also at its own tuned
+    leaf size, and mixed against FP64 each at its best."""
     r = c["roofline"]
+    mixed_ms = c["solves"]["mixed"]["ms"]
+    fp64_best = None
+    if leaves:
+        lf = leaves["fp64"]
+        fp64_best = dict(leaf=lf["best"], ms=lf["ms_per_solve"][str(lf["best"])])
     return {
         "workload": workload,
         "n_dofs": c["wc"]["n_dofs"],
-        "mixed_ms_per_solve": c["solves"]["mixed"]["ms"],
+        "max_nodes_per_leaf": c["pr"].max_nodes_per_leaf,
+        "mixed_ms_per_solve": mixed_ms,
         "fp64_ms_per_solve": c["solves"]["fp64"]["ms"],
-        "mixed_speedup_vs_fp64": c["solves"]["fp64"]["ms"] / c["solves"]["mixed"]["ms"],
+        "mixed_speedup_vs_fp64": c["solves"]["fp64"]["ms"] / mixed_ms,
+        "fp64_at_its_tuned_leaf": fp64_best,
+        "mixed_speedup_vs_fp64_each_tuned": fp64_best["ms"] / mixed_ms if fp64_best else None,
+        "leaf_tuning": leaves,
         "solves": c["solves"],
         "operator_applications_per_s": {k: v["per_s"] for k, v in c["operator"].items()},
         "fmm_matvecs_per_s": {k: v["per_s"] for k, v in c["matvec"].items()},
@@ -312,13 +350,13 @@ def config_summary(c, workload):
     }
 
 
-def headline(c, args):
+def headline(c, args, leaves=None):
     """The JSON line's contract fields for configuration c (mixed solve),
     with the roofline of the dominant kernel of its FP32 operator."""
     ph = c["operator_phases"]["fp32"]
     dom = max(PHASES, key=lambda k: ph[k])
     roof = dict(c["roofline"]["fp32"][dom])
-    roof["traffic"] = traffic_for(roof["kernel"], "config 3")
+    roof["traffic"] = traffic_for(roof["kernel"], f"config 3, leaf {c['pr'].max_nodes_per_leaf}")
     hl = dict(bound=roof["bound"], achieved=roof["achieved"], peak=roof["peak"], unit=roof["unit"],
               frac=roof["frac"], traffic=roof["traffic"], kernel=roof["kernel"],
               algorithmic_per_launch=roof["algorithmic_per_launch"], launch_ms=roof["launch_ms"])
@@ -337,21 +375,21 @@ def headline(c, args):
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": WORKLOAD[3], "n_dofs": wc["n_dofs"], "trunc_order": wc["trunc_order"],
-                   "parallelism": "single GPU"},
+                   "max_nodes_per_leaf": c["pr"].max_nodes_per_leaf, "parallelism": "single GPU"},
         "roofline": hl,
     }
-    summ = config_summary(c, WORKLOAD[3])
-    for k in ("workload", "n_dofs"):
+    summ = config_summary(c, WORKLOAD[3], leaves)
+    for k in ("workload", "n_dofs", "max_nodes_per_leaf"):
         summ.pop(k)
     out.update(summ)
     return out
 
 
-def measure_config(config, steps, warmup):
+def measure_config(config, steps, warmup, leaf=None):
     from pibem_amd import hip
-    pr = make_problem(config)
+    pr = make_problem(config, leaf)
     n = pr.mesh.n_dofs
-    log(f"config {config}: N={n} cells={pr.mesh.n_cells} p={pr.p}")
+    log(f"config {config}: N={n} cells={pr.mesh.n_cells} p={pr.p} leaf {pr.max_nodes_per_leaf}")
     stream = hip.Stream()
     f, bp, setup = setup_handle(pr, stream)
     log("setup: " + " ".join(f"{k}={v:.2f}" for k, v in setup.items()))
@@ -420,19 +458,16 @@ def measure_multi_rhs(steps):
     return out
 
 
-def measure_batch():
-    """Configuration 4: 64 independent problems of uneven size (seed 4) as
-    one batch (pbfmm_batch_create: a forest of octrees, one launch per phase
-    for all problems), solved with every problem's own relative residual
-    <= 1e-8 (mixed and FP64), each against its own exact solution."""
+def _batch_solves(probs, leaf, timed=True, precisions=("mixed", "fp64")):
+    """Set up probs as one batch at the given leaf size and solve every
+    problem to its own true relative residual 1e-8; returns set-up seconds
+    and, per precision, ms, operator applications, max relres and the max
+    error against each problem's exact solution."""
+    import dataclasses
     import pibem_amd as P
     from pibem_amd import hip
     from pibem_amd import mesh as M
-    t = time.perf_counter()
-    probs = M.config_batch()
-    t_mesh = time.perf_counter() - t
-    ntot = sum(p.mesh.n_dofs for p in probs)
-    log(f"config 4: {len(probs)} problems, {ntot} DoFs in total (meshes {t_mesh:.1f} s)")
+    probs = [dataclasses.replace(p, max_nodes_per_leaf=int(leaf)) for p in probs]
     t = time.perf_counter()
     b = P.setup_batch(probs, P.PBFMM_MIXED)
     bb = P.BEMProblem(b)
@@ -450,10 +485,10 @@ def measure_batch():
     rhs_h = bb.compute_rhs(np.concatenate([pr.tmp_rhs for pr in probs]))
     rhs_h[mc[4] == 1] = mc[3][mc[4] == 1]
     hip.synchronize()
-    setup_s = time.perf_counter() - t
+    out = dict(setup_s=time.perf_counter() - t)
     rhs, x = hip.DeviceArray.from_numpy(rhs_h), hip.DeviceArray(b.n)
-    out = dict(problems=len(probs), n_dofs_total=ntot, setup_s=setup_s)
-    for name, prec in (("mixed", P.PBFMM_MIXED), ("fp64", P.PBFMM_FP64)):
+    for name in precisions:
+        prec = P.PBFMM_MIXED if name == "mixed" else P.PBFMM_FP64
         # one untimed solve first: the first launch of each kernel loads its
         # code object, which would otherwise land in the timed solve
         x.upload(np.zeros(b.n))
@@ -475,26 +510,58 @@ def measure_batch():
         out[name + "_ops"] = it
         out[name + "_max_relres"] = float(rr.max())
         out[name + "_max_error"] = err
-    log(f"config 4: set-up {setup_s:.1f} s; solves mixed {out['mixed_ms']:.0f} ms ({out['mixed_ops']} operator "
-        f"applications), fp64 {out['fp64_ms']:.0f} ms ({out['fp64_ops']}); max relres {out['mixed_max_relres']:.2e}"
-        f", max error vs analytic phi {out['mixed_max_error']:.2e}")
     b.close()
+    return out
+
+
+def measure_batch(tune=True):
+    """Configuration 4: 64 independent problems of uneven size (seed 4) as
+    one batch (pbfmm_batch_create: a forest of octrees, one launch per phase
+    for all problems), solved with every problem's own relative residual
+    <= 1e-8 (mixed and FP64), each against its own exact solution.  With
+    tune, the leaf-size autotuner's walk (pibem_amd.leaf_walk) times mixed
+    batch solves of every eighth problem at each leaf size it visits, and the
+    whole batch is solved at the reference's leaf size 20 and at the one
+    chosen."""
+    import pibem_amd as P
+    from pibem_amd import mesh as M
+    t = time.perf_counter()
+    probs = M.config_batch()
+    t_mesh = time.perf_counter() - t
+    ntot = sum(p.mesh.n_dofs for p in probs)
+    log(f"config 4: {len(probs)} problems, {ntot} DoFs in total (meshes {t_mesh:.1f} s)")
+    out = dict(problems=len(probs), n_dofs_total=ntot)
+    leaf = 20
+    if tune:
+        sub = probs[::8]
+        walk = {}
+
+        def cost(lf):
+            r = _batch_solves(sub, lf, precisions=("mixed",))
+            walk[lf] = r["mixed_ms"]
+            return r["mixed_ms"]
+
+        leaf, br = P.leaf_walk(cost)
+        log("config 4 leaf tuning on every eighth problem, mixed: "
+            + " ".join(f"{k}:{v:.2f}ms" for k, v in sorted(walk.items())) + f" -> {leaf}"
+            + ("" if br else " (not bracketed)"))
+        out["leaf_tuning"] = dict(sample=f"{len(sub)} problems (every eighth), mixed batch solves", best=leaf,
+                                  bracketed=br, ms_per_solve={str(k): v for k, v in sorted(walk.items())})
+    for lf in sorted({20, leaf}):
+        r = _batch_solves(probs, lf)
+        if lf == leaf:
+            out.update(r)
+            out["max_nodes_per_leaf"] = lf
+        if lf == 20:
+            out["at_leaf_20"] = r
+        log(f"config 4, leaf {lf}: set-up {r['setup_s']:.1f} s; solves mixed {r['mixed_ms']:.0f} ms "
+            f"({r['mixed_ops']} operator applications), fp64 {r['fp64_ms']:.0f} ms ({r['fp64_ops']}); max relres "
+            f"{r['mixed_max_relres']:.2e}, max error vs analytic phi {r['mixed_max_error']:.2e}")
     return dict(workload="config 4: 64 problems (seed 4) of 1.5k-25k Q1 DoFs: spheres, spheroids of aspect 1-6, "
                          "double-node cubes, each with its own 8-source exterior solution, FMM order 6, as one batch "
-                         "(one launch per phase for all problems), every problem to its own true rel-res 1e-8",
+                         "(one launch per phase for all problems), every problem to its own true rel-res 1e-8, leaf "
+                         "size chosen by the project's autotuner",
                 **out)
-
-
-def measure_leaf_tuning():
-    """The leaf-size autotuner (pibem_amd.tune_leaf_size) on configuration
-    2: FP32 operator time per application for each candidate leaf size.  The
-    headline keeps the reference's default of 20."""
-    import pibem_amd as P
-    best, times = P.tune_leaf_size(make_problem(2), precision=P.PBFMM_MIXED)
-    log("leaf-size tuning, config 2, FP32 operator: " + " ".join(f"{k}:{v:.3f}ms" for k, v in times.items())
-        + f" -> {best}")
-    return dict(workload="config 2, FP32 operator application", best=best,
-                ms_per_application={str(k): v for k, v in times.items()}, default=20)
 
 
 def host_cpu():
@@ -551,7 +618,7 @@ def cpu_baseline(pr, ops, label):
 def mode_matvec(args):
     import pibem_amd as P
     from pibem_amd import hip
-    pr = make_problem(args.config)
+    pr = make_problem(args.config, args.leaf or None)
     stream = hip.Stream()
     f, _, _ = setup_handle(pr, stream)
     n = pr.mesh.n_dofs
@@ -574,13 +641,13 @@ def mode_matvec(args):
     f.set_phase_timing(False)
     stream.synchronize()
     log(f"{args.steps} {args.precision} FMM matvecs and {args.steps} operator applications at config "
-        f"{args.config} done")
+        f"{args.config}, leaf {pr.max_nodes_per_leaf} done")
 
 
 def mode_solve(args):
     import pibem_amd as P
     from pibem_amd import hip
-    pr = make_problem(args.config)
+    pr = make_problem(args.config, args.leaf or None)
     stream = hip.Stream()
     f, bp, _ = setup_handle(pr, stream)
     n = pr.mesh.n_dofs
@@ -592,8 +659,8 @@ def mode_solve(args):
         x.copy_from(zero)
         its, rr = f.gmres_dev(rhs.value, x.value, 1e-8, 100, 2000, prec)
     stream.synchronize()
-    log(f"{args.steps} {args.precision} solves at config {args.config} done: {its} operator applications, "
-        f"relres {rr:.3e}")
+    log(f"{args.steps} {args.precision} solves at config {args.config}, leaf {pr.max_nodes_per_leaf} done: {its} "
+        f"operator applications, relres {rr:.3e}")
 
 
 def mode_copy(args):
@@ -619,6 +686,7 @@ def main():
     ap.add_argument("--config", type=int, choices=(2, 3), default=3)
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--mib", type=int, default=1024)
+    ap.add_argument("--leaf", type=int, default=0, help="matvec/solve modes: max nodes per leaf (0: 20)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported; this build runs the BEM-FMM path on a "
@@ -635,25 +703,28 @@ def main():
     if args.mode == "solve":
         return mode_solve(args)
 
-    c3 = measure_config(3, args.steps, args.warmup)
-    out = headline(c3, args)
+    leaves3 = None if args.no_leaf_tuning else tune_leaves(3)
+    c3 = measure_config(3, args.steps, args.warmup, leaves3["mixed"]["best"] if leaves3 else None)
+    out = headline(c3, args, leaves3)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(c3["pr"], c3["solves"]["fp64"]["iters"] + 2, "config 3")
+        # the reference's own leaf size (20): the oracle restates the
+        # reference, which has no leaf tuner; its FP64 step count there
+        ops = int(leaves3["fp64"]["steps"]["20"]) if leaves3 else c3["solves"]["fp64"]["iters"]
+        cpu = cpu_baseline(make_problem(3), ops + 2, "config 3 (leaf size 20, the reference's default)")
     out["cpu_baseline"] = cpu
     out["speedup_vs_cpu"] = out["value"] / cpu["value"] if cpu else None
     del c3
     extra = {int(x) for x in args.extra.split(",") if x.strip()}
     if 2 in extra:
-        c2 = measure_config(2, args.steps, args.warmup)
-        out["config2"] = config_summary(c2, WORKLOAD[2])
+        leaves2 = None if args.no_leaf_tuning else tune_leaves(2)
+        c2 = measure_config(2, args.steps, args.warmup, leaves2["mixed"]["best"] if leaves2 else None)
+        out["config2"] = config_summary(c2, WORKLOAD[2], leaves2)
         del c2
     if 5 in extra:
         out["config5"] = measure_multi_rhs(args.steps)
     if 4 in extra:
-        out["config4"] = measure_batch()
-    if not args.no_leaf_tuning:
-        out["leaf_size_tuning"] = measure_leaf_tuning()
+        out["config4"] = measure_batch(tune=not args.no_leaf_tuning)
     print(json.dumps(out))
 
 

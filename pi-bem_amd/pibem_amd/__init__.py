@@ -22,7 +22,7 @@ import numpy as np
 
 from ._lib import PBFMM_FP64, PBFMM_MIXED, PBFMM_SING_DUFFY, PBFMM_SING_TELLES, Params, PbfmmError, load
 
-__all__ = ["BEMFMA", "BatchFMA", "BEMProblem", "Body", "setup_batch", "tune_leaf_size", "PBFMM_FP64", "PBFMM_MIXED",
+__all__ = ["BEMFMA", "BatchFMA", "BEMProblem", "Body", "setup_batch", "tune_leaf_size", "leaf_walk", "PBFMM_FP64", "PBFMM_MIXED",
            "PBFMM_SING_TELLES", "PBFMM_SING_DUFFY", "PbfmmError", "load"]
 
 _d = C.POINTER(C.c_double)
@@ -510,34 +510,90 @@ class BatchFMA(BEMFMA):
         return it.value, rr
 
 
-def tune_leaf_size(problem, candidates=(12, 16, 20, 24, 32, 48), precision=PBFMM_MIXED, reps=20, device=0):
-    """Leaf-size autotuner: sets the problem up with each candidate
-    max_nodes_per_leaf and times the operator GMRES applies (vmult on device
-    vectors, `reps` applications after two warm-ups, timed by wall clock
-    between device synchronisations) in the given precision.  Returns (best leaf size, {leaf size: ms per
-    application}).  The leaf size moves the near/far split, so results differ
-    within the FMM truncation error; the reference's default is 20."""
+# Leaf sizes the tuner walks: a geometric ladder around the reference's 20.
+LEAF_LADDER = (8, 12, 16, 20, 24, 32, 40, 48, 64, 80, 96, 128, 160)
+
+
+def tune_leaf_size(problem, precision=PBFMM_MIXED, solves=2, start=20, ladder=LEAF_LADDER, tol=1e-8,
+                   device=0, log=None):
+    """Leaf-size autotuner: sets the problem up with a leaf size
+    (max_nodes_per_leaf) and times whole GMRES solves to relative residual
+    tol in the given precision (one warm-up solve, then `solves` timed ones,
+    wall clock between device synchronisations, right-hand side on the
+    device), so the operator's cost, the preconditioner's blocks (cut from
+    the leaves) and the step count all count.  From `start` it walks the
+    ladder towards faster solves until a leaf size is slower than both its
+    evaluated neighbours or the ladder ends, so the best one is bracketed
+    where the ladder allows.  Returns (best leaf size, {leaf size: ms per
+    solve}, {leaf size: (steps, relres, set-up s)}, bracketed).  The leaf
+    size moves the near/far split, so solutions differ within the FMM
+    truncation error; the reference's default is 20."""
     import dataclasses
     from . import hip
-    times = {}
-    for leaf in candidates:
+    times, info = {}, {}
+
+    def measure(leaf):
+        if leaf in times:
+            return times[leaf]
         pr = dataclasses.replace(problem, max_nodes_per_leaf=int(leaf))
+        t0 = time.perf_counter()
         f = setup(pr, precision, device)
         bp = BEMProblem(f)
-        bp.compute_alpha()
-        src = hip.DeviceArray.from_numpy(np.random.default_rng(1234).uniform(-1, 1, f.n))
-        dst = hip.DeviceArray(f.n)
-        for _ in range(2):
-            bp.vmult_dev(src.value, dst.value, precision)
+        rhs = hip.DeviceArray.from_numpy(bp.compute_rhs(pr.tmp_rhs))
+        hip.synchronize()
+        t_setup = time.perf_counter() - t0
+        x = hip.DeviceArray(f.n)
+        zero = hip.DeviceArray(f.n)
+        f.gmres_dev(rhs.value, x.value, tol, 100, 2000, precision)
         hip.synchronize()
         t0 = time.perf_counter()
-        for _ in range(reps):
-            bp.vmult_dev(src.value, dst.value, precision)
+        for _ in range(solves):
+            x.copy_from(zero)
+            its, rr = f.gmres_dev(rhs.value, x.value, tol, 100, 2000, precision)
         hip.synchronize()
-        times[int(leaf)] = (time.perf_counter() - t0) * 1e3 / reps
+        times[leaf] = (time.perf_counter() - t0) * 1e3 / solves
+        info[leaf] = (its, rr, t_setup)
         f.close()
-    best = min(times, key=times.get)
-    return best, times
+        if log:
+            log(f"leaf {leaf}: {times[leaf]:.3f} ms per solve, {its} steps, set-up {t_setup:.2f} s")
+        return times[leaf]
+
+    best, bracketed = leaf_walk(measure, start, ladder)
+    return best, times, info, bracketed
+
+
+def leaf_walk(measure, start=20, ladder=LEAF_LADDER):
+    """The tuner's walk over the ladder: measure(leaf) -> cost (each leaf is
+    measured once here).  From start it measures both neighbours, then steps
+    towards the cheaper side until two leaf sizes in a row cost no less than
+    the best so far (one more step than needed, so that a plateau or a noisy
+    point does not end the walk early).  Returns (best leaf, bracketed: both
+    ladder neighbours of the best were measured and cost more)."""
+    lad = list(ladder)
+    cost = {}
+
+    def m(leaf):
+        if leaf not in cost:
+            cost[leaf] = measure(leaf)
+        return cost[leaf]
+
+    k = lad.index(start) if start in lad else 0
+    m(lad[k])
+    for j in (k - 1, k + 1):
+        if 0 <= j < len(lad):
+            m(lad[j])
+    best = min(cost, key=cost.get)
+    kb = lad.index(best)
+    step = 0 if kb == k else (1 if kb > k else -1)
+    j, misses = kb + step, 0
+    while step and 0 <= j < len(lad) and misses < 2:
+        if m(lad[j]) < cost[best]:
+            best, kb, misses = lad[j], j, 0
+        else:
+            misses += 1
+        j += step
+    bracketed = 0 < kb < len(lad) - 1 and lad[kb - 1] in cost and lad[kb + 1] in cost
+    return best, bracketed
 
 
 def setup_batch(problems, precision=PBFMM_FP64, device=0, preconditioner=1):

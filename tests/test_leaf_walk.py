@@ -1,0 +1,47 @@
+"""The leaf-size tuner's walk over its ladder (pibem_amd.leaf_walk), on
+cost functions with a known minimum: it finds an interior minimum and
+brackets it, stops at the ladder's end for monotone costs (not bracketed),
+and measures every leaf size at most once."""
+from collections import Counter
+
+from pibem_amd import LEAF_LADDER, leaf_walk
+
+
+def _walk(cost, start=20):
+    seen = Counter()
+
+    def measure(leaf):
+        seen[leaf] += 1
+        return cost(leaf)
+
+    best, bracketed = leaf_walk(measure, start)
+    return best, bracketed, seen
+
+
+def test_interior_minimum_bracketed():
+    for target in (12, 20, 48, 64, 128):
+        best, bracketed, seen = _walk(lambda l: abs(l - target))
+        assert best == target and bracketed
+        assert max(seen.values()) == 1
+
+
+def test_monotone_costs_end_of_ladder():
+    best, bracketed, _ = _walk(lambda l: l)
+    assert best == LEAF_LADDER[0] and not bracketed
+    best, bracketed, _ = _walk(lambda l: -l)
+    assert best == LEAF_LADDER[-1] and not bracketed
+
+
+def test_walk_stops_after_two_rises():
+    # two valleys: the walk from 20 takes the nearer one and stops two leaf
+    # sizes past it
+    cost = {8: 5, 12: 4, 16: 3, 20: 4, 24: 5, 32: 6, 40: 7, 48: 1, 64: 2}
+    best, bracketed, seen = _walk(lambda l: cost.get(l, 9))
+    assert best == 16 and bracketed and 48 not in seen and 8 in seen
+
+
+def test_walk_crosses_a_plateau():
+    # equal costs at 32 and 40 do not end the walk: 64 is found
+    cost = {16: 9, 20: 8, 24: 7, 32: 6, 40: 6, 48: 5, 64: 4, 80: 4.5, 96: 5}
+    best, bracketed, seen = _walk(lambda l: cost.get(l, 10))
+    assert best == 64 and bracketed and 128 not in seen
