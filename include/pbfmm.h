@@ -275,9 +275,8 @@ int pbfmm_last_timings(pbfmm_handle *h, double *ms);
  * [11] trunc_order [12] padded expansion length kp
  * [13] unit-multipole columns (sum over leaves of the DoFs feeding P2M)
  * [14] [15] near-field SpMV kernels chosen at set-up for FP64 / FP32:
- *      kernel of the (N, D) pair storage + 4 * kernel of the operator's
- *      single values (0 workgroup per leaf, 1 wavefront per leaf with 2 rows
- *      in flight, 2 with 4 rows) */
+ *      kernel of the operator's single values + 8 * kernel of the (N, D)
+ *      pair storage (as pbfmm_set_spmv_kernel) */
 int pbfmm_work_counts(pbfmm_handle *h, int64_t *counts);
 
 /* The block-Jacobi preconditioner of pbfmm_fma_preconditioner before
@@ -310,8 +309,9 @@ int pbfmm_set_mixed_tolerances(pbfmm_handle *h, double inner_reduction, double r
 /* Overrides the near-field SpMV kernel that set-up's autotuner chose for one
  * precision (PBFMM_FP64 or PBFMM_MIXED for the FP32 values), for both
  * storages: 0 workgroup per leaf, 1 wavefront per leaf with 2 rows in flight,
- * 2 with 4 rows, 3 with 8 rows summed by one exchange tree.  For tests and
- * tuning; call after
+ * 2 with 4 rows; FP32 also 3..5: kernels 0..2 on the operator's single values
+ * kept in row pairs (two rows per 8-byte load; the (N, D) pair storage then
+ * uses kernel - 3).  For tests and tuning; call after
  * pbfmm_direct_integrals.  No counterpart in the reference (Trilinos
  * chooses its own SpMV). */
 int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel);

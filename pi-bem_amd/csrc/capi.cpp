@@ -927,8 +927,8 @@ int pbfmm_work_counts(pbfmm_handle *h, int64_t *c) {
     c[11] = e.p;
     c[12] = e.kp;
     c[13] = (int64_t)e.far.p2m_col_dof.size();
-    c[14] = e.spmv_variant[0][0] + 4 * e.spmv_variant[0][1];
-    c[15] = e.spmv_variant[1][0] + 4 * e.spmv_variant[1][1];
+    c[14] = e.spmv_variant[0][0] + 8 * e.spmv_variant[0][1];
+    c[15] = e.spmv_variant[1][0] + 8 * e.spmv_variant[1][1];
     return PBFMM_OK;
   }, false);
 }
@@ -1029,13 +1029,15 @@ int pbfmm_set_mixed_tolerances(pbfmm_handle *h, double inner_reduction, double r
 int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel) {
   return guarded(h, [&]() -> int {
     if (int rc = need(h, 3, "set_spmv_kernel")) return rc;
-    if (kernel < 0 || kernel > 3) return fail(h, PBFMM_ERR_ARG, "set_spmv_kernel: kernel must be 0, 1, 2 or 3");
     if (precision != PBFMM_FP64 && precision != PBFMM_MIXED)
       return fail(h, PBFMM_ERR_ARG, "set_spmv_kernel: unknown precision");
+    if (kernel < 0 || kernel > (precision == PBFMM_MIXED ? 5 : 2))
+      return fail(h, PBFMM_ERR_ARG, "set_spmv_kernel: kernel must be 0..2 (FP64) or 0..5 (FP32)");
     if (precision == PBFMM_MIXED && !h->eng.have_f32)
       return fail(h, PBFMM_ERR_STATE, "set_spmv_kernel: handle was not created with PBFMM_MIXED");
     const int pi = precision == PBFMM_MIXED ? 1 : 0;
-    h->eng.spmv_variant[pi][0] = h->eng.spmv_variant[pi][1] = kernel;
+    h->eng.spmv_variant[pi][0] = kernel;
+    h->eng.spmv_variant[pi][1] = kernel % 3;
     return PBFMM_OK;
   });
 }

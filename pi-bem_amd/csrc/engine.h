@@ -44,6 +44,7 @@ struct FarDev {
   // the operator's single-value copies: N on Neumann / -D on Dirichlet
   // columns, U1 on Neumann / -U2 on Dirichlet DoFs ([column][kp])
   DevBuf opvals, opunit;
+  DevBuf opvals_p;  // FP32 only: opvals in row pairs (k_round_f32_pairs)
   DevBuf slots;  // slot M2L: [pair][2 Kt], on first use
 };
 
@@ -82,6 +83,7 @@ struct Engine {
   DevBuf d_p2m_col_beg, d_p2m_col_dof;
   DevBuf d_is_leaf;  // per block: 1 for a leaf (P2M stores its ME)
   ChunkDev m2m_cl, m2l_cl, l2l_cl;
+  DevBuf d_pval_beg;  // FP32 operator values: per-leaf offsets of the row-pair layout
   DevBuf d_row_leaf, d_row_beg, d_rows, d_col_beg, d_cols, d_val_beg, d_ent_beg, d_ent_cell,
       d_ent_mask, d_ent_near, d_ent_colpos, d_work_beg, d_leaf_block;
   DevBuf d_sing;  // singular rules: per js [w, N(dpc), dNx(dpc), dNy(dpc)] per point

@@ -254,6 +254,29 @@ __global__ void k_round_f32(int64_t n, const double *__restrict__ a, float *__re
     b[i] = (float)a[i];
 }
 
+// The FP32 operator's single-value near field in row pairs: leaf l's R x C
+// block (row-major in the FP64 copy at val_beg[l]) goes to pval_beg[l] (an
+// even offset: 8-byte aligned) with rows 2p and 2p + 1 interleaved column by
+// column, (2p + s, c) at 2 p C + 2 c + s, and an odd last row as it is, at
+// (R - 1) C + c.  A lane then loads both rows of a pair at a column with one
+// 8-byte load.  One workgroup per leaf.
+__global__ void __launch_bounds__(256) k_round_f32_pairs(int nleaf, const int *__restrict__ row_beg,
+                                                         const int *__restrict__ col_beg,
+                                                         const int64_t *__restrict__ val_beg,
+                                                         const int64_t *__restrict__ pval_beg,
+                                                         const double *__restrict__ a, float *__restrict__ b) {
+  const int l = blockIdx.x;
+  if (l >= nleaf) return;
+  const int R = row_beg[l + 1] - row_beg[l], C = col_beg[l + 1] - col_beg[l], Re = R & ~1;
+  const double *src = a + val_beg[l];
+  float *dst = b + pval_beg[l];
+  for (int64_t i = threadIdx.x; i < (int64_t)R * C; i += blockDim.x) {
+    const int r = (int)(i / C), c = (int)(i - (int64_t)r * C);
+    const int64_t o = r < Re ? (int64_t)(r & ~1) * C + 2 * c + (r & 1) : i;
+    dst[o] = (float)src[i];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Near-field SpMV, fused N.phi and D.dphi on the shared pattern
 // (prec_neumann_matrix.vmult / prec_dirichlet_matrix.vmult, bem_fma.cc:2408-2409),
@@ -287,6 +310,12 @@ template <> struct NativePair<float> { typedef float type __attribute__((ext_vec
 template <> struct NativePair<double> { typedef double type __attribute__((ext_vector_type(2))); };
 template <typename T, int NK> struct NearEntry { using type = typename NativePair<T>::type; };
 template <typename T> struct NearEntry<T, 1> { using type = T; };
+// The FP32 operator's single values are kept twice: row-major like every
+// other near-field storage, and in row pairs (k_round_f32_pairs) read two
+// rows per 8-byte load by the kernels instantiated with PR (SpMV variants
+// 3..5); set-up's tuner times both.
+template <typename T, int NK>
+constexpr bool kPairedF32 = sizeof(T) == 4 && NK == 1;
 template <typename T, int NK>
 __device__ __forceinline__ typename NearEntry<T, NK>::type near_x(const double *phi, const double *dphi, int col) {
   if constexpr (NK == 2) return typename NearEntry<T, NK>::type{(T)phi[col], (T)dphi[col]};
@@ -331,15 +360,70 @@ __device__ __forceinline__ NearArgs<T> near_rhs(NearArgs<T> a) {
 
 // One leaf by a workgroup of NT threads: its inputs gathered once into LDS
 // (xs, kSpmvTile entries), each wavefront streaming whole rows.
-template <typename T, int NK, int NT>
+template <typename T, int NK, int NT, bool PR = false>
 __device__ __forceinline__ void spmv_lds_leaf(int leaf, typename NearEntry<T, NK>::type *__restrict__ xs,
                                               const NearArgs<T> &a) {
+  static_assert(!PR || kPairedF32<T, NK>, "row pairs hold the FP32 single values only");
   using E = typename NearEntry<T, NK>::type;
   constexpr int NW = NT / 64;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const int r0 = a.row_beg[leaf], R = a.row_beg[leaf + 1] - r0;
   const int cb = a.col_beg[leaf], C = a.col_beg[leaf + 1] - cb;
   const E *V = reinterpret_cast<const E *>(a.vals) + a.val_beg[leaf];
+  if constexpr (PR) {
+    // FP32 single values in row pairs (k_round_f32_pairs): a wavefront takes
+    // rows 2p, 2p + 1 with one 8-byte load per column; an odd last row goes
+    // to wavefront (R / 2) % NW, as a single row
+    using P2 = typename NativePair<float>::type;
+    const int Re = R & ~1;
+    const P2 *V2 = reinterpret_cast<const P2 *>(V);
+    for (int c0 = 0; c0 < C; c0 += kSpmvTile) {
+      const int CT = min(kSpmvTile, C - c0);
+      if (c0 > 0) __syncthreads();
+      for (int c = t; c < CT; c += NT) xs[c] = near_x<T, NK>(a.phi, a.dphi, a.cols[cb + c0 + c]);
+      __syncthreads();
+      // two row pairs per wavefront at a time (pa and pa + NW): twice the
+      // loads in flight
+      const int NP = Re / 2;
+      for (int pa = w; pa < NP; pa += 2 * NW) {
+        const int pb = pa + NW;
+        const bool hb = pb < NP;
+        const P2 *va = V2 + (int64_t)pa * C + c0, *vb = V2 + (int64_t)(hb ? pb : pa) * C + c0;
+        T a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+#pragma unroll 4
+        for (int c = lane; c < CT; c += 64) {
+          const T xv = xs[c];
+          const P2 u = va[c], z = vb[c];
+          a0 += u.x * xv;
+          a1 += u.y * xv;
+          b0 += z.x * xv;
+          b1 += z.y * xv;
+        }
+        a0 = wave_sum(a0);
+        a1 = wave_sum(a1);
+        b0 = wave_sum(b0);
+        b1 = wave_sum(b1);
+        if (lane == 0) {
+          near_store<1>(a.outN, nullptr, a.rows[r0 + 2 * pa], c0 == 0, (double)a0, 0.0);
+          near_store<1>(a.outN, nullptr, a.rows[r0 + 2 * pa + 1], c0 == 0, (double)a1, 0.0);
+          if (hb) {
+            near_store<1>(a.outN, nullptr, a.rows[r0 + 2 * pb], c0 == 0, (double)b0, 0.0);
+            near_store<1>(a.outN, nullptr, a.rows[r0 + 2 * pb + 1], c0 == 0, (double)b1, 0.0);
+          }
+        }
+      }
+      // an odd last row: the last wavefront, in every tile
+      if ((R & 1) && w == NW - 1) {
+        const T *v = V + (int64_t)Re * C + c0;
+        T s = 0;
+#pragma unroll 4
+        for (int c = lane; c < CT; c += 64) s += v[c] * xs[c];
+        s = wave_sum(s);
+        if (lane == 0) near_store<1>(a.outN, nullptr, a.rows[r0 + R - 1], c0 == 0, (double)s, 0.0);
+      }
+    }
+    return;
+  }
   for (int c0 = 0; c0 < C; c0 += kSpmvTile) {
     const int CT = min(kSpmvTile, C - c0);
     if (c0 > 0) __syncthreads();  // previous tile fully consumed
@@ -361,13 +445,13 @@ __device__ __forceinline__ void spmv_lds_leaf(int leaf, typename NearEntry<T, NK
 // Leaves are dealt out so that each XCD takes a contiguous range of per_xcd
 // leaves (workgroups go round-robin over the 8 XCDs): neighbouring leaves,
 // whose columns overlap, share an L2.
-template <typename T, int NK>
+template <typename T, int NK, bool PR = false>
 __global__ void __launch_bounds__(256) k_spmv_lds(int nleaf, int per_xcd, NearArgs<T> a) {
   using E = typename NearEntry<T, NK>::type;
   __shared__ E xs[kSpmvTile];
   const int leaf = (int)(blockIdx.x % kXcd) * per_xcd + (int)(blockIdx.x / kXcd);
   if (leaf >= nleaf) return;
-  spmv_lds_leaf<T, NK, 256>(leaf, xs, near_rhs(a));
+  spmv_lds_leaf<T, NK, 256, PR>(leaf, xs, near_rhs(a));
 }
 
 // One leaf by one wavefront, no LDS and no barrier, RPI rows per iteration.
@@ -376,8 +460,9 @@ __global__ void __launch_bounds__(256) k_spmv_lds(int nleaf, int per_xcd, NearAr
 // lane holds the inputs of its CPL columns in registers, and the wave streams
 // RPI rows at a time against them (RPI CPL coalesced loads per lane in
 // flight).  Outputs as spmv_lds_leaf.
-template <typename T, int NK, int RPI>
+template <typename T, int NK, int RPI, bool PR = false>
 __device__ __forceinline__ void spmv_wave_leaf(int leaf, int lane, const NearArgs<T> &a) {
+  static_assert(!PR || kPairedF32<T, NK>, "row pairs hold the FP32 single values only");
   using E = typename NearEntry<T, NK>::type;
   constexpr int CPL = 64 / (int)sizeof(E);
   constexpr int TILE = 64 * CPL;
@@ -385,6 +470,66 @@ __device__ __forceinline__ void spmv_wave_leaf(int leaf, int lane, const NearArg
   const int cb = a.col_beg[leaf], C = a.col_beg[leaf + 1] - cb;
   const E *V = reinterpret_cast<const E *>(a.vals) + a.val_beg[leaf];
   const E zero{};
+  if constexpr (PR) {
+    // FP32 single values in row pairs (k_round_f32_pairs): RPI row pairs per
+    // iteration, one 8-byte load per (pair, column); an odd last row alone
+    using P2 = typename NativePair<float>::type;
+    const int Re = R & ~1, NP = Re / 2;
+    const P2 *V2 = reinterpret_cast<const P2 *>(V);
+    const P2 zero2{};
+    for (int c0 = 0; c0 < C; c0 += TILE) {
+      T x[CPL];
+#pragma unroll
+      for (int j = 0; j < CPL; ++j) {
+        const int c = c0 + lane + 64 * j;
+        x[j] = c < C ? near_x<T, NK>(a.phi, a.dphi, a.cols[cb + c]) : zero;
+      }
+      for (int p = 0; p < NP; p += RPI) {
+        P2 v[RPI][CPL];
+#pragma unroll
+        for (int q = 0; q < RPI; ++q) {
+          const P2 *vq = V2 + (int64_t)(p + q < NP ? p + q : p) * C;
+#pragma unroll
+          for (int j = 0; j < CPL; ++j) {
+            const int c = c0 + lane + 64 * j;
+            v[q][j] = c < C ? vq[c] : zero2;
+          }
+        }
+        T s0[RPI], s1[RPI];
+#pragma unroll
+        for (int q = 0; q < RPI; ++q) {
+          s0[q] = 0;
+          s1[q] = 0;
+#pragma unroll
+          for (int j = 0; j < CPL; ++j) {
+            s0[q] += v[q][j].x * x[j];
+            s1[q] += v[q][j].y * x[j];
+          }
+          s0[q] = wave_sum(s0[q]);
+          s1[q] = wave_sum(s1[q]);
+        }
+        if (lane == 0)
+#pragma unroll
+          for (int q = 0; q < RPI; ++q)
+            if (p + q < NP) {
+              near_store<1>(a.outN, nullptr, a.rows[r0 + 2 * (p + q)], c0 == 0, (double)s0[q], 0.0);
+              near_store<1>(a.outN, nullptr, a.rows[r0 + 2 * (p + q) + 1], c0 == 0, (double)s1[q], 0.0);
+            }
+      }
+      if (R & 1) {
+        const T *vl = V + (int64_t)Re * C;
+        T s = 0;
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+          const int c = c0 + lane + 64 * j;
+          if (c < C) s += vl[c] * x[j];
+        }
+        s = wave_sum(s);
+        if (lane == 0) near_store<1>(a.outN, nullptr, a.rows[r0 + R - 1], c0 == 0, (double)s, 0.0);
+      }
+    }
+    return;
+  }
   for (int c0 = 0; c0 < C; c0 += TILE) {
     E x[CPL];
 #pragma unroll
@@ -423,95 +568,11 @@ __device__ __forceinline__ void spmv_wave_leaf(int leaf, int lane, const NearArg
   }
 }
 
-// Eight row partials per lane (acc[q]: row q) to one row sum per 8-lane
-// group: three exchange stages halve the partials a lane keeps (lane bit 5
-// picks rows 4..7 or 0..3, bit 4 the upper or lower pair, bit 3 one row), so
-// lane group g = lane >> 3 ends with row g's partials of the wave, and three
-// more stages sum the group.  Ten shuffles in all, against eight full
-// wavefront sums row by row.
-template <typename T>
-__device__ __forceinline__ T rows8_sum(T (&acc)[8], int lane) {
-  {
-    const bool b = (lane >> 5) & 1;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const T mine = b ? acc[j + 4] : acc[j], send = b ? acc[j] : acc[j + 4];
-      acc[j] = mine + __shfl_xor(send, 32, 64);
-    }
-  }
-  {
-    const bool b = (lane >> 4) & 1;
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const T mine = b ? acc[j + 2] : acc[j], send = b ? acc[j] : acc[j + 2];
-      acc[j] = mine + __shfl_xor(send, 16, 64);
-    }
-  }
-  const bool b = (lane >> 3) & 1;
-  const T mine = b ? acc[1] : acc[0], send = b ? acc[0] : acc[1];
-  T s = mine + __shfl_xor(send, 8, 64);
-  s += __shfl_xor(s, 4, 64);
-  s += __shfl_xor(s, 2, 64);
-  s += __shfl_xor(s, 1, 64);
-  return s;
-}
-
-// One leaf by one wavefront, eight rows at a time: every lane keeps the
-// inputs of its columns (c0 + lane + 64 j, the column tiles of
-// spmv_wave_leaf) and accumulates eight row partials over only the column
-// vectors the leaf fills; rows8_sum then leaves row r + (lane >> 3) with the
-// lanes of group lane >> 3, and lane 8 g stores it.
-template <typename T, int NK>
-__device__ __forceinline__ void spmv_wave8_leaf(int leaf, int lane, const NearArgs<T> &a) {
-  using E = typename NearEntry<T, NK>::type;
-  constexpr int CPL = 64 / (int)sizeof(E);
-  constexpr int TILE = 64 * CPL;
-  const int r0 = a.row_beg[leaf], R = a.row_beg[leaf + 1] - r0;
-  const int cb = a.col_beg[leaf], C = a.col_beg[leaf + 1] - cb;
-  const E *V = reinterpret_cast<const E *>(a.vals) + a.val_beg[leaf];
-  const E zero{};
-  for (int c0 = 0; c0 < C; c0 += TILE) {
-    const int J = min(CPL, (C - c0 + 63) / 64);  // column vectors of this tile (uniform)
-    E x[CPL];
-#pragma unroll
-    for (int j = 0; j < CPL; ++j) {
-      const int c = c0 + lane + 64 * j;
-      x[j] = (j < J && c < C) ? near_x<T, NK>(a.phi, a.dphi, a.cols[cb + c]) : zero;
-    }
-    for (int r = 0; r < R; r += 8) {
-      T aN[8], aD[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        aN[q] = 0;
-        aD[q] = 0;
-        if (r + q < R) {
-          const E *vq = V + (int64_t)(r + q) * C + c0 + lane;
-#pragma unroll
-          for (int j = 0; j < CPL; ++j)
-            if (j < J && c0 + lane + 64 * j < C) near_fma<T, NK>(aN[q], aD[q], vq[64 * j], x[j]);
-        }
-      }
-      const T sN = rows8_sum(aN, lane);
-      T sD = 0;
-      if constexpr (NK == 2) sD = rows8_sum(aD, lane);
-      const int row = r + (lane >> 3);
-      if ((lane & 7) == 0 && row < R) near_store<NK>(a.outN, a.outD, a.rows[r0 + row], c0 == 0, (double)sN, (double)sD);
-    }
-  }
-}
-
-template <typename T, int NK>
-__global__ void __launch_bounds__(256) k_spmv_wave8(int nleaf, NearArgs<T> a) {
-  const int leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (leaf >= nleaf) return;
-  spmv_wave8_leaf<T, NK>(leaf, threadIdx.x & 63, near_rhs(a));
-}
-
-template <typename T, int NK, int RPI>
+template <typename T, int NK, int RPI, bool PR = false>
 __global__ void __launch_bounds__(256) k_spmv_wave(int nleaf, NearArgs<T> a) {
   const int leaf = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (leaf >= nleaf) return;
-  spmv_wave_leaf<T, NK, RPI>(leaf, threadIdx.x & 63, near_rhs(a));
+  spmv_wave_leaf<T, NK, RPI, PR>(leaf, threadIdx.x & 63, near_rhs(a));
 }
 
 // ---------------------------------------------------------------------------
@@ -2026,6 +2087,18 @@ void Engine::assemble_near(bool want_f32) {
     alloc(f32.opvals, (size_t)nop * sizeof(float));
     hipLaunchKernelGGL(k_round_f32, dim3((unsigned)std::min<int64_t>(nblk(nop, 256), 8192)), dim3(256), 0, stream,
                        nop, f64.opvals.as<double>(), f32.opvals.as<float>());
+    // and in row pairs, each leaf at an even (8-byte aligned) offset
+    std::vector<int64_t> pvb(near.nleaf + 1, 0);
+    for (int l = 0; l < near.nleaf; ++l) {
+      const int64_t sz = (int64_t)(near.row_beg[l + 1] - near.row_beg[l]) * (near.col_beg[l + 1] - near.col_beg[l]);
+      pvb[l + 1] = pvb[l] + ((sz + 1) & ~(int64_t)1);
+    }
+    up(d_pval_beg, pvb);
+    alloc(f32.opvals_p, (size_t)std::max<int64_t>(pvb[near.nleaf], 2) * sizeof(float));
+    if (near.nleaf > 0)
+      hipLaunchKernelGGL(k_round_f32_pairs, dim3(near.nleaf), dim3(256), 0, stream, near.nleaf, d_row_beg.as<int>(),
+                         d_col_beg.as<int>(), d_val_beg.as<int64_t>(), d_pval_beg.as<int64_t>(),
+                         f64.opvals.as<double>(), f32.opvals_p.as<float>());
     HIPCHK(hipGetLastError());
   }
   tune_spmv<double, 2>();
@@ -2124,28 +2197,50 @@ static void launch_translate_side(Engine &e, const ChunkDev &cl, int c0, int c1,
 #undef PBFMM_TRS
 }
 
+// variant >= 3: the FP32 operator's values in row pairs, at their own (even)
+// per-leaf offsets
 template <typename T, int NK>
-static NearArgs<T> near_args(Engine &e, const double *phi, const double *dphi, double *outN, double *outD) {
-  const T *vals = (NK == 2 ? far_of<T>(e).vals : far_of<T>(e).opvals).template as<T>();
+static NearArgs<T> near_args(Engine &e, const double *phi, const double *dphi, double *outN, double *outD,
+                             int variant) {
+  const bool paired = kPairedF32<T, NK> && variant >= 3;
+  const FarDev<T> &F = far_of<T>(e);
+  const T *vals = (NK == 2 ? F.vals : paired ? F.opvals_p : F.opvals).template as<T>();
+  const int64_t *vb = paired ? e.d_pval_beg.as<int64_t>() : e.d_val_beg.as<int64_t>();
   return NearArgs<T>{e.d_row_beg.as<int>(), e.d_rows.as<int>(), e.d_col_beg.as<int>(), e.d_cols.as<int>(),
-                     e.d_val_beg.as<int64_t>(), vals, phi, dphi, outN, outD, 0};
+                     vb, vals, phi, dphi, outN, outD, 0};
 }
 
 template <typename T, int NK>
 void Engine::launch_spmv(int variant, const double *phi, const double *dphi, double *outN, double *outD, int R,
                          hipStream_t st) {
-  NearArgs<T> a = near_args<T, NK>(*this, phi, dphi, outN, outD);
+  NearArgs<T> a = near_args<T, NK>(*this, phi, dphi, outN, outD, variant);
   a.stride = n;  // R right-hand sides, n apart (blockIdx.y)
   if (!st) st = stream;
+  if constexpr (kPairedF32<T, NK>) {  // 3..5: the kernels of 0..2 on the row pairs
+    if (variant == 3) {
+      const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
+      hipLaunchKernelGGL((k_spmv_lds<T, NK, true>), dim3(per_xcd * kXcd, R), dim3(256), 0, st, near.nleaf, per_xcd,
+                         a);
+      return;
+    }
+    if (variant == 4 || variant == 5) {
+      if (variant == 4)
+        hipLaunchKernelGGL((k_spmv_wave<T, NK, 2, true>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, st,
+                           near.nleaf, a);
+      else
+        hipLaunchKernelGGL((k_spmv_wave<T, NK, 4, true>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, st,
+                           near.nleaf, a);
+      return;
+    }
+  }
+  variant %= 3;
   if (variant == 0) {
     const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
     hipLaunchKernelGGL((k_spmv_lds<T, NK>), dim3(per_xcd * kXcd, R), dim3(256), 0, st, near.nleaf, per_xcd, a);
   } else if (variant == 1) {
     hipLaunchKernelGGL((k_spmv_wave<T, NK, 2>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, st, near.nleaf, a);
-  } else if (variant == 2) {
-    hipLaunchKernelGGL((k_spmv_wave<T, NK, 4>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, st, near.nleaf, a);
   } else {
-    hipLaunchKernelGGL((k_spmv_wave8<T, NK>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, st, near.nleaf, a);
+    hipLaunchKernelGGL((k_spmv_wave<T, NK, 4>), dim3((near.nleaf + 3) / 4, R), dim3(256), 0, st, near.nleaf, a);
   }
 }
 
@@ -2160,7 +2255,7 @@ void Engine::tune_spmv() {
   HIPCHK(hipEventCreate(&e1));
   float best = 0;
   int pick = 0;
-  for (int v = 0; v < 4; ++v) {
+  for (int v = 0; v < (kPairedF32<T, NK> ? 6 : 3); ++v) {
     for (int rep = 0; rep < 2; ++rep)
       launch_spmv<T, NK>(v, d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
     HIPCHK(hipEventRecord(e0, stream));
@@ -2412,11 +2507,13 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
                        dim3(256), 0, stream, me, le, ne, acc, n);
   }
   if (me_lo < 0) consumer_levels();
-  const int var = spmv_variant[sizeof(T) == 4 ? 1 : 0][0];
+  const int var_full = spmv_variant[sizeof(T) == 4 ? 1 : 0][0];
+  const int var = var_full % 3;  // the kernel form; var_full >= 3 reads the row pairs
   const int nl = near.nleaf;
   const int64_t ni = (int64_t)far.m2p_item_node.size();
   SideArgs<T> sd{};
-  sd.near = near_args<T, 1>(*this, x, nullptr, out, nullptr);
+  // side-fused SpMV work reads the row-major values (its kernels' form)
+  sd.near = near_args<T, 1>(*this, x, nullptr, out, nullptr, var);
   sd.lo = 0;
   sd.count = nl;
   sd.per_xcd = (nl + kXcd - 1) / kXcd;
@@ -2444,12 +2541,12 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
     }
     HIPCHK(hipEventRecord(ev_fork, stream));
     HIPCHK(hipStreamWaitEvent(side_stream, ev_fork, 0));
-    launch_spmv<T, 1>(var, x, nullptr, out, nullptr, 1, side_stream);
+    launch_spmv<T, 1>(var_full, x, nullptr, out, nullptr, 1, side_stream);
     HIPCHK(hipEventRecord(ev_join, side_stream));
     near_done = join = true;
   }
   if (side == 0 && !near_done) {
-    launch_spmv<T, 1>(var, x, nullptr, out, nullptr);
+    launch_spmv<T, 1>(var_full, x, nullptr, out, nullptr);
     near_done = true;
   }
   // M2M, the deepest launched level carrying the SpMV (sides 1, 2)
@@ -2475,7 +2572,7 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   const T *mops = F.m2l.template as<T>();
   const bool m2p_here = !m2p_done && side >= 2, near_here = !near_done;
   if (slot) {
-    if (near_here) launch_spmv<T, 1>(var, x, nullptr, out, nullptr);
+    if (near_here) launch_spmv<T, 1>(var_full, x, nullptr, out, nullptr);
     near_done = true;
     launch_m2l_slots<T>(mops, me, le);
   } else if (near_here && var == 0)
