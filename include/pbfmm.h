@@ -293,7 +293,9 @@ int pbfmm_preconditioner_blocks(pbfmm_handle *h, int64_t *sizes, int32_t *row_be
  * the FP32 operator's cost, timed once on this handle; 1 relaxed (FP64
  * operator until the residual is below 2e5 * rel_tol * ||b||, then the FP32
  * operator, in one Krylov space); 2 iterative refinement (FP32-operator GMRES
- * on FP64 residuals).  Either scheme ends on the true FP64 residual. */
+ * on FP64 residuals, the second inner solve recycling the first one's Krylov
+ * space); 3 the same without recycling.  Every scheme ends on the true FP64
+ * residual. */
 int pbfmm_set_mixed_scheme(pbfmm_handle *h, int32_t scheme);
 
 /* The two thresholds of the mixed-precision schemes (no counterpart in the

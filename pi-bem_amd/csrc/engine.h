@@ -49,12 +49,14 @@ struct FarDev {
 };
 
 // Mixed precision: the relaxed scheme spends its first steps (about a
-// third) on the FP64 operator and saves the restarts of refinement;
-// refinement runs almost all steps on the FP32 operator.  Measured: relaxed
-// wins while the FP64 operator costs less than ~1.5x the FP32 one (24,578
-// DoFs, order 6: 1.23x, 10.5 ms against 12.2 ms) and loses above it (393,218
-// DoFs, order 8: 1.56x, 160.8 ms against 156.6 ms).
-constexpr double kRelaxCostRatio = 1.45;
+// third) on the FP64 operator; refinement runs its steps on the FP32 operator
+// and, with the first inner solve's Krylov space recycled into the second,
+// about as many of them as an FP64 solve, plus two FP64 residuals.  With N
+// steps, refinement is cheaper once the FP64 / FP32 cost ratio exceeds about
+// N / (N - 3): 1.1 for N ~ 40.  Measured: 24,578 DoFs, leaf 64, ratio 1.02:
+// relaxed 5.00 ms, refinement 5.21 ms; 393,218 DoFs, leaf 20, ratio 1.78:
+// refinement 85.6 ms, relaxed 98.0 ms.
+constexpr double kRelaxCostRatio = 1.15;
 
 struct Engine {
   hipStream_t stream = nullptr;
@@ -133,6 +135,8 @@ struct Engine {
   DevBuf d_rhs_sum;  // per right-hand side sums of the lockstep operator's mean term
   // GMRES workspace
   DevBuf d_V, d_w, d_r, d_x, d_h;
+  DevBuf d_rec, d_zr, d_qr;  // recycled Krylov space: old basis and C scratch, coefficients, Q
+  bool recycle = true;       // refinement's second inner solve recycles the first's space
   int gm_restart = 0;
   // timings of the last matvec (ms)
   double last_ms[9] = {0};
@@ -228,7 +232,7 @@ struct Engine {
   // Krylov space) when the FP64 operator is not much dearer than the FP32
   // one, else iterative refinement; the cost ratio is timed once
   double op_cost_ratio = 0;
-  int mixed_scheme = 0;  // 0 chosen by the timed cost ratio, 1 relaxed, 2 refinement
+  int mixed_scheme = 0;  // 0 chosen by the timed cost ratio, 1 relaxed, 2 refinement, 3 without recycling
   double inner_reduction = 0;  // refinement's inner residual reduction (0: the default)
   double relax_switch = 0;     // relaxed scheme's FP32 switch over tol ||b|| (0: the default)
   double inner_target(double tol, double bn, double rn) const;

@@ -1567,6 +1567,24 @@ __global__ void k_gemv_add(int64_t n, int64_t ldv, int nv, const double *__restr
   for (int j = 0; j < nv; ++j) acc += V[(int64_t)j * ldv + i] * y[j];
   x[i] = acc;
 }
+// out_j = sum_l V_l Q[j nv + l] for 8 outputs j = 8 blockIdx.y .. per thread
+// (vectors ld apart): the recycled Krylov space's C = V Q
+__global__ void __launch_bounds__(256) k_tsgemm8(int64_t n, int64_t ld, int nv, const double *__restrict__ V,
+                                                 const double *__restrict__ Q, int nout, double *__restrict__ out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j0 = 8 * blockIdx.y;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int l = 0; l < nv; ++l) {
+    const double v = V[(int64_t)l * ld + i];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (j0 + q < nout) acc[q] += v * Q[(int64_t)(j0 + q) * nv + l];
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (j0 + q < nout) out[(int64_t)(j0 + q) * ld + i] = acc[q];
+}
 __global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) y[i] = a * x[i] + b * y[i];
@@ -3026,13 +3044,37 @@ void Engine::step_op_graph(int precision) {
 // microseconds, an unused step costs a whole operator application.
 constexpr double kSpeculateMs = 0.5;
 
+// Krylov space recycled from one inner solve of the refinement into the next
+// (GCRO, de Sturler 1996; Parks et al. 2006, without the deflation step: the
+// whole previous space is kept).  The previous solve's last cycle satisfies
+// B V_k = V_k+1 Hbar (B the preconditioned FP32 operator, Hbar (k+1) x k);
+// with Hbar = Q R, C = V_k+1 Q is orthonormal and B (V_k R^-1) = C.  The next
+// solve projects its residual onto C, runs its Arnoldi process orthogonal to
+// C (C sits in front of the new basis, so the Gram-Schmidt kernels treat it
+// as k more basis vectors) and adds V_k R^-1 (C^T r - E y) to the correction,
+// E the new steps' coefficients on C: its residual is the minimum over the
+// old and the new space together, without applying the operator to the old
+// one again.
+struct Recycle {
+  bool record = false;        // keep the last cycle's Hessenberg (k, Hbar)
+  int k = 0;
+  std::vector<double> Hbar;   // column j at (m + 1) j, zero below row j + 1
+  int kc = 0;                 // > 0: C in d_V[0 .. kc), V_kc in d_rec, R
+  std::vector<double> R;      // kc x kc upper triangular, row-major
+};
+
 // r0 (device) with norm r0n, when given, is the residual b - A x of the
 // starting x: the first cycle then skips its operator application.
 static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int m, int maxit, int precision,
-                      int &it, double &est, double switch_abs = -1.0, const double *r0 = nullptr, double r0n = 0) {
+                      int &it, double &est, double switch_abs = -1.0, const double *r0 = nullptr, double r0n = 0,
+                      Recycle *rc = nullptr) {
   const int64_t n = e.n;
   const unsigned g = nblk(n, 256);
   double *V = e.d_V.as<double>(), *w = e.d_w.as<double>(), *hd = e.d_h.as<double>();
+  // recycled vectors in front of the basis (first cycle only)
+  int kc = rc && rc->kc > 0 && rc->kc < m ? rc->kc : 0;
+  std::vector<double> Ecol, cproj;  // coefficients on C of each step; C^T r
+  if (rc && rc->record) rc->k = 0;
   if (!e.d_cgs.p) alloc(e.d_cgs, (size_t)kDotParts * 128 * 8);
   double *cg = e.d_cgs.as<double>();
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1), h(m + 2), y(m);
@@ -3061,25 +3103,28 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
     if (e.d_vn.bytes < (size_t)n * 8) alloc(e.d_vn, (size_t)n * 8);
     if (e.d_wk.bytes < (size_t)n * 8) alloc(e.d_wk, (size_t)n * 8);
   }
-  // Arnoldi step k: V_k+1 from V_k, its Hessenberg column into slot k
+  // Arnoldi step k: V_k+1 from V_k (the basis after the kc recycled vectors),
+  // its Hessenberg column (the coefficients on C first) into slot k
   auto enqueue_step = [&](int k) {
-    double *vk1 = V + (int64_t)(k + 1) * n;
+    double *Vb = V + (int64_t)kc * n;
+    double *vk1 = Vb + (int64_t)(k + 1) * n;
     const int pk = step_prec();
     double *wv = vk1;  // the new vector before its normalisation
     if (graphs) {
       e.step_op_graph(pk);  // d_vn (= V_k) -> d_wk
       wv = e.d_wk.as<double>();
     } else {
-      e.step_op(V + (int64_t)k * n, vk1, pk);
+      e.step_op(Vb + (int64_t)k * n, vk1, pk);
     }
     ++it;
     // CGS2: two projections and the norm, all on the device, six launches:
     // the projections and the scaling finish the dot products they use
     double *slot = hd + (size_t)k * S;
     double *h1 = slot, *h2 = slot + (m + 2), *hn_d = slot + 2 * (m + 2);
+    const int nv = kc + k + 1;
     for (double *hj : {h1, h2}) {
-      hipLaunchKernelGGL(k_mdot_parts, dim3(kDotParts, k + 1), dim3(kRedThreads), 0, e.stream, n, n, V, wv, cg);
-      hipLaunchKernelGGL(k_gemv_sub_fin, dim3(g), dim3(256), 0, e.stream, n, n, k + 1, V, cg, kDotParts, hj, wv);
+      hipLaunchKernelGGL(k_mdot_parts, dim3(kDotParts, nv), dim3(kRedThreads), 0, e.stream, n, n, V, wv, cg);
+      hipLaunchKernelGGL(k_gemv_sub_fin, dim3(g), dim3(256), 0, e.stream, n, n, nv, V, cg, kDotParts, hj, wv);
     }
     hipLaunchKernelGGL(k_mdot_parts, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, wv, wv, cg);
     if (graphs)
@@ -3102,8 +3147,35 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
     }
     est = beta;
     if (beta <= tol_abs) return 0;
-    hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, rv, 1.0 / beta, V);
-    if (graphs) hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, rv, 1.0 / beta, e.d_vn.as<double>());
+    if (kc > 0) {
+      // r - C (C^T r), twice, then normalised into the basis' first vector
+      // (and the graph's input); C^T r and the norm in the last step slot,
+      // which this cycle's kc + steps <= m never reaches
+      if (e.d_vn.bytes < (size_t)n * 8) alloc(e.d_vn, (size_t)n * 8);
+      double *ps = hd + (size_t)(m - 1) * S;
+      HIPCHK(hipMemcpyAsync(w, rv, (size_t)n * 8, hipMemcpyDeviceToDevice, e.stream));
+      for (double *pj : {ps, ps + (m + 2)}) {
+        hipLaunchKernelGGL(k_mdot_parts, dim3(kDotParts, kc), dim3(kRedThreads), 0, e.stream, n, n, V, w, cg);
+        hipLaunchKernelGGL(k_gemv_sub_fin, dim3(g), dim3(256), 0, e.stream, n, n, kc, V, cg, kDotParts, pj, w);
+      }
+      hipLaunchKernelGGL(k_mdot_parts, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, w, w, cg);
+      hipLaunchKernelGGL(k_scale_fin2, dim3(g), dim3(256), 0, e.stream, n, w, V + (int64_t)kc * n,
+                         e.d_vn.as<double>(), cg, kNormParts, ps + 2 * (m + 2));
+      std::vector<double> pr((size_t)S);
+      HIPCHK(hipMemcpyAsync(pr.data(), ps, (size_t)S * 8, hipMemcpyDeviceToHost, e.stream));
+      e.sync();
+      cproj.assign(kc, 0.0);
+      for (int i = 0; i < kc; ++i) cproj[i] = pr[i] + pr[(m + 2) + i];
+      beta = pr[2 * (m + 2)];
+      est = beta;
+      Ecol.clear();
+    } else {
+      hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, rv, 1.0 / beta, V);
+      if (graphs)
+        hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, rv, 1.0 / beta, e.d_vn.as<double>());
+    }
+    if (rc && rc->record) rc->Hbar.clear();
+    const int mk = m - kc;  // steps this cycle
     std::fill(gv.begin(), gv.end(), 0.0);
     gv[0] = beta;
     // kq: steps enqueued; k_done: columns reduced on the host; the next host
@@ -3115,8 +3187,8 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
     bool done = false, speculate = true;
     auto t_trip = std::chrono::steady_clock::now();
     while (true) {
-      if (kq < m && it < maxit) enqueue_step(kq++);
-      const bool more = kq < m && it < maxit;
+      if (kq < mk && it < maxit) enqueue_step(kq++);
+      const bool more = kq < mk && it < maxit;
       if (kq - 1 < k_sync && more) continue;
       if (k_done == kq) break;  // every enqueued column reduced and no step left
       // host round trip: the columns k_done..kq-1, Givens rotations, residual estimate
@@ -3130,9 +3202,14 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
       const int j0 = k_done;
       for (int j = k_done; j < kk; ++j) {
         const double *hb = hbuf + (size_t)j * S;
-        for (int i = 0; i <= j; ++i) h[i] = hb[i] + hb[(m + 2) + i];
+        for (int i = 0; i < kc; ++i) Ecol.push_back(hb[i] + hb[(m + 2) + i]);
+        for (int i = 0; i <= j; ++i) h[i] = hb[kc + i] + hb[(m + 2) + kc + i];
         const double hn = hb[2 * (m + 2)];
         h[j + 1] = hn;
+        if (rc && rc->record) {  // the column before the rotations, padded to m + 1
+          for (int i = 0; i <= j + 1; ++i) rc->Hbar.push_back(h[i]);
+          for (int i = j + 2; i <= m; ++i) rc->Hbar.push_back(0.0);
+        }
         for (int i = 0; i < j; ++i) {
           const double a = h[i], bb = h[i + 1];
           h[i] = cs[i] * a + sn[i] * bb;
@@ -3183,20 +3260,91 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
       y[i] = s / H[(size_t)i * m + i];
     }
     HIPCHK(hipMemcpyAsync(ybuf, y.data(), k * 8, hipMemcpyHostToDevice, e.stream));
+    // recycled part: V_kc R^-1 (C^T r - E y)
+    std::vector<double> zr;
+    if (kc > 0) {
+      std::vector<double> a(cproj);
+      for (int j = 0; j < k; ++j)
+        for (int i = 0; i < kc; ++i) a[i] -= Ecol[(size_t)j * kc + i] * y[j];
+      zr.assign(kc, 0.0);
+      for (int i = kc - 1; i >= 0; --i) {
+        double s = a[i];
+        for (int j = i + 1; j < kc; ++j) s -= rc->R[(size_t)i * kc + j] * zr[j];
+        zr[i] = s / rc->R[(size_t)i * kc + i];
+      }
+      if (e.d_zr.bytes < (size_t)kc * 8) alloc(e.d_zr, (size_t)std::max(kc, 16) * 8);
+      HIPCHK(hipMemcpyAsync(e.d_zr.p, zr.data(), (size_t)kc * 8, hipMemcpyHostToDevice, e.stream));
+    }
+    const double *Vb = V + (int64_t)kc * n;
     if (e.have_prec) {
-      // x += M^-1 (V y)
+      // x += M^-1 (V y [+ V_kc z])
       double *t = e.d_ptmp2.as<double>(), *z = e.d_ptmp.as<double>();
       hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, e.stream, t, n, 0.0);
-      hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, V, ybuf, t);
+      hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, Vb, ybuf, t);
+      if (kc > 0)
+        hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, kc, e.d_rec.as<double>(),
+                           e.d_zr.as<double>(), t);
       e.apply_prec(t, z);
       hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, z, 1.0, x);
     } else {
-      hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, V, ybuf, x);
+      hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, Vb, ybuf, x);
+      if (kc > 0)
+        hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, kc, e.d_rec.as<double>(),
+                           e.d_zr.as<double>(), x);
     }
+    if (rc && rc->record) rc->k = k;
     e.sync();
     if (done) return 0;
+    kc = 0;  // later cycles restart plainly
   }
   return 1;
+}
+
+// Sets up the recycled space for the next inner solve from the one just
+// finished (rec.k steps, Hbar, its basis still in d_V): Hbar = Q R by
+// Gram-Schmidt with reorthogonalisation on the host, the old V_k copied to
+// d_rec, C = V_k+1 Q formed beside it and moved to the front of d_V.  Returns
+// false (no recycling) when R is numerically singular.
+static bool prepare_recycle(Engine &e, Recycle &rec, int m) {
+  const int k = rec.k;
+  if (k <= 0 || k >= m) return false;
+  const int64_t n = e.n;
+  std::vector<double> Q((size_t)(k + 1) * k, 0.0), R((size_t)k * k, 0.0);
+  double rmax = 0;
+  for (int j = 0; j < k; ++j) {
+    double *q = Q.data() + (size_t)j * (k + 1);
+    for (int i = 0; i <= k; ++i) q[i] = rec.Hbar[(size_t)j * (m + 1) + i];
+    for (int pass = 0; pass < 2; ++pass)
+      for (int l = 0; l < j; ++l) {
+        const double *ql = Q.data() + (size_t)l * (k + 1);
+        double dot = 0;
+        for (int i = 0; i <= k; ++i) dot += ql[i] * q[i];
+        for (int i = 0; i <= k; ++i) q[i] -= dot * ql[i];
+        R[(size_t)l * k + j] += dot;
+      }
+    double nr = 0;
+    for (int i = 0; i <= k; ++i) nr += q[i] * q[i];
+    nr = std::sqrt(nr);
+    R[(size_t)j * k + j] = nr;
+    rmax = std::max(rmax, nr);
+    if (!(nr > 0)) return false;
+    for (int i = 0; i <= k; ++i) q[i] /= nr;
+  }
+  for (int j = 0; j < k; ++j)
+    if (!(R[(size_t)j * k + j] > 1e-12 * rmax)) return false;
+  if (e.d_rec.bytes < (size_t)2 * k * n * 8) alloc(e.d_rec, (size_t)2 * k * n * 8);
+  if (e.d_qr.bytes < Q.size() * 8) alloc(e.d_qr, Q.size() * 8);
+  double *V = e.d_V.as<double>(), *U = e.d_rec.as<double>(), *Ct = U + (int64_t)k * n;
+  HIPCHK(hipMemcpyAsync(U, V, (size_t)k * n * 8, hipMemcpyDeviceToDevice, e.stream));
+  HIPCHK(hipMemcpyAsync(e.d_qr.p, Q.data(), Q.size() * 8, hipMemcpyHostToDevice, e.stream));
+  hipLaunchKernelGGL(k_tsgemm8, dim3(nblk(n, 256), (k + 7) / 8), dim3(256), 0, e.stream, n, n, k + 1, V,
+                     e.d_qr.as<double>(), k, Ct);
+  HIPCHK(hipMemcpyAsync(V, Ct, (size_t)k * n * 8, hipMemcpyDeviceToDevice, e.stream));
+  HIPCHK(hipGetLastError());
+  rec.kc = k;
+  rec.R = std::move(R);
+  rec.record = false;
+  return true;
 }
 
 // Largest residual reduction asked of one FP32-operator inner solve of the
@@ -3296,6 +3444,8 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
   int it = 0;
   double est = 0, rr = 1;
   double *r = d_r.as<double>(), *d = d_x.as<double>();
+  Recycle rec;  // the refinement's first inner solve, recycled into the second
+  int stage = 0;
   for (int outer = 0; outer < 100 && it < maxit; ++outer) {
     // true FP64 residual (b itself for a zero starting guess)
     double rn = bn;
@@ -3320,9 +3470,20 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
         gmres_core(*this, r, d, tol * bn * 0.5, restart, maxit, PBFMM_MIXED, it, est,
                    std::min(rn, relax_sw() * tol * bn), r, rn);
       } else {
-        // iterative refinement: FP32-operator GMRES on the FP64 residual
+        // iterative refinement: FP32-operator GMRES on the FP64 residual;
+        // the second inner solve recycles the first one's Krylov space
         const double tin = inner_target(tol, bn, rn);
-        gmres_core(*this, r, d, tin, restart, maxit, PBFMM_MIXED, it, est, -1.0, r, rn);
+        Recycle *rp = nullptr;
+        const bool recyc = recycle && mixed_scheme != 3;  // 3: refinement without recycling
+        if (recyc && stage == 0) {
+          rec = Recycle();
+          rec.record = true;
+          rp = &rec;
+        } else if (recyc && stage == 1 && prepare_recycle(*this, rec, restart)) {
+          rp = &rec;
+        }
+        gmres_core(*this, r, d, tin, restart, maxit, PBFMM_MIXED, it, est, -1.0, r, rn, rp);
+        ++stage;
       }
       hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, d, 1.0, x);
     } else {

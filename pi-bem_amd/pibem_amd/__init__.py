@@ -107,7 +107,9 @@ class BEMFMA:
     def set_mixed_scheme(self, scheme=0):
         """Mixed-precision solver scheme: 0 chosen by the timed FP64/FP32
         operator cost ratio, 1 relaxed (FP64 then FP32 operator in one Krylov
-        space), 2 iterative refinement (pbfmm_set_mixed_scheme)."""
+        space), 2 iterative refinement (its second inner solve recycling the
+        first's Krylov space), 3 refinement without recycling
+        (pbfmm_set_mixed_scheme)."""
         self._chk(self._L.pbfmm_set_mixed_scheme(self._h, int(scheme)))
 
     def set_mixed_tolerances(self, inner_reduction=0.0, relax_switch=0.0):
