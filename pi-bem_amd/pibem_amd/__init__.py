@@ -565,13 +565,15 @@ def tune_leaf_size(problem, precision=PBFMM_MIXED, solves=2, start=20, ladder=LE
     return best, times, info, bracketed
 
 
-def leaf_walk(measure, start=20, ladder=LEAF_LADDER):
+def leaf_walk(measure, start=20, ladder=LEAF_LADDER, gain=0.01):
     """The tuner's walk over the ladder: measure(leaf) -> cost (each leaf is
     measured once here).  From start it measures both neighbours, then steps
-    towards the cheaper side until two leaf sizes in a row cost no less than
-    the best so far (one more step than needed, so that a plateau or a noisy
-    point does not end the walk early).  Returns (best leaf, bracketed: both
-    ladder neighbours of the best were measured and cost more)."""
+    towards the cheaper side until two leaf sizes in a row do not beat the
+    best so far (one more step than needed, so that a plateau or a noisy
+    point does not end the walk early).  A leaf size replaces the best only
+    when it is cheaper by more than `gain` (1%), so timing noise does not
+    move the choice.  Returns (best leaf, bracketed: both ladder neighbours
+    of the best were measured and cost more)."""
     lad = list(ladder)
     cost = {}
 
@@ -580,17 +582,20 @@ def leaf_walk(measure, start=20, ladder=LEAF_LADDER):
             cost[leaf] = measure(leaf)
         return cost[leaf]
 
+    def beats(leaf, best):
+        return m(leaf) < cost[best] * (1.0 - gain)
+
     k = lad.index(start) if start in lad else 0
     m(lad[k])
+    best = lad[k]
     for j in (k - 1, k + 1):
-        if 0 <= j < len(lad):
-            m(lad[j])
-    best = min(cost, key=cost.get)
+        if 0 <= j < len(lad) and m(lad[j]) < cost[best] and beats(lad[j], lad[k]):
+            best = lad[j]
     kb = lad.index(best)
     step = 0 if kb == k else (1 if kb > k else -1)
     j, misses = kb + step, 0
     while step and 0 <= j < len(lad) and misses < 2:
-        if m(lad[j]) < cost[best]:
+        if beats(lad[j], best):
             best, kb, misses = lad[j], j, 0
         else:
             misses += 1

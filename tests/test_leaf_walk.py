@@ -45,3 +45,10 @@ def test_walk_crosses_a_plateau():
     cost = {16: 9, 20: 8, 24: 7, 32: 6, 40: 6, 48: 5, 64: 4, 80: 4.5, 96: 5}
     best, bracketed, seen = _walk(lambda l: cost.get(l, 10))
     assert best == 64 and bracketed and 128 not in seen
+
+
+def test_noise_does_not_move_the_choice():
+    # 80 is 0.5% cheaper than 64: within the 1% gain the walk asks, 64 stays
+    cost = {16: 96.6, 20: 86.1, 24: 81.6, 32: 76.9, 40: 67.0, 48: 62.3, 64: 55.12, 80: 54.83, 96: 56.3, 128: 64.6}
+    best, bracketed, seen = _walk(lambda l: cost.get(l, 99.0))
+    assert best == 64 and bracketed and 96 in seen
