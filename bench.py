@@ -142,8 +142,9 @@ def roofline_of(phase, model, ms, prec, name):
 
 
 def traffic_for(kernel, workload):
-    """HBM bytes per launch from the committed rocprofv3 counter summary of
-    this kernel and workload (profiles/traffic_*.json), or None."""
+    """HBM bytes per launch of this kernel from the committed rocprofv3
+    counter summary of the workload (profiles/traffic_*.json: the workload
+    and, per kernel, its bytes), or None."""
     d = os.path.join(ROOT, "profiles")
     if not os.path.isdir(d):
         return None
@@ -153,8 +154,8 @@ def traffic_for(kernel, workload):
                 t = json.load(open(os.path.join(d, f)))
             except (OSError, ValueError):
                 continue
-            if t.get("kernel") == kernel and t.get("workload") == workload:
-                return t.get("traffic_bytes")
+            if t.get("workload") == workload and kernel in t.get("kernels", {}):
+                return t["kernels"][kernel].get("traffic_bytes")
     return None
 
 
