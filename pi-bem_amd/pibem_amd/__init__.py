@@ -568,12 +568,12 @@ def tune_leaf_size(problem, precision=PBFMM_MIXED, solves=2, start=20, ladder=LE
 def leaf_walk(measure, start=20, ladder=LEAF_LADDER, gain=0.01):
     """The tuner's walk over the ladder: measure(leaf) -> cost (each leaf is
     measured once here).  From start it measures both neighbours, then steps
-    towards the cheaper side until two leaf sizes in a row do not beat the
-    best so far (one more step than needed, so that a plateau or a noisy
-    point does not end the walk early).  A leaf size replaces the best only
-    when it is cheaper by more than `gain` (1%), so timing noise does not
-    move the choice.  Returns (best leaf, bracketed: both ladder neighbours
-    of the best were measured and cost more)."""
+    towards the cheaper one until three leaf sizes in a row do not beat the
+    best so far (so that a plateau or a noisy point does not end the walk
+    early).  A leaf size replaces the best only when it is cheaper by more
+    than `gain` (1%), so timing noise does not move the choice.  Returns
+    (best leaf, bracketed: both ladder neighbours of the best were measured
+    and cost more)."""
     lad = list(ladder)
     cost = {}
 
@@ -587,14 +587,14 @@ def leaf_walk(measure, start=20, ladder=LEAF_LADDER, gain=0.01):
 
     k = lad.index(start) if start in lad else 0
     m(lad[k])
-    best = lad[k]
-    for j in (k - 1, k + 1):
-        if 0 <= j < len(lad) and m(lad[j]) < cost[best] and beats(lad[j], lad[k]):
-            best = lad[j]
-    kb = lad.index(best)
-    step = 0 if kb == k else (1 if kb > k else -1)
-    j, misses = kb + step, 0
-    while step and 0 <= j < len(lad) and misses < 2:
+    nbr = [j for j in (k - 1, k + 1) if 0 <= j < len(lad)]
+    for j in nbr:
+        m(lad[j])
+    jc = min(nbr, key=lambda j: cost[lad[j]]) if nbr else k
+    step = (1 if jc > k else -1) if cost[lad[jc]] < cost[lad[k]] else 0
+    best, kb = lad[k], k
+    j, misses = k + step, 0
+    while step and 0 <= j < len(lad) and misses < 3:
         if beats(lad[j], best):
             best, kb, misses = lad[j], j, 0
         else:

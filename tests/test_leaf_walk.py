@@ -32,19 +32,26 @@ def test_monotone_costs_end_of_ladder():
     assert best == LEAF_LADDER[-1] and not bracketed
 
 
-def test_walk_stops_after_two_rises():
-    # two valleys: the walk from 20 takes the nearer one and stops two leaf
-    # sizes past it
+def test_walk_stops_after_three_misses():
+    # two valleys: the walk from 20 takes the nearer one and stops at the
+    # ladder's end (three misses would be needed to stop earlier)
     cost = {8: 5, 12: 4, 16: 3, 20: 4, 24: 5, 32: 6, 40: 7, 48: 1, 64: 2}
     best, bracketed, seen = _walk(lambda l: cost.get(l, 9))
     assert best == 16 and bracketed and 48 not in seen and 8 in seen
+
+
+def test_walk_crosses_a_flat_start():
+    # the neighbours of 20 are within the 1% gain, the real minimum is at 48
+    cost = {16: 14.98, 20: 14.04, 24: 13.99, 32: 14.25, 40: 13.06, 48: 12.91, 64: 13.06, 80: 13.84, 96: 14.5}
+    best, bracketed, seen = _walk(lambda l: cost.get(l, 15.0))
+    assert best == 48 and bracketed
 
 
 def test_walk_crosses_a_plateau():
     # equal costs at 32 and 40 do not end the walk: 64 is found
     cost = {16: 9, 20: 8, 24: 7, 32: 6, 40: 6, 48: 5, 64: 4, 80: 4.5, 96: 5}
     best, bracketed, seen = _walk(lambda l: cost.get(l, 10))
-    assert best == 64 and bracketed and 128 not in seen
+    assert best == 64 and bracketed and 160 not in seen
 
 
 def test_noise_does_not_move_the_choice():
