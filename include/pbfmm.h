@@ -324,8 +324,10 @@ int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel);
  * plans 0 and 1 with M2L storing each pair's result into a slot of its own,
  * summed per target by a second kernel, instead of adding it to the target
  * with atomics; 8 and 12: plans 0 and 4 with the SpMV on a second stream of
- * the library, beside the whole FMM; -1 times them all again on the next
- * application).  For tests and tuning.  No counterpart in the reference. */
+ * the library, beside the whole FMM; 24 and 28: plans 8 and 12 with the
+ * second stream forked after the M2M levels, beside M2L onwards; -1 times
+ * them all again on the next application).  For tests and tuning.  No
+ * counterpart in the reference. */
 int pbfmm_set_side_fusion(pbfmm_handle *h, int32_t precision, int32_t mode);
 
 /* The choices this handle's timings made so far (-1 = not timed yet):
@@ -335,7 +337,8 @@ int pbfmm_set_side_fusion(pbfmm_handle *h, int32_t precision, int32_t mode);
  *         1 SpMV beside an M2M level and M2P beside an L2L level, 2 SpMV
  *         beside an M2M level and M2P beside M2L, 3 both beside M2L, 4 and 5
  *         plans 0 and 1 with M2L into per-pair slots summed per target, 8
- *         and 12 plans 0 and 4 with the SpMV on a second stream
+ *         and 12 plans 0 and 4 with the SpMV on a second stream, 24 and 28
+ *         the same forked after the M2M levels
  * [4]     mixed-precision scheme: 1 relaxed, 2 iterative refinement
  * [5]     timed FP64 / FP32 operator cost ratio x 1000
  * [6] [7] the GMRES step's operator, FP64 / FP32 steps: 1 replayed as a HIP

@@ -174,7 +174,7 @@ struct Engine {
   // per-target sum (k_translate<., ., 3, 1> + k_m2l_reduce) instead of atomic
   // adds into the targets; bit 3 (plans 8, 12 with placement 0) the SpMV on a
   // second stream beside the whole FMM, joined before L2P.
-  static constexpr int kPlans[] = {0, 1, 2, 3, 4, 5, 8, 12};
+  static constexpr int kPlans[] = {0, 1, 2, 3, 4, 5, 8, 12, 24, 28};
   static bool valid_plan(int m) {
     for (int v : kPlans)
       if (v == m) return true;

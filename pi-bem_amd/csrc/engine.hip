@@ -2503,6 +2503,7 @@ template <typename T>
 void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   FarDev<T> &F = far_of<T>(*this);
   const bool async = (side & 8) != 0;  // SpMV on the second stream
+  const bool late = (side & 16) != 0;  // ... forked after the M2M levels, beside M2L onwards
   const bool slot = (side & 4) != 0;   // M2L into per-pair slots, summed per target
   side &= 3;
   const int nleaf = (int)far.p2m_leaf.size();
@@ -2549,9 +2550,11 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   sd.m2p_out = acc;
   bool near_done = nl == 0, m2p_done = ni == 0;
   bool join = false;
-  if (async && !near_done) {
-    // the SpMV reads only x, ready when this application starts, and writes
-    // out, which only the final L2P reads: it runs beside the whole FMM
+  // the SpMV reads only x, ready when this application starts, and writes
+  // out, which only the final L2P reads: on the second stream it runs beside
+  // the FMM, from the start or (late) from M2L on, leaving the M2M levels'
+  // latency-bound launches the whole chip
+  auto fork_spmv = [&]() {
     if (!side_stream) {
       HIPCHK(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));
       HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
@@ -2562,8 +2565,10 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
     launch_spmv<T, 1>(var_full, x, nullptr, out, nullptr, 1, side_stream);
     HIPCHK(hipEventRecord(ev_join, side_stream));
     near_done = join = true;
-  }
-  if (side == 0 && !near_done) {
+  };
+  if (async && !late && !near_done) fork_spmv();
+  const bool fork_later = async && late && !near_done;
+  if (side == 0 && !near_done && !fork_later) {
     launch_spmv<T, 1>(var_full, x, nullptr, out, nullptr);
     near_done = true;
   }
@@ -2572,7 +2577,7 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
     const int c0 = lvl<1>(far.m2m, l), c1 = lvl<1>(far.m2m, l + 1);
     if (c1 <= c0) continue;
     const T *ops = F.m2m.template as<T>();
-    if (near_done || side == 3) {
+    if (near_done || side == 3 || fork_later) {
       launch_translate<T, 0, 1>(*this, stream, m2m_cl, c0, c1, ops, me, me);
       continue;
     }
@@ -2584,6 +2589,7 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
       launch_translate_side<T, 0, 2, false>(*this, m2m_cl, c0, c1, ops, me, me, sd);
     near_done = true;
   }
+  if (fork_later) fork_spmv();
   // M2L, carrying M2P (sides 2, 3) and the SpMV (side 3, or whatever no M2M
   // launch took)
   const int c1 = nchunks<1>(far.m2l);
