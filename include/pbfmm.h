@@ -166,8 +166,9 @@ int pbfmm_get_constraints(pbfmm_handle *h, int64_t *nnz, int64_t *c_ptr, int64_t
  * (init_preconditioner, bem_fma.cc:919-975), alpha added on the diagonal of
  * unconstrained Neumann rows, constrained rows replaced by their constraint
  * (with the operator's sign, -c_ij).  The reference factors it with ILU(0);
- * here the rows of the leaves that share a parent block (at most 48 rows per
- * block, larger parents cut into even parts) form dense blocks that are
+ * here the rows of the leaves that share a parent block (at most 128 rows per
+ * block by default, pbfmm_set_preconditioner_rows; larger parents cut into
+ * even parts) form dense blocks that are
  * inverted (block Jacobi), applied as right preconditioner by the GMRES of
  * solve_system and gmres_dev.  alpha NULL uses the handle's alpha (computed
  * if needed). */
@@ -287,6 +288,11 @@ int pbfmm_work_counts(pbfmm_handle *h, int64_t *counts);
  * matrix, row-major, in values (blocks one after another).  Builds the
  * preconditioner if needed. */
 int pbfmm_preconditioner_blocks(pbfmm_handle *h, int64_t *sizes, int32_t *row_beg, int32_t *rows, double *values);
+/* Row cap of the block-Jacobi blocks (8..128, default 128): the near leaves
+ * of one parent block are grouped up to this many rows (a larger leaf stays
+ * alone).  Rebuilds the preconditioner if it exists.  No counterpart in the
+ * reference (its ILU has no blocks). */
+int pbfmm_set_preconditioner_rows(pbfmm_handle *h, int32_t rows);
 
 /* Mixed-precision solver scheme (no counterpart in the reference, whose
  * solve is FP64 throughout): 0 (default) chooses by the ratio of the FP64 to

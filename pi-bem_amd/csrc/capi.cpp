@@ -555,6 +555,16 @@ int pbfmm_fma_preconditioner(pbfmm_handle *h, const double *alpha) {
   });
 }
 
+int pbfmm_set_preconditioner_rows(pbfmm_handle *h, int32_t rows) {
+  return guarded(h, [&]() -> int {
+    if (rows < 8 || rows > 128) return fail(h, PBFMM_ERR_ARG, "set_preconditioner_rows: rows must be 8..128");
+    Engine &e = h->eng;
+    e.prec_rows = rows;
+    if (e.have_prec) e.build_preconditioner();
+    return PBFMM_OK;
+  });
+}
+
 int pbfmm_vmult(pbfmm_handle *h, const double *src, double *dst) {
   return guarded(h, [&]() -> int {
     if (int rc = need(h, 4, "vmult")) return rc;

@@ -113,16 +113,18 @@ struct Engine {
   std::vector<double> hc_val, hc_inh;
   std::vector<uint8_t> hc_is;
   // near-field block-Jacobi preconditioner: blocks of the near leaves that
-  // share a parent block (at most kPrecRows rows, leaf after leaf; a larger
+  // share a parent block (at most prec_rows rows, leaf after leaf; a larger
   // leaf alone), per block the inverse of its rows x rows near-field block,
   // column-major, at pinv_beg[block]; prow_beg / prows / prow_blk: the
   // blocks' row ranges, rows in block order, block of each of them
-  // Row cap of a preconditioner block (k_bjacobi_wg takes up to 64, a lane
-  // per row).  Measured GMRES to 1e-8, mixed / FP64, at 48 / 64 / 40 rows:
-  // 24,578 DoFs (order 6) 6.43 / 6.54 / 6.71 ms and 7.54 / 7.72 / 7.92 ms;
-  // 393,218 DoFs (Q2, order 8) 115.0 / 113.5 / 119.5 ms and 150.4 / 145.7 /
-  // 152.8 ms.
-  static constexpr int kPrecRows = 48;
+  // Row cap of a preconditioner block (k_bjacobi_wg takes up to 128, one or
+  // two rows per lane; pbfmm_set_preconditioner_rows).  Larger blocks cut
+  // GMRES steps and cost bytes per application (n x rows values).  Measured
+  // at 393,218 DoFs (Q2, order 8), GMRES to 1e-8, mixed / FP64 ms (steps):
+  // leaf size 64, 32 / 48 / 64 / 96 / 128 rows: 55.4 (59) / 54.5 (58) / 54.6
+  // (58) / 52.4 (55) / 52.0 (54) and 91.7 / 90.0 / 90.1 / 87.5 / 86.5; leaf
+  // 20 (siblings hold at most ~64 rows): 91.6 / 86.5 / 84.4 / 84.5 / 84.6.
+  int prec_rows = 128;
   bool have_prec = false;
   int nprec = 0;
   std::vector<int64_t> pinv_beg;

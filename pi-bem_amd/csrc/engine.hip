@@ -1488,24 +1488,40 @@ __global__ void __launch_bounds__(256) k_bjacobi(int nrows, const int *__restric
 // from their lanes; the four partial sums meet in LDS.  Load chains are R/4
 // long, and there are four times the wavefronts in flight.
 // (right-hand side blockIdx.y: v and z + y stride)
+template <int RPL>
 __global__ void __launch_bounds__(256) k_bjacobi_wg(const int *__restrict__ row_beg, const int *__restrict__ rows,
                                                     const int64_t *__restrict__ blk_beg,
                                                     const double *__restrict__ pinv, const double *__restrict__ v,
                                                     double *__restrict__ z, int64_t stride) {
-  __shared__ double part[4][64];
+  // RPL rows per lane: blocks of up to 64 RPL rows
+  __shared__ double part[4][64 * RPL];
   const int blk = blockIdx.x;
   v += (int64_t)blockIdx.y * stride;
   z += (int64_t)blockIdx.y * stride;
   const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
   const int r0 = row_beg[blk], R = row_beg[blk + 1] - r0;
-  const double xv = lane < R ? v[rows[r0 + lane]] : 0.0;
-  const double *P = pinv + blk_beg[blk] + (lane < R ? lane : 0);  // column-major block
-  double acc = 0.0;
+  double xv[RPL], acc[RPL];
+  const double *P[RPL];
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) {
+    const int row = lane + 64 * q;
+    xv[q] = row < R ? v[rows[r0 + row]] : 0.0;
+    P[q] = pinv + blk_beg[blk] + (row < R ? row : 0);  // column-major block
+    acc[q] = 0.0;
+  }
 #pragma unroll 4
-  for (int c = w; c < R; c += 4) acc += (lane < R ? P[(int64_t)c * R] : 0.0) * read_lane(xv, c);
-  part[w][lane] = acc;
+  for (int c = w; c < R; c += 4) {
+    const double xc = RPL == 1 || c < 64 ? read_lane(xv[0], c & 63) : read_lane(xv[RPL - 1], c & 63);
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) acc[q] += (lane + 64 * q < R ? P[q][(int64_t)c * R] : 0.0) * xc;
+  }
+#pragma unroll
+  for (int q = 0; q < RPL; ++q) part[w][lane + 64 * q] = acc[q];
   __syncthreads();
-  if (w == 0 && lane < R) z[rows[r0 + lane]] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+  if (w < RPL) {
+    const int row = lane + 64 * w;
+    if (row < R) z[rows[r0 + row]] = (part[0][row] + part[1][row]) + (part[2][row] + part[3][row]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2861,7 +2877,7 @@ void Engine::prec_blocks(std::vector<double> &blk) {
 
 void Engine::build_preconditioner() {
   // blocks: near leaves with the same parent block, in near-leaf order; a
-  // parent of more than kPrecRows rows is cut into ceil(rows / kPrecRows)
+  // parent of more than prec_rows rows is cut into ceil(rows / prec_rows)
   // blocks of about equal rows
   const int nl = near.nleaf;
   std::vector<int> order(nl);
@@ -2876,11 +2892,11 @@ void Engine::build_preconditioner() {
     int total = 0;
     for (int k = i; k < nl && parent_of(order[k]) == par; ++k)
       total += near.row_beg[order[k] + 1] - near.row_beg[order[k]];
-    const int parts = (total + kPrecRows - 1) / kPrecRows, target = (total + parts - 1) / parts;
+    const int parts = (total + prec_rows - 1) / prec_rows, target = (total + parts - 1) / parts;
     int rows_in = 0;
     for (; i < nl && parent_of(order[i]) == par; ++i) {
       const int l = order[i], R = near.row_beg[l + 1] - near.row_beg[l];
-      if (rows_in > 0 && (rows_in + R > kPrecRows || rows_in >= target)) {
+      if (rows_in > 0 && (rows_in + R > prec_rows || rows_in >= target)) {
         prow_beg.push_back((int32_t)pos.size());
         rows_in = 0;
       }
@@ -2948,10 +2964,13 @@ void Engine::build_preconditioner() {
 
 void Engine::apply_prec(const double *v, double *z, int R) {
   const int nrows = (int)prows.size();
-  if (max_leaf_rows <= 64) {  // R right-hand sides, n apart, in one launch
-    if (nprec > 0)
-      hipLaunchKernelGGL(k_bjacobi_wg, dim3(nprec, R), dim3(256), 0, stream, d_prow_beg.as<int>(), d_prows.as<int>(),
-                         d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z, (int64_t)n);
+  if (max_leaf_rows <= 128) {  // R right-hand sides, n apart, in one launch
+    if (nprec > 0 && max_leaf_rows <= 64)
+      hipLaunchKernelGGL(k_bjacobi_wg<1>, dim3(nprec, R), dim3(256), 0, stream, d_prow_beg.as<int>(),
+                         d_prows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z, (int64_t)n);
+    else if (nprec > 0)
+      hipLaunchKernelGGL(k_bjacobi_wg<2>, dim3(nprec, R), dim3(256), 0, stream, d_prow_beg.as<int>(),
+                         d_prows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z, (int64_t)n);
     return;
   }
   for (int r = 1; r < R; ++r) apply_prec(v + (int64_t)r * n, z + (int64_t)r * n, 1);

@@ -142,6 +142,11 @@ class BEMFMA:
                     op_cost_ratio=(st[5] / 1000.0 if st[5] >= 0 else None), graph_fp64=int(st[6]),
                     graph_fp32=int(st[7]))
 
+    def set_preconditioner_rows(self, rows=128):
+        """Row cap of the block-Jacobi blocks, 8..128
+        (pbfmm_set_preconditioner_rows)."""
+        self._chk(self._L.pbfmm_set_preconditioner_rows(self._h, int(rows)))
+
     def set_graph_mode(self, mode=-1):
         """GMRES step operator as a HIP graph replay: -1 timed choice, 0 off,
         1 on (pbfmm_set_graph_mode)."""

@@ -71,6 +71,7 @@ SIGNATURES = {
     "pbfmm_set_mixed_scheme": (C.c_int, [_h, C.c_int32]),
     "pbfmm_set_mixed_tolerances": (C.c_int, [_h, C.c_double, C.c_double]),
     "pbfmm_set_graph_mode": (C.c_int, [_h, C.c_int32]),
+    "pbfmm_set_preconditioner_rows": (C.c_int, [_h, C.c_int32]),
     "pbfmm_graph_timings": (C.c_int, [_h, C.POINTER(C.c_float)]),
     "pbfmm_preconditioner_blocks": (C.c_int, [_h, _i64, _i32, _i32, _d]),
     "pbfmm_set_spmv_kernel": (C.c_int, [_h, C.c_int32, C.c_int32]),
