@@ -1962,6 +1962,7 @@ void Engine::upload_far(bool want_f32) {
   up(d_p2m_col_beg, far.p2m_col_beg);
   up(d_p2m_col_dof, far.p2m_col_dof);
   upload_chunks(m2m_cl, far.m2m);
+  upload_chunks(m2m_g_cl, far.m2m_g);
   upload_chunks(m2l_cl, far.m2l);
   upload_chunks(l2l_cl, far.l2l);
   up(d_m2p_node, far.m2p_item_node);
@@ -2588,21 +2589,26 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
     launch_spmv<T, 1>(var_full, x, nullptr, out, nullptr);
     near_done = true;
   }
-  // M2M, the deepest launched level carrying the SpMV (sides 1, 2)
-  for (int l = L; l > me_lo; --l) {  // M2M from level l into l - 1
-    const int c0 = lvl<1>(far.m2m, l), c1 = lvl<1>(far.m2m, l + 1);
+  // M2M, the deepest launched level (or group of two levels) carrying the
+  // SpMV (sides 1, 2)
+  const bool grouped = !far.m2m_g.op1.empty();
+  const ChunkDev &mcl = grouped ? m2m_g_cl : m2m_cl;
+  const int nlaunch = grouped ? (int)far.m2m_g.level_off1.size() - 1 : L - me_lo;
+  for (int s = 0; s < nlaunch; ++s) {  // group s, or M2M from level L - s into L - s - 1
+    const int c0 = grouped ? far.m2m_g.level_off1[s] : lvl<1>(far.m2m, L - s);
+    const int c1 = grouped ? far.m2m_g.level_off1[s + 1] : lvl<1>(far.m2m, L - s + 1);
     if (c1 <= c0) continue;
     const T *ops = F.m2m.template as<T>();
     if (near_done || side == 3 || fork_later) {
-      launch_translate<T, 0, 1>(*this, stream, m2m_cl, c0, c1, ops, me, me);
+      launch_translate<T, 0, 1>(*this, stream, mcl, c0, c1, ops, me, me);
       continue;
     }
     if (var == 0)
-      launch_translate_side<T, 0, 0, false>(*this, m2m_cl, c0, c1, ops, me, me, sd);
+      launch_translate_side<T, 0, 0, false>(*this, mcl, c0, c1, ops, me, me, sd);
     else if (var == 1)
-      launch_translate_side<T, 0, 1, false>(*this, m2m_cl, c0, c1, ops, me, me, sd);
+      launch_translate_side<T, 0, 1, false>(*this, mcl, c0, c1, ops, me, me, sd);
     else
-      launch_translate_side<T, 0, 2, false>(*this, m2m_cl, c0, c1, ops, me, me, sd);
+      launch_translate_side<T, 0, 2, false>(*this, mcl, c0, c1, ops, me, me, sd);
     near_done = true;
   }
   if (fork_later) fork_spmv();

@@ -140,6 +140,10 @@ struct FarPlan {
   ChunkList m2m;                  // tgt = parent, src = child; op = level*8 + octant
   ChunkList m2l;                  // op = offset class (all levels)
   ChunkList l2l;                  // tgt = child, src = parent; op = level*8 + octant
+  // M2M in launches of two child levels (build_m2m_groups): level_off1[g]
+  // .. level_off1[g + 1] are group g's chunks; ops index m2m_ops, whose
+  // composite two-level operators follow the (L + 1) x 8 single ones
+  ChunkList m2m_g;
   std::vector<int64_t> m2l_per_level;
   int n_m2l_ops = 0;
   // M2P pairs (target block, smaller far source block)
@@ -159,6 +163,7 @@ struct FarPlan {
   std::vector<double> m2l_ops;  // n_m2l_ops: one per offset class (unit box)
 };
 void build_far_plan(const Tree &t, int p, FarPlan &fp);
+void build_m2m_groups(const Tree &t, FarPlan &fp);
 void build_p2m_plan(const HostMesh &m, const QuadData &qd, const Tree &t, FarPlan &fp);
 // R-linear translation operators on [Re, Im] vectors (padded to kp).
 void op_m2m(int p, const double d[3], double *opT);  // d = child - parent

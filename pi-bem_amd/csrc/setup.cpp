@@ -818,6 +818,67 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
       fp.m2l = std::move(cl);
     }
   }
+  build_m2m_groups(t, fp);
+}
+
+// The upward pass in half the launches: M2M child levels a and a - 1 in one
+// launch.  The targets at level a - 1 take their children at level a as
+// usual; the targets at level a - 2 take their leaf children at level a - 1
+// directly and, for every other child c, c's children at level a through
+// the composite operator T_{a-1}(octant c) T_a(octant gc), so that no pair
+// of the launch reads an expansion the launch writes.  Child levels run from
+// L down to the coarsest one whose parent's expansions something reads (as
+// Engine::consumer_levels); group g is m2m_g chunks level_off1[g] ..
+// level_off1[g + 1].  The composite operators are appended to m2m_ops.
+void build_m2m_groups(const Tree &t, FarPlan &fp) {
+  const int L = t.L, kp = fp.kp;
+  fp.m2m_g = ChunkList();
+  fp.m2m_g.level_off1.assign(1, 0);
+  fp.m2m_g.level_off.assign(1, 0);
+  int me_lo = L + 1;
+  for (int l = 1; l <= L; ++l)
+    if (fp.m2l_per_level[l] > 0) me_lo = std::min(me_lo, l);
+  for (int b : fp.m2p_src) me_lo = std::min(me_lo, (int)t.level[b]);
+  const size_t base = fp.m2m_ops.size() / ((size_t)kp * kp);
+  std::vector<double> comp;  // appended composite operators
+  int ncomp = 0;
+  for (int a = L; a > me_lo; a -= 2) {
+    std::map<int, std::vector<std::pair<int, int>>> cls;
+    for (int b = t.startL[a]; b <= t.endL[a]; ++b) cls[a * 8 + t.octant[b]].push_back({t.parent[b], b});
+    if (a - 1 > me_lo) {
+      // composite operators of this level pair, on first use
+      std::vector<int> cop(64, -1);
+      for (int c = t.startL[a - 1]; c <= t.endL[a - 1]; ++c) {
+        if (t.nchild[c] == 0) {
+          cls[(a - 1) * 8 + t.octant[c]].push_back({t.parent[c], c});
+          continue;
+        }
+        for (int q = 0; q < t.nchild[c]; ++q) {
+          const int gc = t.child[8 * c + q];
+          const int key = t.octant[c] * 8 + t.octant[gc];
+          if (cop[key] < 0) {
+            cop[key] = ncomp++;
+            const double *T1 = &fp.m2m_ops[((size_t)(a - 1) * 8 + t.octant[c]) * kp * kp];
+            const double *T2 = &fp.m2m_ops[((size_t)a * 8 + t.octant[gc]) * kp * kp];
+            const size_t o = comp.size();
+            comp.resize(o + (size_t)kp * kp, 0.0);
+            // column-major: C[k kp + row] = sum_j T1[j kp + row] T2[k kp + j]
+            for (int k = 0; k < kp; ++k)
+              for (int j = 0; j < kp; ++j) {
+                const double t2 = T2[(size_t)k * kp + j];
+                if (t2 == 0.0) continue;
+                for (int row = 0; row < kp; ++row) comp[o + (size_t)k * kp + row] += T1[(size_t)j * kp + row] * t2;
+              }
+          }
+          cls[-1 - (int)(base + cop[key])].push_back({t.parent[c], gc});
+        }
+      }
+    }
+    for (auto &kv : cls) add_class(fp.m2m_g, kv.first >= 0 ? kv.first : -1 - kv.first, kv.second);
+    fp.m2m_g.level_off1.push_back((int)fp.m2m_g.op1.size());
+    fp.m2m_g.level_off.push_back((int)fp.m2m_g.op.size());
+  }
+  fp.m2m_ops.insert(fp.m2m_ops.end(), comp.begin(), comp.end());
 }
 
 // The image of cell c is the reflection x -> (x, y, 2 z0 - z) of its mapping

@@ -340,13 +340,16 @@ def test_mixed_schemes(oracle_mod, scheme):
 
 
 def test_tune_leaf_size():
-    """The leaf-size autotuner times each candidate's operator and returns the
-    fastest; every candidate gives a working operator."""
+    """The leaf-size autotuner times whole solves on a short ladder and
+    returns a measured leaf size; every measured leaf size gives a solve to
+    the tolerance, and the walk starts from both neighbours of the start."""
     pr = M.config_sphere(16, 4)
-    best, times = P.tune_leaf_size(pr, candidates=(12, 20, 32), reps=5)
-    print(f"\n  leaf sizes: {times} -> {best}")
-    assert set(times) == {12, 20, 32} and best in times and all(t > 0 for t in times.values())
-    assert times[best] == min(times.values())
+    best, times, info, bracketed = P.tune_leaf_size(pr, P.PBFMM_MIXED, solves=1, start=20,
+                                                    ladder=(12, 20, 32, 48))
+    print(f"\n  leaf sizes: {times} -> {best} (bracketed {bracketed})")
+    assert {12, 20, 32} <= set(times) and best in times and all(t > 0 for t in times.values())
+    assert all(rr <= 1e-8 for _, rr, _ in info.values())
+    assert times[best] <= min(times.values()) / 0.99  # a change of leaf size must gain 1%
 
 
 @pytest.mark.parametrize("name", ["sphere", "wigley"])
