@@ -86,6 +86,7 @@ struct Engine {
   DevBuf d_is_leaf;  // per block: 1 for a leaf (P2M stores its ME)
   ChunkDev m2m_cl, m2l_cl, l2l_cl;
   ChunkDev m2m_g_cl;  // M2M two child levels per launch (FarPlan::m2m_g)
+  ChunkDev l2l_g_cl;  // L2L two target levels per launch (FarPlan::l2l_g)
   DevBuf d_pval_beg;  // FP32 operator values: per-leaf offsets of the row-pair layout
   DevBuf d_row_leaf, d_row_beg, d_rows, d_col_beg, d_cols, d_val_beg, d_ent_beg, d_ent_cell,
       d_ent_mask, d_ent_near, d_ent_colpos, d_work_beg, d_leaf_block;

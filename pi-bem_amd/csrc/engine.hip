@@ -1963,6 +1963,7 @@ void Engine::upload_far(bool want_f32) {
   up(d_p2m_col_dof, far.p2m_col_dof);
   upload_chunks(m2m_cl, far.m2m);
   upload_chunks(m2m_g_cl, far.m2m_g);
+  upload_chunks(l2l_g_cl, far.l2l_g);
   upload_chunks(m2l_cl, far.m2l);
   upload_chunks(l2l_cl, far.l2l);
   up(d_m2p_node, far.m2p_item_node);
@@ -2635,19 +2636,31 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   else
     launch_translate<T, 1, 1>(*this, stream, m2l_cl, 0, c1, mops, me, le);
   m2p_done = m2p_done || m2p_here;
-  // L2L, the deepest launched level carrying M2P (side 1)
+  // L2L, the deepest launched level (or group of two levels) carrying M2P
+  // (side 1)
+  const bool lgrouped = !far.l2l_g.op1.empty();
+  const ChunkDev &lcl = lgrouped ? l2l_g_cl : l2l_cl;
   const int l2l_lo = std::max(2, le_lo + 1);  // L2L from level l - 1 into l
-  int deepest = 0;
-  for (int l = l2l_lo; !m2p_done && side == 1 && l <= L; ++l)
-    if (lvl<1>(far.l2l, l + 1) > lvl<1>(far.l2l, l)) deepest = l;
-  for (int l = l2l_lo; l <= L; ++l) {
-    const int c0 = lvl<1>(far.l2l, l), cc1 = lvl<1>(far.l2l, l + 1);
+  const int nlg = lgrouped ? (int)far.l2l_g.level_off1.size() - 1 : std::max(0, L + 1 - l2l_lo);
+  auto lrange = [&](int s, int &c0, int &cc1) {  // group s, or target level l2l_lo + s
+    c0 = lgrouped ? far.l2l_g.level_off1[s] : lvl<1>(far.l2l, l2l_lo + s);
+    cc1 = lgrouped ? far.l2l_g.level_off1[s + 1] : lvl<1>(far.l2l, l2l_lo + s + 1);
+  };
+  int deepest = -1;
+  for (int s = 0; !m2p_done && side == 1 && s < nlg; ++s) {
+    int c0, cc1;
+    lrange(s, c0, cc1);
+    if (cc1 > c0) deepest = s;
+  }
+  for (int s = 0; s < nlg; ++s) {
+    int c0, cc1;
+    lrange(s, c0, cc1);
     if (cc1 <= c0) continue;
-    if (l != deepest) {
-      launch_translate<T, 2, 1>(*this, stream, l2l_cl, c0, cc1, F.l2l.template as<T>(), le, le);
+    if (s != deepest) {
+      launch_translate<T, 2, 1>(*this, stream, lcl, c0, cc1, F.l2l.template as<T>(), le, le);
       continue;
     }
-    launch_translate_side<T, 2, -1, true>(*this, l2l_cl, c0, cc1, F.l2l.template as<T>(), le, le, sd);
+    launch_translate_side<T, 2, -1, true>(*this, lcl, c0, cc1, F.l2l.template as<T>(), le, le, sd);
     m2p_done = true;
   }
   if (!m2p_done)

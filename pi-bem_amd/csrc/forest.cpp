@@ -179,11 +179,14 @@ void merge_forest(const std::vector<const Engine *> &parts, Engine &e, std::vect
     append_off(fp.p2m_qleaf, q.far.p2m_qleaf, p2m_leaf_off);
     append_off(fp.p2m_qcol, q.far.p2m_qcol, col_off);
     m2m_off[s] = (int32_t)(fp.m2m_ops.size() / ((size_t)fp.kp * fp.kp));
-    // the single-level operators only: the batch runs M2M level by level,
-    // without the parts' composite two-level ones (build_m2m_groups)
+    // the single-level operators only: the batch runs M2M and L2L level by
+    // level, without the parts' composite two-level ones (build_m2m_groups,
+    // build_l2l_groups)
+    const size_t nsingle = (size_t)(L + 1) * 8 * fp.kp * fp.kp;
     fp.m2m_ops.insert(fp.m2m_ops.end(), q.far.m2m_ops.begin(),
-                      q.far.m2m_ops.begin() + std::min(q.far.m2m_ops.size(), (size_t)(L + 1) * 8 * fp.kp * fp.kp));
-    append(fp.l2l_ops, q.far.l2l_ops);
+                      q.far.m2m_ops.begin() + std::min(q.far.m2m_ops.size(), nsingle));
+    fp.l2l_ops.insert(fp.l2l_ops.end(), q.far.l2l_ops.begin(),
+                      q.far.l2l_ops.begin() + std::min(q.far.l2l_ops.size(), nsingle));
     m2l_off[s] = fp.n_m2l_ops;
     append(fp.m2l_ops, q.far.m2l_ops);
     fp.n_m2l_ops += q.far.n_m2l_ops;

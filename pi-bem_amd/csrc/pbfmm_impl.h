@@ -144,6 +144,10 @@ struct FarPlan {
   // .. level_off1[g + 1] are group g's chunks; ops index m2m_ops, whose
   // composite two-level operators follow the (L + 1) x 8 single ones
   ChunkList m2m_g;
+  // L2L in launches of two target levels (build_l2l_groups), likewise over
+  // l2l_ops; the non-leaf targets of a group's first level keep their M2L
+  // part only (their children take the parent's part through composites)
+  ChunkList l2l_g;
   std::vector<int64_t> m2l_per_level;
   int n_m2l_ops = 0;
   // M2P pairs (target block, smaller far source block)
@@ -164,6 +168,7 @@ struct FarPlan {
 };
 void build_far_plan(const Tree &t, int p, FarPlan &fp);
 void build_m2m_groups(const Tree &t, FarPlan &fp);
+void build_l2l_groups(const Tree &t, FarPlan &fp);
 void build_p2m_plan(const HostMesh &m, const QuadData &qd, const Tree &t, FarPlan &fp);
 // R-linear translation operators on [Re, Im] vectors (padded to kp).
 void op_m2m(int p, const double d[3], double *opT);  // d = child - parent

@@ -819,6 +819,19 @@ void build_far_plan(const Tree &t, int p, FarPlan &fp) {
     }
   }
   build_m2m_groups(t, fp);
+  build_l2l_groups(t, fp);
+}
+
+// C = A B for column-major kp x kp operators (C[k kp + row] = sum_j
+// A[j kp + row] B[k kp + j]): B acts first.
+static void compose_ops(const double *A, const double *B, int kp, double *C) {
+  std::fill(C, C + (size_t)kp * kp, 0.0);
+  for (int k = 0; k < kp; ++k)
+    for (int j = 0; j < kp; ++j) {
+      const double b = B[(size_t)k * kp + j];
+      if (b == 0.0) continue;
+      for (int row = 0; row < kp; ++row) C[(size_t)k * kp + row] += A[(size_t)j * kp + row] * b;
+    }
 }
 
 // The upward pass in half the launches: M2M child levels a and a - 1 in one
@@ -858,17 +871,10 @@ void build_m2m_groups(const Tree &t, FarPlan &fp) {
           const int key = t.octant[c] * 8 + t.octant[gc];
           if (cop[key] < 0) {
             cop[key] = ncomp++;
-            const double *T1 = &fp.m2m_ops[((size_t)(a - 1) * 8 + t.octant[c]) * kp * kp];
-            const double *T2 = &fp.m2m_ops[((size_t)a * 8 + t.octant[gc]) * kp * kp];
-            const size_t o = comp.size();
-            comp.resize(o + (size_t)kp * kp, 0.0);
-            // column-major: C[k kp + row] = sum_j T1[j kp + row] T2[k kp + j]
-            for (int k = 0; k < kp; ++k)
-              for (int j = 0; j < kp; ++j) {
-                const double t2 = T2[(size_t)k * kp + j];
-                if (t2 == 0.0) continue;
-                for (int row = 0; row < kp; ++row) comp[o + (size_t)k * kp + row] += T1[(size_t)j * kp + row] * t2;
-              }
+            comp.resize(comp.size() + (size_t)kp * kp);
+            compose_ops(&fp.m2m_ops[((size_t)(a - 1) * 8 + t.octant[c]) * kp * kp],
+                        &fp.m2m_ops[((size_t)a * 8 + t.octant[gc]) * kp * kp], kp,
+                        &comp[comp.size() - (size_t)kp * kp]);
           }
           cls[-1 - (int)(base + cop[key])].push_back({t.parent[c], gc});
         }
@@ -879,6 +885,60 @@ void build_m2m_groups(const Tree &t, FarPlan &fp) {
     fp.m2m_g.level_off.push_back((int)fp.m2m_g.op.size());
   }
   fp.m2m_ops.insert(fp.m2m_ops.end(), comp.begin(), comp.end());
+}
+
+// The downward pass in half the launches: L2L into target levels l and
+// l + 1 in one launch.  A target of level l + 1 takes its parent b's LE and,
+// through the composite T_{l+1}(octant gc) T_l(octant b), its grandparent's;
+// b's LE then holds its M2L part only, so b itself (a non-leaf of level l)
+// takes no L2L: nothing reads its LE afterwards (L2P reads leaves).  Leaves
+// of level l take their parent's LE as usual.  No pair reads an expansion
+// the launch writes: the reads are of level l - 1 (final) and of non-leaf
+// blocks of level l.  Target levels run from max(2, le_lo + 1) (as
+// matvec_fused, le_lo the coarsest M2L level) to L; group g is l2l_g chunks
+// level_off1[g] .. level_off1[g + 1]; the composites are appended to l2l_ops.
+void build_l2l_groups(const Tree &t, FarPlan &fp) {
+  const int L = t.L, kp = fp.kp;
+  fp.l2l_g = ChunkList();
+  fp.l2l_g.level_off1.assign(1, 0);
+  fp.l2l_g.level_off.assign(1, 0);
+  int le_lo = L + 1;
+  for (int l = 1; l <= L; ++l)
+    if (fp.m2l_per_level[l] > 0) le_lo = std::min(le_lo, l);
+  const size_t base = fp.l2l_ops.size() / ((size_t)kp * kp);
+  std::vector<char> filled(t.level.size(), 0);
+  for (int l = 0; l <= L; ++l)
+    for (int b : t.dofs_filled[l]) filled[b] = 1;
+  std::vector<double> comp;
+  int ncomp = 0;
+  for (int l = std::max(2, le_lo + 1); l <= L; l += 2) {
+    std::map<int, std::vector<std::pair<int, int>>> cls;
+    const bool pair = l + 1 <= L;
+    std::vector<int> cop(64, -1);
+    for (int b : t.dofs_filled[l]) {
+      if (!pair || t.nchild[b] == 0) {
+        cls[l * 8 + t.octant[b]].push_back({b, t.parent[b]});
+        continue;
+      }
+      for (int q = 0; q < t.nchild[b]; ++q) {
+        const int gc = t.child[8 * b + q];
+        if (!filled[gc]) continue;
+        cls[(l + 1) * 8 + t.octant[gc]].push_back({gc, b});
+        const int key = t.octant[b] * 8 + t.octant[gc];
+        if (cop[key] < 0) {
+          cop[key] = ncomp++;
+          comp.resize(comp.size() + (size_t)kp * kp);
+          compose_ops(&fp.l2l_ops[((size_t)(l + 1) * 8 + t.octant[gc]) * kp * kp],
+                      &fp.l2l_ops[((size_t)l * 8 + t.octant[b]) * kp * kp], kp, &comp[comp.size() - (size_t)kp * kp]);
+        }
+        cls[-1 - (int)(base + cop[key])].push_back({gc, t.parent[b]});
+      }
+    }
+    for (auto &kv : cls) add_class(fp.l2l_g, kv.first >= 0 ? kv.first : -1 - kv.first, kv.second);
+    fp.l2l_g.level_off1.push_back((int)fp.l2l_g.op1.size());
+    fp.l2l_g.level_off.push_back((int)fp.l2l_g.op.size());
+  }
+  fp.l2l_ops.insert(fp.l2l_ops.end(), comp.begin(), comp.end());
 }
 
 // The image of cell c is the reflection x -> (x, y, 2 z0 - z) of its mapping
