@@ -177,8 +177,10 @@ struct Engine {
   // (plans 4, 5 with placements 0, 1) M2L into per-pair slots and a
   // per-target sum (k_translate<., ., 3, 1> + k_m2l_reduce) instead of atomic
   // adds into the targets; bit 3 (plans 8, 12 with placement 0) the SpMV on a
-  // second stream beside the whole FMM, joined before L2P.
-  static constexpr int kPlans[] = {0, 1, 2, 3, 4, 5, 8, 12, 24, 28};
+  // second stream beside the whole FMM, joined before L2P; bit 4 (plans 24,
+  // 28) that fork after the M2M levels instead; bit 5 (plans 56, 60) M2P
+  // after the SpMV on the second stream too.
+  static constexpr int kPlans[] = {0, 1, 2, 3, 4, 5, 8, 12, 24, 28, 56, 60};
   static bool valid_plan(int m) {
     for (int v : kPlans)
       if (v == m) return true;

@@ -2523,6 +2523,7 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   const bool async = (side & 8) != 0;  // SpMV on the second stream
   const bool late = (side & 16) != 0;  // ... forked after the M2M levels, beside M2L onwards
   const bool slot = (side & 4) != 0;   // M2L into per-pair slots, summed per target
+  const bool m2p_async = (side & 32) != 0;  // M2P after the SpMV on the second stream (late fork)
   side &= 3;
   const int nleaf = (int)far.p2m_leaf.size();
   const int64_t ne = (int64_t)tree.nb * kp;
@@ -2568,6 +2569,13 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   sd.m2p_out = acc;
   bool near_done = nl == 0, m2p_done = ni == 0;
   bool join = false;
+  auto launch_m2p = [&](hipStream_t st) {
+    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_m2p<T, 1, PP>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, st, ni,
+                                         d_m2p_node.as<int>(), d_m2p_isrc.as<int>(), d_xyz.as<double>(),
+                                         d_pmin.as<double>(), d_delta.as<double>(), p, kt, kp, me, acc, nullptr,
+                                         (int64_t)0, (int64_t)0));
+    m2p_done = true;
+  };
   // the SpMV reads only x, ready when this application starts, and writes
   // out, which only the final L2P reads: on the second stream it runs beside
   // the FMM, from the start or (late) from M2L on, leaving the M2M levels'
@@ -2581,6 +2589,9 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
     HIPCHK(hipEventRecord(ev_fork, stream));
     HIPCHK(hipStreamWaitEvent(side_stream, ev_fork, 0));
     launch_spmv<T, 1>(var_full, x, nullptr, out, nullptr, 1, side_stream);
+    // forked after the M2M levels, the multipole expansions are final: M2P
+    // (which adds into the accumulator only) follows the SpMV there
+    if (late && m2p_async && !m2p_done) launch_m2p(side_stream);
     HIPCHK(hipEventRecord(ev_join, side_stream));
     near_done = join = true;
   };
@@ -2663,10 +2674,7 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
     launch_translate_side<T, 2, -1, true>(*this, lcl, c0, cc1, F.l2l.template as<T>(), le, le, sd);
     m2p_done = true;
   }
-  if (!m2p_done)
-    PBFMM_BY_ORDER(p, hipLaunchKernelGGL((k_m2p<T, 1, PP>), dim3((unsigned)nblk(ni, 256)), dim3(256), 0, stream, ni, d_m2p_node.as<int>(),
-                       d_m2p_isrc.as<int>(), d_xyz.as<double>(), d_pmin.as<double>(), d_delta.as<double>(), p, kt,
-                       kp, me, acc, nullptr, (int64_t)0, (int64_t)0));
+  if (!m2p_done) launch_m2p(stream);
   if (join) HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
   const int nrows = (int)near.rows.size();
   if (nrows > 0)

@@ -45,7 +45,7 @@ def test_graph_solves_equal_direct(name):
     bp = P.BEMProblem(f)
     rhs = hip.DeviceArray.from_numpy(bp.compute_rhs(pr.tmp_rhs))
     for prec, bar in ((P.PBFMM_FP64, 1e-10), (P.PBFMM_MIXED, 1e-7)):
-        for plan in (0, 4, 12, 24):
+        for plan in (0, 4, 12, 24, 56):
             f.set_side_fusion(prec, plan)
             f.set_graph_mode(0)
             x0, it0, rr0 = _solve(f, rhs, prec)
