@@ -116,8 +116,10 @@ struct Engine {
   std::vector<uint8_t> hc_is;
   // near-field block-Jacobi preconditioner: blocks of the near leaves that
   // share a parent block (at most prec_rows rows, leaf after leaf; a larger
-  // leaf alone), per block the inverse of its rows x rows near-field block,
-  // column-major, at pinv_beg[block]; prow_beg / prows / prow_blk: the
+  // leaf alone), per block the inverse of its rows x rows near-field block
+  // (inverted in FP64, stored in FP32 column quads at pquad_beg[block]:
+  // pinv_quad_index; pinv_beg[block]: the R x R blocks' own offsets, as
+  // assembled); prow_beg / prows / prow_blk: the
   // blocks' row ranges, rows in block order, block of each of them
   // Row cap of a preconditioner block (k_bjacobi_wg takes up to 128, one or
   // two rows per lane; pbfmm_set_preconditioner_rows).  Larger blocks cut
@@ -129,9 +131,9 @@ struct Engine {
   int prec_rows = 128;
   bool have_prec = false;
   int nprec = 0;
-  std::vector<int64_t> pinv_beg;
+  std::vector<int64_t> pinv_beg, pquad_beg;
   std::vector<int32_t> prow_beg, prows;
-  DevBuf d_pinv_beg, d_pinv, d_ptmp, d_ptmp2, d_prow_beg, d_prows, d_prow_blk;
+  DevBuf d_pinv_beg, d_pquad_beg, d_pinv, d_ptmp, d_ptmp2, d_prow_beg, d_prows, d_prow_blk;
   void build_preconditioner();
   std::vector<int32_t> prec_pos;  // block-ordered row -> leaf-ordered near row
   void prec_blocks(std::vector<double> &blk);

@@ -1465,55 +1465,78 @@ __global__ void k_extract_groups(const int *__restrict__ grp_beg, const int *__r
   }
 }
 
+// The inverse blocks are stored in FP32 and applied in FP64 (a fixed linear
+// preconditioner within FP32 rounding of the inverse, which GMRES with right
+// preconditioning applies to its basis and to the update alike), in column
+// quads: entry (r, c) of an R-row block at quad_beg + ((c / 4) R + r) 4 +
+// c % 4, columns zero-padded to a multiple of four, so that a lane reads
+// four columns of its row in one 16-byte load and a wavefront's lanes (rows)
+// read consecutive addresses.
+__host__ __device__ inline int64_t pinv_quad_index(int r, int c, int R) {
+  return ((int64_t)(c >> 2) * R + r) * 4 + (c & 3);
+}
+
 // z[rows of block] = Pinv_block . v[rows of block]; one thread per block-ordered
 // row (row_leaf: its block)
 __global__ void __launch_bounds__(256) k_bjacobi(int nrows, const int *__restrict__ row_leaf,
                                                  const int *__restrict__ row_beg, const int *__restrict__ rows,
                                                  const int64_t *__restrict__ blk_beg,
-                                                 const double *__restrict__ pinv, const double *__restrict__ v,
+                                                 const float *__restrict__ pinv, const double *__restrict__ v,
                                                  double *__restrict__ z) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nrows) return;
   const int leaf = row_leaf[g];
   const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0, r = g - r0;
-  const double *P = pinv + blk_beg[leaf] + r;  // block stored column-major
+  const float4 *P = reinterpret_cast<const float4 *>(pinv + blk_beg[leaf]) + r;
   double acc = 0.0;
-  for (int c = 0; c < R; ++c) acc += P[(int64_t)c * R] * v[rows[r0 + c]];
+  for (int c = 0; c < R; c += 4) {
+    const float4 a = P[(int64_t)(c >> 2) * R];
+    acc += (double)a.x * v[rows[r0 + c]];
+    if (c + 1 < R) acc += (double)a.y * v[rows[r0 + c + 1]];
+    if (c + 2 < R) acc += (double)a.z * v[rows[r0 + c + 2]];
+    if (c + 3 < R) acc += (double)a.w * v[rows[r0 + c + 3]];
+  }
   z[rows[g]] = acc;
 }
 // The same with one workgroup of four wavefronts per block (blocks of at
-// most 64 rows): every wavefront gathers the block's inputs once (lane c: v
-// at row c) and sums the columns c = w, w + 4, ... of all rows (lane r: row
-// r), reading them with independent coalesced loads and the inputs broadcast
-// from their lanes; the four partial sums meet in LDS.  Load chains are R/4
-// long, and there are four times the wavefronts in flight.
+// most 64 RPL rows, RPL rows per lane): every wavefront gathers the block's
+// inputs once (lane c: v at row c + 64 q) and sums the column quads
+// w, w + 4, ... of all rows (lane r: row r), the inputs broadcast from their
+// lanes; the four partial sums meet in LDS.
 // (right-hand side blockIdx.y: v and z + y stride)
 template <int RPL>
 __global__ void __launch_bounds__(256) k_bjacobi_wg(const int *__restrict__ row_beg, const int *__restrict__ rows,
                                                     const int64_t *__restrict__ blk_beg,
-                                                    const double *__restrict__ pinv, const double *__restrict__ v,
+                                                    const float *__restrict__ pinv, const double *__restrict__ v,
                                                     double *__restrict__ z, int64_t stride) {
-  // RPL rows per lane: blocks of up to 64 RPL rows
   __shared__ double part[4][64 * RPL];
   const int blk = blockIdx.x;
   v += (int64_t)blockIdx.y * stride;
   z += (int64_t)blockIdx.y * stride;
   const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-  const int r0 = row_beg[blk], R = row_beg[blk + 1] - r0;
+  const int r0 = row_beg[blk], R = row_beg[blk + 1] - r0, nq = (R + 3) >> 2;
   double xv[RPL], acc[RPL];
-  const double *P[RPL];
+  const float4 *P[RPL];
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
     const int row = lane + 64 * q;
-    xv[q] = row < R ? v[rows[r0 + row]] : 0.0;
-    P[q] = pinv + blk_beg[blk] + (row < R ? row : 0);  // column-major block
+    xv[q] = row < R ? v[rows[r0 + row]] : 0.0;  // zero past the block: the padded columns
+    P[q] = reinterpret_cast<const float4 *>(pinv + blk_beg[blk]) + (row < R ? row : 0);
     acc[q] = 0.0;
   }
-#pragma unroll 4
-  for (int c = w; c < R; c += 4) {
-    const double xc = RPL == 1 || c < 64 ? read_lane(xv[0], c & 63) : read_lane(xv[RPL - 1], c & 63);
+#pragma unroll 8
+  for (int cq = w; cq < nq; cq += 4) {
+    double xc[4];
 #pragma unroll
-    for (int q = 0; q < RPL; ++q) acc[q] += (lane + 64 * q < R ? P[q][(int64_t)c * R] : 0.0) * xc;
+    for (int k = 0; k < 4; ++k) {
+      const int c = 4 * cq + k;
+      xc[k] = RPL == 1 || c < 64 ? read_lane(xv[0], c & 63) : read_lane(xv[RPL - 1], c & 63);
+    }
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const float4 a = lane + 64 * q < R ? P[q][(int64_t)cq * R] : make_float4(0.f, 0.f, 0.f, 0.f);
+      acc[q] += ((double)a.x * xc[0] + (double)a.y * xc[1]) + ((double)a.z * xc[2] + (double)a.w * xc[3]);
+    }
   }
 #pragma unroll
   for (int q = 0; q < RPL; ++q) part[w][lane + 64 * q] = acc[q];
@@ -2950,7 +2973,13 @@ void Engine::build_preconditioner() {
   up(d_prows, prows);
   up(d_prow_blk, prow_blk);
   prec_pos = pos;
-  alloc(d_pinv, (size_t)std::max<int64_t>(pinv_beg[nprec], 1) * 8);
+  pquad_beg.assign(nprec + 1, 0);
+  for (int g = 0; g < nprec; ++g) {
+    const int64_t R = prow_beg[g + 1] - prow_beg[g];
+    pquad_beg[g + 1] = pquad_beg[g] + R * ((R + 3) / 4 * 4);
+  }
+  up(d_pquad_beg, pquad_beg);
+  std::vector<float> b32((size_t)pquad_beg[nprec], 0.0f);
   std::vector<double> blk;
   prec_blocks(blk);
   // invert the blocks on host threads
@@ -2968,10 +2997,8 @@ void Engine::build_preconditioner() {
             A[(size_t)r * R + c] = (r == c) ? (d != 0.0 ? 1.0 / d : 1.0) : 0.0;
           }
       }
-      // stored column-major (transposed): the apply kernels' lanes, one per
-      // row, then read consecutive addresses
       for (int r = 0; r < R; ++r)
-        for (int c = 0; c < R; ++c) blk[pinv_beg[g] + (size_t)c * R + r] = A[(size_t)r * R + c];
+        for (int c = 0; c < R; ++c) b32[pquad_beg[g] + pinv_quad_index(r, c, R)] = (float)A[(size_t)r * R + c];
     }
   };
   const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -2983,7 +3010,7 @@ void Engine::build_preconditioner() {
       th.emplace_back(invert_range, (int)((int64_t)nprec * k / nth), (int)((int64_t)nprec * (k + 1) / nth));
     for (auto &x : th) x.join();
   }
-  if (!blk.empty()) HIPCHK(hipMemcpy(d_pinv.p, blk.data(), blk.size() * 8, hipMemcpyHostToDevice));
+  up(d_pinv, b32);
   alloc(d_ptmp, n * 8);
   alloc(d_ptmp2, n * 8);
   have_prec = true;
@@ -2994,15 +3021,15 @@ void Engine::apply_prec(const double *v, double *z, int R) {
   if (max_leaf_rows <= 128) {  // R right-hand sides, n apart, in one launch
     if (nprec > 0 && max_leaf_rows <= 64)
       hipLaunchKernelGGL(k_bjacobi_wg<1>, dim3(nprec, R), dim3(256), 0, stream, d_prow_beg.as<int>(),
-                         d_prows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z, (int64_t)n);
+                         d_prows.as<int>(), d_pquad_beg.as<int64_t>(), d_pinv.as<float>(), v, z, (int64_t)n);
     else if (nprec > 0)
       hipLaunchKernelGGL(k_bjacobi_wg<2>, dim3(nprec, R), dim3(256), 0, stream, d_prow_beg.as<int>(),
-                         d_prows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z, (int64_t)n);
+                         d_prows.as<int>(), d_pquad_beg.as<int64_t>(), d_pinv.as<float>(), v, z, (int64_t)n);
     return;
   }
   for (int r = 1; r < R; ++r) apply_prec(v + (int64_t)r * n, z + (int64_t)r * n, 1);
   hipLaunchKernelGGL(k_bjacobi, dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_prow_blk.as<int>(),
-                     d_prow_beg.as<int>(), d_prows.as<int>(), d_pinv_beg.as<int64_t>(), d_pinv.as<double>(), v, z);
+                     d_prow_beg.as<int>(), d_prows.as<int>(), d_pquad_beg.as<int64_t>(), d_pinv.as<float>(), v, z);
 }
 
 // One Arnoldi step's operator: w = A M^-1 vin (right preconditioning).
