@@ -58,7 +58,8 @@ PHASES = ("spmv", "p2m", "m2m", "l2l", "m2l", "m2p", "l2p", "m2l_sum")
 # combined N.phi - D.dphi)
 SPMV_KERNEL = {0: "k_spmv_lds<{t}, {k}, false>", 1: "k_spmv_wave<{t}, {k}, 2, false>",
                2: "k_spmv_wave<{t}, {k}, 4, false>", 3: "k_spmv_lds<{t}, {k}, true>",
-               4: "k_spmv_wave<{t}, {k}, 2, true>", 5: "k_spmv_wave<{t}, {k}, 4, true>"}
+               4: "k_spmv_wave<{t}, {k}, 2, true>", 5: "k_spmv_wave<{t}, {k}, 4, true>",
+               6: "k_spmv_quad<8>"}
 KERNEL = {
     "p2m": "k_p2m<{t}, {k}>", "m2m": "k_translate<{t}, {p}, 0, {k}>", "m2l": "k_translate<{t}, {p}, 1, {k}>",
     "l2l": "k_translate<{t}, {p}, 2, {k}>", "m2p": "k_m2p<{t}, {k}, {p}>", "m2l_sum": "k_m2l_reduce<{t}>",

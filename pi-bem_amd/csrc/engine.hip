@@ -277,6 +277,29 @@ __global__ void __launch_bounds__(256) k_round_f32_pairs(int nleaf, const int *_
   }
 }
 
+// The same values in column quads: leaf l's block goes to qval_beg[l] with
+// (r, c) at ((c / 4) R + r) 4 + c % 4, its columns zero-padded to a multiple
+// of four (R x 4 ceil(C / 4) floats, 16-byte aligned).  A lane (a row) then
+// loads four columns of its row with one 16-byte load, and a wavefront's
+// lanes read consecutive addresses (k_spmv_quad).  One workgroup per leaf.
+__global__ void __launch_bounds__(256) k_round_f32_quads(int nleaf, const int *__restrict__ row_beg,
+                                                         const int *__restrict__ col_beg,
+                                                         const int64_t *__restrict__ val_beg,
+                                                         const int64_t *__restrict__ qval_beg,
+                                                         const double *__restrict__ a, float *__restrict__ b) {
+  const int l = blockIdx.x;
+  if (l >= nleaf) return;
+  const int R = row_beg[l + 1] - row_beg[l], C = col_beg[l + 1] - col_beg[l];
+  const int64_t n4 = (int64_t)R * ((C + 3) / 4 * 4);
+  const double *src = a + val_beg[l];
+  float *dst = b + qval_beg[l];
+  for (int64_t o = threadIdx.x; o < n4; o += blockDim.x) {
+    const int64_t q = o / (4 * (int64_t)R);
+    const int rem = (int)(o - q * 4 * R), r = rem >> 2, c = 4 * (int)q + (rem & 3);
+    dst[o] = c < C ? (float)src[(int64_t)r * C + c] : 0.0f;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Near-field SpMV, fused N.phi and D.dphi on the shared pattern
 // (prec_neumann_matrix.vmult / prec_dirichlet_matrix.vmult, bem_fma.cc:2408-2409),
@@ -452,6 +475,74 @@ __global__ void __launch_bounds__(256) k_spmv_lds(int nleaf, int per_xcd, NearAr
   const int leaf = (int)(blockIdx.x % kXcd) * per_xcd + (int)(blockIdx.x / kXcd);
   if (leaf >= nleaf) return;
   spmv_lds_leaf<T, NK, 256, PR>(leaf, xs, near_rhs(a));
+}
+
+// The FP32 operator's single values in column quads (k_round_f32_quads), one
+// workgroup per leaf: the leaf's inputs gathered once into LDS as in
+// k_spmv_lds, then lane = row and the four wavefronts split the column
+// quads, each lane streaming its row with 16-byte loads against the quads'
+// inputs broadcast from LDS; no wavefront sums.  A leaf of at most 32 (16)
+// rows runs two (four) lane groups per wavefront on alternate quads; one of
+// more than 64 rows runs in blocks of 64.  The 4 x groups partial sums of a
+// row meet in LDS.  Leaves dealt out per XCD as k_spmv_lds.
+template <int UNR>
+__global__ void __launch_bounds__(256) k_spmv_quad(int nleaf, int per_xcd, NearArgs<float> a) {
+  __shared__ float4 xs4[kSpmvTile / 4];
+  __shared__ float part[16][64];
+  const int leaf = (int)(blockIdx.x % kXcd) * per_xcd + (int)(blockIdx.x / kXcd);
+  if (leaf >= nleaf) return;
+  a = near_rhs(a);
+  const int t = (int)threadIdx.x, w = t >> 6, lane = t & 63;
+  const int r0 = a.row_beg[leaf], R = a.row_beg[leaf + 1] - r0;
+  const int cb = a.col_beg[leaf], C = a.col_beg[leaf + 1] - cb;
+  const int NQ = (C + 3) >> 2;
+  const float4 *V4 = reinterpret_cast<const float4 *>(a.vals + a.val_beg[leaf]);
+  float *xs = reinterpret_cast<float *>(xs4);
+  const int H = R <= 16 ? 4 : (R <= 32 ? 2 : 1), RL = 64 / H;
+  const int h = lane / RL, rl = lane - h * RL;
+  const int S = 4 * H, s = w * H + h;  // quad streams of the workgroup, this lane's
+  constexpr int QTILE = kSpmvTile / 4;
+  auto gather = [&](int q0, int QT) {
+    for (int c = t; c < 4 * QT; c += 256) {
+      const int cc = 4 * q0 + c;
+      xs[c] = cc < C ? near_x<float, 1>(a.phi, nullptr, a.cols[cb + cc]) : 0.0f;
+    }
+  };
+  const bool one_tile = NQ <= QTILE;
+  if (one_tile) {
+    gather(0, NQ);
+    __syncthreads();
+  }
+  for (int rb = 0; rb < R; rb += 64) {
+    const int r = rb + rl;
+    float acc = 0.0f;
+    for (int q0 = 0; q0 < NQ; q0 += QTILE) {
+      const int QT = min(QTILE, NQ - q0);
+      if (!one_tile) {
+        __syncthreads();  // the previous tile consumed
+        gather(q0, QT);
+        __syncthreads();
+      }
+      if (r < R) {
+        const float4 *v = V4 + (int64_t)q0 * R + r;
+#pragma unroll UNR
+        for (int q = s; q < QT; q += S) {
+          const float4 m = v[(int64_t)q * R];
+          const float4 xq = xs4[q];
+          acc += (m.x * xq.x + m.y * xq.y) + (m.z * xq.z + m.w * xq.w);
+        }
+      }
+    }
+    part[s][lane] = acc;
+    __syncthreads();
+    if (t < min(RL, R - rb)) {
+      float z = 0.0f;
+      for (int ww = 0; ww < 4; ++ww)
+        for (int g = 0; g < H; ++g) z += part[ww * H + g][g * RL + t];
+      a.outN[a.rows[r0 + rb + t]] = (double)z;
+    }
+    __syncthreads();  // part reused by the next row block
+  }
 }
 
 // One leaf by one wavefront, no LDS and no barrier, RPI rows per iteration.
@@ -2158,6 +2249,17 @@ void Engine::assemble_near(bool want_f32) {
       hipLaunchKernelGGL(k_round_f32_pairs, dim3(near.nleaf), dim3(256), 0, stream, near.nleaf, d_row_beg.as<int>(),
                          d_col_beg.as<int>(), d_val_beg.as<int64_t>(), d_pval_beg.as<int64_t>(),
                          f64.opvals.as<double>(), f32.opvals_p.as<float>());
+    // and in column quads
+    std::vector<int64_t> qvb(near.nleaf + 1, 0);
+    for (int l = 0; l < near.nleaf; ++l)
+      qvb[l + 1] = qvb[l] + (int64_t)(near.row_beg[l + 1] - near.row_beg[l]) *
+                                ((near.col_beg[l + 1] - near.col_beg[l] + 3) / 4 * 4);
+    up(d_qval_beg, qvb);
+    alloc(f32.opvals_q, (size_t)std::max<int64_t>(qvb[near.nleaf], 4) * sizeof(float));
+    if (near.nleaf > 0)
+      hipLaunchKernelGGL(k_round_f32_quads, dim3(near.nleaf), dim3(256), 0, stream, near.nleaf, d_row_beg.as<int>(),
+                         d_col_beg.as<int>(), d_val_beg.as<int64_t>(), d_qval_beg.as<int64_t>(),
+                         f64.opvals.as<double>(), f32.opvals_q.as<float>());
     HIPCHK(hipGetLastError());
   }
   tune_spmv<double, 2>();
@@ -2256,15 +2358,18 @@ static void launch_translate_side(Engine &e, const ChunkDev &cl, int c0, int c1,
 #undef PBFMM_TRS
 }
 
-// variant >= 3: the FP32 operator's values in row pairs, at their own (even)
-// per-leaf offsets
+// variant 3..5: the FP32 operator's values in row pairs, at their own (even)
+// per-leaf offsets; 6: in column quads
 template <typename T, int NK>
 static NearArgs<T> near_args(Engine &e, const double *phi, const double *dphi, double *outN, double *outD,
                              int variant) {
-  const bool paired = kPairedF32<T, NK> && variant >= 3;
+  const bool quads = kPairedF32<T, NK> && variant == 6;
+  const bool paired = kPairedF32<T, NK> && variant >= 3 && !quads;
   const FarDev<T> &F = far_of<T>(e);
-  const T *vals = (NK == 2 ? F.vals : paired ? F.opvals_p : F.opvals).template as<T>();
-  const int64_t *vb = paired ? e.d_pval_beg.as<int64_t>() : e.d_val_beg.as<int64_t>();
+  const T *vals = (NK == 2 ? F.vals : quads ? F.opvals_q : paired ? F.opvals_p : F.opvals).template as<T>();
+  const int64_t *vb = quads    ? e.d_qval_beg.as<int64_t>()
+                      : paired ? e.d_pval_beg.as<int64_t>()
+                               : e.d_val_beg.as<int64_t>();
   return NearArgs<T>{e.d_row_beg.as<int>(), e.d_rows.as<int>(), e.d_col_beg.as<int>(), e.d_cols.as<int>(),
                      vb, vals, phi, dphi, outN, outD, 0};
 }
@@ -2275,7 +2380,12 @@ void Engine::launch_spmv(int variant, const double *phi, const double *dphi, dou
   NearArgs<T> a = near_args<T, NK>(*this, phi, dphi, outN, outD, variant);
   a.stride = n;  // R right-hand sides, n apart (blockIdx.y)
   if (!st) st = stream;
-  if constexpr (kPairedF32<T, NK>) {  // 3..5: the kernels of 0..2 on the row pairs
+  if constexpr (kPairedF32<T, NK>) {  // 3..5: the kernels of 0..2 on the row pairs; 6 quads
+    if (variant == 6) {
+      const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
+      hipLaunchKernelGGL(k_spmv_quad<8>, dim3(per_xcd * kXcd, R), dim3(256), 0, st, near.nleaf, per_xcd, a);
+      return;
+    }
     if (variant == 3) {
       const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
       hipLaunchKernelGGL((k_spmv_lds<T, NK, true>), dim3(per_xcd * kXcd, R), dim3(256), 0, st, near.nleaf, per_xcd,
@@ -2314,7 +2424,7 @@ void Engine::tune_spmv() {
   HIPCHK(hipEventCreate(&e1));
   float best = 0;
   int pick = 0;
-  for (int v = 0; v < (kPairedF32<T, NK> ? 6 : 3); ++v) {
+  for (int v = 0; v < (kPairedF32<T, NK> ? 7 : 3); ++v) {
     for (int rep = 0; rep < 2; ++rep)
       launch_spmv<T, NK>(v, d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
     HIPCHK(hipEventRecord(e0, stream));
