@@ -23,7 +23,7 @@ restated in C++, OpenMP) on the host cores.  Configuration 2 (unit sphere,
 "config4" and "config5" objects.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--extra 2,4,5]
-    python bench.py --mode matvec --config {2,3} --precision {fp32,fp64} --steps K [--leaf L]
+    python bench.py --mode matvec --config {2,3} --precision {fp32,fp64} --steps K [--leaf L] [--plan P]
     python bench.py --mode copy --mib M --steps K
     python bench.py --mode solve --config {2,3} --precision {fp32,fp64} --steps K [--leaf L]
 
@@ -634,7 +634,11 @@ def mode_matvec(args):
         f.matvec_dev(a.value, b.value, oN.value, oD.value, prec)
     bp = P.BEMProblem(f)
     # one fused application first: it times the launch plans (as the solves
-    # do), so that the phase-timed runs below use the M2L form it chose
+    # do), so that the phase-timed runs below use the M2L form it chose; with
+    # --plan the plan is set instead and no timing runs, so that every launch
+    # of the run is a kernel alone on the device (per-kernel durations)
+    if args.plan >= 0:
+        f.set_side_fusion(prec, args.plan)
     bp.vmult_dev(a.value, oN.value, prec)
     # phase-timed, as the operator phases bench.py reports: every phase in its
     # own launch (the solve's fused launches are profiled with --mode solve)
@@ -690,6 +694,7 @@ def main():
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--leaf", type=int, default=0, help="matvec/solve modes: max nodes per leaf (0: 20)")
+    ap.add_argument("--plan", type=int, default=-1, help="matvec mode: operator launch plan (-1: timed)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported; this build runs the BEM-FMM path on a "
