@@ -1022,7 +1022,7 @@ int pbfmm_preconditioner_blocks(pbfmm_handle *h, int64_t *sizes, int32_t *row_be
 
 int pbfmm_set_mixed_scheme(pbfmm_handle *h, int32_t scheme) {
   if (!h) return PBFMM_ERR_ARG;
-  if (scheme < 0 || scheme > 3) return fail(h, PBFMM_ERR_ARG, "set_mixed_scheme: scheme must be 0, 1, 2 or 3");
+  if (scheme < 0 || scheme > 4) return fail(h, PBFMM_ERR_ARG, "set_mixed_scheme: scheme must be 0..4");
   h->eng.mixed_scheme = scheme;
   return PBFMM_OK;
 }

@@ -242,7 +242,9 @@ struct Engine {
   // Krylov space) when the FP64 operator is not much dearer than the FP32
   // one, else iterative refinement; the cost ratio is timed once
   double op_cost_ratio = 0;
-  int mixed_scheme = 0;  // 0 chosen by the timed cost ratio, 1 relaxed, 2 refinement, 3 without recycling
+  // 0 chosen by the timed cost ratio, 1 relaxed, 2 refinement (FP32 basis,
+  // recycling), 3 without recycling, 4 with an FP64 basis
+  int mixed_scheme = 0;
   double inner_reduction = 0;  // refinement's inner residual reduction (0: the default)
   double relax_switch = 0;     // relaxed scheme's FP32 switch over tol ||b|| (0: the default)
   double inner_target(double tol, double bn, double rn) const;

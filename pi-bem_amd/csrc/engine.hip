@@ -1688,32 +1688,35 @@ __global__ void k_gemv_sub(int64_t n, int64_t ldv, int nv, const double *__restr
   for (int j = 0; j < nv; ++j) acc -= V[(int64_t)j * ldv + i] * h[j];
   w[i] = acc;
 }
-// x += sum_j V_j y_j
-__global__ void k_gemv_add(int64_t n, int64_t ldv, int nv, const double *__restrict__ V,
-                           const double *__restrict__ y, double *__restrict__ x) {
+// x += sum_j V_j y_j (basis vectors stored as VT: double, or float for the
+// refinement's inner solves; arithmetic in FP64)
+template <typename VT>
+__global__ void k_gemv_add(int64_t n, int64_t ldv, int nv, const VT *__restrict__ V, const double *__restrict__ y,
+                           double *__restrict__ x) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = x[i];
-  for (int j = 0; j < nv; ++j) acc += V[(int64_t)j * ldv + i] * y[j];
+  for (int j = 0; j < nv; ++j) acc += (double)V[(int64_t)j * ldv + i] * y[j];
   x[i] = acc;
 }
 // out_j = sum_l V_l Q[j nv + l] for 8 outputs j = 8 blockIdx.y .. per thread
 // (vectors ld apart): the recycled Krylov space's C = V Q
-__global__ void __launch_bounds__(256) k_tsgemm8(int64_t n, int64_t ld, int nv, const double *__restrict__ V,
-                                                 const double *__restrict__ Q, int nout, double *__restrict__ out) {
+template <typename VT>
+__global__ void __launch_bounds__(256) k_tsgemm8(int64_t n, int64_t ld, int nv, const VT *__restrict__ V,
+                                                 const double *__restrict__ Q, int nout, VT *__restrict__ out) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int j0 = 8 * blockIdx.y;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int l = 0; l < nv; ++l) {
-    const double v = V[(int64_t)l * ld + i];
+    const double v = (double)V[(int64_t)l * ld + i];
 #pragma unroll
     for (int q = 0; q < 8; ++q)
       if (j0 + q < nout) acc[q] += v * Q[(int64_t)(j0 + q) * nv + l];
   }
 #pragma unroll
   for (int q = 0; q < 8; ++q)
-    if (j0 + q < nout) out[(int64_t)(j0 + q) * ld + i] = acc[q];
+    if (j0 + q < nout) out[(int64_t)(j0 + q) * ld + i] = (VT)acc[q];
 }
 __global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1722,6 +1725,16 @@ __global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, doubl
 __global__ void k_scale_to(int64_t n, const double *__restrict__ x, double s, double *__restrict__ y) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) y[i] = s * x[i];
+}
+// y = s x stored as VT, and the stored value also into y2 (FP64) when given
+template <typename VT>
+__global__ void k_scale_to_basis(int64_t n, const double *__restrict__ x, double s, VT *__restrict__ y,
+                                 double *__restrict__ y2) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const VT v = (VT)(s * x[i]);
+  y[i] = v;
+  if (y2) y2[i] = (double)v;
 }
 // v /= *nrm when *nrm > 0 (the Arnoldi normalisation, norm kept on the device)
 __global__ void k_scale_inv(int64_t n, double *__restrict__ v, const double *__restrict__ nrm) {
@@ -1736,23 +1749,25 @@ __global__ void k_scale_inv(int64_t n, double *__restrict__ v, const double *__r
 // into part[j * P + range].
 constexpr int kDotParts = 16;   // per column of V . w (consumers re-read nv * 16 partials)
 constexpr int kNormParts = 64;  // of |w|^2 (one wavefront finishes them)
-__global__ void __launch_bounds__(kRedThreads) k_mdot_parts(int64_t n, int64_t ldv, const double *__restrict__ V,
+template <typename VT>
+__global__ void __launch_bounds__(kRedThreads) k_mdot_parts(int64_t n, int64_t ldv, const VT *__restrict__ V,
                                                             const double *__restrict__ w, double *__restrict__ part) {
   __shared__ double sh[kRedThreads / 64];
   const int j = blockIdx.y, P = (int)gridDim.x;
-  const double *v = V + (int64_t)j * ldv;
+  const VT *v = V + (int64_t)j * ldv;
   const int64_t len = (n + P - 1) / P;
   const int64_t beg = (int64_t)blockIdx.x * len, end = min(n, beg + len);
   double acc = 0;
 #pragma unroll 4
-  for (int64_t i = beg + threadIdx.x; i < end; i += kRedThreads) acc += v[i] * w[i];
+  for (int64_t i = beg + threadIdx.x; i < end; i += kRedThreads) acc += (double)v[i] * w[i];
   const double r = block_sum(acc, sh);
   if (threadIdx.x == 0) part[(int64_t)j * P + blockIdx.x] = r;
 }
 // w -= sum_j V_j h_j, h_j the sum of column j's P partials, summed by every
 // block in the same order; block 0 also stores h (a Hessenberg column).
 // nv <= 128 (GMRES restart <= 127).
-__global__ void __launch_bounds__(256) k_gemv_sub_fin(int64_t n, int64_t ldv, int nv, const double *__restrict__ V,
+template <typename VT>
+__global__ void __launch_bounds__(256) k_gemv_sub_fin(int64_t n, int64_t ldv, int nv, const VT *__restrict__ V,
                                                       const double *__restrict__ part, int P,
                                                       double *__restrict__ h_out, double *__restrict__ w) {
   __shared__ double hs[128];
@@ -1766,7 +1781,7 @@ __global__ void __launch_bounds__(256) k_gemv_sub_fin(int64_t n, int64_t ldv, in
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   double acc = w[i];
-  for (int j = 0; j < nv; ++j) acc -= V[(int64_t)j * ldv + i] * hs[j];
+  for (int j = 0; j < nv; ++j) acc -= (double)V[(int64_t)j * ldv + i] * hs[j];
   w[i] = acc;
 }
 // v /= |v| with |v|^2 the sum of P <= 64 partials (one wavefront per block
@@ -1786,9 +1801,10 @@ __global__ void __launch_bounds__(256) k_scale_fin(int64_t n, double *__restrict
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n && s > 0) v[i] /= s;
 }
-// k_scale_fin from w into two places: the new basis vector and the next
-// step's fixed input (the graph-replayed operator's)
-__global__ void __launch_bounds__(256) k_scale_fin2(int64_t n, const double *__restrict__ w, double *__restrict__ v,
+// k_scale_fin from w into two places: the new basis vector (stored as VT)
+// and, as stored, the next step's fixed input (the operator's, FP64)
+template <typename VT>
+__global__ void __launch_bounds__(256) k_scale_fin2(int64_t n, const double *__restrict__ w, VT *__restrict__ v,
                                                     double *__restrict__ v2, const double *__restrict__ part, int P,
                                                     double *__restrict__ nrm_out) {
   __shared__ double s_nrm;
@@ -1803,9 +1819,9 @@ __global__ void __launch_bounds__(256) k_scale_fin2(int64_t n, const double *__r
   const double s = s_nrm;
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) {
-    const double x = s > 0 ? w[i] / s : w[i];
+    const VT x = (VT)(s > 0 ? w[i] / s : w[i]);
     v[i] = x;
-    v2[i] = x;
+    v2[i] = (double)x;
   }
 }
 
@@ -3254,12 +3270,20 @@ struct Recycle {
 
 // r0 (device) with norm r0n, when given, is the residual b - A x of the
 // starting x: the first cycle then skips its operator application.
+// VT: the basis vectors' storage, double, or float for the refinement's
+// inner solves (half the bytes the Gram-Schmidt kernels stream, which read
+// the basis four times per step; orthogonality to FP32 rounding is far
+// below an inner solve's stopping reduction).  Every dot, projection and
+// update runs in FP64, and the operator reads each new basis vector as
+// stored, so the Arnoldi relation holds for the stored basis.
+template <typename VT>
 static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int m, int maxit, int precision,
                       int &it, double &est, double switch_abs = -1.0, const double *r0 = nullptr, double r0n = 0,
                       Recycle *rc = nullptr) {
   const int64_t n = e.n;
   const unsigned g = nblk(n, 256);
-  double *V = e.d_V.as<double>(), *w = e.d_w.as<double>(), *hd = e.d_h.as<double>();
+  VT *V = e.d_V.as<VT>();
+  double *w = e.d_w.as<double>(), *hd = e.d_h.as<double>();
   // recycled vectors in front of the basis (first cycle only)
   int kc = rc && rc->kc > 0 && rc->kc < m ? rc->kc : 0;
   std::vector<double> Ecol, cproj;  // coefficients on C of each step; C^T r
@@ -3288,21 +3312,27 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
   // into a fixed output, so that it can be replayed as a graph
   // (Engine::step_op_graph); the fused vmult is the one a graph can hold
   const bool graphs = e.graph_enabled && !e.time_phases && (int64_t)e.near.rows.size() == n;
-  if (graphs) {
+  // the operator from d_vn into d_wk: for graphs, and for an FP32 basis
+  // (the operator reads FP64 vectors)
+  constexpr bool f32_basis = sizeof(VT) == 4;
+  const bool fixed_io = graphs || f32_basis;
+  if (fixed_io) {
     if (e.d_vn.bytes < (size_t)n * 8) alloc(e.d_vn, (size_t)n * 8);
     if (e.d_wk.bytes < (size_t)n * 8) alloc(e.d_wk, (size_t)n * 8);
   }
   // Arnoldi step k: V_k+1 from V_k (the basis after the kc recycled vectors),
   // its Hessenberg column (the coefficients on C first) into slot k
   auto enqueue_step = [&](int k) {
-    double *Vb = V + (int64_t)kc * n;
-    double *vk1 = Vb + (int64_t)(k + 1) * n;
+    VT *Vb = V + (int64_t)kc * n;
+    VT *vk1 = Vb + (int64_t)(k + 1) * n;
     const int pk = step_prec();
-    double *wv = vk1;  // the new vector before its normalisation
+    double *wv = e.d_wk.as<double>();  // the new vector before its normalisation
     if (graphs) {
       e.step_op_graph(pk);  // d_vn (= V_k) -> d_wk
-      wv = e.d_wk.as<double>();
+    } else if constexpr (f32_basis) {
+      e.step_op(e.d_vn.as<double>(), wv, pk);
     } else {
+      wv = vk1;
       e.step_op(Vb + (int64_t)k * n, vk1, pk);
     }
     ++it;
@@ -3312,15 +3342,20 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
     double *h1 = slot, *h2 = slot + (m + 2), *hn_d = slot + 2 * (m + 2);
     const int nv = kc + k + 1;
     for (double *hj : {h1, h2}) {
-      hipLaunchKernelGGL(k_mdot_parts, dim3(kDotParts, nv), dim3(kRedThreads), 0, e.stream, n, n, V, wv, cg);
-      hipLaunchKernelGGL(k_gemv_sub_fin, dim3(g), dim3(256), 0, e.stream, n, n, nv, V, cg, kDotParts, hj, wv);
+      hipLaunchKernelGGL(k_mdot_parts<VT>, dim3(kDotParts, nv), dim3(kRedThreads), 0, e.stream, n, n, V, wv, cg);
+      hipLaunchKernelGGL(k_gemv_sub_fin<VT>, dim3(g), dim3(256), 0, e.stream, n, n, nv, V, cg, kDotParts, hj, wv);
     }
-    hipLaunchKernelGGL(k_mdot_parts, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, wv, wv, cg);
-    if (graphs)
-      hipLaunchKernelGGL(k_scale_fin2, dim3(g), dim3(256), 0, e.stream, n, wv, vk1, e.d_vn.as<double>(), cg,
+    hipLaunchKernelGGL(k_mdot_parts<double>, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, wv, wv, cg);
+    if constexpr (f32_basis) {
+      hipLaunchKernelGGL(k_scale_fin2<VT>, dim3(g), dim3(256), 0, e.stream, n, wv, vk1, e.d_vn.as<double>(), cg,
                          kNormParts, hn_d);
-    else
-      hipLaunchKernelGGL(k_scale_fin, dim3(g), dim3(256), 0, e.stream, n, vk1, cg, kNormParts, hn_d);
+    } else {
+      if (graphs)
+        hipLaunchKernelGGL(k_scale_fin2<double>, dim3(g), dim3(256), 0, e.stream, n, wv, vk1, e.d_vn.as<double>(),
+                           cg, kNormParts, hn_d);
+      else
+        hipLaunchKernelGGL(k_scale_fin, dim3(g), dim3(256), 0, e.stream, n, vk1, cg, kNormParts, hn_d);
+    }
   };
   while (it < maxit) {
     // r = b - A x into V_0
@@ -3344,11 +3379,11 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
       double *ps = hd + (size_t)(m - 1) * S;
       HIPCHK(hipMemcpyAsync(w, rv, (size_t)n * 8, hipMemcpyDeviceToDevice, e.stream));
       for (double *pj : {ps, ps + (m + 2)}) {
-        hipLaunchKernelGGL(k_mdot_parts, dim3(kDotParts, kc), dim3(kRedThreads), 0, e.stream, n, n, V, w, cg);
-        hipLaunchKernelGGL(k_gemv_sub_fin, dim3(g), dim3(256), 0, e.stream, n, n, kc, V, cg, kDotParts, pj, w);
+        hipLaunchKernelGGL(k_mdot_parts<VT>, dim3(kDotParts, kc), dim3(kRedThreads), 0, e.stream, n, n, V, w, cg);
+        hipLaunchKernelGGL(k_gemv_sub_fin<VT>, dim3(g), dim3(256), 0, e.stream, n, n, kc, V, cg, kDotParts, pj, w);
       }
-      hipLaunchKernelGGL(k_mdot_parts, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, w, w, cg);
-      hipLaunchKernelGGL(k_scale_fin2, dim3(g), dim3(256), 0, e.stream, n, w, V + (int64_t)kc * n,
+      hipLaunchKernelGGL(k_mdot_parts<double>, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, w, w, cg);
+      hipLaunchKernelGGL(k_scale_fin2<VT>, dim3(g), dim3(256), 0, e.stream, n, w, V + (int64_t)kc * n,
                          e.d_vn.as<double>(), cg, kNormParts, ps + 2 * (m + 2));
       std::vector<double> pr((size_t)S);
       HIPCHK(hipMemcpyAsync(pr.data(), ps, (size_t)S * 8, hipMemcpyDeviceToHost, e.stream));
@@ -3359,9 +3394,8 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
       est = beta;
       Ecol.clear();
     } else {
-      hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, rv, 1.0 / beta, V);
-      if (graphs)
-        hipLaunchKernelGGL(k_scale_to, dim3(g), dim3(256), 0, e.stream, n, rv, 1.0 / beta, e.d_vn.as<double>());
+      hipLaunchKernelGGL(k_scale_to_basis<VT>, dim3(g), dim3(256), 0, e.stream, n, rv, 1.0 / beta, V,
+                         fixed_io ? e.d_vn.as<double>() : nullptr);
     }
     if (rc && rc->record) rc->Hbar.clear();
     const int mk = m - kc;  // steps this cycle
@@ -3464,21 +3498,21 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
       if (e.d_zr.bytes < (size_t)kc * 8) alloc(e.d_zr, (size_t)std::max(kc, 16) * 8);
       HIPCHK(hipMemcpyAsync(e.d_zr.p, zr.data(), (size_t)kc * 8, hipMemcpyHostToDevice, e.stream));
     }
-    const double *Vb = V + (int64_t)kc * n;
+    const VT *Vb = V + (int64_t)kc * n;
     if (e.have_prec) {
       // x += M^-1 (V y [+ V_kc z])
       double *t = e.d_ptmp2.as<double>(), *z = e.d_ptmp.as<double>();
       hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, e.stream, t, n, 0.0);
-      hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, Vb, ybuf, t);
+      hipLaunchKernelGGL(k_gemv_add<VT>, dim3(g), dim3(256), 0, e.stream, n, n, k, Vb, ybuf, t);
       if (kc > 0)
-        hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, kc, e.d_rec.as<double>(),
+        hipLaunchKernelGGL(k_gemv_add<VT>, dim3(g), dim3(256), 0, e.stream, n, n, kc, e.d_rec.as<VT>(),
                            e.d_zr.as<double>(), t);
       e.apply_prec(t, z);
       hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, e.stream, n, 1.0, z, 1.0, x);
     } else {
-      hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, k, Vb, ybuf, x);
+      hipLaunchKernelGGL(k_gemv_add<VT>, dim3(g), dim3(256), 0, e.stream, n, n, k, Vb, ybuf, x);
       if (kc > 0)
-        hipLaunchKernelGGL(k_gemv_add, dim3(g), dim3(256), 0, e.stream, n, n, kc, e.d_rec.as<double>(),
+        hipLaunchKernelGGL(k_gemv_add<VT>, dim3(g), dim3(256), 0, e.stream, n, n, kc, e.d_rec.as<VT>(),
                            e.d_zr.as<double>(), x);
     }
     if (rc && rc->record) rc->k = k;
@@ -3494,6 +3528,7 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
 // Gram-Schmidt with reorthogonalisation on the host, the old V_k copied to
 // d_rec, C = V_k+1 Q formed beside it and moved to the front of d_V.  Returns
 // false (no recycling) when R is numerically singular.
+template <typename VT>
 static bool prepare_recycle(Engine &e, Recycle &rec, int m) {
   const int k = rec.k;
   if (k <= 0 || k >= m) return false;
@@ -3521,14 +3556,14 @@ static bool prepare_recycle(Engine &e, Recycle &rec, int m) {
   }
   for (int j = 0; j < k; ++j)
     if (!(R[(size_t)j * k + j] > 1e-12 * rmax)) return false;
-  if (e.d_rec.bytes < (size_t)2 * k * n * 8) alloc(e.d_rec, (size_t)2 * k * n * 8);
+  if (e.d_rec.bytes < (size_t)2 * k * n * sizeof(VT)) alloc(e.d_rec, (size_t)2 * k * n * sizeof(VT));
   if (e.d_qr.bytes < Q.size() * 8) alloc(e.d_qr, Q.size() * 8);
-  double *V = e.d_V.as<double>(), *U = e.d_rec.as<double>(), *Ct = U + (int64_t)k * n;
-  HIPCHK(hipMemcpyAsync(U, V, (size_t)k * n * 8, hipMemcpyDeviceToDevice, e.stream));
+  VT *V = e.d_V.as<VT>(), *U = e.d_rec.as<VT>(), *Ct = U + (int64_t)k * n;
+  HIPCHK(hipMemcpyAsync(U, V, (size_t)k * n * sizeof(VT), hipMemcpyDeviceToDevice, e.stream));
   HIPCHK(hipMemcpyAsync(e.d_qr.p, Q.data(), Q.size() * 8, hipMemcpyHostToDevice, e.stream));
-  hipLaunchKernelGGL(k_tsgemm8, dim3(nblk(n, 256), (k + 7) / 8), dim3(256), 0, e.stream, n, n, k + 1, V,
+  hipLaunchKernelGGL(k_tsgemm8<VT>, dim3(nblk(n, 256), (k + 7) / 8), dim3(256), 0, e.stream, n, n, k + 1, V,
                      e.d_qr.as<double>(), k, Ct);
-  HIPCHK(hipMemcpyAsync(V, Ct, (size_t)k * n * 8, hipMemcpyDeviceToDevice, e.stream));
+  HIPCHK(hipMemcpyAsync(V, Ct, (size_t)k * n * sizeof(VT), hipMemcpyDeviceToDevice, e.stream));
   HIPCHK(hipGetLastError());
   rec.kc = k;
   rec.R = std::move(R);
@@ -3656,27 +3691,32 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
         // below kRelaxSwitch tol ||b|| (inexact Krylov: the operator error a
         // step may carry grows as the residual shrinks); the outer FP64
         // residual catches what the FP32 steps leave
-        gmres_core(*this, r, d, tol * bn * 0.5, restart, maxit, PBFMM_MIXED, it, est,
-                   std::min(rn, relax_sw() * tol * bn), r, rn);
+        gmres_core<double>(*this, r, d, tol * bn * 0.5, restart, maxit, PBFMM_MIXED, it, est,
+                           std::min(rn, relax_sw() * tol * bn), r, rn);
       } else {
         // iterative refinement: FP32-operator GMRES on the FP64 residual;
         // the second inner solve recycles the first one's Krylov space
         const double tin = inner_target(tol, bn, rn);
         Recycle *rp = nullptr;
         const bool recyc = recycle && mixed_scheme != 3;  // 3: refinement without recycling
+        const bool f32b = mixed_scheme != 4;              // 4: refinement with an FP64 basis
         if (recyc && stage == 0) {
           rec = Recycle();
           rec.record = true;
           rp = &rec;
-        } else if (recyc && stage == 1 && prepare_recycle(*this, rec, restart)) {
+        } else if (recyc && stage == 1 &&
+                   (f32b ? prepare_recycle<float>(*this, rec, restart) : prepare_recycle<double>(*this, rec, restart))) {
           rp = &rec;
         }
-        gmres_core(*this, r, d, tin, restart, maxit, PBFMM_MIXED, it, est, -1.0, r, rn, rp);
+        if (f32b)
+          gmres_core<float>(*this, r, d, tin, restart, maxit, PBFMM_MIXED, it, est, -1.0, r, rn, rp);
+        else
+          gmres_core<double>(*this, r, d, tin, restart, maxit, PBFMM_MIXED, it, est, -1.0, r, rn, rp);
         ++stage;
       }
       hipLaunchKernelGGL(k_axpby, dim3(g), dim3(256), 0, stream, n, 1.0, d, 1.0, x);
     } else {
-      gmres_core(*this, b, x, tol * bn * 0.5, restart, maxit, PBFMM_FP64, it, est, -1.0, r, rn);
+      gmres_core<double>(*this, b, x, tol * bn * 0.5, restart, maxit, PBFMM_FP64, it, est, -1.0, r, rn);
     }
   }
   sync();
