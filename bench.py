@@ -59,7 +59,8 @@ PHASES = ("spmv", "p2m", "m2m", "l2l", "m2l", "m2p", "l2p", "m2l_sum")
 SPMV_KERNEL = {0: "k_spmv_lds<{t}, {k}, false>", 1: "k_spmv_wave<{t}, {k}, 2, false>",
                2: "k_spmv_wave<{t}, {k}, 4, false>", 3: "k_spmv_lds<{t}, {k}, true>",
                4: "k_spmv_wave<{t}, {k}, 2, true>", 5: "k_spmv_wave<{t}, {k}, 4, true>",
-               6: "k_spmv_quad<8>"}
+               6: "k_spmv_cols<float, 8>"}
+SPMV_KERNEL_FP64 = {3: "k_spmv_cols<double, 8>"}  # the FP64 variant numbers past 2
 KERNEL = {
     "p2m": "k_p2m<{t}, {k}>", "m2m": "k_translate<{t}, {p}, 0, {k}>", "m2l": "k_translate<{t}, {p}, 1, {k}>",
     "l2l": "k_translate<{t}, {p}, 2, {k}>", "m2p": "k_m2p<{t}, {k}, {p}>", "m2l_sum": "k_m2l_reduce<{t}>",
@@ -75,7 +76,8 @@ def kernel_name(phase, prec, p, nk, tuning):
     t = "float" if prec == "fp32" else "double"
     plan = int(tuning.get(f"side_fusion_{prec}", 0)) if nk == 1 else 0
     if phase == "spmv":
-        fmt = (SPMV_KERNEL[int(tuning.get(f"spmv_kernel_{prec}", 0))] if nk == 1
+        v = int(tuning.get(f"spmv_kernel_{prec}", 0))
+        fmt = (((SPMV_KERNEL_FP64 if prec == "fp64" and v > 2 else SPMV_KERNEL)[v]) if nk == 1
                else "k_spmv_lds/k_spmv_wave<{t}, {k}> (variant timed for the matvec)")
     elif phase == "l2p":
         fmt = "k_l2p_post<{t}, {p}>" if nk == 1 else "k_l2p<{t}, {k}, {p}>"

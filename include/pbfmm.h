@@ -321,8 +321,9 @@ int pbfmm_set_mixed_tolerances(pbfmm_handle *h, double inner_reduction, double r
  * 2 with 4 rows; FP32 also 3..5: kernels 0..2 on the operator's single values
  * kept in row pairs (two rows per 8-byte load; the (N, D) pair storage then
  * uses kernel - 3), and 6: workgroup per leaf on the single values kept in
- * column quads, a row per lane (four columns per 16-byte load; the pair
- * storage uses kernel 0).  For tests and tuning; call after
+ * column vectors, a row per lane (four columns per 16-byte load; the pair
+ * storage uses kernel 0); FP64 also 3: the same with two columns per
+ * 16-byte load.  For tests and tuning; call after
  * pbfmm_direct_integrals.  No counterpart in the reference (Trilinos
  * chooses its own SpMV). */
 int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel);

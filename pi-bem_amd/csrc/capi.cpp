@@ -1041,8 +1041,8 @@ int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel) {
     if (int rc = need(h, 3, "set_spmv_kernel")) return rc;
     if (precision != PBFMM_FP64 && precision != PBFMM_MIXED)
       return fail(h, PBFMM_ERR_ARG, "set_spmv_kernel: unknown precision");
-    if (kernel < 0 || kernel > (precision == PBFMM_MIXED ? 6 : 2))
-      return fail(h, PBFMM_ERR_ARG, "set_spmv_kernel: kernel must be 0..2 (FP64) or 0..6 (FP32)");
+    if (kernel < 0 || kernel > (precision == PBFMM_MIXED ? 6 : 3))
+      return fail(h, PBFMM_ERR_ARG, "set_spmv_kernel: kernel must be 0..3 (FP64) or 0..6 (FP32)");
     if (precision == PBFMM_MIXED && !h->eng.have_f32)
       return fail(h, PBFMM_ERR_STATE, "set_spmv_kernel: handle was not created with PBFMM_MIXED");
     const int pi = precision == PBFMM_MIXED ? 1 : 0;

@@ -36,6 +36,7 @@ struct ChunkDev {
 // Per-precision copies of the far-field operators and expansion storage.
 template <typename T>
 struct FarDev {
+  using value_type = T;
   DevBuf m2m, l2l, m2l;  // transposed operators
   DevBuf me, le;         // [nb][2 kernels][kp]
   DevBuf qx, qn, qw, qsh;  // regular quadrature data (FP64 copy only)
@@ -45,7 +46,8 @@ struct FarDev {
   // columns, U1 on Neumann / -U2 on Dirichlet DoFs ([column][kp])
   DevBuf opvals, opunit;
   DevBuf opvals_p;  // FP32 only: opvals in row pairs (k_round_f32_pairs)
-  DevBuf opvals_q;  // FP32 only: opvals in column quads (k_round_f32_quads)
+  // opvals in column vectors of 16 bytes (k_round_cols), per-leaf offsets
+  DevBuf opvals_q, qval_beg;
   DevBuf slots;  // slot M2L: [pair][2 Kt], on first use
 };
 
@@ -89,7 +91,6 @@ struct Engine {
   ChunkDev m2m_g_cl;  // M2M two child levels per launch (FarPlan::m2m_g)
   ChunkDev l2l_g_cl;  // L2L two target levels per launch (FarPlan::l2l_g)
   DevBuf d_pval_beg;  // FP32 operator values: per-leaf offsets of the row-pair layout
-  DevBuf d_qval_beg;  // ... and of the column-quad layout
   DevBuf d_row_leaf, d_row_beg, d_rows, d_col_beg, d_cols, d_val_beg, d_ent_beg, d_ent_cell,
       d_ent_mask, d_ent_near, d_ent_colpos, d_work_beg, d_leaf_block;
   DevBuf d_sing;  // singular rules: per js [w, N(dpc), dNx(dpc), dNy(dpc)] per point

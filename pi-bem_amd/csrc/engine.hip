@@ -277,26 +277,29 @@ __global__ void __launch_bounds__(256) k_round_f32_pairs(int nleaf, const int *_
   }
 }
 
-// The same values in column quads: leaf l's block goes to qval_beg[l] with
-// (r, c) at ((c / 4) R + r) 4 + c % 4, its columns zero-padded to a multiple
-// of four (R x 4 ceil(C / 4) floats, 16-byte aligned).  A lane (a row) then
-// loads four columns of its row with one 16-byte load, and a wavefront's
-// lanes read consecutive addresses (k_spmv_quad).  One workgroup per leaf.
-__global__ void __launch_bounds__(256) k_round_f32_quads(int nleaf, const int *__restrict__ row_beg,
-                                                         const int *__restrict__ col_beg,
-                                                         const int64_t *__restrict__ val_beg,
-                                                         const int64_t *__restrict__ qval_beg,
-                                                         const double *__restrict__ a, float *__restrict__ b) {
+// The operator's single values in column vectors of 16 bytes (CW = 4 FP32
+// or 2 FP64 columns): leaf l's block goes to qval_beg[l] with (r, c) at
+// ((c / CW) R + r) CW + c % CW, its columns zero-padded to a multiple of CW
+// (16-byte aligned).  A lane (a row) then loads CW columns of its row with
+// one 16-byte load, and a wavefront's lanes read consecutive addresses
+// (k_spmv_cols).  From the FP64 row-major copy; one workgroup per leaf.
+template <typename T>
+__global__ void __launch_bounds__(256) k_round_cols(int nleaf, const int *__restrict__ row_beg,
+                                                    const int *__restrict__ col_beg,
+                                                    const int64_t *__restrict__ val_beg,
+                                                    const int64_t *__restrict__ qval_beg, const double *__restrict__ a,
+                                                    T *__restrict__ b) {
+  constexpr int CW = 16 / sizeof(T);
   const int l = blockIdx.x;
   if (l >= nleaf) return;
   const int R = row_beg[l + 1] - row_beg[l], C = col_beg[l + 1] - col_beg[l];
-  const int64_t n4 = (int64_t)R * ((C + 3) / 4 * 4);
+  const int64_t nv = (int64_t)R * ((C + CW - 1) / CW * CW);
   const double *src = a + val_beg[l];
-  float *dst = b + qval_beg[l];
-  for (int64_t o = threadIdx.x; o < n4; o += blockDim.x) {
-    const int64_t q = o / (4 * (int64_t)R);
-    const int rem = (int)(o - q * 4 * R), r = rem >> 2, c = 4 * (int)q + (rem & 3);
-    dst[o] = c < C ? (float)src[(int64_t)r * C + c] : 0.0f;
+  T *dst = b + qval_beg[l];
+  for (int64_t o = threadIdx.x; o < nv; o += blockDim.x) {
+    const int64_t q = o / (CW * (int64_t)R);
+    const int rem = (int)(o - q * CW * R), r = rem / CW, c = CW * (int)q + rem % CW;
+    dst[o] = c < C ? (T)src[(int64_t)r * C + c] : T(0);
   }
 }
 
@@ -477,35 +480,42 @@ __global__ void __launch_bounds__(256) k_spmv_lds(int nleaf, int per_xcd, NearAr
   spmv_lds_leaf<T, NK, 256, PR>(leaf, xs, near_rhs(a));
 }
 
-// The FP32 operator's single values in column quads (k_round_f32_quads), one
-// workgroup per leaf: the leaf's inputs gathered once into LDS as in
-// k_spmv_lds, then lane = row and the four wavefronts split the column
-// quads, each lane streaming its row with 16-byte loads against the quads'
-// inputs broadcast from LDS; no wavefront sums.  A leaf of at most 32 (16)
-// rows runs two (four) lane groups per wavefront on alternate quads; one of
-// more than 64 rows runs in blocks of 64.  The 4 x groups partial sums of a
-// row meet in LDS.  Leaves dealt out per XCD as k_spmv_lds.
-template <int UNR>
-__global__ void __launch_bounds__(256) k_spmv_quad(int nleaf, int per_xcd, NearArgs<float> a) {
-  __shared__ float4 xs4[kSpmvTile / 4];
-  __shared__ float part[16][64];
+// The operator's single values in column vectors (k_round_cols: CW = 4 FP32
+// or 2 FP64 columns per 16 bytes), one workgroup per leaf: the leaf's inputs
+// gathered once into LDS as in k_spmv_lds, then lane = row and the four
+// wavefronts split the column vectors, each lane streaming its row with
+// 16-byte loads against the vectors' inputs broadcast from LDS; no wavefront
+// sums.  A leaf of at most 32 (16) rows runs two (four) lane groups per
+// wavefront on alternate vectors; one of more than 64 rows runs in blocks of
+// 64.  The 4 x groups partial sums of a row meet in LDS.  Leaves dealt out
+// per XCD as k_spmv_lds.
+__device__ __forceinline__ float vdot16(const float4 &m, const float4 &x) {
+  return (m.x * x.x + m.y * x.y) + (m.z * x.z + m.w * x.w);
+}
+__device__ __forceinline__ double vdot16(const double2 &m, const double2 &x) { return m.x * x.x + m.y * x.y; }
+template <typename T, int UNR>
+__global__ void __launch_bounds__(256) k_spmv_cols(int nleaf, int per_xcd, NearArgs<T> a) {
+  using TV = std::conditional_t<sizeof(T) == 4, float4, double2>;
+  constexpr int CW = 16 / sizeof(T);
+  constexpr int QTILE = kSpmvTile / CW;  // column vectors per LDS tile
+  __shared__ TV xsv[QTILE];
+  __shared__ T part[16][64];
   const int leaf = (int)(blockIdx.x % kXcd) * per_xcd + (int)(blockIdx.x / kXcd);
   if (leaf >= nleaf) return;
   a = near_rhs(a);
   const int t = (int)threadIdx.x, w = t >> 6, lane = t & 63;
   const int r0 = a.row_beg[leaf], R = a.row_beg[leaf + 1] - r0;
   const int cb = a.col_beg[leaf], C = a.col_beg[leaf + 1] - cb;
-  const int NQ = (C + 3) >> 2;
-  const float4 *V4 = reinterpret_cast<const float4 *>(a.vals + a.val_beg[leaf]);
-  float *xs = reinterpret_cast<float *>(xs4);
+  const int NQ = (C + CW - 1) / CW;
+  const TV *VV = reinterpret_cast<const TV *>(a.vals + a.val_beg[leaf]);
+  T *xs = reinterpret_cast<T *>(xsv);
   const int H = R <= 16 ? 4 : (R <= 32 ? 2 : 1), RL = 64 / H;
   const int h = lane / RL, rl = lane - h * RL;
-  const int S = 4 * H, s = w * H + h;  // quad streams of the workgroup, this lane's
-  constexpr int QTILE = kSpmvTile / 4;
+  const int S = 4 * H, s = w * H + h;  // vector streams of the workgroup, this lane's
   auto gather = [&](int q0, int QT) {
-    for (int c = t; c < 4 * QT; c += 256) {
-      const int cc = 4 * q0 + c;
-      xs[c] = cc < C ? near_x<float, 1>(a.phi, nullptr, a.cols[cb + cc]) : 0.0f;
+    for (int c = t; c < CW * QT; c += 256) {
+      const int cc = CW * q0 + c;
+      xs[c] = cc < C ? near_x<T, 1>(a.phi, nullptr, a.cols[cb + cc]) : T(0);
     }
   };
   const bool one_tile = NQ <= QTILE;
@@ -515,7 +525,7 @@ __global__ void __launch_bounds__(256) k_spmv_quad(int nleaf, int per_xcd, NearA
   }
   for (int rb = 0; rb < R; rb += 64) {
     const int r = rb + rl;
-    float acc = 0.0f;
+    T acc = 0;
     for (int q0 = 0; q0 < NQ; q0 += QTILE) {
       const int QT = min(QTILE, NQ - q0);
       if (!one_tile) {
@@ -524,19 +534,15 @@ __global__ void __launch_bounds__(256) k_spmv_quad(int nleaf, int per_xcd, NearA
         __syncthreads();
       }
       if (r < R) {
-        const float4 *v = V4 + (int64_t)q0 * R + r;
+        const TV *v = VV + (int64_t)q0 * R + r;
 #pragma unroll UNR
-        for (int q = s; q < QT; q += S) {
-          const float4 m = v[(int64_t)q * R];
-          const float4 xq = xs4[q];
-          acc += (m.x * xq.x + m.y * xq.y) + (m.z * xq.z + m.w * xq.w);
-        }
+        for (int q = s; q < QT; q += S) acc += vdot16(v[(int64_t)q * R], xsv[q]);
       }
     }
     part[s][lane] = acc;
     __syncthreads();
     if (t < min(RL, R - rb)) {
-      float z = 0.0f;
+      T z = 0;
       for (int ww = 0; ww < 4; ++ww)
         for (int g = 0; g < H; ++g) z += part[ww * H + g][g * RL + t];
       a.outN[a.rows[r0 + rb + t]] = (double)z;
@@ -2265,19 +2271,26 @@ void Engine::assemble_near(bool want_f32) {
       hipLaunchKernelGGL(k_round_f32_pairs, dim3(near.nleaf), dim3(256), 0, stream, near.nleaf, d_row_beg.as<int>(),
                          d_col_beg.as<int>(), d_val_beg.as<int64_t>(), d_pval_beg.as<int64_t>(),
                          f64.opvals.as<double>(), f32.opvals_p.as<float>());
-    // and in column quads
+    HIPCHK(hipGetLastError());
+  }
+  // and in column vectors, per precision
+  auto cols_copy = [&](auto &F) {
+    using T = typename std::remove_reference_t<decltype(F)>::value_type;
+    constexpr int CW = 16 / sizeof(T);
     std::vector<int64_t> qvb(near.nleaf + 1, 0);
     for (int l = 0; l < near.nleaf; ++l)
       qvb[l + 1] = qvb[l] + (int64_t)(near.row_beg[l + 1] - near.row_beg[l]) *
-                                ((near.col_beg[l + 1] - near.col_beg[l] + 3) / 4 * 4);
-    up(d_qval_beg, qvb);
-    alloc(f32.opvals_q, (size_t)std::max<int64_t>(qvb[near.nleaf], 4) * sizeof(float));
+                                ((near.col_beg[l + 1] - near.col_beg[l] + CW - 1) / CW * CW);
+    up(F.qval_beg, qvb);
+    alloc(F.opvals_q, (size_t)std::max<int64_t>(qvb[near.nleaf], CW) * sizeof(T));
     if (near.nleaf > 0)
-      hipLaunchKernelGGL(k_round_f32_quads, dim3(near.nleaf), dim3(256), 0, stream, near.nleaf, d_row_beg.as<int>(),
-                         d_col_beg.as<int>(), d_val_beg.as<int64_t>(), d_qval_beg.as<int64_t>(),
-                         f64.opvals.as<double>(), f32.opvals_q.as<float>());
+      hipLaunchKernelGGL(k_round_cols<T>, dim3(near.nleaf), dim3(256), 0, stream, near.nleaf, d_row_beg.as<int>(),
+                         d_col_beg.as<int>(), d_val_beg.as<int64_t>(), F.qval_beg.template as<int64_t>(),
+                         f64.opvals.as<double>(), F.opvals_q.template as<T>());
     HIPCHK(hipGetLastError());
-  }
+  };
+  cols_copy(f64);
+  if (want_f32) cols_copy(f32);
   tune_spmv<double, 2>();
   tune_spmv<double, 1>();
   if (want_f32) {
@@ -2374,16 +2387,20 @@ static void launch_translate_side(Engine &e, const ChunkDev &cl, int c0, int c1,
 #undef PBFMM_TRS
 }
 
-// variant 3..5: the FP32 operator's values in row pairs, at their own (even)
-// per-leaf offsets; 6: in column quads
+// The SpMV variant on the operator's single values in column vectors
+// (k_spmv_cols): 6 for FP32 (after the row-pair forms 3..5), 3 for FP64.
+template <typename T>
+constexpr int kColsVariant = sizeof(T) == 4 ? 6 : 3;
+// variant 3..5 (FP32): the operator's values in row pairs, at their own
+// (even) per-leaf offsets; kColsVariant: in column vectors
 template <typename T, int NK>
 static NearArgs<T> near_args(Engine &e, const double *phi, const double *dphi, double *outN, double *outD,
                              int variant) {
-  const bool quads = kPairedF32<T, NK> && variant == 6;
-  const bool paired = kPairedF32<T, NK> && variant >= 3 && !quads;
+  const bool cols = NK == 1 && variant == kColsVariant<T>;
+  const bool paired = kPairedF32<T, NK> && variant >= 3 && !cols;
   const FarDev<T> &F = far_of<T>(e);
-  const T *vals = (NK == 2 ? F.vals : quads ? F.opvals_q : paired ? F.opvals_p : F.opvals).template as<T>();
-  const int64_t *vb = quads    ? e.d_qval_beg.as<int64_t>()
+  const T *vals = (NK == 2 ? F.vals : cols ? F.opvals_q : paired ? F.opvals_p : F.opvals).template as<T>();
+  const int64_t *vb = cols     ? F.qval_beg.template as<int64_t>()
                       : paired ? e.d_pval_beg.as<int64_t>()
                                : e.d_val_beg.as<int64_t>();
   return NearArgs<T>{e.d_row_beg.as<int>(), e.d_rows.as<int>(), e.d_col_beg.as<int>(), e.d_cols.as<int>(),
@@ -2396,12 +2413,14 @@ void Engine::launch_spmv(int variant, const double *phi, const double *dphi, dou
   NearArgs<T> a = near_args<T, NK>(*this, phi, dphi, outN, outD, variant);
   a.stride = n;  // R right-hand sides, n apart (blockIdx.y)
   if (!st) st = stream;
-  if constexpr (kPairedF32<T, NK>) {  // 3..5: the kernels of 0..2 on the row pairs; 6 quads
-    if (variant == 6) {
+  if constexpr (NK == 1) {  // column vectors (k_spmv_cols)
+    if (variant == kColsVariant<T>) {
       const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
-      hipLaunchKernelGGL(k_spmv_quad<8>, dim3(per_xcd * kXcd, R), dim3(256), 0, st, near.nleaf, per_xcd, a);
+      hipLaunchKernelGGL((k_spmv_cols<T, 8>), dim3(per_xcd * kXcd, R), dim3(256), 0, st, near.nleaf, per_xcd, a);
       return;
     }
+  }
+  if constexpr (kPairedF32<T, NK>) {  // 3..5: the kernels of 0..2 on the row pairs
     if (variant == 3) {
       const int per_xcd = (near.nleaf + kXcd - 1) / kXcd;
       hipLaunchKernelGGL((k_spmv_lds<T, NK, true>), dim3(per_xcd * kXcd, R), dim3(256), 0, st, near.nleaf, per_xcd,
@@ -2440,7 +2459,7 @@ void Engine::tune_spmv() {
   HIPCHK(hipEventCreate(&e1));
   float best = 0;
   int pick = 0;
-  for (int v = 0; v < (kPairedF32<T, NK> ? 7 : 3); ++v) {
+  for (int v = 0; v < (NK == 2 ? 3 : kColsVariant<T> + 1); ++v) {
     for (int rep = 0; rep < 2; ++rep)
       launch_spmv<T, NK>(v, d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
     HIPCHK(hipEventRecord(e0, stream));

@@ -122,7 +122,7 @@ class BEMFMA:
         """Overrides the autotuned near-field SpMV kernel for one precision: 0
         workgroup per leaf, 1 / 2 wavefront per leaf with 2 / 4 rows in
         flight; FP32 also 3..5, the same on the operator's values in row
-        pairs, and 6, workgroup per leaf on them in column quads
+        pairs, and 6, workgroup per leaf on them in column vectors (FP64: 3)
         (pbfmm_set_spmv_kernel)."""
         self._chk(self._L.pbfmm_set_spmv_kernel(self._h, int(precision), int(kernel)))
 

@@ -407,7 +407,7 @@ def test_fused_operator_pipeline(name):
     dst = hip.DeviceArray(n)
     for prec, bar in ((P.PBFMM_FP64, 1e-14), (P.PBFMM_MIXED, 1e-6)):
         ref = None
-        for kernel in ((0, 1, 2) if prec == P.PBFMM_FP64 else (0, 1, 2, 3, 4, 5, 6)):
+        for kernel in ((0, 1, 2, 3) if prec == P.PBFMM_FP64 else (0, 1, 2, 3, 4, 5, 6)):
             f.set_spmv_kernel(prec, kernel)
             f.set_phase_timing(True)
             bp.vmult_dev(src.value, dst.value, prec)
@@ -448,7 +448,7 @@ def test_matvec_spmv_kernels(name):
     oN, oD = hip.DeviceArray(n), hip.DeviceArray(n)
     for prec, bar in ((P.PBFMM_FP64, 1e-14), (P.PBFMM_MIXED, 1e-6)):
         ref = None
-        for kernel in ((0, 1, 2) if prec == P.PBFMM_FP64 else (0, 1, 2, 3, 4, 5, 6)):
+        for kernel in ((0, 1, 2, 3) if prec == P.PBFMM_FP64 else (0, 1, 2, 3, 4, 5, 6)):
             f.set_spmv_kernel(prec, kernel)
             f.matvec_dev(a.value, b.value, oN.value, oD.value, prec)
             hip.synchronize()
@@ -462,11 +462,12 @@ def test_matvec_spmv_kernels(name):
 
 @pytest.mark.parametrize("leaf", [12, 40, 400])
 def test_operator_spmv_layouts(leaf):
-    """The operator (vmult_dev, FP32) with the near field's single values
-    row-major (kernel 0), in row pairs (3) and in column quads (6): leaves of
-    at most 16 rows (four lane groups per wavefront in kernel 6), up to 40
-    rows (two groups or one), and up to 400 rows with more than 2048 columns
-    (row blocks of 64, column tiles): within 1e-6 of kernel 0."""
+    """The operator (vmult_dev) with the near field's single values
+    row-major (kernel 0), in FP32 row pairs (3) and in column vectors (FP32
+    kernel 6, four columns per 16 bytes; FP64 kernel 3, two): leaves of at
+    most 16 rows (four lane groups per wavefront), up to 40 rows (two groups
+    or one), and up to 400 rows with more than 2048 columns (row blocks of
+    64, column tiles): FP32 within 1e-6, FP64 within 1e-14 of kernel 0."""
     from pibem_amd import hip
     pr = M.config_sphere(24, 6)
     pr.max_nodes_per_leaf = leaf
@@ -479,16 +480,17 @@ def test_operator_spmv_layouts(leaf):
     wc = f.work_counts()
     print(f"\n[case] sphere N={n} leaf {leaf}: {wc['leaves']} leaves, "
           f"{wc['near_values'] / max(wc['near_rows'], 1):.0f} values per row", flush=True)
-    ref = None
-    for kernel in (0, 3, 6):
-        f.set_spmv_kernel(P.PBFMM_MIXED, kernel)
-        bp.vmult_dev(src.value, dst.value, P.PBFMM_MIXED)
-        hip.synchronize()
-        y = dst.numpy()
-        ref = y if ref is None else ref
-        e = rel(y, ref)
-        print(f"  kernel {kernel}: vs kernel 0 {e:.2e}")
-        assert np.all(np.isfinite(y)) and e <= 1e-6
+    for prec, kernels, bar in ((P.PBFMM_MIXED, (0, 3, 6), 1e-6), (P.PBFMM_FP64, (0, 3), 1e-14)):
+        ref = None
+        for kernel in kernels:
+            f.set_spmv_kernel(prec, kernel)
+            bp.vmult_dev(src.value, dst.value, prec)
+            hip.synchronize()
+            y = dst.numpy()
+            ref = y if ref is None else ref
+            e = rel(y, ref)
+            print(f"  precision={prec} kernel {kernel}: vs kernel 0 {e:.2e}")
+            assert np.all(np.isfinite(y)) and e <= bar
     f.close()
 
 
