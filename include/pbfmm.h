@@ -300,9 +300,11 @@ int pbfmm_set_preconditioner_rows(pbfmm_handle *h, int32_t rows);
  * operator until the residual is below 2e5 * rel_tol * ||b||, then the FP32
  * operator, in one Krylov space); 2 iterative refinement (FP32-operator GMRES
  * on FP64 residuals, the second inner solve recycling the first one's Krylov
- * space; the inner solves keep their basis vectors in FP32, their arithmetic
- * in FP64); 3 the same without recycling; 4 scheme 2 with FP64 basis
- * vectors.  Every scheme ends on the true FP64 residual. */
+ * space); 3 the same without recycling; 4 scheme 2 with the inner solves'
+ * basis vectors stored in FP32 (their arithmetic in FP64; an option, since
+ * the FP64 Arnoldi of the other schemes is what the mixed mode promises).
+ * Every scheme ends on the true FP64 residual.  Mixed solves apply the
+ * block-Jacobi inverses stored in FP32, FP64 solves the FP64 copy. */
 int pbfmm_set_mixed_scheme(pbfmm_handle *h, int32_t scheme);
 
 /* The two thresholds of the mixed-precision schemes (no counterpart in the

@@ -120,9 +120,10 @@ struct Engine {
   // near-field block-Jacobi preconditioner: blocks of the near leaves that
   // share a parent block (at most prec_rows rows, leaf after leaf; a larger
   // leaf alone), per block the inverse of its rows x rows near-field block
-  // (inverted in FP64, stored in FP32 column quads at pquad_beg[block]:
-  // pinv_quad_index; pinv_beg[block]: the R x R blocks' own offsets, as
-  // assembled); prow_beg / prows / prow_blk: the
+  // (inverted in FP64; d_pinv[0] FP64 in column pairs for FP64 solves,
+  // d_pinv[1] FP32 in column quads for mixed solves, at d_pvec_beg[copy]
+  // [block]: pinv_vec_index; pinv_beg[block]: the R x R blocks' own
+  // offsets, as assembled); prow_beg / prows / prow_blk: the
   // blocks' row ranges, rows in block order, block of each of them
   // Row cap of a preconditioner block (k_bjacobi_wg takes up to 128, one or
   // two rows per lane; pbfmm_set_preconditioner_rows).  Larger blocks cut
@@ -133,10 +134,11 @@ struct Engine {
   // 20 (siblings hold at most ~64 rows): 91.6 / 86.5 / 84.4 / 84.5 / 84.6.
   int prec_rows = 128;
   bool have_prec = false;
+  bool prec_f32 = false;  // set for the duration of a mixed solve: apply d_pinv[1]
   int nprec = 0;
-  std::vector<int64_t> pinv_beg, pquad_beg;
+  std::vector<int64_t> pinv_beg;
   std::vector<int32_t> prow_beg, prows;
-  DevBuf d_pinv_beg, d_pquad_beg, d_pinv, d_ptmp, d_ptmp2, d_prow_beg, d_prows, d_prow_blk;
+  DevBuf d_pinv_beg, d_pvec_beg[2], d_pinv[2], d_ptmp, d_ptmp2, d_prow_beg, d_prows, d_prow_blk;
   void build_preconditioner();
   std::vector<int32_t> prec_pos;  // block-ordered row -> leaf-ordered near row
   void prec_blocks(std::vector<double> &blk);
@@ -243,8 +245,8 @@ struct Engine {
   // Krylov space) when the FP64 operator is not much dearer than the FP32
   // one, else iterative refinement; the cost ratio is timed once
   double op_cost_ratio = 0;
-  // 0 chosen by the timed cost ratio, 1 relaxed, 2 refinement (FP32 basis,
-  // recycling), 3 without recycling, 4 with an FP64 basis
+  // 0 chosen by the timed cost ratio, 1 relaxed, 2 refinement (recycling),
+  // 3 without recycling, 4 recycling with an FP32 basis in the inner solves
   int mixed_scheme = 0;
   double inner_reduction = 0;  // refinement's inner residual reduction (0: the default)
   double relax_switch = 0;     // relaxed scheme's FP32 switch over tol ||b|| (0: the default)

@@ -1562,77 +1562,94 @@ __global__ void k_extract_groups(const int *__restrict__ grp_beg, const int *__r
   }
 }
 
-// The inverse blocks are stored in FP32 and applied in FP64 (a fixed linear
-// preconditioner within FP32 rounding of the inverse, which GMRES with right
-// preconditioning applies to its basis and to the update alike), in column
-// quads: entry (r, c) of an R-row block at quad_beg + ((c / 4) R + r) 4 +
-// c % 4, columns zero-padded to a multiple of four, so that a lane reads
-// four columns of its row in one 16-byte load and a wavefront's lanes (rows)
-// read consecutive addresses.
-__host__ __device__ inline int64_t pinv_quad_index(int r, int c, int R) {
-  return ((int64_t)(c >> 2) * R + r) * 4 + (c & 3);
+// The inverse blocks are stored in column vectors of 16 bytes (PT = double:
+// two columns, the FP64 solves' copy; PT = float: four, the mixed solves'
+// copy, a fixed linear preconditioner within FP32 rounding of the inverse,
+// which GMRES with right preconditioning applies to its basis and to the
+// update alike) and applied in FP64: entry (r, c) of an R-row block at
+// vec_beg + ((c / CW) R + r) CW + c % CW, columns zero-padded to a multiple
+// of CW, so that a lane reads CW columns of its row in one 16-byte load and
+// a wavefront's lanes (rows) read consecutive addresses.
+template <typename PT>
+__host__ __device__ inline int64_t pinv_vec_index(int r, int c, int R) {
+  constexpr int CW = 16 / sizeof(PT);
+  return ((int64_t)(c / CW) * R + r) * CW + c % CW;
+}
+__device__ __forceinline__ void pinv_load(const float *p, double *a) {
+  const float4 u = *reinterpret_cast<const float4 *>(p);
+  a[0] = u.x, a[1] = u.y, a[2] = u.z, a[3] = u.w;
+}
+__device__ __forceinline__ void pinv_load(const double *p, double *a) {
+  const double2 u = *reinterpret_cast<const double2 *>(p);
+  a[0] = u.x, a[1] = u.y;
 }
 
 // z[rows of block] = Pinv_block . v[rows of block]; one thread per block-ordered
 // row (row_leaf: its block)
+template <typename PT>
 __global__ void __launch_bounds__(256) k_bjacobi(int nrows, const int *__restrict__ row_leaf,
                                                  const int *__restrict__ row_beg, const int *__restrict__ rows,
-                                                 const int64_t *__restrict__ blk_beg,
-                                                 const float *__restrict__ pinv, const double *__restrict__ v,
-                                                 double *__restrict__ z) {
+                                                 const int64_t *__restrict__ blk_beg, const PT *__restrict__ pinv,
+                                                 const double *__restrict__ v, double *__restrict__ z) {
+  constexpr int CW = 16 / sizeof(PT);
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nrows) return;
   const int leaf = row_leaf[g];
   const int r0 = row_beg[leaf], R = row_beg[leaf + 1] - r0, r = g - r0;
-  const float4 *P = reinterpret_cast<const float4 *>(pinv + blk_beg[leaf]) + r;
+  const PT *P = pinv + blk_beg[leaf] + (int64_t)r * CW;
   double acc = 0.0;
-  for (int c = 0; c < R; c += 4) {
-    const float4 a = P[(int64_t)(c >> 2) * R];
-    acc += (double)a.x * v[rows[r0 + c]];
-    if (c + 1 < R) acc += (double)a.y * v[rows[r0 + c + 1]];
-    if (c + 2 < R) acc += (double)a.z * v[rows[r0 + c + 2]];
-    if (c + 3 < R) acc += (double)a.w * v[rows[r0 + c + 3]];
+  for (int c = 0; c < R; c += CW) {
+    double a[CW];
+    pinv_load(P + (int64_t)(c / CW) * R * CW, a);
+#pragma unroll
+    for (int k = 0; k < CW; ++k)
+      if (c + k < R) acc += a[k] * v[rows[r0 + c + k]];
   }
   z[rows[g]] = acc;
 }
 // The same with one workgroup of four wavefronts per block (blocks of at
 // most 64 RPL rows, RPL rows per lane): every wavefront gathers the block's
-// inputs once (lane c: v at row c + 64 q) and sums the column quads
+// inputs once (lane c: v at row c + 64 q) and sums the column vectors
 // w, w + 4, ... of all rows (lane r: row r), the inputs broadcast from their
 // lanes; the four partial sums meet in LDS.
 // (right-hand side blockIdx.y: v and z + y stride)
-template <int RPL>
+template <int RPL, typename PT>
 __global__ void __launch_bounds__(256) k_bjacobi_wg(const int *__restrict__ row_beg, const int *__restrict__ rows,
                                                     const int64_t *__restrict__ blk_beg,
-                                                    const float *__restrict__ pinv, const double *__restrict__ v,
+                                                    const PT *__restrict__ pinv, const double *__restrict__ v,
                                                     double *__restrict__ z, int64_t stride) {
+  constexpr int CW = 16 / sizeof(PT);
   __shared__ double part[4][64 * RPL];
   const int blk = blockIdx.x;
   v += (int64_t)blockIdx.y * stride;
   z += (int64_t)blockIdx.y * stride;
   const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-  const int r0 = row_beg[blk], R = row_beg[blk + 1] - r0, nq = (R + 3) >> 2;
+  const int r0 = row_beg[blk], R = row_beg[blk + 1] - r0, nq = (R + CW - 1) / CW;
   double xv[RPL], acc[RPL];
-  const float4 *P[RPL];
+  const PT *P[RPL];
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
     const int row = lane + 64 * q;
     xv[q] = row < R ? v[rows[r0 + row]] : 0.0;  // zero past the block: the padded columns
-    P[q] = reinterpret_cast<const float4 *>(pinv + blk_beg[blk]) + (row < R ? row : 0);
+    P[q] = pinv + blk_beg[blk] + (int64_t)(row < R ? row : 0) * CW;
     acc[q] = 0.0;
   }
 #pragma unroll 8
   for (int cq = w; cq < nq; cq += 4) {
-    double xc[4];
+    double xc[CW];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int c = 4 * cq + k;
+    for (int k = 0; k < CW; ++k) {
+      const int c = CW * cq + k;
       xc[k] = RPL == 1 || c < 64 ? read_lane(xv[0], c & 63) : read_lane(xv[RPL - 1], c & 63);
     }
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
-      const float4 a = lane + 64 * q < R ? P[q][(int64_t)cq * R] : make_float4(0.f, 0.f, 0.f, 0.f);
-      acc[q] += ((double)a.x * xc[0] + (double)a.y * xc[1]) + ((double)a.z * xc[2] + (double)a.w * xc[3]);
+      double a[CW] = {};
+      if (lane + 64 * q < R) pinv_load(P[q] + (int64_t)cq * R * CW, a);
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < CW; ++k) s += a[k] * xc[k];
+      acc[q] += s;
     }
   }
 #pragma unroll
@@ -3118,13 +3135,19 @@ void Engine::build_preconditioner() {
   up(d_prows, prows);
   up(d_prow_blk, prow_blk);
   prec_pos = pos;
-  pquad_beg.assign(nprec + 1, 0);
-  for (int g = 0; g < nprec; ++g) {
-    const int64_t R = prow_beg[g + 1] - prow_beg[g];
-    pquad_beg[g + 1] = pquad_beg[g] + R * ((R + 3) / 4 * 4);
+  // per copy f (0 FP64 pairs, 1 FP32 quads) the blocks' offsets
+  std::vector<int64_t> pvb[2];
+  for (int f = 0; f < 2; ++f) {
+    const int CW = f ? 4 : 2;
+    pvb[f].assign(nprec + 1, 0);
+    for (int g = 0; g < nprec; ++g) {
+      const int64_t R = prow_beg[g + 1] - prow_beg[g];
+      pvb[f][g + 1] = pvb[f][g] + R * ((R + CW - 1) / CW * CW);
+    }
+    up(d_pvec_beg[f], pvb[f]);
   }
-  up(d_pquad_beg, pquad_beg);
-  std::vector<float> b32((size_t)pquad_beg[nprec], 0.0f);
+  std::vector<double> b64((size_t)pvb[0][nprec], 0.0);
+  std::vector<float> b32((size_t)pvb[1][nprec], 0.0f);
   std::vector<double> blk;
   prec_blocks(blk);
   // invert the blocks on host threads
@@ -3143,7 +3166,11 @@ void Engine::build_preconditioner() {
           }
       }
       for (int r = 0; r < R; ++r)
-        for (int c = 0; c < R; ++c) b32[pquad_beg[g] + pinv_quad_index(r, c, R)] = (float)A[(size_t)r * R + c];
+        for (int c = 0; c < R; ++c) {
+          const double a = A[(size_t)r * R + c];
+          b64[pvb[0][g] + pinv_vec_index<double>(r, c, R)] = a;
+          b32[pvb[1][g] + pinv_vec_index<float>(r, c, R)] = (float)a;
+        }
     }
   };
   const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -3155,26 +3182,40 @@ void Engine::build_preconditioner() {
       th.emplace_back(invert_range, (int)((int64_t)nprec * k / nth), (int)((int64_t)nprec * (k + 1) / nth));
     for (auto &x : th) x.join();
   }
-  up(d_pinv, b32);
+  up(d_pinv[0], b64);
+  up(d_pinv[1], b32);
   alloc(d_ptmp, n * 8);
   alloc(d_ptmp2, n * 8);
   have_prec = true;
 }
 
-void Engine::apply_prec(const double *v, double *z, int R) {
-  const int nrows = (int)prows.size();
-  if (max_leaf_rows <= 128) {  // R right-hand sides, n apart, in one launch
-    if (nprec > 0 && max_leaf_rows <= 64)
-      hipLaunchKernelGGL(k_bjacobi_wg<1>, dim3(nprec, R), dim3(256), 0, stream, d_prow_beg.as<int>(),
-                         d_prows.as<int>(), d_pquad_beg.as<int64_t>(), d_pinv.as<float>(), v, z, (int64_t)n);
-    else if (nprec > 0)
-      hipLaunchKernelGGL(k_bjacobi_wg<2>, dim3(nprec, R), dim3(256), 0, stream, d_prow_beg.as<int>(),
-                         d_prows.as<int>(), d_pquad_beg.as<int64_t>(), d_pinv.as<float>(), v, z, (int64_t)n);
+template <typename PT>
+static void apply_prec_copy(Engine &e, const double *v, double *z, int R) {
+  const int f = sizeof(PT) == 4 ? 1 : 0;
+  const int64_t *vb = e.d_pvec_beg[f].as<int64_t>();
+  const PT *pinv = e.d_pinv[f].as<PT>();
+  const int64_t n = e.n;
+  if (e.max_leaf_rows <= 128) {  // R right-hand sides, n apart, in one launch
+    if (e.nprec > 0 && e.max_leaf_rows <= 64)
+      hipLaunchKernelGGL((k_bjacobi_wg<1, PT>), dim3(e.nprec, R), dim3(256), 0, e.stream, e.d_prow_beg.as<int>(),
+                         e.d_prows.as<int>(), vb, pinv, v, z, n);
+    else if (e.nprec > 0)
+      hipLaunchKernelGGL((k_bjacobi_wg<2, PT>), dim3(e.nprec, R), dim3(256), 0, e.stream, e.d_prow_beg.as<int>(),
+                         e.d_prows.as<int>(), vb, pinv, v, z, n);
     return;
   }
-  for (int r = 1; r < R; ++r) apply_prec(v + (int64_t)r * n, z + (int64_t)r * n, 1);
-  hipLaunchKernelGGL(k_bjacobi, dim3(nblk(nrows, 256)), dim3(256), 0, stream, nrows, d_prow_blk.as<int>(),
-                     d_prow_beg.as<int>(), d_prows.as<int>(), d_pquad_beg.as<int64_t>(), d_pinv.as<float>(), v, z);
+  const int nrows = (int)e.prows.size();
+  for (int r = 0; r < R; ++r)
+    hipLaunchKernelGGL(k_bjacobi<PT>, dim3(nblk(nrows, 256)), dim3(256), 0, e.stream, nrows,
+                       e.d_prow_blk.as<int>(), e.d_prow_beg.as<int>(), e.d_prows.as<int>(), vb, pinv,
+                       v + (int64_t)r * n, z + (int64_t)r * n);
+}
+// The FP32 copy for mixed solves (prec_f32, set per solve), else the FP64 one
+void Engine::apply_prec(const double *v, double *z, int R) {
+  if (prec_f32)
+    apply_prec_copy<float>(*this, v, z, R);
+  else
+    apply_prec_copy<double>(*this, v, z, R);
 }
 
 // One Arnoldi step's operator: w = A M^-1 vin (right preconditioning).
@@ -3192,7 +3233,8 @@ void Engine::step_op(const double *vin, double *w, int precision) {
 // allocation count).
 std::vector<int64_t> Engine::graph_key(int precision) const {
   const int f = precision == PBFMM_MIXED && have_f32 ? 1 : 0;
-  return {precision, side_fusion[f], spmv_variant[f][0], have_prec ? 1 : 0, nprec, have_constraints ? 1 : 0,
+  return {precision, side_fusion[f], spmv_variant[f][0], have_prec ? 1 : 0, prec_f32 ? 1 : 0, nprec,
+          have_constraints ? 1 : 0,
           constraint_version, neumann_mean ? 1 : 0, have_dirichlet ? 1 : 0, g_alloc_epoch};
 }
 
@@ -3290,9 +3332,9 @@ struct Recycle {
 // r0 (device) with norm r0n, when given, is the residual b - A x of the
 // starting x: the first cycle then skips its operator application.
 // VT: the basis vectors' storage, double, or float for the refinement's
-// inner solves (half the bytes the Gram-Schmidt kernels stream, which read
-// the basis four times per step; orthogonality to FP32 rounding is far
-// below an inner solve's stopping reduction).  Every dot, projection and
+// inner solves when asked (mixed scheme 4: half the bytes the Gram-Schmidt
+// kernels stream, which read the basis four times per step; orthogonality
+// to FP32 rounding is far below an inner solve's stopping reduction).  Every dot, projection and
 // update runs in FP64, and the operator reads each new basis vector as
 // stored, so the Arnoldi relation holds for the stored basis.
 template <typename VT>
@@ -3684,6 +3726,8 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
     MeanMode(bool &f, bool v) : flag(f), old(f) { flag = v; }
     ~MeanMode() { flag = old; }
   } mean_mode(neumann_mean, !have_dirichlet);
+  // the preconditioner's FP32 copy for a mixed solve, its FP64 copy else
+  MeanMode prec_mode(prec_f32, precision == PBFMM_MIXED && have_f32);
   int it = 0;
   double est = 0, rr = 1;
   double *r = d_r.as<double>(), *d = d_x.as<double>();
@@ -3718,7 +3762,7 @@ int Engine::gmres(const double *b, double *x, double tol, int restart, int maxit
         const double tin = inner_target(tol, bn, rn);
         Recycle *rp = nullptr;
         const bool recyc = recycle && mixed_scheme != 3;  // 3: refinement without recycling
-        const bool f32b = mixed_scheme != 4;              // 4: refinement with an FP64 basis
+        const bool f32b = mixed_scheme == 4;              // 4: refinement with an FP32 basis
         if (recyc && stage == 0) {
           rec = Recycle();
           rec.record = true;
@@ -4051,6 +4095,12 @@ int Engine::gmres_forest(const double *b, double *x, double tol, int restart, in
   if (!d_ptmp.p) alloc(d_ptmp, n * 8);
   if (!d_ptmp2.p) alloc(d_ptmp2, n * 8);
   const bool mixed = precision == PBFMM_MIXED && have_f32;
+  struct PrecMode {  // the preconditioner's FP32 copy for a mixed solve
+    bool &flag;
+    bool old;
+    PrecMode(bool &f, bool v) : flag(f), old(f) { flag = v; }
+    ~PrecMode() { flag = old; }
+  } prec_mode(prec_f32, mixed);
   SegCtx c;
   c.N = n;
   c.S = S;
@@ -4238,6 +4288,12 @@ int Engine::gmres_multi(const double *b, double *x, int R, double tol, int resta
     ~MeanMode() { flag = old; }
   } mean_mode(neumann_mean, !have_dirichlet);
   const bool mixed = precision == PBFMM_MIXED && have_f32;
+  struct PrecMode {  // the preconditioner's FP32 copy for a mixed solve
+    bool &flag;
+    bool old;
+    PrecMode(bool &f, bool v) : flag(f), old(f) { flag = v; }
+    ~PrecMode() { flag = old; }
+  } prec_mode(prec_f32, mixed);
   std::vector<int> pit(R, 0);
   SegCtx c;
   c.N = N;

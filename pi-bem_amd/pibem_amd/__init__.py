@@ -108,8 +108,8 @@ class BEMFMA:
         """Mixed-precision solver scheme: 0 chosen by the timed FP64/FP32
         operator cost ratio, 1 relaxed (FP64 then FP32 operator in one Krylov
         space), 2 iterative refinement (its second inner solve recycling the
-        first's Krylov space; FP32 basis vectors in the inner solves), 3
-        refinement without recycling, 4 scheme 2 with FP64 basis vectors
+        first's Krylov space), 3 refinement without recycling, 4 scheme 2
+        with FP32 basis vectors in the inner solves
         (pbfmm_set_mixed_scheme)."""
         self._chk(self._L.pbfmm_set_mixed_scheme(self._h, int(scheme)))
 

@@ -312,7 +312,7 @@ def test_config3_properties():
         assert rr <= 1e-8 and err <= 1e-4 and err_dn <= 2e-2
 
 
-@pytest.mark.parametrize("scheme", [1, 2, 3, 4], ids=["relaxed", "refinement", "no_recycling", "fp64_basis"])
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4], ids=["relaxed", "refinement", "no_recycling", "fp32_basis"])
 def test_mixed_schemes(oracle_mod, scheme):
     """Every mixed-precision scheme reaches true relative residual 1e-8 and
     the FP64 solution (the scheme is otherwise chosen by timing, so each is

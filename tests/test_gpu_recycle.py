@@ -1,15 +1,14 @@
 """Iterative refinement with a recycled Krylov space (pbfmm_set_mixed_scheme
 2: the second FP32 inner solve keeps the first one's basis, GCRO-style)
-against refinement without it (scheme 3), the same with FP64 basis vectors
-(scheme 4; scheme 2 stores the inner solves' basis in FP32) and the FP64
-solve.
+against refinement without it (scheme 3), the same with the inner solves'
+basis vectors stored in FP32 (scheme 4) and the FP64 solve.
 
 Bars, each solve to true relative residual 1e-8:
 * the same solution as the FP64 solve within 1e-6 (both are converged to
   1e-8 with operators 1e-6 apart) and as refinement without recycling;
 * no more Arnoldi steps than refinement without recycling, on a Q1 sphere, a
   Q2 spheroid and a constrained double-node cube, and within two steps of
-  the FP64-basis refinement;
+  the FP32-basis refinement;
 * with a short restart (12), so that the recycled space is the first inner
   solve's last cycle and the second inner solve restarts too: still 1e-8.
 """
@@ -45,7 +44,7 @@ def test_recycled_refinement(name):
     out = {}
     for key, prec, scheme, restart in (("fp64", P.PBFMM_FP64, 0, 100), ("plain", P.PBFMM_MIXED, 3, 100),
                                        ("recycled", P.PBFMM_MIXED, 2, 100), ("recycled_r12", P.PBFMM_MIXED, 2, 12),
-                                       ("recycled_fp64_basis", P.PBFMM_MIXED, 4, 100)):
+                                       ("recycled_fp32_basis", P.PBFMM_MIXED, 4, 100)):
         f.set_mixed_scheme(scheme)
         x = hip.DeviceArray(f.n)
         its, rr = f.gmres_dev(rhs.value, x.value, 1e-8, restart, 2000, prec)
@@ -54,10 +53,10 @@ def test_recycled_refinement(name):
         print(f"  {key}: {its} steps, relres {rr:.2e}")
         assert rr <= 1e-8
     x64 = out["fp64"][0]
-    for key in ("plain", "recycled", "recycled_r12", "recycled_fp64_basis"):
+    for key in ("plain", "recycled", "recycled_r12", "recycled_fp32_basis"):
         e = rel(out[key][0], x64)
         print(f"  {key} vs FP64 solution {e:.2e}")
         assert e <= 1e-6
     assert out["recycled"][1] <= out["plain"][1]
-    assert abs(out["recycled"][1] - out["recycled_fp64_basis"][1]) <= 2
+    assert abs(out["recycled"][1] - out["recycled_fp32_basis"][1]) <= 2
     f.close()
