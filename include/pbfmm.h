@@ -338,8 +338,9 @@ int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel);
  * with atomics; 8 and 12: plans 0 and 4 with the SpMV on a second stream of
  * the library, beside the whole FMM; 24 and 28: plans 8 and 12 with the
  * second stream forked after the M2M levels, beside M2L onwards; 56 and 60:
- * plans 24 and 28 with M2P following the SpMV on the second stream; -1
- * times them all again on the next application).  For tests and tuning.  No
+ * plans 24 and 28 with M2P following the SpMV on the second stream; 88 and
+ * 92: with M2P before the SpMV there; -1 times them all again on the next
+ * application).  For tests and tuning.  No
  * counterpart in the reference. */
 int pbfmm_set_side_fusion(pbfmm_handle *h, int32_t precision, int32_t mode);
 
@@ -352,7 +353,7 @@ int pbfmm_set_side_fusion(pbfmm_handle *h, int32_t precision, int32_t mode);
  *         plans 0 and 1 with M2L into per-pair slots summed per target, 8
  *         and 12 plans 0 and 4 with the SpMV on a second stream, 24 and 28
  *         the same forked after the M2M levels, 56 and 60 plans 24 and 28
- *         with M2P on the second stream after the SpMV
+ *         with M2P on the second stream after the SpMV, 88 and 92 before it
  * [4]     mixed-precision scheme: 1 relaxed, 2 iterative refinement
  * [5]     timed FP64 / FP32 operator cost ratio x 1000
  * [6] [7] the GMRES step's operator, FP64 / FP32 steps: 1 replayed as a HIP

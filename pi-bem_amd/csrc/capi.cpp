@@ -1055,7 +1055,7 @@ int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel) {
 int pbfmm_set_side_fusion(pbfmm_handle *h, int32_t precision, int32_t mode) {
   if (!h) return PBFMM_ERR_ARG;
   if (mode != -1 && !Engine::valid_plan(mode))
-    return fail(h, PBFMM_ERR_ARG, "set_side_fusion: plan must be -1, 0, 1, 2, 3, 4, 5, 8, 12, 24, 28, 56 or 60");
+    return fail(h, PBFMM_ERR_ARG, "set_side_fusion: plan must be -1, 0, 1, 2, 3, 4, 5, 8, 12, 24, 28, 56, 60, 88 or 92");
   if (precision != PBFMM_FP64 && precision != PBFMM_MIXED)
     return fail(h, PBFMM_ERR_ARG, "set_side_fusion: unknown precision");
   h->eng.side_fusion[precision == PBFMM_MIXED ? 1 : 0] = mode;

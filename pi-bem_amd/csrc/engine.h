@@ -186,8 +186,9 @@ struct Engine {
   // adds into the targets; bit 3 (plans 8, 12 with placement 0) the SpMV on a
   // second stream beside the whole FMM, joined before L2P; bit 4 (plans 24,
   // 28) that fork after the M2M levels instead; bit 5 (plans 56, 60) M2P
-  // after the SpMV on the second stream too.
-  static constexpr int kPlans[] = {0, 1, 2, 3, 4, 5, 8, 12, 24, 28, 56, 60};
+  // after the SpMV on the second stream too; bit 6 (plans 88, 92) M2P before
+  // the SpMV there instead.
+  static constexpr int kPlans[] = {0, 1, 2, 3, 4, 5, 8, 12, 24, 28, 56, 60, 88, 92};
   static bool valid_plan(int m) {
     for (int v : kPlans)
       if (v == m) return true;

@@ -2709,6 +2709,7 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
   const bool late = (side & 16) != 0;  // ... forked after the M2M levels, beside M2L onwards
   const bool slot = (side & 4) != 0;   // M2L into per-pair slots, summed per target
   const bool m2p_async = (side & 32) != 0;  // M2P after the SpMV on the second stream (late fork)
+  const bool m2p_first = (side & 64) != 0;  // ... before it instead, beside M2L
   side &= 3;
   const int nleaf = (int)far.p2m_leaf.size();
   const int64_t ne = (int64_t)tree.nb * kp;
@@ -2773,6 +2774,8 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
     }
     HIPCHK(hipEventRecord(ev_fork, stream));
     HIPCHK(hipStreamWaitEvent(side_stream, ev_fork, 0));
+    // M2P first (plans 88, 92): compute-bound, beside the M2L's atomics
+    if (late && m2p_first && !m2p_done) launch_m2p(side_stream);
     launch_spmv<T, 1>(var_full, x, nullptr, out, nullptr, 1, side_stream);
     // forked after the M2M levels, the multipole expansions are final: M2P
     // (which adds into the accumulator only) follows the SpMV there

@@ -131,8 +131,8 @@ class BEMFMA:
         beside an M2M level and M2P beside an L2L level, 2 M2P beside M2L
         instead, 3 both beside M2L; 4, 5 plans 0, 1 with the slot M2L; 8, 12
         plans 0, 4 with the SpMV on a second stream; 24, 28 forked after the
-        M2M levels; 56, 60 with M2P after the SpMV there; -1 times them again
-        (pbfmm_set_side_fusion)."""
+        M2M levels; 56, 60 with M2P after the SpMV there, 88, 92 before it;
+        -1 times them again (pbfmm_set_side_fusion)."""
         self._chk(self._L.pbfmm_set_side_fusion(self._h, int(precision), int(mode)))
 
     def tuning_state(self):

@@ -19,7 +19,7 @@ from pibem_amd import mesh as M
 pytestmark = pytest.mark.gpu
 
 # the operator's launch plans (pbfmm_set_side_fusion)
-PLANS = (0, 1, 2, 3, 4, 5, 8, 12, 24, 28, 56, 60)
+PLANS = (0, 1, 2, 3, 4, 5, 8, 12, 24, 28, 56, 60, 88, 92)
 
 
 def rel(a, b):
@@ -394,7 +394,8 @@ def test_fused_operator_pipeline(name):
     in its own launch, for each of the three SpMV kernels and each launch
     plan: M2L adding into its targets (plans 0-3) or into per-pair slots
     summed per target (plans 4, 5), the SpMV on a second stream (plans 8, 12;
-    24, 28 forked after the M2M levels; 56, 60 with M2P after it there):
+    24, 28 forked after the M2M levels; 56, 60 with M2P after it there, 88,
+    92 before it):
     FP64 within 1e-14, FP32 within 1e-6 (sums in another order)."""
     from pibem_amd import hip
     pr = {"sphere_p4": lambda: M.config_sphere(16, 4), "sphere_p6": lambda: M.config_sphere(24, 6),
