@@ -468,7 +468,9 @@ def test_operator_spmv_layouts(leaf):
     kernel 6, four columns per 16 bytes; FP64 kernel 3, two): leaves of at
     most 16 rows (four lane groups per wavefront), up to 40 rows (two groups
     or one), and up to 400 rows with more than 2048 columns (row blocks of
-    64, column tiles): FP32 within 1e-6, FP64 within 1e-14 of kernel 0."""
+    64, column tiles): FP32 within 1e-6, FP64 within 1e-14 of kernel 0; and
+    three right-hand sides in one application (vmult_dev_multi, the kernels'
+    blockIdx.y), each column against its single vmult_dev."""
     from pibem_amd import hip
     pr = M.config_sphere(24, 6)
     pr.max_nodes_per_leaf = leaf
@@ -492,6 +494,20 @@ def test_operator_spmv_layouts(leaf):
             e = rel(y, ref)
             print(f"  precision={prec} kernel {kernel}: vs kernel 0 {e:.2e}")
             assert np.all(np.isfinite(y)) and e <= bar
+        # three columns at once with the column-vector kernel
+        f.set_spmv_kernel(prec, kernels[-1])
+        X = np.random.default_rng(17).uniform(-1, 1, (3, n))
+        ds, dd, col = hip.DeviceArray.from_numpy(X.ravel()), hip.DeviceArray(3 * n), hip.DeviceArray(n)
+        bp.vmult_dev_multi(ds.value, dd.value, 3, n, prec)
+        hip.synchronize()
+        Y = dd.numpy().reshape(3, n)
+        for r in range(3):
+            one = hip.DeviceArray.from_numpy(X[r])
+            bp.vmult_dev(one.value, col.value, prec)
+            hip.synchronize()
+            e = rel(Y[r], col.numpy())
+            print(f"  precision={prec} kernel {kernels[-1]}, column {r} of 3: vs single {e:.2e}")
+            assert e <= bar
     f.close()
 
 
