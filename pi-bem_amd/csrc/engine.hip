@@ -759,6 +759,9 @@ __global__ void __launch_bounds__(256) k_p2m_unit(
 // Set-up, FP64, one workgroup per leaf whose unit multipoles fit in LDS: the
 // block [column][2 kp] of the leaf is summed in LDS (atomic adds there) and
 // written once with plain stores, since no other leaf owns those columns.
+// half 0 / 1: only the first / second kernel's kp values of every column
+// ([column][kp] in LDS, for a leaf whose whole block does not fit), written
+// into their half of the [column][2 kp] block; -1: both.
 // Work items are (quad point, m): the item runs the m-recurrence to R_m^m
 // and then the n-recurrence of column m of the harmonics, as k_p2m_unit
 // does for all m of its point.
@@ -767,14 +770,15 @@ __global__ void __launch_bounds__(256) k_p2m_unit_leaf(
     const int *__restrict__ qbeg, const int *__restrict__ col_beg, const int *__restrict__ qpt,
     const int *__restrict__ qcol, int dpc, const double *__restrict__ qx, const double *__restrict__ qn,
     const double *__restrict__ qw, const double *__restrict__ qsh, const double *__restrict__ pmin,
-    const double *__restrict__ delta, double *__restrict__ U) {
+    const double *__restrict__ delta, double *__restrict__ U, int half) {
   extern __shared__ double sU[];
   const int li = leaf_ids[blockIdx.x];
   const int b = leaves[li];
   const int c0 = col_beg[li], nc = col_beg[li + 1] - c0;
   const int q0 = qbeg[li], nq = qbeg[li + 1] - q0;
   const int K2 = 2 * kp;
-  for (int i = threadIdx.x; i < nc * K2; i += blockDim.x) sU[i] = 0.0;
+  const int S = half < 0 ? K2 : kp;  // LDS values per column
+  for (int i = threadIdx.x; i < nc * S; i += blockDim.x) sU[i] = 0.0;
   __syncthreads();
   const double hd = delta[b] * 0.5, inv_delta = 1.0 / delta[b];
   const double cx = pmin[3 * b] + hd, cy = pmin[3 * b + 1] + hd, cz = pmin[3 * b + 2] + hd;
@@ -789,7 +793,7 @@ __global__ void __launch_bounds__(256) k_p2m_unit_leaf(
     int cj[9];
     for (int j = 0; j < dpc; ++j) {
       wj[j] = qsh[g * dpc + j] * qw[g] / (4 * kPi);
-      cj[j] = (qcol[(int64_t)i * dpc + j] - c0) * K2;
+      cj[j] = (qcol[(int64_t)i * dpc + j] - c0) * S;
     }
     double Rmr = 1, Rmi = 0, Gmr = 0, Gmi = 0;  // R_m^m, D R_m^m
     double sc = 1.0;                            // 1 / delta^n
@@ -805,10 +809,17 @@ __global__ void __launch_bounds__(256) k_p2m_unit_leaf(
       Rr *= s; Ri *= s; Gr *= s; Gi *= s;
       for (int j = 0; j < dpc; ++j) {
         double *col = sU + cj[j];
-        atomicAdd(col + k, wj[j] * Gr);
-        atomicAdd(col + kt + k, wj[j] * Gi);
-        atomicAdd(col + kp + k, wj[j] * Rr);
-        atomicAdd(col + kp + kt + k, wj[j] * Ri);
+        if (half <= 0) {
+          atomicAdd(col + k, wj[j] * Gr);
+          atomicAdd(col + kt + k, wj[j] * Gi);
+        }
+        if (half < 0) {
+          atomicAdd(col + kp + k, wj[j] * Rr);
+          atomicAdd(col + kp + kt + k, wj[j] * Ri);
+        } else if (half == 1) {
+          atomicAdd(col + k, wj[j] * Rr);
+          atomicAdd(col + kt + k, wj[j] * Ri);
+        }
       }
     };
     double Rcr = Rmr, Rci = Rmi, Gcr = Gmr, Gci = Gmi, Rpr = 0, Rpi = 0, Gpr = 0, Gpi = 0;
@@ -826,7 +837,14 @@ __global__ void __launch_bounds__(256) k_p2m_unit_leaf(
   }
   __syncthreads();
   double *out = U + (int64_t)c0 * K2;
-  for (int i = threadIdx.x; i < nc * K2; i += blockDim.x) out[i] = sU[i];
+  if (half < 0) {
+    for (int i = threadIdx.x; i < nc * K2; i += blockDim.x) out[i] = sU[i];
+  } else {
+    for (int i = threadIdx.x; i < nc * kp; i += blockDim.x) {
+      const int c = i / kp;
+      out[(int64_t)c * K2 + half * kp + (i - c * kp)] = sU[i];
+    }
+  }
 }
 
 // Per matvec: one workgroup per leaf.  The leaf's 2 kp outputs are read as
@@ -2134,19 +2152,23 @@ void Engine::build_unit_multipoles(bool want_f32) {
   const int npts = (int)far.p2m_qpt.size();
   if (npts > 0) {
     // leaves whose [column][2 kp] block fits in LDS: one workgroup each, sums
-    // in LDS; the others (none on the test meshes) by global atomics
+    // in LDS; those whose [column][kp] half fits: two workgroups each, one
+    // per kernel; the others by global atomics
     const int nleaf = (int)far.p2m_leaf.size();
     const size_t col_bytes = (size_t)2 * kp * sizeof(double);
     constexpr size_t kLdsMax = 160 * 1024 - 1024;
-    std::vector<int32_t> qbeg(nleaf + 1, 0), fit, big_qpt, big_qleaf, big_qcol;
+    std::vector<int32_t> qbeg(nleaf + 1, 0), fit, fit_half, big_qpt, big_qleaf, big_qcol;
     for (int i = 0; i < npts; ++i) qbeg[far.p2m_qleaf[i] + 1]++;
     for (int li = 0; li < nleaf; ++li) qbeg[li + 1] += qbeg[li];
-    size_t max_bytes = 0;
+    size_t max_bytes = 0, max_half = 0;
     for (int li = 0; li < nleaf; ++li) {
       const size_t bytes = (size_t)(far.p2m_col_beg[li + 1] - far.p2m_col_beg[li]) * col_bytes;
       if (bytes <= kLdsMax) {
         if (qbeg[li + 1] > qbeg[li]) fit.push_back(li);
         max_bytes = std::max(max_bytes, bytes);
+      } else if (bytes / 2 <= kLdsMax) {
+        if (qbeg[li + 1] > qbeg[li]) fit_half.push_back(li);
+        max_half = std::max(max_half, bytes / 2);
       } else {
         for (int i = qbeg[li]; i < qbeg[li + 1]; ++i) {
           big_qpt.push_back(far.p2m_qpt[i]);
@@ -2155,19 +2177,29 @@ void Engine::build_unit_multipoles(bool want_f32) {
         }
       }
     }
-    DevBuf qpt, qcol, dqbeg, dfit;
+    DevBuf qpt, qcol, dqbeg, dfit, dfit_half;
     up(qpt, far.p2m_qpt);
     up(qcol, far.p2m_qcol);
     up(dqbeg, qbeg);
     up(dfit, fit);
-    if (!fit.empty()) {
+    up(dfit_half, fit_half);
+    if (!fit.empty() || !fit_half.empty())
       HIPCHK(hipFuncSetAttribute((const void *)k_p2m_unit_leaf, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)std::max<size_t>(max_bytes, 16)));
+                                 (int)std::max<size_t>(std::max(max_bytes, max_half), 16)));
+    if (!fit.empty()) {
       hipLaunchKernelGGL(k_p2m_unit_leaf, dim3((unsigned)fit.size()), dim3(256), max_bytes, stream, p, kt, kp,
                          dfit.as<int>(), d_p2m_leaf.as<int>(), dqbeg.as<int>(), d_p2m_col_beg.as<int>(),
                          qpt.as<int>(), qcol.as<int>(), dpc, f64.qx.as<double>(), f64.qn.as<double>(),
                          f64.qw.as<double>(), f64.qsh.as<double>(), d_pmin.as<double>(), d_delta.as<double>(),
-                         f64.unit.as<double>());
+                         f64.unit.as<double>(), -1);
+      HIPCHK(hipGetLastError());
+    }
+    for (int half = 0; half < 2 && !fit_half.empty(); ++half) {
+      hipLaunchKernelGGL(k_p2m_unit_leaf, dim3((unsigned)fit_half.size()), dim3(256), max_half, stream, p, kt, kp,
+                         dfit_half.as<int>(), d_p2m_leaf.as<int>(), dqbeg.as<int>(), d_p2m_col_beg.as<int>(),
+                         qpt.as<int>(), qcol.as<int>(), dpc, f64.qx.as<double>(), f64.qn.as<double>(),
+                         f64.qw.as<double>(), f64.qsh.as<double>(), d_pmin.as<double>(), d_delta.as<double>(),
+                         f64.unit.as<double>(), half);
       HIPCHK(hipGetLastError());
     }
     if (!big_qpt.empty()) {
@@ -2183,7 +2215,7 @@ void Engine::build_unit_multipoles(bool want_f32) {
       HIPCHK(hipGetLastError());
       sync();  // the index copies are freed on return
     }
-    sync();  // qpt, qcol, dqbeg, dfit are freed on return
+    sync();  // qpt, qcol, dqbeg, dfit, dfit_half are freed on return
   }
   // the operator's unit multipoles (U1 on Neumann, -U2 on Dirichlet DoFs)
   const int64_t nou = std::max<int64_t>(ncol, 1) * kp;

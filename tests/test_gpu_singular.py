@@ -99,11 +99,17 @@ def test_q2_fmm_vs_dense_duffy(oracle_mod, p):
     f.close()
 
 
-def test_q2_mid_size_matvec(oracle_mod):
+@pytest.mark.parametrize("leaf", [20, 80])
+def test_q2_mid_size_matvec(oracle_mod, leaf):
+    """The FMM matvec on a 24,578-DoF Q2 spheroid at order 8 against the
+    oracle's at the same leaf size: 20 (every leaf's unit multipoles fit LDS
+    whole) and 80 (leaves of more columns, built one kernel at a time):
+    FP64 within 1e-10, FP32 within 2e-5."""
     from pibem_amd import hip
     pr = M.config_spheroid(32, 8)
+    pr.max_nodes_per_leaf = leaf
     n = pr.mesh.n_dofs
-    print(f"\n[case] {pr.mesh.name} N={n} p=8", flush=True)
+    print(f"\n[case] {pr.mesh.name} N={n} p=8 leaf {leaf}", flush=True)
     o = oracle_mod.OracleFMM(pr).setup()
     rng = np.random.default_rng(1234)
     a, b = rng.uniform(-1, 1, n), rng.uniform(-1, 1, n)
