@@ -2508,11 +2508,12 @@ void Engine::tune_spmv() {
   HIPCHK(hipEventCreate(&e1));
   float best = 0;
   int pick = 0;
+  // one warm-up and three timed launches per variant (the variants differ by
+  // 5-30% at the sizes measured, the repetitions by ~1%)
   for (int v = 0; v < (NK == 2 ? 3 : kColsVariant<T> + 1); ++v) {
-    for (int rep = 0; rep < 2; ++rep)
-      launch_spmv<T, NK>(v, d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
+    launch_spmv<T, NK>(v, d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
     HIPCHK(hipEventRecord(e0, stream));
-    for (int rep = 0; rep < 5; ++rep)
+    for (int rep = 0; rep < 3; ++rep)
       launch_spmv<T, NK>(v, d_sp.as<double>(), d_sd.as<double>(), d_outN.as<double>(), d_outD.as<double>());
     HIPCHK(hipEventRecord(e1, stream));
     HIPCHK(hipEventSynchronize(e1));
@@ -2928,7 +2929,7 @@ double Engine::norm2(const double *v) {
 // the SpMV and M2P are short beside them (tens of thousands of DoFs); on
 // large trees the fused workgroups only crowd each other.  The modes are
 // matvec_fused's side 0..3.  Timed once per precision, on the first operator
-// application: ten applications each way on scratch vectors, after one
+// application: six applications each way on scratch vectors, after one
 // warm-up; a fused mode is kept only if it beats the unfused one by 2%.
 int Engine::use_side_fusion(bool f32) {
   int &mode = side_fusion[f32 ? 1 : 0];
@@ -2943,13 +2944,13 @@ int Engine::use_side_fusion(bool f32) {
   constexpr int np = (int)(sizeof(kPlans) / sizeof(kPlans[0]));
   float ms[np] = {};
   for (int s = 0; s < np; ++s)
-    for (int rep = 0; rep < 11; ++rep) {
+    for (int rep = 0; rep < 7; ++rep) {
       if (rep == 1) HIPCHK(hipEventRecord(t0, stream));
       if (f32)
         matvec_fused<float>(u, d_outN.as<double>(), v, kPlans[s]);
       else
         matvec_fused<double>(u, d_outN.as<double>(), v, kPlans[s]);
-      if (rep == 10) {
+      if (rep == 6) {
         HIPCHK(hipEventRecord(t1, stream));
         HIPCHK(hipEventSynchronize(t1));
         HIPCHK(hipEventElapsedTime(&ms[s], t0, t1));
