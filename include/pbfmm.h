@@ -317,6 +317,16 @@ int pbfmm_set_mixed_scheme(pbfmm_handle *h, int32_t scheme);
  * <= 0 restores the default.  For tuning. */
 int pbfmm_set_mixed_tolerances(pbfmm_handle *h, double inner_reduction, double relax_switch);
 
+/* Gram-Schmidt of the GMRES solves (pbfmm_gmres_dev, pbfmm_solve_system): an
+ * Arnoldi step projects the new vector against the basis once and projects
+ * it a second time only when the first projection left less than a fraction
+ * eta of its norm (default 0.05; eta >= 1 always projects twice, classical
+ * Gram-Schmidt with reorthogonalisation; 0 never does; < 0 restores the
+ * default).  Every solve still ends on the true relative residual.  No
+ * counterpart in the reference (deal.II's SolverGMRES orthogonalises by its
+ * own rule). */
+int pbfmm_set_gmres_reorth(pbfmm_handle *h, double eta);
+
 /* Overrides the near-field SpMV kernel that set-up's autotuner chose for one
  * precision (PBFMM_FP64 or PBFMM_MIXED for the FP32 values), for both
  * storages: 0 workgroup per leaf, 1 wavefront per leaf with 2 rows in flight,

@@ -1036,6 +1036,13 @@ int pbfmm_set_mixed_tolerances(pbfmm_handle *h, double inner_reduction, double r
   return PBFMM_OK;
 }
 
+int pbfmm_set_gmres_reorth(pbfmm_handle *h, double eta) {
+  if (!h) return PBFMM_ERR_ARG;
+  if (std::isnan(eta)) return fail(h, PBFMM_ERR_ARG, "set_gmres_reorth: eta is NaN");
+  h->eng.reorth_eta = eta < 0 ? Engine::kReorthEta : eta;
+  return PBFMM_OK;
+}
+
 int pbfmm_set_spmv_kernel(pbfmm_handle *h, int32_t precision, int32_t kernel) {
   return guarded(h, [&]() -> int {
     if (int rc = need(h, 3, "set_spmv_kernel")) return rc;

@@ -249,6 +249,13 @@ struct Engine {
   // 0 chosen by the timed cost ratio, 1 relaxed, 2 refinement (recycling),
   // 3 without recycling, 4 recycling with an FP32 basis in the inner solves
   int mixed_scheme = 0;
+  // GMRES (gmres_core): the second Gram-Schmidt projection of a step runs
+  // when the first left |w1| < reorth_eta |w0| (pbfmm_set_gmres_reorth);
+  // >= 1: always (CGS2), 0: never (CGS).  At 393,218 DoFs the first
+  // projection leaves 0.12-0.95 of |w0|, and a second one changed |w| by
+  // less than 1e-6.
+  static constexpr double kReorthEta = 0.05;
+  double reorth_eta = kReorthEta;
   double inner_reduction = 0;  // refinement's inner residual reduction (0: the default)
   double relax_switch = 0;     // relaxed scheme's FP32 switch over tol ||b|| (0: the default)
   double inner_target(double tol, double bn, double rn) const;

@@ -1790,9 +1790,13 @@ __global__ void k_scale_inv(int64_t n, double *__restrict__ v, const double *__r
 // into part[j * P + range].
 constexpr int kDotParts = 16;   // per column of V . w (consumers re-read nv * 16 partials)
 constexpr int kNormParts = 64;  // of |w|^2 (one wavefront finishes them)
+// (skip: when set and *skip != 0, the launch does nothing: the second
+// projection of a step that the first left without severe cancellation)
 template <typename VT>
 __global__ void __launch_bounds__(kRedThreads) k_mdot_parts(int64_t n, int64_t ldv, const VT *__restrict__ V,
-                                                            const double *__restrict__ w, double *__restrict__ part) {
+                                                            const double *__restrict__ w, double *__restrict__ part,
+                                                            const int *__restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ double sh[kRedThreads / 64];
   const int j = blockIdx.y, P = (int)gridDim.x;
   const VT *v = V + (int64_t)j * ldv;
@@ -1810,7 +1814,13 @@ __global__ void __launch_bounds__(kRedThreads) k_mdot_parts(int64_t n, int64_t l
 template <typename VT>
 __global__ void __launch_bounds__(256) k_gemv_sub_fin(int64_t n, int64_t ldv, int nv, const VT *__restrict__ V,
                                                       const double *__restrict__ part, int P,
-                                                      double *__restrict__ h_out, double *__restrict__ w) {
+                                                      double *__restrict__ h_out, double *__restrict__ w,
+                                                      const int *__restrict__ skip) {
+  if (skip && *skip) {  // a skipped second projection: coefficients zero, w as it is
+    if (blockIdx.x == 0)
+      for (int j = threadIdx.x; j < nv; j += blockDim.x) h_out[j] = 0.0;
+    return;
+  }
   __shared__ double hs[128];
   for (int j = threadIdx.x; j < nv; j += blockDim.x) {
     double s = 0;
@@ -1824,6 +1834,21 @@ __global__ void __launch_bounds__(256) k_gemv_sub_fin(int64_t n, int64_t ldv, in
   double acc = w[i];
   for (int j = 0; j < nv; ++j) acc -= (double)V[(int64_t)j * ldv + i] * hs[j];
   w[i] = acc;
+}
+// Selective reorthogonalisation (Daniel, Gragg, Kaufman & Stewart 1976, with
+// a smaller threshold): after the first projection w1 = w0 - V h1, with
+// |w0|^2 = |h1|^2 + |w1|^2, the second one is needed only when |w1| <
+// eta |w0| (cancellation has eaten all but a fraction eta of w0).  One
+// wavefront sums the nv coefficients h1 and the P <= 64 norm partials of w1
+// and sets *skip for the second projection's launches.
+__global__ void k_reorth_flag(const double *__restrict__ h1, int nv, const double *__restrict__ npart, int P,
+                              double eta, int *__restrict__ skip) {
+  const int l = (int)threadIdx.x;
+  double sh = 0.0;
+  for (int j = l; j < nv; j += 64) sh += h1[j] * h1[j];
+  sh = wave_sum(sh);
+  const double sw = wave_sum(l < P ? npart[l] : 0.0);
+  if (l == 0) *skip = sw >= eta * eta * (sh + sw) ? 1 : 0;
 }
 // v /= |v| with |v|^2 the sum of P <= 64 partials (one wavefront per block
 // finishes them, in the same order in every block); block 0 stores |v|.
@@ -3385,8 +3410,14 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
   int kc = rc && rc->kc > 0 && rc->kc < m ? rc->kc : 0;
   std::vector<double> Ecol, cproj;  // coefficients on C of each step; C^T r
   if (rc && rc->record) rc->k = 0;
-  if (!e.d_cgs.p) alloc(e.d_cgs, (size_t)kDotParts * 128 * 8);
+  // dot partials (kDotParts x 128), then the first projection's norm
+  // partials (kNormParts) and the second projection's skip flag
+  const size_t cg_doubles = (size_t)kDotParts * 128 + kNormParts + 2;
+  if (e.d_cgs.bytes < cg_doubles * 8) alloc(e.d_cgs, cg_doubles * 8);
   double *cg = e.d_cgs.as<double>();
+  double *cg_norm1 = cg + (size_t)kDotParts * 128;
+  int *skip2 = reinterpret_cast<int *>(cg_norm1 + kNormParts);
+  const bool selective = e.reorth_eta < 1.0;
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1), h(m + 2), y(m);
   // Hessenberg slots on the device, one per Arnoldi step: [h1 (m+2)][h2 (m+2)][|w|]
   const int S = 2 * (m + 2) + 1;
@@ -3438,11 +3469,24 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
     double *slot = hd + (size_t)k * S;
     double *h1 = slot, *h2 = slot + (m + 2), *hn_d = slot + 2 * (m + 2);
     const int nv = kc + k + 1;
-    for (double *hj : {h1, h2}) {
-      hipLaunchKernelGGL(k_mdot_parts<VT>, dim3(kDotParts, nv), dim3(kRedThreads), 0, e.stream, n, n, V, wv, cg);
-      hipLaunchKernelGGL(k_gemv_sub_fin<VT>, dim3(g), dim3(256), 0, e.stream, n, n, nv, V, cg, kDotParts, hj, wv);
+    // CGS with the second projection when the first cancelled severely
+    // (selective, k_reorth_flag), or always (reorth_eta >= 1)
+    hipLaunchKernelGGL(k_mdot_parts<VT>, dim3(kDotParts, nv), dim3(kRedThreads), 0, e.stream, n, n, V, wv, cg,
+                       nullptr);
+    hipLaunchKernelGGL(k_gemv_sub_fin<VT>, dim3(g), dim3(256), 0, e.stream, n, n, nv, V, cg, kDotParts, h1, wv,
+                       nullptr);
+    const int *sk = nullptr;
+    if (selective) {
+      hipLaunchKernelGGL(k_mdot_parts<double>, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, wv, wv,
+                         cg_norm1, nullptr);
+      hipLaunchKernelGGL(k_reorth_flag, dim3(1), dim3(64), 0, e.stream, h1, nv, cg_norm1, kNormParts,
+                         e.reorth_eta, skip2);
+      sk = skip2;
     }
-    hipLaunchKernelGGL(k_mdot_parts<double>, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, wv, wv, cg);
+    hipLaunchKernelGGL(k_mdot_parts<VT>, dim3(kDotParts, nv), dim3(kRedThreads), 0, e.stream, n, n, V, wv, cg, sk);
+    hipLaunchKernelGGL(k_gemv_sub_fin<VT>, dim3(g), dim3(256), 0, e.stream, n, n, nv, V, cg, kDotParts, h2, wv, sk);
+    hipLaunchKernelGGL(k_mdot_parts<double>, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, wv, wv, cg,
+                       nullptr);
     if constexpr (f32_basis) {
       hipLaunchKernelGGL(k_scale_fin2<VT>, dim3(g), dim3(256), 0, e.stream, n, wv, vk1, e.d_vn.as<double>(), cg,
                          kNormParts, hn_d);
@@ -3476,10 +3520,13 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
       double *ps = hd + (size_t)(m - 1) * S;
       HIPCHK(hipMemcpyAsync(w, rv, (size_t)n * 8, hipMemcpyDeviceToDevice, e.stream));
       for (double *pj : {ps, ps + (m + 2)}) {
-        hipLaunchKernelGGL(k_mdot_parts<VT>, dim3(kDotParts, kc), dim3(kRedThreads), 0, e.stream, n, n, V, w, cg);
-        hipLaunchKernelGGL(k_gemv_sub_fin<VT>, dim3(g), dim3(256), 0, e.stream, n, n, kc, V, cg, kDotParts, pj, w);
+        hipLaunchKernelGGL(k_mdot_parts<VT>, dim3(kDotParts, kc), dim3(kRedThreads), 0, e.stream, n, n, V, w, cg,
+                           nullptr);
+        hipLaunchKernelGGL(k_gemv_sub_fin<VT>, dim3(g), dim3(256), 0, e.stream, n, n, kc, V, cg, kDotParts, pj, w,
+                           nullptr);
       }
-      hipLaunchKernelGGL(k_mdot_parts<double>, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, w, w, cg);
+      hipLaunchKernelGGL(k_mdot_parts<double>, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, w, w, cg,
+                         nullptr);
       hipLaunchKernelGGL(k_scale_fin2<VT>, dim3(g), dim3(256), 0, e.stream, n, w, V + (int64_t)kc * n,
                          e.d_vn.as<double>(), cg, kNormParts, ps + 2 * (m + 2));
       std::vector<double> pr((size_t)S);

@@ -118,6 +118,12 @@ class BEMFMA:
         FP32 switch (pbfmm_set_mixed_tolerances; 0 = the defaults)."""
         self._chk(self._L.pbfmm_set_mixed_tolerances(self._h, float(inner_reduction), float(relax_switch)))
 
+    def set_gmres_reorth(self, eta=-1.0):
+        """GMRES Gram-Schmidt: the second projection of an Arnoldi step runs
+        when the first left less than eta of the vector's norm (default
+        0.05; >= 1 always, 0 never; pbfmm_set_gmres_reorth)."""
+        self._chk(self._L.pbfmm_set_gmres_reorth(self._h, float(eta)))
+
     def set_spmv_kernel(self, precision, kernel):
         """Overrides the autotuned near-field SpMV kernel for one precision: 0
         workgroup per leaf, 1 / 2 wavefront per leaf with 2 / 4 rows in

@@ -70,6 +70,7 @@ SIGNATURES = {
     "pbfmm_work_counts": (C.c_int, [_h, _i64]),
     "pbfmm_set_mixed_scheme": (C.c_int, [_h, C.c_int32]),
     "pbfmm_set_mixed_tolerances": (C.c_int, [_h, C.c_double, C.c_double]),
+    "pbfmm_set_gmres_reorth": (C.c_int, [_h, C.c_double]),
     "pbfmm_set_graph_mode": (C.c_int, [_h, C.c_int32]),
     "pbfmm_set_preconditioner_rows": (C.c_int, [_h, C.c_int32]),
     "pbfmm_graph_timings": (C.c_int, [_h, C.POINTER(C.c_float)]),

@@ -60,3 +60,31 @@ def test_recycled_refinement(name):
     assert out["recycled"][1] <= out["plain"][1]
     assert abs(out["recycled"][1] - out["recycled_fp32_basis"][1]) <= 2
     f.close()
+
+
+@pytest.mark.parametrize("name", ["sphere_p5", "spheroid_q2"])
+def test_gmres_reorth(name):
+    """Gram-Schmidt with the second projection always (eta 1, CGS2), when
+    the first cancels severely (the default 0.05) and never (0): FP64 and
+    mixed solves reach true relative residual 1e-8 with each, the solutions
+    within 1e-10 (FP64) / 1e-6 (mixed) of the CGS2 ones, the step counts
+    within two."""
+    pr = CASES[name]()
+    print(f"\n[case] {pr.mesh.name} N={pr.mesh.n_dofs}", flush=True)
+    f = P.setup(pr, P.PBFMM_MIXED)
+    bp = P.BEMProblem(f)
+    rhs = hip.DeviceArray.from_numpy(bp.compute_rhs(pr.tmp_rhs))
+    for prec, bar in ((P.PBFMM_FP64, 1e-10), (P.PBFMM_MIXED, 1e-6)):
+        ref = None
+        for eta in (1.0, -1.0, 0.0):
+            f.set_gmres_reorth(eta)
+            x = hip.DeviceArray(f.n)
+            its, rr = f.gmres_dev(rhs.value, x.value, 1e-8, 100, 2000, prec)
+            hip.synchronize()
+            xs = x.numpy()
+            ref = (xs, its) if ref is None else ref
+            e = rel(xs, ref[0])
+            print(f"  precision={prec} eta {eta}: {its} steps, relres {rr:.2e}, vs CGS2 {e:.2e}")
+            assert rr <= 1e-8 and e <= bar and abs(its - ref[1]) <= 2
+    f.set_gmres_reorth()
+    f.close()
