@@ -1840,7 +1840,8 @@ __global__ void __launch_bounds__(256) k_gemv_sub_fin(int64_t n, int64_t ldv, in
 // |w0|^2 = |h1|^2 + |w1|^2, the second one is needed only when |w1| <
 // eta |w0| (cancellation has eaten all but a fraction eta of w0).  One
 // wavefront sums the nv coefficients h1 and the P <= 64 norm partials of w1
-// and sets *skip for the second projection's launches.
+// and sets skip[0] for the second projection's launches, skip[1] for the
+// norm launch that follows only a second projection.
 __global__ void k_reorth_flag(const double *__restrict__ h1, int nv, const double *__restrict__ npart, int P,
                               double eta, int *__restrict__ skip) {
   const int l = (int)threadIdx.x;
@@ -1848,7 +1849,10 @@ __global__ void k_reorth_flag(const double *__restrict__ h1, int nv, const doubl
   for (int j = l; j < nv; j += 64) sh += h1[j] * h1[j];
   sh = wave_sum(sh);
   const double sw = wave_sum(l < P ? npart[l] : 0.0);
-  if (l == 0) *skip = sw >= eta * eta * (sh + sw) ? 1 : 0;
+  if (l == 0) {
+    skip[0] = sw >= eta * eta * (sh + sw) ? 1 : 0;
+    skip[1] = 1 - skip[0];  // the norm launch after the second projection: skipped with it
+  }
 }
 // v /= |v| with |v|^2 the sum of P <= 64 partials (one wavefront per block
 // finishes them, in the same order in every block); block 0 stores |v|.
@@ -3412,7 +3416,7 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
   if (rc && rc->record) rc->k = 0;
   // dot partials (kDotParts x 128), then the first projection's norm
   // partials (kNormParts) and the second projection's skip flag
-  const size_t cg_doubles = (size_t)kDotParts * 128 + kNormParts + 2;
+  const size_t cg_doubles = (size_t)kDotParts * 128 + kNormParts + 4;
   if (e.d_cgs.bytes < cg_doubles * 8) alloc(e.d_cgs, cg_doubles * 8);
   double *cg = e.d_cgs.as<double>();
   double *cg_norm1 = cg + (size_t)kDotParts * 128;
@@ -3485,17 +3489,18 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
     }
     hipLaunchKernelGGL(k_mdot_parts<VT>, dim3(kDotParts, nv), dim3(kRedThreads), 0, e.stream, n, n, V, wv, cg, sk);
     hipLaunchKernelGGL(k_gemv_sub_fin<VT>, dim3(g), dim3(256), 0, e.stream, n, n, nv, V, cg, kDotParts, h2, wv, sk);
-    hipLaunchKernelGGL(k_mdot_parts<double>, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, wv, wv, cg,
-                       nullptr);
+    // the norm again after a second projection (with none, w1's stands)
+    hipLaunchKernelGGL(k_mdot_parts<double>, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, wv, wv,
+                       cg_norm1, selective ? skip2 + 1 : nullptr);
     if constexpr (f32_basis) {
-      hipLaunchKernelGGL(k_scale_fin2<VT>, dim3(g), dim3(256), 0, e.stream, n, wv, vk1, e.d_vn.as<double>(), cg,
-                         kNormParts, hn_d);
+      hipLaunchKernelGGL(k_scale_fin2<VT>, dim3(g), dim3(256), 0, e.stream, n, wv, vk1, e.d_vn.as<double>(),
+                         cg_norm1, kNormParts, hn_d);
     } else {
       if (graphs)
         hipLaunchKernelGGL(k_scale_fin2<double>, dim3(g), dim3(256), 0, e.stream, n, wv, vk1, e.d_vn.as<double>(),
-                           cg, kNormParts, hn_d);
+                           cg_norm1, kNormParts, hn_d);
       else
-        hipLaunchKernelGGL(k_scale_fin, dim3(g), dim3(256), 0, e.stream, n, vk1, cg, kNormParts, hn_d);
+        hipLaunchKernelGGL(k_scale_fin, dim3(g), dim3(256), 0, e.stream, n, vk1, cg_norm1, kNormParts, hn_d);
     }
   };
   while (it < maxit) {
