@@ -3620,7 +3620,9 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
         const double goal = (relaxed && est > switch_abs) ? switch_abs : tol_abs;
         const double left = std::log(goal / est) / rate;
         ahead = (int)std::max(1.0, std::min(8.0, 0.5 * left));
-        speculate = speculate && left > ahead + 1.5;
+        // long steps too while convergence is predicted well past the next
+        // round trip (an unused step would take a misprediction by 4 steps)
+        speculate = (speculate && left > ahead + 1.5) || left > ahead + 4.0;
       }
       k_sync = kk - 1 + ahead;
     }
