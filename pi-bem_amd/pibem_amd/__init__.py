@@ -583,7 +583,8 @@ def tune_leaf_size(problem, precision=PBFMM_MIXED, solves=2, start=20, ladder=LE
 def leaf_walk(measure, start=20, ladder=LEAF_LADDER, gain=0.01):
     """The tuner's walk over the ladder: measure(leaf) -> cost (each leaf is
     measured once here).  From start it measures both neighbours, then steps
-    towards the cheaper one until three leaf sizes in a row do not beat the
+    towards the cheaper one (also when neither beats the start: a flat start
+    does not end the walk) until three leaf sizes in a row do not beat the
     best so far (so that a plateau or a noisy point does not end the walk
     early).  A leaf size replaces the best only when it is cheaper by more
     than `gain` (1%), so timing noise does not move the choice.  Returns
@@ -606,7 +607,7 @@ def leaf_walk(measure, start=20, ladder=LEAF_LADDER, gain=0.01):
     for j in nbr:
         m(lad[j])
     jc = min(nbr, key=lambda j: cost[lad[j]]) if nbr else k
-    step = (1 if jc > k else -1) if cost[lad[jc]] < cost[lad[k]] else 0
+    step = (1 if jc > k else -1) if nbr else 0
     best, kb = lad[k], k
     j, misses = k + step, 0
     while step and 0 <= j < len(lad) and misses < 3:

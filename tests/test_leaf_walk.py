@@ -47,6 +47,22 @@ def test_walk_crosses_a_flat_start():
     assert best == 48 and bracketed
 
 
+def test_walk_leaves_a_start_cheaper_than_both_neighbours():
+    # 20 is a shade cheaper than 24 and 16 (noise), the minimum is at 96: the
+    # walk still goes towards the cheaper neighbour and finds it
+    cost = {16: 14.38, 20: 13.35, 24: 13.38, 32: 12.93, 40: 11.58, 48: 11.01, 64: 10.29, 80: 10.85, 96: 9.56,
+            128: 10.55, 160: 13.99}
+    best, bracketed, seen = _walk(lambda l: cost.get(l, 15.0))
+    assert best == 96 and bracketed
+
+
+def test_start_at_the_minimum_stays():
+    # a true minimum at the start: three misses on the cheaper side end the walk
+    cost = {8: 7, 12: 6, 16: 5, 20: 4, 24: 5.5, 32: 6.5}
+    best, bracketed, seen = _walk(lambda l: cost.get(l, 9))
+    assert best == 20 and bracketed and 32 not in seen
+
+
 def test_walk_crosses_a_plateau():
     # equal costs at 32 and 40 do not end the walk: 64 is found
     cost = {16: 9, 20: 8, 24: 7, 32: 6, 40: 6, 48: 5, 64: 4, 80: 4.5, 96: 5}
