@@ -104,8 +104,10 @@ def work_model(wc, s_v, nk):
 
     spmv: nk values per leaf-block entry, one column index per leaf column, one
           row id per row, nk input vectors read once, nk outputs written once.
-    p2m:  nk kp unit-multipole values per leaf column, one DoF index per
-          column, nk input vectors once, nk kp values per leaf written.
+    p2m:  2 kp unit-multipole values per leaf column (nk 2), or the
+          operator's compact kpc (its real parts and the imaginary parts of
+          m >= 1, padded to a multiple of 4), one DoF index per column, nk
+          input vectors once, nk kp values per leaf written.
     m2m, l2l, m2l: 8 Kt^2 flops per (pair, expansion): a complex Kt x Kt map.
     m2p, l2p: ~(8 nk Kt + 8 Kt + 40) flops per evaluation point.
     m2l_sum (slot plans): every pair's 2 Kt slot values read once and its
@@ -114,10 +116,11 @@ def work_model(wc, s_v, nk):
     """
     n, p, kp = wc["n_dofs"], wc["trunc_order"], wc["kp"]
     kt = (p + 1) * (p + 2) // 2
+    kpc = (2 * kt - (p + 1) + 3) // 4 * 4
     out = {
         "spmv": dict(bound="hbm", bytes=wc["near_values"] * nk * s_v + wc["near_cols"] * 4 + wc["near_rows"] * 4
                      + nk * n * 8 + nk * n * 8),
-        "p2m": dict(bound="hbm", bytes=wc["p2m_cols"] * (nk * kp * s_v + 4) + nk * n * 8
+        "p2m": dict(bound="hbm", bytes=wc["p2m_cols"] * ((2 * kp if nk == 2 else kpc) * s_v + 4) + nk * n * 8
                     + wc["leaves"] * nk * kp * s_v),
     }
     for k, pairs in (("m2m", wc["m2m_translations"]), ("l2l", wc["l2l_translations"]), ("m2l", wc["m2l_pairs"])):

@@ -66,6 +66,8 @@ struct Engine {
   bool own_stream = false;
   int device = 0;
   int p = 6, kt = 0, kp = 0, L = 10;
+  int kpc = 0;     // compact unit-multipole length of the operator (set with the unit multipoles)
+  DevBuf d_cmap;   // its rows' places in a kp-row expansion (-1: padding)
   int dpc = 4, nq = 16, sing_order = 5;
   int sing_rule = 0;  // 0 Telles tensor rule, 1 Duffy split (PBFMM_SING_*)
   int64_t n = 0;

@@ -892,12 +892,18 @@ struct P2mClear {
   double *acc = nullptr;
   int64_t ne = 0, na = 0;  // expansion values (blocks x kp), accumulator length
 };
+// NK = 1 reads the compact unit multipoles: kpc values per column, compact
+// row c holding full row cmap[c] (-1: padding), without the imaginary parts
+// of the m = 0 coefficients (zero for real sources) and without the
+// expansion's padding; the leaf's ME is expanded to kp rows in LDS.
+constexpr int kMaxKp = 144;  // kp at order 10
 template <typename T, int NK>
 __global__ void __launch_bounds__(256) k_p2m(int nleaf, const int *__restrict__ leaves,
                                              const int *__restrict__ col_beg, const int *__restrict__ col_dof,
-                                             const T *__restrict__ U, int kp, const double *__restrict__ phi,
-                                             const double *__restrict__ dphi, T *__restrict__ ME, int64_t vs,
-                                             int64_t es, P2mClear<T> cl = P2mClear<T>()) {
+                                             const T *__restrict__ U, int kp, int kpc, const int *__restrict__ cmap,
+                                             const double *__restrict__ phi, const double *__restrict__ dphi,
+                                             T *__restrict__ ME, int64_t vs, int64_t es,
+                                             P2mClear<T> cl = P2mClear<T>()) {
   if ((int)blockIdx.x >= nleaf) {  // clearing workgroups (gridDim.y is 1 here)
     const int64_t nt = (int64_t)(gridDim.x - nleaf) * blockDim.x;
     for (int64_t i = (int64_t)(blockIdx.x - nleaf) * blockDim.x + threadIdx.x; i < 2 * cl.ne + cl.na; i += nt) {
@@ -921,7 +927,8 @@ __global__ void __launch_bounds__(256) k_p2m(int nleaf, const int *__restrict__ 
   __shared__ TV part[256];
   const int li = blockIdx.x;
   const int t = threadIdx.x;
-  const int NO = NK * kp, NOV = NO / VW;  // kp is a multiple of 16: NOV <= 144
+  // values per column: 2 kp (both kernels) or kpc (a multiple of 4): NOV <= 144
+  const int NO = NK == 2 ? 2 * kp : kpc, NOV = NO / VW;
   const int nph = 256 / NOV;
   const int o = t % NOV, ph = t / NOV;
   const bool active = ph < nph;
@@ -947,10 +954,31 @@ __global__ void __launch_bounds__(256) k_p2m(int nleaf, const int *__restrict__ 
   }
   part[t] = acc;
   __syncthreads();
-  if (t < NOV) {
-    TV a = part[t];
-    for (int q = 1; q < nph; ++q) add16(a, part[q * NOV + t]);
-    reinterpret_cast<TV *>(ME + (int64_t)leaves[li] * NO)[t] = a;
+  if constexpr (NK == 2) {
+    if (t < NOV) {
+      TV a = part[t];
+      for (int q = 1; q < nph; ++q) add16(a, part[q * NOV + t]);
+      reinterpret_cast<TV *>(ME + (int64_t)leaves[li] * NO)[t] = a;
+    }
+  } else {
+    // the compact sums into their rows of the leaf's kp-row ME (zeros
+    // elsewhere), then stored whole
+    __shared__ T sme[kMaxKp];
+    for (int i = t; i < kp; i += 256) sme[i] = T(0);
+    __syncthreads();
+    if (t < NOV) {
+      TV a = part[t];
+      for (int q = 1; q < nph; ++q) add16(a, part[q * NOV + t]);
+      const T *av = reinterpret_cast<const T *>(&a);
+#pragma unroll
+      for (int k = 0; k < VW; ++k) {
+        const int r = cmap[t * VW + k];
+        if (r >= 0) sme[r] = av[k];
+      }
+    }
+    __syncthreads();
+    T *me = ME + (int64_t)leaves[li] * kp;
+    for (int i = t; i < kp; i += 256) me[i] = sme[i];
   }
 }
 
@@ -971,13 +999,15 @@ __global__ void k_op_near(int nleaf, const int *__restrict__ row_beg, const int 
     op[vb + i] = d ? -vals[2 * (vb + i) + 1] : vals[2 * (vb + i)];
   }
 }
-__global__ void k_op_unit(int64_t ncol, int kp, const int *__restrict__ col_dof, const uint8_t *__restrict__ dir,
+// (compact: kpc values per column, value o of row cmap[o], zero for padding)
+__global__ void k_op_unit(int64_t ncol, int kp, int kpc, const int *__restrict__ cmap,
+                          const int *__restrict__ col_dof, const uint8_t *__restrict__ dir,
                           const double *__restrict__ U, double *__restrict__ op) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ncol * kp;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ncol * kpc;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = i / kp;
-    const int o = (int)(i - c * kp);
-    op[i] = dir[col_dof[c]] ? -U[c * 2 * kp + kp + o] : U[c * 2 * kp + o];
+    const int64_t c = i / kpc;
+    const int r = cmap[(int)(i - c * kpc)];
+    op[i] = r < 0 ? 0.0 : (dir[col_dof[c]] ? -U[c * 2 * kp + kp + r] : U[c * 2 * kp + r]);
   }
 }
 
@@ -2246,13 +2276,24 @@ void Engine::build_unit_multipoles(bool want_f32) {
     }
     sync();  // qpt, qcol, dqbeg, dfit, dfit_half are freed on return
   }
-  // the operator's unit multipoles (U1 on Neumann, -U2 on Dirichlet DoFs)
-  const int64_t nou = std::max<int64_t>(ncol, 1) * kp;
+  // the operator's unit multipoles (U1 on Neumann, -U2 on Dirichlet DoFs),
+  // compact: the real parts, then the imaginary parts of m >= 1 (those of
+  // m = 0 vanish for real sources), padded to a multiple of 4
+  {
+    std::vector<int32_t> cmap;
+    for (int i = 0; i < kt; ++i) cmap.push_back(i);
+    for (int nn = 0; nn <= p; ++nn)
+      for (int m = 1; m <= nn; ++m) cmap.push_back(kt + nn * (nn + 1) / 2 + m);
+    while (cmap.size() % 4) cmap.push_back(-1);
+    kpc = (int)cmap.size();
+    up(d_cmap, cmap);
+  }
+  const int64_t nou = std::max<int64_t>(ncol, 1) * kpc;
   alloc(f64.opunit, (size_t)nou * sizeof(double));
   if (ncol > 0) {
-    hipLaunchKernelGGL(k_op_unit, dim3((unsigned)std::min<int64_t>(nblk(ncol * kp, 256), 8192)), dim3(256), 0,
-                       stream, ncol, kp, d_p2m_col_dof.as<int>(), d_dir.as<uint8_t>(), f64.unit.as<double>(),
-                       f64.opunit.as<double>());
+    hipLaunchKernelGGL(k_op_unit, dim3((unsigned)std::min<int64_t>(nblk(ncol * kpc, 256), 8192)), dim3(256), 0,
+                       stream, ncol, kp, kpc, d_cmap.as<int>(), d_p2m_col_dof.as<int>(), d_dir.as<uint8_t>(),
+                       f64.unit.as<double>(), f64.opunit.as<double>());
     HIPCHK(hipGetLastError());
   }
   if (want_f32) {
@@ -2579,8 +2620,8 @@ void Engine::generate(const double *phi, const double *dphi) {
   if (nleaf > 0)
     hipLaunchKernelGGL((k_p2m<T, NK>), dim3(nleaf), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
                        d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(),
-                       (NK == 2 ? F.unit : F.opunit).template as<T>(), kp, phi, dphi, F.me.template as<T>(),
-                       (int64_t)0, (int64_t)0);
+                       (NK == 2 ? F.unit : F.opunit).template as<T>(), kp, kpc, d_cmap.as<int>(), phi, dphi,
+                       F.me.template as<T>(), (int64_t)0, (int64_t)0);
   mark(1);
   for (int l = L; l >= 1; --l)
     launch_translate<T, 0, NK>(*this, stream, m2m_cl, lvl<NK>(far.m2m, l), lvl<NK>(far.m2m, l + 1),
@@ -2650,8 +2691,8 @@ void Engine::matvec(const double *phi, const double *dphi, double *outN, double 
     cl.ne = ne;
     const unsigned nclear = (unsigned)std::min<int64_t>(nblk(2 * ne, 256), 1024);
     hipLaunchKernelGGL((k_p2m<T, 2>), dim3(nleaf + nclear), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
-                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.unit.template as<T>(), kp, phi, dphi, me,
-                       (int64_t)0, (int64_t)0, cl);
+                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.unit.template as<T>(), kp, kpc,
+                       nullptr, phi, dphi, me, (int64_t)0, (int64_t)0, cl);
   } else {
     hipLaunchKernelGGL(k_clear_operator<T>, dim3((unsigned)std::min<int64_t>(nblk(2 * ne, 256), 4096)), dim3(256),
                        0, stream, me, le, ne, nullptr, (int64_t)0);
@@ -2786,8 +2827,8 @@ void Engine::matvec_fused(const double *x, double *out, double *dst, int side) {
     cl.na = n;
     const unsigned nclear = (unsigned)std::min<int64_t>(nblk(2 * ne + n, 256), 1024);
     hipLaunchKernelGGL((k_p2m<T, 1>), dim3(nleaf + nclear), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
-                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp, x, nullptr, me,
-                       (int64_t)0, (int64_t)0, cl);
+                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp, kpc,
+                       d_cmap.as<int>(), x, nullptr, me, (int64_t)0, (int64_t)0, cl);
   } else {
     hipLaunchKernelGGL(k_clear_operator<T>, dim3((unsigned)std::min<int64_t>(nblk(2 * ne + n, 256), 4096)),
                        dim3(256), 0, stream, me, le, ne, acc, n);
@@ -4299,8 +4340,8 @@ void Engine::matvec_combined_multi(const double *x, double *out, int R) {
                      ne, T(0));
   if (nleaf > 0)  // every right-hand side in one launch (blockIdx.y)
     hipLaunchKernelGGL((k_p2m<T, 1>), dim3(nleaf, R), dim3(256), 0, stream, nleaf, d_p2m_leaf.as<int>(),
-                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp, x, nullptr,
-                       ME, (int64_t)n, sb);
+                       d_p2m_col_beg.as<int>(), d_p2m_col_dof.as<int>(), F.opunit.template as<T>(), kp, kpc,
+                       d_cmap.as<int>(), x, nullptr, ME, (int64_t)n, sb);
   if (me_lo < 0) consumer_levels();
   for (int l = L; l > me_lo; --l)
     launch_translate<T, 0, 1>(*this, stream, mr_m2m, lvl<1>(mr_m2m_h, l), lvl<1>(mr_m2m_h, l + 1),
