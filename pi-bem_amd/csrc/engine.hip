@@ -1656,34 +1656,59 @@ __global__ void __launch_bounds__(256) k_bjacobi(int nrows, const int *__restric
   z[rows[g]] = acc;
 }
 // The same with one workgroup of four wavefronts per block (blocks of at
-// most 64 RPL rows, RPL rows per lane): every wavefront gathers the block's
-// inputs once (lane c: v at row c + 64 q) and sums the column vectors
-// w, w + 4, ... of all rows (lane r: row r), the inputs broadcast from their
-// lanes; the four partial sums meet in LDS.
-// (right-hand side blockIdx.y: v and z + y stride)
+// most 64 RPL rows, RPL rows per lane): wavefront w sums the column vectors
+// w, w + 4, ... of all rows (lane r: row r).  It loads all of them first,
+// since their addresses need only the block's offsets, so that they are in
+// flight while the block's inputs are gathered (lane c: v at row c + 64 q);
+// the inputs are then broadcast from their lanes, and the four partial sums
+// meet in LDS.  (right-hand side blockIdx.y: v and z + y stride)
+template <typename PT> struct PinvVec;
+template <> struct PinvVec<float> {
+  using type = float4;
+  static __device__ __forceinline__ double at(const float4 &u, int k) {
+    return k == 0 ? u.x : k == 1 ? u.y : k == 2 ? u.z : u.w;
+  }
+};
+template <> struct PinvVec<double> {
+  using type = double2;
+  static __device__ __forceinline__ double at(const double2 &u, int k) { return k == 0 ? u.x : u.y; }
+};
 template <int RPL, typename PT>
 __global__ void __launch_bounds__(256) k_bjacobi_wg(const int *__restrict__ row_beg, const int *__restrict__ rows,
                                                     const int64_t *__restrict__ blk_beg,
                                                     const PT *__restrict__ pinv, const double *__restrict__ v,
                                                     double *__restrict__ z, int64_t stride) {
   constexpr int CW = 16 / sizeof(PT);
+  constexpr int NI = 64 * RPL / CW / 4;  // column vectors per wavefront in a block of 64 RPL rows
+  using VT = typename PinvVec<PT>::type;
   __shared__ double part[4][64 * RPL];
   const int blk = blockIdx.x;
   v += (int64_t)blockIdx.y * stride;
   z += (int64_t)blockIdx.y * stride;
   const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
   const int r0 = row_beg[blk], R = row_beg[blk + 1] - r0, nq = (R + CW - 1) / CW;
+  const PT *P = pinv + blk_beg[blk];
+  VT a[NI][RPL];
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int cq = w + 4 * i;
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int row = lane + 64 * q;
+      a[i][q] = cq < nq && row < R ? *reinterpret_cast<const VT *>(P + ((int64_t)cq * R + row) * CW) : VT{};
+    }
+  }
   double xv[RPL], acc[RPL];
-  const PT *P[RPL];
 #pragma unroll
   for (int q = 0; q < RPL; ++q) {
     const int row = lane + 64 * q;
     xv[q] = row < R ? v[rows[r0 + row]] : 0.0;  // zero past the block: the padded columns
-    P[q] = pinv + blk_beg[blk] + (int64_t)(row < R ? row : 0) * CW;
     acc[q] = 0.0;
   }
-#pragma unroll 8
-  for (int cq = w; cq < nq; cq += 4) {
+#pragma unroll
+  for (int i = 0; i < NI; ++i) {
+    const int cq = w + 4 * i;
+    if (cq >= nq) break;  // the same for the whole wavefront
     double xc[CW];
 #pragma unroll
     for (int k = 0; k < CW; ++k) {
@@ -1692,11 +1717,9 @@ __global__ void __launch_bounds__(256) k_bjacobi_wg(const int *__restrict__ row_
     }
 #pragma unroll
     for (int q = 0; q < RPL; ++q) {
-      double a[CW] = {};
-      if (lane + 64 * q < R) pinv_load(P[q] + (int64_t)cq * R * CW, a);
       double s = 0.0;
 #pragma unroll
-      for (int k = 0; k < CW; ++k) s += a[k] * xc[k];
+      for (int k = 0; k < CW; ++k) s += PinvVec<PT>::at(a[i][q], k) * xc[k];
       acc[q] += s;
     }
   }
