@@ -102,7 +102,7 @@ struct Engine {
   bool have_f32 = false;
   // operator vectors
   DevBuf d_alpha, d_sp, d_sd, d_outN, d_outD, d_in, d_out, d_red;
-  DevBuf d_cgs;  // partial sums of the Arnoldi step's dots (kDotParts per column)
+  DevBuf d_cgs;  // partial sums of the Arnoldi step's dots (kDotParts per column) and norms (one per block)
   int max_leaf_rows = 1 << 30;  // largest block of the preconditioner
   // pinned host copy of the Hessenberg slots (asynchronous device-to-host
   // copies) and the event that marks a copy done

@@ -1841,8 +1841,11 @@ __global__ void k_scale_inv(int64_t n, double *__restrict__ v, const double *__r
 // per column (P row ranges), which lets the kernels that use the sums finish
 // them themselves instead of in launches of their own: V_j . w for j < nv
 // into part[j * P + range].
-constexpr int kDotParts = 16;   // per column of V . w (consumers re-read nv * 16 partials)
-constexpr int kNormParts = 64;  // of |w|^2 (one wavefront finishes them)
+// Enough row ranges that each thread streams only a few dozen entries of a
+// column: with 16 the projection's dots waited on their load chains (about
+// 11 us for any number of columns up to 16 at 393k rows).
+constexpr int kDotParts = 64;    // per column of V . w (a wavefront finishes a column: one partial per lane)
+constexpr int kNormParts = 256;  // of |w|^2 (each scaling workgroup finishes them)
 // (skip: when set and *skip != 0, the launch does nothing: the second
 // projection of a step that the first left without severe cancellation)
 template <typename VT>
@@ -1861,66 +1864,101 @@ __global__ void __launch_bounds__(kRedThreads) k_mdot_parts(int64_t n, int64_t l
   const double r = block_sum(acc, sh);
   if (threadIdx.x == 0) part[(int64_t)j * P + blockIdx.x] = r;
 }
-// w -= sum_j V_j h_j, h_j the sum of column j's P partials, summed by every
-// block in the same order; block 0 also stores h (a Hessenberg column).
-// nv <= 128 (GMRES restart <= 127).
+// w -= sum_j V_j h_j, h_j the sum of column j's P <= 64 partials, summed
+// by every block in the same order (wavefront j mod 4, which loads all of
+// its columns' partials first); block 0 also stores h (a Hessenberg
+// column), and every block the sum of squares of its rows of the new w in
+// npart_out[block] when npart_out is given.  nv <= 128 (GMRES restart <=
+// 127).
 template <typename VT>
 __global__ void __launch_bounds__(256) k_gemv_sub_fin(int64_t n, int64_t ldv, int nv, const VT *__restrict__ V,
                                                       const double *__restrict__ part, int P,
                                                       double *__restrict__ h_out, double *__restrict__ w,
-                                                      const int *__restrict__ skip) {
+                                                      const int *__restrict__ skip, double *__restrict__ npart_out) {
   if (skip && *skip) {  // a skipped second projection: coefficients zero, w as it is
     if (blockIdx.x == 0)
       for (int j = threadIdx.x; j < nv; j += blockDim.x) h_out[j] = 0.0;
     return;
   }
+  constexpr int kMaxJ = 128 / 4;  // columns per wavefront
   __shared__ double hs[128];
-  for (int j = threadIdx.x; j < nv; j += blockDim.x) {
-    double s = 0;
-    for (int c = 0; c < P; ++c) s += part[(int64_t)j * P + c];
-    hs[j] = s;
-    if (blockIdx.x == 0) h_out[j] = s;
+  {
+    const int wf = (int)(threadIdx.x >> 6), l = (int)(threadIdx.x & 63);
+    double pv[kMaxJ];
+#pragma unroll
+    for (int q = 0; q < kMaxJ; ++q) {
+      const int j = wf + 4 * q;
+      pv[q] = j < nv && l < P ? part[(int64_t)j * P + l] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxJ; ++q) {
+      const int j = wf + 4 * q;
+      if (j >= nv) break;  // the same for the whole wavefront
+      const double s = wave_sum(pv[q]);
+      if (l == 0) {
+        hs[j] = s;
+        if (blockIdx.x == 0) h_out[j] = s;
+      }
+    }
   }
   __syncthreads();
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double acc = w[i];
-  for (int j = 0; j < nv; ++j) acc -= (double)V[(int64_t)j * ldv + i] * hs[j];
-  w[i] = acc;
+  double acc = 0.0;
+  if (i < n) {
+    acc = w[i];
+    for (int j = 0; j < nv; ++j) acc -= (double)V[(int64_t)j * ldv + i] * hs[j];
+    w[i] = acc;
+  }
+  if (npart_out) {  // the same for the whole block
+    __shared__ double sh[4];
+    const double r = block_sum(acc * acc, sh);
+    if (threadIdx.x == 0) npart_out[blockIdx.x] = r;
+  }
 }
 // Selective reorthogonalisation (Daniel, Gragg, Kaufman & Stewart 1976, with
 // a smaller threshold): after the first projection w1 = w0 - V h1, with
 // |w0|^2 = |h1|^2 + |w1|^2, the second one is needed only when |w1| <
 // eta |w0| (cancellation has eaten all but a fraction eta of w0).  One
-// wavefront sums the nv coefficients h1 and the P <= 64 norm partials of w1
-// and sets skip[0] for the second projection's launches, skip[1] for the
-// norm launch that follows only a second projection.
-__global__ void k_reorth_flag(const double *__restrict__ h1, int nv, const double *__restrict__ npart, int P,
-                              double eta, int *__restrict__ skip) {
-  const int l = (int)threadIdx.x;
-  double sh = 0.0;
-  for (int j = l; j < nv; j += 64) sh += h1[j] * h1[j];
-  sh = wave_sum(sh);
-  const double sw = wave_sum(l < P ? npart[l] : 0.0);
-  if (l == 0) {
-    skip[0] = sw >= eta * eta * (sh + sw) ? 1 : 0;
-    skip[1] = 1 - skip[0];  // the norm launch after the second projection: skipped with it
+// workgroup sums the nv coefficients h1 and the P norm partials of w1 (the
+// first projection's blocks), sets skip[0] for the second projection's
+// launches and stores |w1|^2 in nrm1_sq for the scaling that follows.
+__global__ void __launch_bounds__(256) k_reorth_flag(const double *__restrict__ h1, int nv,
+                                                     const double *__restrict__ npart, int P, double eta,
+                                                     int *__restrict__ skip, double *__restrict__ nrm1_sq) {
+  __shared__ double sh[4];
+  double a = 0.0, b = 0.0;
+  for (int j = (int)threadIdx.x; j < nv; j += 256) a += h1[j] * h1[j];
+  for (int c = (int)threadIdx.x; c < P; c += 256) b += npart[c];
+  const double shh = block_sum(a, sh);
+  __syncthreads();
+  const double sw = block_sum(b, sh);
+  if (threadIdx.x == 0) {
+    skip[0] = sw >= eta * eta * (shh + sw) ? 1 : 0;
+    *nrm1_sq = sw;
   }
 }
-// v /= |v| with |v|^2 the sum of P <= 64 partials (one wavefront per block
-// finishes them, in the same order in every block); block 0 stores |v|.
-__global__ void __launch_bounds__(256) k_scale_fin(int64_t n, double *__restrict__ v, const double *__restrict__ part,
-                                                   int P, double *__restrict__ nrm_out) {
-  __shared__ double s_nrm;
-  if (threadIdx.x < 64) {
-    const double q = wave_sum((int)threadIdx.x < P ? part[threadIdx.x] : 0.0);
-    if (threadIdx.x == 0) {
-      s_nrm = sqrt(q);
-      if (blockIdx.x == 0) *nrm_out = s_nrm;
-    }
-  }
+// |w|^2 for the scaling after the projections: nrm1_sq when the second
+// projection was skipped (skip set and *skip != 0), else the sum of the P
+// partials of the last projection's blocks (summed by the whole block, in
+// the same order in every block).  The result is returned to every thread.
+__device__ __forceinline__ double proj_norm_sq(const int *__restrict__ skip, const double *__restrict__ nrm1_sq,
+                                               const double *__restrict__ part, int P) {
+  __shared__ double sh[4], q;
+  if (skip && *skip) return *nrm1_sq;  // the same for the whole block
+  double a = 0.0;
+  for (int c = (int)threadIdx.x; c < P; c += 256) a += part[c];
+  const double r = block_sum(a, sh);
+  if (threadIdx.x == 0) q = r;
   __syncthreads();
-  const double s = s_nrm;
+  return q;
+}
+// v /= |v| with |v|^2 from proj_norm_sq; block 0 stores |v|.
+__global__ void __launch_bounds__(256) k_scale_fin(int64_t n, double *__restrict__ v, const int *__restrict__ skip,
+                                                   const double *__restrict__ nrm1_sq,
+                                                   const double *__restrict__ part, int P,
+                                                   double *__restrict__ nrm_out) {
+  const double s = sqrt(proj_norm_sq(skip, nrm1_sq, part, P));
+  if (blockIdx.x == 0 && threadIdx.x == 0) *nrm_out = s;
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n && s > 0) v[i] /= s;
 }
@@ -1928,18 +1966,12 @@ __global__ void __launch_bounds__(256) k_scale_fin(int64_t n, double *__restrict
 // and, as stored, the next step's fixed input (the operator's, FP64)
 template <typename VT>
 __global__ void __launch_bounds__(256) k_scale_fin2(int64_t n, const double *__restrict__ w, VT *__restrict__ v,
-                                                    double *__restrict__ v2, const double *__restrict__ part, int P,
+                                                    double *__restrict__ v2, const int *__restrict__ skip,
+                                                    const double *__restrict__ nrm1_sq,
+                                                    const double *__restrict__ part, int P,
                                                     double *__restrict__ nrm_out) {
-  __shared__ double s_nrm;
-  if (threadIdx.x < 64) {
-    const double q = wave_sum((int)threadIdx.x < P ? part[threadIdx.x] : 0.0);
-    if (threadIdx.x == 0) {
-      s_nrm = sqrt(q);
-      if (blockIdx.x == 0) *nrm_out = s_nrm;
-    }
-  }
-  __syncthreads();
-  const double s = s_nrm;
+  const double s = sqrt(proj_norm_sq(skip, nrm1_sq, part, P));
+  if (blockIdx.x == 0 && threadIdx.x == 0) *nrm_out = s;
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) {
     const VT x = (VT)(s > 0 ? w[i] / s : w[i]);
@@ -3478,13 +3510,15 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
   int kc = rc && rc->kc > 0 && rc->kc < m ? rc->kc : 0;
   std::vector<double> Ecol, cproj;  // coefficients on C of each step; C^T r
   if (rc && rc->record) rc->k = 0;
-  // dot partials (kDotParts x 128), then the first projection's norm
-  // partials (kNormParts) and the second projection's skip flag
-  const size_t cg_doubles = (size_t)kDotParts * 128 + kNormParts + 4;
+  // dot partials (kDotParts x 128), the norm partials of the two
+  // projections (one per block of k_gemv_sub_fin each), |w1|^2 and the
+  // second projection's skip flag
+  const size_t cg_doubles = (size_t)kDotParts * 128 + 2 * (size_t)g + 4;
   if (e.d_cgs.bytes < cg_doubles * 8) alloc(e.d_cgs, cg_doubles * 8);
   double *cg = e.d_cgs.as<double>();
-  double *cg_norm1 = cg + (size_t)kDotParts * 128;
-  int *skip2 = reinterpret_cast<int *>(cg_norm1 + kNormParts);
+  double *npart1 = cg + (size_t)kDotParts * 128, *npart2 = npart1 + g, *nrm1_sq = npart2 + g;
+  int *skip2 = reinterpret_cast<int *>(nrm1_sq + 1);
+  static_assert(kNormParts <= kDotParts * 128, "the recycled vectors' norm partials share the dot partials");
   const bool selective = e.reorth_eta < 1.0;
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1), h(m + 2), y(m);
   // Hessenberg slots on the device, one per Arnoldi step: [h1 (m+2)][h2 (m+2)][|w|]
@@ -3532,8 +3566,9 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
       e.step_op(Vb + (int64_t)k * n, vk1, pk);
     }
     ++it;
-    // CGS2: two projections and the norm, all on the device, six launches:
-    // the projections and the scaling finish the dot products they use
+    // CGS with a second projection when needed, all on the device, six
+    // launches: the projections and the scaling finish the dot products they
+    // use, and the projections sum the squares of their results by block
     double *slot = hd + (size_t)k * S;
     double *h1 = slot, *h2 = slot + (m + 2), *hn_d = slot + 2 * (m + 2);
     const int nv = kc + k + 1;
@@ -3542,29 +3577,26 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
     hipLaunchKernelGGL(k_mdot_parts<VT>, dim3(kDotParts, nv), dim3(kRedThreads), 0, e.stream, n, n, V, wv, cg,
                        nullptr);
     hipLaunchKernelGGL(k_gemv_sub_fin<VT>, dim3(g), dim3(256), 0, e.stream, n, n, nv, V, cg, kDotParts, h1, wv,
-                       nullptr);
+                       nullptr, selective ? npart1 : nullptr);
     const int *sk = nullptr;
     if (selective) {
-      hipLaunchKernelGGL(k_mdot_parts<double>, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, wv, wv,
-                         cg_norm1, nullptr);
-      hipLaunchKernelGGL(k_reorth_flag, dim3(1), dim3(64), 0, e.stream, h1, nv, cg_norm1, kNormParts,
-                         e.reorth_eta, skip2);
+      hipLaunchKernelGGL(k_reorth_flag, dim3(1), dim3(256), 0, e.stream, h1, nv, npart1, (int)g, e.reorth_eta, skip2,
+                         nrm1_sq);
       sk = skip2;
     }
     hipLaunchKernelGGL(k_mdot_parts<VT>, dim3(kDotParts, nv), dim3(kRedThreads), 0, e.stream, n, n, V, wv, cg, sk);
-    hipLaunchKernelGGL(k_gemv_sub_fin<VT>, dim3(g), dim3(256), 0, e.stream, n, n, nv, V, cg, kDotParts, h2, wv, sk);
-    // the norm again after a second projection (with none, w1's stands)
-    hipLaunchKernelGGL(k_mdot_parts<double>, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, wv, wv,
-                       cg_norm1, selective ? skip2 + 1 : nullptr);
+    // the norm after the second projection, or w1's when it was skipped
+    hipLaunchKernelGGL(k_gemv_sub_fin<VT>, dim3(g), dim3(256), 0, e.stream, n, n, nv, V, cg, kDotParts, h2, wv, sk,
+                       npart2);
     if constexpr (f32_basis) {
-      hipLaunchKernelGGL(k_scale_fin2<VT>, dim3(g), dim3(256), 0, e.stream, n, wv, vk1, e.d_vn.as<double>(),
-                         cg_norm1, kNormParts, hn_d);
+      hipLaunchKernelGGL(k_scale_fin2<VT>, dim3(g), dim3(256), 0, e.stream, n, wv, vk1, e.d_vn.as<double>(), sk,
+                         nrm1_sq, npart2, (int)g, hn_d);
     } else {
       if (graphs)
-        hipLaunchKernelGGL(k_scale_fin2<double>, dim3(g), dim3(256), 0, e.stream, n, wv, vk1, e.d_vn.as<double>(),
-                           cg_norm1, kNormParts, hn_d);
+        hipLaunchKernelGGL(k_scale_fin2<double>, dim3(g), dim3(256), 0, e.stream, n, wv, vk1, e.d_vn.as<double>(), sk,
+                           nrm1_sq, npart2, (int)g, hn_d);
       else
-        hipLaunchKernelGGL(k_scale_fin, dim3(g), dim3(256), 0, e.stream, n, vk1, cg_norm1, kNormParts, hn_d);
+        hipLaunchKernelGGL(k_scale_fin, dim3(g), dim3(256), 0, e.stream, n, vk1, sk, nrm1_sq, npart2, (int)g, hn_d);
     }
   };
   while (it < maxit) {
@@ -3592,12 +3624,12 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
         hipLaunchKernelGGL(k_mdot_parts<VT>, dim3(kDotParts, kc), dim3(kRedThreads), 0, e.stream, n, n, V, w, cg,
                            nullptr);
         hipLaunchKernelGGL(k_gemv_sub_fin<VT>, dim3(g), dim3(256), 0, e.stream, n, n, kc, V, cg, kDotParts, pj, w,
-                           nullptr);
+                           nullptr, nullptr);
       }
       hipLaunchKernelGGL(k_mdot_parts<double>, dim3(kNormParts, 1), dim3(kRedThreads), 0, e.stream, n, n, w, w, cg,
                          nullptr);
       hipLaunchKernelGGL(k_scale_fin2<VT>, dim3(g), dim3(256), 0, e.stream, n, w, V + (int64_t)kc * n,
-                         e.d_vn.as<double>(), cg, kNormParts, ps + 2 * (m + 2));
+                         e.d_vn.as<double>(), nullptr, nullptr, cg, kNormParts, ps + 2 * (m + 2));
       std::vector<double> pr((size_t)S);
       HIPCHK(hipMemcpyAsync(pr.data(), ps, (size_t)S * 8, hipMemcpyDeviceToHost, e.stream));
       e.sync();
