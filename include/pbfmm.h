@@ -322,7 +322,8 @@ int pbfmm_set_mixed_tolerances(pbfmm_handle *h, double inner_reduction, double r
  * it a second time only when the first projection left less than a fraction
  * eta of its norm (default 0.05; eta >= 1 always projects twice, classical
  * Gram-Schmidt with reorthogonalisation; 0 never does; < 0 restores the
- * default).  Every solve still ends on the true relative residual.  No
+ * default).  Inner solves with FP32 basis vectors (mixed scheme 4) always
+ * project twice.  Every solve still ends on the true relative residual.  No
  * counterpart in the reference (deal.II's SolverGMRES orthogonalises by its
  * own rule). */
 int pbfmm_set_gmres_reorth(pbfmm_handle *h, double eta);

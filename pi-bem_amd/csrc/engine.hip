@@ -3519,7 +3519,10 @@ static int gmres_core(Engine &e, const double *b, double *x, double tol_abs, int
   double *npart1 = cg + (size_t)kDotParts * 128, *npart2 = npart1 + g, *nrm1_sq = npart2 + g;
   int *skip2 = reinterpret_cast<int *>(nrm1_sq + 1);
   static_assert(kNormParts <= kDotParts * 128, "the recycled vectors' norm partials share the dot partials");
-  const bool selective = e.reorth_eta < 1.0;
+  // An FP32 basis always takes the second projection: with the selective
+  // test its steps lose orthogonality (config 3: 113 Arnoldi steps against
+  // 53, where the FP64 basis needs 54 either way)
+  const bool selective = e.reorth_eta < 1.0 && sizeof(VT) == 8;
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1), h(m + 2), y(m);
   // Hessenberg slots on the device, one per Arnoldi step: [h1 (m+2)][h2 (m+2)][|w|]
   const int S = 2 * (m + 2) + 1;

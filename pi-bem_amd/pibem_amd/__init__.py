@@ -121,7 +121,8 @@ class BEMFMA:
     def set_gmres_reorth(self, eta=-1.0):
         """GMRES Gram-Schmidt: the second projection of an Arnoldi step runs
         when the first left less than eta of the vector's norm (default
-        0.05; >= 1 always, 0 never; pbfmm_set_gmres_reorth)."""
+        0.05; >= 1 always, 0 never; inner solves on FP32 basis vectors
+        always project twice; pbfmm_set_gmres_reorth)."""
         self._chk(self._L.pbfmm_set_gmres_reorth(self._h, float(eta)))
 
     def set_spmv_kernel(self, precision, kernel):
