@@ -94,7 +94,7 @@ struct Engine {
   ChunkDev l2l_g_cl;  // L2L two target levels per launch (FarPlan::l2l_g)
   DevBuf d_pval_beg;  // FP32 operator values: per-leaf offsets of the row-pair layout
   DevBuf d_row_leaf, d_row_beg, d_rows, d_col_beg, d_cols, d_val_beg, d_ent_beg, d_ent_cell,
-      d_ent_mask, d_ent_near, d_ent_colpos, d_work_beg, d_leaf_block;
+      d_ent_mask, d_ent_near, d_ent_colpos, d_leaf_block;
   DevBuf d_sing;  // singular rules: per js [w, N(dpc), dNx(dpc), dNy(dpc)] per point
   int sing_nq = 0;
   FarDev<double> f64;

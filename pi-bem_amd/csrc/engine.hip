@@ -128,12 +128,17 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 // ---------------------------------------------------------------------------
 // Near-field assembly (BEMFMA::direct_integrals, bem_fma.cc:670-913 regular
-// and singular, 997-1126 bigger blocks).  One thread per (row, entry): the
+// and singular, 997-1126 bigger blocks).  An item is a (row, entry) pair: the
 // regular rule over the entry's quad points, or the singular rule (Telles or
 // Duffy split, chosen at set-up) over the whole cell when the cell holds a
-// double of the row's node, both in FP64.
-// Results are added into the leaf's rows x cols block of (N, D) pairs; the
-// FP32 operator is this block rounded once (k_round_f32), since in FP32 the
+// double of the row's node, both in FP64.  One workgroup per row; G lanes
+// (G >= quad points per cell) share an item, a lane per quad point (or every
+// G-th singular point), so that a load of the cell's quad data is one
+// contiguous run.  The group sums the (N, D) parts of the cell's shape
+// functions and adds them, a lane per value, into the row's (N, D) pairs in
+// LDS (columns in tiles of kAsmTile; a row with more columns computes its
+// items once per tile), and the row is stored once, whole.  The FP32
+// operator is this block rounded once (k_round_f32), since in FP32 the
 // difference R = y - x loses ~|x|/|R| ulps and R.n cancels again, which at a
 // radius-4 surface costs 1e-4 of the double-layer product.
 // ---------------------------------------------------------------------------
@@ -142,8 +147,20 @@ __device__ __forceinline__ T wave_sum(T v) {
 // entries follow the last bit of X; evaluating X and R exactly as the
 // reference's FEValues path does (separately rounded products and sums) keeps
 // FP64 entries reproducible to rounding.
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  v += dpp_move<0x140>(v);  // row_mirror: every lane holds its 16-lane row's sum
+  if constexpr (G >= 32) v += __shfl_xor(v, 16);
+  if constexpr (G >= 64) v += __shfl_xor(v, 32);
+  return v;
+}
+constexpr int kAsmTile = 2048;
+template <int G>
 __global__ void __launch_bounds__(256) k_assemble_near(
-    int nleaf, const int64_t *__restrict__ work_beg, const int *__restrict__ row_beg,
+    int nleaf, const int *__restrict__ row_beg,
     const int *__restrict__ rows, const int *__restrict__ col_beg, const int *__restrict__ ent_beg,
     const int *__restrict__ ent_cell, const uint64_t *__restrict__ ent_mask,
     const uint8_t *__restrict__ ent_near, const int *__restrict__ ent_colpos,
@@ -154,98 +171,115 @@ __global__ void __launch_bounds__(256) k_assemble_near(
     const int *__restrict__ sing_cnt, int sing_stride, double *__restrict__ vals) {
 #pragma clang fp contract(off)
   using T = double;
-  const int64_t total = work_beg[nleaf];
-  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < total;
-       w += (int64_t)gridDim.x * blockDim.x) {
-    int lo = 0, hi = nleaf - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (work_beg[mid] <= w) lo = mid;
-      else hi = mid - 1;
-    }
-    const int leaf = lo;
-    const int e0 = ent_beg[leaf], ne = ent_beg[leaf + 1] - e0;
-    const int64_t local = w - work_beg[leaf];
-    const int r = (int)(local / ne);
-    const int e = e0 + (int)(local % ne);
-    const int node = rows[row_beg[leaf] + r];
-    const int cell = ent_cell[e];
-    const int *cd = cdofs + (int64_t)cell * dpc;
-    int js = -1;
-    if (ent_near[e]) {
-      for (int j = 0; j < dpc && js < 0; ++j)
-        for (int k = dbl_ptr[node]; k < dbl_ptr[node + 1]; ++k)
-          if (dbl_idx[k] == cd[j]) {
-            js = j;
-            break;
-          }
-    }
-    const int C = col_beg[leaf + 1] - col_beg[leaf];
-    T *out = vals + 2 * (val_beg[leaf] + (int64_t)r * C);
-    const int *cp = ent_colpos + (int64_t)e * dpc;
-    if (js < 0) {
+  constexpr int IPB = 256 / G;  // items per workgroup pass
+  __shared__ double acc[2 * kAsmTile];
+  const int lane = (int)(threadIdx.x & (G - 1));
+  const int g = (int)blockIdx.x;  // row g of the leaf-ordered rows
+  int lo = 0, hi = nleaf - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (row_beg[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const int leaf = lo;
+  const int r = g - row_beg[leaf], node = rows[g];
+  const int e0 = ent_beg[leaf], e1 = ent_beg[leaf + 1];
+  const int C = col_beg[leaf + 1] - col_beg[leaf];
+  T *out = vals + 2 * (val_beg[leaf] + (int64_t)r * C);
+  const T x0 = (T)xyz[3 * node], x1 = (T)xyz[3 * node + 1], x2 = (T)xyz[3 * node + 2];
+  for (int c0 = 0; c0 < C; c0 += kAsmTile) {
+    const int cw = min(kAsmTile, C - c0);
+    for (int i = (int)threadIdx.x; i < 2 * cw; i += 256) acc[i] = 0.0;
+    __syncthreads();
+    // every lane runs every pass (the group sums need the whole wavefront)
+    for (int eb = e0; eb < e1; eb += IPB) {
+      const int e = eb + (int)(threadIdx.x / G);
+      const bool valid = e < e1;
       T Nj[9], Dj[9];
       for (int j = 0; j < 9; ++j) Nj[j] = Dj[j] = T(0);
-      const T x0 = (T)xyz[3 * node], x1 = (T)xyz[3 * node + 1], x2 = (T)xyz[3 * node + 2];
-      uint64_t mask = ent_mask[e];
-      while (mask) {
-        const int q = __ffsll((unsigned long long)mask) - 1;
-        mask &= mask - 1;
-        const int64_t g = (int64_t)cell * nq + q;
-        const T R0 = qx[3 * g] - x0, R1 = qx[3 * g + 1] - x1, R2 = qx[3 * g + 2] - x2;
-        const T r = dsqrt(R0 * R0 + R1 * R1 + R2 * R2);
-        const T r2 = r * r;
-        const T s = T(1) / (r * T(4 * kPi));
-        const T den = T(-4 * kPi) * r2 * r;
-        const T dn = (R0 / den) * qn[3 * g] + (R1 / den) * qn[3 * g + 1] + (R2 / den) * qn[3 * g + 2];
-        const T wq = qw[g];
-        for (int j = 0; j < dpc; ++j) {
-          const T sh = qsh[g * dpc + j];
-          Nj[j] += dn * sh * wq;
-          Dj[j] += s * sh * wq;
+      const int *cp = nullptr;
+      if (valid) {
+        const int cell = ent_cell[e];
+        const int *cd = cdofs + (int64_t)cell * dpc;
+        int js = -1;
+        if (ent_near[e]) {
+          for (int j = 0; j < dpc && js < 0; ++j)
+            for (int k = dbl_ptr[node]; k < dbl_ptr[node + 1]; ++k)
+              if (dbl_idx[k] == cd[j]) {
+                js = j;
+                break;
+              }
         }
-      }
-      for (int j = 0; j < dpc; ++j) {
-        atomicAdd(out + 2 * cp[j], Nj[j]);
-        atomicAdd(out + 2 * cp[j] + 1, Dj[j]);
-      }
-    } else {
-      double Nj[9], Dj[9];
-      for (int j = 0; j < 9; ++j) Nj[j] = Dj[j] = 0.0;
-      const double x0 = xyz[3 * node], x1 = xyz[3 * node + 1], x2 = xyz[3 * node + 2];
-      const double *G = geom + (int64_t)cell * dpc * 3;
-      const double *rule = sing + (int64_t)js * sing_stride;
-      const int nsq = sing_cnt[js];
-      const int ps = 1 + 3 * dpc;
-      for (int q = 0; q < nsq; ++q) {
-        const double *pt = rule + q * ps;
-        double X[3] = {0, 0, 0}, a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
-        for (int j = 0; j < dpc; ++j)
-          for (int k = 0; k < 3; ++k) {
-            X[k] += pt[1 + j] * G[3 * j + k];
-            a[k] += pt[1 + dpc + j] * G[3 * j + k];
-            b[k] += pt[1 + 2 * dpc + j] * G[3 * j + k];
+        cp = ent_colpos + (int64_t)e * dpc;
+        if (js < 0) {
+          const int q = lane;
+          if (q < nq && ((ent_mask[e] >> q) & 1)) {
+            const int64_t g = (int64_t)cell * nq + q;
+            const T R0 = qx[3 * g] - x0, R1 = qx[3 * g + 1] - x1, R2 = qx[3 * g + 2] - x2;
+            const T r = dsqrt(R0 * R0 + R1 * R1 + R2 * R2);
+            const T r2 = r * r;
+            const T s = T(1) / (r * T(4 * kPi));
+            const T den = T(-4 * kPi) * r2 * r;
+            const T dn = (R0 / den) * qn[3 * g] + (R1 / den) * qn[3 * g + 1] + (R2 / den) * qn[3 * g + 2];
+            const T wq = qw[g];
+            for (int j = 0; j < dpc; ++j) {
+              const T sh = qsh[g * dpc + j];
+              Nj[j] = dn * sh * wq;
+              Dj[j] = s * sh * wq;
+            }
           }
-        const double cr0 = a[1] * b[2] - a[2] * b[1], cr1 = a[2] * b[0] - a[0] * b[2],
-                     cr2 = a[0] * b[1] - a[1] * b[0];
-        const double J = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
-        const double R0 = X[0] - x0, R1 = X[1] - x1, R2 = X[2] - x2;
-        const double r = sqrt(R0 * R0 + R1 * R1 + R2 * R2);
-        const double r2 = r * r;
-        const double s = 1. / (r * 4 * kPi);
-        const double den = -4 * kPi * r2 * r;
-        const double dn = (R0 / den) * (cr0 / J) + (R1 / den) * (cr1 / J) + (R2 / den) * (cr2 / J);
-        const double wq = J * pt[0];
-        for (int j = 0; j < dpc; ++j) {
-          Nj[j] += dn * pt[1 + j] * wq;
-          Dj[j] += s * pt[1 + j] * wq;
+        } else {
+          const double *Gm = geom + (int64_t)cell * dpc * 3;
+          const double *rule = sing + (int64_t)js * sing_stride;
+          const int nsq = sing_cnt[js];
+          const int ps = 1 + 3 * dpc;
+          for (int q = lane; q < nsq; q += G) {
+            const double *pt = rule + q * ps;
+            double X[3] = {0, 0, 0}, a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+            for (int j = 0; j < dpc; ++j)
+              for (int k = 0; k < 3; ++k) {
+                X[k] += pt[1 + j] * Gm[3 * j + k];
+                a[k] += pt[1 + dpc + j] * Gm[3 * j + k];
+                b[k] += pt[1 + 2 * dpc + j] * Gm[3 * j + k];
+              }
+            const double cr0 = a[1] * b[2] - a[2] * b[1], cr1 = a[2] * b[0] - a[0] * b[2],
+                         cr2 = a[0] * b[1] - a[1] * b[0];
+            const double J = sqrt(cr0 * cr0 + cr1 * cr1 + cr2 * cr2);
+            const double R0 = X[0] - x0, R1 = X[1] - x1, R2 = X[2] - x2;
+            const double r = sqrt(R0 * R0 + R1 * R1 + R2 * R2);
+            const double r2 = r * r;
+            const double s = 1. / (r * 4 * kPi);
+            const double den = -4 * kPi * r2 * r;
+            const double dn = (R0 / den) * (cr0 / J) + (R1 / den) * (cr1 / J) + (R2 / den) * (cr2 / J);
+            const double wq = J * pt[0];
+            for (int j = 0; j < dpc; ++j) {
+              Nj[j] += dn * pt[1 + j] * wq;
+              Dj[j] += s * pt[1 + j] * wq;
+            }
+          }
         }
       }
-      for (int j = 0; j < dpc; ++j) {
-        atomicAdd(out + 2 * cp[j], Nj[j]);
-        atomicAdd(out + 2 * cp[j] + 1, Dj[j]);
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        Nj[j] = group_sum<G>(Nj[j]);
+        Dj[j] = group_sum<G>(Dj[j]);
+      }
+      if (valid) {
+        // lane t adds value t of the item's 2 dpc: N of shape function t, or D of t - dpc
+        for (int t = lane; t < 2 * dpc; t += G) {
+          const int j = t < dpc ? t : t - dpc;
+          T v = T(0);
+#pragma unroll
+          for (int k = 0; k < 9; ++k)
+            if (k == j) v = t < dpc ? Nj[k] : Dj[k];
+          const int c = cp[j] - c0;
+          if (c >= 0 && c < cw) atomicAdd(&acc[2 * c + (t < dpc ? 0 : 1)], v);
+        }
       }
     }
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < 2 * cw; i += 256) out[2 * c0 + i] = acc[i];
+    __syncthreads();
   }
 }
 
@@ -2202,7 +2236,6 @@ void Engine::upload_near_plan() {
   up(d_ent_mask, near.ent_mask);
   up(d_ent_near, near.ent_near);
   up(d_ent_colpos, near.ent_colpos);
-  up(d_work_beg, near.work_beg);
   up(d_leaf_block, near.leaf_block);
 }
 
@@ -2407,14 +2440,22 @@ void Engine::assemble_near(bool want_f32) {
   HIPCHK(hipMemsetAsync(f64.vals.p, 0, f64.vals.bytes, stream));
   const int64_t total = near.work_beg.back();
   if (near.nleaf > 0 && total > 0) {
-    const unsigned grid = (unsigned)std::min<int64_t>(nblk(total, 256), 65536);
-    hipLaunchKernelGGL(k_assemble_near, dim3(grid), dim3(256), 0, stream, near.nleaf, d_work_beg.as<int64_t>(),
-                       d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_ent_beg.as<int>(),
-                       d_ent_cell.as<int>(), d_ent_mask.as<uint64_t>(), d_ent_near.as<uint8_t>(),
-                       d_ent_colpos.as<int>(), d_val_beg.as<int64_t>(), d_cdofs.as<int>(), dpc, nq,
-                       d_dbl_ptr.as<int>(), d_dbl_idx.as<int>(), d_xyz.as<double>(), f64.qx.as<double>(),
-                       f64.qn.as<double>(), f64.qw.as<double>(), f64.qsh.as<double>(), d_geom.as<double>(),
-                       d_sing.as<double>(), d_scnt.as<int>(), maxq * ps, f64.vals.as<double>());
+    // G lanes per item: the quad points per cell (at most 64, the entry masks' width), rounded up
+    if (nq > 64 || dpc > 9) throw std::runtime_error("near assembly: more than 64 quad points or 9 DoFs per cell");
+    const int G = nq <= 16 ? 16 : nq <= 32 ? 32 : 64;
+    const unsigned grid = (unsigned)near.row_beg[near.nleaf];  // a workgroup per row
+#define PBFMM_ASM(GG)                                                                                              \
+  hipLaunchKernelGGL(k_assemble_near<GG>, dim3(grid), dim3(256), 0, stream, near.nleaf,                            \
+                     d_row_beg.as<int>(), d_rows.as<int>(), d_col_beg.as<int>(), d_ent_beg.as<int>(),                \
+                     d_ent_cell.as<int>(), d_ent_mask.as<uint64_t>(), d_ent_near.as<uint8_t>(),                    \
+                     d_ent_colpos.as<int>(), d_val_beg.as<int64_t>(), d_cdofs.as<int>(), dpc, nq,                  \
+                     d_dbl_ptr.as<int>(), d_dbl_idx.as<int>(), d_xyz.as<double>(), f64.qx.as<double>(),            \
+                     f64.qn.as<double>(), f64.qw.as<double>(), f64.qsh.as<double>(), d_geom.as<double>(),          \
+                     d_sing.as<double>(), d_scnt.as<int>(), maxq * ps, f64.vals.as<double>())
+    if (G == 16) PBFMM_ASM(16);
+    else if (G == 32) PBFMM_ASM(32);
+    else PBFMM_ASM(64);
+#undef PBFMM_ASM
     HIPCHK(hipGetLastError());
   }
   // the operator's single-value near field (N on Neumann, -D on Dirichlet columns)
