@@ -583,14 +583,15 @@ def tune_leaf_size(problem, precision=PBFMM_MIXED, solves=2, start=20, ladder=LE
 
 def leaf_walk(measure, start=20, ladder=LEAF_LADDER, gain=0.01):
     """The tuner's walk over the ladder: measure(leaf) -> cost (each leaf is
-    measured once here).  From start it measures both neighbours, then steps
+    measured once here).  From start it measures both neighbours, then walks
     towards the cheaper one (also when neither beats the start: a flat start
     does not end the walk) until three leaf sizes in a row do not beat the
     best so far (so that a plateau or a noisy point does not end the walk
-    early).  A leaf size replaces the best only when it is cheaper by more
-    than `gain` (1%), so timing noise does not move the choice.  Returns
-    (best leaf, bracketed: both ladder neighbours of the best were measured
-    and cost more)."""
+    early), and then the same way from start in the other direction (so that
+    one noisy neighbour cannot hide a minimum on its side).  A leaf size
+    replaces the best only when it is cheaper by more than `gain` (1%), so
+    timing noise does not move the choice.  Returns (best leaf, bracketed:
+    both ladder neighbours of the best were measured and cost more)."""
     lad = list(ladder)
     cost = {}
 
@@ -608,15 +609,16 @@ def leaf_walk(measure, start=20, ladder=LEAF_LADDER, gain=0.01):
     for j in nbr:
         m(lad[j])
     jc = min(nbr, key=lambda j: cost[lad[j]]) if nbr else k
-    step = (1 if jc > k else -1) if nbr else 0
+    first = 1 if jc > k else -1
     best, kb = lad[k], k
-    j, misses = k + step, 0
-    while step and 0 <= j < len(lad) and misses < 3:
-        if beats(lad[j], best):
-            best, kb, misses = lad[j], j, 0
-        else:
-            misses += 1
-        j += step
+    for step in ((first, -first) if nbr else ()):
+        j, misses = k + step, 0
+        while 0 <= j < len(lad) and misses < 3:
+            if beats(lad[j], best):
+                best, kb, misses = lad[j], j, 0
+            else:
+                misses += 1
+            j += step
     bracketed = 0 < kb < len(lad) - 1 and lad[kb - 1] in cost and lad[kb + 1] in cost
     return best, bracketed
 

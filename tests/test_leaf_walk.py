@@ -1,7 +1,8 @@
 """The leaf-size tuner's walk over its ladder (pibem_amd.leaf_walk), on
 cost functions with a known minimum: it finds an interior minimum and
 brackets it, stops at the ladder's end for monotone costs (not bracketed),
-and measures every leaf size at most once."""
+searches both sides of the start, and measures every leaf size at most
+once."""
 from collections import Counter
 
 from pibem_amd import LEAF_LADDER, leaf_walk
@@ -57,10 +58,19 @@ def test_walk_leaves_a_start_cheaper_than_both_neighbours():
 
 
 def test_start_at_the_minimum_stays():
-    # a true minimum at the start: three misses on the cheaper side end the walk
+    # a true minimum at the start: three misses on each side end the walk
     cost = {8: 7, 12: 6, 16: 5, 20: 4, 24: 5.5, 32: 6.5}
     best, bracketed, seen = _walk(lambda l: cost.get(l, 9))
-    assert best == 20 and bracketed and 32 not in seen
+    assert best == 20 and bracketed and 48 not in seen
+
+
+def test_noisy_neighbour_does_not_hide_its_side():
+    # 24 timed high once (16.48 ms, about 13.6 otherwise), so the walk goes
+    # towards 16 first; the walk from 20 upwards still finds 96
+    cost = {8: 28.21, 12: 17.36, 16: 14.62, 20: 13.55, 24: 16.48, 32: 12.58, 40: 11.27, 48: 10.94, 64: 10.19,
+            80: 10.46, 96: 9.61, 128: 10.51, 160: 13.90}
+    best, bracketed, seen = _walk(lambda l: cost[l])
+    assert best == 96 and bracketed and max(seen.values()) == 1
 
 
 def test_walk_crosses_a_plateau():
